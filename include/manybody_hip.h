@@ -1,0 +1,199 @@
+/* manybody_hip.h — C ABI of the MI355X (gfx950) Hubbard H.v path.
+ *
+ * Drop-in boundary for keyehzy/manybody's fixed-particle-number Hubbard path:
+ * the basis (FermionBasis), the model (HubbardModel / 2D / 3D), the operator
+ * behind LinearOperator::apply, and the Krylov consumers (Lanczos, Krylov
+ * exp(-iHt), KPM).  Each entry point cites the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every function returns an int status (MB_OK = 0) and never throws; the
+ *    message of the last failure on the calling thread is mb_last_error().
+ *    Status codes map to the reference's exceptions: MB_ERR_RUNTIME ~
+ *    std::runtime_error, MB_ERR_INVALID ~ std::invalid_argument,
+ *    MB_ERR_RANGE ~ std::out_of_range.
+ *  - Handles are opaque.  A model handle keeps a reference to its basis handle:
+ *    the caller keeps the basis alive at least as long as the model (as the
+ *    reference's operators hold `const FermionBasis&`).
+ *  - Vectors are device pointers.  "ref" vectors are indexed by the reference
+ *    basis index (the DFS position, basis.cpp:267-302); "dev" vectors use the
+ *    device product order x[u * D_dn + d], u and d the combinatorial ranks of
+ *    the up and down occupation bitstrings.  Complex vectors are interleaved
+ *    (re, im): complex128 = 2 doubles, complex64 = 2 floats.
+ *  - `stream` is a hipStream_t (NULL = default stream).  Entry points that
+ *    return host scalars synchronize that stream before returning.
+ *  - Concurrent calls on one const model from different host threads and
+ *    streams are safe for mb_hv_* and the permutation calls (read-only tables,
+ *    per-call scratch), mirroring the reference's concurrent const apply().
+ */
+#ifndef MANYBODY_HIP_H
+#define MANYBODY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MB_OK 0
+#define MB_ERR_INVALID 1 /* std::invalid_argument */
+#define MB_ERR_RUNTIME 2 /* std::runtime_error */
+#define MB_ERR_RANGE 3   /* std::out_of_range */
+#define MB_ERR_DEVICE 4  /* HIP runtime failure */
+#define MB_ERR_NOMEM 5   /* device allocation failed or admission refused */
+
+typedef struct mb_basis mb_basis;
+typedef struct mb_hubbard mb_hubbard;
+typedef void *mb_stream; /* hipStream_t */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *mb_last_error(void);
+
+/* Library version string. */
+const char *mb_version(void);
+
+/* ------------------------------------------------------------------ basis */
+
+/* FermionBasis::with_fixed_particle_number_and_spin(orbitals, particles,
+ * spin_projection)  — src/algebra/basis.h:25-26, basis.cpp:91-126.
+ * up = (particles + spin_projection) / 2, down = particles - up.  Violations of
+ * the reference's asserts (basis.cpp:97-117) return MB_ERR_INVALID.
+ * orbitals <= 32. */
+int mb_basis_create(int orbitals, int particles, int spin_projection, mb_basis **out);
+void mb_basis_destroy(mb_basis *b);
+
+/* basis.set.size() — compute_basis_size_spin, basis.cpp:39-50. */
+int64_t mb_basis_size(const mb_basis *b);
+
+/* Occupations of basis.set[index] as bitmasks (bit i = orbital i) —
+ * IndexedHashSet::operator[] (indexed_hash_set.h).  MB_ERR_RANGE if index is
+ * out of range. */
+int mb_basis_state(const mb_basis *b, int64_t index, uint64_t *up_bits, uint64_t *dn_bits);
+
+/* basis.set.index_of(key) — indexed_hash_set.h:158-164.  MB_ERR_RANGE if the
+ * state is not in the basis (the reference throws std::out_of_range). */
+int mb_basis_index_of(const mb_basis *b, uint64_t up_bits, uint64_t dn_bits, int64_t *index);
+
+/* Reference index -> device product index, for n consecutive reference indices
+ * starting at `first` (host arrays).  Used by hosts that keep their own vectors
+ * in reference order. */
+int mb_basis_product_index(const mb_basis *b, int64_t first, int64_t n, int64_t *product_index);
+
+/* ------------------------------------------------------------------ model */
+
+/* HubbardModel(t, u, size) / HubbardModel2D(t, u, sx, sy) /
+ * HubbardModel3D(t, u, sx, sy, sz) bound to a basis —
+ * src/algebra/fermion/model/hubbard_model.h:10-212.  ndim in {1, 2, 3};
+ * sizes[0..ndim) are the linear sizes; their product must equal the basis'
+ * orbital count.  Bonds, their loop order, the merge of repeated bonds
+ * (linear size 2) and self bonds (linear size 1) follow the reference.
+ * Allocates the model's device tables on the current HIP device. */
+int mb_hubbard_create(const mb_basis *b, int ndim, const int *sizes, double t, double u,
+                      mb_hubbard **out);
+
+/* Same Hamiltonian form over an explicit bond list: H = sum_k -t (hopping(from[k], to[k], up)
+ * + hopping(from[k], to[k], down)) + u sum_i n_i,up n_i,dn, hopping as in
+ * hopping_generic (expression_base.h:394-403). */
+int mb_hubbard_create_bonds(const mb_basis *b, int n_bonds, const int *from, const int *to,
+                            double t, double u, mb_hubbard **out);
+void mb_hubbard_destroy(mb_hubbard *h);
+
+/* LinearOperator::dimension() — linear_operator.h:42. */
+int64_t mb_hubbard_dimension(const mb_hubbard *h);
+
+/* Device-order bookkeeping: D_up, D_dn (dimension = D_up * D_dn). */
+int mb_hubbard_shape(const mb_hubbard *h, int64_t *d_up, int64_t *d_dn);
+
+/* --------------------------------------------------------------- H . v */
+
+/* y = H x, complex128, device order.  Replaces compute_matrix_elements
+ * (matrix_elements.h:74-105) followed by SparseComplexMatrixOperator::apply
+ * (linear_operator.h:81-91).  x and y must not alias. */
+int mb_hv_c128(const mb_hubbard *h, const void *x_dev, void *y_dev, mb_stream stream);
+
+/* y = H x, complex64, device order (FP32 arithmetic; the reference has no
+ * FP32 path). */
+int mb_hv_c64(const mb_hubbard *h, const void *x_dev, void *y_dev, mb_stream stream);
+
+/* y = H x for nvec complex64 vectors stored element-major, x[e * nvec + r]
+ * (one sweep of the index tables serves all nvec vectors). nvec in {1,2,4,8,16}. */
+int mb_hv_c64_multi(const mb_hubbard *h, int nvec, const void *x_dev, void *y_dev,
+                    mb_stream stream);
+
+/* Reference order <-> device order for complex128 / complex64 vectors
+ * (a gather and a scatter; the permutation is evaluated by rank arithmetic on
+ * the device, no table is stored). */
+int mb_to_device_order_c128(const mb_hubbard *h, const void *x_ref, void *x_dev, mb_stream stream);
+int mb_to_reference_order_c128(const mb_hubbard *h, const void *x_dev, void *x_ref,
+                               mb_stream stream);
+
+/* The whole reference-order apply: y_ref = H x_ref, using `work` (2 device
+ * complex128 vectors of the operator's dimension, caller-owned). */
+int mb_apply_ref_c128(const mb_hubbard *h, const void *x_ref, void *y_ref, void *work,
+                      mb_stream stream);
+
+/* Seeded start vectors, written in device order but drawn per reference index
+ * i: re, im ~ N(0,1) i.i.d. from (seed, i) (SplitMix64 -> Box-Muller).  Not
+ * normalized (Lanczos normalizes, lanczos.h:96). */
+int mb_fill_gaussian_c128(const mb_hubbard *h, uint64_t seed, void *x_dev, mb_stream stream);
+
+/* nvec random-phase complex64 vectors e^{i phi}, phi ~ U[0, 2 pi) from
+ * (seed, vector, reference index), element-major x[e * nvec + r]. */
+int mb_fill_phase_c64_multi(const mb_hubbard *h, uint64_t seed, int nvec, void *x_dev,
+                            mb_stream stream);
+
+/* The same random-phase vectors in complex128 (e^{i phi} evaluated in FP64). */
+int mb_fill_phase_c128_multi(const mb_hubbard *h, uint64_t seed, int nvec, void *x_dev,
+                             mb_stream stream);
+
+/* ----------------------------------------------------------- Krylov layer */
+
+/* lanczos_pass_one(op, b, k) — lanczos.h:130-175 (recurrence :15-40, no
+ * reorthogonalization).  b is a device-order complex128 vector.  alphas holds k
+ * doubles, betas k doubles (the first steps-1 are meaningful, as
+ * LanczosDecomposition::betas); *steps = steps_taken, *b_norm = ||b||.
+ * Zero-norm b -> MB_ERR_RUNTIME "Input vector has zero norm". */
+int mb_lanczos_pass_one(const mb_hubbard *h, const void *b_dev, int k, double *alphas,
+                        double *betas, int *steps, double *b_norm, mb_stream stream);
+
+/* find_min_eigenpair(op, k) with an explicit start vector b (the reference
+ * draws it with arma::randu, linear_operator.h:26-32) — lanczos.h:246-290.
+ * *value = lowest Ritz value; if x_dev != NULL it receives the eigenvector
+ * (scaled by ||b|| as lanczos.h:276, rebuilt by pass two :177-218). */
+int mb_find_min_eigenpair(const mb_hubbard *h, const void *b_dev, int k, double *value,
+                          void *x_dev, mb_stream stream);
+
+/* time_evolve_state(H, psi, t, {krylov_steps}) — evolve_state.h:164-178:
+ * out = exp(-i t H) psi, complex128 device order.  krylov_steps 0 -> D. */
+int mb_time_evolve(const mb_hubbard *h, const void *psi_dev, double t, int krylov_steps,
+                   void *out_dev, mb_stream stream);
+
+/* time_evolve_state_steps(H, psi, t0, t1, dt, opts) — evolve_state.h:204-231
+ * (the callback is not exposed).  out = psi(t1). */
+int mb_time_evolve_steps(const mb_hubbard *h, const void *psi_dev, double t0, double t1,
+                         double dt, int krylov_steps, void *out_dev, mb_stream stream);
+
+/* Symmetric tridiagonal eigen-decomposition on the host (replaces arma::eig_sym
+ * of T_k, lanczos.h:258-271): w ascending, V (m*m, column j = eigenvector j,
+ * V[i + m*j]) optional. */
+int mb_tridiag_eig(int m, const double *alphas, const double *betas, double *w, double *V);
+
+/* KPM density-of-states moments mu_0..mu_{M-1} of Hs = (H - b)/a, averaged
+ * over nvec random-phase vectors (seeded as mb_fill_phase_c64_multi):
+ * mu_n = (1/(nvec D)) sum_r <r|T_n(Hs)|r>.  The Chebyshev recurrence of
+ * SparseKPMProjector::apply (kpm.h:212-229); H.v in FP32 complex (fp32 = 1) or
+ * FP64 complex (fp32 = 0); inner products and moments accumulate in FP64.
+ * M even, nvec in {1,2,4,8,16}. */
+int mb_kpm_dos_moments(const mb_hubbard *h, double a, double b, int M, int nvec, uint64_t seed,
+                       int fp32, double *mu, mb_stream stream);
+
+/* kpm::jackson_kernel(N) — kpm.h:19-30: g[0..N]. */
+int mb_jackson_kernel(int N, double *g);
+
+/* rescaling::from_bounds(E_min, E_max, padding) — rescaling.h:33-38. */
+int mb_rescaling_from_bounds(double e_min, double e_max, double padding, double *a, double *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

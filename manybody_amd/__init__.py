@@ -1,0 +1,386 @@
+"""manybody_amd — MI355X-native Hubbard H.v path (Python mirror of the C ABI).
+
+The product is the HIP library ``libmanybody_hip.so`` in this directory and its C
+ABI (``include/manybody_hip.h``).  This module binds it with ctypes and mirrors
+the reference's names (keyehzy/manybody):
+
+=============================================  =====================================
+reference                                      here
+=============================================  =====================================
+FermionBasis::with_fixed_particle_number_and_   FermionBasis.with_fixed_particle_
+spin (src/algebra/basis.cpp:91-126)            number_and_spin
+HubbardModel / 2D / 3D                          HubbardModel, HubbardModel2D,
+(src/algebra/fermion/model/hubbard_model.h)    HubbardModel3D
+SparseComplexMatrixOperator::apply              HubbardOperator.apply (reference
+(src/numerics/linear_operator.h:81-91)          order) / .hv (device order)
+lanczos_pass_one (src/numerics/lanczos.h:130)   lanczos_pass_one
+find_min_eigenpair (lanczos.h:283-290)          find_min_eigenpair
+time_evolve_state[_steps] (evolve_state.h)      time_evolve_state[_steps]
+kpm::jackson_kernel (kpm.h:19-30)               jackson_kernel
+rescaling::from_bounds (rescaling.h:33-38)      rescaling_from_bounds
+=============================================  =====================================
+
+Vectors are torch tensors on the GPU (torch is used for device memory and the
+stream only).  Errors raise the Python analogue of the reference's exception:
+std::runtime_error -> RuntimeError, std::invalid_argument -> ValueError,
+std::out_of_range -> IndexError.  There is no CPU fallback: without the built
+library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmanybody_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "manybody_hip.h")
+
+MB_OK, MB_ERR_INVALID, MB_ERR_RUNTIME, MB_ERR_RANGE, MB_ERR_DEVICE, MB_ERR_NOMEM = range(6)
+
+_lib = None
+
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_SIGS = {
+    "mb_last_error": (C.c_char_p, []),
+    "mb_version": (C.c_char_p, []),
+    "mb_basis_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "mb_basis_destroy": (None, [_vp]),
+    "mb_basis_size": (C.c_int64, [_vp]),
+    "mb_basis_state": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "mb_basis_index_of": (C.c_int, [_vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_int64)]),
+    "mb_basis_product_index": (C.c_int, [_vp, C.c_int64, C.c_int64, _vp]),
+    "mb_hubbard_create": (C.c_int, [_vp, C.c_int, _vp, C.c_double, C.c_double, C.POINTER(_vp)]),
+    "mb_hubbard_create_bonds": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_double, C.c_double,
+                                          C.POINTER(_vp)]),
+    "mb_hubbard_destroy": (None, [_vp]),
+    "mb_hubbard_dimension": (C.c_int64, [_vp]),
+    "mb_hubbard_shape": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "mb_hv_c128": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mb_hv_c64": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mb_hv_c64_multi": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
+    "mb_to_device_order_c128": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mb_to_reference_order_c128": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mb_apply_ref_c128": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "mb_fill_gaussian_c128": (C.c_int, [_vp, C.c_uint64, _vp, _vp]),
+    "mb_fill_phase_c64_multi": (C.c_int, [_vp, C.c_uint64, C.c_int, _vp, _vp]),
+    "mb_fill_phase_c128_multi": (C.c_int, [_vp, C.c_uint64, C.c_int, _vp, _vp]),
+    "mb_lanczos_pass_one": (C.c_int, [_vp, _vp, C.c_int, _dp, _dp, C.POINTER(C.c_int), _dp, _vp]),
+    "mb_find_min_eigenpair": (C.c_int, [_vp, _vp, C.c_int, _dp, _vp, _vp]),
+    "mb_time_evolve": (C.c_int, [_vp, _vp, C.c_double, C.c_int, _vp, _vp]),
+    "mb_time_evolve_steps": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                       _vp, _vp]),
+    "mb_tridiag_eig": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp]),
+    "mb_kpm_dos_moments": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_uint64,
+                                     C.c_int, _dp, _vp]),
+    "mb_jackson_kernel": (C.c_int, [C.c_int, _dp]),
+    "mb_rescaling_from_bounds": (C.c_int, [C.c_double, C.c_double, C.c_double, _dp, _dp]),
+}
+
+
+def lib():
+    """Load libmanybody_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status: int):
+    if status == MB_OK:
+        return
+    msg = lib().mb_last_error().decode()
+    if status == MB_ERR_INVALID:
+        raise ValueError(msg)
+    if status == MB_ERR_RANGE:
+        raise IndexError(msg)
+    if status == MB_ERR_NOMEM:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t) -> C.c_void_p:
+    if not t.is_cuda:
+        raise ValueError("expected a GPU tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+# ------------------------------------------------------------------ basis
+
+class FermionBasis:
+    """FermionBasis restricted to fixed particle number and spin projection."""
+
+    def __init__(self, handle, orbitals: int, n_up: int, n_dn: int):
+        self._h = handle
+        self.orbitals = orbitals
+        self.n_up = n_up
+        self.n_dn = n_dn
+        self.particles = n_up + n_dn
+
+    @staticmethod
+    def with_fixed_particle_number_and_spin(orbitals: int, particles: int,
+                                            spin_projection: int) -> "FermionBasis":
+        h = C.c_void_p()
+        _check(lib().mb_basis_create(orbitals, particles, spin_projection, C.byref(h)))
+        up = (particles + spin_projection) // 2
+        return FermionBasis(h, orbitals, up, particles - up)
+
+    @staticmethod
+    def sector(orbitals: int, n_up: int, n_dn: int) -> "FermionBasis":
+        return FermionBasis.with_fixed_particle_number_and_spin(orbitals, n_up + n_dn, n_up - n_dn)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mb_basis_destroy(self._h)
+            self._h = None
+
+    def size(self) -> int:
+        return int(lib().mb_basis_size(self._h))
+
+    __len__ = size
+
+    def state(self, index: int):
+        """(up_bits, dn_bits) of basis.set[index]."""
+        u, d = C.c_uint64(), C.c_uint64()
+        _check(lib().mb_basis_state(self._h, index, C.byref(u), C.byref(d)))
+        return int(u.value), int(d.value)
+
+    def index_of(self, up_bits: int, dn_bits: int) -> int:
+        i = C.c_int64()
+        _check(lib().mb_basis_index_of(self._h, up_bits, dn_bits, C.byref(i)))
+        return int(i.value)
+
+    def contains(self, up_bits: int, dn_bits: int) -> bool:
+        try:
+            self.index_of(up_bits, dn_bits)
+            return True
+        except IndexError:
+            return False
+
+    def product_index(self, first: int = 0, n: Optional[int] = None):
+        import numpy as np
+        n = self.size() - first if n is None else n
+        out = np.zeros(n, np.int64)
+        _check(lib().mb_basis_product_index(self._h, first, n, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+
+# ------------------------------------------------------------------ models
+
+class _HubbardBase:
+    def _bonds_args(self):
+        raise NotImplementedError
+
+    def operator(self, basis: FermionBasis) -> "HubbardOperator":
+        sizes = (C.c_int * len(self.sizes))(*self.sizes)
+        h = C.c_void_p()
+        _check(lib().mb_hubbard_create(basis._h, len(self.sizes), sizes, self.t, self.u, C.byref(h)))
+        return HubbardOperator(h, basis, self)
+
+
+class HubbardModel(_HubbardBase):
+    """HubbardModel(t, u, size) — hubbard_model.h:10-59 (ring of `size` sites)."""
+
+    def __init__(self, t: float, u: float, size: int):
+        self.t, self.u, self.size = float(t), float(u), int(size)
+        self.sizes = (self.size,)
+
+
+class HubbardModel2D(_HubbardBase):
+    """HubbardModel2D(t, u, size_x, size_y) — hubbard_model.h:61-129."""
+
+    def __init__(self, t: float, u: float, size_x: int, size_y: int):
+        self.t, self.u = float(t), float(u)
+        self.size_x, self.size_y = int(size_x), int(size_y)
+        self.sizes = (self.size_x, self.size_y)
+
+
+class HubbardModel3D(_HubbardBase):
+    """HubbardModel3D(t, u, size_x, size_y, size_z) — hubbard_model.h:131-212."""
+
+    def __init__(self, t: float, u: float, size_x: int, size_y: int, size_z: int):
+        self.t, self.u = float(t), float(u)
+        self.sizes = (int(size_x), int(size_y), int(size_z))
+
+
+class HubbardOperator:
+    """The LinearOperator of a Hubbard model on a fixed-N basis (device tables)."""
+
+    def __init__(self, handle, basis: FermionBasis, model):
+        self._h = handle
+        self.basis = basis  # keeps the basis alive (the model references it)
+        self.model = model
+        du, dd = C.c_int64(), C.c_int64()
+        _check(lib().mb_hubbard_shape(self._h, C.byref(du), C.byref(dd)))
+        self.d_up, self.d_dn = int(du.value), int(dd.value)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mb_hubbard_destroy(self._h)
+            self._h = None
+
+    def dimension(self) -> int:
+        return int(lib().mb_hubbard_dimension(self._h))
+
+    # --- device order
+    def empty(self, dtype=None, nvec: int = 1):
+        import torch
+        dtype = dtype or torch.complex128
+        n = self.dimension() * nvec
+        return torch.empty(n, dtype=dtype, device="cuda")
+
+    def hv(self, x, y=None):
+        """y = H x in device order (complex128 or complex64)."""
+        import torch
+        y = torch.empty_like(x) if y is None else y
+        if x.dtype == torch.complex128:
+            _check(lib().mb_hv_c128(self._h, _ptr(x), _ptr(y), _stream()))
+        elif x.dtype == torch.complex64:
+            nvec = x.numel() // self.dimension()
+            if nvec == 1:
+                _check(lib().mb_hv_c64(self._h, _ptr(x), _ptr(y), _stream()))
+            else:
+                _check(lib().mb_hv_c64_multi(self._h, nvec, _ptr(x), _ptr(y), _stream()))
+        else:
+            raise ValueError("x must be complex128 or complex64")
+        return y
+
+    def to_device_order(self, x_ref):
+        out = self.empty()
+        _check(lib().mb_to_device_order_c128(self._h, _ptr(x_ref), _ptr(out), _stream()))
+        return out
+
+    def to_reference_order(self, x_dev):
+        out = self.empty()
+        _check(lib().mb_to_reference_order_c128(self._h, _ptr(x_dev), _ptr(out), _stream()))
+        return out
+
+    def apply(self, x_ref):
+        """LinearOperator::apply in reference order: returns a new vector."""
+        import torch
+        y = torch.empty_like(x_ref)
+        work = self.empty(nvec=2)
+        _check(lib().mb_apply_ref_c128(self._h, _ptr(x_ref), _ptr(y), _ptr(work), _stream()))
+        return y
+
+    def gaussian(self, seed: int):
+        """Device-order complex128 vector, re/im ~ N(0,1) per reference index."""
+        x = self.empty()
+        _check(lib().mb_fill_gaussian_c128(self._h, seed, _ptr(x), _stream()))
+        return x
+
+    def random_phase(self, seed: int, nvec: int, fp32: bool = True):
+        """nvec random-phase vectors e^{i phi}, element-major x[e * nvec + r]."""
+        import torch
+        if fp32:
+            x = self.empty(torch.complex64, nvec)
+            _check(lib().mb_fill_phase_c64_multi(self._h, seed, nvec, _ptr(x), _stream()))
+        else:
+            x = self.empty(torch.complex128, nvec)
+            _check(lib().mb_fill_phase_c128_multi(self._h, seed, nvec, _ptr(x), _stream()))
+        return x
+
+
+# ------------------------------------------------------------------ Krylov
+
+@dataclass
+class LanczosDecomposition:
+    alphas: List[float] = field(default_factory=list)
+    betas: List[float] = field(default_factory=list)
+    steps_taken: int = 0
+    b_norm: float = 0.0
+
+
+@dataclass
+class EigenPair:
+    value: float
+    vector: object = None
+
+
+@dataclass
+class EvolutionOptions:
+    krylov_steps: int = 30
+
+
+def lanczos_pass_one(op: HubbardOperator, b, k: int) -> LanczosDecomposition:
+    al = (C.c_double * max(k, 1))()
+    be = (C.c_double * max(k, 1))()
+    steps = C.c_int()
+    bn = C.c_double()
+    _check(lib().mb_lanczos_pass_one(op._h, _ptr(b), k, al, be, C.byref(steps), C.byref(bn),
+                                     _stream()))
+    m = steps.value
+    return LanczosDecomposition(list(al[:m]), list(be[:max(m - 1, 0)]), m, bn.value)
+
+
+def find_min_eigenpair(op: HubbardOperator, k: int, b, want_vector: bool = True) -> EigenPair:
+    import torch
+    val = C.c_double()
+    x = op.empty() if want_vector else None
+    _check(lib().mb_find_min_eigenpair(op._h, _ptr(b), k, C.byref(val),
+                                       _ptr(x) if x is not None else None, _stream()))
+    return EigenPair(val.value, x)
+
+
+def time_evolve_state(op: HubbardOperator, psi, t: float,
+                      opts: EvolutionOptions = EvolutionOptions()):
+    out = op.empty()
+    _check(lib().mb_time_evolve(op._h, _ptr(psi), t, opts.krylov_steps, _ptr(out), _stream()))
+    return out
+
+
+def time_evolve_state_steps(op: HubbardOperator, psi, t0: float, t1: float, dt: float,
+                            opts: EvolutionOptions = EvolutionOptions()):
+    out = op.empty()
+    _check(lib().mb_time_evolve_steps(op._h, _ptr(psi), t0, t1, dt, opts.krylov_steps, _ptr(out),
+                                      _stream()))
+    return out
+
+
+def kpm_dos_moments(op: HubbardOperator, a: float, b: float, M: int, nvec: int, seed: int,
+                    fp32: bool = True):
+    mu = (C.c_double * M)()
+    _check(lib().mb_kpm_dos_moments(op._h, a, b, M, nvec, seed, int(fp32), mu, _stream()))
+    return list(mu)
+
+
+def tridiag_eig(alphas, betas, vectors: bool = True):
+    import numpy as np
+    a = np.ascontiguousarray(alphas, np.float64)
+    m = len(a)
+    b = np.ascontiguousarray(betas, np.float64) if m > 1 else np.zeros(1)
+    w = np.zeros(max(m, 1))
+    V = np.zeros((max(m, 1), max(m, 1)))
+    _check(lib().mb_tridiag_eig(m, a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p),
+                                w.ctypes.data_as(C.c_void_p),
+                                V.ctypes.data_as(C.c_void_p) if vectors else None))
+    # V is column-major: V_cm[i + m*j] = component i of vector j
+    return w[:m], (V.reshape(max(m, 1), max(m, 1)).T[:m, :m] if vectors else None)
+
+
+def jackson_kernel(N: int):
+    g = (C.c_double * (N + 1))()
+    _check(lib().mb_jackson_kernel(N, g))
+    return list(g)
+
+
+def rescaling_from_bounds(e_min: float, e_max: float, padding: float = 0.01):
+    a, b = C.c_double(), C.c_double()
+    _check(lib().mb_rescaling_from_bounds(e_min, e_max, padding, C.byref(a), C.byref(b)))
+    return a.value, b.value

@@ -1,0 +1,390 @@
+// extern "C" boundary of libmanybody_hip.so (declarations and reference
+// citations: include/manybody_hip.h).  Every entry point converts C++
+// exceptions into status codes and a thread-local message.
+#include <cstring>
+#include <memory>
+
+#include "../../include/manybody_hip.h"
+#include "krylov.hpp"
+#include "mb_internal.hpp"
+
+struct mb_basis {
+  mb::Sector sec;
+};
+struct mb_hubbard {
+  mb::Hubbard h;
+  const mb_basis *basis;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    g_err.clear();
+    return MB_OK;
+  } catch (const mb::RangeError &e) {
+    g_err = e.what();
+    return MB_ERR_RANGE;
+  } catch (const std::out_of_range &e) {
+    g_err = e.what();
+    return MB_ERR_RANGE;
+  } catch (const std::invalid_argument &e) {
+    g_err = e.what();
+    return MB_ERR_INVALID;
+  } catch (const mb::NoMemError &e) {
+    g_err = e.what();
+    return MB_ERR_NOMEM;
+  } catch (const mb::DeviceError &e) {
+    g_err = e.what();
+    return MB_ERR_DEVICE;
+  } catch (const std::bad_alloc &e) {
+    g_err = "host allocation failed";
+    return MB_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return MB_ERR_RUNTIME;
+  } catch (...) {
+    g_err = "unknown error";
+    return MB_ERR_RUNTIME;
+  }
+}
+
+void need(bool cond, const char *msg) {
+  if (!cond) throw std::invalid_argument(msg);
+}
+
+hipStream_t S(mb_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+const mb::Hubbard &H(const mb_hubbard *h) {
+  need(h != nullptr, "null model handle");
+  return h->h;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mb_last_error(void) { return g_err.c_str(); }
+const char *mb_version(void) { return "manybody_amd 0.1 (gfx950)"; }
+
+int mb_basis_create(int orbitals, int particles, int spin_projection, mb_basis **out) {
+  return guarded([&] {
+    need(out != nullptr, "null output");
+    *out = nullptr;
+    need(orbitals >= 1 && orbitals <= mb::kMaxSites, "orbitals must be in [1, 32]");
+    need(particles >= 0 && particles <= 2 * orbitals, "particles must be in [0, 2*orbitals]");
+    need(std::abs(spin_projection) <= particles, "|spin_projection| must not exceed particles");
+    need((particles + spin_projection) % 2 == 0, "particles + spin_projection must be even");
+    const int up = (particles + spin_projection) / 2, dn = particles - up;
+    need(up <= orbitals && dn <= orbitals, "spin sector exceeds the orbital count");
+    auto b = std::make_unique<mb_basis>();
+    mb::Sector &s = b->sec;
+    s.L = orbitals;
+    s.n_up = up;
+    s.n_dn = dn;
+    const mb::Binom &C = mb::binom();
+    s.d_up = (int64_t)C(orbitals, up);
+    s.d_dn = (int64_t)C(orbitals, dn);
+    need(s.d_up * s.d_dn < ((int64_t)1 << 40), "basis too large");
+    need(s.d_up < ((int64_t)1 << 31) && s.d_dn < ((int64_t)1 << 31), "species sector too large");
+    for (int n = 0; n < 2; ++n) {
+      const int k = n == 0 ? up : dn;
+      auto &v = n == 0 ? s.s_up : s.s_dn;
+      v.reserve(n == 0 ? s.d_up : s.d_dn);
+      // ascending bitstrings with popcount k (Gosper's hack)
+      if (k == 0) {
+        v.push_back(0u);
+      } else {
+        uint64_t x = (1ull << k) - 1;
+        while (x < (1ull << orbitals)) {
+          v.push_back((uint32_t)x);
+          const uint64_t c = x & (~x + 1), r = x + c;
+          x = (((r ^ x) >> 2) / c) | r;
+        }
+      }
+    }
+    *out = b.release();
+  });
+}
+
+void mb_basis_destroy(mb_basis *b) { delete b; }
+
+int64_t mb_basis_size(const mb_basis *b) { return b ? b->sec.dim() : -1; }
+
+int mb_basis_state(const mb_basis *b, int64_t index, uint64_t *up_bits, uint64_t *dn_bits) {
+  return guarded([&] {
+    need(b && up_bits && dn_bits, "null argument");
+    if (index < 0 || index >= b->sec.dim()) throw mb::RangeError("basis index out of range");
+    uint32_t u, d;
+    b->sec.ref_state(index, &u, &d);
+    *up_bits = u;
+    *dn_bits = d;
+  });
+}
+
+int mb_basis_index_of(const mb_basis *b, uint64_t up_bits, uint64_t dn_bits, int64_t *index) {
+  return guarded([&] {
+    need(b && index, "null argument");
+    const auto &s = b->sec;
+    const uint64_t mask = (s.L == 64) ? ~0ull : ((1ull << s.L) - 1);
+    if ((up_bits & ~mask) || (dn_bits & ~mask) || __builtin_popcountll(up_bits) != s.n_up ||
+        __builtin_popcountll(dn_bits) != s.n_dn)
+      throw mb::RangeError("state not in basis");
+    *index = s.ref_index((uint32_t)up_bits, (uint32_t)dn_bits);
+  });
+}
+
+int mb_basis_product_index(const mb_basis *b, int64_t first, int64_t n, int64_t *product_index) {
+  return guarded([&] {
+    need(b && product_index, "null argument");
+    const auto &s = b->sec;
+    if (first < 0 || n < 0 || first + n > s.dim()) throw mb::RangeError("index range out of bounds");
+    for (int64_t i = 0; i < n; ++i) {
+      uint32_t u, d;
+      s.ref_state(first + i, &u, &d);
+      product_index[i] = mb::Sector::colex_rank(u) * s.d_dn + mb::Sector::colex_rank(d);
+    }
+  });
+}
+
+static int hubbard_common(const mb_basis *b, std::vector<int> from, std::vector<int> to, double t,
+                          double u, mb_hubbard **out) {
+  return guarded([&] {
+    need(b && out, "null argument");
+    *out = nullptr;
+    for (size_t k = 0; k < from.size(); ++k)
+      need(from[k] >= 0 && from[k] < b->sec.L && to[k] >= 0 && to[k] < b->sec.L,
+           "bond endpoint outside the lattice");
+    auto h = std::make_unique<mb_hubbard>();
+    h->basis = b;
+    h->h.sec = &b->sec;
+    h->h.t = t;
+    h->h.U = u;
+    h->h.bond_from = std::move(from);
+    h->h.bond_to = std::move(to);
+    const_cast<mb::Sector &>(b->sec).ensure_device();
+    mb::build_hubbard(h->h);
+    *out = h.release();
+  });
+}
+
+int mb_hubbard_create(const mb_basis *b, int ndim, const int *sizes, double t, double u,
+                      mb_hubbard **out) {
+  std::vector<int> from, to;
+  int st = guarded([&] {
+    need(b && sizes, "null argument");
+    need(ndim >= 1 && ndim <= 3, "ndim must be 1, 2 or 3");
+    int prod = 1;
+    for (int i = 0; i < ndim; ++i) {
+      need(sizes[i] >= 1, "Dimension with size zero is invalid.");
+      prod *= sizes[i];
+    }
+    need(prod == b->sec.L, "lattice size does not match the basis' orbital count");
+    if (ndim == 1) {  // hubbard_model.h:16-20
+      const int L = sizes[0];
+      for (int i = 0; i < L; ++i) { from.push_back(i); to.push_back((i + 1) % L); }
+    } else if (ndim == 2) {  // hubbard_model.h:72-82
+      const int Lx = sizes[0], Ly = sizes[1];
+      for (int x = 0; x < Lx; ++x)
+        for (int y = 0; y < Ly; ++y) {
+          const int s = x + Lx * y;
+          from.push_back(s); to.push_back(((x + 1) % Lx) + Lx * y);
+          from.push_back(s); to.push_back(x + Lx * ((y + 1) % Ly));
+        }
+    } else {  // hubbard_model.h:144-159
+      const int Lx = sizes[0], Ly = sizes[1], Lz = sizes[2];
+      for (int x = 0; x < Lx; ++x)
+        for (int y = 0; y < Ly; ++y)
+          for (int z = 0; z < Lz; ++z) {
+            const int s = x + Lx * y + Lx * Ly * z;
+            from.push_back(s); to.push_back(((x + 1) % Lx) + Lx * y + Lx * Ly * z);
+            from.push_back(s); to.push_back(x + Lx * ((y + 1) % Ly) + Lx * Ly * z);
+            from.push_back(s); to.push_back(x + Lx * y + Lx * Ly * ((z + 1) % Lz));
+          }
+    }
+  });
+  if (st != MB_OK) return st;
+  return hubbard_common(b, std::move(from), std::move(to), t, u, out);
+}
+
+int mb_hubbard_create_bonds(const mb_basis *b, int n_bonds, const int *from, const int *to,
+                            double t, double u, mb_hubbard **out) {
+  if (n_bonds < 0 || (n_bonds > 0 && (!from || !to))) {
+    g_err = "invalid bond list";
+    return MB_ERR_INVALID;
+  }
+  return hubbard_common(b, std::vector<int>(from, from + n_bonds),
+                        std::vector<int>(to, to + n_bonds), t, u, out);
+}
+
+void mb_hubbard_destroy(mb_hubbard *h) { delete h; }
+
+int64_t mb_hubbard_dimension(const mb_hubbard *h) { return h ? h->h.dim() : -1; }
+
+int mb_hubbard_shape(const mb_hubbard *h, int64_t *d_up, int64_t *d_dn) {
+  return guarded([&] {
+    need(h && d_up && d_dn, "null argument");
+    *d_up = h->h.sec->d_up;
+    *d_dn = h->h.sec->d_dn;
+  });
+}
+
+int mb_hv_c128(const mb_hubbard *h, const void *x_dev, void *y_dev, mb_stream stream) {
+  return guarded([&] {
+    need(x_dev && y_dev && x_dev != y_dev, "x and y must be distinct device pointers");
+    mb::launch_hv_c128(mb::tables_of(H(h)), x_dev, y_dev, S(stream));
+  });
+}
+
+int mb_hv_c64(const mb_hubbard *h, const void *x_dev, void *y_dev, mb_stream stream) {
+  return guarded([&] {
+    need(x_dev && y_dev && x_dev != y_dev, "x and y must be distinct device pointers");
+    mb::launch_hv_c64(mb::tables_of(H(h)), x_dev, y_dev, S(stream));
+  });
+}
+
+int mb_hv_c64_multi(const mb_hubbard *h, int nvec, const void *x_dev, void *y_dev,
+                    mb_stream stream) {
+  return guarded([&] {
+    need(x_dev && y_dev && x_dev != y_dev, "x and y must be distinct device pointers");
+    mb::launch_hv_c64_multi(mb::tables_of(H(h)), nvec, x_dev, y_dev, S(stream));
+  });
+}
+
+int mb_to_device_order_c128(const mb_hubbard *h, const void *x_ref, void *x_dev, mb_stream stream) {
+  return guarded([&] {
+    need(x_ref && x_dev && x_ref != x_dev, "in and out must be distinct device pointers");
+    mb::launch_gather_c128(mb::perm_of(*H(h).sec), x_ref, x_dev, S(stream));
+  });
+}
+
+int mb_to_reference_order_c128(const mb_hubbard *h, const void *x_dev, void *x_ref,
+                               mb_stream stream) {
+  return guarded([&] {
+    need(x_ref && x_dev && x_ref != x_dev, "in and out must be distinct device pointers");
+    mb::launch_scatter_c128(mb::perm_of(*H(h).sec), x_dev, x_ref, S(stream));
+  });
+}
+
+int mb_apply_ref_c128(const mb_hubbard *h, const void *x_ref, void *y_ref, void *work,
+                      mb_stream stream) {
+  return guarded([&] {
+    need(x_ref && y_ref && work, "null argument");
+    const auto &hh = H(h);
+    char *w0 = (char *)work, *w1 = w0 + (size_t)hh.dim() * 16;
+    mb::launch_gather_c128(mb::perm_of(*hh.sec), x_ref, w0, S(stream));
+    mb::launch_hv_c128(mb::tables_of(hh), w0, w1, S(stream));
+    mb::launch_scatter_c128(mb::perm_of(*hh.sec), w1, y_ref, S(stream));
+  });
+}
+
+int mb_fill_gaussian_c128(const mb_hubbard *h, uint64_t seed, void *x_dev, mb_stream stream) {
+  return guarded([&] {
+    need(x_dev, "null argument");
+    mb::launch_fill_gaussian_c128(mb::perm_of(*H(h).sec), seed, x_dev, S(stream));
+  });
+}
+
+int mb_fill_phase_c64_multi(const mb_hubbard *h, uint64_t seed, int nvec, void *x_dev,
+                            mb_stream stream) {
+  return guarded([&] {
+    need(x_dev, "null argument");
+    need(nvec >= 1 && nvec <= 64, "nvec out of range");
+    mb::launch_fill_phase_c64_multi(mb::perm_of(*H(h).sec), seed, nvec, x_dev, S(stream));
+  });
+}
+
+int mb_fill_phase_c128_multi(const mb_hubbard *h, uint64_t seed, int nvec, void *x_dev,
+                             mb_stream stream) {
+  return guarded([&] {
+    need(x_dev, "null argument");
+    need(nvec >= 1 && nvec <= 64, "nvec out of range");
+    mb::launch_fill_phase_c128_multi(mb::perm_of(*H(h).sec), seed, nvec, x_dev, S(stream));
+  });
+}
+
+int mb_lanczos_pass_one(const mb_hubbard *h, const void *b_dev, int k, double *alphas,
+                        double *betas, int *steps, double *b_norm, mb_stream stream) {
+  return guarded([&] {
+    need(b_dev && alphas && betas && steps && b_norm, "null argument");
+    need(k >= 0, "k must be non-negative");
+    auto r = mb::lanczos_pass_one(H(h), b_dev, k, S(stream));
+    *steps = r.steps;
+    *b_norm = r.b_norm;
+    for (int i = 0; i < k; ++i) {
+      alphas[i] = i < (int)r.alphas.size() ? r.alphas[i] : 0.0;
+      betas[i] = i < (int)r.betas.size() ? r.betas[i] : 0.0;
+    }
+  });
+}
+
+int mb_find_min_eigenpair(const mb_hubbard *h, const void *b_dev, int k, double *value,
+                          void *x_dev, mb_stream stream) {
+  return guarded([&] {
+    need(b_dev && value, "null argument");
+    need(k >= 0, "k must be non-negative");
+    *value = mb::find_min_eigenpair(H(h), b_dev, k, x_dev, S(stream));
+  });
+}
+
+int mb_time_evolve(const mb_hubbard *h, const void *psi_dev, double t, int krylov_steps,
+                   void *out_dev, mb_stream stream) {
+  return guarded([&] {
+    need(psi_dev && out_dev, "null argument");
+    need(krylov_steps >= 0, "krylov_steps must be non-negative");
+    mb::time_evolve(H(h), psi_dev, t, krylov_steps, out_dev, S(stream));
+  });
+}
+
+int mb_time_evolve_steps(const mb_hubbard *h, const void *psi_dev, double t0, double t1,
+                         double dt, int krylov_steps, void *out_dev, mb_stream stream) {
+  return guarded([&] {
+    need(psi_dev && out_dev, "null argument");
+    need(krylov_steps >= 0, "krylov_steps must be non-negative");
+    mb::time_evolve_steps(H(h), psi_dev, t0, t1, dt, krylov_steps, out_dev, S(stream));
+  });
+}
+
+int mb_tridiag_eig(int m, const double *alphas, const double *betas, double *w, double *V) {
+  return guarded([&] {
+    need(m >= 0 && (m == 0 || (alphas && w)) && (m <= 1 || betas), "null argument");
+    mb::tridiag_eig(m, alphas, betas, w, V);
+  });
+}
+
+int mb_kpm_dos_moments(const mb_hubbard *h, double a, double b, int M, int nvec, uint64_t seed,
+                       int fp32, double *mu, mb_stream stream) {
+  return guarded([&] {
+    need(mu != nullptr, "null argument");
+    mb::kpm_dos_moments(H(h), *H(h).sec, a, b, M, nvec, seed, fp32 != 0, mu, S(stream));
+  });
+}
+
+int mb_jackson_kernel(int N, double *g) {
+  return guarded([&] {
+    need(N >= 0 && g, "invalid argument");
+    // kpm.h:19-30
+    const double pi = 3.14159265358979323846;
+    const double denom = (double)(N + 1);
+    const double cot_val = 1.0 / std::tan(pi / denom);
+    for (int n = 0; n <= N; ++n) {
+      const double nd = (double)n, arg = pi * nd / denom;
+      g[n] = ((denom - nd) * std::cos(arg) + std::sin(arg) * cot_val) / denom;
+    }
+  });
+}
+
+int mb_rescaling_from_bounds(double e_min, double e_max, double padding, double *a, double *b) {
+  return guarded([&] {
+    need(a && b, "null argument");
+    // rescaling.h:33-38
+    *a = (e_max - e_min) / 2.0 * (1.0 + padding);
+    *b = (e_max + e_min) / 2.0;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,148 @@
+// Hubbard model tables: bonds in the reference's loop order, merged hopping
+// monomials, per-species hop tables with fermionic signs, and the interaction
+// table.
+//
+// Reference: HubbardModel / HubbardModel2D / HubbardModel3D
+// (src/algebra/fermion/model/hubbard_model.h:10-212) build
+//   H = sum_bonds sum_spin -t (c+_i c_j + c+_j c_i) + U sum_i n_i,up n_i,dn
+// as a FermionExpression, where identical monomials merge by coefficient
+// addition (ExpressionBase::add_to, expression_base.h:85-97): a linear size of
+// 2 yields -2t on the repeated pair, a linear size of 1 yields the self bond
+// c+_i c_i + c+_i c_i = 2 n_i (hopping_generic, expression_base.h:394-403).
+//
+// Sign: a basis key orders all up creators (ascending orbital) before all down
+// creators (operator.h:69-79), so moving c_b to its partner and inserting c+_a
+// in canonical position passes, in total, the same-spin creators strictly
+// between a and b: <f| c+_a c_b |s> = (-1)^popcount(s & between(a, b)).  The
+// up-spin block in front of the down creators is passed twice by a down hop
+// and drops out, so each species' sign depends on its own string only.
+#include <algorithm>
+
+#include "mb_internal.hpp"
+
+namespace mb {
+
+namespace {
+
+struct Mono {
+  int a, b;  // coefficient * c+_a c_b
+  double c;
+};
+
+std::vector<Mono> merge_monomials(const std::vector<int> &from, const std::vector<int> &to,
+                                  double t) {
+  std::vector<Mono> m;
+  const double coeff = -t;
+  for (size_t k = 0; k < from.size(); ++k) {
+    const int pairs[2][2] = {{from[k], to[k]}, {to[k], from[k]}};
+    for (const auto &p : pairs) {
+      auto it = std::find_if(m.begin(), m.end(),
+                             [&](const Mono &x) { return x.a == p[0] && x.b == p[1]; });
+      if (it != m.end())
+        it->c += coeff;
+      else
+        m.push_back({p[0], p[1], coeff});
+    }
+  }
+  // ExpressionBase::is_zero (expression_base.h:79-83)
+  m.erase(std::remove_if(m.begin(), m.end(), [](const Mono &x) { return x.c * x.c < kTol * kTol; }),
+          m.end());
+  return m;
+}
+
+void species_hops(const std::vector<uint32_t> &strings, const std::vector<Mono> &mono,
+                  SpeciesHops &out) {
+  const size_t n = strings.size();
+  out.row_ptr.assign(n + 1, 0);
+  out.tgt.clear();
+  out.amp.clear();
+  out.onsite.assign(n, 0.0);
+  out.max_row = 0;
+  for (size_t r = 0; r < n; ++r) {
+    const uint32_t s = strings[r];
+    double onsite = 0.0;
+    for (const Mono &m : mono) {
+      const uint32_t ba = 1u << m.a, bb = 1u << m.b;
+      if (!(s & bb)) continue;
+      if (m.a == m.b) {  // c+_i c_i = n_i
+        onsite += m.c;
+        continue;
+      }
+      if (s & ba) continue;
+      const int lo = std::min(m.a, m.b), hi = std::max(m.a, m.b);
+      const uint32_t between = (hi - lo > 1) ? (((1u << hi) - 1u) & ~((2u << lo) - 1u)) : 0u;
+      const int sign = (__builtin_popcount(s & between) & 1) ? -1 : 1;
+      const uint32_t f = s ^ ba ^ bb;
+      out.tgt.push_back((int32_t)Sector::colex_rank(f));
+      out.amp.push_back(sign * m.c);
+    }
+    out.onsite[r] = onsite;
+    out.row_ptr[r + 1] = (int32_t)out.tgt.size();
+    out.max_row = std::max(out.max_row, out.row_ptr[r + 1] - out.row_ptr[r]);
+  }
+}
+
+void upload_species(SpeciesHops &h) {
+  h.d_row_ptr = upload(h.row_ptr);
+  h.d_tgt = upload(h.tgt);
+  h.d_amp = upload(h.amp);
+  h.d_onsite = upload(h.onsite);
+  std::vector<float> af(h.amp.begin(), h.amp.end()), of(h.onsite.begin(), h.onsite.end());
+  h.d_ampf = upload(af);
+  h.d_onsitef = upload(of);
+}
+
+}  // namespace
+
+void build_hubbard(Hubbard &h) {
+  const Sector &s = *h.sec;
+  const auto mono = merge_monomials(h.bond_from, h.bond_to, h.t);
+  species_hops(s.s_up, mono, h.up);
+  species_hops(s.s_dn, mono, h.dn);
+  h.has_onsite = false;
+  for (double v : h.up.onsite) h.has_onsite |= (v != 0.0);
+  for (double v : h.dn.onsite) h.has_onsite |= (v != 0.0);
+  // U n_up n_dn per doubly occupied site, summed term by term
+  h.diagU.assign(s.L + 1, 0.0);
+  if (!(h.U * h.U < kTol * kTol))
+    for (int k = 0; k <= s.L; ++k) {
+      double acc = 0.0;
+      for (int i = 0; i < k; ++i) acc += h.U;
+      h.diagU[k] = acc;
+    }
+  int dev = 0;
+  MB_HIP(hipGetDevice(&dev));
+  upload_species(h.up);
+  upload_species(h.dn);
+  h.d_diagU = upload(h.diagU);
+  std::vector<float> df(h.diagU.begin(), h.diagU.end());
+  h.d_diagUf = upload(df);
+  h.device = dev;
+}
+
+HvTables tables_of(const Hubbard &h) {
+  HvTables t;
+  t.d_up = h.sec->d_up;
+  t.d_dn = h.sec->d_dn;
+  t.s_up = h.sec->d_s_up.as<uint32_t>();
+  t.s_dn = h.sec->d_s_dn.as<uint32_t>();
+  t.up_ptr = h.up.d_row_ptr.as<int32_t>();
+  t.up_tgt = h.up.d_tgt.as<int32_t>();
+  t.dn_ptr = h.dn.d_row_ptr.as<int32_t>();
+  t.dn_tgt = h.dn.d_tgt.as<int32_t>();
+  t.up_amp = h.up.d_amp.as<double>();
+  t.dn_amp = h.dn.d_amp.as<double>();
+  t.up_on = h.up.d_onsite.as<double>();
+  t.dn_on = h.dn.d_onsite.as<double>();
+  t.diagU = h.d_diagU.as<double>();
+  t.up_ampf = h.up.d_ampf.as<float>();
+  t.dn_ampf = h.dn.d_ampf.as<float>();
+  t.up_onf = h.up.d_onsitef.as<float>();
+  t.dn_onf = h.dn.d_onsitef.as<float>();
+  t.diagUf = h.d_diagUf.as<float>();
+  t.has_onsite = h.has_onsite ? 1 : 0;
+  t.max_up_row = h.up.max_row;
+  return t;
+}
+
+}  // namespace mb

@@ -1,0 +1,341 @@
+// Krylov consumers of H.v on the device: Lanczos pass one / pass two, the
+// lowest eigenpair, Krylov exp(-iHt), and KPM Chebyshev moments; plus the FP64
+// symmetric tridiagonal eigensolver on the host.
+//
+// Reference: src/numerics/lanczos.h:15-290, src/numerics/evolve_state.h:22-231,
+// src/numerics/kpm.h:19-30, 184-241.  The reference keeps its vectors on the
+// host and calls LAPACK (arma::eig_sym) for T_k; here the vectors stay in HBM,
+// alpha/beta are reduced on the device without host round trips, and T_k is
+// solved on the host by implicit QL.
+#include "krylov.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+
+namespace mb {
+
+// ------------------------------------------------------ tridiagonal eig
+
+// Implicit QL with Wilkinson-style shifts on a symmetric tridiagonal matrix
+// (Golub & Van Loan, Matrix Computations, 4th ed., sec. 8.3).  d = diagonal,
+// e = off-diagonal.  Eigenvectors accumulate into Z (column-major m x m) when
+// Z != nullptr.
+void tridiag_eig(int m, const double *a, const double *b, double *w, double *V) {
+  if (m <= 0) return;
+  std::vector<double> d(a, a + m), e(m, 0.0);
+  for (int i = 0; i + 1 < m; ++i) e[i] = b[i];
+  std::vector<double> Z;
+  if (V) {
+    Z.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i) Z[i + (size_t)m * i] = 1.0;
+  }
+  const double eps = 2.220446049250313e-16;
+  for (int l = 0; l < m; ++l) {
+    int iter = 0;
+    while (true) {
+      int n = l;
+      for (; n + 1 < m; ++n) {
+        const double dd = std::fabs(d[n]) + std::fabs(d[n + 1]);
+        if (std::fabs(e[n]) <= eps * dd) break;
+      }
+      if (n == l) break;
+      if (++iter > 100) throw std::runtime_error("Eigenvalue decomposition of T_k failed");
+      // shift from the trailing 2x2 of the active block [l, n]
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = std::hypot(g, 1.0);
+      g = d[n] - d[l] + e[l] / (g + std::copysign(r, g));
+      double s = 1.0, c = 1.0, p = 0.0;
+      int i = n - 1;
+      bool deflated = false;
+      for (; i >= l; --i) {
+        const double f = s * e[i];
+        const double h = c * e[i];
+        r = std::hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[n] = 0.0;
+          deflated = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * h;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - h;
+        if (V) {
+          double *zi = &Z[(size_t)m * i], *zi1 = &Z[(size_t)m * (i + 1)];
+          for (int k = 0; k < m; ++k) {
+            const double t = zi1[k];
+            zi1[k] = s * zi[k] + c * t;
+            zi[k] = c * zi[k] - s * t;
+          }
+        }
+      }
+      if (deflated) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[n] = 0.0;
+    }
+  }
+  // ascending order
+  std::vector<int> idx(m);
+  for (int i = 0; i < m; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
+  for (int j = 0; j < m; ++j) {
+    w[j] = d[idx[j]];
+    if (V)
+      for (int k = 0; k < m; ++k) V[k + (size_t)m * j] = Z[k + (size_t)m * idx[j]];
+  }
+}
+
+// ------------------------------------------------------------ Lanczos
+
+LanczosWork::LanczosWork(int64_t D, int k)
+    : D(D), k(k), v(D * 16), vp(D * 16), w(D * 16),
+      scal(sizeof(double) * (2 * (size_t)std::max(k, 1) + 2 +
+                             (size_t)std::max(kRedBlocks, lanczos_partial_blocks(D))) +
+           16) {
+  double *base = scal.as<double>();
+  st.alpha = base;
+  st.beta = base + std::max(k, 1);
+  st.bnorm = base + 2 * std::max(k, 1);
+  st.partial = base + 2 * std::max(k, 1) + 2;
+  st.flags = reinterpret_cast<int *>(base + 2 * std::max(k, 1) + 1);
+}
+
+LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream_t s,
+                               LanczosWork *work) {
+  const int64_t D = h.dim();
+  std::unique_ptr<LanczosWork> own;
+  if (!work || work->D != D || work->k < k) {
+    own.reset(new LanczosWork(D, k));
+    work = own.get();
+  }
+  LanczosWork &W = *work;
+  const HvTables t = tables_of(h);
+  LanczosResult res;
+  launch_norm_c128((const double *)b, 2 * D, W.st.partial, W.st.bnorm, s);
+  MB_HIP(hipMemcpyAsync(&res.b_norm, W.st.bnorm, sizeof(double), hipMemcpyDeviceToHost, s));
+  MB_HIP(hipStreamSynchronize(s));
+  if (res.b_norm <= kTol) throw std::runtime_error("Input vector has zero norm");
+  if (k == 0) return res;
+  launch_scale_div_c128((const double *)b, W.st.bnorm, W.v.as<double>(), 2 * D, s);
+  MB_HIP(hipMemsetAsync(W.vp.p, 0, (size_t)D * 16, s));
+  MB_HIP(hipMemsetAsync(W.st.flags, 0, 2 * sizeof(int), s));
+  double *v = W.v.as<double>(), *vp = W.vp.as<double>(), *w = W.w.as<double>();
+  const int nb = lanczos_partial_blocks(D);
+  for (int i = 0; i < k; ++i) {
+    launch_lanczos_hv(t, v, vp, w, W.st, i, s);
+    launch_lanczos_finish_alpha(W.st, i, nb, s);
+    launch_lanczos_axpy_norm(v, w, 2 * D, W.st, i, s);
+    launch_lanczos_finish_beta(W.st, i, k, s);
+    launch_lanczos_normalize(w, 2 * D, W.st, i, s);
+    double *tmp = vp;
+    vp = v;
+    v = w;
+    w = tmp;
+  }
+  std::vector<double> al(k), be(k);
+  int flags[2];
+  MB_HIP(hipMemcpyAsync(al.data(), W.st.alpha, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+  MB_HIP(hipMemcpyAsync(be.data(), W.st.beta, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+  MB_HIP(hipMemcpyAsync(flags, W.st.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+  MB_HIP(hipStreamSynchronize(s));
+  res.steps = flags[1];
+  res.alphas.assign(al.begin(), al.begin() + res.steps);
+  res.betas.assign(be.begin(), be.begin() + std::max(res.steps - 1, 0));
+  return res;
+}
+
+void lanczos_pass_two(const Hubbard &h, const void *b, const LanczosResult &dec,
+                      const std::vector<std::complex<double>> &y, void *x, hipStream_t s,
+                      LanczosWork *work) {
+  const int64_t D = h.dim();
+  if ((int)y.size() != dec.steps)
+    throw std::runtime_error("Dimension mismatch: y_k size must match decomposition steps");
+  if (dec.b_norm <= kTol) throw std::runtime_error("Input vector has zero norm");
+  if (dec.steps == 0) {
+    MB_HIP(hipMemsetAsync(x, 0, (size_t)D * 16, s));
+    return;
+  }
+  std::unique_ptr<LanczosWork> own;
+  if (!work || work->D != D) {
+    own.reset(new LanczosWork(D, 1));
+    work = own.get();
+  }
+  LanczosWork &W = *work;
+  const HvTables t = tables_of(h);
+  MB_HIP(hipMemcpyAsync(W.st.bnorm, &dec.b_norm, sizeof(double), hipMemcpyHostToDevice, s));
+  double *v = W.v.as<double>(), *vp = W.vp.as<double>(), *w = W.w.as<double>();
+  launch_scale_div_c128((const double *)b, W.st.bnorm, v, 2 * D, s);
+  MB_HIP(hipMemsetAsync(vp, 0, (size_t)D * 16, s));
+  launch_scale_complex_c128(v, y[0].real(), y[0].imag(), (double *)x, D, s);
+  for (int j = 0; j + 1 < dec.steps; ++j) {
+    const double aj = dec.alphas[j], bj = dec.betas[j], bp = j == 0 ? 0.0 : dec.betas[j - 1];
+    launch_recon_hv(t, v, vp, w, aj, bp, s);
+    launch_recon_update(w, (double *)x, bj, y[j + 1].real(), y[j + 1].imag(), D, s);
+    double *tmp = vp;
+    vp = v;
+    v = w;
+    w = tmp;
+  }
+  MB_HIP(hipStreamSynchronize(s));
+}
+
+double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipStream_t s) {
+  LanczosWork work(h.dim(), k);
+  LanczosResult dec = lanczos_pass_one(h, b, k, s, &work);
+  const int m = dec.steps;
+  if (m == 0) {
+    if (x) MB_HIP(hipMemsetAsync(x, 0, (size_t)h.dim() * 16, s));
+    MB_HIP(hipStreamSynchronize(s));
+    return 0.0;
+  }
+  std::vector<double> wv(m), V((size_t)m * m);
+  tridiag_eig(m, dec.alphas.data(), dec.betas.data(), wv.data(), x ? V.data() : nullptr);
+  if (x) {
+    std::vector<std::complex<double>> y(m);
+    for (int i = 0; i < m; ++i) y[i] = V[i] * dec.b_norm;  // column 0: lowest
+    lanczos_pass_two(h, b, dec, y, x, s, &work);
+  }
+  return wv[0];
+}
+
+// y = exp(-i t T) e1 (evolve_state.h:22-61)
+std::vector<std::complex<double>> expm_first_column(const LanczosResult &dec, double t) {
+  const int m = dec.steps;
+  std::vector<double> wv(m), Q((size_t)m * m);
+  tridiag_eig(m, dec.alphas.data(), dec.betas.data(), wv.data(), Q.data());
+  std::vector<std::complex<double>> y(m, 0.0);
+  for (int j = 0; j < m; ++j) {
+    const std::complex<double> ph = std::exp(std::complex<double>(0.0, -t) * wv[j]);
+    const double q0 = Q[0 + (size_t)m * j];
+    for (int i = 0; i < m; ++i) y[i] += Q[i + (size_t)m * j] * (ph * q0);
+  }
+  return y;
+}
+
+void time_evolve(const Hubbard &h, const void *psi, double t, int krylov_steps, void *out,
+                 hipStream_t s) {
+  const int64_t D = h.dim();
+  std::vector<double> scratch;
+  int k = krylov_steps;
+  if (k == 0 || k > D) k = (int)D;
+  LanczosWork work(D, k);
+  // norm first: a zero state is returned unchanged (evolve_state.h:131-134)
+  double nrm = 0.0;
+  launch_norm_c128((const double *)psi, 2 * D, work.st.partial, work.st.bnorm, s);
+  MB_HIP(hipMemcpyAsync(&nrm, work.st.bnorm, sizeof(double), hipMemcpyDeviceToHost, s));
+  MB_HIP(hipStreamSynchronize(s));
+  if (nrm <= kTol) {
+    if (out != psi) MB_HIP(hipMemcpyAsync(out, psi, (size_t)D * 16, hipMemcpyDeviceToDevice, s));
+    MB_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  LanczosResult dec = lanczos_pass_one(h, psi, k, s, &work);
+  if (dec.steps == 0) {
+    MB_HIP(hipMemsetAsync(out, 0, (size_t)D * 16, s));
+    MB_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  auto y = expm_first_column(dec, t);
+  for (auto &v : y) v *= dec.b_norm;
+  if (out == psi) {
+    DevBuf tmp((size_t)D * 16);
+    lanczos_pass_two(h, psi, dec, y, tmp.p, s, &work);
+    MB_HIP(hipMemcpyAsync(out, tmp.p, (size_t)D * 16, hipMemcpyDeviceToDevice, s));
+    MB_HIP(hipStreamSynchronize(s));
+  } else {
+    lanczos_pass_two(h, psi, dec, y, out, s, &work);
+  }
+}
+
+void time_evolve_steps(const Hubbard &h, const void *psi, double t0, double t1, double dt,
+                       int krylov_steps, void *out, hipStream_t s) {
+  const int64_t D = h.dim();
+  if (out != psi) MB_HIP(hipMemcpyAsync(out, psi, (size_t)D * 16, hipMemcpyDeviceToDevice, s));
+  if (dt == 0.0 || (t1 - t0) == 0.0) {
+    MB_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  const double direction = (t1 >= t0) ? 1.0 : -1.0;
+  dt = std::fabs(dt) * direction;
+  double t = t0;
+  while ((direction > 0 && t < t1) || (direction < 0 && t > t1)) {
+    const double step = (direction > 0) ? std::min(dt, t1 - t) : std::max(dt, t1 - t);
+    time_evolve(h, out, step, krylov_steps, out, s);
+    t += step;
+  }
+}
+
+// ------------------------------------------------------------ KPM
+
+void kpm_dos_moments(const Hubbard &h, const Sector &sec, double a, double b, int M, int nvec,
+                     uint64_t seed, bool fp32, double *mu, hipStream_t s) {
+  if (M < 2 || (M & 1)) throw std::invalid_argument("M must be even and >= 2");
+  if (nvec != 1 && nvec != 2 && nvec != 4 && nvec != 8 && nvec != 16)
+    throw std::invalid_argument("nvec must be 1, 2, 4, 8 or 16");
+  if (!(a > 0.0)) throw std::invalid_argument("rescaling factor a must be positive");
+  const int64_t D = h.dim();
+  const size_t esz = fp32 ? 8 : 16;
+  const size_t bytes = (size_t)D * nvec * esz;
+  DevBuf T0(bytes), T1(bytes), T2(bytes);
+  const int nb = cheb_partial_blocks();
+  const int width = 2 * nvec;
+  const int steps = M / 2;
+  DevBuf partial((size_t)nb * width * sizeof(double));
+  DevBuf raw((size_t)(steps + 1) * width * sizeof(double));
+  const HvTables t = tables_of(h);
+  const PermTables p = perm_of(sec);
+  if (fp32)
+    launch_fill_phase_c64_multi(p, seed, nvec, T0.p, s);
+  else
+    launch_fill_phase_c128_multi(p, seed, nvec, T0.p, s);
+  double *rawp = raw.as<double>();
+  // row 0: <T0|T0> per vector (second half of the row)
+  launch_sumsq_multi(fp32, nvec, T0.p, D, partial.as<double>(), s);
+  launch_reduce_partials(partial.as<double>(), nb, width, rawp, s);
+  void *tn = T0.p, *tprev = T2.p, *tnext = T1.p;
+  for (int n = 0; n < steps; ++n) {
+    const int first = n == 0;
+    const double scale2 = first ? 1.0 / a : 2.0 / a;
+    if (fp32)
+      launch_cheb_c64(t, nvec, (const float *)tn, (const float *)tprev, (float *)tnext,
+                      (float)scale2, (float)b, first, partial.as<double>(), s);
+    else
+      launch_cheb_c128(t, nvec, (const double *)tn, (const double *)tprev, (double *)tnext,
+                       scale2, b, first, partial.as<double>(), s);
+    launch_reduce_partials(partial.as<double>(), nb, width, rawp + (size_t)(n + 1) * width, s);
+    void *old_prev = tprev;
+    tprev = tn;
+    tn = tnext;
+    tnext = old_prev;
+  }
+  std::vector<double> hr((size_t)(steps + 1) * width);
+  MB_HIP(hipMemcpyAsync(hr.data(), rawp, hr.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  MB_HIP(hipStreamSynchronize(s));
+  // per vector r: mu0 = <T0|T0>, mu1 = <T1|T0>; step n (T_{n+1} from T_n):
+  // mu_{2n+1} = 2 <T_{n+1}|T_n> - mu1, mu_{2n+2} = 2 <T_{n+1}|T_{n+1}> - mu0
+  // (step 0: mu1 itself and mu2 = 2 <T1|T1> - mu0)
+  std::vector<double> acc(M + 1, 0.0);
+  for (int r = 0; r < nvec; ++r) {
+    const double mu0 = hr[nvec + r];
+    const double mu1 = hr[(size_t)1 * width + r];
+    acc[0] += mu0;
+    for (int n = 0; n < steps; ++n) {
+      const double cross = hr[(size_t)(n + 1) * width + r];
+      const double self = hr[(size_t)(n + 1) * width + nvec + r];
+      const int o = 2 * n + 1;
+      acc[o] += (n == 0) ? cross : 2.0 * cross - mu1;
+      if (o + 1 <= M) acc[o + 1] += 2.0 * self - mu0;
+    }
+  }
+  const double norm = 1.0 / ((double)nvec * (double)D);
+  for (int n = 0; n < M; ++n) mu[n] = acc[n] * norm;
+}
+
+}  // namespace mb

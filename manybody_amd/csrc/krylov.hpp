@@ -1,0 +1,47 @@
+// Host drivers of the device Krylov layer (see krylov.cpp).
+#pragma once
+
+#include <complex>
+#include <memory>
+#include <vector>
+
+#include "mb_internal.hpp"
+
+namespace mb {
+
+// LanczosDecomposition (lanczos.h:60-68)
+struct LanczosResult {
+  std::vector<double> alphas, betas;
+  int steps = 0;
+  double b_norm = 0.0;
+};
+
+// Device workspace of one Lanczos run: three complex128 vectors and the
+// device-resident scalars (alpha, beta, partial sums, flags).
+struct LanczosWork {
+  int64_t D;
+  int k;
+  DevBuf v, vp, w, scal;
+  LanczosState st;
+  LanczosWork(int64_t D, int k);
+};
+
+LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream_t s,
+                               LanczosWork *work = nullptr);
+void lanczos_pass_two(const Hubbard &h, const void *b, const LanczosResult &dec,
+                      const std::vector<std::complex<double>> &y, void *x, hipStream_t s,
+                      LanczosWork *work = nullptr);
+double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipStream_t s);
+std::vector<std::complex<double>> expm_first_column(const LanczosResult &dec, double t);
+void time_evolve(const Hubbard &h, const void *psi, double t, int krylov_steps, void *out,
+                 hipStream_t s);
+void time_evolve_steps(const Hubbard &h, const void *psi, double t0, double t1, double dt,
+                       int krylov_steps, void *out, hipStream_t s);
+void kpm_dos_moments(const Hubbard &h, const Sector &sec, double a, double b, int M, int nvec,
+                     uint64_t seed, bool fp32, double *mu, hipStream_t s);
+
+// sum |x_r|^2 per vector into partial[blk * 2 nvec + nvec + r] (first half zero)
+void launch_sumsq_multi(bool fp32, int nvec, const void *x, int64_t D, double *partial,
+                        hipStream_t s);
+
+}  // namespace mb

@@ -1,0 +1,288 @@
+"""numpy front end of the oracle (oracle/hubbard_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline,
+never by the product package.
+
+Every function mirrors one reference routine; the citations are on the C side.
+Complex vectors are numpy complex128 arrays in reference index order unless a
+function says "product order".
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_c128p = np.ctypeslib.ndpointer(np.complex128, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(_LIB_PATH)
+        L.orc_choose.restype = C.c_uint64
+        L.orc_choose.argtypes = [C.c_uint64, C.c_uint64]
+        L.orc_basis_enumerate.restype = C.c_int64
+        L.orc_basis_enumerate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_hubbard_bonds.restype = C.c_int
+        L.orc_hubbard_bonds.argtypes = [C.c_int, _i32p, _i32p, _i32p, C.c_int]
+        L.orc_build_csc.restype = C.c_int64
+        L.orc_build_csc.argtypes = [C.c_int, _u64p, _u64p, C.c_int64, C.c_int, _i32p, _i32p,
+                                    C.c_double, C.c_double, _i64p, C.c_void_p, C.c_void_p,
+                                    C.c_int64, C.c_int]
+        L.orc_spmv_csc.restype = None
+        L.orc_spmv_csc.argtypes = [C.c_int64, _i64p, _i64p, _c128p, _c128p, _c128p]
+        L.orc_species_table.restype = C.c_int64
+        L.orc_species_table.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, C.c_double,
+                                        _u64p, _i64p, _i32p, _f64p, _f64p, C.c_int64]
+        prod_args = [C.c_int64, C.c_int64, _u64p, _u64p, _i64p, _i32p, _f64p, _f64p,
+                     _i64p, _i32p, _f64p, _f64p, _f64p]
+        L.orc_hv_product.restype = None
+        L.orc_hv_product.argtypes = prod_args + [_c128p, _c128p, C.c_int]
+        L.orc_diag_table.restype = None
+        L.orc_diag_table.argtypes = [C.c_double, C.c_int, _f64p]
+        L.orc_lanczos_csc.restype = C.c_int
+        L.orc_lanczos_csc.argtypes = [C.c_int64, _i64p, _i64p, _c128p, C.c_int, _c128p, C.c_int,
+                                      _f64p, _f64p, C.POINTER(C.c_double)]
+        L.orc_lanczos_product.restype = C.c_int
+        L.orc_lanczos_product.argtypes = prod_args + [_c128p, C.c_int, _f64p, _f64p,
+                                                      C.POINTER(C.c_double), C.c_int]
+        L.orc_tridiag_eigvals.restype = None
+        L.orc_tridiag_eigvals.argtypes = [C.c_int, _f64p, _f64p, _f64p]
+        L.orc_jacobi_eig.restype = None
+        L.orc_jacobi_eig.argtypes = [C.c_int, _f64p, _f64p, _f64p]
+        L.orc_lanczos_pass_two_csc.restype = None
+        L.orc_lanczos_pass_two_csc.argtypes = [C.c_int64, _i64p, _i64p, _c128p, C.c_int, _c128p,
+                                               C.c_int, _f64p, _f64p, C.c_double, _f64p, _c128p]
+        L.orc_time_evolve_csc.restype = None
+        L.orc_time_evolve_csc.argtypes = [C.c_int64, _i64p, _i64p, _c128p, _c128p, C.c_double,
+                                          C.c_int, _c128p]
+        L.orc_kpm_moments_csc.restype = None
+        L.orc_kpm_moments_csc.argtypes = [C.c_int64, _i64p, _i64p, _c128p, C.c_double, C.c_double,
+                                          C.c_int, C.c_int, _c128p, _f64p]
+        L.orc_jackson_kernel.restype = None
+        L.orc_jackson_kernel.argtypes = [C.c_int, _f64p]
+        _lib = L
+    return _lib
+
+
+def choose(n: int, m: int) -> int:
+    return int(lib().orc_choose(n, m))
+
+
+def basis_states(n_sites: int, n_up: int, n_dn: int):
+    """(up_bits, dn_bits) arrays in reference index order (basis.cpp:267-302)."""
+    D = lib().orc_basis_enumerate(n_sites, n_up, n_dn, None, None, 0)
+    if D < 0:
+        raise ValueError("invalid sector")
+    up = np.zeros(D, np.uint64)
+    dn = np.zeros(D, np.uint64)
+    got = lib().orc_basis_enumerate(n_sites, n_up, n_dn, up.ctypes.data, dn.ctypes.data, D)
+    assert got == D
+    return up, dn
+
+
+def hubbard_bonds(sizes) -> np.ndarray:
+    """Bond list (from, to) of HubbardModel{,2D,3D} in loop order."""
+    sz = np.asarray(sizes, np.int32)
+    cap = 3 * int(np.prod(sz)) + 3
+    f = np.zeros(cap, np.int32)
+    t = np.zeros(cap, np.int32)
+    n = lib().orc_hubbard_bonds(len(sz), sz, f, t, cap)
+    if n < 0:
+        raise ValueError("bad lattice")
+    return np.stack([f[:n], t[:n]], axis=1)
+
+
+@dataclass
+class CSC:
+    D: int
+    col_ptr: np.ndarray
+    row_idx: np.ndarray
+    val: np.ndarray  # complex128
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        y = np.zeros(self.D, np.complex128)
+        lib().orc_spmv_csc(self.D, self.col_ptr, self.row_idx, self.val,
+                           np.ascontiguousarray(x, np.complex128), y)
+        return y
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csc_matrix((self.val, self.row_idx, self.col_ptr), shape=(self.D, self.D))
+
+
+def build_csc(sizes, n_up: int, n_dn: int, t: float, U: float, threads: int = 0,
+              states=None) -> CSC:
+    """compute_matrix_elements<sp_cx_mat>(basis, HubbardModel*.hamiltonian())."""
+    L = int(np.prod(sizes))
+    up, dn = states if states is not None else basis_states(L, n_up, n_dn)
+    bonds = hubbard_bonds(sizes)
+    f = np.ascontiguousarray(bonds[:, 0])
+    to = np.ascontiguousarray(bonds[:, 1])
+    D = len(up)
+    cp = np.zeros(D + 1, np.int64)
+    nnz = lib().orc_build_csc(L, up, dn, D, len(f), f, to, t, U, cp, None, None, 0, threads)
+    if nnz < 0:
+        raise RuntimeError("CSC build failed")
+    ri = np.zeros(max(nnz, 1), np.int64)
+    val = np.zeros(max(nnz, 1), np.complex128)
+    got = lib().orc_build_csc(L, up, dn, D, len(f), f, to, t, U, cp, ri.ctypes.data,
+                              val.ctypes.data, nnz, threads)
+    assert got == nnz
+    return CSC(D, cp, ri[:nnz].copy(), val[:nnz].copy())
+
+
+@dataclass
+class Species:
+    strings: np.ndarray
+    row_ptr: np.ndarray
+    cols: np.ndarray
+    vals: np.ndarray
+    onsite: np.ndarray
+
+
+def species_table(sizes, n: int, t: float) -> Species:
+    L = int(np.prod(sizes))
+    bonds = hubbard_bonds(sizes)
+    f = np.ascontiguousarray(bonds[:, 0])
+    to = np.ascontiguousarray(bonds[:, 1])
+    Ds = choose(L, n)
+    cap = Ds * (2 * len(f) + 1)
+    strings = np.zeros(Ds, np.uint64)
+    rp = np.zeros(Ds + 1, np.int64)
+    cols = np.zeros(max(cap, 1), np.int32)
+    vals = np.zeros(max(cap, 1), np.float64)
+    onsite = np.zeros(Ds, np.float64)
+    nnz = lib().orc_species_table(L, n, len(f), f, to, t, strings, rp, cols, vals, onsite, cap)
+    if nnz < 0:
+        raise RuntimeError("species table failed")
+    return Species(strings, rp, cols[:nnz].copy(), vals[:nnz].copy(), onsite)
+
+
+class ProductOperator:
+    """Matrix-free H in product order x[u * D_dn + d] (u, d = ascending bitstring ranks)."""
+
+    def __init__(self, sizes, n_up: int, n_dn: int, t: float, U: float, threads: int = 0):
+        L = int(np.prod(sizes))
+        self.up = species_table(sizes, n_up, t)
+        self.dn = species_table(sizes, n_dn, t)
+        self.diagU = np.zeros(L + 1, np.float64)
+        lib().orc_diag_table(U, L, self.diagU)
+        self.Dup = len(self.up.strings)
+        self.Ddn = len(self.dn.strings)
+        self.D = self.Dup * self.Ddn
+        self.threads = threads
+
+    def _args(self):
+        u, d = self.up, self.dn
+        return (self.Dup, self.Ddn, u.strings, d.strings, u.row_ptr, u.cols, u.vals, u.onsite,
+                d.row_ptr, d.cols, d.vals, d.onsite, self.diagU)
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        y = np.zeros(self.D, np.complex128)
+        lib().orc_hv_product(*self._args(), np.ascontiguousarray(x, np.complex128), y,
+                             self.threads)
+        return y
+
+    def lanczos(self, b: np.ndarray, k: int):
+        al = np.zeros(max(k, 1))
+        be = np.zeros(max(k, 1))
+        bn = C.c_double(0.0)
+        m = lib().orc_lanczos_product(*self._args(), np.ascontiguousarray(b, np.complex128), k,
+                                      al, be, C.byref(bn), self.threads)
+        if m < 0:
+            raise RuntimeError("Input vector has zero norm")
+        return al[:m].copy(), be[:max(m - 1, 0)].copy(), bn.value
+
+
+def product_permutation(sizes, n_up: int, n_dn: int, ref_states=None) -> np.ndarray:
+    """perm[ref_index] = product index u * D_dn + d, from the DFS enumeration
+    and sorted single-species strings (independent of any closed-form rank)."""
+    L = int(np.prod(sizes))
+    up, dn = ref_states if ref_states is not None else basis_states(L, n_up, n_dn)
+    su = species_table(sizes, n_up, 1.0).strings
+    sd = species_table(sizes, n_dn, 1.0).strings
+    u = np.searchsorted(su, up)
+    d = np.searchsorted(sd, dn)
+    return (u.astype(np.int64) * len(sd) + d).astype(np.int64)
+
+
+def lanczos_csc(A: CSC, b: np.ndarray, k: int, negate: bool = False):
+    al = np.zeros(max(k, 1))
+    be = np.zeros(max(k, 1))
+    bn = C.c_double(0.0)
+    m = lib().orc_lanczos_csc(A.D, A.col_ptr, A.row_idx, A.val, int(negate),
+                              np.ascontiguousarray(b, np.complex128), k, al, be, C.byref(bn))
+    if m < 0:
+        raise RuntimeError("Input vector has zero norm")
+    return al[:m].copy(), be[:max(m - 1, 0)].copy(), bn.value
+
+
+def tridiag_eigvals(a, b) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.float64)
+    m = len(a)
+    bb = np.ascontiguousarray(np.resize(np.asarray(b, np.float64), max(m - 1, 1)), np.float64)
+    if m > 1:
+        bb[: m - 1] = np.asarray(b, np.float64)[: m - 1]
+    w = np.zeros(m)
+    lib().orc_tridiag_eigvals(m, a, bb, w)
+    return w
+
+
+def jacobi_eig(A: np.ndarray):
+    A = np.ascontiguousarray(A, np.float64)
+    m = A.shape[0]
+    w = np.zeros(m)
+    V = np.zeros((m, m))
+    lib().orc_jacobi_eig(m, A, w, V)
+    return w, V
+
+
+def find_min_eigenpair_csc(A: CSC, b: np.ndarray, k: int):
+    """find_min_eigenpair (lanczos.h:283-290) with an explicit start vector:
+    Lanczos on -H, top eigenpair of T, pass two on -H."""
+    al, be, bn = lanczos_csc(A, b, k, negate=True)
+    m = len(al)
+    T = np.diag(al) + np.diag(be[: m - 1], 1) + np.diag(be[: m - 1], -1)
+    w, V = jacobi_eig(T)
+    y = np.ascontiguousarray(V[:, m - 1] * bn)
+    x = np.zeros(A.D, np.complex128)
+    lib().orc_lanczos_pass_two_csc(A.D, A.col_ptr, A.row_idx, A.val, 1,
+                                   np.ascontiguousarray(b, np.complex128), m, al,
+                                   np.ascontiguousarray(np.resize(be, max(m, 1))), bn, y, x)
+    return -w[m - 1], x
+
+
+def time_evolve_csc(A: CSC, psi: np.ndarray, t: float, k: int = 30) -> np.ndarray:
+    out = np.zeros(A.D, np.complex128)
+    lib().orc_time_evolve_csc(A.D, A.col_ptr, A.row_idx, A.val,
+                              np.ascontiguousarray(psi, np.complex128), t, k, out)
+    return out
+
+
+def kpm_moments_csc(A: CSC, a: float, b: float, M: int, vecs: np.ndarray) -> np.ndarray:
+    vecs = np.ascontiguousarray(vecs, np.complex128)
+    R = vecs.shape[0]
+    mu = np.zeros(M)
+    lib().orc_kpm_moments_csc(A.D, A.col_ptr, A.row_idx, A.val, a, b, M, R, vecs, mu)
+    return mu
+
+
+def jackson_kernel(N: int) -> np.ndarray:
+    g = np.zeros(N + 1)
+    lib().orc_jackson_kernel(N, g)
+    return g

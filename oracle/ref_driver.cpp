@@ -1,0 +1,112 @@
+// Checker-only driver around the reference's own algebra core (compiled from the
+// reference sources where they lie; see oracle/Makefile).  It is NOT part of the
+// product and is never built or run on the GPU box: it exists to write the small
+// golden fixtures under tests/golden/ that pin the oracle's basis order, bond
+// lists, fermionic signs and diagonal against the reference itself.
+//
+// Usage (all output is plain text on stdout):
+//   ref_driver basis  L  Nup Ndn            -> "S up_bits dn_bits" per state, in index order
+//   ref_driver h1d    L  Nup Ndn t U        -> states, then "E i j re im" per nonzero
+//   ref_driver h2d    Lx Ly Nup Ndn t U
+//   ref_driver h3d    Lx Ly Lz Nup Ndn t U
+//
+// Reference entry points exercised:
+//   FermionBasis::with_fixed_particle_number_and_spin   src/algebra/basis.cpp:91-126
+//   HubbardModel / HubbardModel2D / HubbardModel3D      src/algebra/fermion/model/hubbard_model.h:10-212
+//   compute_matrix_elements<MatrixType>                  src/algebra/matrix_elements.h:74-105
+//
+// The reference key holds at most 7 operators and silently drops longer
+// products (src/algebra/fermion/term.h:11-19, src/algebra/expression_base.h:184-186),
+// so the caller must keep Nup+Ndn <= 7 for "basis", and Nup+Ndn <= 3 for full
+// Hamiltonians (<= 5 when U == 0).  The driver refuses larger sectors.
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "algebra/fermion/basis.h"
+#include "algebra/fermion/model/hubbard_model.h"
+#include "algebra/matrix_elements.h"
+
+namespace {
+
+// Minimal dense matrix satisfying what compute_matrix_elements needs:
+// (rows, cols) construction, zeros() and operator()(i, j).
+struct DenseComplex {
+  size_t n_rows, n_cols;
+  std::vector<std::complex<double>> a;
+  DenseComplex(size_t r, size_t c) : n_rows(r), n_cols(c), a(r * c) {}
+  void zeros() { std::fill(a.begin(), a.end(), std::complex<double>(0.0, 0.0)); }
+  std::complex<double>& operator()(size_t i, size_t j) { return a[i * n_cols + j]; }
+};
+
+void print_states(const FermionBasis& basis) {
+  for (size_t i = 0; i < basis.set.size(); ++i) {
+    uint64_t up = 0, dn = 0;
+    for (const auto& op : basis.set[i]) {
+      if (op.spin() == FermionOperator::Spin::Up)
+        up |= (uint64_t{1} << op.value());
+      else
+        dn |= (uint64_t{1} << op.value());
+    }
+    std::printf("S %llu %llu\n", (unsigned long long)up, (unsigned long long)dn);
+  }
+}
+
+template <typename Model>
+void print_matrix(const FermionBasis& basis, const Model& model) {
+  auto H = model.hamiltonian();
+  DenseComplex M = compute_matrix_elements<DenseComplex>(basis, H);
+  for (size_t i = 0; i < M.n_rows; ++i)
+    for (size_t j = 0; j < M.n_cols; ++j) {
+      const auto v = M(i, j);
+      if (v.real() != 0.0 || v.imag() != 0.0)
+        std::printf("E %zu %zu %.17g %.17g\n", i, j, v.real(), v.imag());
+    }
+}
+
+int usage() {
+  std::fprintf(stderr, "usage: ref_driver basis|h1d|h2d|h3d ...\n");
+  return 2;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2) return usage();
+  const std::string mode = argv[1];
+  std::vector<double> a;
+  for (int i = 2; i < argc; ++i) a.push_back(std::atof(argv[i]));
+
+  size_t sites = 0, nup = 0, ndn = 0;
+  double t = 0, u = 0;
+  if (mode == "basis" && a.size() == 3) {
+    sites = (size_t)a[0]; nup = (size_t)a[1]; ndn = (size_t)a[2];
+  } else if (mode == "h1d" && a.size() == 5) {
+    sites = (size_t)a[0]; nup = (size_t)a[1]; ndn = (size_t)a[2]; t = a[3]; u = a[4];
+  } else if (mode == "h2d" && a.size() == 6) {
+    sites = (size_t)a[0] * (size_t)a[1]; nup = (size_t)a[2]; ndn = (size_t)a[3]; t = a[4]; u = a[5];
+  } else if (mode == "h3d" && a.size() == 7) {
+    sites = (size_t)a[0] * (size_t)a[1] * (size_t)a[2];
+    nup = (size_t)a[3]; ndn = (size_t)a[4]; t = a[5]; u = a[6];
+  } else {
+    return usage();
+  }
+  const size_t n = nup + ndn;
+  if (n > 7) { std::fprintf(stderr, "sector exceeds the 7-operator key\n"); return 3; }
+  if (mode != "basis" && n > (u == 0.0 ? 5u : 3u)) {
+    std::fprintf(stderr, "sector exceeds the exact range of compute_matrix_elements\n");
+    return 3;
+  }
+
+  auto basis = FermionBasis::with_fixed_particle_number_and_spin(sites, n, (int)nup - (int)ndn);
+  print_states(basis);
+  if (mode == "h1d") print_matrix(basis, HubbardModel(t, u, sites));
+  if (mode == "h2d") print_matrix(basis, HubbardModel2D(t, u, (size_t)a[0], (size_t)a[1]));
+  if (mode == "h3d")
+    print_matrix(basis, HubbardModel3D(t, u, (size_t)a[0], (size_t)a[1], (size_t)a[2]));
+  return 0;
+}

@@ -1,0 +1,119 @@
+"""Write the golden fixtures in tests/golden/ from the compiled reference.
+
+Runs oracle/_ref/ref_driver (the reference's own algebra core, compiled from the
+reference sources by `make -C oracle ref`) on small sectors and stores what it
+prints: the basis states in reference index order, and the nonzero matrix
+elements of HubbardModel / HubbardModel2D / HubbardModel3D.  Only sectors the
+reference represents exactly are used: N_up + N_dn <= 7 for the basis, <= 3 for
+the full Hamiltonian, <= 5 with U = 0 (7-operator key, fermion/term.h:11-19,
+dropped long products, expression_base.h:184-186).
+
+Run in the build container only (the reference does not exist on the GPU box):
+    python tests/golden/make_golden.py
+Outputs: tests/golden/ref_basis.npz, tests/golden/ref_hamiltonian.npz,
+tests/golden/manifest.json.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+DRIVER = os.path.join(REPO, "oracle", "_ref", "ref_driver")
+
+
+def run(args):
+    out = subprocess.run([DRIVER] + [str(a) for a in args], check=True, capture_output=True,
+                         text=True).stdout
+    states, elems = [], []
+    for line in out.splitlines():
+        p = line.split()
+        if p[0] == "S":
+            states.append((int(p[1]), int(p[2])))
+        elif p[0] == "E":
+            elems.append((int(p[1]), int(p[2]), float(p[3]), float(p[4])))
+    return np.array(states, np.uint64).reshape(-1, 2), elems
+
+
+def digest(states: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(states, np.uint64).tobytes()).hexdigest()
+
+
+def main():
+    if not os.path.exists(DRIVER):
+        sys.exit("build the reference checker first: make -C oracle ref")
+    basis_arrays, basis_manifest = {}, []
+    # every sector with L <= 7 and N <= 7: full lists for L <= 5, digests beyond
+    for L in range(1, 8):
+        for nu in range(0, L + 1):
+            for nd in range(0, L + 1):
+                if nu + nd > 7 or nu + nd == 0:
+                    continue
+                st, _ = run(["basis", L, nu, nd])
+                name = f"b_{L}_{nu}_{nd}"
+                entry = {"name": name, "L": L, "n_up": nu, "n_dn": nd, "dim": int(len(st)),
+                         "sha256": digest(st)}
+                if L <= 5:
+                    basis_arrays[name] = st
+                    entry["stored"] = True
+                basis_manifest.append(entry)
+
+    ham_cases = [
+        # (kind, dims, n_up, n_dn, t, U)
+        ("h1d", [2], 1, 1, 1.0, 4.0),
+        ("h1d", [3], 1, 1, 1.0, 4.0),
+        ("h1d", [4], 1, 1, 1.0, 4.0),
+        ("h1d", [4], 2, 1, 1.0, 4.0),
+        ("h1d", [5], 2, 1, 0.7, 2.5),
+        ("h1d", [6], 1, 2, 1.0, 8.0),
+        ("h1d", [8], 2, 1, 1.0, 4.0),
+        ("h1d", [8], 3, 0, 1.0, 4.0),
+        ("h1d", [8], 0, 3, -0.3, 1.0),
+        ("h1d", [7], 3, 2, 1.0, 0.0),
+        ("h1d", [8], 3, 2, 1.25, 0.0),
+        ("h1d", [1], 1, 1, 1.0, 3.0),
+        ("h2d", [2, 2], 1, 1, 1.0, 4.0),
+        ("h2d", [2, 2], 2, 1, 1.0, 4.0),
+        ("h2d", [3, 3], 1, 1, 1.0, 8.0),
+        ("h2d", [3, 3], 2, 1, 1.0, 8.0),
+        ("h2d", [4, 4], 2, 1, 1.0, 8.0),
+        ("h2d", [4, 4], 1, 1, 0.5, 3.0),
+        ("h2d", [2, 3], 2, 1, 1.0, 2.0),
+        ("h2d", [3, 2], 1, 2, 1.0, 2.0),
+        ("h2d", [3, 1], 1, 1, 1.0, 2.0),
+        ("h2d", [1, 3], 2, 1, 1.0, 2.0),
+        ("h2d", [4, 3], 1, 2, 1.0, 6.0),
+        ("h2d", [3, 3], 3, 2, 1.0, 0.0),
+        ("h2d", [4, 2], 2, 2, 1.0, 0.0),
+        ("h3d", [2, 2, 2], 1, 1, 1.0, 4.0),
+        ("h3d", [2, 2, 2], 2, 1, 1.0, 4.0),
+        ("h3d", [3, 2, 2], 1, 1, 1.0, 5.0),
+    ]
+    ham_arrays, ham_manifest = {}, []
+    for kind, dims, nu, nd, t, U in ham_cases:
+        st, el = run([kind] + dims + [nu, nd, t, U])
+        name = f"{kind}_{'x'.join(map(str, dims))}_{nu}_{nd}_t{t}_U{U}"
+        el = np.array(el, np.float64).reshape(-1, 4)
+        ham_arrays[name + "__states"] = st
+        ham_arrays[name + "__ij"] = el[:, :2].astype(np.int64)
+        ham_arrays[name + "__val"] = el[:, 2] + 1j * el[:, 3]
+        ham_manifest.append({"name": name, "kind": kind, "dims": dims, "n_up": nu, "n_dn": nd,
+                             "t": t, "U": U, "dim": int(len(st)), "nnz": int(len(el))})
+
+    np.savez_compressed(os.path.join(HERE, "ref_basis.npz"), **basis_arrays)
+    np.savez_compressed(os.path.join(HERE, "ref_hamiltonian.npz"), **ham_arrays)
+    with open(os.path.join(HERE, "manifest.json"), "w") as f:
+        json.dump({"generator": "tests/golden/make_golden.py",
+                   "source": "reference algebra core compiled by oracle/Makefile (target ref)",
+                   "basis": basis_manifest, "hamiltonian": ham_manifest}, f, indent=1)
+    print(f"{len(basis_manifest)} basis sectors, {len(ham_manifest)} Hamiltonians")
+
+
+if __name__ == "__main__":
+    main()

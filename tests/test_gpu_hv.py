@@ -1,0 +1,157 @@
+"""GPU parity of the matrix-free H.v kernels against the oracle.
+
+Integer-valued inputs (entries in [-4, 4]) make every sum exact in FP64 (and in
+FP32 for these sizes), so the device result must equal the oracle's bit for
+bit: that pins indices, signs, bonds and the diagonal.  Gaussian inputs are
+checked to a relative 1e-14 (FP64; the summation order differs from the
+oracle's).  The full 4 x 4, 7 + 7 sector (D = 130,873,600) is checked bit for
+bit too.
+"""
+import numpy as np
+import pytest
+
+import manybody_amd as mb
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ("1d_L8_4_4", [8], 4, 4, 1.0, 4.0),
+    ("1d_L10_5_5", [10], 5, 5, 1.0, 6.0),
+    ("1d_L12_6_5", [12], 6, 5, 1.0, 3.0),
+    ("1d_L7_0_3", [7], 0, 3, 1.0, 2.0),
+    ("2d_2x2_2_1", [2, 2], 2, 1, 1.0, 4.0),
+    ("2d_3x3_4_3", [3, 3], 4, 3, 1.0, 8.0),
+    ("2d_4x3_3_3", [4, 3], 3, 3, 0.5, 2.5),
+    ("2d_3x1_2_1_selfbond", [3, 1], 2, 1, 1.0, 2.0),
+    ("2d_4x4_3_2", [4, 4], 3, 2, 1.0, 8.0),
+    ("3d_2x2x2_3_3", [2, 2, 2], 3, 3, 1.0, 4.0),
+]
+
+
+def model_of(sizes, t, U):
+    if len(sizes) == 1:
+        return mb.HubbardModel(t, U, sizes[0])
+    if len(sizes) == 2:
+        return mb.HubbardModel2D(t, U, *sizes)
+    return mb.HubbardModel3D(t, U, *sizes)
+
+
+def make_op(sizes, nu, nd, t, U):
+    L = int(np.prod(sizes))
+    basis = mb.FermionBasis.sector(L, nu, nd)
+    return model_of(sizes, t, U).operator(basis)
+
+
+def int_vec(rng, n):
+    return (rng.integers(-4, 5, n) + 1j * rng.integers(-4, 5, n)).astype(np.complex128)
+
+
+@pytest.mark.parametrize("name,sizes,nu,nd,t,U", CASES, ids=[c[0] for c in CASES])
+def test_hv_c128_bitexact_vs_product_oracle(cuda, name, sizes, nu, nd, t, U):
+    import torch
+    op = make_op(sizes, nu, nd, t, U)
+    P = po.ProductOperator(sizes, nu, nd, t, U)
+    assert (op.d_up, op.d_dn) == (P.Dup, P.Ddn)
+    x = int_vec(np.random.default_rng(1), P.D)
+    y = op.hv(torch.from_numpy(x).to(cuda)).cpu().numpy()
+    assert np.array_equal(y, P.matvec(x))
+
+
+@pytest.mark.parametrize("name,sizes,nu,nd,t,U", CASES[:6], ids=[c[0] for c in CASES[:6]])
+def test_apply_reference_order_bitexact_vs_csc_oracle(cuda, name, sizes, nu, nd, t, U):
+    """The whole boundary: reference order in, gather, H.v, scatter, reference order out."""
+    import torch
+    op = make_op(sizes, nu, nd, t, U)
+    A = po.build_csc(sizes, nu, nd, t, U)
+    x = int_vec(np.random.default_rng(2), A.D)
+    y = op.apply(torch.from_numpy(x).to(cuda)).cpu().numpy()
+    assert np.array_equal(y, A.matvec(x))
+
+
+def test_permutation_matches_oracle_and_roundtrips(cuda):
+    import torch
+    sizes, nu, nd = [3, 3], 4, 3
+    op = make_op(sizes, nu, nd, 1.0, 1.0)
+    perm = po.product_permutation(sizes, nu, nd)
+    x_ref = np.arange(op.dimension()).astype(np.complex128) * (1 + 2j)
+    xd = op.to_device_order(torch.from_numpy(x_ref).to(cuda))
+    expect = np.zeros_like(x_ref)
+    expect[perm] = x_ref
+    assert np.array_equal(xd.cpu().numpy(), expect)
+    back = op.to_reference_order(xd).cpu().numpy()
+    assert np.array_equal(back, x_ref)
+
+
+@pytest.mark.parametrize("name,sizes,nu,nd,t,U", CASES[:4], ids=[c[0] for c in CASES[:4]])
+def test_hv_c64_bitexact_vs_product_oracle(cuda, name, sizes, nu, nd, t, U):
+    import torch
+    op = make_op(sizes, nu, nd, t, U)
+    P = po.ProductOperator(sizes, nu, nd, t, U)
+    x = int_vec(np.random.default_rng(3), P.D)
+    y = op.hv(torch.from_numpy(x.astype(np.complex64)).to(cuda)).cpu().numpy()
+    assert np.array_equal(y.astype(np.complex128), P.matvec(x))
+
+
+def test_hv_c64_multi_matches_single(cuda):
+    import torch
+    sizes, nu, nd = [10], 5, 4
+    op = make_op(sizes, nu, nd, 1.0, 4.0)
+    P = po.ProductOperator(sizes, nu, nd, 1.0, 4.0)
+    rng = np.random.default_rng(4)
+    for nvec in (2, 4, 8, 16):
+        X = np.stack([int_vec(rng, P.D) for _ in range(nvec)], axis=1)  # element-major
+        Y = op.hv(torch.from_numpy(X.astype(np.complex64).ravel()).to(cuda)).cpu().numpy()
+        Y = Y.reshape(P.D, nvec).astype(np.complex128)
+        for r in range(nvec):
+            assert np.array_equal(Y[:, r], P.matvec(np.ascontiguousarray(X[:, r]))), (nvec, r)
+
+
+@pytest.mark.parametrize("name,sizes,nu,nd,t,U", CASES[:3], ids=[c[0] for c in CASES[:3]])
+def test_hv_c128_gaussian_tolerance(cuda, name, sizes, nu, nd, t, U):
+    import torch
+    op = make_op(sizes, nu, nd, t, U)
+    P = po.ProductOperator(sizes, nu, nd, t, U)
+    x_t = op.gaussian(seed=123)
+    x = x_t.cpu().numpy()
+    y = op.hv(x_t).cpu().numpy()
+    ref = P.matvec(x)
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-14
+
+
+def test_gaussian_fill_is_seeded_per_reference_index(cuda):
+    import torch
+    op = make_op([10], 5, 5, 1.0, 4.0)
+    a = op.gaussian(7)
+    b = op.gaussian(7)
+    c = op.gaussian(8)
+    assert torch.equal(a, b) and not torch.equal(a, c)
+    v = a.cpu().numpy()
+    assert abs(v.real.mean()) < 0.01 and abs(v.imag.mean()) < 0.01
+    assert abs(v.real.var() - 1) < 0.01 and abs(v.imag.var() - 1) < 0.01
+
+
+def test_hv_c2_full_size_bitexact(cuda):
+    """4 x 4 periodic, U = 8, 7 + 7: D = 130,873,600, integer inputs, bit for bit."""
+    import torch
+    sizes, nu, nd, t, U = [4, 4], 7, 7, 1.0, 8.0
+    op = make_op(sizes, nu, nd, t, U)
+    assert op.dimension() == 130873600
+    P = po.ProductOperator(sizes, nu, nd, t, U, threads=16)
+    x = int_vec(np.random.default_rng(5), P.D)
+    xd = torch.from_numpy(x).to(cuda)
+    y = op.hv(xd).cpu().numpy()
+    del xd
+    ref = P.matvec(x)
+    assert np.array_equal(y, ref)
+
+
+def test_hv_c2_hermitian(cuda):
+    """<x|H y> = <H x|y> on seeded Gaussian vectors at full size (FP64)."""
+    import torch
+    op = make_op([4, 4], 7, 7, 1.0, 8.0)
+    x = op.gaussian(1)
+    y = op.gaussian(2)
+    lhs = torch.vdot(x, op.hv(y)).item()
+    rhs = torch.vdot(op.hv(x), y).item()
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12

@@ -1,0 +1,104 @@
+"""CPU-side checks of libmanybody_hip.so: it loads, exports every symbol the C
+header declares, and its host-only entry points (basis ranks, the tridiagonal
+eigensolver, Jackson kernel, rescaling, argument validation) agree with the
+oracle.  No GPU compute is called here."""
+import re
+
+import numpy as np
+import pytest
+
+import manybody_amd as mb
+from oracle import pyoracle as po
+
+
+def header_symbols():
+    with open(mb.HEADER_PATH) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mb_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = mb.lib()
+    syms = header_symbols()
+    assert len(syms) >= 29
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(mb._SIGS), "ctypes signatures must cover the header"
+
+
+def test_version():
+    assert b"gfx950" in mb.lib().mb_version()
+
+
+@pytest.mark.parametrize("L,nu,nd", [(2, 1, 1), (4, 2, 2), (6, 3, 2), (7, 0, 3), (8, 4, 4), (5, 5, 1),
+                                     (9, 4, 3), (1, 1, 1)])
+def test_basis_state_and_index_match_dfs(L, nu, nd):
+    B = mb.FermionBasis.sector(L, nu, nd)
+    up, dn = po.basis_states(L, nu, nd)
+    assert B.size() == len(up)
+    for i in range(len(up)):
+        assert B.state(i) == (int(up[i]), int(dn[i]))
+        assert B.index_of(int(up[i]), int(dn[i])) == i
+
+
+def test_product_index_matches_oracle_permutation():
+    for sizes, nu, nd in [([8], 4, 4), ([3, 3], 4, 3), ([10], 5, 5)]:
+        L = int(np.prod(sizes))
+        B = mb.FermionBasis.sector(L, nu, nd)
+        assert np.array_equal(B.product_index(), po.product_permutation(sizes, nu, nd))
+
+
+def test_c2_rank_subsample_matches_dfs():
+    # 4 x 4, 7 + 7: D = 130,873,600; compare a strided subsample of the DFS order
+    up, dn = po.basis_states(16, 7, 7)
+    B = mb.FermionBasis.sector(16, 7, 7)
+    assert B.size() == len(up) == 130873600
+    for i in list(range(0, len(up), 9_999_991)) + [len(up) - 1]:
+        assert B.state(i) == (int(up[i]), int(dn[i]))
+        assert B.index_of(int(up[i]), int(dn[i])) == i
+
+
+def test_basis_errors_follow_reference_contracts():
+    with pytest.raises(ValueError):
+        mb.FermionBasis.with_fixed_particle_number_and_spin(4, 3, 0)  # parity assert
+    with pytest.raises(ValueError):
+        mb.FermionBasis.with_fixed_particle_number_and_spin(4, 2, 4)  # |Sz| > N
+    with pytest.raises(ValueError):
+        mb.FermionBasis.with_fixed_particle_number_and_spin(3, 8, 0)  # N > 2L
+    B = mb.FermionBasis.sector(4, 2, 1)
+    with pytest.raises(IndexError):
+        B.state(B.size())
+    with pytest.raises(IndexError):
+        B.index_of(0b0111, 0b0001)  # wrong up count: not in the basis
+    assert not B.contains(0b1000, 0b0001) and B.contains(0b1001, 0b0001)
+
+
+def test_tridiag_eig_matches_numpy_and_toeplitz():
+    rng = np.random.default_rng(11)
+    for m in [1, 2, 5, 30, 200]:
+        a = rng.standard_normal(m)
+        b = rng.standard_normal(max(m - 1, 0))
+        T = np.diag(a) + (np.diag(b, 1) + np.diag(b, -1) if m > 1 else 0)
+        w, V = mb.tridiag_eig(a, b)
+        assert np.abs(w - np.linalg.eigvalsh(T)).max() < 1e-12 * max(1, np.abs(w).max())
+        assert np.abs(T @ V - V * w).max() < 1e-11
+        assert np.abs(V.T @ V - np.eye(m)).max() < 1e-12
+    n = 300
+    w, _ = mb.tridiag_eig(np.full(n, 2.0), np.full(n - 1, -1.0))
+    exact = np.sort(2.0 - 2.0 * np.cos(np.arange(1, n + 1) * np.pi / (n + 1)))
+    assert np.abs(w - exact).max() < 1e-12
+
+
+def test_tridiag_eig_agrees_with_oracle_bisection():
+    rng = np.random.default_rng(12)
+    a, b = rng.standard_normal(150), np.abs(rng.standard_normal(149))
+    w, _ = mb.tridiag_eig(a, b, vectors=False)
+    assert np.abs(w - po.tridiag_eigvals(a, b)).max() < 1e-12
+
+
+def test_jackson_kernel_and_rescaling_match_oracle():
+    for N in [1, 50, 1023]:
+        assert np.array_equal(np.array(mb.jackson_kernel(N)), po.jackson_kernel(N))
+    a, b = mb.rescaling_from_bounds(-3.0, 5.0, 0.01)
+    assert a == (5.0 - -3.0) / 2.0 * 1.01 and b == 1.0

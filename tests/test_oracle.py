@@ -1,0 +1,277 @@
+"""Pins the oracle (oracle/hubbard_oracle.c) before it is trusted as the checker.
+
+Each oracle function is checked against something that can catch a plausible
+mistake in it:
+  basis order            golden vectors written by the compiled reference (146 sectors)
+  CSC matrix build       golden matrices from the compiled reference (28 models: 1D/2D/3D,
+                         merged double bonds, self bonds, non-integer t/U), Hermiticity,
+                         U = 0 free-fermion closed forms and t = 0 energies beyond N = 3
+  product-order H.v      the CSC build through the DFS permutation, bit for bit
+  Lanczos                scipy eigsh, ports of test_lanczos.cpp known answers
+  tridiagonal/Jacobi     Toeplitz closed form, numpy
+  Krylov expm            scipy expm, port of test_evolve_state.cpp
+  KPM moments            exact spectral sum from a dense eigendecomposition
+  Jackson kernel         port of test_kpm.cpp properties, closed form
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse.linalg as spla
+
+from oracle import pyoracle as po
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _manifest():
+    with open(os.path.join(GOLD, "manifest.json")) as f:
+        return json.load(f)
+
+
+def csc_from_dense(M):
+    import scipy.sparse as sp
+    A = sp.csc_matrix(np.asarray(M, np.complex128))
+    A.sort_indices()
+    return po.CSC(A.shape[0], A.indptr.astype(np.int64), A.indices.astype(np.int64),
+                  A.data.astype(np.complex128))
+
+
+def free_fermion_e0(sizes, n_up, n_dn, t=1.0):
+    """Sum of the lowest single-particle levels of -t * (periodic nearest-neighbour
+    hopping, merged bonds included), per spin."""
+    L = int(np.prod(sizes))
+    h = np.zeros((L, L))
+    for f, to in po.hubbard_bonds(sizes):
+        h[f, to] += -t
+        h[to, f] += -t
+    eps = np.sort(np.linalg.eigvalsh(h))
+    return eps[:n_up].sum() + eps[:n_dn].sum()
+
+
+# ------------------------------------------------------------------ basis
+
+def test_choose_matches_math():
+    import math
+    for n in range(0, 33):
+        for m in range(0, n + 2):
+            assert po.choose(n, m) == (math.comb(n, m) if m <= n else 0)
+
+
+def test_basis_order_matches_reference_golden():
+    man = _manifest()
+    stored = np.load(os.path.join(GOLD, "ref_basis.npz"))
+    assert len(man["basis"]) == 146
+    for e in man["basis"]:
+        up, dn = po.basis_states(e["L"], e["n_up"], e["n_dn"])
+        st = np.stack([up, dn], axis=1)
+        assert len(st) == e["dim"]
+        assert hashlib.sha256(np.ascontiguousarray(st, np.uint64).tobytes()).hexdigest() == e["sha256"], e["name"]
+        if e.get("stored"):
+            assert np.array_equal(st, stored[e["name"]])
+
+
+def test_basis_small_example_order():
+    # L = 2, one up, one down: (up0 dn0), (up0 dn1), (up1 dn1), (dn0 up1)
+    up, dn = po.basis_states(2, 1, 1)
+    assert list(zip(up.tolist(), dn.tolist())) == [(1, 1), (1, 2), (2, 2), (2, 1)]
+
+
+def test_basis_size_formula_and_membership():
+    for L, a, b in [(8, 4, 4), (6, 3, 1), (10, 5, 5), (9, 0, 4)]:
+        up, dn = po.basis_states(L, a, b)
+        assert len(up) == po.choose(L, a) * po.choose(L, b)
+        assert all(bin(int(u)).count("1") == a for u in up[:200])
+        assert len(set(zip(up.tolist(), dn.tolist()))) == len(up)
+
+
+# ------------------------------------------------------------------ matrix
+
+def test_csc_matches_reference_golden():
+    man = _manifest()
+    H = np.load(os.path.join(GOLD, "ref_hamiltonian.npz"))
+    assert len(man["hamiltonian"]) == 28
+    for e in man["hamiltonian"]:
+        n = e["name"]
+        A = po.build_csc(e["dims"], e["n_up"], e["n_dn"], e["t"], e["U"])
+        M = A.to_scipy().toarray()
+        R = np.zeros_like(M)
+        ij = H[n + "__ij"]
+        R[ij[:, 0], ij[:, 1]] = H[n + "__val"]
+        assert np.array_equal(M, R), n
+
+
+@pytest.mark.parametrize("sizes,nu,nd,U", [([8], 4, 4, 4.0), ([3, 3], 3, 3, 8.0), ([2, 2, 2], 3, 2, 5.0)])
+def test_csc_hermitian_beyond_reference_range(sizes, nu, nd, U):
+    A = po.build_csc(sizes, nu, nd, 1.0, U).to_scipy()
+    assert abs(A - A.getH()).max() == 0
+
+
+@pytest.mark.parametrize("sizes,nu,nd", [([8], 4, 4), ([10], 5, 5), ([3, 3], 4, 3), ([4, 2], 3, 3),
+                                         ([2, 2, 2], 2, 3), ([4, 4], 2, 2)])
+def test_u0_ground_state_is_free_fermion_sum(sizes, nu, nd):
+    A = po.build_csc(sizes, nu, nd, 1.0, 0.0).to_scipy().real
+    e0 = spla.eigsh(A, k=1, which="SA", tol=1e-12)[0][0]
+    assert abs(e0 - free_fermion_e0(sizes, nu, nd)) < 1e-9
+
+
+def test_u0_closed_form_c1():
+    # 1D L = 8, 4 + 4, U = 0: E0 = 2 * sum of the lowest four -2 cos(2 pi k / 8)
+    A = po.build_csc([8], 4, 4, 1.0, 0.0).to_scipy().real
+    e0 = spla.eigsh(A, k=1, which="SA", tol=1e-13)[0][0]
+    assert abs(e0 - (-9.656854249492380)) < 1e-10
+
+
+@pytest.mark.parametrize("sizes,nu,nd,U", [([6], 3, 3, 4.0), ([6], 4, 3, 2.5), ([3, 3], 5, 5, 8.0)])
+def test_t0_energy_counts_double_occupancy(sizes, nu, nd, U):
+    A = po.build_csc(sizes, nu, nd, 0.0, U).to_scipy()
+    L = int(np.prod(sizes))
+    d = A.diagonal().real
+    assert A.nnz == np.count_nonzero(d)
+    assert abs(d.min() - U * max(0, nu + nd - L)) < 1e-12
+
+
+# --------------------------------------------------------- product order
+
+@pytest.mark.parametrize("sizes,nu,nd,t,U", [([8], 4, 4, 1.0, 4.0), ([3, 3], 4, 3, 1.0, 8.0),
+                                             ([4, 2], 3, 2, 0.5, 3.0), ([2, 2, 2], 3, 3, 1.0, 4.0),
+                                             ([3, 1], 2, 1, 1.0, 2.0), ([10], 3, 2, 1.0, 6.0),
+                                             ([2, 3], 3, 3, -1.0, 5.0)])
+def test_product_order_oracle_matches_csc_bitwise(sizes, nu, nd, t, U):
+    A = po.build_csc(sizes, nu, nd, t, U)
+    P = po.ProductOperator(sizes, nu, nd, t, U)
+    perm = po.product_permutation(sizes, nu, nd)
+    rng = np.random.default_rng(7)
+    x_ref = (rng.integers(-4, 5, A.D) + 1j * rng.integers(-4, 5, A.D)).astype(np.complex128)
+    y_ref = A.matvec(x_ref)
+    x_dev = np.zeros_like(x_ref)
+    x_dev[perm] = x_ref
+    y_dev = P.matvec(x_dev)
+    assert np.array_equal(y_dev[perm], y_ref)
+
+
+# ------------------------------------------------------------------ Lanczos
+
+def test_lanczos_oracle_energy_matches_eigsh_c1():
+    A = po.build_csc([8], 4, 4, 1.0, 4.0)
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(A.D) + 1j * rng.standard_normal(A.D)
+    al, be, bn = po.lanczos_csc(A, b, 200)
+    w = po.tridiag_eigvals(al, be)
+    e_ref = spla.eigsh(A.to_scipy().real, k=1, which="SA", tol=1e-14)[0][0]
+    assert abs(w[0] - e_ref) / abs(e_ref) < 1e-10
+    assert abs(bn - np.linalg.norm(b)) < 1e-12
+
+
+def test_lanczos_negated_mirrors_alphas():
+    A = po.build_csc([6], 3, 2, 1.0, 3.0)
+    rng = np.random.default_rng(2)
+    b = rng.standard_normal(A.D) + 1j * rng.standard_normal(A.D)
+    al, be, _ = po.lanczos_csc(A, b, 30)
+    aln, ben, _ = po.lanczos_csc(A, b, 30, negate=True)
+    assert np.array_equal(aln, -al) and np.array_equal(ben, be)
+
+
+def test_lanczos_diagonal_known_answer():
+    # test_lanczos.cpp: DiagonalOperator {1, 3, -2}: max 3, min -2
+    A = csc_from_dense(np.diag([1.0, 3.0, -2.0]))
+    b = np.array([0.3, 0.5, 0.7], np.complex128)
+    val, x = po.find_min_eigenpair_csc(A, b, 3)
+    assert abs(val + 2.0) < 1e-6
+    Ax = A.matvec(x)
+    assert abs(np.vdot(x, Ax).real / np.vdot(x, x).real - val) < 1e-6
+
+
+def test_lanczos_chain_known_answer():
+    # test_lanczos.cpp:221-236: 80-site chain, diag 2, hopping -1, 50 steps
+    n = 80
+    M = 2.0 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    A = csc_from_dense(-M)  # max eigenpair of M = -(min of -M)
+    b = np.ones(n, np.complex128) + 0.1 * np.arange(n)
+    val, _ = po.find_min_eigenpair_csc(A, b, 50)
+    exact_max = 2.0 - 2.0 * np.cos(n * np.pi / (n + 1.0))
+    assert abs(-val - exact_max) < 2e-2
+
+
+def test_lanczos_breakdown_and_zero_norm():
+    A = csc_from_dense(np.diag([1.0, 2.0, 3.0, 4.0]))
+    b = np.array([1.0, 0, 0, 0], np.complex128)  # eigenvector: breakdown after one step
+    al, be, bn = po.lanczos_csc(A, b, 4)
+    assert len(al) == 1 and len(be) == 0 and al[0] == 1.0
+    with pytest.raises(RuntimeError):
+        po.lanczos_csc(A, np.zeros(4, np.complex128), 4)
+
+
+# ------------------------------------------------------- small eigensolvers
+
+def test_tridiag_eigvals_toeplitz_closed_form():
+    n, a, b = 40, 1.5, -0.7
+    w = po.tridiag_eigvals(np.full(n, a), np.full(n - 1, b))
+    exact = np.sort(a + 2 * b * np.cos(np.arange(1, n + 1) * np.pi / (n + 1)))
+    assert np.abs(w - exact).max() < 1e-12
+
+
+def test_tridiag_and_jacobi_match_numpy():
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(25)
+    b = rng.standard_normal(24)
+    T = np.diag(a) + np.diag(b, 1) + np.diag(b, -1)
+    ref = np.linalg.eigvalsh(T)
+    assert np.abs(po.tridiag_eigvals(a, b) - ref).max() < 1e-12
+    w, V = po.jacobi_eig(T)
+    assert np.abs(w - ref).max() < 1e-12
+    assert np.abs(T @ V - V * w).max() < 1e-11
+
+
+# ------------------------------------------------------------------ Krylov
+
+def test_time_evolve_matches_expm_full_krylov():
+    A = po.build_csc([4], 2, 2, 1.0, 3.0)
+    H = A.to_scipy().toarray()
+    rng = np.random.default_rng(4)
+    psi = rng.standard_normal(A.D) + 1j * rng.standard_normal(A.D)
+    out = po.time_evolve_csc(A, psi, 0.37, k=0)
+    ref = sla.expm(-1j * 0.37 * H) @ psi
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-10
+
+
+def test_time_evolve_two_level_known_answer():
+    # test_evolve_state.cpp: H = diag(1, 2), psi0 = (1, i), t = 0.3 (three steps of 0.1)
+    A = csc_from_dense(np.diag([1.0, 2.0]))
+    psi = np.array([1.0, 1j])
+    for _ in range(3):
+        psi = po.time_evolve_csc(A, psi, 0.1, k=2)
+    ref = np.array([np.exp(-0.3j), 1j * np.exp(-0.6j)])
+    assert np.linalg.norm(psi - ref) / np.linalg.norm(ref) < 1e-8
+
+
+# ------------------------------------------------------------------ KPM
+
+def test_kpm_moments_match_spectral_sum():
+    A = po.build_csc([6], 2, 2, 1.0, 4.0)
+    H = A.to_scipy().toarray().real
+    lam, Q = np.linalg.eigh(H)
+    a, b = (lam[-1] - lam[0]) / 2 * 1.05, (lam[-1] + lam[0]) / 2
+    rng = np.random.default_rng(5)
+    R, M = 3, 40
+    vecs = np.exp(2j * np.pi * rng.random((R, A.D)))
+    mu = po.kpm_moments_csc(A, a, b, M, vecs)
+    x = (lam - b) / a
+    w = (np.abs(Q.T @ vecs.T) ** 2).sum(axis=1)
+    ref = np.array([(w * np.cos(n * np.arccos(x))).sum() for n in range(M)]) / (R * A.D)
+    assert np.abs(mu - ref).max() < 1e-12
+    assert abs(mu[0] - 1.0) < 1e-14
+
+
+def test_jackson_kernel_properties_and_closed_form():
+    g = po.jackson_kernel(50)
+    assert abs(g[0] - 1.0) < 1e-10
+    assert np.all(g >= -1e-10) and np.all(g <= 1 + 1e-10)
+    assert np.all(np.diff(g) <= 1e-10)
+    N = 50
+    n = np.arange(N + 1)
+    ref = ((N + 1 - n) * np.cos(np.pi * n / (N + 1)) + np.sin(np.pi * n / (N + 1)) / np.tan(np.pi / (N + 1))) / (N + 1)
+    assert np.abs(g - ref).max() < 1e-13
