@@ -1,0 +1,278 @@
+// manybody_amd.hpp — C++ host API over the C ABI (manybody_hip.h), shaped like
+// keyehzy/manybody's own API for the Hubbard H.v path so that a reference user
+// can switch with few edits.  Header-only; link libmanybody_hip.so and
+// libamdhip64.
+//
+//   reference                                       here
+//   FermionBasis::with_fixed_particle_number_and_    mbamd::FermionBasis::with_fixed_...
+//     spin            (src/algebra/basis.h:25-26)
+//   HubbardModel{,2D,3D}(t, u, sizes...)             mbamd::HubbardModel{,2D,3D}
+//     (src/algebra/fermion/model/hubbard_model.h)
+//   LinearOperator<cx_vec>::apply / dimension        mbamd::HubbardOperator::apply / dimension
+//     (src/numerics/linear_operator.h:34-43)
+//   LanczosDecomposition, lanczos_pass_one           mbamd::LanczosDecomposition, lanczos_pass_one
+//     (src/numerics/lanczos.h:60-68, 130-175)
+//   EigenPair, find_min_eigenpair (:239-290)         mbamd::EigenPair, find_min_eigenpair
+//   EvolutionOptions, time_evolve_state[_steps]      mbamd::EvolutionOptions, time_evolve_state[_steps]
+//     (src/numerics/evolve_state.h:110-231)
+//   kpm::jackson_kernel, rescaling::from_bounds      mbamd::kpm::jackson_kernel, mbamd::rescaling::from_bounds
+//
+// Vectors are device buffers (DeviceVector, complex128, device product order);
+// to_device_order / to_reference_order convert from and to the reference's
+// basis index order.  Errors throw the reference's exception types.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "manybody_hip.h"
+
+namespace mbamd {
+
+inline void check(int status) {
+  if (status == MB_OK) return;
+  const std::string msg = mb_last_error();
+  switch (status) {
+    case MB_ERR_INVALID: throw std::invalid_argument(msg);
+    case MB_ERR_RANGE: throw std::out_of_range(msg);
+    default: throw std::runtime_error(msg);
+  }
+}
+
+// Owning device buffer of n complex128 values.
+class DeviceVector {
+ public:
+  DeviceVector() = default;
+  explicit DeviceVector(size_t n) : n_(n) {
+    if (hipMalloc(&p_, n * sizeof(std::complex<double>) + 1) != hipSuccess)
+      throw std::runtime_error("hipMalloc failed");
+  }
+  DeviceVector(const DeviceVector &) = delete;
+  DeviceVector &operator=(const DeviceVector &) = delete;
+  DeviceVector(DeviceVector &&o) noexcept : p_(std::exchange(o.p_, nullptr)), n_(o.n_) {}
+  DeviceVector &operator=(DeviceVector &&o) noexcept {
+    std::swap(p_, o.p_);
+    std::swap(n_, o.n_);
+    return *this;
+  }
+  ~DeviceVector() {
+    if (p_) (void)hipFree(p_);
+  }
+  void *data() const { return p_; }
+  size_t size() const { return n_; }
+  std::vector<std::complex<double>> to_host() const {
+    std::vector<std::complex<double>> h(n_);
+    if (hipMemcpy(h.data(), p_, n_ * sizeof(h[0]), hipMemcpyDeviceToHost) != hipSuccess)
+      throw std::runtime_error("hipMemcpy failed");
+    return h;
+  }
+  static DeviceVector from_host(const std::vector<std::complex<double>> &h) {
+    DeviceVector d(h.size());
+    if (hipMemcpy(d.p_, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice) != hipSuccess)
+      throw std::runtime_error("hipMemcpy failed");
+    return d;
+  }
+
+ private:
+  void *p_ = nullptr;
+  size_t n_ = 0;
+};
+
+class FermionBasis {
+ public:
+  static FermionBasis with_fixed_particle_number_and_spin(size_t orbitals, size_t particles,
+                                                          int spin_projection) {
+    mb_basis *b = nullptr;
+    check(mb_basis_create((int)orbitals, (int)particles, spin_projection, &b));
+    return FermionBasis(b, orbitals, particles);
+  }
+  FermionBasis(FermionBasis &&o) noexcept
+      : h_(std::exchange(o.h_, nullptr)), orbitals(o.orbitals), particles(o.particles) {}
+  FermionBasis(const FermionBasis &) = delete;
+  ~FermionBasis() { mb_basis_destroy(h_); }
+
+  size_t size() const { return (size_t)mb_basis_size(h_); }
+  // basis.set[index] as (up, down) occupation bitmasks
+  std::pair<uint64_t, uint64_t> state(size_t index) const {
+    uint64_t u, d;
+    check(mb_basis_state(h_, (int64_t)index, &u, &d));
+    return {u, d};
+  }
+  size_t index_of(uint64_t up, uint64_t dn) const {
+    int64_t i;
+    check(mb_basis_index_of(h_, up, dn, &i));
+    return (size_t)i;
+  }
+  bool contains(uint64_t up, uint64_t dn) const {
+    int64_t i;
+    return mb_basis_index_of(h_, up, dn, &i) == MB_OK;
+  }
+  const mb_basis *handle() const { return h_; }
+
+ private:
+  FermionBasis(mb_basis *h, size_t o, size_t p) : h_(h), orbitals(o), particles(p) {}
+  mb_basis *h_;
+
+ public:
+  size_t orbitals;
+  size_t particles;
+};
+
+class HubbardOperator {
+ public:
+  using ScalarType = std::complex<double>;
+  HubbardOperator(const FermionBasis &basis, const std::vector<int> &sizes, double t, double u) {
+    check(mb_hubbard_create(basis.handle(), (int)sizes.size(), sizes.data(), t, u, &h_));
+  }
+  HubbardOperator(HubbardOperator &&o) noexcept : h_(std::exchange(o.h_, nullptr)) {}
+  HubbardOperator(const HubbardOperator &) = delete;
+  ~HubbardOperator() { mb_hubbard_destroy(h_); }
+
+  size_t dimension() const { return (size_t)mb_hubbard_dimension(h_); }
+  // LinearOperator::apply: a new vector (device order)
+  DeviceVector apply(const DeviceVector &v, hipStream_t s = nullptr) const {
+    DeviceVector y(dimension());
+    check(mb_hv_c128(h_, v.data(), y.data(), s));
+    check(hipStreamSynchronize(s) == hipSuccess ? MB_OK : MB_ERR_DEVICE);
+    return y;
+  }
+  void apply_into(const DeviceVector &v, DeviceVector &y, hipStream_t s = nullptr) const {
+    check(mb_hv_c128(h_, v.data(), y.data(), s));
+  }
+  DeviceVector to_device_order(const DeviceVector &x_ref, hipStream_t s = nullptr) const {
+    DeviceVector y(dimension());
+    check(mb_to_device_order_c128(h_, x_ref.data(), y.data(), s));
+    return y;
+  }
+  DeviceVector to_reference_order(const DeviceVector &x_dev, hipStream_t s = nullptr) const {
+    DeviceVector y(dimension());
+    check(mb_to_reference_order_c128(h_, x_dev.data(), y.data(), s));
+    return y;
+  }
+  DeviceVector gaussian(uint64_t seed, hipStream_t s = nullptr) const {
+    DeviceVector y(dimension());
+    check(mb_fill_gaussian_c128(h_, seed, y.data(), s));
+    return y;
+  }
+  const mb_hubbard *handle() const { return h_; }
+
+ private:
+  mb_hubbard *h_ = nullptr;
+};
+
+struct HubbardModel {
+  HubbardModel(double t_val, double u_val, size_t size_val) : t(t_val), u(u_val), size(size_val) {}
+  HubbardOperator op(const FermionBasis &b) const { return {b, {(int)size}, t, u}; }
+  double t, u;
+  size_t size;
+};
+struct HubbardModel2D {
+  HubbardModel2D(double t_val, double u_val, size_t sx, size_t sy)
+      : t(t_val), u(u_val), size_x(sx), size_y(sy) {}
+  HubbardOperator op(const FermionBasis &b) const { return {b, {(int)size_x, (int)size_y}, t, u}; }
+  double t, u;
+  size_t size_x, size_y;
+};
+struct HubbardModel3D {
+  HubbardModel3D(double t_val, double u_val, size_t sx, size_t sy, size_t sz)
+      : t(t_val), u(u_val), size_x(sx), size_y(sy), size_z(sz) {}
+  HubbardOperator op(const FermionBasis &b) const {
+    return {b, {(int)size_x, (int)size_y, (int)size_z}, t, u};
+  }
+  double t, u;
+  size_t size_x, size_y, size_z;
+};
+
+struct LanczosDecomposition {
+  std::vector<double> alphas{};
+  std::vector<double> betas{};
+  size_t steps_taken = 0;
+  double b_norm = 0.0;
+};
+
+inline LanczosDecomposition lanczos_pass_one(const HubbardOperator &op, const DeviceVector &b,
+                                             size_t k, hipStream_t s = nullptr) {
+  if (op.dimension() != b.size())
+    throw std::runtime_error("Operator dimension does not match input vector");
+  std::vector<double> al(k ? k : 1), be(k ? k : 1);
+  int steps = 0;
+  double bn = 0.0;
+  check(mb_lanczos_pass_one(op.handle(), b.data(), (int)k, al.data(), be.data(), &steps, &bn, s));
+  LanczosDecomposition d;
+  d.alphas.assign(al.begin(), al.begin() + steps);
+  d.betas.assign(be.begin(), be.begin() + (steps > 0 ? steps - 1 : 0));
+  d.steps_taken = (size_t)steps;
+  d.b_norm = bn;
+  return d;
+}
+
+struct EigenPair {
+  double value;
+  DeviceVector vector;
+};
+
+// find_min_eigenpair(op, k) with an explicit start vector (the reference seeds
+// it with arma::randu internally).
+inline EigenPair find_min_eigenpair(const HubbardOperator &op, size_t k, const DeviceVector &b,
+                                    hipStream_t s = nullptr) {
+  EigenPair ep{0.0, DeviceVector(op.dimension())};
+  check(mb_find_min_eigenpair(op.handle(), b.data(), (int)k, &ep.value, ep.vector.data(), s));
+  return ep;
+}
+
+struct EvolutionOptions {
+  size_t krylov_steps = 30;
+};
+
+inline DeviceVector time_evolve_state(const HubbardOperator &H, const DeviceVector &psi, double t,
+                                      EvolutionOptions opts = {}, hipStream_t s = nullptr) {
+  DeviceVector out(H.dimension());
+  check(mb_time_evolve(H.handle(), psi.data(), t, (int)opts.krylov_steps, out.data(), s));
+  return out;
+}
+
+inline DeviceVector time_evolve_state_steps(const HubbardOperator &H, const DeviceVector &psi,
+                                            double t0, double t1, double dt,
+                                            EvolutionOptions opts = {}, hipStream_t s = nullptr) {
+  DeviceVector out(H.dimension());
+  check(mb_time_evolve_steps(H.handle(), psi.data(), t0, t1, dt, (int)opts.krylov_steps,
+                             out.data(), s));
+  return out;
+}
+
+namespace kpm {
+inline std::vector<double> jackson_kernel(size_t N) {
+  std::vector<double> g(N + 1);
+  check(mb_jackson_kernel((int)N, g.data()));
+  return g;
+}
+// Chebyshev DOS moments mu_0..mu_{M-1} of (H - b)/a over nvec random-phase vectors.
+inline std::vector<double> dos_moments(const HubbardOperator &H, double a, double b, size_t M,
+                                       int nvec, uint64_t seed, bool fp32 = true,
+                                       hipStream_t s = nullptr) {
+  std::vector<double> mu(M);
+  check(mb_kpm_dos_moments(H.handle(), a, b, (int)M, nvec, seed, fp32 ? 1 : 0, mu.data(), s));
+  return mu;
+}
+}  // namespace kpm
+
+namespace rescaling {
+struct Rescaling {
+  double a, b;
+  double rescale(double E) const { return (E - b) / a; }
+  double inverse(double E_scaled) const { return E_scaled * a + b; }
+};
+inline Rescaling from_bounds(double E_min, double E_max, double padding = 0.01) {
+  Rescaling r{};
+  check(mb_rescaling_from_bounds(E_min, E_max, padding, &r.a, &r.b));
+  return r;
+}
+}  // namespace rescaling
+
+}  // namespace mbamd

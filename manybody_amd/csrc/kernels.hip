@@ -171,6 +171,7 @@ __global__ __launch_bounds__(kBlock) void perm_kernel(PermTables p, const double
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < total; e += stride) {
     const int64_t u = e / p.d_dn, d = e - u * p.d_dn;
     const int64_t r = ref_rank(B, p.L, p.n_up, p.n_dn, p.s_up[u], p.s_dn[d]);
+    if (r < 0 || r >= total) continue;  // unreachable for a valid sector; never index out of range
     if (kGather)
       dst[e] = src[r];
     else

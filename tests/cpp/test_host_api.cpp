@@ -1,0 +1,105 @@
+// C++ host API test (include/manybody_amd.hpp), written the way the
+// reference's Catch2 tests read (tests/numerics/test_lanczos.cpp,
+// tests/algebra/test_basis.cpp).  Run on a GPU by tests/test_gpu_cpp_host.py;
+// prints one line per case and exits non-zero on the first failure.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+
+#include "manybody_amd.hpp"
+
+static int g_fail = 0;
+#define CHECK(cond)                                                         \
+  do {                                                                      \
+    if (!(cond)) {                                                          \
+      std::printf("  FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);       \
+      ++g_fail;                                                             \
+    }                                                                       \
+  } while (0)
+#define CASE(name) std::printf("case %s\n", name); std::fflush(stdout);
+
+int main() {
+  using namespace mbamd;
+  {
+    CASE("basis_order_small_example");
+    auto basis = FermionBasis::with_fixed_particle_number_and_spin(2, 2, 0);
+    CHECK(basis.size() == 4);
+    CHECK((basis.state(0) == std::pair<uint64_t, uint64_t>{1, 1}));
+    CHECK((basis.state(3) == std::pair<uint64_t, uint64_t>{2, 1}));
+    CHECK(basis.index_of(1, 2) == 1);
+    CHECK(!basis.contains(3, 1));
+  }
+  {
+    CASE("basis_errors_are_exceptions");
+    bool thrown = false;
+    try {
+      (void)FermionBasis::with_fixed_particle_number_and_spin(4, 3, 0);
+    } catch (const std::invalid_argument &) {
+      thrown = true;
+    }
+    CHECK(thrown);
+    auto basis = FermionBasis::with_fixed_particle_number_and_spin(4, 3, 1);
+    thrown = false;
+    try {
+      (void)basis.state(basis.size());
+    } catch (const std::out_of_range &) {
+      thrown = true;
+    }
+    CHECK(thrown);
+  }
+  {
+    CASE("ground_state_c1_u0_closed_form");
+    auto basis = FermionBasis::with_fixed_particle_number_and_spin(8, 8, 0);
+    auto H = HubbardModel(1.0, 0.0, 8).op(basis);
+    CHECK(H.dimension() == 4900);
+    auto b = H.gaussian(11);
+    auto ep = find_min_eigenpair(H, 200, b);
+    std::printf("  E0 = %.15f\n", ep.value);
+    CHECK(std::abs(ep.value + 9.656854249492380) < 1e-9);
+    // residual of the rebuilt eigenvector
+    auto Hx = H.apply(ep.vector);
+    auto x = ep.vector.to_host(), hx = Hx.to_host();
+    double r = 0, n = 0;
+    for (size_t i = 0; i < x.size(); ++i) {
+      r += std::norm(hx[i] - ep.value * x[i]);
+      n += std::norm(x[i]);
+    }
+    CHECK(std::sqrt(r / n) < 1e-6);
+  }
+  {
+    CASE("lanczos_decomposition_shape");
+    auto basis = FermionBasis::with_fixed_particle_number_and_spin(6, 5, 1);
+    auto H = HubbardModel2D(1.0, 4.0, 3, 2).op(basis);
+    auto b = H.gaussian(3);
+    auto d = lanczos_pass_one(H, b, 25);
+    CHECK(d.steps_taken == 25);
+    CHECK(d.alphas.size() == 25 && d.betas.size() == 24);
+    for (double beta : d.betas) CHECK(beta > 0.0);
+  }
+  {
+    CASE("time_evolution_conserves_norm");
+    auto basis = FermionBasis::with_fixed_particle_number_and_spin(8, 6, 0);
+    auto H = HubbardModel(1.0, 4.0, 8).op(basis);
+    auto psi = H.gaussian(5);
+    auto out = time_evolve_state_steps(H, psi, 0.0, 0.5, 0.1, EvolutionOptions{30});
+    auto a = psi.to_host(), c = out.to_host();
+    double na = 0, nc = 0;
+    for (size_t i = 0; i < a.size(); ++i) { na += std::norm(a[i]); nc += std::norm(c[i]); }
+    CHECK(std::abs(std::sqrt(nc) - std::sqrt(na)) / std::sqrt(na) < 1e-10);
+  }
+  {
+    CASE("kpm_moments_and_kernel");
+    auto g = kpm::jackson_kernel(50);
+    CHECK(std::abs(g[0] - 1.0) < 1e-10);
+    auto basis = FermionBasis::with_fixed_particle_number_and_spin(8, 6, 0);
+    auto H = HubbardModel(1.0, 4.0, 8).op(basis);
+    // Gershgorin: at most 12 hops of size t per state, diagonal in [0, 3U]
+    auto r = rescaling::from_bounds(-12.0, 24.0, 0.05);
+    auto mu = kpm::dos_moments(H, r.a, r.b, 64, 4, 1, true);
+    CHECK(std::abs(mu[0] - 1.0) < 1e-6);
+    for (double m : mu) CHECK(std::abs(m) <= 1.0 + 1e-6);
+  }
+  std::printf(g_fail ? "FAILED (%d)\n" : "ALL PASSED\n", g_fail);
+  return g_fail ? 1 : 0;
+}

@@ -1,0 +1,150 @@
+"""GPU parity of the Krylov layer (Lanczos, lowest eigenpair, Krylov exp(-iHt),
+KPM moments) against the oracle, plus closed forms at full 4 x 4 size.
+
+Tolerances (FP64): Lanczos alpha/beta 1e-12 relative over 40 steps (rounding
+grows along the recurrence), energies 1e-10 relative, evolved states 1e-10
+relative, FP64 KPM moments 1e-11 absolute.  FP32 KPM moments against the FP64
+run: 5e-4 absolute (stated, not hidden).
+"""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import manybody_amd as mb
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def make_op(sizes, nu, nd, t, U):
+    L = int(np.prod(sizes))
+    basis = mb.FermionBasis.sector(L, nu, nd)
+    if len(sizes) == 1:
+        model = mb.HubbardModel(t, U, sizes[0])
+    else:
+        model = mb.HubbardModel2D(t, U, *sizes)
+    return model.operator(basis)
+
+
+def test_lanczos_alpha_beta_match_oracle_c1(cuda):
+    sizes, nu, nd, t, U = [8], 4, 4, 1.0, 4.0
+    op = make_op(sizes, nu, nd, t, U)
+    P = po.ProductOperator(sizes, nu, nd, t, U)
+    b = op.gaussian(seed=2024)
+    dec = mb.lanczos_pass_one(op, b, 40)
+    al, be, bn = P.lanczos(b.cpu().numpy(), 40)
+    assert dec.steps_taken == len(al) == 40 and len(dec.betas) == 39
+    assert abs(dec.b_norm - bn) / bn < 1e-14
+    assert np.abs(np.array(dec.alphas) - al).max() / np.abs(al).max() < 1e-12
+    assert np.abs(np.array(dec.betas) - be).max() / np.abs(be).max() < 1e-12
+
+
+def test_ground_state_c1_matches_oracle_and_eigsh(cuda):
+    """1D L = 8, U = 4, 4 + 4 (D = 4,900), 200-step FP64 Lanczos."""
+    sizes, nu, nd, t, U = [8], 4, 4, 1.0, 4.0
+    op = make_op(sizes, nu, nd, t, U)
+    b = op.gaussian(seed=1)
+    ep = mb.find_min_eigenpair(op, 200, b)
+    A = po.build_csc(sizes, nu, nd, t, U)
+    e_ref = spla.eigsh(A.to_scipy().real, k=1, which="SA", tol=1e-14)[0][0]
+    assert abs(ep.value - e_ref) / abs(e_ref) < 1e-10
+    # oracle's own 200-step Lanczos on the same start vector (reference order)
+    b_ref = op.to_reference_order(b).cpu().numpy()
+    e_orc, x_orc = po.find_min_eigenpair_csc(A, b_ref, 200)
+    assert abs(ep.value - e_orc) / abs(e_orc) < 1e-10
+    # eigenvector from pass two: residual and overlap with the oracle's
+    x = op.to_reference_order(ep.vector).cpu().numpy()
+    r = np.linalg.norm(A.matvec(x) - ep.value * x) / np.linalg.norm(x)
+    assert r < 1e-6
+    ov = abs(np.vdot(x, x_orc)) / (np.linalg.norm(x) * np.linalg.norm(x_orc))
+    assert abs(ov - 1.0) < 1e-8
+
+
+def test_ground_state_u0_closed_form_c1(cuda):
+    op = make_op([8], 4, 4, 1.0, 0.0)
+    ep = mb.find_min_eigenpair(op, 200, op.gaussian(seed=3), want_vector=False)
+    assert abs(ep.value - (-9.656854249492380)) < 1e-10 * 9.66
+
+
+def test_ground_state_2d_4x4_u0_closed_form_full_size(cuda):
+    """4 x 4 periodic, U = 0, 7 + 7 (D = 130,873,600): E0 = -24 exactly."""
+    op = make_op([4, 4], 7, 7, 1.0, 0.0)
+    ep = mb.find_min_eigenpair(op, 150, op.gaussian(seed=4), want_vector=False)
+    assert abs(ep.value + 24.0) / 24.0 < 1e-10
+
+
+def test_lanczos_first_steps_c2_match_oracle(cuda):
+    """4 x 4, U = 8, 7 + 7: the first 4 alpha/beta against the CPU oracle."""
+    sizes, nu, nd, t, U = [4, 4], 7, 7, 1.0, 8.0
+    op = make_op(sizes, nu, nd, t, U)
+    b = op.gaussian(seed=5)
+    dec = mb.lanczos_pass_one(op, b, 4)
+    P = po.ProductOperator(sizes, nu, nd, t, U, threads=16)
+    al, be, bn = P.lanczos(b.cpu().numpy(), 4)
+    assert np.abs(np.array(dec.alphas) - al).max() / np.abs(al).max() < 1e-12
+    assert np.abs(np.array(dec.betas) - be).max() / np.abs(be).max() < 1e-12
+
+
+def test_lanczos_breakdown_and_zero_norm(cuda):
+    import torch
+    op = make_op([4], 0, 4, 1.0, 2.0)  # D = 1: breakdown after the first step
+    b = op.gaussian(seed=6)
+    dec = mb.lanczos_pass_one(op, b, 5)
+    assert dec.steps_taken == 1 and dec.betas == []
+    op2 = make_op([6], 3, 3, 1.0, 2.0)
+    with pytest.raises(RuntimeError, match="zero norm"):
+        mb.lanczos_pass_one(op2, torch.zeros(op2.dimension(), dtype=torch.complex128,
+                                             device=cuda), 5)
+    dec0 = mb.lanczos_pass_one(op2, op2.gaussian(1), 0)
+    assert dec0.steps_taken == 0 and dec0.b_norm > 0
+
+
+def test_time_evolve_matches_oracle(cuda):
+    sizes, nu, nd, t, U = [6], 3, 2, 1.0, 3.0
+    op = make_op(sizes, nu, nd, t, U)
+    A = po.build_csc(sizes, nu, nd, t, U)
+    psi = op.gaussian(seed=7)
+    out = op.to_reference_order(mb.time_evolve_state(op, psi, 0.45, mb.EvolutionOptions(30)))
+    ref = po.time_evolve_csc(A, op.to_reference_order(psi).cpu().numpy(), 0.45, 30)
+    out = out.cpu().numpy()
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-10
+
+
+def test_time_evolve_steps_conserves_norm_and_matches_single_steps(cuda):
+    import torch
+    op = make_op([10], 5, 4, 1.0, 4.0)
+    psi = op.gaussian(seed=8)
+    n0 = torch.linalg.vector_norm(psi).item()
+    stepped = mb.time_evolve_state_steps(op, psi, 0.0, 0.3, 0.1)
+    manual = psi
+    for _ in range(3):
+        manual = mb.time_evolve_state(op, manual, 0.1)
+    assert abs(torch.linalg.vector_norm(stepped).item() - n0) / n0 < 1e-10
+    assert torch.linalg.vector_norm(stepped - manual).item() / n0 < 1e-12
+
+
+def test_kpm_fp64_moments_match_oracle(cuda):
+    sizes, nu, nd, t, U = [8], 3, 3, 1.0, 4.0
+    op = make_op(sizes, nu, nd, t, U)
+    A = po.build_csc(sizes, nu, nd, t, U)
+    ev = np.linalg.eigvalsh(A.to_scipy().toarray().real)
+    a, b = (ev[-1] - ev[0]) / 2 * 1.02, (ev[-1] + ev[0]) / 2  # spectrum inside [-1, 1]
+    M, nvec, seed = 64, 4, 99
+    mu = np.array(mb.kpm_dos_moments(op, a, b, M, nvec, seed, fp32=False))
+    vecs = op.random_phase(seed, nvec, fp32=False).reshape(op.dimension(), nvec)
+    vecs_ref = np.stack([op.to_reference_order(vecs[:, r].contiguous()).cpu().numpy()
+                         for r in range(nvec)])
+    ref = po.kpm_moments_csc(A, a, b, M, vecs_ref)
+    assert np.abs(mu - ref).max() < 1e-11
+    assert abs(mu[0] - 1.0) < 1e-14
+
+
+def test_kpm_fp32_error_vs_fp64_is_small(cuda):
+    op = make_op([12], 6, 6, 1.0, 4.0)
+    # the spectrum of the L = 12 ring, 6 + 6, U = 4 is [-6.92, 30.92] (scipy eigsh)
+    a, b = 19.5, 12.0
+    mu64 = np.array(mb.kpm_dos_moments(op, a, b, 256, 4, 5, fp32=False))
+    mu32 = np.array(mb.kpm_dos_moments(op, a, b, 256, 4, 5, fp32=True))
+    err = np.abs(mu64 - mu32).max()
+    print(f"KPM FP32 vs FP64 max |d mu| (L=12, 6+6, M=256, R=4): {err:.3e}")
+    assert err < 5e-4
