@@ -81,7 +81,7 @@ def main():
         from oracle import pyoracle as po
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         P = po.ProductOperator([Lx, Ly], nu, nd, t, U, threads=threads)
-        xh = x.cpu().numpy()
+        xh = mb.from_planar(x)
         P.matvec(xh)  # warm-up (page faults)
         runs = []
         for _ in range(3):

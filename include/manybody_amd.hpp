@@ -45,7 +45,9 @@ inline void check(int status) {
   }
 }
 
-// Owning device buffer of n complex128 values.
+// Owning device buffer of n complex128 values.  Device-order vectors are
+// planar (n real parts, then n imaginary parts); reference-order vectors (the
+// input of to_device_order, the output of to_reference_order) are interleaved.
 class DeviceVector {
  public:
   DeviceVector() = default;
@@ -66,13 +68,32 @@ class DeviceVector {
   }
   void *data() const { return p_; }
   size_t size() const { return n_; }
+  // device order (planar) -> host complex values
   std::vector<std::complex<double>> to_host() const {
+    std::vector<double> raw(2 * n_);
+    copy_out(raw.data());
     std::vector<std::complex<double>> h(n_);
-    if (hipMemcpy(h.data(), p_, n_ * sizeof(h[0]), hipMemcpyDeviceToHost) != hipSuccess)
-      throw std::runtime_error("hipMemcpy failed");
+    for (size_t i = 0; i < n_; ++i) h[i] = {raw[i], raw[n_ + i]};
+    return h;
+  }
+  // reference order (interleaved) -> host complex values
+  std::vector<std::complex<double>> to_host_interleaved() const {
+    std::vector<std::complex<double>> h(n_);
+    copy_out(h.data());
     return h;
   }
   static DeviceVector from_host(const std::vector<std::complex<double>> &h) {
+    std::vector<double> raw(2 * h.size());
+    for (size_t i = 0; i < h.size(); ++i) {
+      raw[i] = h[i].real();
+      raw[h.size() + i] = h[i].imag();
+    }
+    DeviceVector d(h.size());
+    if (hipMemcpy(d.p_, raw.data(), raw.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      throw std::runtime_error("hipMemcpy failed");
+    return d;
+  }
+  static DeviceVector from_host_interleaved(const std::vector<std::complex<double>> &h) {
     DeviceVector d(h.size());
     if (hipMemcpy(d.p_, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice) != hipSuccess)
       throw std::runtime_error("hipMemcpy failed");
@@ -80,6 +101,10 @@ class DeviceVector {
   }
 
  private:
+  void copy_out(void *dst) const {
+    if (hipMemcpy(dst, p_, n_ * sizeof(std::complex<double>), hipMemcpyDeviceToHost) != hipSuccess)
+      throw std::runtime_error("hipMemcpy failed");
+  }
   void *p_ = nullptr;
   size_t n_ = 0;
 };

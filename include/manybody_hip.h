@@ -15,10 +15,12 @@
  *    the caller keeps the basis alive at least as long as the model (as the
  *    reference's operators hold `const FermionBasis&`).
  *  - Vectors are device pointers.  "ref" vectors are indexed by the reference
- *    basis index (the DFS position, basis.cpp:267-302); "dev" vectors use the
- *    device product order x[u * D_dn + d], u and d the combinatorial ranks of
- *    the up and down occupation bitstrings.  Complex vectors are interleaved
- *    (re, im): complex128 = 2 doubles, complex64 = 2 floats.
+ *    basis index (the DFS position, basis.cpp:267-302) and are interleaved
+ *    complex (re, im per element), like arma::cx_vec.  "dev" vectors use the
+ *    device order: planar, all D real parts then all D imaginary parts, each
+ *    plane in product order x[u * D_dn + d] (u, d = combinatorial ranks of the
+ *    up and down occupation bitstrings).  nvec complex vectors in device order
+ *    are 2 nvec consecutive planes (vector r = planes 2r, 2r+1).
  *  - `stream` is a hipStream_t (NULL = default stream).  Entry points that
  *    return host scalars synchronize that stream before returning.
  *  - Concurrent calls on one const model from different host threads and
@@ -115,8 +117,7 @@ int mb_hv_c128(const mb_hubbard *h, const void *x_dev, void *y_dev, mb_stream st
  * FP32 path). */
 int mb_hv_c64(const mb_hubbard *h, const void *x_dev, void *y_dev, mb_stream stream);
 
-/* y = H x for nvec complex64 vectors stored element-major, x[e * nvec + r]
- * (one sweep of the index tables serves all nvec vectors). nvec in {1,2,4,8,16}. */
+/* y = H x for nvec complex64 vectors in device order (2 nvec planes). */
 int mb_hv_c64_multi(const mb_hubbard *h, int nvec, const void *x_dev, void *y_dev,
                     mb_stream stream);
 
@@ -138,7 +139,7 @@ int mb_apply_ref_c128(const mb_hubbard *h, const void *x_ref, void *y_ref, void 
 int mb_fill_gaussian_c128(const mb_hubbard *h, uint64_t seed, void *x_dev, mb_stream stream);
 
 /* nvec random-phase complex64 vectors e^{i phi}, phi ~ U[0, 2 pi) from
- * (seed, vector, reference index), element-major x[e * nvec + r]. */
+ * (seed, vector, reference index), device order (2 nvec planes). */
 int mb_fill_phase_c64_multi(const mb_hubbard *h, uint64_t seed, int nvec, void *x_dev,
                             mb_stream stream);
 
@@ -183,7 +184,7 @@ int mb_tridiag_eig(int m, const double *alphas, const double *betas, double *w, 
  * mu_n = (1/(nvec D)) sum_r <r|T_n(Hs)|r>.  The Chebyshev recurrence of
  * SparseKPMProjector::apply (kpm.h:212-229); H.v in FP32 complex (fp32 = 1) or
  * FP64 complex (fp32 = 0); inner products and moments accumulate in FP64.
- * M even, nvec in {1,2,4,8,16}. */
+ * M even, 1 <= nvec <= 64. */
 int mb_kpm_dos_moments(const mb_hubbard *h, double a, double b, int M, int nvec, uint64_t seed,
                        int fp32, double *mu, mb_stream stream);
 

@@ -238,41 +238,51 @@ class HubbardOperator:
     def dimension(self) -> int:
         return int(lib().mb_hubbard_dimension(self._h))
 
-    # --- device order
+    # --- device order: planar real tensors of shape (2 * nvec, D)
     def empty(self, dtype=None, nvec: int = 1):
+        """Uninitialised device-order storage for nvec complex vectors: a real
+        tensor of shape (2 nvec, D), rows 2r and 2r+1 = re and im of vector r."""
         import torch
-        dtype = dtype or torch.complex128
-        n = self.dimension() * nvec
-        return torch.empty(n, dtype=dtype, device="cuda")
+        dtype = dtype or torch.float64
+        return torch.empty((2 * nvec, self.dimension()), dtype=dtype, device="cuda")
 
     def hv(self, x, y=None):
-        """y = H x in device order (complex128 or complex64)."""
+        """y = H x in device order: float64 (2, D) = one complex128 vector, float32
+        (2 nvec, D) = nvec complex64 vectors."""
         import torch
         y = torch.empty_like(x) if y is None else y
-        if x.dtype == torch.complex128:
+        D = self.dimension()
+        if x.numel() % (2 * D):
+            raise ValueError("x must hold 2 * nvec planes of the operator's dimension")
+        nvec = x.numel() // (2 * D)
+        if x.dtype == torch.float64:
+            if nvec != 1:
+                raise ValueError("float64 H.v takes one complex vector")
             _check(lib().mb_hv_c128(self._h, _ptr(x), _ptr(y), _stream()))
-        elif x.dtype == torch.complex64:
-            nvec = x.numel() // self.dimension()
+        elif x.dtype == torch.float32:
             if nvec == 1:
                 _check(lib().mb_hv_c64(self._h, _ptr(x), _ptr(y), _stream()))
             else:
                 _check(lib().mb_hv_c64_multi(self._h, nvec, _ptr(x), _ptr(y), _stream()))
         else:
-            raise ValueError("x must be complex128 or complex64")
+            raise ValueError("x must be float64 or float32 planes")
         return y
 
     def to_device_order(self, x_ref):
+        """complex128 vector in reference index order -> device order (2, D)."""
         out = self.empty()
         _check(lib().mb_to_device_order_c128(self._h, _ptr(x_ref), _ptr(out), _stream()))
         return out
 
     def to_reference_order(self, x_dev):
-        out = self.empty()
+        """device order (2, D) -> complex128 vector in reference index order."""
+        import torch
+        out = torch.empty(self.dimension(), dtype=torch.complex128, device="cuda")
         _check(lib().mb_to_reference_order_c128(self._h, _ptr(x_dev), _ptr(out), _stream()))
         return out
 
     def apply(self, x_ref):
-        """LinearOperator::apply in reference order: returns a new vector."""
+        """LinearOperator::apply in reference order: returns a new complex128 vector."""
         import torch
         y = torch.empty_like(x_ref)
         work = self.empty(nvec=2)
@@ -280,21 +290,41 @@ class HubbardOperator:
         return y
 
     def gaussian(self, seed: int):
-        """Device-order complex128 vector, re/im ~ N(0,1) per reference index."""
+        """Device-order complex vector (2, D), re/im ~ N(0,1) per reference index."""
         x = self.empty()
         _check(lib().mb_fill_gaussian_c128(self._h, seed, _ptr(x), _stream()))
         return x
 
     def random_phase(self, seed: int, nvec: int, fp32: bool = True):
-        """nvec random-phase vectors e^{i phi}, element-major x[e * nvec + r]."""
+        """nvec random-phase vectors e^{i phi} in device order (2 nvec, D)."""
         import torch
+        x = self.empty(torch.float32 if fp32 else torch.float64, nvec)
         if fp32:
-            x = self.empty(torch.complex64, nvec)
             _check(lib().mb_fill_phase_c64_multi(self._h, seed, nvec, _ptr(x), _stream()))
         else:
-            x = self.empty(torch.complex128, nvec)
             _check(lib().mb_fill_phase_c128_multi(self._h, seed, nvec, _ptr(x), _stream()))
         return x
+
+
+def to_planar(z, device="cuda", dtype=None):
+    """numpy complex array (D,) or (nvec, D) -> device-order tensor (2 nvec, D)."""
+    import numpy as np
+    import torch
+    z = np.atleast_2d(np.asarray(z))
+    planes = np.empty((2 * z.shape[0], z.shape[1]), dtype=np.float64)
+    planes[0::2] = z.real
+    planes[1::2] = z.imag
+    t = torch.from_numpy(planes)
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(device)
+
+
+def from_planar(t):
+    """device-order tensor (2 nvec, D) -> numpy complex128 (D,) or (nvec, D)."""
+    a = t.detach().cpu().numpy().astype("float64").reshape(-1, t.shape[-1])
+    z = a[0::2] + 1j * a[1::2]
+    return z[0] if z.shape[0] == 1 else z
 
 
 # ------------------------------------------------------------------ Krylov

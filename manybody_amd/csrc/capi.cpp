@@ -236,14 +236,16 @@ int mb_hubbard_shape(const mb_hubbard *h, int64_t *d_up, int64_t *d_dn) {
 int mb_hv_c128(const mb_hubbard *h, const void *x_dev, void *y_dev, mb_stream stream) {
   return guarded([&] {
     need(x_dev && y_dev && x_dev != y_dev, "x and y must be distinct device pointers");
-    mb::launch_hv_c128(mb::tables_of(H(h)), x_dev, y_dev, S(stream));
+    mb::launch_hv<double>(mb::tables_of(H(h)), 2, (const double *)x_dev, (double *)y_dev,
+                          mb::Epi::Plain, mb::EpiArgs{}, S(stream));
   });
 }
 
 int mb_hv_c64(const mb_hubbard *h, const void *x_dev, void *y_dev, mb_stream stream) {
   return guarded([&] {
     need(x_dev && y_dev && x_dev != y_dev, "x and y must be distinct device pointers");
-    mb::launch_hv_c64(mb::tables_of(H(h)), x_dev, y_dev, S(stream));
+    mb::launch_hv<float>(mb::tables_of(H(h)), 2, (const float *)x_dev, (float *)y_dev,
+                         mb::Epi::Plain, mb::EpiArgs{}, S(stream));
   });
 }
 
@@ -251,7 +253,9 @@ int mb_hv_c64_multi(const mb_hubbard *h, int nvec, const void *x_dev, void *y_de
                     mb_stream stream) {
   return guarded([&] {
     need(x_dev && y_dev && x_dev != y_dev, "x and y must be distinct device pointers");
-    mb::launch_hv_c64_multi(mb::tables_of(H(h)), nvec, x_dev, y_dev, S(stream));
+    need(nvec >= 1 && nvec <= 64, "nvec must be in [1, 64]");
+    mb::launch_hv<float>(mb::tables_of(H(h)), 2 * nvec, (const float *)x_dev, (float *)y_dev,
+                         mb::Epi::Plain, mb::EpiArgs{}, S(stream));
   });
 }
 
@@ -277,7 +281,8 @@ int mb_apply_ref_c128(const mb_hubbard *h, const void *x_ref, void *y_ref, void 
     const auto &hh = H(h);
     char *w0 = (char *)work, *w1 = w0 + (size_t)hh.dim() * 16;
     mb::launch_gather_c128(mb::perm_of(*hh.sec), x_ref, w0, S(stream));
-    mb::launch_hv_c128(mb::tables_of(hh), w0, w1, S(stream));
+    mb::launch_hv<double>(mb::tables_of(hh), 2, (const double *)w0, (double *)w1, mb::Epi::Plain,
+                          mb::EpiArgs{}, S(stream));
     mb::launch_scatter_c128(mb::perm_of(*hh.sec), w1, y_ref, S(stream));
   });
 }
@@ -294,7 +299,7 @@ int mb_fill_phase_c64_multi(const mb_hubbard *h, uint64_t seed, int nvec, void *
   return guarded([&] {
     need(x_dev, "null argument");
     need(nvec >= 1 && nvec <= 64, "nvec out of range");
-    mb::launch_fill_phase_c64_multi(mb::perm_of(*H(h).sec), seed, nvec, x_dev, S(stream));
+    mb::launch_fill_phase(mb::perm_of(*H(h).sec), seed, nvec, true, x_dev, S(stream));
   });
 }
 
@@ -303,7 +308,7 @@ int mb_fill_phase_c128_multi(const mb_hubbard *h, uint64_t seed, int nvec, void 
   return guarded([&] {
     need(x_dev, "null argument");
     need(nvec >= 1 && nvec <= 64, "nvec out of range");
-    mb::launch_fill_phase_c128_multi(mb::perm_of(*H(h).sec), seed, nvec, x_dev, S(stream));
+    mb::launch_fill_phase(mb::perm_of(*H(h).sec), seed, nvec, false, x_dev, S(stream));
   });
 }
 

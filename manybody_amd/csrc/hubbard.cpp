@@ -17,6 +17,7 @@
 // up-spin block in front of the down creators is passed twice by a down hop
 // and drops out, so each species' sign depends on its own string only.
 #include <algorithm>
+#include <cmath>
 
 #include "mb_internal.hpp"
 
@@ -82,6 +83,47 @@ void species_hops(const std::vector<uint32_t> &strings, const std::vector<Mono> 
   }
 }
 
+// Column-major packed copy of the hop table for the row kernel: entry k of
+// string d at ell[k * D + d] = target << 8 | (amplitude < 0) << 7 | magnitude
+// index.  Needs D < 2^24 and at most 128 distinct magnitudes.
+void pack_ell(SpeciesHops &h) {
+  const size_t n = h.row_ptr.size() - 1;
+  h.ell_ok = false;
+  h.mags.clear();
+  if (n >= (1u << 24) || h.max_row > 255) return;
+  std::vector<uint32_t> code(h.tgt.size());
+  for (size_t e = 0; e < h.tgt.size(); ++e) {
+    const double m = std::fabs(h.amp[e]);
+    auto it = std::find(h.mags.begin(), h.mags.end(), m);
+    int idx = (int)(it - h.mags.begin());
+    if (it == h.mags.end()) {
+      if (h.mags.size() >= 128) return;
+      h.mags.push_back(m);
+    }
+    code[e] = ((uint32_t)h.tgt[e] << 8) | (h.amp[e] < 0 ? 0x80u : 0u) | (uint32_t)idx;
+  }
+  h.ell_k = h.max_row;
+  h.ell.assign((size_t)std::max(h.ell_k, 1) * n, 0u);
+  h.cnt.assign(n, 0);
+  for (size_t d = 0; d < n; ++d) {
+    const int c = h.row_ptr[d + 1] - h.row_ptr[d];
+    h.cnt[d] = (uint8_t)c;
+    for (int k = 0; k < c; ++k) h.ell[(size_t)k * n + d] = code[h.row_ptr[d] + k];
+  }
+  if (h.mags.empty()) h.mags.push_back(0.0);
+  h.ell_ok = true;
+  // compact copy: one magnitude, targets below 2^15, at most 32 entries per string
+  h.comp_ok = h.mags.size() == 1 && n < (1u << 15) && h.max_row <= 32;
+  h.comp.clear();
+  if (h.comp_ok) {
+    h.comp.assign(n * 32, 0);
+    for (size_t d = 0; d < n; ++d)
+      for (int k = h.row_ptr[d]; k < h.row_ptr[d + 1]; ++k)
+        h.comp[d * 32 + (k - h.row_ptr[d])] =
+            (uint16_t)(((uint32_t)h.tgt[k] << 1) | (h.amp[k] < 0 ? 1u : 0u));
+  }
+}
+
 void upload_species(SpeciesHops &h) {
   h.d_row_ptr = upload(h.row_ptr);
   h.d_tgt = upload(h.tgt);
@@ -90,6 +132,14 @@ void upload_species(SpeciesHops &h) {
   std::vector<float> af(h.amp.begin(), h.amp.end()), of(h.onsite.begin(), h.onsite.end());
   h.d_ampf = upload(af);
   h.d_onsitef = upload(of);
+  if (h.ell_ok) {
+    h.d_ell = upload(h.ell);
+    h.d_cnt = upload(h.cnt);
+    h.d_mags = upload(h.mags);
+    std::vector<float> mf(h.mags.begin(), h.mags.end());
+    h.d_magsf = upload(mf);
+  }
+  if (h.comp_ok) h.d_comp = upload(h.comp);
 }
 
 }  // namespace
@@ -99,6 +149,7 @@ void build_hubbard(Hubbard &h) {
   const auto mono = merge_monomials(h.bond_from, h.bond_to, h.t);
   species_hops(s.s_up, mono, h.up);
   species_hops(s.s_dn, mono, h.dn);
+  pack_ell(h.dn);
   h.has_onsite = false;
   for (double v : h.up.onsite) h.has_onsite |= (v != 0.0);
   for (double v : h.dn.onsite) h.has_onsite |= (v != 0.0);
@@ -141,6 +192,15 @@ HvTables tables_of(const Hubbard &h) {
   t.dn_onf = h.dn.d_onsitef.as<float>();
   t.diagUf = h.d_diagUf.as<float>();
   t.has_onsite = h.has_onsite ? 1 : 0;
+  t.ell_ok = h.dn.ell_ok ? 1 : 0;
+  t.ell_k = h.dn.ell_k;
+  t.n_mags = (int)h.dn.mags.size();
+  t.dn_ell = h.dn.d_ell.as<uint32_t>();
+  t.dn_cnt = h.dn.d_cnt.as<uint8_t>();
+  t.mags = h.dn.d_mags.as<double>();
+  t.magsf = h.dn.d_magsf.as<float>();
+  t.comp_ok = h.dn.comp_ok ? 1 : 0;
+  t.dn_comp = h.dn.d_comp.as<uint16_t>();
   t.max_up_row = h.up.max_row;
   return t;
 }

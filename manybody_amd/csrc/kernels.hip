@@ -1,13 +1,34 @@
 // HIP kernels for gfx950 (CDNA4, wave64): matrix-free Hubbard H.v, the
-// reference-order <-> product-order permutation, fused Lanczos vector steps,
-// Chebyshev (KPM) steps and seeded vector fills.
+// reference-order <-> device-order permutation, fused Lanczos and Chebyshev
+// epilogues and seeded vector fills.
 //
-// Device layout: a state vector is the D_up x D_dn matrix x[u * D_dn + d]
-// (u, d = combinatorial ranks of the up / down bitstrings).  Up hops move whole
-// rows with one sign per row pair; down hops gather inside a row.  Reductions
-// are two-level and atomic-free (fixed grid, fixed tree), so results do not
-// depend on scheduling.
+// H = H_up (x) 1 + 1 (x) H_dn + diagonal acts on the D_up x D_dn matrix of each
+// real plane.  Up hops combine whole rows with one signed amplitude per row
+// pair (row u of H x gathers rows u' of x); down hops gather inside a row.  The
+// two need different locality, so H.v runs as two kernels:
+//
+//  1. up_strip_kernel: the plane is cut into column strips of 256 bytes per row
+//     (32 doubles / 64 floats).  One strip of all D_up rows (2.9 MB for the
+//     4 x 4, 7 + 7 sector) stays resident in the L2 of one XCD while all rows
+//     of that strip are produced, so every x element is fetched from HBM once
+//     and its ~17 reuses hit L2.  Workgroup -> (strip, row chunk) is XCD-aware:
+//     workgroups are dispatched round-robin over the 8 XCDs, so the chunks of a
+//     strip are given ids congruent mod 8.  Hop tables are read with scalar
+//     loads (row index made wave-uniform).  Writes y = (H_up x).
+//  2. row_kernel: one workgroup per plane row stages x[u, :] in LDS (91.5 KB
+//     for D_dn = 11,440 doubles), applies the diagonal and the down hops from
+//     LDS (packed column-major hop table, coalesced), adds the strip result and
+//     applies the epilogue (plain, Lanczos, reconstruction, Chebyshev) with a
+//     per-workgroup partial sum for the fused inner products.
+//
+// Reductions are two-level and atomic-free (fixed partial slots, fixed tree),
+// so results do not depend on scheduling.  A one-kernel generic path (thread
+// per element, both tables from global memory) covers sectors whose rows do not
+// fit in LDS or whose tables cannot be packed.
 #include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
 
 #include "krylov.hpp"
 #include "mb_internal.hpp"
@@ -16,29 +37,28 @@ namespace mb {
 
 namespace {
 
-constexpr int kBlock = 256;
 constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kMaxRowLds = 160 * 1024 - 1024;  // bytes of LDS a staged row may use
+constexpr int kGenericGrid = 2048;              // blocks per plane of the generic path
+constexpr int kCompactK = 32;                   // entries per string of the compact down table
 
 __device__ __forceinline__ int popc(uint32_t v) { return __builtin_popcount(v); }
 
-// ------------------------------------------------------------ reductions
-
-// Fixed-order block reduction of one double (kBlock threads).
-__device__ double block_sum(double v, double *lds) {
+// Fixed-order sum over the workgroup (blockDim multiple of 64); valid in thread 0.
+__device__ double wg_sum(double v, double *lds) {
   const int tid = threadIdx.x;
   for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
   if ((tid & (kWave - 1)) == 0) lds[tid / kWave] = v;
   __syncthreads();
   double r = 0.0;
   if (tid == 0)
-    for (int w = 0; w < kBlock / kWave; ++w) r += lds[w];
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) r += lds[w];
   __syncthreads();
-  return r;  // valid in thread 0
+  return r;
 }
 
-// ------------------------------------------------------------ H.v core
-
-template <typename R>
+template <typename T>
 struct Tabs;
 template <>
 struct Tabs<double> {
@@ -47,6 +67,7 @@ struct Tabs<double> {
   __device__ static const double *up_on(const HvTables &t) { return t.up_on; }
   __device__ static const double *dn_on(const HvTables &t) { return t.dn_on; }
   __device__ static const double *diag(const HvTables &t) { return t.diagU; }
+  __device__ static const double *mags(const HvTables &t) { return t.mags; }
 };
 template <>
 struct Tabs<float> {
@@ -55,55 +76,283 @@ struct Tabs<float> {
   __device__ static const float *up_on(const HvTables &t) { return t.up_onf; }
   __device__ static const float *dn_on(const HvTables &t) { return t.dn_onf; }
   __device__ static const float *diag(const HvTables &t) { return t.diagUf; }
+  __device__ static const float *mags(const HvTables &t) { return t.magsf; }
 };
 
-template <typename R>
-struct C2 {
-  R re, im;
+// ------------------------------------------------------------ epilogue
+
+struct Partial2 {
+  double s0 = 0.0, s1 = 0.0;
 };
 
-// (H x)[e] for element e = u * D_dn + d of complex vector r (element-major
-// multi-vector layout x[e * NV + r]).  Summation order: diagonal, up hops in
-// table order, down hops in table order.
-template <typename R, int NV>
-__device__ __forceinline__ C2<R> hv_elem(const HvTables &t, const C2<R> *__restrict__ x,
-                                         int64_t u, int64_t d, int r) {
-  const int64_t Ddn = t.d_dn;
-  const int64_t e = u * Ddn + d;
-  R dg = Tabs<R>::diag(t)[popc(t.s_up[u] & t.s_dn[d])];
-  if (t.has_onsite) dg = dg + Tabs<R>::up_on(t)[u] + Tabs<R>::dn_on(t)[d];
-  const C2<R> xe = x[e * NV + r];
-  R ar = dg * xe.re, ai = dg * xe.im;
-  const int32_t ub = t.up_ptr[u], ue = t.up_ptr[u + 1];
-  const R *uamp = Tabs<R>::up_amp(t);
-  for (int32_t p = ub; p < ue; ++p) {
-    const C2<R> v = x[((int64_t)t.up_tgt[p] * Ddn + d) * NV + r];
-    const R a = uamp[p];
-    ar += a * v.re;
-    ai += a * v.im;
+// Applies the epilogue to r = (H x)[i] at plane offset i; xv = x[i].
+template <typename T, Epi E>
+__device__ __forceinline__ void epilogue(T r, T xv, int64_t i, T *__restrict__ y,
+                                         const EpiArgs &a, T bp, Partial2 &acc) {
+  if constexpr (E == Epi::Plain) {
+    y[i] = r;
+  } else if constexpr (E == Epi::Lanczos) {
+    if (bp != T(0)) r -= bp * static_cast<const T *>(a.prev)[i];
+    y[i] = r;
+    acc.s0 += (double)xv * (double)r;
+  } else if constexpr (E == Epi::Recon) {
+    if (bp != T(0)) r -= bp * static_cast<const T *>(a.prev)[i];
+    r -= (T)a.alpha * xv;
+    y[i] = r;
+  } else {  // Cheb
+    T o = (T)a.scale2 * (r - (T)a.shift * xv);
+    if (!a.first) o -= static_cast<const T *>(a.prev)[i];
+    y[i] = o;
+    acc.s0 += (double)o * (double)xv;
+    acc.s1 += (double)o * (double)o;
   }
-  const int32_t db = t.dn_ptr[d], de = t.dn_ptr[d + 1];
-  const R *damp = Tabs<R>::dn_amp(t);
-  for (int32_t p = db; p < de; ++p) {
-    const C2<R> v = x[(u * Ddn + t.dn_tgt[p]) * NV + r];
-    const R a = damp[p];
-    ar += a * v.re;
-    ai += a * v.im;
-  }
-  return {ar, ai};
 }
 
-template <typename R, int NV>
-__global__ __launch_bounds__(kBlock) void hv_kernel(HvTables t, const C2<R> *__restrict__ x,
-                                                    C2<R> *__restrict__ y) {
-  const int64_t total = t.d_up * t.d_dn * NV;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < total; g += stride) {
-    const int64_t e = g / NV;
-    const int r = (int)(g - e * NV);
-    const int64_t u = e / t.d_dn;
-    const int64_t d = e - u * t.d_dn;
-    y[g] = hv_elem<R, NV>(t, x, u, d, r);
+template <typename T, Epi E>
+__device__ __forceinline__ T beta_prev(const EpiArgs &a) {
+  if constexpr (E == Epi::Lanczos) return a.step > 0 ? (T)a.beta_dev[a.step - 1] : T(0);
+  if constexpr (E == Epi::Recon) return (T)a.beta;
+  return T(0);
+}
+
+// ------------------------------------------------------------ kernel 1: up hops
+
+// VEC consecutive values of one row, loaded/stored as one 16-byte access when VEC > 1.
+template <typename T, int VEC>
+struct Vec {
+  T v[VEC];
+  __device__ __forceinline__ void load(const T *p) {
+    if constexpr (VEC * sizeof(T) == 16) {
+      const uint4 q = *reinterpret_cast<const uint4 *>(p);
+      __builtin_memcpy(v, &q, 16);
+    } else {
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) v[j] = p[j];
+    }
+  }
+};
+
+// 16 lanes per row, 4 rows per wave, 32 rows per workgroup.  The workgroup's hop
+// lists are staged in LDS; y is written with non-temporal stores so that the
+// output stream does not evict the strip of x from L2.
+template <typename T, int VEC, int MAXH>
+__global__ __launch_bounds__(kBlock) void up_strip_kernel(HvTables t, int P, const T *__restrict__ x,
+                                                          T *__restrict__ y, int nJ, int nC,
+                                                          const int *flags) {
+  constexpr int LPR = 16, RPW = kWave / LPR, RC = 32, W = LPR * VEC;
+  __shared__ int s_tgt[RC][MAXH];
+  __shared__ T s_amp[RC][MAXH];
+  __shared__ int s_n[RC];
+  if (flags && flags[0]) return;
+  const int S = P * nJ;
+  const int i = blockIdx.x;
+  const int xcd = i & 7, q = i >> 3;
+  const int s = xcd + 8 * (q / nC);
+  const int c = q % nC;
+  if (s >= S) return;
+  const int p = s / nJ, J = s - p * nJ;
+  const int u_begin = c * RC;
+  const int u_end = (int)min((int64_t)u_begin + RC, t.d_up);
+  const T *__restrict__ amp = Tabs<T>::up_amp(t);
+  for (int e = threadIdx.x; e < RC * MAXH; e += kBlock) {
+    const int r = e / MAXH, k = e % MAXH;
+    const int u = u_begin + r;
+    if (u < u_end) {
+      const int b = t.up_ptr[u], n = t.up_ptr[u + 1] - b;
+      if (k == 0) s_n[r] = n;
+      if (k < n) {
+        s_tgt[r][k] = t.up_tgt[b + k];
+        s_amp[r][k] = amp[b + k];
+      }
+    } else if (k == 0) {
+      s_n[r] = 0;
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  const int64_t Ddn = t.d_dn;
+  const int64_t col = (int64_t)J * W + VEC * (lane % LPR);
+  const bool col_ok = col < Ddn;  // D_dn is a multiple of VEC
+  const int64_t plane = (int64_t)p * t.d_up * Ddn;
+  const T *__restrict__ xp = x + plane;
+  T *__restrict__ yp = y + plane;
+  const int sub = lane / LPR;
+  for (int rb = wv * RPW; rb < RC; rb += (kBlock / kWave) * RPW) {
+    const int r = rb + sub;
+    const int u = u_begin + r;
+    const bool ok = col_ok && u < u_end;
+    const int n = ok ? s_n[r] : 0;
+    T acc[VEC];
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) acc[j] = T(0);
+#pragma unroll 4
+    for (int k = 0; k < n; ++k) {
+      const T a = s_amp[r][k];
+      Vec<T, VEC> v;
+      v.load(xp + (int64_t)s_tgt[r][k] * Ddn + col);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) acc[j] += a * v.v[j];
+    }
+    if (ok) {
+      T *dst = yp + (int64_t)u * Ddn + col;
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) __builtin_nontemporal_store(acc[j], dst + j);
+    }
+  }
+}
+
+// ------------------------------------------------------------ kernel 2: row
+
+// Shared prologue of the row kernels: stage x[plane row] in LDS.
+template <typename T>
+__device__ __forceinline__ void stage_row(T *row, const T *__restrict__ x, int64_t base, int64_t Ddn) {
+  for (int64_t d = threadIdx.x; d < Ddn; d += blockDim.x) row[d] = x[base + d];
+  __syncthreads();
+}
+
+template <typename T, Epi E>
+__device__ __forceinline__ void row_reduce(const Partial2 &acc2, const EpiArgs &a, int64_t wg,
+                                           double (*red)[1024 / kWave]) {
+  if constexpr (E == Epi::Lanczos || E == Epi::Cheb) {
+    const double s0 = wg_sum(acc2.s0, red[0]);
+    double s1 = 0.0;
+    if constexpr (E == Epi::Cheb) s1 = wg_sum(acc2.s1, red[1]);
+    if (threadIdx.x == 0) {
+      if constexpr (E == Epi::Lanczos) {
+        a.partial[wg] = s0;
+      } else {
+        a.partial[2 * wg] = s0;
+        a.partial[2 * wg + 1] = s1;
+      }
+    }
+  }
+}
+
+// Compact down table: 32 uint16 entries per string, target << 1 | sign, one
+// amplitude magnitude; the entries of a string are fetched as four 16-byte loads.
+template <typename T, Epi E>
+__global__ __launch_bounds__(1024) void row_kernel_compact(HvTables t, const T *__restrict__ x,
+                                                           T *__restrict__ y, EpiArgs a) {
+  constexpr int K = kCompactK;
+  extern __shared__ double smem[];
+  __shared__ double red[2][1024 / kWave];
+  T *row = reinterpret_cast<T *>(smem);
+  if constexpr (E == Epi::Lanczos) {
+    if (a.flags[0]) return;
+  }
+  const int64_t wg = blockIdx.x;  // = p * D_up + u
+  const int64_t Ddn = t.d_dn, Dup = t.d_up;
+  const int64_t u = wg % Dup;
+  const int64_t base = wg * Ddn;
+  stage_row(row, x, base, Ddn);
+  const uint32_t su = t.s_up[u];
+  const T *__restrict__ diag = Tabs<T>::diag(t);
+  const T h = Tabs<T>::mags(t)[0];
+  const T bp = beta_prev<T, E>(a);
+  const T on_u = t.has_onsite ? Tabs<T>::up_on(t)[u] : T(0);
+  Partial2 acc2;
+  for (int64_t d = threadIdx.x; d < Ddn; d += blockDim.x) {
+    const uint4 *tp = reinterpret_cast<const uint4 *>(t.dn_comp + d * K);
+    uint4 q[K / 8];
+#pragma unroll
+    for (int j = 0; j < K / 8; ++j) q[j] = tp[j];
+    const int cnt = t.dn_cnt[d];
+    const T xv = row[d];
+    T dg = diag[popc(su & t.s_dn[d])];
+    if (t.has_onsite) dg = dg + on_u + Tabs<T>::dn_on(t)[d];
+    T acc = dg * xv;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const uint32_t w = reinterpret_cast<const uint32_t *>(&q[k / 8])[(k % 8) / 2];
+      const uint32_t e = (k & 1) ? (w >> 16) : (w & 0xFFFFu);
+      if (k < cnt) acc += ((e & 1u) ? -h : h) * row[e >> 1];
+    }
+    const T r = y[base + d] + acc;  // (H_up x) from kernel 1, then diagonal and down hops
+    epilogue<T, E>(r, xv, base + d, y, a, bp, acc2);
+  }
+  row_reduce<T, E>(acc2, a, wg, red);
+}
+
+// General down table (any number of amplitude magnitudes): column-major uint32
+// entries target << 8 | sign << 7 | magnitude index.
+template <typename T, Epi E>
+__global__ __launch_bounds__(1024) void row_kernel_ell(HvTables t, const T *__restrict__ x,
+                                                       T *__restrict__ y, EpiArgs a) {
+  extern __shared__ double smem[];
+  __shared__ double red[2][1024 / kWave];
+  T *row = reinterpret_cast<T *>(smem);
+  if constexpr (E == Epi::Lanczos) {
+    if (a.flags[0]) return;
+  }
+  const int64_t wg = blockIdx.x;
+  const int64_t Ddn = t.d_dn, Dup = t.d_up;
+  const int64_t u = wg % Dup;
+  const int64_t base = wg * Ddn;
+  stage_row(row, x, base, Ddn);
+  const uint32_t su = t.s_up[u];
+  const T *__restrict__ diag = Tabs<T>::diag(t);
+  const T *__restrict__ mags = Tabs<T>::mags(t);
+  const T bp = beta_prev<T, E>(a);
+  const T on_u = t.has_onsite ? Tabs<T>::up_on(t)[u] : T(0);
+  Partial2 acc2;
+  for (int64_t d = threadIdx.x; d < Ddn; d += blockDim.x) {
+    const T xv = row[d];
+    T dg = diag[popc(su & t.s_dn[d])];
+    if (t.has_onsite) dg = dg + on_u + Tabs<T>::dn_on(t)[d];
+    T acc = dg * xv;
+    const int cnt = t.dn_cnt[d];
+    for (int k = 0; k < cnt; ++k) {
+      const uint32_t e = t.dn_ell[(int64_t)k * Ddn + d];
+      const T m = mags[e & 127u];
+      acc += ((e & 128u) ? -m : m) * row[e >> 8];
+    }
+    const T r = y[base + d] + acc;
+    epilogue<T, E>(r, xv, base + d, y, a, bp, acc2);
+  }
+  row_reduce<T, E>(acc2, a, wg, red);
+}
+
+// ------------------------------------------------------------ generic path
+
+// Thread per element of plane blockIdx.y; both tables read from global memory.
+template <typename T, Epi E>
+__global__ __launch_bounds__(kBlock) void generic_kernel(HvTables t, const T *__restrict__ x,
+                                                         T *__restrict__ y, EpiArgs a) {
+  __shared__ double red[2][kBlock / kWave];
+  if constexpr (E == Epi::Lanczos) {
+    if (a.flags[0]) return;
+  }
+  const int64_t Ddn = t.d_dn, D = t.d_up * Ddn;
+  const int64_t p = blockIdx.y;
+  const T *__restrict__ xp = x + p * D;
+  const T bp = beta_prev<T, E>(a);
+  Partial2 acc2;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
+    const int64_t u = e / Ddn, d = e - u * Ddn;
+    T dg = Tabs<T>::diag(t)[popc(t.s_up[u] & t.s_dn[d])];
+    if (t.has_onsite) dg = dg + Tabs<T>::up_on(t)[u] + Tabs<T>::dn_on(t)[d];
+    const T xv = xp[e];
+    T up = T(0);
+    for (int32_t k = t.up_ptr[u]; k < t.up_ptr[u + 1]; ++k)
+      up += Tabs<T>::up_amp(t)[k] * xp[(int64_t)t.up_tgt[k] * Ddn + d];
+    T acc = dg * xv;
+    for (int32_t k = t.dn_ptr[d]; k < t.dn_ptr[d + 1]; ++k)
+      acc += Tabs<T>::dn_amp(t)[k] * xp[u * Ddn + t.dn_tgt[k]];
+    epilogue<T, E>(up + acc, xv, p * D + e, y, a, bp, acc2);
+  }
+  if constexpr (E == Epi::Lanczos || E == Epi::Cheb) {
+    const double s0 = wg_sum(acc2.s0, red[0]);
+    double s1 = 0.0;
+    if constexpr (E == Epi::Cheb) s1 = wg_sum(acc2.s1, red[1]);
+    if (threadIdx.x == 0) {
+      const int64_t slot = p * gridDim.x + blockIdx.x;
+      if constexpr (E == Epi::Lanczos) {
+        a.partial[slot] = s0;
+      } else {
+        a.partial[2 * slot] = s0;
+        a.partial[2 * slot + 1] = s1;
+      }
+    }
   }
 }
 
@@ -113,8 +362,6 @@ int grid_for(int64_t work, int cap) {
   if (g > cap) g = cap;
   return (int)g;
 }
-
-constexpr int kGridCap = 1 << 20;
 
 // ------------------------------------------------------------ permutation
 
@@ -161,21 +408,24 @@ __device__ int64_t ref_rank(const BinomLds &B, int L, int n_up, int n_dn, uint32
   return rank;
 }
 
+// Reference order is interleaved complex128; device order is planar (re, im).
 template <bool kGather>
-__global__ __launch_bounds__(kBlock) void perm_kernel(PermTables p, const double2 *__restrict__ src,
-                                                      double2 *__restrict__ dst) {
+__global__ __launch_bounds__(kBlock) void perm_kernel(PermTables p, const double *__restrict__ src,
+                                                      double *__restrict__ dst) {
   __shared__ BinomLds B;
   load_binom(B, p.binom);
-  const int64_t total = p.d_up * p.d_dn;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < total; e += stride) {
+  const int64_t D = p.d_up * p.d_dn;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
     const int64_t u = e / p.d_dn, d = e - u * p.d_dn;
     const int64_t r = ref_rank(B, p.L, p.n_up, p.n_dn, p.s_up[u], p.s_dn[d]);
-    if (r < 0 || r >= total) continue;  // unreachable for a valid sector; never index out of range
-    if (kGather)
-      dst[e] = src[r];
-    else
-      dst[r] = src[e];
+    if (r < 0 || r >= D) continue;  // unreachable for a valid sector; never index out of range
+    if (kGather) {
+      dst[e] = src[2 * r];
+      dst[D + e] = src[2 * r + 1];
+    } else {
+      dst[2 * r] = src[e];
+      dst[2 * r + 1] = src[D + e];
+    }
   }
 }
 
@@ -194,64 +444,63 @@ __device__ __forceinline__ double unit_half(uint64_t z) {  // [0, 1)
 }
 
 __global__ __launch_bounds__(kBlock) void fill_gaussian_kernel(PermTables p, uint64_t seed,
-                                                               double2 *__restrict__ x) {
+                                                               double *__restrict__ x) {
   __shared__ BinomLds B;
   load_binom(B, p.binom);
-  const int64_t total = p.d_up * p.d_dn;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < total; e += stride) {
+  const int64_t D = p.d_up * p.d_dn;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
     const int64_t u = e / p.d_dn, d = e - u * p.d_dn;
     const uint64_t i = (uint64_t)ref_rank(B, p.L, p.n_up, p.n_dn, p.s_up[u], p.s_dn[d]);
     const double u1 = unit_open(mix64(seed, 2 * i));
     const double u2 = unit_half(mix64(seed, 2 * i + 1));
     const double rad = sqrt(-2.0 * log(u1));
     const double th = 6.283185307179586 * u2;
-    x[e] = make_double2(rad * cos(th), rad * sin(th));
+    x[e] = rad * cos(th);
+    x[D + e] = rad * sin(th);
   }
 }
 
-template <typename R>
+template <typename T>
 __global__ __launch_bounds__(kBlock) void fill_phase_kernel(PermTables p, uint64_t seed, int nvec,
-                                                            C2<R> *__restrict__ x) {
+                                                            T *__restrict__ x) {
   __shared__ BinomLds B;
   load_binom(B, p.binom);
-  const int64_t total = p.d_up * p.d_dn;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < total; e += stride) {
+  const int64_t D = p.d_up * p.d_dn;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
     const int64_t u = e / p.d_dn, d = e - u * p.d_dn;
     const uint64_t i = (uint64_t)ref_rank(B, p.L, p.n_up, p.n_dn, p.s_up[u], p.s_dn[d]);
     for (int r = 0; r < nvec; ++r) {
       const double ph = 6.283185307179586 * unit_half(mix64(seed, i * (uint64_t)nvec + r));
-      x[e * nvec + r] = {(R)cos(ph), (R)sin(ph)};
+      x[(int64_t)(2 * r) * D + e] = (T)cos(ph);
+      x[(int64_t)(2 * r + 1) * D + e] = (T)sin(ph);
     }
   }
 }
 
-// ------------------------------------------------------------ Lanczos
+// ------------------------------------------------------------ vector steps
 
-constexpr int kLanczosGrid = 4096;
-
-__global__ __launch_bounds__(kBlock) void norm_partial_kernel(const double *__restrict__ x,
-                                                              int64_t n, double *partial) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void sumsq_partial_kernel(const T *__restrict__ x, int64_t n,
+                                                               double *partial) {
   __shared__ double lds[kBlock / kWave];
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    s += x[i] * x[i];
-  s = block_sum(s, lds);
+    s += (double)x[i] * (double)x[i];
+  s = wg_sum(s, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__device__ double final_sum(const double *partial, int n, double *lds) {
+__device__ double final_sum(const double *partial, int64_t n, double *lds) {
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += kBlock) s += partial[i];
-  return block_sum(s, lds);
+  for (int64_t i = threadIdx.x; i < n; i += kBlock) s += partial[i];
+  return wg_sum(s, lds);
 }
 
-__global__ __launch_bounds__(kBlock) void sqrt_final_kernel(const double *partial, int n,
-                                                            double *out) {
+__global__ __launch_bounds__(kBlock) void final_kernel(const double *partial, int64_t n, double *out,
+                                                       int take_sqrt) {
   __shared__ double lds[kBlock / kWave];
   const double s = final_sum(partial, n, lds);
-  if (threadIdx.x == 0) out[0] = sqrt(s);
+  if (threadIdx.x == 0) out[0] = take_sqrt ? sqrt(s) : s;
 }
 
 __global__ __launch_bounds__(kBlock) void scale_div_kernel(const double *__restrict__ x,
@@ -262,39 +511,11 @@ __global__ __launch_bounds__(kBlock) void scale_div_kernel(const double *__restr
     y[i] = x[i] / dv;
 }
 
-// w = H v - beta_{step-1} v_prev ; partial[b] = sum Re(conj(v) w)
-__global__ __launch_bounds__(kBlock) void lanczos_hv_kernel(HvTables t,
-                                                            const C2<double> *__restrict__ v,
-                                                            const C2<double> *__restrict__ vp,
-                                                            C2<double> *__restrict__ w,
-                                                            LanczosState st, int step) {
-  __shared__ double lds[kBlock / kWave];
-  if (st.flags[0]) return;
-  const double bp = step > 0 ? st.beta[step - 1] : 0.0;
-  const int64_t total = t.d_up * t.d_dn;
-  double s = 0.0;
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * kBlock) {
-    const int64_t u = e / t.d_dn, d = e - u * t.d_dn;
-    C2<double> h = hv_elem<double, 1>(t, v, u, d, 0);
-    if (bp != 0.0) {
-      const C2<double> q = vp[e];
-      h.re -= bp * q.re;
-      h.im -= bp * q.im;
-    }
-    w[e] = h;
-    const C2<double> ve = v[e];
-    s += ve.re * h.re + ve.im * h.im;
-  }
-  s = block_sum(s, lds);
-  if (threadIdx.x == 0) st.partial[blockIdx.x] = s;
-}
-
 __global__ __launch_bounds__(kBlock) void lanczos_finish_alpha_kernel(LanczosState st, int step,
-                                                                      int nblocks) {
+                                                                      int64_t nparts) {
   __shared__ double lds[kBlock / kWave];
   if (st.flags[0]) return;
-  const double s = final_sum(st.partial, nblocks, lds);
+  const double s = final_sum(st.partial, nparts, lds);
   if (threadIdx.x == 0) {
     st.alpha[step] = s;
     st.flags[1] = step + 1;
@@ -314,15 +535,14 @@ __global__ __launch_bounds__(kBlock) void lanczos_axpy_norm_kernel(const double 
     w[i] = q;
     s += q * q;
   }
-  s = block_sum(s, lds);
+  s = wg_sum(s, lds);
   if (threadIdx.x == 0) st.partial[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kBlock) void lanczos_finish_beta_kernel(LanczosState st, int step,
-                                                                     int nblocks) {
+__global__ __launch_bounds__(kBlock) void lanczos_finish_beta_kernel(LanczosState st, int step) {
   __shared__ double lds[kBlock / kWave];
   if (st.flags[0]) return;
-  const double s = final_sum(st.partial, nblocks, lds);
+  const double s = final_sum(st.partial, kRedBlocks, lds);
   if (threadIdx.x == 0) {
     const double b = sqrt(s);
     st.beta[step] = b;
@@ -338,233 +558,194 @@ __global__ __launch_bounds__(kBlock) void lanczos_normalize_kernel(double *__res
     w[i] = w[i] / b;
 }
 
-// pass two: w = H v - beta_prev v_prev - alpha v
-__global__ __launch_bounds__(kBlock) void recon_hv_kernel(HvTables t, const C2<double> *__restrict__ v,
-                                                          const C2<double> *__restrict__ vp,
-                                                          C2<double> *__restrict__ w, double alpha,
-                                                          double bp) {
-  const int64_t total = t.d_up * t.d_dn;
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * kBlock) {
-    const int64_t u = e / t.d_dn, d = e - u * t.d_dn;
-    C2<double> h = hv_elem<double, 1>(t, v, u, d, 0);
-    if (bp != 0.0) {
-      const C2<double> q = vp[e];
-      h.re -= bp * q.re;
-      h.im -= bp * q.im;
-    }
-    const C2<double> ve = v[e];
-    h.re -= alpha * ve.re;
-    h.im -= alpha * ve.im;
-    w[e] = h;
+// w /= beta ; x += y * w   (complex y, planar vectors)
+__global__ __launch_bounds__(kBlock) void recon_update_kernel(double *__restrict__ w,
+                                                              double *__restrict__ x, double beta,
+                                                              double yr, double yi, int64_t D) {
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
+    const double wr = w[e] / beta, wi = w[D + e] / beta;
+    w[e] = wr;
+    w[D + e] = wi;
+    x[e] += wr * yr - wi * yi;
+    x[D + e] += wr * yi + wi * yr;
   }
 }
 
-// w /= beta ; x += y * w   (complex y)
-__global__ __launch_bounds__(kBlock) void recon_update_kernel(C2<double> *__restrict__ w,
-                                                              C2<double> *__restrict__ x, double beta,
-                                                              double yr, double yi, int64_t n) {
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
-    C2<double> q = w[e];
-    q.re = q.re / beta;
-    q.im = q.im / beta;
-    w[e] = q;
-    C2<double> o = x[e];
-    o.re += q.re * yr - q.im * yi;
-    o.im += q.re * yi + q.im * yr;
-    x[e] = o;
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void scale_complex_kernel(const C2<double> *__restrict__ x,
+__global__ __launch_bounds__(kBlock) void scale_complex_kernel(const double *__restrict__ x,
                                                                double yr, double yi,
-                                                               C2<double> *__restrict__ o, int64_t n) {
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
-    const C2<double> q = x[e];
-    o[e] = {q.re * yr - q.im * yi, q.re * yi + q.im * yr};
+                                                               double *__restrict__ o, int64_t D) {
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
+    const double xr = x[e], xi = x[D + e];
+    o[e] = xr * yr - xi * yi;
+    o[D + e] = xr * yi + xi * yr;
   }
 }
 
-// ------------------------------------------------------------ Chebyshev
-
-constexpr int kChebGrid = 2048;
-
-// tnext = scale2 * (H tn - shift tn) - tprev  (first: tnext = scale2 * (H tn - shift tn))
-// partial[b][r] = <tnext|tn>, partial[b][NV + r] = <tnext|tnext>   (FP64)
-template <typename R, int NV>
-__global__ __launch_bounds__(kBlock) void cheb_kernel(HvTables t, const C2<R> *__restrict__ tn,
-                                                      const C2<R> *__restrict__ tprev,
-                                                      C2<R> *__restrict__ tnext, R scale2, R shift,
-                                                      int first, double *partial) {
-  __shared__ double lds[2 * NV][kBlock / kWave];
-  const int64_t total = t.d_up * t.d_dn * NV;
-  double s_cross = 0.0, s_self = 0.0;
-  // kBlock is a multiple of NV and so is the grid stride: each thread keeps one r
-  const int r = threadIdx.x % NV;
-  for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < total; g += (int64_t)gridDim.x * kBlock) {
-    const int64_t e = g / NV;
-    const int64_t u = e / t.d_dn, d = e - u * t.d_dn;
-    C2<R> h = hv_elem<R, NV>(t, tn, u, d, r);
-    const C2<R> c = tn[g];
-    C2<R> o;
-    o.re = scale2 * (h.re - shift * c.re);
-    o.im = scale2 * (h.im - shift * c.im);
-    if (!first) {
-      const C2<R> q = tprev[g];
-      o.re -= q.re;
-      o.im -= q.im;
-    }
-    tnext[g] = o;
-    s_cross += (double)o.re * (double)c.re + (double)o.im * (double)c.im;
-    s_self += (double)o.re * (double)o.re + (double)o.im * (double)o.im;
-  }
-  // reduce per r: lanes with equal r are NV apart
-  const int tid = threadIdx.x;
-  for (int off = kWave / 2; off >= NV; off >>= 1) {
-    s_cross += __shfl_down(s_cross, off, kWave);
-    s_self += __shfl_down(s_self, off, kWave);
-  }
-  if ((tid & (kWave - 1)) < NV) {
-    lds[r][tid / kWave] = s_cross;
-    lds[NV + r][tid / kWave] = s_self;
-  }
-  __syncthreads();
-  if (tid < 2 * NV) {
-    double a = 0.0;
-    for (int w = 0; w < kBlock / kWave; ++w) a += lds[tid][w];
-    partial[(int64_t)blockIdx.x * 2 * NV + tid] = a;
-  }
-}
-
-// out[j] = sum_b partial[b * width + j]
-__global__ __launch_bounds__(kBlock) void reduce_partials_kernel(const double *partial, int nblocks,
-                                                                 int width, double *out) {
+// out[r] = sum over the slots of planes 2r, 2r+1 of partial[2 slot]; out[nvec + r] the odd entries
+__global__ __launch_bounds__(kBlock) void reduce_cheb_kernel(const double *partial, int64_t rows,
+                                                             int nvec, double *out) {
   __shared__ double lds[kBlock / kWave];
-  for (int j = 0; j < width; ++j) {
+  const int r = blockIdx.x;
+  for (int j = 0; j < 2; ++j) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < nblocks; i += kBlock) s += partial[(int64_t)i * width + j];
-    s = block_sum(s, lds);
-    if (threadIdx.x == 0) out[j] = s;
+    for (int64_t i = threadIdx.x; i < 2 * rows; i += kBlock)
+      s += partial[2 * ((int64_t)2 * r * rows + i) + j];
+    s = wg_sum(s, lds);
+    if (threadIdx.x == 0) out[j * nvec + r] = s;
   }
 }
 
-// partial[b][NV + r] = sum |x_r|^2 over the block's elements, partial[b][r] = 0
-template <typename R, int NV>
-__global__ __launch_bounds__(kBlock) void sumsq_multi_kernel(const C2<R> *__restrict__ x, int64_t D,
-                                                             double *partial) {
-  __shared__ double lds[NV][kBlock / kWave];
-  const int64_t total = D * NV;
-  const int r = threadIdx.x % NV;
-  double acc = 0.0;
-  for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < total; g += (int64_t)gridDim.x * kBlock) {
-    const C2<R> c = x[g];
-    acc += (double)c.re * (double)c.re + (double)c.im * (double)c.im;
+bool force_generic() {
+  const char *e = std::getenv("MANYBODY_AMD_HV_PATH");
+  return e && std::strcmp(e, "generic") == 0;
+}
+
+template <typename T>
+bool use_two_pass(const HvTables &t) {
+  return t.ell_ok && t.max_up_row <= 64 && t.d_dn * (int64_t)sizeof(T) <= kMaxRowLds &&
+         !force_generic();
+}
+
+int row_block(int64_t Ddn) {
+  if (Ddn >= 8192) return 1024;
+  if (Ddn >= 2048) return 512;
+  if (Ddn >= 256) return 256;
+  return (int)((Ddn + kWave - 1) / kWave) * kWave;
+}
+
+template <typename T, Epi E>
+void launch_row(const HvTables &t, int P, const T *x, T *y, const EpiArgs &a, hipStream_t s) {
+  const size_t lds = (size_t)t.d_dn * sizeof(T);
+  static bool attr_set = false;  // once per template instance
+  const void *kc = reinterpret_cast<const void *>(&row_kernel_compact<T, E>);
+  const void *ke = reinterpret_cast<const void *>(&row_kernel_ell<T, E>);
+  if (!attr_set) {
+    MB_HIP(hipFuncSetAttribute(kc, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxRowLds));
+    MB_HIP(hipFuncSetAttribute(ke, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxRowLds));
+    attr_set = true;
   }
-  const int tid = threadIdx.x;
-  for (int off = kWave / 2; off >= NV; off >>= 1) acc += __shfl_down(acc, off, kWave);
-  if ((tid & (kWave - 1)) < NV) lds[r][tid / kWave] = acc;
-  __syncthreads();
-  if (tid < NV) {
-    double a = 0.0;
-    for (int w = 0; w < kBlock / kWave; ++w) a += lds[tid][w];
-    partial[(int64_t)blockIdx.x * 2 * NV + NV + tid] = a;
-    partial[(int64_t)blockIdx.x * 2 * NV + tid] = 0.0;
-  }
+  const dim3 g((unsigned)(P * t.d_up)), b(row_block(t.d_dn));
+  if (t.comp_ok)
+    hipLaunchKernelGGL((row_kernel_compact<T, E>), g, b, lds, s, t, x, y, a);
+  else
+    hipLaunchKernelGGL((row_kernel_ell<T, E>), g, b, lds, s, t, x, y, a);
+  MB_HIP(hipGetLastError());
+}
+
+template <typename T, int VEC>
+void launch_strip(const HvTables &t, int P, const T *x, T *y, const int *flags, hipStream_t s) {
+  constexpr int W = 16 * VEC, RC = 32;
+  const int nJ = (int)((t.d_dn + W - 1) / W);
+  const int nC = (int)((t.d_up + RC - 1) / RC);
+  const int64_t S8 = ((int64_t)P * nJ + 7) / 8;
+  const int64_t grid = 8 * (int64_t)nC * S8;
+  if (grid >= (1ll << 31)) throw std::invalid_argument("sector too large for the strip kernel");
+  if (t.max_up_row <= 32)
+    hipLaunchKernelGGL((up_strip_kernel<T, VEC, 32>), dim3((unsigned)grid), dim3(kBlock), 0, s, t, P,
+                       x, y, nJ, nC, flags);
+  else
+    hipLaunchKernelGGL((up_strip_kernel<T, VEC, 64>), dim3((unsigned)grid), dim3(kBlock), 0, s, t, P,
+                       x, y, nJ, nC, flags);
+  MB_HIP(hipGetLastError());
 }
 
 }  // namespace
 
 // ------------------------------------------------------------ launchers
 
-void launch_hv_c128(const HvTables &t, const void *x, void *y, hipStream_t s) {
-  const int64_t D = t.d_up * t.d_dn;
-  hipLaunchKernelGGL((hv_kernel<double, 1>), dim3(grid_for(D, kGridCap)), dim3(kBlock), 0, s, t,
-                     (const C2<double> *)x, (C2<double> *)y);
-  MB_HIP(hipGetLastError());
+int64_t hv_partials(const HvTables &t, int P) {
+  const int64_t two = (int64_t)P * t.d_up, gen = (int64_t)P * kGenericGrid;
+  return two > gen ? two : gen;
 }
 
-void launch_hv_c64(const HvTables &t, const void *x, void *y, hipStream_t s) {
-  const int64_t D = t.d_up * t.d_dn;
-  hipLaunchKernelGGL((hv_kernel<float, 1>), dim3(grid_for(D, kGridCap)), dim3(kBlock), 0, s, t,
-                     (const C2<float> *)x, (C2<float> *)y);
-  MB_HIP(hipGetLastError());
-}
-
-void launch_hv_c64_multi(const HvTables &t, int nvec, const void *x, void *y, hipStream_t s) {
-  const int64_t W = t.d_up * t.d_dn * nvec;
-  const dim3 g(grid_for(W, kGridCap)), b(kBlock);
-  const auto *xi = (const C2<float> *)x;
-  auto *yo = (C2<float> *)y;
-  switch (nvec) {
-    case 1: hipLaunchKernelGGL((hv_kernel<float, 1>), g, b, 0, s, t, xi, yo); break;
-    case 2: hipLaunchKernelGGL((hv_kernel<float, 2>), g, b, 0, s, t, xi, yo); break;
-    case 4: hipLaunchKernelGGL((hv_kernel<float, 4>), g, b, 0, s, t, xi, yo); break;
-    case 8: hipLaunchKernelGGL((hv_kernel<float, 8>), g, b, 0, s, t, xi, yo); break;
-    case 16: hipLaunchKernelGGL((hv_kernel<float, 16>), g, b, 0, s, t, xi, yo); break;
-    default: throw std::invalid_argument("nvec must be 1, 2, 4, 8 or 16");
+template <typename T>
+int64_t launch_hv(const HvTables &t, int P, const T *x, T *y, Epi epi, const EpiArgs &a,
+                  hipStream_t s) {
+  if (use_two_pass<T>(t)) {
+    if ((int64_t)P * t.d_up >= (1ll << 31))
+      throw std::invalid_argument("sector too large for the two-pass H.v");
+    const int *flags = epi == Epi::Lanczos ? a.flags : nullptr;
+    constexpr int V = 16 / sizeof(T);
+    if (t.d_dn % V == 0)
+      launch_strip<T, V>(t, P, x, y, flags, s);
+    else
+      launch_strip<T, 1>(t, P, x, y, flags, s);
+    switch (epi) {
+      case Epi::Plain: launch_row<T, Epi::Plain>(t, P, x, y, a, s); break;
+      case Epi::Lanczos: launch_row<T, Epi::Lanczos>(t, P, x, y, a, s); break;
+      case Epi::Recon: launch_row<T, Epi::Recon>(t, P, x, y, a, s); break;
+      case Epi::Cheb: launch_row<T, Epi::Cheb>(t, P, x, y, a, s); break;
+    }
+    return t.d_up;  // partial slots per plane
+  }
+  const dim3 g(grid_for(t.d_up * t.d_dn, kGenericGrid), (unsigned)P);
+  switch (epi) {
+    case Epi::Plain: hipLaunchKernelGGL((generic_kernel<T, Epi::Plain>), g, dim3(kBlock), 0, s, t, x, y, a); break;
+    case Epi::Lanczos: hipLaunchKernelGGL((generic_kernel<T, Epi::Lanczos>), g, dim3(kBlock), 0, s, t, x, y, a); break;
+    case Epi::Recon: hipLaunchKernelGGL((generic_kernel<T, Epi::Recon>), g, dim3(kBlock), 0, s, t, x, y, a); break;
+    case Epi::Cheb: hipLaunchKernelGGL((generic_kernel<T, Epi::Cheb>), g, dim3(kBlock), 0, s, t, x, y, a); break;
   }
   MB_HIP(hipGetLastError());
+  return g.x;
 }
+
+template int64_t launch_hv<double>(const HvTables &, int, const double *, double *, Epi,
+                                   const EpiArgs &, hipStream_t);
+template int64_t launch_hv<float>(const HvTables &, int, const float *, float *, Epi,
+                                  const EpiArgs &, hipStream_t);
 
 void launch_gather_c128(const PermTables &p, const void *x_ref, void *x_dev, hipStream_t s) {
   hipLaunchKernelGGL((perm_kernel<true>), dim3(grid_for(p.d_up * p.d_dn, 1 << 16)), dim3(kBlock), 0,
-                     s, p, (const double2 *)x_ref, (double2 *)x_dev);
+                     s, p, (const double *)x_ref, (double *)x_dev);
   MB_HIP(hipGetLastError());
 }
 
 void launch_scatter_c128(const PermTables &p, const void *x_dev, void *x_ref, hipStream_t s) {
   hipLaunchKernelGGL((perm_kernel<false>), dim3(grid_for(p.d_up * p.d_dn, 1 << 16)), dim3(kBlock),
-                     0, s, p, (const double2 *)x_dev, (double2 *)x_ref);
+                     0, s, p, (const double *)x_dev, (double *)x_ref);
   MB_HIP(hipGetLastError());
 }
 
 void launch_fill_gaussian_c128(const PermTables &p, uint64_t seed, void *x, hipStream_t s) {
   hipLaunchKernelGGL(fill_gaussian_kernel, dim3(grid_for(p.d_up * p.d_dn, 1 << 16)), dim3(kBlock),
-                     0, s, p, seed, (double2 *)x);
+                     0, s, p, seed, (double *)x);
   MB_HIP(hipGetLastError());
 }
 
-void launch_fill_phase_c64_multi(const PermTables &p, uint64_t seed, int nvec, void *x,
-                                 hipStream_t s) {
-  hipLaunchKernelGGL((fill_phase_kernel<float>), dim3(grid_for(p.d_up * p.d_dn, 1 << 16)),
-                     dim3(kBlock), 0, s, p, seed, nvec, (C2<float> *)x);
+void launch_fill_phase(const PermTables &p, uint64_t seed, int nvec, bool fp32, void *x,
+                       hipStream_t s) {
+  const dim3 g(grid_for(p.d_up * p.d_dn, 1 << 16)), b(kBlock);
+  if (fp32)
+    hipLaunchKernelGGL((fill_phase_kernel<float>), g, b, 0, s, p, seed, nvec, (float *)x);
+  else
+    hipLaunchKernelGGL((fill_phase_kernel<double>), g, b, 0, s, p, seed, nvec, (double *)x);
   MB_HIP(hipGetLastError());
 }
 
-void launch_fill_phase_c128_multi(const PermTables &p, uint64_t seed, int nvec, void *x,
-                                  hipStream_t s) {
-  hipLaunchKernelGGL((fill_phase_kernel<double>), dim3(grid_for(p.d_up * p.d_dn, 1 << 16)),
-                     dim3(kBlock), 0, s, p, seed, nvec, (C2<double> *)x);
+void launch_sumsq(const void *x, bool fp32, int64_t n, double *partial, double *out,
+                  bool take_sqrt, hipStream_t s) {
+  if (fp32)
+    hipLaunchKernelGGL((sumsq_partial_kernel<float>), dim3(kRedBlocks), dim3(kBlock), 0, s,
+                       (const float *)x, n, partial);
+  else
+    hipLaunchKernelGGL((sumsq_partial_kernel<double>), dim3(kRedBlocks), dim3(kBlock), 0, s,
+                       (const double *)x, n, partial);
+  hipLaunchKernelGGL(final_kernel, dim3(1), dim3(kBlock), 0, s, (const double *)partial,
+                     (int64_t)kRedBlocks, out, take_sqrt ? 1 : 0);
   MB_HIP(hipGetLastError());
 }
 
-void launch_norm_c128(const double *x, int64_t n, double *partial, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(norm_partial_kernel, dim3(kRedBlocks), dim3(kBlock), 0, s, x, n, partial);
-  hipLaunchKernelGGL(sqrt_final_kernel, dim3(1), dim3(kBlock), 0, s, (const double *)partial,
-                     kRedBlocks, out);
-  MB_HIP(hipGetLastError());
+void launch_norm(const double *x, int64_t n, double *partial, double *out, hipStream_t s) {
+  launch_sumsq(x, false, n, partial, out, true, s);
 }
 
-void launch_scale_div_c128(const double *x, const double *den, double *y, int64_t n,
-                           hipStream_t s) {
+void launch_scale_div(const double *x, const double *den, double *y, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(scale_div_kernel, dim3(grid_for(n, 1 << 16)), dim3(kBlock), 0, s, x, den, y, n);
   MB_HIP(hipGetLastError());
 }
 
-static int lanczos_grid(int64_t D) { return grid_for(D, kLanczosGrid); }
-
-void launch_lanczos_hv(const HvTables &t, const double *v, const double *vprev, double *w,
-                       const LanczosState &st, int step, hipStream_t s) {
-  hipLaunchKernelGGL(lanczos_hv_kernel, dim3(lanczos_grid(t.d_up * t.d_dn)), dim3(kBlock), 0, s, t,
-                     (const C2<double> *)v, (const C2<double> *)vprev, (C2<double> *)w, st, step);
-  MB_HIP(hipGetLastError());
-}
-
-int lanczos_partial_blocks(int64_t D) { return lanczos_grid(D); }
-
-void launch_lanczos_finish_alpha(const LanczosState &st, int step, int nblocks, hipStream_t s) {
-  // nblocks = grid of the preceding lanczos_hv launch (one partial per block)
-  hipLaunchKernelGGL(lanczos_finish_alpha_kernel, dim3(1), dim3(kBlock), 0, s, st, step, nblocks);
+void launch_lanczos_finish_alpha(const LanczosState &st, int step, int64_t nparts, hipStream_t s) {
+  hipLaunchKernelGGL(lanczos_finish_alpha_kernel, dim3(1), dim3(kBlock), 0, s, st, step, nparts);
   MB_HIP(hipGetLastError());
 }
 
@@ -575,9 +756,8 @@ void launch_lanczos_axpy_norm(const double *v, double *w, int64_t n, const Lancz
   MB_HIP(hipGetLastError());
 }
 
-void launch_lanczos_finish_beta(const LanczosState &st, int step, int k, hipStream_t s) {
-  (void)k;
-  hipLaunchKernelGGL(lanczos_finish_beta_kernel, dim3(1), dim3(kBlock), 0, s, st, step, kRedBlocks);
+void launch_lanczos_finish_beta(const LanczosState &st, int step, hipStream_t s) {
+  hipLaunchKernelGGL(lanczos_finish_beta_kernel, dim3(1), dim3(kBlock), 0, s, st, step);
   MB_HIP(hipGetLastError());
 }
 
@@ -588,91 +768,25 @@ void launch_lanczos_normalize(double *w, int64_t n, const LanczosState &st, int 
   MB_HIP(hipGetLastError());
 }
 
-void launch_recon_hv(const HvTables &t, const double *v, const double *vprev, double *w,
-                     double alpha, double beta_prev, hipStream_t s) {
-  hipLaunchKernelGGL(recon_hv_kernel, dim3(grid_for(t.d_up * t.d_dn, kGridCap)), dim3(kBlock), 0, s,
-                     t, (const C2<double> *)v, (const C2<double> *)vprev, (C2<double> *)w, alpha,
-                     beta_prev);
-  MB_HIP(hipGetLastError());
-}
-
-void launch_recon_update(double *w, double *x, double beta, double yr, double yi, int64_t n,
+void launch_recon_update(double *w, double *x, double beta, double yr, double yi, int64_t D,
                          hipStream_t s) {
-  hipLaunchKernelGGL(recon_update_kernel, dim3(grid_for(n, 1 << 16)), dim3(kBlock), 0, s,
-                     (C2<double> *)w, (C2<double> *)x, beta, yr, yi, n);
+  hipLaunchKernelGGL(recon_update_kernel, dim3(grid_for(D, 1 << 16)), dim3(kBlock), 0, s, w, x,
+                     beta, yr, yi, D);
   MB_HIP(hipGetLastError());
 }
 
-void launch_scale_complex_c128(const double *x, double yr, double yi, double *out, int64_t n,
-                               hipStream_t s) {
-  hipLaunchKernelGGL(scale_complex_kernel, dim3(grid_for(n, 1 << 16)), dim3(kBlock), 0, s,
-                     (const C2<double> *)x, yr, yi, (C2<double> *)out, n);
+void launch_scale_complex(const double *x, double yr, double yi, double *out, int64_t D,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(scale_complex_kernel, dim3(grid_for(D, 1 << 16)), dim3(kBlock), 0, s, x, yr, yi,
+                     out, D);
   MB_HIP(hipGetLastError());
 }
 
-int cheb_partial_blocks() { return kChebGrid; }
-
-template <typename R>
-static void launch_cheb(const HvTables &t, int nvec, const R *tn, const R *tprev, R *tnext, R scale2,
-                        R shift, int first, double *partial, hipStream_t s) {
-  const dim3 g(kChebGrid), b(kBlock);
-  const auto *a = (const C2<R> *)tn;
-  const auto *p = (const C2<R> *)tprev;
-  auto *o = (C2<R> *)tnext;
-  switch (nvec) {
-    case 1: hipLaunchKernelGGL((cheb_kernel<R, 1>), g, b, 0, s, t, a, p, o, scale2, shift, first, partial); break;
-    case 2: hipLaunchKernelGGL((cheb_kernel<R, 2>), g, b, 0, s, t, a, p, o, scale2, shift, first, partial); break;
-    case 4: hipLaunchKernelGGL((cheb_kernel<R, 4>), g, b, 0, s, t, a, p, o, scale2, shift, first, partial); break;
-    case 8: hipLaunchKernelGGL((cheb_kernel<R, 8>), g, b, 0, s, t, a, p, o, scale2, shift, first, partial); break;
-    case 16: hipLaunchKernelGGL((cheb_kernel<R, 16>), g, b, 0, s, t, a, p, o, scale2, shift, first, partial); break;
-    default: throw std::invalid_argument("nvec must be 1, 2, 4, 8 or 16");
-  }
-  MB_HIP(hipGetLastError());
-}
-
-void launch_cheb_c64(const HvTables &t, int nvec, const float *tn, const float *tprev, float *tnext,
-                     float scale2, float shift, int first, double *partial, hipStream_t s) {
-  launch_cheb<float>(t, nvec, tn, tprev, tnext, scale2, shift, first, partial, s);
-}
-
-void launch_cheb_c128(const HvTables &t, int nvec, const double *tn, const double *tprev,
-                      double *tnext, double scale2, double shift, int first, double *partial,
-                      hipStream_t s) {
-  launch_cheb<double>(t, nvec, tn, tprev, tnext, scale2, shift, first, partial, s);
-}
-
-void launch_reduce_partials(const double *partial, int nblocks, int width, double *out,
-                            hipStream_t s) {
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kBlock), 0, s, partial, nblocks, width,
-                     out);
-  MB_HIP(hipGetLastError());
-}
-
-}  // namespace mb
-
-namespace mb {
-
-template <typename R>
-static void sumsq_launch(int nvec, const void *x, int64_t D, double *partial, hipStream_t s) {
-  const dim3 g(kChebGrid), b(kBlock);
-  const auto *a = (const C2<R> *)x;
-  switch (nvec) {
-    case 1: hipLaunchKernelGGL((sumsq_multi_kernel<R, 1>), g, b, 0, s, a, D, partial); break;
-    case 2: hipLaunchKernelGGL((sumsq_multi_kernel<R, 2>), g, b, 0, s, a, D, partial); break;
-    case 4: hipLaunchKernelGGL((sumsq_multi_kernel<R, 4>), g, b, 0, s, a, D, partial); break;
-    case 8: hipLaunchKernelGGL((sumsq_multi_kernel<R, 8>), g, b, 0, s, a, D, partial); break;
-    case 16: hipLaunchKernelGGL((sumsq_multi_kernel<R, 16>), g, b, 0, s, a, D, partial); break;
-    default: throw std::invalid_argument("nvec must be 1, 2, 4, 8 or 16");
-  }
-  MB_HIP(hipGetLastError());
-}
-
-void launch_sumsq_multi(bool fp32, int nvec, const void *x, int64_t D, double *partial,
+void launch_reduce_cheb(const double *partial, int64_t rows_per_plane, int nvec, double *out,
                         hipStream_t s) {
-  if (fp32)
-    sumsq_launch<float>(nvec, x, D, partial, s);
-  else
-    sumsq_launch<double>(nvec, x, D, partial, s);
+  hipLaunchKernelGGL(reduce_cheb_kernel, dim3(nvec), dim3(kBlock), 0, s, partial, rows_per_plane,
+                     nvec, out);
+  MB_HIP(hipGetLastError());
 }
 
 }  // namespace mb

@@ -94,10 +94,10 @@ void tridiag_eig(int m, const double *a, const double *b, double *w, double *V) 
 
 // ------------------------------------------------------------ Lanczos
 
-LanczosWork::LanczosWork(int64_t D, int k)
+LanczosWork::LanczosWork(int64_t D, int k, int64_t nparts)
     : D(D), k(k), v(D * 16), vp(D * 16), w(D * 16),
-      scal(sizeof(double) * (2 * (size_t)std::max(k, 1) + 2 +
-                             (size_t)std::max(kRedBlocks, lanczos_partial_blocks(D))) +
+      scal(sizeof(double) *
+               (2 * (size_t)std::max(k, 1) + 2 + (size_t)std::max<int64_t>(kRedBlocks, nparts)) +
            16) {
   double *base = scal.as<double>();
   st.alpha = base;
@@ -111,28 +111,33 @@ LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream
                                LanczosWork *work) {
   const int64_t D = h.dim();
   std::unique_ptr<LanczosWork> own;
+  const HvTables t = tables_of(h);
   if (!work || work->D != D || work->k < k) {
-    own.reset(new LanczosWork(D, k));
+    own.reset(new LanczosWork(D, k, hv_partials(t, 2)));
     work = own.get();
   }
   LanczosWork &W = *work;
-  const HvTables t = tables_of(h);
   LanczosResult res;
-  launch_norm_c128((const double *)b, 2 * D, W.st.partial, W.st.bnorm, s);
+  launch_norm((const double *)b, 2 * D, W.st.partial, W.st.bnorm, s);
   MB_HIP(hipMemcpyAsync(&res.b_norm, W.st.bnorm, sizeof(double), hipMemcpyDeviceToHost, s));
   MB_HIP(hipStreamSynchronize(s));
   if (res.b_norm <= kTol) throw std::runtime_error("Input vector has zero norm");
   if (k == 0) return res;
-  launch_scale_div_c128((const double *)b, W.st.bnorm, W.v.as<double>(), 2 * D, s);
+  launch_scale_div((const double *)b, W.st.bnorm, W.v.as<double>(), 2 * D, s);
   MB_HIP(hipMemsetAsync(W.vp.p, 0, (size_t)D * 16, s));
   MB_HIP(hipMemsetAsync(W.st.flags, 0, 2 * sizeof(int), s));
   double *v = W.v.as<double>(), *vp = W.vp.as<double>(), *w = W.w.as<double>();
-  const int nb = lanczos_partial_blocks(D);
   for (int i = 0; i < k; ++i) {
-    launch_lanczos_hv(t, v, vp, w, W.st, i, s);
-    launch_lanczos_finish_alpha(W.st, i, nb, s);
+    EpiArgs a;
+    a.prev = vp;
+    a.beta_dev = W.st.beta;
+    a.flags = W.st.flags;
+    a.step = i;
+    a.partial = W.st.partial;
+    const int64_t slots = launch_hv<double>(t, 2, v, w, Epi::Lanczos, a, s);
+    launch_lanczos_finish_alpha(W.st, i, 2 * slots, s);
     launch_lanczos_axpy_norm(v, w, 2 * D, W.st, i, s);
-    launch_lanczos_finish_beta(W.st, i, k, s);
+    launch_lanczos_finish_beta(W.st, i, s);
     launch_lanczos_normalize(w, 2 * D, W.st, i, s);
     double *tmp = vp;
     vp = v;
@@ -162,21 +167,25 @@ void lanczos_pass_two(const Hubbard &h, const void *b, const LanczosResult &dec,
     MB_HIP(hipMemsetAsync(x, 0, (size_t)D * 16, s));
     return;
   }
+  const HvTables t = tables_of(h);
   std::unique_ptr<LanczosWork> own;
   if (!work || work->D != D) {
-    own.reset(new LanczosWork(D, 1));
+    own.reset(new LanczosWork(D, 1, hv_partials(t, 2)));
     work = own.get();
   }
   LanczosWork &W = *work;
-  const HvTables t = tables_of(h);
   MB_HIP(hipMemcpyAsync(W.st.bnorm, &dec.b_norm, sizeof(double), hipMemcpyHostToDevice, s));
   double *v = W.v.as<double>(), *vp = W.vp.as<double>(), *w = W.w.as<double>();
-  launch_scale_div_c128((const double *)b, W.st.bnorm, v, 2 * D, s);
+  launch_scale_div((const double *)b, W.st.bnorm, v, 2 * D, s);
   MB_HIP(hipMemsetAsync(vp, 0, (size_t)D * 16, s));
-  launch_scale_complex_c128(v, y[0].real(), y[0].imag(), (double *)x, D, s);
+  launch_scale_complex(v, y[0].real(), y[0].imag(), (double *)x, D, s);
   for (int j = 0; j + 1 < dec.steps; ++j) {
     const double aj = dec.alphas[j], bj = dec.betas[j], bp = j == 0 ? 0.0 : dec.betas[j - 1];
-    launch_recon_hv(t, v, vp, w, aj, bp, s);
+    EpiArgs a;
+    a.prev = vp;
+    a.alpha = aj;
+    a.beta = bp;
+    launch_hv<double>(t, 2, v, w, Epi::Recon, a, s);
     launch_recon_update(w, (double *)x, bj, y[j + 1].real(), y[j + 1].imag(), D, s);
     double *tmp = vp;
     vp = v;
@@ -187,7 +196,7 @@ void lanczos_pass_two(const Hubbard &h, const void *b, const LanczosResult &dec,
 }
 
 double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipStream_t s) {
-  LanczosWork work(h.dim(), k);
+  LanczosWork work(h.dim(), k, hv_partials(tables_of(h), 2));
   LanczosResult dec = lanczos_pass_one(h, b, k, s, &work);
   const int m = dec.steps;
   if (m == 0) {
@@ -225,10 +234,10 @@ void time_evolve(const Hubbard &h, const void *psi, double t, int krylov_steps, 
   std::vector<double> scratch;
   int k = krylov_steps;
   if (k == 0 || k > D) k = (int)D;
-  LanczosWork work(D, k);
+  LanczosWork work(D, k, hv_partials(tables_of(h), 2));
   // norm first: a zero state is returned unchanged (evolve_state.h:131-134)
   double nrm = 0.0;
-  launch_norm_c128((const double *)psi, 2 * D, work.st.partial, work.st.bnorm, s);
+  launch_norm((const double *)psi, 2 * D, work.st.partial, work.st.bnorm, s);
   MB_HIP(hipMemcpyAsync(&nrm, work.st.bnorm, sizeof(double), hipMemcpyDeviceToHost, s));
   MB_HIP(hipStreamSynchronize(s));
   if (nrm <= kTol) {
@@ -277,39 +286,41 @@ void time_evolve_steps(const Hubbard &h, const void *psi, double t0, double t1, 
 void kpm_dos_moments(const Hubbard &h, const Sector &sec, double a, double b, int M, int nvec,
                      uint64_t seed, bool fp32, double *mu, hipStream_t s) {
   if (M < 2 || (M & 1)) throw std::invalid_argument("M must be even and >= 2");
-  if (nvec != 1 && nvec != 2 && nvec != 4 && nvec != 8 && nvec != 16)
-    throw std::invalid_argument("nvec must be 1, 2, 4, 8 or 16");
+  if (nvec < 1 || nvec > 64) throw std::invalid_argument("nvec must be in [1, 64]");
   if (!(a > 0.0)) throw std::invalid_argument("rescaling factor a must be positive");
   const int64_t D = h.dim();
-  const size_t esz = fp32 ? 8 : 16;
-  const size_t bytes = (size_t)D * nvec * esz;
+  const int P = 2 * nvec;
+  const size_t esz = fp32 ? 4 : 8;
+  const size_t bytes = (size_t)D * P * esz;
   DevBuf T0(bytes), T1(bytes), T2(bytes);
-  const int nb = cheb_partial_blocks();
+  const HvTables t = tables_of(h);
+  const int64_t nslots = hv_partials(t, P);
   const int width = 2 * nvec;
   const int steps = M / 2;
-  DevBuf partial((size_t)nb * width * sizeof(double));
+  DevBuf partial((size_t)std::max<int64_t>(2 * nslots, kRedBlocks) * sizeof(double));
   DevBuf raw((size_t)(steps + 1) * width * sizeof(double));
-  const HvTables t = tables_of(h);
+  MB_HIP(hipMemsetAsync(raw.p, 0, raw.bytes, s));
   const PermTables p = perm_of(sec);
-  if (fp32)
-    launch_fill_phase_c64_multi(p, seed, nvec, T0.p, s);
-  else
-    launch_fill_phase_c128_multi(p, seed, nvec, T0.p, s);
+  launch_fill_phase(p, seed, nvec, fp32, T0.p, s);
   double *rawp = raw.as<double>();
-  // row 0: <T0|T0> per vector (second half of the row)
-  launch_sumsq_multi(fp32, nvec, T0.p, D, partial.as<double>(), s);
-  launch_reduce_partials(partial.as<double>(), nb, width, rawp, s);
+  // row 0: <T0|T0> per vector in its second half
+  for (int r = 0; r < nvec; ++r)
+    launch_sumsq((const char *)T0.p + (size_t)2 * r * D * esz, fp32, 2 * D, partial.as<double>(),
+                 rawp + nvec + r, false, s);
   void *tn = T0.p, *tprev = T2.p, *tnext = T1.p;
   for (int n = 0; n < steps; ++n) {
-    const int first = n == 0;
-    const double scale2 = first ? 1.0 / a : 2.0 / a;
+    EpiArgs ea;
+    ea.prev = tprev;
+    ea.first = n == 0;
+    ea.scale2 = ea.first ? 1.0 / a : 2.0 / a;
+    ea.shift = b;
+    ea.partial = partial.as<double>();
+    int64_t rows;
     if (fp32)
-      launch_cheb_c64(t, nvec, (const float *)tn, (const float *)tprev, (float *)tnext,
-                      (float)scale2, (float)b, first, partial.as<double>(), s);
+      rows = launch_hv<float>(t, P, (const float *)tn, (float *)tnext, Epi::Cheb, ea, s);
     else
-      launch_cheb_c128(t, nvec, (const double *)tn, (const double *)tprev, (double *)tnext,
-                       scale2, b, first, partial.as<double>(), s);
-    launch_reduce_partials(partial.as<double>(), nb, width, rawp + (size_t)(n + 1) * width, s);
+      rows = launch_hv<double>(t, P, (const double *)tn, (double *)tnext, Epi::Cheb, ea, s);
+    launch_reduce_cheb(partial.as<double>(), rows, nvec, rawp + (size_t)(n + 1) * width, s);
     void *old_prev = tprev;
     tprev = tn;
     tn = tnext;

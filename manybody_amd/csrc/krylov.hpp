@@ -23,7 +23,7 @@ struct LanczosWork {
   int k;
   DevBuf v, vp, w, scal;
   LanczosState st;
-  LanczosWork(int64_t D, int k);
+  LanczosWork(int64_t D, int k, int64_t nparts);
 };
 
 LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream_t s,
@@ -40,8 +40,8 @@ void time_evolve_steps(const Hubbard &h, const void *psi, double t0, double t1, 
 void kpm_dos_moments(const Hubbard &h, const Sector &sec, double a, double b, int M, int nvec,
                      uint64_t seed, bool fp32, double *mu, hipStream_t s);
 
-// sum |x_r|^2 per vector into partial[blk * 2 nvec + nvec + r] (first half zero)
-void launch_sumsq_multi(bool fp32, int nvec, const void *x, int64_t D, double *partial,
-                        hipStream_t s);
+// out[0] = sum x_i^2 over n values (FP64 accumulation), or its square root
+void launch_sumsq(const void *x, bool fp32, int64_t n, double *partial, double *out,
+                  bool take_sqrt, hipStream_t s);
 
 }  // namespace mb

@@ -1,5 +1,12 @@
 // Internal declarations shared by the host code and the HIP kernels of
 // libmanybody_hip.so.  Not part of the public C ABI (include/manybody_hip.h).
+//
+// Device vector layout ("device order"): P real planes of D = D_up * D_dn
+// values each, plane p at offset p * D, element (u, d) at u * D_dn + d, where u
+// and d are the combinatorial (colex) ranks of the up and down bitstrings.  A
+// complex vector is P = 2 planes (re, im); nvec complex vectors are P = 2 nvec
+// planes, vector r = planes 2r, 2r+1.  H is real, so every plane is transformed
+// independently by the same tables.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -53,7 +60,7 @@ struct DevBuf {
 
 template <typename T>
 DevBuf upload(const std::vector<T> &v) {
-  DevBuf b(v.size() * sizeof(T) + 1);
+  DevBuf b(v.size() * sizeof(T) + 16);
   if (!v.empty()) MB_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   return b;
 }
@@ -92,13 +99,25 @@ struct Sector {
 
 // ---------------------------------------------------------------- model
 // Per-species hop table: CSR rows by string rank; each entry the target rank
-// and the signed amplitude (coefficient * fermionic sign).
+// and the signed amplitude (coefficient * fermionic sign).  The down species
+// also gets a column-major packed copy (ELL): entry k of string d at
+// ell[k * D + d] = target << 8 | sign << 7 | magnitude index.
 struct SpeciesHops {
   std::vector<int32_t> row_ptr, tgt;
   std::vector<double> amp;
   std::vector<double> onsite;  // one-body diagonal of self bonds (all zero unless a size is 1)
   int max_row = 0;
   DevBuf d_row_ptr, d_tgt, d_amp, d_ampf, d_onsite, d_onsitef;
+  // packed column-major copy
+  bool ell_ok = false;
+  int ell_k = 0;
+  std::vector<uint32_t> ell;
+  std::vector<uint8_t> cnt;
+  std::vector<double> mags;
+  DevBuf d_ell, d_cnt, d_mags, d_magsf;
+  bool comp_ok = false;
+  std::vector<uint16_t> comp;
+  DevBuf d_comp;
 };
 
 struct Hubbard {
@@ -124,6 +143,15 @@ struct HvTables {
   const double *up_amp, *dn_amp, *up_on, *dn_on, *diagU;
   const float *up_ampf, *dn_ampf, *up_onf, *dn_onf, *diagUf;
   int has_onsite;
+  // packed down table
+  int ell_ok, ell_k, n_mags;
+  const uint32_t *dn_ell;
+  const uint8_t *dn_cnt;
+  const double *mags;
+  const float *magsf;
+  // compact down table: 32 uint16 entries per string (target << 1 | sign), one magnitude
+  int comp_ok;
+  const uint16_t *dn_comp;
   int max_up_row;
 };
 HvTables tables_of(const Hubbard &h);
@@ -136,59 +164,67 @@ struct PermTables {
 };
 PermTables perm_of(const Sector &s);
 
-// Number of blocks used by the deterministic reductions.
-constexpr int kRedBlocks = 512;
+// What the second H.v kernel does with (H x) before storing it.
+enum class Epi : int {
+  Plain = 0,    // y = H x
+  Lanczos = 1,  // w = H v - beta[step-1] v_prev ; partial[wg] = sum v . w
+  Recon = 2,    // w = H v - beta v_prev - alpha v
+  Cheb = 3,     // t' = scale2 (H t - shift t) - t_prev ; partial[2 wg + {0,1}] = <t'|t>, <t'|t'>
+};
 
-void launch_hv_c128(const HvTables &t, const void *x, void *y, hipStream_t s);
-void launch_hv_c64(const HvTables &t, const void *x, void *y, hipStream_t s);
-void launch_hv_c64_multi(const HvTables &t, int nvec, const void *x, void *y, hipStream_t s);
+struct EpiArgs {
+  const void *prev = nullptr;     // v_prev / t_prev planes (same layout as x)
+  const double *beta_dev = nullptr;  // Lanczos: device beta array, uses beta_dev[step-1]
+  const int *flags = nullptr;     // Lanczos: flags[0] != 0 -> kernels are no-ops
+  int step = 0;
+  double alpha = 0, beta = 0;     // Recon
+  double scale2 = 0, shift = 0;   // Cheb
+  int first = 0;                  // Cheb: no t_prev term
+  double *partial = nullptr;      // per-row-workgroup partial sums
+};
+
+// y = epilogue(H x) over P planes; T = double or float.  Returns the number of
+// partial sums written (one, or two for Cheb, per row workgroup).
+template <typename T>
+int64_t launch_hv(const HvTables &t, int P, const T *x, T *y, Epi epi, const EpiArgs &a,
+                  hipStream_t s);
+// Number of row workgroups (= partial slots) for P planes.
+int64_t hv_partials(const HvTables &t, int P);
+
+// Number of blocks used by the elementwise deterministic reductions.
+constexpr int kRedBlocks = 512;
 
 void launch_gather_c128(const PermTables &p, const void *x_ref, void *x_dev, hipStream_t s);
 void launch_scatter_c128(const PermTables &p, const void *x_dev, void *x_ref, hipStream_t s);
 void launch_fill_gaussian_c128(const PermTables &p, uint64_t seed, void *x, hipStream_t s);
-void launch_fill_phase_c64_multi(const PermTables &p, uint64_t seed, int nvec, void *x,
-                                 hipStream_t s);
-void launch_fill_phase_c128_multi(const PermTables &p, uint64_t seed, int nvec, void *x,
-                                  hipStream_t s);
+void launch_fill_phase(const PermTables &p, uint64_t seed, int nvec, bool fp32, void *x,
+                       hipStream_t s);
 
 // Lanczos device-resident scalars for one run of k steps.
 struct LanczosState {
   double *alpha;     // [k]
   double *beta;      // [k]
-  double *partial;   // [kRedBlocks]
+  double *partial;   // [max(kRedBlocks, hv_partials)]
   int *flags;        // [0] = done, [1] = steps taken
   double *bnorm;     // [1]
 };
-void launch_norm_c128(const double *x, int64_t n, double *partial, double *out, hipStream_t s);
-void launch_scale_div_c128(const double *x, const double *den, double *y, int64_t n,
-                           hipStream_t s);
-// w = H v - beta_prev v_prev; alpha partials; guarded by flags[0]
-void launch_lanczos_hv(const HvTables &t, const double *v, const double *vprev, double *w,
-                       const LanczosState &st, int step, hipStream_t s);
-int lanczos_partial_blocks(int64_t D);
-void launch_lanczos_finish_alpha(const LanczosState &st, int step, int nblocks, hipStream_t s);
+void launch_norm(const double *x, int64_t n, double *partial, double *out, hipStream_t s);
+void launch_scale_div(const double *x, const double *den, double *y, int64_t n, hipStream_t s);
+void launch_lanczos_finish_alpha(const LanczosState &st, int step, int64_t nparts, hipStream_t s);
 void launch_lanczos_axpy_norm(const double *v, double *w, int64_t n, const LanczosState &st,
                               int step, hipStream_t s);
-void launch_lanczos_finish_beta(const LanczosState &st, int step, int k, hipStream_t s);
+void launch_lanczos_finish_beta(const LanczosState &st, int step, hipStream_t s);
 void launch_lanczos_normalize(double *w, int64_t n, const LanczosState &st, int step,
                               hipStream_t s);
-// pass two: w = H v - beta_prev v_prev - alpha v; w /= beta; x += y_j w
-void launch_recon_hv(const HvTables &t, const double *v, const double *vprev, double *w,
-                     double alpha, double beta_prev, hipStream_t s);
-void launch_recon_update(double *w, double *x, double beta, double yr, double yi, int64_t n,
+// pass two: w /= beta ; x += y w (complex y, planar vectors of D elements)
+void launch_recon_update(double *w, double *x, double beta, double yr, double yi, int64_t D,
                          hipStream_t s);
-void launch_scale_complex_c128(const double *x, double yr, double yi, double *out, int64_t n,
-                               hipStream_t s);
-
-// KPM: element-major nvec complex vectors.
-void launch_cheb_c64(const HvTables &t, int nvec, const float *tn, const float *tprev, float *tnext,
-                     float scale2, float shift, int first, double *partial, hipStream_t s);
-void launch_cheb_c128(const HvTables &t, int nvec, const double *tn, const double *tprev,
-                      double *tnext, double scale2, double shift, int first, double *partial,
-                      hipStream_t s);
-void launch_reduce_partials(const double *partial, int nblocks, int width, double *out,
-                            hipStream_t s);
-int cheb_partial_blocks();
+void launch_scale_complex(const double *x, double yr, double yi, double *out, int64_t D,
+                          hipStream_t s);
+// out[r] = sum over row workgroups of planes 2r, 2r+1 of partial[2 wg + j] for j = 0,
+// out[nvec + r] likewise for j = 1.
+void launch_reduce_cheb(const double *partial, int64_t rows_per_plane, int nvec, double *out,
+                        hipStream_t s);
 
 // host numerics
 void tridiag_eig(int m, const double *a, const double *b, double *w, double *V /*nullable*/);

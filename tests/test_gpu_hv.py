@@ -26,6 +26,14 @@ CASES = [
     ("2d_3x1_2_1_selfbond", [3, 1], 2, 1, 1.0, 2.0),
     ("2d_4x4_3_2", [4, 4], 3, 2, 1.0, 8.0),
     ("3d_2x2x2_3_3", [2, 2, 2], 3, 3, 1.0, 4.0),
+    ("2d_4x2_3_3_two_magnitudes", [4, 2], 3, 3, 1.0, 4.0),
+    ("2d_2x3_2_2_two_magnitudes", [2, 3], 2, 2, 1.0, 2.5),
+]
+
+# amplitudes that are not exact binary fractions: parity at a relative 1e-14
+FP_CASES = [
+    ("2d_2x3_2_2_t07", [2, 3], 2, 2, 0.7, 2.3),
+    ("1d_L10_5_4_t03", [10], 5, 4, -0.3, 1.7),
 ]
 
 
@@ -54,7 +62,7 @@ def test_hv_c128_bitexact_vs_product_oracle(cuda, name, sizes, nu, nd, t, U):
     P = po.ProductOperator(sizes, nu, nd, t, U)
     assert (op.d_up, op.d_dn) == (P.Dup, P.Ddn)
     x = int_vec(np.random.default_rng(1), P.D)
-    y = op.hv(torch.from_numpy(x).to(cuda)).cpu().numpy()
+    y = mb.from_planar(op.hv(mb.to_planar(x)))
     assert np.array_equal(y, P.matvec(x))
 
 
@@ -78,7 +86,7 @@ def test_permutation_matches_oracle_and_roundtrips(cuda):
     xd = op.to_device_order(torch.from_numpy(x_ref).to(cuda))
     expect = np.zeros_like(x_ref)
     expect[perm] = x_ref
-    assert np.array_equal(xd.cpu().numpy(), expect)
+    assert np.array_equal(mb.from_planar(xd), expect)
     back = op.to_reference_order(xd).cpu().numpy()
     assert np.array_equal(back, x_ref)
 
@@ -89,8 +97,8 @@ def test_hv_c64_bitexact_vs_product_oracle(cuda, name, sizes, nu, nd, t, U):
     op = make_op(sizes, nu, nd, t, U)
     P = po.ProductOperator(sizes, nu, nd, t, U)
     x = int_vec(np.random.default_rng(3), P.D)
-    y = op.hv(torch.from_numpy(x.astype(np.complex64)).to(cuda)).cpu().numpy()
-    assert np.array_equal(y.astype(np.complex128), P.matvec(x))
+    y = mb.from_planar(op.hv(mb.to_planar(x, dtype=torch.float32)))
+    assert np.array_equal(y, P.matvec(x))
 
 
 def test_hv_c64_multi_matches_single(cuda):
@@ -100,21 +108,23 @@ def test_hv_c64_multi_matches_single(cuda):
     P = po.ProductOperator(sizes, nu, nd, 1.0, 4.0)
     rng = np.random.default_rng(4)
     for nvec in (2, 4, 8, 16):
-        X = np.stack([int_vec(rng, P.D) for _ in range(nvec)], axis=1)  # element-major
-        Y = op.hv(torch.from_numpy(X.astype(np.complex64).ravel()).to(cuda)).cpu().numpy()
-        Y = Y.reshape(P.D, nvec).astype(np.complex128)
+        X = np.stack([int_vec(rng, P.D) for _ in range(nvec)])  # (nvec, D)
+        Y = mb.from_planar(op.hv(mb.to_planar(X, dtype=torch.float32)))
         for r in range(nvec):
-            assert np.array_equal(Y[:, r], P.matvec(np.ascontiguousarray(X[:, r]))), (nvec, r)
+            assert np.array_equal(Y[r], P.matvec(np.ascontiguousarray(X[r]))), (nvec, r)
 
 
-@pytest.mark.parametrize("name,sizes,nu,nd,t,U", CASES[:3], ids=[c[0] for c in CASES[:3]])
+GAUSS_CASES = CASES[:3] + FP_CASES
+
+
+@pytest.mark.parametrize("name,sizes,nu,nd,t,U", GAUSS_CASES, ids=[c[0] for c in GAUSS_CASES])
 def test_hv_c128_gaussian_tolerance(cuda, name, sizes, nu, nd, t, U):
     import torch
     op = make_op(sizes, nu, nd, t, U)
     P = po.ProductOperator(sizes, nu, nd, t, U)
     x_t = op.gaussian(seed=123)
-    x = x_t.cpu().numpy()
-    y = op.hv(x_t).cpu().numpy()
+    x = mb.from_planar(x_t)
+    y = mb.from_planar(op.hv(x_t))
     ref = P.matvec(x)
     assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-14
 
@@ -126,7 +136,7 @@ def test_gaussian_fill_is_seeded_per_reference_index(cuda):
     b = op.gaussian(7)
     c = op.gaussian(8)
     assert torch.equal(a, b) and not torch.equal(a, c)
-    v = a.cpu().numpy()
+    v = mb.from_planar(a)
     assert abs(v.real.mean()) < 0.01 and abs(v.imag.mean()) < 0.01
     assert abs(v.real.var() - 1) < 0.01 and abs(v.imag.var() - 1) < 0.01
 
@@ -139,8 +149,8 @@ def test_hv_c2_full_size_bitexact(cuda):
     assert op.dimension() == 130873600
     P = po.ProductOperator(sizes, nu, nd, t, U, threads=16)
     x = int_vec(np.random.default_rng(5), P.D)
-    xd = torch.from_numpy(x).to(cuda)
-    y = op.hv(xd).cpu().numpy()
+    xd = mb.to_planar(x)
+    y = mb.from_planar(op.hv(xd))
     del xd
     ref = P.matvec(x)
     assert np.array_equal(y, ref)
@@ -150,8 +160,22 @@ def test_hv_c2_hermitian(cuda):
     """<x|H y> = <H x|y> on seeded Gaussian vectors at full size (FP64)."""
     import torch
     op = make_op([4, 4], 7, 7, 1.0, 8.0)
+    def c(t):
+        return torch.complex(t[0], t[1])
     x = op.gaussian(1)
     y = op.gaussian(2)
-    lhs = torch.vdot(x, op.hv(y)).item()
-    rhs = torch.vdot(op.hv(x), y).item()
+    lhs = torch.vdot(c(x), c(op.hv(y))).item()
+    rhs = torch.vdot(c(op.hv(x)), c(y)).item()
     assert abs(lhs - rhs) / abs(lhs) < 1e-12
+
+
+@pytest.mark.parametrize("name,sizes,nu,nd,t,U", CASES[:5], ids=[c[0] for c in CASES[:5]])
+def test_generic_path_matches_two_pass(cuda, name, sizes, nu, nd, t, U, monkeypatch):
+    """The one-kernel fallback path (rows too long for LDS) gives the same H.v."""
+    op = make_op(sizes, nu, nd, t, U)
+    P = po.ProductOperator(sizes, nu, nd, t, U)
+    x = int_vec(np.random.default_rng(9), P.D)
+    monkeypatch.setenv("MANYBODY_AMD_HV_PATH", "generic")
+    y = mb.from_planar(op.hv(mb.to_planar(x)))
+    monkeypatch.delenv("MANYBODY_AMD_HV_PATH")
+    assert np.array_equal(y, P.matvec(x))

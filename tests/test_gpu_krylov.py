@@ -32,7 +32,7 @@ def test_lanczos_alpha_beta_match_oracle_c1(cuda):
     P = po.ProductOperator(sizes, nu, nd, t, U)
     b = op.gaussian(seed=2024)
     dec = mb.lanczos_pass_one(op, b, 40)
-    al, be, bn = P.lanczos(b.cpu().numpy(), 40)
+    al, be, bn = P.lanczos(mb.from_planar(b), 40)
     assert dec.steps_taken == len(al) == 40 and len(dec.betas) == 39
     assert abs(dec.b_norm - bn) / bn < 1e-14
     assert np.abs(np.array(dec.alphas) - al).max() / np.abs(al).max() < 1e-12
@@ -80,7 +80,7 @@ def test_lanczos_first_steps_c2_match_oracle(cuda):
     b = op.gaussian(seed=5)
     dec = mb.lanczos_pass_one(op, b, 4)
     P = po.ProductOperator(sizes, nu, nd, t, U, threads=16)
-    al, be, bn = P.lanczos(b.cpu().numpy(), 4)
+    al, be, bn = P.lanczos(mb.from_planar(b), 4)
     assert np.abs(np.array(dec.alphas) - al).max() / np.abs(al).max() < 1e-12
     assert np.abs(np.array(dec.betas) - be).max() / np.abs(be).max() < 1e-12
 
@@ -93,7 +93,7 @@ def test_lanczos_breakdown_and_zero_norm(cuda):
     assert dec.steps_taken == 1 and dec.betas == []
     op2 = make_op([6], 3, 3, 1.0, 2.0)
     with pytest.raises(RuntimeError, match="zero norm"):
-        mb.lanczos_pass_one(op2, torch.zeros(op2.dimension(), dtype=torch.complex128,
+        mb.lanczos_pass_one(op2, torch.zeros((2, op2.dimension()), dtype=torch.float64,
                                              device=cuda), 5)
     dec0 = mb.lanczos_pass_one(op2, op2.gaussian(1), 0)
     assert dec0.steps_taken == 0 and dec0.b_norm > 0
@@ -131,8 +131,8 @@ def test_kpm_fp64_moments_match_oracle(cuda):
     a, b = (ev[-1] - ev[0]) / 2 * 1.02, (ev[-1] + ev[0]) / 2  # spectrum inside [-1, 1]
     M, nvec, seed = 64, 4, 99
     mu = np.array(mb.kpm_dos_moments(op, a, b, M, nvec, seed, fp32=False))
-    vecs = op.random_phase(seed, nvec, fp32=False).reshape(op.dimension(), nvec)
-    vecs_ref = np.stack([op.to_reference_order(vecs[:, r].contiguous()).cpu().numpy()
+    vecs = op.random_phase(seed, nvec, fp32=False)  # (2 nvec, D)
+    vecs_ref = np.stack([op.to_reference_order(vecs[2 * r:2 * r + 2].contiguous()).cpu().numpy()
                          for r in range(nvec)])
     ref = po.kpm_moments_csc(A, a, b, M, vecs_ref)
     assert np.abs(mu - ref).max() < 1e-11
