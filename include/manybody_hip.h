@@ -194,6 +194,33 @@ int mb_jackson_kernel(int N, double *g);
 /* rescaling::from_bounds(E_min, E_max, padding) — rescaling.h:33-38. */
 int mb_rescaling_from_bounds(double e_min, double e_max, double padding, double *a, double *b);
 
+/* ------------------------------------------------------------ batched solves */
+
+/* Many independent Lanczos solves advanced in lockstep by one launch per phase
+ * of a step for the whole batch (no reference counterpart: the reference runs
+ * lanczos_pass_one, lanczos.h:130-175, once per operator).  Models may repeat;
+ * the caller keeps every model (and its basis) alive while the batch exists. */
+typedef struct mb_batch mb_batch;
+int mb_batch_create(int n, const mb_hubbard *const *models, mb_batch **out);
+void mb_batch_destroy(mb_batch *b);
+/* Device bytes held by the batch's vectors. */
+int64_t mb_batch_footprint(const mb_batch *b);
+/* Start vector of every instance: seeded N(0,1) per reference index as
+ * mb_fill_gaussian_c128 with seeds[i]; or copied from a device-order vector. */
+int mb_batch_fill_gaussian(mb_batch *b, const uint64_t *seeds, mb_stream stream);
+int mb_batch_set_start(mb_batch *b, int i, const void *b_dev, mb_stream stream);
+/* Work mapping: elements per tile (one workgroup) and workgroup size. */
+int mb_batch_set_mapping(mb_batch *b, int tile, int block);
+/* Times k_probe steps for each candidate mapping and keeps the fastest. */
+int mb_batch_autotune(mb_batch *b, int k_probe, int *tile, int *block, mb_stream stream);
+/* k steps of lanczos_pass_one for every instance (optionally replayed as one
+ * hipGraph).  alphas/betas: n*k (instance-major), steps/b_norms: n. */
+int mb_batch_lanczos(mb_batch *b, int k, int use_graph, double *alphas, double *betas,
+                     int *steps, double *b_norms, mb_stream stream);
+/* Lowest Ritz value of every instance after k steps (FP64 tridiagonal solve on
+ * the host), as find_min_eigenpair's value (lanczos.h:283-290). */
+int mb_batch_ground_energies(mb_batch *b, int k, int use_graph, double *e0, mb_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,15 @@ _SIGS = {
     "mb_kpm_dos_moments": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_uint64,
                                      C.c_int, _dp, _vp]),
     "mb_jackson_kernel": (C.c_int, [C.c_int, _dp]),
+    "mb_batch_create": (C.c_int, [C.c_int, _vp, C.POINTER(_vp)]),
+    "mb_batch_destroy": (None, [_vp]),
+    "mb_batch_footprint": (C.c_int64, [_vp]),
+    "mb_batch_fill_gaussian": (C.c_int, [_vp, _vp, _vp]),
+    "mb_batch_set_start": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "mb_batch_set_mapping": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "mb_batch_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
+    "mb_batch_lanczos": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "mb_batch_ground_energies": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp]),
     "mb_rescaling_from_bounds": (C.c_int, [C.c_double, C.c_double, C.c_double, _dp, _dp]),
 }
 
@@ -414,3 +423,64 @@ def rescaling_from_bounds(e_min: float, e_max: float, padding: float = 0.01):
     a, b = C.c_double(), C.c_double()
     _check(lib().mb_rescaling_from_bounds(e_min, e_max, padding, C.byref(a), C.byref(b)))
     return a.value, b.value
+
+
+# ------------------------------------------------------------------ batched
+
+class Batch:
+    """Many independent Lanczos solves in lockstep (one launch per phase of a
+    step for the whole batch).  Keeps its operators (and their bases) alive."""
+
+    def __init__(self, ops):
+        self.ops = list(ops)
+        arr = (C.c_void_p * len(self.ops))(*[op._h for op in self.ops])
+        h = C.c_void_p()
+        _check(lib().mb_batch_create(len(self.ops), arr, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mb_batch_destroy(self._h)
+            self._h = None
+
+    def __len__(self):
+        return len(self.ops)
+
+    def footprint(self) -> int:
+        return int(lib().mb_batch_footprint(self._h))
+
+    def fill_gaussian(self, seeds):
+        import numpy as np
+        s = np.ascontiguousarray(seeds, np.uint64)
+        _check(lib().mb_batch_fill_gaussian(self._h, s.ctypes.data_as(C.c_void_p), _stream()))
+
+    def set_start(self, i: int, b):
+        _check(lib().mb_batch_set_start(self._h, i, _ptr(b), _stream()))
+
+    def set_mapping(self, tile: int, block: int):
+        _check(lib().mb_batch_set_mapping(self._h, tile, block))
+
+    def autotune(self, k_probe: int = 3):
+        t, b = C.c_int(), C.c_int()
+        _check(lib().mb_batch_autotune(self._h, k_probe, C.byref(t), C.byref(b), _stream()))
+        return t.value, b.value
+
+    def lanczos(self, k: int, use_graph: bool = True):
+        import numpy as np
+        n = len(self.ops)
+        al = np.zeros((n, k))
+        be = np.zeros((n, k))
+        st = np.zeros(n, np.int32)
+        bn = np.zeros(n)
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+        _check(lib().mb_batch_lanczos(self._h, k, int(use_graph), vp(al), vp(be), vp(st), vp(bn),
+                                      _stream()))
+        return [LanczosDecomposition(list(al[i, :st[i]]), list(be[i, :max(st[i] - 1, 0)]),
+                                     int(st[i]), float(bn[i])) for i in range(n)]
+
+    def ground_energies(self, k: int, use_graph: bool = True):
+        import numpy as np
+        e0 = np.zeros(len(self.ops))
+        _check(lib().mb_batch_ground_energies(self._h, k, int(use_graph),
+                                              e0.ctypes.data_as(C.c_void_p), _stream()))
+        return e0

@@ -1,0 +1,520 @@
+// Batched Lanczos: many independent Hubbard instances of uneven dimension
+// advanced in lockstep, one launch per phase of a step for the whole batch.
+//
+// Work mapping: every instance's elements (u, d) are cut into tiles of `tile`
+// elements; one workgroup processes one tile of one instance, both planes
+// (re, im) at once so the index arithmetic and table reads are shared.  A tile
+// table (instance, first element, count) lets one launch cover all instances;
+// the tile size and workgroup size are chosen by the autotuner.  Per-instance
+// reductions are two-level and atomic-free: per-tile partial sums, then one
+// wave per instance adds its tiles in tile order.
+//
+// The per-step sequence (hv + alpha, axpy + beta^2, beta + breakdown,
+// normalise) is the recurrence of lanczos.h:15-40 for every instance; an
+// instance whose beta falls to 1000 eps is marked done and skipped from then
+// on (lanczos.h:35-37, 165-167).  The whole k-step sequence can be captured in
+// one hipGraph: no memset nodes are recorded (initialisation uses kernels).
+#include "batch.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+namespace mb {
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int popc(uint32_t v) { return __builtin_popcount(v); }
+
+template <int BS>
+__device__ double block_sum(double v, double *lds) {
+  const int tid = threadIdx.x;
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+  if ((tid & (kWave - 1)) == 0) lds[tid / kWave] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (tid == 0)
+    for (int w = 0; w < BS / kWave; ++w) r += lds[w];
+  __syncthreads();
+  return r;
+}
+
+// (H x)[e] for both planes of one instance, x planar with plane stride D.
+__device__ __forceinline__ void hv_pair(const HvTables &t, const double *__restrict__ x, int64_t D,
+                                        int64_t e, double &re, double &im) {
+  const int64_t Ddn = t.d_dn;
+  const int64_t u = e / Ddn, d = e - u * Ddn;
+  double dg = t.diagU[popc(t.s_up[u] & t.s_dn[d])];
+  if (t.has_onsite) dg = dg + t.up_on[u] + t.dn_on[d];
+  const double *xr = x, *xi = x + D;
+  double ur = 0.0, ui = 0.0;
+  for (int32_t k = t.up_ptr[u]; k < t.up_ptr[u + 1]; ++k) {
+    const int64_t f = (int64_t)t.up_tgt[k] * Ddn + d;
+    const double a = t.up_amp[k];
+    ur += a * xr[f];
+    ui += a * xi[f];
+  }
+  double ar = dg * xr[e], ai = dg * xi[e];
+  for (int32_t k = t.dn_ptr[d]; k < t.dn_ptr[d + 1]; ++k) {
+    const int64_t f = u * Ddn + t.dn_tgt[k];
+    const double a = t.dn_amp[k];
+    ar += a * xr[f];
+    ai += a * xi[f];
+  }
+  re = ur + ar;
+  im = ui + ai;
+}
+
+// Row-tiled variant for instances with the compact down table: the tile is
+// rows [u0, u0 + R) (R * D_dn elements); both planes of those rows are staged in
+// LDS and down hops read LDS; up hops read whole neighbour rows (coalesced).
+template <int BS>
+__device__ __forceinline__ double hv_rows_tile(const BInst &I, const BTile &tl,
+                                               const double *__restrict__ v,
+                                               const double *__restrict__ vp, double *__restrict__ w,
+                                               double bp, double *lds_row) {
+  const HvTables &t = I.t;
+  const int64_t D = I.D, Ddn = t.d_dn;
+  const int64_t u0 = tl.e0 / Ddn;
+  const int n = tl.n;
+  double *lr = lds_row, *li = lds_row + n;
+  for (int j = threadIdx.x; j < n; j += BS) {
+    lr[j] = v[tl.e0 + j];
+    li[j] = v[D + tl.e0 + j];
+  }
+  __syncthreads();
+  const double h = t.mags[0];
+  double s = 0.0;
+  for (int j = threadIdx.x; j < n; j += BS) {
+    const int64_t e = tl.e0 + j;
+    const int64_t r = j / Ddn, d = j - r * Ddn, u = u0 + r;
+    double dg = t.diagU[popc(t.s_up[u] & t.s_dn[d])];
+    if (t.has_onsite) dg = dg + t.up_on[u] + t.dn_on[d];
+    double ur = 0.0, ui = 0.0;
+    for (int32_t k = t.up_ptr[u]; k < t.up_ptr[u + 1]; ++k) {
+      const int64_t f = (int64_t)t.up_tgt[k] * Ddn + d;
+      const double a = t.up_amp[k];
+      ur += a * v[f];
+      ui += a * v[D + f];
+    }
+    double ar = dg * lr[j], ai = dg * li[j];
+    const int cnt = t.dn_cnt[d];
+    const uint16_t *ct = t.dn_comp + d * 32;
+    const int64_t rb = r * Ddn;
+    for (int k = 0; k < cnt; ++k) {
+      const uint32_t c = ct[k];
+      const double a = (c & 1u) ? -h : h;
+      const int64_t q = rb + (c >> 1);
+      ar += a * lr[q];
+      ai += a * li[q];
+    }
+    double re = ur + ar, im = ui + ai;
+    if (bp != 0.0) {
+      re -= bp * vp[e];
+      im -= bp * vp[D + e];
+    }
+    w[e] = re;
+    w[D + e] = im;
+    s += lr[j] * re + li[j] * im;
+  }
+  return s;
+}
+
+// w = H v - beta[step-1] v_prev ; partial[tile] = sum v . w
+template <int BS>
+__global__ __launch_bounds__(BS) void bl_hv_kernel(const BInst *__restrict__ inst,
+                                                   const BTile *__restrict__ tiles,
+                                                   const double *__restrict__ V,
+                                                   const double *__restrict__ VP,
+                                                   double *__restrict__ Wv, const double *beta,
+                                                   const int *done, int k, int step,
+                                                   double *partial) {
+  __shared__ double lds[BS / kWave];
+  extern __shared__ double lds_row[];
+  const BTile tl = tiles[blockIdx.x];
+  if (done[tl.inst]) {
+    if (threadIdx.x == 0) partial[blockIdx.x] = 0.0;
+    return;
+  }
+  const BInst I = inst[tl.inst];
+  const double bp = step > 0 ? beta[(int64_t)tl.inst * k + step - 1] : 0.0;
+  const double *v = V + I.off, *vp = VP + I.off;
+  double *w = Wv + I.off;
+  const int64_t D = I.D;
+  double s = 0.0;
+  if (I.rows) {
+    s = hv_rows_tile<BS>(I, tl, v, vp, w, bp, lds_row);
+    s = block_sum<BS>(s, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    return;
+  }
+  for (int64_t e = tl.e0 + threadIdx.x; e < tl.e0 + tl.n; e += BS) {
+    double re, im;
+    hv_pair(I.t, v, D, e, re, im);
+    if (bp != 0.0) {
+      re -= bp * vp[e];
+      im -= bp * vp[D + e];
+    }
+    w[e] = re;
+    w[D + e] = im;
+    s += v[e] * re + v[D + e] * im;
+  }
+  s = block_sum<BS>(s, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// mode 0: w -= alpha v, partial = |w|^2 ; mode 1: w /= beta ; mode 2: partial = |w|^2 ;
+// mode 3: v = w / bnorm (start vector), vp = 0
+template <int BS>
+__global__ __launch_bounds__(BS) void bl_vec_kernel(const BInst *__restrict__ inst,
+                                                    const BTile *__restrict__ tiles,
+                                                    double *__restrict__ V, double *__restrict__ VP,
+                                                    double *__restrict__ Wv, const double *alpha,
+                                                    const double *beta, const double *bnorm,
+                                                    const int *done, int k, int step, int mode,
+                                                    double *partial) {
+  __shared__ double lds[BS / kWave];
+  const BTile tl = tiles[blockIdx.x];
+  if (done[tl.inst] && mode != 3) {
+    if (threadIdx.x == 0 && partial) partial[blockIdx.x] = 0.0;
+    return;
+  }
+  const BInst I = inst[tl.inst];
+  const int64_t D = I.D;
+  double *v = V + I.off, *vp = VP + I.off, *w = Wv + I.off;
+  double s = 0.0;
+  const double a = mode == 0 ? alpha[(int64_t)tl.inst * k + step] : 0.0;
+  const double b = mode == 1 ? beta[(int64_t)tl.inst * k + step] : (mode == 3 ? bnorm[tl.inst] : 1.0);
+  for (int64_t e = tl.e0 + threadIdx.x; e < tl.e0 + tl.n; e += BS) {
+    for (int p = 0; p < 2; ++p) {
+      const int64_t i = p * D + e;
+      if (mode == 0) {
+        const double q = w[i] - a * v[i];
+        w[i] = q;
+        s += q * q;
+      } else if (mode == 1) {
+        w[i] = w[i] / b;
+      } else if (mode == 2) {
+        s += w[i] * w[i];
+      } else {
+        v[i] = w[i] / b;
+        vp[i] = 0.0;
+      }
+    }
+  }
+  if (mode == 0 || mode == 2) {
+    s = block_sum<BS>(s, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
+// One wave per instance: sum of its tiles' partials in tile order.
+// what 0: alpha[i][step] = sum, steps[i] = step + 1
+// what 1: beta[i][step] = sqrt(sum); breakdown -> done[i] = 1
+// what 2: bnorm[i] = sqrt(sum); zero-norm start -> done[i] = 2
+// what 3: reset done and steps
+__global__ __launch_bounds__(kWave) void bl_reduce_kernel(const BInst *__restrict__ inst, int n,
+                                                          const double *partial, double *alpha,
+                                                          double *beta, double *bnorm, int *done,
+                                                          int *steps, int k, int step, int what) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  if (what == 3) {
+    if (threadIdx.x == 0) {
+      done[i] = 0;
+      steps[i] = 0;
+    }
+    return;
+  }
+  if (done[i] && what != 2) return;
+  const BInst I = inst[i];
+  double s = 0.0;
+  for (int j = I.tile0 + threadIdx.x; j < I.tile1; j += kWave) s += partial[j];
+  for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_down(s, off, kWave);
+  if (threadIdx.x == 0) {
+    if (what == 0) {
+      alpha[(int64_t)i * k + step] = s;
+      steps[i] = step + 1;
+    } else if (what == 1) {
+      const double b = sqrt(s);
+      beta[(int64_t)i * k + step] = b;
+      if (b <= kTol) done[i] = 1;
+    } else {
+      const double b = sqrt(s);
+      bnorm[i] = b;
+      if (b <= kTol) done[i] = 2;
+    }
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ host engine
+
+Batch::Batch(std::vector<const Hubbard *> models) : models_(std::move(models)) {
+  int64_t off = 0;
+  for (const Hubbard *h : models_) {
+    BInst I{};
+    I.t = tables_of(*h);
+    I.D = h->dim();
+    I.off = off;
+    off += 2 * I.D;
+    host_inst_.push_back(I);
+  }
+  total_ = off;
+  const size_t bytes = (size_t)std::max<int64_t>(total_, 1) * sizeof(double);
+  V_.alloc(bytes);
+  VP_.alloc(bytes);
+  W_.alloc(bytes);
+  B_.alloc(bytes);
+  d_inst_.alloc(sizeof(BInst) * std::max<size_t>(host_inst_.size(), 1));
+  done_.alloc(sizeof(int) * std::max<size_t>(host_inst_.size(), 1));
+  steps_.alloc(sizeof(int) * std::max<size_t>(host_inst_.size(), 1));
+  bnorm_.alloc(sizeof(double) * std::max<size_t>(host_inst_.size(), 1));
+  set_mapping(1024, 256);
+}
+
+int64_t Batch::footprint_bytes() const {
+  return 4 * total_ * (int64_t)sizeof(double);
+}
+
+void Batch::set_mapping(int tile, int block) {
+  if (block != 128 && block != 256 && block != 512) throw std::invalid_argument("block must be 128, 256 or 512");
+  if (tile < block || tile % block) throw std::invalid_argument("tile must be a multiple of block");
+  tile_ = tile;
+  block_ = block;
+  tiles_.clear();
+  lds_bytes_ = 0;
+  for (size_t i = 0; i < host_inst_.size(); ++i) {
+    BInst &I = host_inst_[i];
+    I.tile0 = (int)tiles_.size();
+    const int64_t Ddn = I.t.d_dn;
+    // row tiles when the compact down table exists and a row is short
+    I.rows = (I.t.comp_ok && Ddn <= kMaxRowsTileDn) ? 1 : 0;
+    const int64_t step = I.rows ? std::max<int64_t>(1, tile / Ddn) * Ddn : tile;
+    for (int64_t e = 0; e < I.D; e += step) {
+      BTile tl;
+      tl.inst = (int)i;
+      tl.e0 = e;
+      tl.n = (int)std::min<int64_t>(step, I.D - e);
+      tiles_.push_back(tl);
+      if (I.rows) lds_bytes_ = std::max<size_t>(lds_bytes_, (size_t)tl.n * 2 * sizeof(double));
+    }
+    I.tile1 = (int)tiles_.size();
+  }
+  d_tiles_ = upload(tiles_);
+  MB_HIP(hipMemcpy(d_inst_.p, host_inst_.data(), sizeof(BInst) * host_inst_.size(),
+                   hipMemcpyHostToDevice));
+  partial_.alloc(sizeof(double) * std::max<size_t>(tiles_.size(), 1));
+  graph_k_ = -1;  // invalidate a captured graph
+  if (graph_exec_) {
+    (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+  }
+}
+
+Batch::~Batch() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+}
+
+template <int BS>
+static void launch_hv_bs(const Batch &b, const double *V, const double *VP, double *W, int k,
+                         int step, hipStream_t s) {
+  static bool attr = false;
+  if (!attr) {
+    MB_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&bl_hv_kernel<BS>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
+    attr = true;
+  }
+  hipLaunchKernelGGL((bl_hv_kernel<BS>), dim3((unsigned)b.tiles_.size()), dim3(BS), b.lds_bytes_, s,
+                     b.d_inst_.as<BInst>(), b.d_tiles_.as<BTile>(), V, VP, W,
+                     b.beta_.as<double>(), b.done_.as<int>(), k, step, b.partial_.as<double>());
+}
+
+template <int BS>
+static void launch_vec_bs(const Batch &b, double *V, double *VP, double *W, int k, int step,
+                          int mode, hipStream_t s) {
+  hipLaunchKernelGGL((bl_vec_kernel<BS>), dim3((unsigned)b.tiles_.size()), dim3(BS), 0, s,
+                     b.d_inst_.as<BInst>(), b.d_tiles_.as<BTile>(), V, VP, W,
+                     b.alpha_.as<double>(), b.beta_.as<double>(), b.bnorm_.as<double>(),
+                     b.done_.as<int>(), k, step, mode, b.partial_.as<double>());
+}
+
+void Batch::hv_step(const double *V, const double *VP, double *W, int k, int step,
+                    hipStream_t s) const {
+  switch (block_) {
+    case 128: launch_hv_bs<128>(*this, V, VP, W, k, step, s); break;
+    case 256: launch_hv_bs<256>(*this, V, VP, W, k, step, s); break;
+    default: launch_hv_bs<512>(*this, V, VP, W, k, step, s); break;
+  }
+}
+
+void Batch::vec_step(double *V, double *VP, double *W, int k, int step, int mode,
+                     hipStream_t s) const {
+  switch (block_) {
+    case 128: launch_vec_bs<128>(*this, V, VP, W, k, step, mode, s); break;
+    case 256: launch_vec_bs<256>(*this, V, VP, W, k, step, mode, s); break;
+    default: launch_vec_bs<512>(*this, V, VP, W, k, step, mode, s); break;
+  }
+}
+
+void Batch::reduce(int k, int step, int what, hipStream_t s) const {
+  const int n = (int)host_inst_.size();
+  hipLaunchKernelGGL(bl_reduce_kernel, dim3((unsigned)std::max(n, 1)), dim3(kWave), 0, s,
+                     d_inst_.as<BInst>(), n, partial_.as<double>(), alpha_.as<double>(),
+                     beta_.as<double>(), bnorm_.as<double>(), done_.as<int>(), steps_.as<int>(), k,
+                     step, what);
+}
+
+void Batch::ensure_k(int k) {
+  if (k > kcap_) {
+    const size_t n = std::max<size_t>(host_inst_.size(), 1);
+    alpha_.alloc(sizeof(double) * n * k);
+    beta_.alloc(sizeof(double) * n * k);
+    kcap_ = k;
+    graph_k_ = -1;
+    if (graph_exec_) {
+      (void)hipGraphExecDestroy(graph_exec_);
+      graph_exec_ = nullptr;
+    }
+  }
+}
+
+// Start vectors (in B_) -> normalised V, zero VP, reset flags; then k steps.
+void Batch::record_run(int k, hipStream_t s) {
+  const int n = (int)host_inst_.size();
+  (void)n;
+  reduce(kcap_, 0, 3, s);  // reset done / steps
+  vec_step(nullptr, nullptr, B_.as<double>(), kcap_, 0, 2, s);  // |b|^2 partials
+  reduce(kcap_, 0, 2, s);                                      // bnorm, zero-norm flag
+  vec_step(V_.as<double>(), VP_.as<double>(), B_.as<double>(), kcap_, 0, 3, s);
+  double *v = V_.as<double>(), *vp = VP_.as<double>(), *w = W_.as<double>();
+  for (int i = 0; i < k; ++i) {
+    hv_step(v, vp, w, kcap_, i, s);
+    reduce(kcap_, i, 0, s);
+    vec_step(v, nullptr, w, kcap_, i, 0, s);
+    reduce(kcap_, i, 1, s);
+    vec_step(nullptr, nullptr, w, kcap_, i, 1, s);
+    double *tmp = vp;
+    vp = v;
+    v = w;
+    w = tmp;
+  }
+  MB_HIP(hipGetLastError());
+}
+
+void Batch::run(int k, bool use_graph, hipStream_t s) {
+  if (k <= 0) throw std::invalid_argument("k must be positive");
+  ensure_k(k);
+  if (!use_graph) {
+    record_run(k, s);
+    return;
+  }
+  if (graph_k_ != k || !graph_exec_) {
+    if (graph_exec_) {
+      (void)hipGraphExecDestroy(graph_exec_);
+      graph_exec_ = nullptr;
+    }
+    hipStream_t cs;
+    MB_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t g;
+    MB_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    record_run(k, cs);
+    MB_HIP(hipStreamEndCapture(cs, &g));
+    MB_HIP(hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(cs);
+    graph_k_ = k;
+  }
+  MB_HIP(hipGraphLaunch(graph_exec_, s));
+}
+
+void Batch::results(int k, std::vector<double> &alphas, std::vector<double> &betas,
+                    std::vector<int> &steps, std::vector<double> &bnorms, hipStream_t s) const {
+  const size_t n = host_inst_.size();
+  std::vector<double> a(n * kcap_), b(n * kcap_);
+  steps.assign(n, 0);
+  bnorms.assign(n, 0.0);
+  std::vector<int> done(n);
+  MB_HIP(hipMemcpyAsync(a.data(), alpha_.p, sizeof(double) * n * kcap_, hipMemcpyDeviceToHost, s));
+  MB_HIP(hipMemcpyAsync(b.data(), beta_.p, sizeof(double) * n * kcap_, hipMemcpyDeviceToHost, s));
+  MB_HIP(hipMemcpyAsync(steps.data(), steps_.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  MB_HIP(hipMemcpyAsync(bnorms.data(), bnorm_.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  MB_HIP(hipStreamSynchronize(s));
+  alphas.assign(n * (size_t)k, 0.0);
+  betas.assign(n * (size_t)k, 0.0);
+  for (size_t i = 0; i < n; ++i)
+    for (int j = 0; j < k; ++j) {
+      alphas[i * k + j] = a[i * kcap_ + j];
+      betas[i * k + j] = (j + 1 < steps[i]) ? b[i * kcap_ + j] : 0.0;
+    }
+}
+
+std::vector<double> Batch::ground_energies(int k, bool use_graph, hipStream_t s) {
+  run(k, use_graph, s);
+  std::vector<double> al, be, bn;
+  std::vector<int> st;
+  results(k, al, be, st, bn, s);
+  const size_t n = host_inst_.size();
+  std::vector<double> e0(n, 0.0);
+  for (size_t i = 0; i < n; ++i) {
+    if (bn[i] <= kTol) throw std::runtime_error("Input vector has zero norm");
+    const int m = st[i];
+    if (m == 0) continue;
+    e0[i] = tridiag_min_eig(m, &al[i * k], &be[i * k]);
+  }
+  return e0;
+}
+
+void Batch::set_start(size_t i, const void *b_dev, hipStream_t s) {
+  if (i >= host_inst_.size()) throw RangeError("instance index out of range");
+  const BInst &I = host_inst_[i];
+  MB_HIP(hipMemcpyAsync(B_.as<double>() + I.off, b_dev, sizeof(double) * 2 * I.D,
+                        hipMemcpyDeviceToDevice, s));
+}
+
+void Batch::fill_gaussian(const uint64_t *seeds, hipStream_t s) {
+  for (size_t i = 0; i < host_inst_.size(); ++i)
+    launch_fill_gaussian_c128(perm_of(*models_[i]->sec), seeds[i], B_.as<double>() + host_inst_[i].off,
+                              s);
+}
+
+double Batch::time_steps(int k, hipStream_t s) {
+  hipEvent_t a, b;
+  MB_HIP(hipEventCreate(&a));
+  MB_HIP(hipEventCreate(&b));
+  ensure_k(k);
+  record_run(1, s);  // warm
+  MB_HIP(hipEventRecord(a, s));
+  record_run(k, s);
+  MB_HIP(hipEventRecord(b, s));
+  MB_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  MB_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms;
+}
+
+std::pair<int, int> Batch::autotune(int k_probe, hipStream_t s) {
+  const int tiles[] = {256, 512, 1024, 2048, 4096};
+  const int blocks[] = {128, 256, 512};
+  double best = 1e300;
+  std::pair<int, int> arg{tile_, block_};
+  for (int tl : tiles)
+    for (int bs : blocks) {
+      if (tl < bs || tl % bs) continue;
+      set_mapping(tl, bs);
+      const double ms = time_steps(k_probe, s);
+      if (ms < best) {
+        best = ms;
+        arg = {tl, bs};
+      }
+    }
+  set_mapping(arg.first, arg.second);
+  return arg;
+}
+
+}  // namespace mb

@@ -1,0 +1,76 @@
+// Batched Lanczos over many Hubbard instances (see batch.hip).
+#pragma once
+
+#include <utility>
+#include <vector>
+
+#include "krylov.hpp"
+#include "mb_internal.hpp"
+
+namespace mb {
+
+// One instance of the batch: its tables, dimension, offset of its planar
+// vector in the concatenated batch buffers, and its range of tiles.
+struct BInst {
+  HvTables t;
+  int64_t off;  // in doubles; plane 0 at off, plane 1 at off + D
+  int64_t D;
+  int tile0, tile1;
+  int rows;     // 1: tiles are whole rows, down hops read the rows from LDS
+};
+
+// Longest row (D_dn) handled by row tiles: a tile of R rows needs 16 R D_dn
+// bytes of LDS, at most 64 KB.
+constexpr int64_t kMaxRowsTileDn = 2048;
+
+struct BTile {
+  int inst;
+  int n;
+  int64_t e0;
+};
+
+class Batch {
+ public:
+  explicit Batch(std::vector<const Hubbard *> models);
+  ~Batch();
+  Batch(const Batch &) = delete;
+  Batch &operator=(const Batch &) = delete;
+
+  size_t size() const { return host_inst_.size(); }
+  int64_t footprint_bytes() const;
+  void set_mapping(int tile, int block);
+  std::pair<int, int> mapping() const { return {tile_, block_}; }
+  void fill_gaussian(const uint64_t *seeds, hipStream_t s);
+  void set_start(size_t i, const void *b_dev, hipStream_t s);
+  // k Lanczos steps for every instance from the start vectors; optionally as one hipGraph.
+  void run(int k, bool use_graph, hipStream_t s);
+  void results(int k, std::vector<double> &alphas, std::vector<double> &betas,
+               std::vector<int> &steps, std::vector<double> &bnorms, hipStream_t s) const;
+  std::vector<double> ground_energies(int k, bool use_graph, hipStream_t s);
+  // time k steps (ms) for the current mapping
+  double time_steps(int k, hipStream_t s);
+  // try tile x block candidates on k_probe steps; keep and return the fastest
+  std::pair<int, int> autotune(int k_probe, hipStream_t s);
+
+  // launch helpers (used by the templates in batch.hip)
+  void hv_step(const double *V, const double *VP, double *W, int k, int step, hipStream_t s) const;
+  void vec_step(double *V, double *VP, double *W, int k, int step, int mode, hipStream_t s) const;
+  void reduce(int k, int step, int what, hipStream_t s) const;
+
+  std::vector<const Hubbard *> models_;
+  std::vector<BInst> host_inst_;
+  std::vector<BTile> tiles_;
+  DevBuf d_inst_, d_tiles_, V_, VP_, W_, B_, partial_, alpha_, beta_, done_, steps_, bnorm_;
+  int64_t total_ = 0;
+  size_t lds_bytes_ = 0;
+  int tile_ = 1024, block_ = 256;
+  int kcap_ = 0;
+  int graph_k_ = -1;
+  hipGraphExec_t graph_exec_ = nullptr;
+
+ private:
+  void ensure_k(int k);
+  void record_run(int k, hipStream_t s);
+};
+
+}  // namespace mb

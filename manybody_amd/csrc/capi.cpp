@@ -5,6 +5,7 @@
 #include <memory>
 
 #include "../../include/manybody_hip.h"
+#include "batch.hpp"
 #include "krylov.hpp"
 #include "mb_internal.hpp"
 
@@ -14,6 +15,9 @@ struct mb_basis {
 struct mb_hubbard {
   mb::Hubbard h;
   const mb_basis *basis;
+};
+struct mb_batch {
+  std::unique_ptr<mb::Batch> b;
 };
 
 namespace {
@@ -389,6 +393,78 @@ int mb_rescaling_from_bounds(double e_min, double e_max, double padding, double 
     // rescaling.h:33-38
     *a = (e_max - e_min) / 2.0 * (1.0 + padding);
     *b = (e_max + e_min) / 2.0;
+  });
+}
+
+int mb_batch_create(int n, const mb_hubbard *const *models, mb_batch **out) {
+  return guarded([&] {
+    need(out && models && n > 0, "invalid batch");
+    *out = nullptr;
+    std::vector<const mb::Hubbard *> v;
+    for (int i = 0; i < n; ++i) v.push_back(&H(models[i]));
+    auto b = std::make_unique<mb_batch>();
+    b->b = std::make_unique<mb::Batch>(std::move(v));
+    *out = b.release();
+  });
+}
+
+void mb_batch_destroy(mb_batch *b) { delete b; }
+
+int64_t mb_batch_footprint(const mb_batch *b) { return b ? b->b->footprint_bytes() : -1; }
+
+int mb_batch_fill_gaussian(mb_batch *b, const uint64_t *seeds, mb_stream stream) {
+  return guarded([&] {
+    need(b && seeds, "null argument");
+    b->b->fill_gaussian(seeds, S(stream));
+  });
+}
+
+int mb_batch_set_start(mb_batch *b, int i, const void *b_dev, mb_stream stream) {
+  return guarded([&] {
+    need(b && b_dev, "null argument");
+    if (i < 0) throw mb::RangeError("instance index out of range");
+    b->b->set_start((size_t)i, b_dev, S(stream));
+  });
+}
+
+int mb_batch_set_mapping(mb_batch *b, int tile, int block) {
+  return guarded([&] {
+    need(b != nullptr, "null argument");
+    b->b->set_mapping(tile, block);
+  });
+}
+
+int mb_batch_autotune(mb_batch *b, int k_probe, int *tile, int *block, mb_stream stream) {
+  return guarded([&] {
+    need(b && k_probe > 0, "invalid argument");
+    auto m = b->b->autotune(k_probe, S(stream));
+    if (tile) *tile = m.first;
+    if (block) *block = m.second;
+  });
+}
+
+int mb_batch_lanczos(mb_batch *b, int k, int use_graph, double *alphas, double *betas,
+                     int *steps, double *b_norms, mb_stream stream) {
+  return guarded([&] {
+    need(b && alphas && betas && steps && b_norms && k > 0, "invalid argument");
+    b->b->run(k, use_graph != 0, S(stream));
+    std::vector<double> al, be, bn;
+    std::vector<int> st;
+    b->b->results(k, al, be, st, bn, S(stream));
+    for (size_t i = 0; i < bn.size(); ++i)
+      if (bn[i] <= mb::kTol) throw std::runtime_error("Input vector has zero norm");
+    std::copy(al.begin(), al.end(), alphas);
+    std::copy(be.begin(), be.end(), betas);
+    std::copy(st.begin(), st.end(), steps);
+    std::copy(bn.begin(), bn.end(), b_norms);
+  });
+}
+
+int mb_batch_ground_energies(mb_batch *b, int k, int use_graph, double *e0, mb_stream stream) {
+  return guarded([&] {
+    need(b && e0 && k > 0, "invalid argument");
+    auto v = b->b->ground_energies(k, use_graph != 0, S(stream));
+    std::copy(v.begin(), v.end(), e0);
   });
 }
 

@@ -92,6 +92,44 @@ void tridiag_eig(int m, const double *a, const double *b, double *w, double *V) 
   }
 }
 
+// Lowest eigenvalue of a symmetric tridiagonal matrix by Sturm-sequence
+// bisection (Golub & Van Loan sec. 8.4.1): O(m) per bisection step, to the
+// last bit; used when only the value is needed.
+double tridiag_min_eig(int m, const double *a, const double *b) {
+  if (m <= 0) return 0.0;
+  double lo = a[0], hi = a[0];
+  for (int i = 0; i < m; ++i) {
+    double r = 0.0;
+    if (i > 0) r += std::fabs(b[i - 1]);
+    if (i + 1 < m) r += std::fabs(b[i]);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  const double pad = 1e-12 * (std::fabs(lo) + std::fabs(hi) + 1.0);
+  lo -= pad;
+  hi += pad;
+  auto count_below = [&](double x) {
+    int c = 0;
+    double q = a[0] - x;
+    if (q < 0) ++c;
+    for (int i = 1; i < m; ++i) {
+      if (q == 0.0) q = 1e-300;
+      q = a[i] - x - b[i - 1] * b[i - 1] / q;
+      if (q < 0) ++c;
+    }
+    return c;
+  };
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (count_below(mid) >= 1)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
 // ------------------------------------------------------------ Lanczos
 
 LanczosWork::LanczosWork(int64_t D, int k, int64_t nparts)
@@ -204,9 +242,10 @@ double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipSt
     MB_HIP(hipStreamSynchronize(s));
     return 0.0;
   }
+  if (!x) return tridiag_min_eig(m, dec.alphas.data(), dec.betas.data());
   std::vector<double> wv(m), V((size_t)m * m);
-  tridiag_eig(m, dec.alphas.data(), dec.betas.data(), wv.data(), x ? V.data() : nullptr);
-  if (x) {
+  tridiag_eig(m, dec.alphas.data(), dec.betas.data(), wv.data(), V.data());
+  {
     std::vector<std::complex<double>> y(m);
     for (int i = 0; i < m; ++i) y[i] = V[i] * dec.b_norm;  // column 0: lowest
     lanczos_pass_two(h, b, dec, y, x, s, &work);

@@ -31,6 +31,7 @@ LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream
 void lanczos_pass_two(const Hubbard &h, const void *b, const LanczosResult &dec,
                       const std::vector<std::complex<double>> &y, void *x, hipStream_t s,
                       LanczosWork *work = nullptr);
+double tridiag_min_eig(int m, const double *a, const double *b);
 double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipStream_t s);
 std::vector<std::complex<double>> expm_first_column(const LanczosResult &dec, double t);
 void time_evolve(const Hubbard &h, const void *psi, double t, int krylov_steps, void *out,
