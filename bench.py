@@ -37,11 +37,23 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-n", type=int, default=512)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="time a plain device copy of one full vector (2 D doubles) instead; "
+                         "run under rocprofv3 --pmc to calibrate FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--traffic", nargs=4, metavar=("HV_FETCH", "HV_WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="(no GPU) turn four rocprofv3 --pmc output folders into the traffic "
+                         "record read by the roofline object, then exit")
     args = ap.parse_args()
+    if args.traffic:
+        write_traffic(*args.traffic)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
               "multi-GPU path (--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.steps, args.warmup)
+        return
 
     import numpy as np
     import torch
@@ -121,6 +133,76 @@ def main():
         "batched": batched,
     }
     print(json.dumps(line))
+
+
+D_4X4_7P7 = 11440 * 11440
+
+
+def copy_calibration(steps, warmup):
+    """Plain device copy of 2 D doubles (one complex128 vector of the 4x4 7+7
+    sector, 2.09 GB): the known-size kernel that calibrates the HBM counters."""
+    import torch
+    torch.cuda.set_device(0)
+    n = 2 * D_4X4_7P7
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(steps):
+        y.copy_(x)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "ms": round(ms, 5),
+                      "GB/s": round(16 * n / (ms * 1e-3) / 1e9, 1)}))
+
+
+def _pmc_per_kernel(folder, counter):
+    """Mean counter value (KiB per dispatch) per kernel name from a rocprofv3
+    --pmc --output-format csv folder."""
+    import csv
+    import glob
+    vals = {}
+    for path in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row["Counter_Name"] == counter:
+                    vals.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in vals.items()}
+
+
+def write_traffic(hv_fetch, hv_write, copy_fetch, copy_write):
+    """HBM bytes per H.v launch pair (strip kernel + row kernel) from FETCH_SIZE
+    and WRITE_SIZE, each scaled by the factor that makes the same counter read
+    the known byte count of a plain copy of one vector."""
+    known = 16.0 * D_4X4_7P7  # bytes read (and written) by the calibration copy
+
+    def copy_kernel(d):
+        big = {k: v for k, v in d.items() if "opy" in k or "elementwise" in k}
+        return max(big.values())
+
+    f_copy = copy_kernel(_pmc_per_kernel(copy_fetch, "FETCH_SIZE")) * 1024.0
+    w_copy = copy_kernel(_pmc_per_kernel(copy_write, "WRITE_SIZE")) * 1024.0
+    ff, wf = known / f_copy, known / w_copy
+    fetch = _pmc_per_kernel(hv_fetch, "FETCH_SIZE")
+    write = _pmc_per_kernel(hv_write, "WRITE_SIZE")
+    kernels = {}
+    for name in fetch:
+        for short in ("up_strip_kernel", "row_kernel_compact", "row_kernel_ell"):
+            if short + "<" in name:
+                kernels[short] = {"fetch_bytes": round(fetch[name] * 1024.0 * ff),
+                                  "write_bytes": round(write.get(name, 0.0) * 1024.0 * wf)}
+    total = sum(v["fetch_bytes"] + v["write_bytes"] for v in kernels.values())
+    out = {"workload": "hubbard2d_4x4_pbc_t1_U8_7up7dn_hv_c128", "D": D_4X4_7P7,
+           "bytes_per_launch": round(total), "algorithmic_bytes": 32 * D_4X4_7P7,
+           "fetch_factor": round(ff, 4), "write_factor": round(wf, 4), "kernels": kernels}
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
 
 
 def bench_batched(n):

@@ -102,11 +102,11 @@ __device__ __forceinline__ double hv_rows_tile(const BInst &I, const BTile &tl,
     double ar = dg * lr[j], ai = dg * li[j];
     const int cnt = t.dn_cnt[d];
     const uint16_t *ct = t.dn_comp + d * 32;
+    const uint32_t sm = t.dn_csign[d];
     const int64_t rb = r * Ddn;
     for (int k = 0; k < cnt; ++k) {
-      const uint32_t c = ct[k];
-      const double a = (c & 1u) ? -h : h;
-      const int64_t q = rb + (c >> 1);
+      const double a = ((sm >> k) & 1u) ? -h : h;
+      const int64_t q = rb + ct[k];
       ar += a * lr[q];
       ai += a * li[q];
     }

@@ -112,15 +112,28 @@ void pack_ell(SpeciesHops &h) {
   }
   if (h.mags.empty()) h.mags.push_back(0.0);
   h.ell_ok = true;
-  // compact copy: one magnitude, targets below 2^15, at most 32 entries per string
-  h.comp_ok = h.mags.size() == 1 && n < (1u << 15) && h.max_row <= 32;
+  // compact copy: one magnitude, at most 32 entries per string, indices up to
+  // 65535 (the zero slot n included).  Strings padded to a multiple of 64 (one
+  // wave); unused slots point at the zero slot n; bit k of sign[d] is the sign
+  // of slot k; gmax[g] is the largest hop count among strings 64 g .. 64 g + 63.
+  h.comp_ok = h.mags.size() == 1 && n <= 65535 && h.max_row <= 32;
   h.comp.clear();
+  h.csign.clear();
+  h.cgmax.clear();
   if (h.comp_ok) {
-    h.comp.assign(n * 32, 0);
-    for (size_t d = 0; d < n; ++d)
-      for (int k = h.row_ptr[d]; k < h.row_ptr[d + 1]; ++k)
-        h.comp[d * 32 + (k - h.row_ptr[d])] =
-            (uint16_t)(((uint32_t)h.tgt[k] << 1) | (h.amp[k] < 0 ? 1u : 0u));
+    const size_t np = (n + 63) / 64 * 64;
+    h.comp.assign(np * 32, (uint16_t)n);
+    h.csign.assign(np, 0u);
+    h.cgmax.assign(np / 64, 0);
+    for (size_t d = 0; d < n; ++d) {
+      const int c = h.row_ptr[d + 1] - h.row_ptr[d];
+      for (int k = 0; k < c; ++k) {
+        const int e = h.row_ptr[d] + k;
+        h.comp[d * 32 + k] = (uint16_t)h.tgt[e];
+        if (h.amp[e] < 0) h.csign[d] |= 1u << k;
+      }
+      h.cgmax[d / 64] = std::max<uint8_t>(h.cgmax[d / 64], (uint8_t)c);
+    }
   }
 }
 
@@ -139,7 +152,11 @@ void upload_species(SpeciesHops &h) {
     std::vector<float> mf(h.mags.begin(), h.mags.end());
     h.d_magsf = upload(mf);
   }
-  if (h.comp_ok) h.d_comp = upload(h.comp);
+  if (h.comp_ok) {
+    h.d_comp = upload(h.comp);
+    h.d_csign = upload(h.csign);
+    h.d_cgmax = upload(h.cgmax);
+  }
 }
 
 }  // namespace
@@ -201,6 +218,8 @@ HvTables tables_of(const Hubbard &h) {
   t.magsf = h.dn.d_magsf.as<float>();
   t.comp_ok = h.dn.comp_ok ? 1 : 0;
   t.dn_comp = h.dn.d_comp.as<uint16_t>();
+  t.dn_csign = h.dn.d_csign.as<uint32_t>();
+  t.dn_cgmax = h.dn.d_cgmax.as<uint8_t>();
   t.max_up_row = h.up.max_row;
   return t;
 }

@@ -228,12 +228,24 @@ __device__ __forceinline__ void row_reduce(const Partial2 &acc2, const EpiArgs &
   }
 }
 
-// Compact down table: 32 uint16 entries per string, target << 1 | sign, one
-// amplitude magnitude; the entries of a string are fetched as four 16-byte loads.
+// Flips the sign of v when sbit (0 or the sign-bit mask) is set.
+__device__ __forceinline__ double flip_sign(double v, uint32_t sbit) {
+  return __hiloint2double(__double2hiint(v) ^ (int)sbit, __double2loint(v));
+}
+__device__ __forceinline__ float flip_sign(float v, uint32_t sbit) {
+  return __int_as_float(__float_as_int(v) ^ (int)sbit);
+}
+
+// Compact down table, one amplitude magnitude h: 32 uint16 target indices per
+// string (fetched as four 16-byte loads), unused slots pointing at a zero slot
+// row[D_dn]; bit k of a per-string mask is the sign of slot k; per group of 64
+// strings (one wave) the largest hop count, so every lane runs the same number
+// of slots (rounded up to 8) with no per-lane predicate.  The 8 LDS reads of a
+// group are issued together; signs are applied by flipping the sign bit.
 template <typename T, Epi E>
 __global__ __launch_bounds__(1024) void row_kernel_compact(HvTables t, const T *__restrict__ x,
                                                            T *__restrict__ y, EpiArgs a) {
-  constexpr int K = kCompactK;
+  constexpr int K = kCompactK, G = 8;
   extern __shared__ double smem[];
   __shared__ double red[2][1024 / kWave];
   T *row = reinterpret_cast<T *>(smem);
@@ -244,31 +256,52 @@ __global__ __launch_bounds__(1024) void row_kernel_compact(HvTables t, const T *
   const int64_t Ddn = t.d_dn, Dup = t.d_up;
   const int64_t u = wg % Dup;
   const int64_t base = wg * Ddn;
+  if (threadIdx.x == 0) row[Ddn] = T(0);
   stage_row(row, x, base, Ddn);
   const uint32_t su = t.s_up[u];
   const T *__restrict__ diag = Tabs<T>::diag(t);
   const T h = Tabs<T>::mags(t)[0];
   const T bp = beta_prev<T, E>(a);
   const T on_u = t.has_onsite ? Tabs<T>::up_on(t)[u] : T(0);
+  const int lane = threadIdx.x & (kWave - 1);
   Partial2 acc2;
-  for (int64_t d = threadIdx.x; d < Ddn; d += blockDim.x) {
+  for (int64_t d0 = (int64_t)threadIdx.x - lane; d0 < Ddn; d0 += blockDim.x) {
+    const int64_t d = d0 + lane;  // tables are padded to a multiple of 64 strings
+    const bool ok = d < Ddn;
     const uint4 *tp = reinterpret_cast<const uint4 *>(t.dn_comp + d * K);
     uint4 q[K / 8];
 #pragma unroll
     for (int j = 0; j < K / 8; ++j) q[j] = tp[j];
-    const int cnt = t.dn_cnt[d];
-    const T xv = row[d];
-    T dg = diag[popc(su & t.s_dn[d])];
-    if (t.has_onsite) dg = dg + on_u + Tabs<T>::dn_on(t)[d];
-    T acc = dg * xv;
+    const uint32_t sm = t.dn_csign[d];
+    const int wm = __builtin_amdgcn_readfirstlane((int)t.dn_cgmax[d0 / kWave]);
+    const T yin = ok ? y[base + d] : T(0);
+    const T xv = row[ok ? d : Ddn];
+    T dg = diag[popc(su & t.s_dn[ok ? d : 0])];
+    if (t.has_onsite) dg = dg + on_u + Tabs<T>::dn_on(t)[ok ? d : 0];
+    T h0 = T(0), h1 = T(0);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint32_t w = reinterpret_cast<const uint32_t *>(&q[k / 8])[(k % 8) / 2];
-      const uint32_t e = (k & 1) ? (w >> 16) : (w & 0xFFFFu);
-      if (k < cnt) acc += ((e & 1u) ? -h : h) * row[e >> 1];
+    for (int g = 0; g < K / G; ++g) {
+      if (g * G < wm) {
+        T v[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const int k = g * G + j;
+          const uint32_t w = reinterpret_cast<const uint32_t *>(&q[k / 8])[(k % 8) / 2];
+          v[j] = row[(k & 1) ? (w >> 16) : (w & 0xFFFFu)];
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const int k = g * G + j;
+          const T fv = flip_sign(v[j], (sm << (31 - k)) & 0x80000000u);
+          if (j & 1) h1 += fv; else h0 += fv;
+        }
+      }
     }
-    const T r = y[base + d] + acc;  // (H_up x) from kernel 1, then diagonal and down hops
-    epilogue<T, E>(r, xv, base + d, y, a, bp, acc2);
+    if (ok) {
+      // (H_up x) from kernel 1, then the diagonal and the down hops
+      const T r = yin + (dg * xv + h * (h0 + h1));
+      epilogue<T, E>(r, xv, base + d, y, a, bp, acc2);
+    }
   }
   row_reduce<T, E>(acc2, a, wg, red);
 }
@@ -602,7 +635,7 @@ bool force_generic() {
 
 template <typename T>
 bool use_two_pass(const HvTables &t) {
-  return t.ell_ok && t.max_up_row <= 64 && t.d_dn * (int64_t)sizeof(T) <= kMaxRowLds &&
+  return t.ell_ok && t.max_up_row <= 64 && (t.d_dn + 1) * (int64_t)sizeof(T) <= kMaxRowLds &&
          !force_generic();
 }
 
@@ -615,7 +648,7 @@ int row_block(int64_t Ddn) {
 
 template <typename T, Epi E>
 void launch_row(const HvTables &t, int P, const T *x, T *y, const EpiArgs &a, hipStream_t s) {
-  const size_t lds = (size_t)t.d_dn * sizeof(T);
+  const size_t lds = (size_t)(t.d_dn + 1) * sizeof(T);  // + the zero slot of the compact table
   static bool attr_set = false;  // once per template instance
   const void *kc = reinterpret_cast<const void *>(&row_kernel_compact<T, E>);
   const void *ke = reinterpret_cast<const void *>(&row_kernel_ell<T, E>);

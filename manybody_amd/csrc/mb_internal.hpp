@@ -115,9 +115,13 @@ struct SpeciesHops {
   std::vector<uint8_t> cnt;
   std::vector<double> mags;
   DevBuf d_ell, d_cnt, d_mags, d_magsf;
+  // compact copy (single magnitude): padded target indices, sign masks, and the
+  // largest hop count per group of 64 strings
   bool comp_ok = false;
   std::vector<uint16_t> comp;
-  DevBuf d_comp;
+  std::vector<uint32_t> csign;
+  std::vector<uint8_t> cgmax;
+  DevBuf d_comp, d_csign, d_cgmax;
 };
 
 struct Hubbard {
@@ -149,9 +153,13 @@ struct HvTables {
   const uint8_t *dn_cnt;
   const double *mags;
   const float *magsf;
-  // compact down table: 32 uint16 entries per string (target << 1 | sign), one magnitude
+  // compact down table, one magnitude: 32 uint16 target indices per string
+  // (unused slots = d_dn, a zero slot), per-string sign masks, per-64-string
+  // largest hop count; padded to a multiple of 64 strings
   int comp_ok;
   const uint16_t *dn_comp;
+  const uint32_t *dn_csign;
+  const uint8_t *dn_cgmax;
   int max_up_row;
 };
 HvTables tables_of(const Hubbard &h);
