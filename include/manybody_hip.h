@@ -221,6 +221,61 @@ int mb_batch_lanczos(mb_batch *b, int k, int use_graph, double *alphas, double *
  * the host), as find_min_eigenpair's value (lanczos.h:283-290). */
 int mb_batch_ground_energies(mb_batch *b, int k, int use_graph, double *e0, mb_stream stream);
 
+/* ------------------------------------------------------------ serving loop */
+
+/* Continuous admission of mixed requests (no reference counterpart: the
+ * reference solves one model at a time).  `slots` requests run concurrently,
+ * each on its own stream; a request is admitted when it has arrived, a slot is
+ * free and its device footprint fits in the free part of the server's arena
+ * (allocated once at creation).  All of a request's state stays on the device
+ * until it finishes.  Results do not depend on what runs beside a request. */
+#define MB_REQ_GROUND 0 /* FP64 Lanczos, `steps` steps, lowest Ritz value (find_min_eigenpair) */
+#define MB_REQ_KPM 1    /* FP64 Lanczos bounds (`steps`), then `moments` FP32 Chebyshev moments */
+#define MB_REQ_EVOLVE 2 /* exp(-iHt) psi0 for t = 0 .. t_final in steps dt, Krylov dim `steps` */
+
+typedef struct mb_request {
+  int kind;             /* MB_REQ_* */
+  int ndim;             /* 1, 2 or 3: periodic chain / square / cubic lattice */
+  int sizes[3];         /* lattice sizes (first ndim used) */
+  int n_up, n_dn;
+  double t, u;
+  int steps;            /* see MB_REQ_* */
+  int moments, nvec;    /* KPM: even number of moments, random-phase vectors */
+  double t_final, dt;   /* EVOLVE */
+  uint64_t seed;        /* start vector: N(0,1) per reference index (mb_fill_gaussian_c128) */
+  double arrival;       /* seconds after mb_server_create; < 0: on submission */
+} mb_request;
+
+typedef struct mb_result {
+  int status;           /* MB_OK or the request's error code */
+  int slot;
+  int64_t dim, footprint, peak; /* D; admitted device bytes; bytes actually used */
+  double e0;            /* GROUND */
+  double e_min, e_max;  /* KPM: spectral bounds used for the rescaling */
+  double overlap_re, overlap_im, norm; /* EVOLVE: <psi0|psi(t)> / |psi0|^2, |psi(t)| / |psi0| */
+  double arrival, admitted, finished;  /* seconds after mb_server_create */
+  int n_moments;        /* KPM: moments written to the caller's array */
+  char error[256];
+} mb_result;
+
+typedef struct mb_server mb_server;
+/* lane_bytes > 0 reserves that much of the arena for the Lanczos lane: ground-
+ * state requests small enough for it (dimension up to an eighth of a lane
+ * buffer, at most 2^20, and at most 512 steps) join a persistent batch at the
+ * next Lanczos step and share its launches with every other member, instead of
+ * taking a slot; they leave after their last step. */
+int mb_server_create(int slots, int64_t arena_bytes, int64_t lane_bytes, mb_server **out);
+/* Finishes every submitted request, then frees the arena. */
+void mb_server_destroy(mb_server *s);
+int mb_server_submit(mb_server *s, const mb_request *r, int64_t *id);
+/* Blocks until request id has finished; moments (may be NULL) receives up to
+ * moments_cap KPM moments. */
+int mb_server_wait(mb_server *s, int64_t id, mb_result *out, double *moments, int moments_cap);
+int mb_server_drain(mb_server *s);
+double mb_server_now(const mb_server *s);
+/* Device bytes a request is admitted with (-1 if the request is invalid). */
+int64_t mb_request_footprint(const mb_request *r);
+
 #ifdef __cplusplus
 }
 #endif

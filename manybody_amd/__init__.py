@@ -85,6 +85,13 @@ _SIGS = {
     "mb_batch_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
     "mb_batch_lanczos": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "mb_batch_ground_energies": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp]),
+    "mb_server_create": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.POINTER(_vp)]),
+    "mb_server_destroy": (None, [_vp]),
+    "mb_server_submit": (C.c_int, [_vp, _vp, C.POINTER(C.c_int64)]),
+    "mb_server_wait": (C.c_int, [_vp, C.c_int64, _vp, _vp, C.c_int]),
+    "mb_server_drain": (C.c_int, [_vp]),
+    "mb_server_now": (C.c_double, [_vp]),
+    "mb_request_footprint": (C.c_int64, [_vp]),
     "mb_rescaling_from_bounds": (C.c_int, [C.c_double, C.c_double, C.c_double, _dp, _dp]),
 }
 
@@ -484,3 +491,112 @@ class Batch:
         _check(lib().mb_batch_ground_energies(self._h, k, int(use_graph),
                                               e0.ctypes.data_as(C.c_void_p), _stream()))
         return e0
+
+
+# ------------------------------------------------------------------ serving
+
+REQ_GROUND, REQ_KPM, REQ_EVOLVE = 0, 1, 2
+
+
+class _Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("ndim", C.c_int), ("sizes", C.c_int * 3),
+                ("n_up", C.c_int), ("n_dn", C.c_int), ("t", C.c_double), ("u", C.c_double),
+                ("steps", C.c_int), ("moments", C.c_int), ("nvec", C.c_int),
+                ("t_final", C.c_double), ("dt", C.c_double), ("seed", C.c_uint64),
+                ("arrival", C.c_double)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("status", C.c_int), ("slot", C.c_int), ("dim", C.c_int64),
+                ("footprint", C.c_int64), ("peak", C.c_int64), ("e0", C.c_double),
+                ("e_min", C.c_double), ("e_max", C.c_double), ("overlap_re", C.c_double),
+                ("overlap_im", C.c_double), ("norm", C.c_double), ("arrival", C.c_double),
+                ("admitted", C.c_double), ("finished", C.c_double), ("n_moments", C.c_int),
+                ("error", C.c_char * 256)]
+
+
+@dataclass
+class Request:
+    """One serving request.  kind: REQ_GROUND (FP64 Lanczos, `steps` steps),
+    REQ_KPM (`steps` FP64 Lanczos steps for the bounds, then `moments` FP32
+    Chebyshev moments over `nvec` random-phase vectors) or REQ_EVOLVE
+    (exp(-iHt) psi0 up to t_final in steps of dt, Krylov dimension `steps`).
+    sizes: lattice sizes (1, 2 or 3 of them, periodic).  arrival: seconds after
+    the server started (< 0: on submission)."""
+    kind: int
+    sizes: tuple
+    n_up: int
+    n_dn: int
+    t: float = 1.0
+    u: float = 0.0
+    steps: int = 150
+    moments: int = 256
+    nvec: int = 4
+    t_final: float = 0.0
+    dt: float = 0.05
+    seed: int = 0
+    arrival: float = -1.0
+
+    def _c(self) -> _Request:
+        r = _Request()
+        r.kind, r.ndim = self.kind, len(self.sizes)
+        for i in range(3):
+            r.sizes[i] = int(self.sizes[i]) if i < len(self.sizes) else 1
+        r.n_up, r.n_dn, r.t, r.u = self.n_up, self.n_dn, self.t, self.u
+        r.steps, r.moments, r.nvec = self.steps, self.moments, self.nvec
+        r.t_final, r.dt, r.seed, r.arrival = self.t_final, self.dt, self.seed, self.arrival
+        return r
+
+    def footprint(self) -> int:
+        r = self._c()
+        return int(lib().mb_request_footprint(C.byref(r)))
+
+
+class Server:
+    """Serving loop: `slots` concurrent requests, device memory from one arena
+    of `arena_bytes`; requests are admitted as they arrive and as slots and
+    memory free up.  lane_bytes > 0 reserves part of the arena for the Lanczos
+    lane, where small ground-state requests join a persistent batch at step
+    boundaries."""
+
+    def __init__(self, slots: int = 8, arena_bytes: int = 1 << 34, lane_bytes: int = 0):
+        h = C.c_void_p()
+        _check(lib().mb_server_create(slots, int(arena_bytes), int(lane_bytes), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mb_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, req: Request) -> int:
+        r = req._c()
+        i = C.c_int64()
+        _check(lib().mb_server_submit(self._h, C.byref(r), C.byref(i)))
+        return i.value
+
+    def wait(self, rid: int, moments_cap: int = 4096) -> dict:
+        import numpy as np
+        res = _Result()
+        mu = np.zeros(moments_cap)
+        _check(lib().mb_server_wait(self._h, rid, C.byref(res), mu.ctypes.data_as(C.c_void_p),
+                                    moments_cap))
+        out = {f: getattr(res, f) for f, _ in _Result._fields_ if f != "error"}
+        out["error"] = res.error.decode()
+        out["moments"] = mu[:res.n_moments].copy()
+        return out
+
+    def drain(self):
+        _check(lib().mb_server_drain(self._h))
+
+    def now(self) -> float:
+        return float(lib().mb_server_now(self._h))

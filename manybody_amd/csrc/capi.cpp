@@ -6,6 +6,7 @@
 
 #include "../../include/manybody_hip.h"
 #include "batch.hpp"
+#include "serve.hpp"
 #include "krylov.hpp"
 #include "mb_internal.hpp"
 
@@ -18,6 +19,9 @@ struct mb_hubbard {
 };
 struct mb_batch {
   std::unique_ptr<mb::Batch> b;
+};
+struct mb_server {
+  std::unique_ptr<mb::Server> s;
 };
 
 namespace {
@@ -86,31 +90,7 @@ int mb_basis_create(int orbitals, int particles, int spin_projection, mb_basis *
     const int up = (particles + spin_projection) / 2, dn = particles - up;
     need(up <= orbitals && dn <= orbitals, "spin sector exceeds the orbital count");
     auto b = std::make_unique<mb_basis>();
-    mb::Sector &s = b->sec;
-    s.L = orbitals;
-    s.n_up = up;
-    s.n_dn = dn;
-    const mb::Binom &C = mb::binom();
-    s.d_up = (int64_t)C(orbitals, up);
-    s.d_dn = (int64_t)C(orbitals, dn);
-    need(s.d_up * s.d_dn < ((int64_t)1 << 40), "basis too large");
-    need(s.d_up < ((int64_t)1 << 31) && s.d_dn < ((int64_t)1 << 31), "species sector too large");
-    for (int n = 0; n < 2; ++n) {
-      const int k = n == 0 ? up : dn;
-      auto &v = n == 0 ? s.s_up : s.s_dn;
-      v.reserve(n == 0 ? s.d_up : s.d_dn);
-      // ascending bitstrings with popcount k (Gosper's hack)
-      if (k == 0) {
-        v.push_back(0u);
-      } else {
-        uint64_t x = (1ull << k) - 1;
-        while (x < (1ull << orbitals)) {
-          v.push_back((uint32_t)x);
-          const uint64_t c = x & (~x + 1), r = x + c;
-          x = (((r ^ x) >> 2) / c) | r;
-        }
-      }
-    }
+    mb::build_sector(b->sec, orbitals, up, dn);
     *out = b.release();
   });
 }
@@ -188,28 +168,7 @@ int mb_hubbard_create(const mb_basis *b, int ndim, const int *sizes, double t, d
       prod *= sizes[i];
     }
     need(prod == b->sec.L, "lattice size does not match the basis' orbital count");
-    if (ndim == 1) {  // hubbard_model.h:16-20
-      const int L = sizes[0];
-      for (int i = 0; i < L; ++i) { from.push_back(i); to.push_back((i + 1) % L); }
-    } else if (ndim == 2) {  // hubbard_model.h:72-82
-      const int Lx = sizes[0], Ly = sizes[1];
-      for (int x = 0; x < Lx; ++x)
-        for (int y = 0; y < Ly; ++y) {
-          const int s = x + Lx * y;
-          from.push_back(s); to.push_back(((x + 1) % Lx) + Lx * y);
-          from.push_back(s); to.push_back(x + Lx * ((y + 1) % Ly));
-        }
-    } else {  // hubbard_model.h:144-159
-      const int Lx = sizes[0], Ly = sizes[1], Lz = sizes[2];
-      for (int x = 0; x < Lx; ++x)
-        for (int y = 0; y < Ly; ++y)
-          for (int z = 0; z < Lz; ++z) {
-            const int s = x + Lx * y + Lx * Ly * z;
-            from.push_back(s); to.push_back(((x + 1) % Lx) + Lx * y + Lx * Ly * z);
-            from.push_back(s); to.push_back(x + Lx * ((y + 1) % Ly) + Lx * Ly * z);
-            from.push_back(s); to.push_back(x + Lx * y + Lx * Ly * ((z + 1) % Lz));
-          }
-    }
+    mb::lattice_bonds(ndim, sizes, from, to);
   });
   if (st != MB_OK) return st;
   return hubbard_common(b, std::move(from), std::move(to), t, u, out);
@@ -469,3 +428,92 @@ int mb_batch_ground_energies(mb_batch *b, int k, int use_graph, double *e0, mb_s
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ serving loop
+
+static mb::ServeRequest to_internal(const mb_request &r) {
+  mb::ServeRequest q;
+  q.kind = r.kind;
+  q.ndim = r.ndim;
+  for (int i = 0; i < 3; ++i) q.sizes[i] = r.sizes[i];
+  q.n_up = r.n_up;
+  q.n_dn = r.n_dn;
+  q.t = r.t;
+  q.u = r.u;
+  q.steps = r.steps;
+  q.moments = r.moments;
+  q.nvec = r.nvec;
+  q.t_final = r.t_final;
+  q.dt = r.dt;
+  q.seed = r.seed;
+  q.arrival = r.arrival;
+  return q;
+}
+
+int mb_server_create(int slots, int64_t arena_bytes, int64_t lane_bytes, mb_server **out) {
+  return guarded([&] {
+    need(out != nullptr, "null output");
+    *out = nullptr;
+    need(lane_bytes >= 0 && lane_bytes < arena_bytes, "lane_bytes must be in [0, arena_bytes)");
+    auto s = std::make_unique<mb_server>();
+    s->s = std::make_unique<mb::Server>(slots, arena_bytes, lane_bytes);
+    *out = s.release();
+  });
+}
+
+void mb_server_destroy(mb_server *s) { delete s; }
+
+int mb_server_submit(mb_server *s, const mb_request *r, int64_t *id) {
+  return guarded([&] {
+    need(s && r && id, "null argument");
+    need(r->kind >= MB_REQ_GROUND && r->kind <= MB_REQ_EVOLVE, "unknown request kind");
+    need(r->steps >= 1, "steps must be positive");
+    if (r->kind == MB_REQ_KPM) need(r->nvec >= 1 && r->nvec <= 64, "nvec must be in [1, 64]");
+    if (r->kind == MB_REQ_EVOLVE) need(r->t_final >= 0.0 && r->dt > 0.0, "need t_final >= 0, dt > 0");
+    *id = s->s->submit(to_internal(*r));
+  });
+}
+
+int mb_server_wait(mb_server *s, int64_t id, mb_result *out, double *moments, int moments_cap) {
+  return guarded([&] {
+    need(s && out, "null argument");
+    const mb::ServeResult &r = s->s->wait(id);
+    std::memset(out, 0, sizeof(*out));
+    out->status = r.status;
+    out->slot = r.slot;
+    out->dim = r.dim;
+    out->footprint = r.footprint;
+    out->peak = r.peak;
+    out->e0 = r.e0;
+    out->e_min = r.e_min;
+    out->e_max = r.e_max;
+    out->overlap_re = r.ov_re;
+    out->overlap_im = r.ov_im;
+    out->norm = r.norm;
+    out->arrival = r.arrival;
+    out->admitted = r.admitted;
+    out->finished = r.finished;
+    const int n = moments ? std::min<int>(moments_cap, (int)r.mu.size()) : 0;
+    for (int i = 0; i < n; ++i) moments[i] = r.mu[i];
+    out->n_moments = n;
+    std::snprintf(out->error, sizeof(out->error), "%s", r.error.c_str());
+  });
+}
+
+int mb_server_drain(mb_server *s) {
+  return guarded([&] {
+    need(s != nullptr, "null argument");
+    s->s->drain();
+  });
+}
+
+double mb_server_now(const mb_server *s) { return s ? s->s->now() : -1.0; }
+
+int64_t mb_request_footprint(const mb_request *r) {
+  if (!r) return -1;
+  try {
+    return mb::request_footprint(to_internal(*r));
+  } catch (...) {
+    return -1;
+  }
+}

@@ -161,6 +161,35 @@ void upload_species(SpeciesHops &h) {
 
 }  // namespace
 
+void lattice_bonds(int ndim, const int *sizes, std::vector<int> &from, std::vector<int> &to) {
+  from.clear();
+  to.clear();
+  if (ndim == 1) {  // hubbard_model.h:16-20
+    const int L = sizes[0];
+    for (int i = 0; i < L; ++i) { from.push_back(i); to.push_back((i + 1) % L); }
+  } else if (ndim == 2) {  // hubbard_model.h:72-82: x outer, y inner; x-bond then y-bond
+    const int Lx = sizes[0], Ly = sizes[1];
+    for (int x = 0; x < Lx; ++x)
+      for (int y = 0; y < Ly; ++y) {
+        const int s = x + Lx * y;
+        from.push_back(s); to.push_back(((x + 1) % Lx) + Lx * y);
+        from.push_back(s); to.push_back(x + Lx * ((y + 1) % Ly));
+      }
+  } else if (ndim == 3) {  // hubbard_model.h:144-159
+    const int Lx = sizes[0], Ly = sizes[1], Lz = sizes[2];
+    for (int x = 0; x < Lx; ++x)
+      for (int y = 0; y < Ly; ++y)
+        for (int z = 0; z < Lz; ++z) {
+          const int s = x + Lx * y + Lx * Ly * z;
+          from.push_back(s); to.push_back(((x + 1) % Lx) + Lx * y + Lx * Ly * z);
+          from.push_back(s); to.push_back(x + Lx * ((y + 1) % Ly) + Lx * Ly * z);
+          from.push_back(s); to.push_back(x + Lx * y + Lx * Ly * ((z + 1) % Lz));
+        }
+  } else {
+    throw std::invalid_argument("ndim must be 1, 2 or 3");
+  }
+}
+
 void build_hubbard(Hubbard &h) {
   const Sector &s = *h.sec;
   const auto mono = merge_monomials(h.bond_from, h.bond_to, h.t);

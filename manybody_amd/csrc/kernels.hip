@@ -523,6 +523,25 @@ __global__ __launch_bounds__(kBlock) void sumsq_partial_kernel(const T *__restri
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// Planar complex dot <a|b> = sum conj(a) b: partial[blk] = Re, partial[nb + blk] = Im.
+__global__ __launch_bounds__(kBlock) void cdot_partial_kernel(const double *__restrict__ a,
+                                                              const double *__restrict__ b,
+                                                              int64_t D, double *partial) {
+  __shared__ double lds[kBlock / kWave];
+  double re = 0.0, im = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < D; i += (int64_t)gridDim.x * kBlock) {
+    const double ar = a[i], ai = a[D + i], br = b[i], bi = b[D + i];
+    re += ar * br + ai * bi;
+    im += ar * bi - ai * br;
+  }
+  re = wg_sum(re, lds);
+  im = wg_sum(im, lds);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = re;
+    partial[gridDim.x + blockIdx.x] = im;
+  }
+}
+
 __device__ double final_sum(const double *partial, int64_t n, double *lds) {
   double s = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += kBlock) s += partial[i];
@@ -765,6 +784,16 @@ void launch_sumsq(const void *x, bool fp32, int64_t n, double *partial, double *
                        (const double *)x, n, partial);
   hipLaunchKernelGGL(final_kernel, dim3(1), dim3(kBlock), 0, s, (const double *)partial,
                      (int64_t)kRedBlocks, out, take_sqrt ? 1 : 0);
+  MB_HIP(hipGetLastError());
+}
+
+void launch_cdot(const double *a, const double *b, int64_t D, double *partial, double *out,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(cdot_partial_kernel, dim3(kRedBlocks), dim3(kBlock), 0, s, a, b, D, partial);
+  hipLaunchKernelGGL(final_kernel, dim3(1), dim3(kBlock), 0, s, (const double *)partial,
+                     (int64_t)kRedBlocks, out, 0);
+  hipLaunchKernelGGL(final_kernel, dim3(1), dim3(kBlock), 0, s, (const double *)partial + kRedBlocks,
+                     (int64_t)kRedBlocks, out + 1, 0);
   MB_HIP(hipGetLastError());
 }
 

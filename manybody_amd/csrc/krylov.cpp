@@ -97,6 +97,7 @@ void tridiag_eig(int m, const double *a, const double *b, double *w, double *V) 
 // last bit; used when only the value is needed.
 double tridiag_min_eig(int m, const double *a, const double *b) {
   if (m <= 0) return 0.0;
+  if (m == 1) return a[0];
   double lo = a[0], hi = a[0];
   for (int i = 0; i < m; ++i) {
     double r = 0.0;

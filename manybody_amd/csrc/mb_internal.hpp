@@ -39,17 +39,43 @@ struct NoMemError : std::runtime_error {
       throw ::mb::DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_));         \
   } while (0)
 
-// Owning device buffer (hipMalloc / hipFree).
+// Source of device memory other than hipMalloc (the server's per-request
+// regions).  A thread that sets thread_dev_alloc() gets every DevBuf it
+// allocates from that source until it resets it; the buffer remembers its
+// owner and returns the memory there.
+struct DevAlloc {
+  virtual void *alloc(size_t n) = 0;  // nullptr when the source is exhausted
+  virtual void free(void *p) = 0;
+  virtual ~DevAlloc() = default;
+};
+DevAlloc *&thread_dev_alloc();
+// Stream used by upload() on this thread (null: the null stream).
+hipStream_t &thread_upload_stream();
+
+// Owning device buffer (hipMalloc / hipFree, or the thread's DevAlloc).
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  DevAlloc *owner = nullptr;
   DevBuf() = default;
   explicit DevBuf(size_t n) { alloc(n); }
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
-  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), owner(o.owner) {
+    o.p = nullptr;
+    o.bytes = 0;
+    o.owner = nullptr;
+  }
   DevBuf &operator=(DevBuf &&o) noexcept {
-    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      owner = o.owner;
+      o.p = nullptr;
+      o.bytes = 0;
+      o.owner = nullptr;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
@@ -58,10 +84,16 @@ struct DevBuf {
   template <typename T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Host vector -> new device buffer, copied on the thread's upload stream and
+// complete on return.
 template <typename T>
 DevBuf upload(const std::vector<T> &v) {
   DevBuf b(v.size() * sizeof(T) + 16);
-  if (!v.empty()) MB_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (!v.empty()) {
+    hipStream_t s = thread_upload_stream();
+    MB_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    MB_HIP(hipStreamSynchronize(s));
+  }
   return b;
 }
 
@@ -96,6 +128,8 @@ struct Sector {
   void ref_state(int64_t index, uint32_t *up, uint32_t *dn) const;
   void ensure_device();
 };
+// Fills the species strings of the (L, n_up, n_dn) sector (host only).
+void build_sector(Sector &s, int L, int n_up, int n_dn);
 
 // ---------------------------------------------------------------- model
 // Per-species hop table: CSR rows by string rank; each entry the target rank
@@ -137,6 +171,9 @@ struct Hubbard {
 };
 
 void build_hubbard(Hubbard &h);
+// Bond list of the periodic hypercubic lattice in the reference's loop order
+// (HubbardModel / HubbardModel2D / HubbardModel3D).
+void lattice_bonds(int ndim, const int *sizes, std::vector<int> &from, std::vector<int> &to);
 
 // ------------------------------------------------------------ kernels
 // Views passed to kernels (plain pointers).
@@ -217,6 +254,10 @@ struct LanczosState {
   double *bnorm;     // [1]
 };
 void launch_norm(const double *x, int64_t n, double *partial, double *out, hipStream_t s);
+// out[0], out[1] = Re, Im of <a|b> for planar complex vectors of D elements;
+// partial holds 2 kRedBlocks doubles
+void launch_cdot(const double *a, const double *b, int64_t D, double *partial, double *out,
+                 hipStream_t s);
 void launch_scale_div(const double *x, const double *den, double *y, int64_t n, hipStream_t s);
 void launch_lanczos_finish_alpha(const LanczosState &st, int step, int64_t nparts, hipStream_t s);
 void launch_lanczos_axpy_norm(const double *v, double *w, int64_t n, const LanczosState &st,

@@ -15,10 +15,24 @@
 
 namespace mb {
 
+DevAlloc *&thread_dev_alloc() {
+  static thread_local DevAlloc *a = nullptr;
+  return a;
+}
+
+hipStream_t &thread_upload_stream() {
+  static thread_local hipStream_t s = nullptr;
+  return s;
+}
+
 void DevBuf::alloc(size_t n) {
   release();
   if (n == 0) n = 1;
-  if (hipMalloc(&p, n) != hipSuccess) {
+  if (DevAlloc *a = thread_dev_alloc()) {
+    p = a->alloc(n);
+    if (!p) throw NoMemError("request region exhausted (" + std::to_string(n) + " bytes)");
+    owner = a;
+  } else if (hipMalloc(&p, n) != hipSuccess) {
     p = nullptr;
     (void)hipGetLastError();
     throw NoMemError("hipMalloc of " + std::to_string(n) + " bytes failed");
@@ -27,9 +41,15 @@ void DevBuf::alloc(size_t n) {
 }
 
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p) {
+    if (owner)
+      owner->free(p);
+    else
+      (void)hipFree(p);
+  }
   p = nullptr;
   bytes = 0;
+  owner = nullptr;
 }
 
 Binom::Binom() {
@@ -38,6 +58,38 @@ Binom::Binom() {
   for (int n = 0; n <= kMaxSites; ++n) {
     c[n][0] = 1;
     for (int k = 1; k <= n; ++k) c[n][k] = c[n - 1][k - 1] + (k <= n - 1 ? c[n - 1][k] : 0);
+  }
+}
+
+void build_sector(Sector &s, int L, int n_up, int n_dn) {
+  if (L < 1 || L > kMaxSites) throw std::invalid_argument("orbitals must be in [1, 32]");
+  if (n_up < 0 || n_dn < 0 || n_up > L || n_dn > L)
+    throw std::invalid_argument("spin sector exceeds the orbital count");
+  s.L = L;
+  s.n_up = n_up;
+  s.n_dn = n_dn;
+  const Binom &C = binom();
+  s.d_up = (int64_t)C(L, n_up);
+  s.d_dn = (int64_t)C(L, n_dn);
+  if (s.d_up * s.d_dn >= ((int64_t)1 << 40)) throw std::invalid_argument("basis too large");
+  if (s.d_up >= ((int64_t)1 << 31) || s.d_dn >= ((int64_t)1 << 31))
+    throw std::invalid_argument("species sector too large");
+  for (int n = 0; n < 2; ++n) {
+    const int k = n == 0 ? n_up : n_dn;
+    auto &v = n == 0 ? s.s_up : s.s_dn;
+    v.clear();
+    v.reserve(n == 0 ? s.d_up : s.d_dn);
+    // ascending bitstrings with popcount k (Gosper's hack)
+    if (k == 0) {
+      v.push_back(0u);
+    } else {
+      uint64_t x = (1ull << k) - 1;
+      while (x < (1ull << L)) {
+        v.push_back((uint32_t)x);
+        const uint64_t c = x & (~x + 1), r = x + c;
+        x = (((r ^ x) >> 2) / c) | r;
+      }
+    }
   }
 }
 
