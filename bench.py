@@ -37,6 +37,13 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-n", type=int, default=512)
+    ap.add_argument("--no-lanczos", action="store_true")
+    ap.add_argument("--c2-cpu-energy", action="store_true",
+                    help="also run the oracle's 300-step OpenMP Lanczos on the host (minutes)")
+    ap.add_argument("--no-kpm", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-n", type=int, default=200)
+    ap.add_argument("--serving-rate", type=float, default=40.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain device copy of one full vector (2 D doubles) instead; "
                          "run under rocprofv3 --pmc to calibrate FETCH_SIZE / WRITE_SIZE")
@@ -109,7 +116,14 @@ def main():
                          f"matrix-free restatement, {tc * 1e3:.0f} ms each"}
         del xh
 
+    lanczos = None
+    if not args.no_lanczos:
+        lanczos = bench_lanczos_c2(op, x, basis, (Lx, Ly, nu, nd, t, U), args.c2_cpu_energy,
+                                   args.cpu_threads)
+    del y
     batched = None if args.no_batched else bench_batched(args.batched_n)
+    kpm = None if args.no_kpm else bench_kpm_c3()
+    serving = None if args.no_serving else bench_serving(args.serving_n, args.serving_rate)
 
     line = {
         "metric": "Hubbard H·v GB/s (% HBM peak); batched solves/s + p99 latency at |ΔE|<1e-8",
@@ -130,7 +144,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
+        "lanczos_4x4": lanczos,
         "batched": batched,
+        "kpm_l14": kpm,
+        "serving": serving,
     }
     print(json.dumps(line))
 
@@ -203,6 +220,126 @@ def write_traffic(hv_fetch, hv_write, copy_fetch, copy_write):
     with open(TRAFFIC_FILE, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))
+
+
+def bench_lanczos_c2(op, x, basis, model, cpu_energy, cpu_threads, k=300):
+    """300-step FP64 Lanczos on the 4x4 7+7 sector from the bench vector: lowest
+    Ritz value and time per step; optionally the oracle's OpenMP Lanczos on the
+    host from the same vector (minutes) for the energy comparison."""
+    import numpy as np
+    import torch
+
+    import manybody_amd as mb
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dec = mb.lanczos_pass_one(op, x, k)
+    t_gpu = time.perf_counter() - t0
+    w, _ = mb.tridiag_eig(dec.alphas, dec.betas, vectors=False)
+    out = {"steps": dec.steps_taken, "E0": float(w[0]), "s": round(t_gpu, 3),
+           "ms_per_step": round(t_gpu / k * 1e3, 3),
+           "algorithmic_GBps": round(96.0 * op.dimension() * k / t_gpu / 1e9, 1)}
+    if cpu_energy:
+        import scipy.linalg as sla
+        from oracle import pyoracle as po
+        Lx, Ly, nu, nd, t, U = model
+        threads = cpu_threads or min(16, os.cpu_count() or 1)
+        P = po.ProductOperator([Lx, Ly], nu, nd, t, U, threads=threads)
+        xh = mb.from_planar(x)
+        c0 = time.perf_counter()
+        al, be, _ = P.lanczos(xh, k)
+        t_cpu = time.perf_counter() - c0
+        e_cpu = float(sla.eigvalsh_tridiagonal(np.asarray(al), np.asarray(be))[0])
+        out.update({"E0_cpu_oracle": e_cpu, "cpu_s": round(t_cpu, 1), "cpu_threads": threads,
+                    "rel_dE": abs(out["E0"] - e_cpu) / abs(e_cpu)})
+    return out
+
+
+def bench_kpm_c3(M=1024, R=16, seed=7):
+    """KPM density of states, 1D ring L = 14, 7 + 7, t = 1, U = 4
+    (D = 11,778,624): bounds from a 100-step FP64 Lanczos, spectrum mapped to
+    [-0.99, 0.99], 1024 moments over 16 random-phase vectors, H.v in FP32 and in
+    FP64; the FP32 error against the FP64 run is stated."""
+    import numpy as np
+    import torch
+
+    import manybody_amd as mb
+    L, nu, nd, t, U = 14, 7, 7, 1.0, 4.0
+    op = mb.HubbardModel(t, U, L).operator(mb.FermionBasis.sector(L, nu, nd))
+    D = op.dimension()
+    b = op.gaussian(11)
+    dec = mb.lanczos_pass_one(op, b, 100)
+    w, _ = mb.tridiag_eig(dec.alphas, dec.betas, vectors=False)
+    a, c = mb.rescaling_from_bounds(float(w[0]), float(w[-1]), 1.0 / 0.99 - 1.0)
+    del b
+    res = {}
+    for fp32 in (True, False):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mu = np.array(mb.kpm_dos_moments(op, a, c, M, R, seed, fp32=fp32))
+        res[fp32] = (mu, time.perf_counter() - t0)
+    mu32, t32 = res[True]
+    mu64, t64 = res[False]
+    g = np.array(mb.jackson_kernel(M - 1))
+    xs = np.linspace(-0.99, 0.99, 4001)
+    Tn = np.cos(np.outer(np.arange(M), np.arccos(xs)))
+
+    def dos(mu):
+        c_n = g * mu
+        c_n[1:] *= 2.0
+        return (c_n @ Tn) / (np.pi * np.sqrt(1.0 - xs ** 2))
+
+    r32, r64 = dos(mu32), dos(mu64)
+    l1 = float(np.trapezoid(np.abs(r32 - r64), xs) / np.trapezoid(np.abs(r64), xs))
+    steps = M // 2
+    return {"workload": "KPM DOS, 1D ring L=14 7+7 U=4, 1024 moments x 16 random-phase vectors",
+            "D": D, "bounds": [float(w[0]), float(w[-1])], "a": a, "b": c,
+            "fp32_s": round(t32, 3), "fp64_s": round(t64, 3),
+            "fp32_GBps": round(24.0 * R * D * steps / t32 / 1e9, 1),  # 24 B per complex element
+            "max_abs_dmu_fp32_vs_fp64": float(np.abs(mu32 - mu64).max()),
+            "dos_rel_L1_fp32_vs_fp64": l1, "mu0": float(mu64[0]),
+            "max_abs_mu": float(np.abs(mu64).max())}
+
+
+def bench_serving(n, rate, slots=4, arena=64 << 30, lane=8 << 30):
+    """Poisson request stream (manybody_amd.workloads.serving_requests): 70 %
+    ground state, 20 % KPM DOS (FP32 H.v), 10 % Krylov exp(-iHt) on 1D L = 8..14
+    and 2D 3x3 / 4x3 rings, U ~ U[0, 12].  Served with `slots` stream slots and
+    the Lanczos lane: p50 / p99 latency (arrival -> result) at `rate`; then the
+    same requests all at once (saturated) against one request at a time without
+    the lane."""
+    import numpy as np
+    from dataclasses import replace
+
+    import manybody_amd as mb
+    from manybody_amd.workloads import serving_requests
+    reqs = serving_requests(n, rate=rate, seed=5)
+
+    def run(rs, n_slots, lane_bytes):
+        with mb.Server(n_slots, arena, lane_bytes=lane_bytes) as S:
+            ids = [S.submit(r) for r in rs]
+            out = [S.wait(i) for i in ids]
+        bad = [x for x in out if x["status"] != 0]
+        if bad:
+            raise RuntimeError("served request failed: " + bad[0]["error"])
+        span = max(x["finished"] for x in out) - min(x["arrival"] for x in out)
+        lat = np.array([x["finished"] - x["arrival"] for x in out])
+        return out, span, lat
+
+    out, span, lat = run(reqs, slots, lane)
+    sat = [replace(r, arrival=-1.0) for r in reqs]
+    _, span_sat, _ = run(sat, slots, lane)
+    _, span_one, _ = run(sat, 1, 0)
+    kinds = np.bincount([r.kind for r in reqs], minlength=3)
+    return {"workload": f"{n} requests, Poisson {rate}/s: {kinds[0]} ground, {kinds[1]} KPM, "
+                        f"{kinds[2]} Krylov; 1D L 8-14, 2D 3x3/4x3",
+            "slots": slots, "lane_bytes": lane,
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+            "solves_per_s": round(n / span, 1),
+            "saturated_solves_per_s": round(n / span_sat, 1),
+            "one_at_a_time_solves_per_s": round(n / span_one, 1),
+            "speedup_vs_one_at_a_time": round(span_one / span_sat, 2),
+            "in_lane": int(sum(1 for x in out if x["slot"] >= slots))}
 
 
 def bench_batched(n):

@@ -539,11 +539,25 @@ static void prod_apply(void *ctx, const double *x, double *y) {
                  o->dO, o->diagU, x, y, o->threads);
 }
 
-static double cnorm(int64_t D, const double *x) {
+/* sum of a[i] * b[i]: 256 fixed chunks summed in parallel, then added in chunk
+ * order, so the result does not depend on the thread count or schedule */
+static double dot_fixed(int64_t n, const double *a, const double *b) {
+  enum { NC = 256 };
+  double part[NC];
+  const int64_t len = (n + NC - 1) / NC;
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < NC; ++c) {
+    const int64_t lo = c * len, hi = lo + len < n ? lo + len : n;
+    double s = 0.0;
+    for (int64_t i = lo; i < hi; ++i) s += a[i] * b[i];
+    part[c] = s;
+  }
   double s = 0.0;
-  for (int64_t i = 0; i < 2 * D; ++i) s += x[i] * x[i];
-  return sqrt(s);
+  for (int c = 0; c < NC; ++c) s += part[c];
+  return s;
 }
+
+static double cnorm(int64_t D, const double *x) { return sqrt(dot_fixed(2 * D, x, x)); }
 
 /* lanczos_pass_one (lanczos.h:130-175) with lanczos_recurrence_step (:15-40)
  * and LanczosIteration::next (:104-127). */
@@ -555,20 +569,26 @@ static int lanczos_generic(int64_t D, apply_fn f, void *ctx, const double *b, in
   double *vp = (double *)calloc(2 * (size_t)D, sizeof(double));
   double *v = (double *)malloc(sizeof(double) * 2 * (size_t)D);
   double *w = (double *)malloc(sizeof(double) * 2 * (size_t)D);
+  /* the vector passes use the threads of the last omp_set_num_threads (the
+   * product-order H.v sets them); sums use fixed chunks (dot_fixed) */
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < 2 * D; ++i) v[i] = b[i] / *b_norm;
   double beta_prev = 0.0;
   int steps = 0;
   for (int i = 0; i < k; ++i) {
     f(ctx, v, w);
-    if (beta_prev != 0.0)
+    if (beta_prev != 0.0) {
+#pragma omp parallel for schedule(static)
       for (int64_t e = 0; e < 2 * D; ++e) w[e] -= beta_prev * vp[e];
-    double alpha = 0.0;
-    for (int64_t e = 0; e < 2 * D; ++e) alpha += v[e] * w[e]; /* Re(conj(v) w) */
+    }
+    const double alpha = dot_fixed(2 * D, v, w); /* Re(conj(v) w) */
+#pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < 2 * D; ++e) w[e] -= alpha * v[e];
     const double beta = cnorm(D, w);
     alphas[steps++] = alpha;
     if (beta <= ORC_TOL) break;
     if (i + 1 < k) betas[i] = beta;
+#pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < 2 * D; ++e) w[e] /= beta;
     double *tmp = vp;
     vp = v;
