@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <thread>
 
 namespace mb {
 
@@ -27,15 +28,18 @@ namespace {
 
 using namespace bdev;
 
-// w = H v - beta[step-1] v_prev ; partial[tile] = sum v . w
+// v = sigma X (sigma = 1 / |b| at step 0, 1 / beta[step-1] after: the previous
+// step's normalisation, applied here instead of in a pass of its own), written
+// to V; w = H v - beta[step-1] v_prev ; partial[tile] = sum v . w
 template <int BS>
 __global__ __launch_bounds__(BS) void bl_hv_kernel(const BInst *__restrict__ inst,
                                                    const BTile *__restrict__ tiles,
-                                                   const double *__restrict__ V,
+                                                   const double *__restrict__ X,
+                                                   double *__restrict__ V,
                                                    const double *__restrict__ VP,
                                                    double *__restrict__ Wv, const double *beta,
-                                                   const int *done, int k, int step,
-                                                   double *partial) {
+                                                   const double *bnorm, const int *done, int k,
+                                                   int step, double *partial) {
   __shared__ double lds[BS / kWave];
   extern __shared__ double lds_row[];
   const BTile tl = tiles[blockIdx.x];
@@ -45,26 +49,32 @@ __global__ __launch_bounds__(BS) void bl_hv_kernel(const BInst *__restrict__ ins
   }
   const BInst I = inst[tl.inst];
   const double bp = step > 0 ? beta[(int64_t)tl.inst * k + step - 1] : 0.0;
-  const double *v = V + I.off, *vp = VP + I.off;
-  double *w = Wv + I.off;
+  const double sigma = 1.0 / (step > 0 ? bp : bnorm[tl.inst]);
+  const double *x = X + I.off, *vp = VP + I.off;
+  double *v = V + I.off, *w = Wv + I.off;
   const int64_t D = I.D;
   double s = 0.0;
   if (I.rows) {
-    s = hv_rows_tile<BS>(I, tl, v, vp, w, bp, lds_row);
+    s = hv_rows_tile<BS>(I, tl, x, sigma, v, vp, w, bp, lds_row);
     s = block_sum<BS>(s, lds);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
     return;
   }
   for (int64_t e = tl.e0 + threadIdx.x; e < tl.e0 + tl.n; e += BS) {
     double re, im;
-    hv_pair(I.t, v, D, e, re, im);
+    hv_pair(I.t, x, D, e, re, im);
+    re *= sigma;
+    im *= sigma;
+    const double vr = sigma * x[e], vi = sigma * x[D + e];
+    v[e] = vr;
+    v[D + e] = vi;
     if (bp != 0.0) {
       re -= bp * vp[e];
       im -= bp * vp[D + e];
     }
     w[e] = re;
     w[D + e] = im;
-    s += v[e] * re + v[D + e] * im;
+    s += vr * re + vi * im;
   }
   s = block_sum<BS>(s, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -174,6 +184,7 @@ Batch::Batch(std::vector<const Hubbard *> models) : models_(std::move(models)) {
   VP_.alloc(bytes);
   W_.alloc(bytes);
   B_.alloc(bytes);
+  Q_.alloc(bytes);
   d_inst_.alloc(sizeof(BInst) * std::max<size_t>(host_inst_.size(), 1));
   done_.alloc(sizeof(int) * std::max<size_t>(host_inst_.size(), 1));
   steps_.alloc(sizeof(int) * std::max<size_t>(host_inst_.size(), 1));
@@ -182,7 +193,7 @@ Batch::Batch(std::vector<const Hubbard *> models) : models_(std::move(models)) {
 }
 
 int64_t Batch::footprint_bytes() const {
-  return 4 * total_ * (int64_t)sizeof(double);
+  return 5 * total_ * (int64_t)sizeof(double);
 }
 
 void Batch::set_mapping(int tile, int block) {
@@ -197,15 +208,17 @@ void Batch::set_mapping(int tile, int block) {
     I.tile0 = (int)tiles_.size();
     const int64_t Ddn = I.t.d_dn;
     // row tiles when the compact down table exists and a row is short
-    I.rows = (I.t.comp_ok && Ddn <= kMaxRowsTileDn) ? 1 : 0;
-    const int64_t step = I.rows ? std::max<int64_t>(1, tile / Ddn) * Ddn : tile;
+    const int64_t rows_n = std::max<int64_t>(1, tile / Ddn) * Ddn;
+    I.rows = (I.t.comp_ok && Ddn <= kMaxRowsTileDn &&
+              rows_tile_lds(Ddn, rows_n, I.t.max_up_row) <= kRowsTileLdsMax) ? 1 : 0;
+    const int64_t step = I.rows ? rows_n : tile;
     for (int64_t e = 0; e < I.D; e += step) {
       BTile tl;
       tl.inst = (int)i;
       tl.e0 = e;
       tl.n = (int)std::min<int64_t>(step, I.D - e);
       tiles_.push_back(tl);
-      if (I.rows) lds_bytes_ = std::max<size_t>(lds_bytes_, (size_t)tl.n * 2 * sizeof(double));
+      if (I.rows) lds_bytes_ = std::max<size_t>(lds_bytes_, rows_tile_lds(Ddn, tl.n, I.t.max_up_row));
     }
     I.tile1 = (int)tiles_.size();
   }
@@ -225,8 +238,8 @@ Batch::~Batch() {
 }
 
 template <int BS>
-static void launch_hv_bs(const Batch &b, const double *V, const double *VP, double *W, int k,
-                         int step, hipStream_t s) {
+static void launch_hv_bs(const Batch &b, const double *X, double *V, const double *VP, double *W,
+                         int k, int step, hipStream_t s) {
   static bool attr = false;
   if (!attr) {
     MB_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&bl_hv_kernel<BS>),
@@ -234,8 +247,9 @@ static void launch_hv_bs(const Batch &b, const double *V, const double *VP, doub
     attr = true;
   }
   hipLaunchKernelGGL((bl_hv_kernel<BS>), dim3((unsigned)b.tiles_.size()), dim3(BS), b.lds_bytes_, s,
-                     b.d_inst_.as<BInst>(), b.d_tiles_.as<BTile>(), V, VP, W,
-                     b.beta_.as<double>(), b.done_.as<int>(), k, step, b.partial_.as<double>());
+                     b.d_inst_.as<BInst>(), b.d_tiles_.as<BTile>(), X, V, VP, W,
+                     b.beta_.as<double>(), b.bnorm_.as<double>(), b.done_.as<int>(), k, step,
+                     b.partial_.as<double>());
 }
 
 template <int BS>
@@ -247,12 +261,12 @@ static void launch_vec_bs(const Batch &b, double *V, double *VP, double *W, int 
                      b.done_.as<int>(), k, step, mode, b.partial_.as<double>());
 }
 
-void Batch::hv_step(const double *V, const double *VP, double *W, int k, int step,
+void Batch::hv_step(const double *X, double *V, const double *VP, double *W, int k, int step,
                     hipStream_t s) const {
   switch (block_) {
-    case 128: launch_hv_bs<128>(*this, V, VP, W, k, step, s); break;
-    case 256: launch_hv_bs<256>(*this, V, VP, W, k, step, s); break;
-    default: launch_hv_bs<512>(*this, V, VP, W, k, step, s); break;
+    case 128: launch_hv_bs<128>(*this, X, V, VP, W, k, step, s); break;
+    case 256: launch_hv_bs<256>(*this, X, V, VP, W, k, step, s); break;
+    default: launch_hv_bs<512>(*this, X, V, VP, W, k, step, s); break;
   }
 }
 
@@ -294,18 +308,22 @@ void Batch::record_run(int k, hipStream_t s) {
   reduce(kcap_, 0, 3, s);  // reset done / steps
   vec_step(nullptr, nullptr, B_.as<double>(), kcap_, 0, 2, s);  // |b|^2 partials
   reduce(kcap_, 0, 2, s);                                      // bnorm, zero-norm flag
-  vec_step(V_.as<double>(), VP_.as<double>(), B_.as<double>(), kcap_, 0, 3, s);
-  double *v = V_.as<double>(), *vp = VP_.as<double>(), *w = W_.as<double>();
+  // x: the unnormalised input of the step (b, then the previous w); the step
+  // writes v = x / (|b| or beta) and w.  B_ is never overwritten, so a replay
+  // starts from the same vectors.
+  double *x = B_.as<double>(), *v = V_.as<double>(), *vp = VP_.as<double>(), *w = W_.as<double>();
+  double *spare = Q_.as<double>();
   for (int i = 0; i < k; ++i) {
-    hv_step(v, vp, w, kcap_, i, s);
-    reduce(kcap_, i, 0, s);
-    vec_step(v, nullptr, w, kcap_, i, 0, s);
-    reduce(kcap_, i, 1, s);
-    vec_step(nullptr, nullptr, w, kcap_, i, 1, s);
-    double *tmp = vp;
+    hv_step(x, v, vp, w, kcap_, i, s);
+    reduce(kcap_, i, 0, s);                   // alpha
+    vec_step(v, nullptr, w, kcap_, i, 0, s);  // w -= alpha v, |w|^2
+    reduce(kcap_, i, 1, s);                   // beta, breakdown
+    double *free_buf = i == 0 ? spare : x;
+    double *old_vp = vp;
+    x = w;
     vp = v;
-    v = w;
-    w = tmp;
+    v = old_vp;
+    w = free_buf;
   }
   MB_HIP(hipGetLastError());
 }
@@ -363,13 +381,19 @@ std::vector<double> Batch::ground_energies(int k, bool use_graph, hipStream_t s)
   std::vector<int> st;
   results(k, al, be, st, bn, s);
   const size_t n = host_inst_.size();
-  std::vector<double> e0(n, 0.0);
-  for (size_t i = 0; i < n; ++i) {
+  for (size_t i = 0; i < n; ++i)
     if (bn[i] <= kTol) throw std::runtime_error("Input vector has zero norm");
-    const int m = st[i];
-    if (m == 0) continue;
-    e0[i] = tridiag_min_eig(m, &al[i * k], &be[i * k]);
-  }
+  // the tridiagonal solves are independent: split them over host threads
+  std::vector<double> e0(n, 0.0);
+  const size_t nt = std::min<size_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
+  auto work = [&](size_t t0) {
+    for (size_t i = t0; i < n; i += nt)
+      if (st[i] > 0) e0[i] = tridiag_min_eig(st[i], &al[i * k], &be[i * k]);
+  };
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < nt && t < n; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto &th : pool) th.join();
   return e0;
 }
 

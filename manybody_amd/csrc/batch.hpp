@@ -19,9 +19,16 @@ struct BInst {
   int rows;     // 1: tiles are whole rows, down hops read the rows from LDS
 };
 
-// Longest row (D_dn) handled by row tiles: a tile of R rows needs 16 R D_dn
-// bytes of LDS, at most 64 KB.
+// Longest row (D_dn) handled by row tiles.
 constexpr int64_t kMaxRowsTileDn = 2048;
+// LDS of a row tile of n elements (whole rows of D_dn): both planes at row
+// stride D_dn + 1, then the up-hop lists of its rows (amplitudes, targets,
+// counts); a row tile may use at most kRowsTileLdsMax bytes.
+constexpr size_t kRowsTileLdsMax = 64 * 1024;
+inline size_t rows_tile_lds(int64_t Ddn, int64_t n, int max_up) {
+  const int64_t R = n / Ddn, S = Ddn + 1;
+  return (size_t)(8 * (2 * R * S + R * max_up) + 4 * (R * max_up + R) + 64);
+}
 
 struct BTile {
   int inst;
@@ -53,14 +60,15 @@ class Batch {
   std::pair<int, int> autotune(int k_probe, hipStream_t s);
 
   // launch helpers (used by the templates in batch.hip)
-  void hv_step(const double *V, const double *VP, double *W, int k, int step, hipStream_t s) const;
+  void hv_step(const double *X, double *V, const double *VP, double *W, int k, int step,
+               hipStream_t s) const;
   void vec_step(double *V, double *VP, double *W, int k, int step, int mode, hipStream_t s) const;
   void reduce(int k, int step, int what, hipStream_t s) const;
 
   std::vector<const Hubbard *> models_;
   std::vector<BInst> host_inst_;
   std::vector<BTile> tiles_;
-  DevBuf d_inst_, d_tiles_, V_, VP_, W_, B_, partial_, alpha_, beta_, done_, steps_, bnorm_;
+  DevBuf d_inst_, d_tiles_, V_, VP_, W_, B_, Q_, partial_, alpha_, beta_, done_, steps_, bnorm_;
   int64_t total_ = 0;
   size_t lds_bytes_ = 0;
   int tile_ = 1024, block_ = 256;

@@ -53,56 +53,128 @@ __device__ __forceinline__ void hv_pair(const HvTables &t, const double *__restr
 }
 
 // Row-tiled variant for instances with the compact down table: the tile is
-// rows [u0, u0 + R) (R * D_dn elements); both planes of those rows are staged in
-// LDS and down hops read LDS; up hops read whole neighbour rows (coalesced).
+// rows [u0, u0 + R) (R * D_dn elements).  LDS holds both planes of those rows
+// (row stride D_dn + 1: a zero slot ends each row, the target of the unused
+// slots of the compact table) and the up-hop lists of the R rows; down hops
+// read LDS in groups of 8 slots up to the wave's largest hop count (no per-lane
+// predicate, signs by flipping the sign bit), up hops gather whole neighbour
+// rows of x from global memory (coalesced along d).  The input is x with
+// v = sigma x (the normalisation of the previous Lanczos step applied on the
+// fly); v of the tile is written to vout when it is given.  Returns the tile's
+// partial v . w for w = H v - bp vp.  LDS layout: rows_tile_lds (batch.hpp).
+__device__ __forceinline__ double flip_sign(double v, uint32_t sbit) {
+  return __hiloint2double(__double2hiint(v) ^ (int)sbit, __double2loint(v));
+}
+
 template <int BS>
 __device__ __forceinline__ double hv_rows_tile(const BInst &I, const BTile &tl,
-                                               const double *__restrict__ v,
+                                               const double *__restrict__ x, double sigma,
+                                               double *__restrict__ vout,
                                                const double *__restrict__ vp, double *__restrict__ w,
                                                double bp, double *lds_row) {
+  constexpr int K = 32, G = 8;
   const HvTables &t = I.t;
-  const int64_t D = I.D, Ddn = t.d_dn;
+  const int64_t D = I.D;
+  const int Ddn = (int)t.d_dn, S = Ddn + 1, MU = t.max_up_row;
   const int64_t u0 = tl.e0 / Ddn;
-  const int n = tl.n;
-  double *lr = lds_row, *li = lds_row + n;
+  const int n = tl.n, R = n / Ddn;
+  double *lr = lds_row, *li = lds_row + R * S, *ua = li + R * S;
+  int *ut = reinterpret_cast<int *>(ua + R * MU), *un = ut + R * MU;
   for (int j = threadIdx.x; j < n; j += BS) {
-    lr[j] = v[tl.e0 + j];
-    li[j] = v[D + tl.e0 + j];
+    const int r = j / Ddn, d = j - r * Ddn;
+    const double a = sigma * x[tl.e0 + j], b = sigma * x[D + tl.e0 + j];
+    lr[r * S + d] = a;
+    li[r * S + d] = b;
+    if (vout) {
+      vout[tl.e0 + j] = a;
+      vout[D + tl.e0 + j] = b;
+    }
+  }
+  for (int r = threadIdx.x; r < R; r += BS) {
+    lr[r * S + Ddn] = 0.0;
+    li[r * S + Ddn] = 0.0;
+    const int64_t u = u0 + r;
+    un[r] = t.up_ptr[u + 1] - t.up_ptr[u];
+  }
+  for (int q = threadIdx.x; q < R * MU; q += BS) {
+    const int r = q / MU, k = q - r * MU;
+    const int64_t u = u0 + r;
+    const int32_t b = t.up_ptr[u];
+    if (k < t.up_ptr[u + 1] - b) {
+      ut[q] = t.up_tgt[b + k];
+      ua[q] = t.up_amp[b + k];
+    }
   }
   __syncthreads();
   const double h = t.mags[0];
+  const int lane = threadIdx.x & (kWave - 1);
   double s = 0.0;
-  for (int j = threadIdx.x; j < n; j += BS) {
-    const int64_t e = tl.e0 + j;
-    const int64_t r = j / Ddn, d = j - r * Ddn, u = u0 + r;
+  for (int j0 = (int)threadIdx.x - lane; j0 < n; j0 += BS) {
+    const int j = j0 + lane;
+    const bool ok = j < n;
+    const int jj = ok ? j : j0;
+    const int r = jj / Ddn, d = jj - r * Ddn;
+    const int64_t u = u0 + r, e = tl.e0 + jj;
+    const uint4 *tp = reinterpret_cast<const uint4 *>(t.dn_comp + (int64_t)d * K);
+    uint4 q[K / 8];
+#pragma unroll
+    for (int c = 0; c < K / 8; ++c) q[c] = tp[c];
+    const uint32_t sm = t.dn_csign[d];
+    int wm = t.dn_cnt[d];
+    for (int off = kWave / 2; off > 0; off >>= 1) wm = max(wm, __shfl_xor(wm, off, kWave));
+    wm = __builtin_amdgcn_readfirstlane(wm);
     double dg = t.diagU[popc(t.s_up[u] & t.s_dn[d])];
     if (t.has_onsite) dg = dg + t.up_on[u] + t.dn_on[d];
+    // up hops: rows of x from global memory
     double ur = 0.0, ui = 0.0;
-    for (int32_t k = t.up_ptr[u]; k < t.up_ptr[u + 1]; ++k) {
-      const int64_t f = (int64_t)t.up_tgt[k] * Ddn + d;
-      const double a = t.up_amp[k];
-      ur += a * v[f];
-      ui += a * v[D + f];
+    const int nu = un[r];
+    for (int k = 0; k < nu; ++k) {
+      const int64_t f = (int64_t)ut[r * MU + k] * Ddn + d;
+      const double a = ua[r * MU + k];
+      ur += a * x[f];
+      ui += a * x[D + f];
     }
-    double ar = dg * lr[j], ai = dg * li[j];
-    const int cnt = t.dn_cnt[d];
-    const uint16_t *ct = t.dn_comp + d * 32;
-    const uint32_t sm = t.dn_csign[d];
-    const int64_t rb = r * Ddn;
-    for (int k = 0; k < cnt; ++k) {
-      const double a = ((sm >> k) & 1u) ? -h : h;
-      const int64_t q = rb + ct[k];
-      ar += a * lr[q];
-      ai += a * li[q];
+    // down hops: LDS
+    const double *rr = lr + r * S, *ri = li + r * S;
+    double hr0 = 0.0, hr1 = 0.0, hi0 = 0.0, hi1 = 0.0;
+#pragma unroll
+    for (int g = 0; g < K / G; ++g) {
+      if (g * G < wm) {
+        double vr[G], vi[G];
+#pragma unroll
+        for (int c = 0; c < G; ++c) {
+          const int k = g * G + c;
+          const uint32_t wd = reinterpret_cast<const uint32_t *>(&q[k / 8])[(k % 8) / 2];
+          const uint32_t idx = (k & 1) ? (wd >> 16) : (wd & 0xFFFFu);
+          vr[c] = rr[idx];
+          vi[c] = ri[idx];
+        }
+#pragma unroll
+        for (int c = 0; c < G; ++c) {
+          const int k = g * G + c;
+          const uint32_t sb = (sm << (31 - k)) & 0x80000000u;
+          if (c & 1) {
+            hr1 += flip_sign(vr[c], sb);
+            hi1 += flip_sign(vi[c], sb);
+          } else {
+            hr0 += flip_sign(vr[c], sb);
+            hi0 += flip_sign(vi[c], sb);
+          }
+        }
+      }
     }
-    double re = ur + ar, im = ui + ai;
-    if (bp != 0.0) {
-      re -= bp * vp[e];
-      im -= bp * vp[D + e];
+    const double xr = rr[d], xi = ri[d];
+    double re = sigma * ur + (dg * xr + h * (hr0 + hr1));
+    double im = sigma * ui + (dg * xi + h * (hi0 + hi1));
+    if (ok) {
+      if (bp != 0.0) {
+        re -= bp * vp[e];
+        im -= bp * vp[D + e];
+      }
+      w[e] = re;
+      w[D + e] = im;
+      s += xr * re + xi * im;
     }
-    w[e] = re;
-    w[D + e] = im;
-    s += lr[j] * re + li[j] * im;
   }
   return s;
 }

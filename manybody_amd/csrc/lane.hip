@@ -3,13 +3,15 @@
 // medium Hubbard models share every launch, and models join or leave between
 // steps without stopping the others.
 //
-// Per tick, for all members at once: (1) w = H v - beta[s-1] v_prev with the
-// per-tile partial v.w, (2) alpha[s] per member, (3) w -= alpha v with |w|^2
-// partials, (4) beta[s] per member, breakdown (1000 eps) or step count reached ->
-// done, next step index, (5) w /= beta.  The three vector buffers rotate roles
-// after each tick for every member together; each member keeps its own step
-// index (double-buffered: a tick reads one copy and writes the other), so a
-// member that joined late simply counts from 0.  Tiles are per member and the
+// Per tick, for all members at once: (1) v = x / (|b| at step 0, beta[s-1]
+// after) written out, w = H v - beta[s-1] v_prev, per-tile partial v.w, (2)
+// alpha[s] per member, (3) w -= alpha v with |w|^2 partials, (4) beta[s] per
+// member, breakdown (1000 eps) or step count reached -> done, next step index.
+// The normalisation w / beta of a step is applied when the next step reads it,
+// so there is no pass of its own.  The four vector buffers (x, v, v_prev, w)
+// rotate roles after each tick for every member together; each member keeps its
+// own step index (double-buffered: a tick reads one copy and writes the other),
+// so a member that joined late simply counts from 0.  Tiles are per member and the
 // per-member sums run over its own tiles in order, so a member's numbers do not
 // depend on who else is in the lane.
 #include "lane.hpp"
@@ -27,9 +29,10 @@ using namespace bdev;
 constexpr int kLaneBlock = 256;
 
 __global__ __launch_bounds__(kLaneBlock) void ln_hv_kernel(
-    const LaneInst *__restrict__ inst, const BTile *__restrict__ tiles, const double *__restrict__ V,
-    const double *__restrict__ VP, double *__restrict__ Wv, const double *beta, const int *step,
-    const int *done, int kmax, double *partial) {
+    const LaneInst *__restrict__ inst, const BTile *__restrict__ tiles, const double *__restrict__ X,
+    double *__restrict__ V, const double *__restrict__ VP, double *__restrict__ Wv,
+    const double *beta, const double *bnorm, const int *step, const int *done, int kmax,
+    double *partial) {
   __shared__ double lds[kLaneBlock / kWave];
   extern __shared__ double lds_row[];
   const BTile tl = tiles[blockIdx.x];
@@ -40,8 +43,9 @@ __global__ __launch_bounds__(kLaneBlock) void ln_hv_kernel(
   }
   const int s = step[I.slot];
   const double bp = s > 0 ? beta[(int64_t)I.slot * kmax + s - 1] : 0.0;
-  const double *v = V + I.off, *vp = VP + I.off;
-  double *w = Wv + I.off;
+  const double sigma = 1.0 / (s > 0 ? bp : bnorm[I.slot]);
+  const double *x = X + I.off, *vp = VP + I.off;
+  double *v = V + I.off, *w = Wv + I.off;
   const int64_t D = I.D;
   double acc = 0.0;
   if (I.rows) {
@@ -52,59 +56,56 @@ __global__ __launch_bounds__(kLaneBlock) void ln_hv_kernel(
     B.tile0 = I.tile0;
     B.tile1 = I.tile1;
     B.rows = 1;
-    acc = hv_rows_tile<kLaneBlock>(B, tl, v, vp, w, bp, lds_row);
+    acc = hv_rows_tile<kLaneBlock>(B, tl, x, sigma, v, vp, w, bp, lds_row);
   } else {
     for (int64_t e = tl.e0 + threadIdx.x; e < tl.e0 + tl.n; e += kLaneBlock) {
       double re, im;
-      hv_pair(I.t, v, D, e, re, im);
+      hv_pair(I.t, x, D, e, re, im);
+      re *= sigma;
+      im *= sigma;
+      const double vr = sigma * x[e], vi = sigma * x[D + e];
+      v[e] = vr;
+      v[D + e] = vi;
       if (bp != 0.0) {
         re -= bp * vp[e];
         im -= bp * vp[D + e];
       }
       w[e] = re;
       w[D + e] = im;
-      acc += v[e] * re + v[D + e] * im;
+      acc += vr * re + vi * im;
     }
   }
   acc = block_sum<kLaneBlock>(acc, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
-// mode 0: w -= alpha v, partial = |w|^2 ; mode 1: w /= beta
-__global__ __launch_bounds__(kLaneBlock) void ln_vec_kernel(
+// w -= alpha v, partial = |w|^2
+__global__ __launch_bounds__(kLaneBlock) void ln_axpy_kernel(
     const LaneInst *__restrict__ inst, const BTile *__restrict__ tiles, const double *__restrict__ V,
-    double *__restrict__ Wv, const double *alpha, const double *beta, const int *step,
-    const int *done, int kmax, int mode, double *partial) {
+    double *__restrict__ Wv, const double *alpha, const int *step, const int *done, int kmax,
+    double *partial) {
   __shared__ double lds[kLaneBlock / kWave];
   const BTile tl = tiles[blockIdx.x];
   const LaneInst I = inst[tl.inst];
   if (done[I.slot]) {
-    if (threadIdx.x == 0 && mode == 0) partial[blockIdx.x] = 0.0;
+    if (threadIdx.x == 0) partial[blockIdx.x] = 0.0;
     return;
   }
   const int64_t D = I.D;
-  const int s = step[I.slot];
   const double *v = V + I.off;
   double *w = Wv + I.off;
-  const double a = alpha[(int64_t)I.slot * kmax + s];
-  const double b = beta[(int64_t)I.slot * kmax + s];
+  const double a = alpha[(int64_t)I.slot * kmax + step[I.slot]];
   double acc = 0.0;
   for (int64_t e = tl.e0 + threadIdx.x; e < tl.e0 + tl.n; e += kLaneBlock) {
     for (int p = 0; p < 2; ++p) {
       const int64_t i = p * D + e;
-      if (mode == 0) {
-        const double q = w[i] - a * v[i];
-        w[i] = q;
-        acc += q * q;
-      } else {
-        w[i] = w[i] / b;
-      }
+      const double q = w[i] - a * v[i];
+      w[i] = q;
+      acc += q * q;
     }
   }
-  if (mode == 0) {
-    acc = block_sum<kLaneBlock>(acc, lds);
-    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
-  }
+  acc = block_sum<kLaneBlock>(acc, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
 // One wave per member: its tiles' partials in tile order.
@@ -135,20 +136,11 @@ __global__ __launch_bounds__(kWave) void ln_reduce_kernel(const LaneInst *__rest
   }
 }
 
-// New member: v = b / |b| in place, v_prev = 0, step = 0 (done = 2 for a zero start).
-__global__ __launch_bounds__(kLaneBlock) void ln_init_kernel(double *v, double *vp, int64_t n,
-                                                             const double *bnorm, int slot,
-                                                             int *step, int *done) {
-  const double nb = bnorm[slot];
-  const bool zero = !(nb > kTol);
-  for (int64_t i = (int64_t)blockIdx.x * kLaneBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kLaneBlock) {
-    v[i] = zero ? 0.0 : v[i] / nb;
-    vp[i] = 0.0;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+// New member: step 0; done = 2 for a zero start vector (its first step divides by |b|).
+__global__ void ln_init_kernel(const double *bnorm, int slot, int *step, int *done) {
+  if (threadIdx.x == 0) {
     step[slot] = 0;
-    done[slot] = zero ? 2 : 0;
+    done[slot] = bnorm[slot] > kTol ? 0 : 2;
   }
 }
 
@@ -158,7 +150,7 @@ int64_t align_up(int64_t n, int64_t a) { return (n + a - 1) / a * a; }
 
 int64_t LaneEngine::bytes_needed(int64_t buf_doubles, int nslots, int kmax, int tile) {
   const int64_t max_tiles = buf_doubles / tile + nslots + 1;
-  return 3 * align_up(buf_doubles * 8, 256) + 3 * align_up((int64_t)nslots * kmax * 8, 256) +
+  return 4 * align_up(buf_doubles * 8, 256) + 3 * align_up((int64_t)nslots * kmax * 8, 256) +
          align_up((int64_t)nslots * 8, 256) + align_up((max_tiles + 2 * kRedBlocks) * 8, 256) +
          3 * align_up((int64_t)nslots * 4, 256) + align_up((int64_t)nslots * sizeof(LaneInst), 256) +
          align_up(max_tiles * (int64_t)sizeof(BTile), 256) + 4096;
@@ -254,13 +246,11 @@ void LaneEngine::slot_give(int slot) { free_slots_.push_back(slot); }
 
 void LaneEngine::init_member(const LaneInst &I, const PermTables &p, uint64_t seed, hipStream_t s) {
   if (I.off < 0 || I.off + 2 * I.D > buf_) throw std::out_of_range("lane vector range");
-  double *v = role_[0] + I.off, *vp = role_[1] + I.off;
-  launch_fill_gaussian_c128(p, seed, v, s);
-  launch_norm(v, 2 * I.D, partial_ + max_tiles_, bnorm_ + I.slot, s);
-  const int64_t n = 2 * I.D;
-  const unsigned g = (unsigned)std::min<int64_t>((n + kLaneBlock - 1) / kLaneBlock, 1024);
-  hipLaunchKernelGGL(ln_init_kernel, dim3(std::max(g, 1u)), dim3(kLaneBlock), 0, s, v, vp, n,
-                     (const double *)bnorm_, I.slot, step_[cur_], done_);
+  double *x = role_[0] + I.off;  // the input role: the first step normalises it
+  launch_fill_gaussian_c128(p, seed, x, s);
+  launch_norm(x, 2 * I.D, partial_ + max_tiles_, bnorm_ + I.slot, s);
+  hipLaunchKernelGGL(ln_init_kernel, dim3(1), dim3(kWave), 0, s, (const double *)bnorm_, I.slot,
+                     step_[cur_], done_);
   MB_HIP(hipGetLastError());
 }
 
@@ -273,15 +263,17 @@ void LaneEngine::set_members(const std::vector<LaneInst> &members, hipStream_t s
     LaneInst &I = insts[i];
     I.tile0 = (int)tiles.size();
     const int64_t Ddn = I.t.d_dn;
-    I.rows = (I.t.comp_ok && Ddn <= kMaxRowsTileDn) ? 1 : 0;
-    const int64_t stp = I.rows ? std::max<int64_t>(1, tile_ / Ddn) * Ddn : tile_;
+    const int64_t rows_n = std::max<int64_t>(1, tile_ / Ddn) * Ddn;
+    I.rows = (I.t.comp_ok && Ddn <= kMaxRowsTileDn &&
+              rows_tile_lds(Ddn, rows_n, I.t.max_up_row) <= kRowsTileLdsMax) ? 1 : 0;
+    const int64_t stp = I.rows ? rows_n : tile_;
     for (int64_t e = 0; e < I.D; e += stp) {
       BTile tl;
       tl.inst = (int)i;
       tl.e0 = e;
       tl.n = (int)std::min<int64_t>(stp, I.D - e);
       tiles.push_back(tl);
-      if (I.rows) lds_bytes_ = std::max<size_t>(lds_bytes_, (size_t)tl.n * 2 * sizeof(double));
+      if (I.rows) lds_bytes_ = std::max<size_t>(lds_bytes_, rows_tile_lds(Ddn, tl.n, I.t.max_up_row));
     }
     I.tile1 = (int)tiles.size();
   }
@@ -305,27 +297,25 @@ void LaneEngine::set_members(const std::vector<LaneInst> &members, hipStream_t s
 
 void LaneEngine::tick(hipStream_t s) {
   if (n_members_ == 0) return;
-  double *v = role_[0], *vp = role_[1], *w = role_[2];
+  double *x = role_[0], *v = role_[1], *vp = role_[2], *w = role_[3];
   const int *sin = step_[cur_];
   int *sout = step_[cur_ ^ 1];
   const dim3 gt((unsigned)n_tiles_), gm((unsigned)n_members_), bl(kLaneBlock), bw(kWave);
-  hipLaunchKernelGGL(ln_hv_kernel, gt, bl, lds_bytes_, s, d_inst_, d_tiles_, v, vp, w,
-                     (const double *)beta_, sin, (const int *)done_, kmax_, partial_);
+  hipLaunchKernelGGL(ln_hv_kernel, gt, bl, lds_bytes_, s, d_inst_, d_tiles_, (const double *)x, v,
+                     (const double *)vp, w, (const double *)beta_, (const double *)bnorm_, sin,
+                     (const int *)done_, kmax_, partial_);
   hipLaunchKernelGGL(ln_reduce_kernel, gm, bw, 0, s, d_inst_, (const double *)partial_, alpha_, beta_,
                      done_, sin, sout, kmax_, 0);
-  hipLaunchKernelGGL(ln_vec_kernel, gt, bl, 0, s, d_inst_, d_tiles_, (const double *)v, w,
-                     (const double *)alpha_, (const double *)beta_, sin, (const int *)done_, kmax_, 0,
-                     partial_);
+  hipLaunchKernelGGL(ln_axpy_kernel, gt, bl, 0, s, d_inst_, d_tiles_, (const double *)v, w,
+                     (const double *)alpha_, sin, (const int *)done_, kmax_, partial_);
   hipLaunchKernelGGL(ln_reduce_kernel, gm, bw, 0, s, d_inst_, (const double *)partial_, alpha_, beta_,
                      done_, sin, sout, kmax_, 1);
-  hipLaunchKernelGGL(ln_vec_kernel, gt, bl, 0, s, d_inst_, d_tiles_, (const double *)v, w,
-                     (const double *)alpha_, (const double *)beta_, sin, (const int *)done_, kmax_, 1,
-                     partial_);
   MB_HIP(hipGetLastError());
-  // rotate: v_prev <- v, v <- w, w <- old v_prev; the next tick reads the new step indices
+  // rotate: the next input is w, v becomes v_prev; the old v_prev and x are free
   role_[0] = w;
-  role_[1] = v;
-  role_[2] = vp;
+  role_[1] = vp;
+  role_[2] = v;
+  role_[3] = x;
   cur_ ^= 1;
 }
 

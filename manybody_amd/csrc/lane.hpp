@@ -18,7 +18,7 @@ struct LaneInst {
   int k;           // Lanczos steps requested
 };
 
-// Device side of the lane: three role buffers (v, v_prev, w) whose roles rotate
+// Device side of the lane: four role buffers (x, v, v_prev, w) whose roles rotate
 // every tick, per-slot Lanczos state (alpha, beta, step, done, start norm),
 // member and tile tables.  Members join and leave between ticks; every member
 // does one Lanczos step per tick, counted by its own step index, so members at
@@ -39,8 +39,8 @@ class LaneEngine {
   int slot_take();  // -1 if all slots are in use
   void slot_give(int slot);
 
-  // Seeded N(0,1) start vector (per reference index) of a new member, normalised
-  // into the current v role, v_prev = 0, step 0.  Call before the member's first tick.
+  // Seeded N(0,1) start vector (per reference index) of a new member in the
+  // current input role, its norm, step 0.  Call before the member's first tick.
   void init_member(const LaneInst &I, const PermTables &p, uint64_t seed, hipStream_t s);
   // Replace the member list (tiles rebuilt and uploaded on s).
   void set_members(const std::vector<LaneInst> &members, hipStream_t s);
@@ -56,7 +56,7 @@ class LaneEngine {
   int nslots_, kmax_, tile_;
   int64_t max_tiles_;
   DevBuf mem_;
-  double *role_[3];                    // role_[0] = v, [1] = v_prev, [2] = w
+  double *role_[4];                    // role_[0] = x (step input), [1] = v, [2] = v_prev, [3] = w
   double *alpha_, *beta_, *bnorm_, *partial_;
   int *done_, *step_[2];
   int cur_ = 0;                        // step_[cur_] is read by the next tick

@@ -255,7 +255,7 @@ Server::Server(int slots, int64_t arena_bytes, int64_t lane_bytes, int64_t lane_
   MB_HIP(hipGetDevice(&device_));
   n_slots_ = slots;
   if (lane_bytes > 0) {
-    int64_t buf = std::max<int64_t>((lane_bytes - (32 << 20)) / 24, 0);
+    int64_t buf = std::max<int64_t>((lane_bytes - (32 << 20)) / 32, 0);
     while (buf >= 1024 && LaneEngine::bytes_needed(buf, kLaneSlots, kLaneKmax) > lane_bytes)
       buf -= std::max<int64_t>(buf / 64, 1024);
     if (buf < 1024) throw std::invalid_argument("lane_bytes too small for the Lanczos lane");
@@ -300,8 +300,8 @@ int64_t Server::submit(const ServeRequest &r) {
   j->res.arrival = r.arrival >= 0.0 ? r.arrival : now();
   j->fp = footprint_parts(r);  // throws on an invalid request
   j->lane = lane_ && r.kind == kReqGround && j->fp.dim <= lane_max_dim_ && r.steps <= kLaneKmax;
-  // lane members hold their three vectors in the lane's buffers
-  j->res.footprint = j->lane ? j->fp.tables + 64 * 1024 + 3 * 16 * j->fp.dim : request_footprint(r);
+  // lane members hold their four vectors in the lane's buffers
+  j->res.footprint = j->lane ? j->fp.tables + 64 * 1024 + 4 * 16 * j->fp.dim : request_footprint(r);
   std::lock_guard<std::mutex> lk(m_);
   j->id = next_id_++;
   Job *p = j.get();
@@ -631,7 +631,7 @@ void Server::lane_worker() {
         res.e0 = steps > 0 ? tridiag_min_eig(steps, pin_alpha(m.slot), pin_beta(m.slot)) : 0.0;
         res.status = MB_OK;
       }
-      res.peak = m.reg ? m.reg->peak() + 3 * 16 * res.dim : 0;
+      res.peak = m.reg ? m.reg->peak() + 4 * 16 * res.dim : 0;
       res.finished = now();
       release(m);
       Job *j = m.job;
