@@ -166,6 +166,18 @@ int mb_find_min_eigenpair(const mb_hubbard *h, const void *b_dev, int k, double 
 
 /* time_evolve_state(H, psi, t, {krylov_steps}) — evolve_state.h:164-178:
  * out = exp(-i t H) psi, complex128 device order.  krylov_steps 0 -> D. */
+/* Lanczos options (no reference counterpart): MB_LANCZOS_REORTHOGONALIZE
+ * orthogonalises every new vector twice (classical Gram-Schmidt, FP64 on the
+ * device) against all previous ones and keeps the k + 1 vectors in HBM
+ * (16 D (k + 1) bytes; MB_ERR_NOMEM if they do not fit).  The recurrence of
+ * lanczos.h:15-40 is otherwise unchanged; flags = 0 is mb_lanczos_pass_one. */
+#define MB_LANCZOS_REORTHOGONALIZE 1
+int mb_lanczos_pass_one_ex(const mb_hubbard *h, const void *b_dev, int k, int flags, double *alphas,
+                           double *betas, int *steps, double *b_norm, mb_stream stream);
+/* As mb_find_min_eigenpair; with MB_LANCZOS_REORTHOGONALIZE the eigenvector is
+ * assembled from the stored vectors (no second pass). */
+int mb_find_min_eigenpair_ex(const mb_hubbard *h, const void *b_dev, int k, int flags,
+                             double *value, void *x_dev, mb_stream stream);
 int mb_time_evolve(const mb_hubbard *h, const void *psi_dev, double t, int krylov_steps,
                    void *out_dev, mb_stream stream);
 

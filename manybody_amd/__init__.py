@@ -85,6 +85,10 @@ _SIGS = {
     "mb_batch_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
     "mb_batch_lanczos": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "mb_batch_ground_energies": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp]),
+    "mb_lanczos_pass_one_ex": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_double), _vp]),
+    "mb_find_min_eigenpair_ex": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_double), _vp,
+                                           _vp]),
     "mb_server_create": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.POINTER(_vp)]),
     "mb_server_destroy": (None, [_vp]),
     "mb_server_submit": (C.c_int, [_vp, _vp, C.POINTER(C.c_int64)]),
@@ -364,23 +368,36 @@ class EvolutionOptions:
     krylov_steps: int = 30
 
 
-def lanczos_pass_one(op: HubbardOperator, b, k: int) -> LanczosDecomposition:
+LANCZOS_REORTHOGONALIZE = 1
+
+
+def lanczos_pass_one(op: HubbardOperator, b, k: int,
+                     reorthogonalize: bool = False) -> LanczosDecomposition:
     al = (C.c_double * max(k, 1))()
     be = (C.c_double * max(k, 1))()
     steps = C.c_int()
     bn = C.c_double()
-    _check(lib().mb_lanczos_pass_one(op._h, _ptr(b), k, al, be, C.byref(steps), C.byref(bn),
-                                     _stream()))
+    if reorthogonalize:
+        _check(lib().mb_lanczos_pass_one_ex(op._h, _ptr(b), k, LANCZOS_REORTHOGONALIZE, al, be,
+                                            C.byref(steps), C.byref(bn), _stream()))
+    else:
+        _check(lib().mb_lanczos_pass_one(op._h, _ptr(b), k, al, be, C.byref(steps), C.byref(bn),
+                                         _stream()))
     m = steps.value
     return LanczosDecomposition(list(al[:m]), list(be[:max(m - 1, 0)]), m, bn.value)
 
 
-def find_min_eigenpair(op: HubbardOperator, k: int, b, want_vector: bool = True) -> EigenPair:
+def find_min_eigenpair(op: HubbardOperator, k: int, b, want_vector: bool = True,
+                       reorthogonalize: bool = False) -> EigenPair:
     import torch
     val = C.c_double()
     x = op.empty() if want_vector else None
-    _check(lib().mb_find_min_eigenpair(op._h, _ptr(b), k, C.byref(val),
-                                       _ptr(x) if x is not None else None, _stream()))
+    xp = _ptr(x) if x is not None else None
+    if reorthogonalize:
+        _check(lib().mb_find_min_eigenpair_ex(op._h, _ptr(b), k, LANCZOS_REORTHOGONALIZE,
+                                              C.byref(val), xp, _stream()))
+    else:
+        _check(lib().mb_find_min_eigenpair(op._h, _ptr(b), k, C.byref(val), xp, _stream()))
     return EigenPair(val.value, x)
 
 

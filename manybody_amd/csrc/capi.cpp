@@ -299,6 +299,36 @@ int mb_find_min_eigenpair(const mb_hubbard *h, const void *b_dev, int k, double 
   });
 }
 
+int mb_lanczos_pass_one_ex(const mb_hubbard *h, const void *b_dev, int k, int flags, double *alphas,
+                           double *betas, int *steps, double *b_norm, mb_stream stream) {
+  return guarded([&] {
+    need(b_dev && alphas && betas && steps && b_norm, "null argument");
+    need(k >= 0, "k must be non-negative");
+    need((flags & ~MB_LANCZOS_REORTHOGONALIZE) == 0, "unknown Lanczos flags");
+    auto r = (flags & MB_LANCZOS_REORTHOGONALIZE)
+                 ? mb::lanczos_pass_one_reorth(H(h), b_dev, k, S(stream))
+                 : mb::lanczos_pass_one(H(h), b_dev, k, S(stream));
+    *steps = r.steps;
+    *b_norm = r.b_norm;
+    for (int i = 0; i < k; ++i) {
+      alphas[i] = i < (int)r.alphas.size() ? r.alphas[i] : 0.0;
+      betas[i] = i < (int)r.betas.size() ? r.betas[i] : 0.0;
+    }
+  });
+}
+
+int mb_find_min_eigenpair_ex(const mb_hubbard *h, const void *b_dev, int k, int flags,
+                             double *value, void *x_dev, mb_stream stream) {
+  return guarded([&] {
+    need(b_dev && value, "null argument");
+    need(k >= 0, "k must be non-negative");
+    need((flags & ~MB_LANCZOS_REORTHOGONALIZE) == 0, "unknown Lanczos flags");
+    *value = (flags & MB_LANCZOS_REORTHOGONALIZE)
+                 ? mb::find_min_eigenpair_reorth(H(h), b_dev, k, x_dev, S(stream))
+                 : mb::find_min_eigenpair(H(h), b_dev, k, x_dev, S(stream));
+  });
+}
+
 int mb_time_evolve(const mb_hubbard *h, const void *psi_dev, double t, int krylov_steps,
                    void *out_dev, mb_stream stream) {
   return guarded([&] {

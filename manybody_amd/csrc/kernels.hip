@@ -13,13 +13,16 @@
 //     of that strip are produced, so every x element is fetched from HBM once
 //     and its ~17 reuses hit L2.  Workgroup -> (strip, row chunk) is XCD-aware:
 //     workgroups are dispatched round-robin over the 8 XCDs, so the chunks of a
-//     strip are given ids congruent mod 8.  Hop tables are read with scalar
-//     loads (row index made wave-uniform).  Writes y = (H_up x).
+//     strip are given ids congruent mod 8.  The hop lists of a workgroup's 32
+//     rows are staged in LDS as 32-bit byte offsets of the target rows (a
+//     plane is < 4 GiB) and amplitudes, so a gather is one add on a uniform
+//     base.  Writes y = (H_up x).
 //  2. row_kernel: one workgroup per plane row stages x[u, :] in LDS (91.5 KB
 //     for D_dn = 11,440 doubles), applies the diagonal and the down hops from
-//     LDS (packed column-major hop table, coalesced), adds the strip result and
-//     applies the epilogue (plain, Lanczos, reconstruction, Chebyshev) with a
-//     per-workgroup partial sum for the fused inner products.
+//     LDS (compact table: padded target indices and sign masks, one trip count
+//     per wave), adds the strip result and applies the epilogue (plain,
+//     Lanczos, reconstruction, Chebyshev) with a per-workgroup partial sum for
+//     the fused inner products.
 //
 // Reductions are two-level and atomic-free (fixed partial slots, fixed tree),
 // so results do not depend on scheduling.  A one-kernel generic path (thread
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void up_strip_kernel(HvTables t, int P, con
                                                           T *__restrict__ y, int nJ, int nC,
                                                           const int *flags) {
   constexpr int LPR = 16, RPW = kWave / LPR, RC = 32, W = LPR * VEC;
-  __shared__ int s_tgt[RC][MAXH];
+  __shared__ uint32_t s_off[RC][MAXH];  // byte offset of the target row in the plane
   __shared__ T s_amp[RC][MAXH];
   __shared__ int s_n[RC];
   if (flags && flags[0]) return;
@@ -161,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void up_strip_kernel(HvTables t, int P, con
       const int b = t.up_ptr[u], n = t.up_ptr[u + 1] - b;
       if (k == 0) s_n[r] = n;
       if (k < n) {
-        s_tgt[r][k] = t.up_tgt[b + k];
+        s_off[r][k] = (uint32_t)((uint64_t)t.up_tgt[b + k] * (uint64_t)t.d_dn * sizeof(T));
         s_amp[r][k] = amp[b + k];
       }
     } else if (k == 0) {
@@ -174,7 +177,9 @@ __global__ __launch_bounds__(kBlock) void up_strip_kernel(HvTables t, int P, con
   const int64_t col = (int64_t)J * W + VEC * (lane % LPR);
   const bool col_ok = col < Ddn;  // D_dn is a multiple of VEC
   const int64_t plane = (int64_t)p * t.d_up * Ddn;
-  const T *__restrict__ xp = x + plane;
+  // gathers: uniform plane base + 32-bit byte offset (row offset from LDS + column)
+  const char *__restrict__ xb = reinterpret_cast<const char *>(x + plane);
+  const uint32_t colb = (uint32_t)(col * (int64_t)sizeof(T));
   T *__restrict__ yp = y + plane;
   const int sub = lane / LPR;
   for (int rb = wv * RPW; rb < RC; rb += (kBlock / kWave) * RPW) {
@@ -189,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void up_strip_kernel(HvTables t, int P, con
     for (int k = 0; k < n; ++k) {
       const T a = s_amp[r][k];
       Vec<T, VEC> v;
-      v.load(xp + (int64_t)s_tgt[r][k] * Ddn + col);
+      v.load(reinterpret_cast<const T *>(xb + (s_off[r][k] + colb)));
 #pragma unroll
       for (int j = 0; j < VEC; ++j) acc[j] += a * v.v[j];
     }
@@ -610,6 +615,64 @@ __global__ __launch_bounds__(kBlock) void lanczos_normalize_kernel(double *__res
     w[i] = w[i] / b;
 }
 
+// Reorthogonalisation against m stored Krylov vectors V_j (planar, stride
+// 2 D): partial[(2 j + c) * gridDim.x + blk] = Re / Im of the block's share of
+// <V_j|w>.  Fixed blocks and a fixed tree, so the coefficients are reproducible.
+__global__ __launch_bounds__(kBlock) void proj_partial_kernel(const double *__restrict__ V,
+                                                              const double *__restrict__ w,
+                                                              int64_t D, double *partial,
+                                                              const int *flags) {
+  __shared__ double lds[kBlock / kWave];
+  if (flags[0]) return;
+  const int j = blockIdx.y;
+  const double *vj = V + (int64_t)j * 2 * D;
+  double re = 0.0, im = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
+    const double vr = vj[e], vi = vj[D + e], wr = w[e], wi = w[D + e];
+    re += vr * wr + vi * wi;
+    im += vr * wi - vi * wr;
+  }
+  re = wg_sum(re, lds);
+  im = wg_sum(im, lds);
+  if (threadIdx.x == 0) {
+    partial[(2 * (int64_t)j) * gridDim.x + blockIdx.x] = re;
+    partial[(2 * (int64_t)j + 1) * gridDim.x + blockIdx.x] = im;
+  }
+}
+
+// c[2 j + c] = sum of the nb partials of coefficient (j, c); one block per j.
+__global__ __launch_bounds__(kBlock) void proj_final_kernel(const double *partial, int nb,
+                                                            double *c, const int *flags) {
+  __shared__ double lds[kBlock / kWave];
+  if (flags[0]) return;
+  const int j = blockIdx.x;
+  const double re = final_sum(partial + (2 * (int64_t)j) * nb, nb, lds);
+  const double im = final_sum(partial + (2 * (int64_t)j + 1) * nb, nb, lds);
+  if (threadIdx.x == 0) {
+    c[2 * j] = re;
+    c[2 * j + 1] = im;
+  }
+}
+
+// w -= sum_j c_j V_j (complex c_j, j in order).
+__global__ __launch_bounds__(kBlock) void proj_sub_kernel(const double *__restrict__ V, int m,
+                                                          const double *__restrict__ c,
+                                                          double *__restrict__ w, int64_t D,
+                                                          const int *flags) {
+  if (flags[0]) return;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
+    double sr = 0.0, si = 0.0;
+    for (int j = 0; j < m; ++j) {
+      const double *vj = V + (int64_t)j * 2 * D;
+      const double cr = c[2 * j], ci = c[2 * j + 1], vr = vj[e], vi = vj[D + e];
+      sr += cr * vr - ci * vi;
+      si += cr * vi + ci * vr;
+    }
+    w[e] -= sr;
+    w[D + e] -= si;
+  }
+}
+
 // w /= beta ; x += y * w   (complex y, planar vectors)
 __global__ __launch_bounds__(kBlock) void recon_update_kernel(double *__restrict__ w,
                                                               double *__restrict__ x, double beta,
@@ -654,7 +717,10 @@ bool force_generic() {
 
 template <typename T>
 bool use_two_pass(const HvTables &t) {
+  // the strip kernel addresses a plane with 32-bit byte offsets
+  const bool plane_fits = t.d_up * t.d_dn * (int64_t)sizeof(T) < ((int64_t)1 << 32);
   return t.ell_ok && t.max_up_row <= 64 && (t.d_dn + 1) * (int64_t)sizeof(T) <= kMaxRowLds &&
+         plane_fits &&
          !force_generic();
 }
 
@@ -794,6 +860,31 @@ void launch_cdot(const double *a, const double *b, int64_t D, double *partial, d
                      (int64_t)kRedBlocks, out, 0);
   hipLaunchKernelGGL(final_kernel, dim3(1), dim3(kBlock), 0, s, (const double *)partial + kRedBlocks,
                      (int64_t)kRedBlocks, out + 1, 0);
+  MB_HIP(hipGetLastError());
+}
+
+void launch_sumsq_partial(const double *x, int64_t n, double *partial, hipStream_t s) {
+  hipLaunchKernelGGL((sumsq_partial_kernel<double>), dim3(kRedBlocks), dim3(kBlock), 0, s, x, n,
+                     partial);
+  MB_HIP(hipGetLastError());
+}
+
+void launch_reorth(const double *V, int m, double *w, int64_t D, double *partial, double *c,
+                   const int *flags, hipStream_t s) {
+  constexpr int nb = kProjBlocks;
+  hipLaunchKernelGGL(proj_partial_kernel, dim3(nb, (unsigned)m), dim3(kBlock), 0, s, V,
+                     (const double *)w, D, partial, flags);
+  hipLaunchKernelGGL(proj_final_kernel, dim3((unsigned)m), dim3(kBlock), 0, s, (const double *)partial,
+                     nb, c, flags);
+  hipLaunchKernelGGL(proj_sub_kernel, dim3(grid_for(D, 4096)), dim3(kBlock), 0, s, V, m,
+                     (const double *)c, w, D, flags);
+  MB_HIP(hipGetLastError());
+}
+
+void launch_reorth_sub(const double *V, int m, const double *c, double *w, int64_t D,
+                       const int *flags, hipStream_t s) {
+  hipLaunchKernelGGL(proj_sub_kernel, dim3(grid_for(D, 4096)), dim3(kBlock), 0, s, V, m, c, w, D,
+                     flags);
   MB_HIP(hipGetLastError());
 }
 

@@ -195,6 +195,93 @@ LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream
   return res;
 }
 
+// lanczos_pass_one with full reorthogonalisation (an option the reference does
+// not have; its recurrence, lanczos.h:15-40, is kept step for step): after
+// w -= alpha v, w is orthogonalised twice against every stored Lanczos vector
+// (classical Gram-Schmidt twice, FP64 on the device), then beta = |w|.  All
+// k + 1 vectors stay in HBM (16 D (k + 1) bytes); they are handed to `basis`
+// when it is given.
+LanczosResult lanczos_pass_one_reorth(const Hubbard &h, const void *b, int k, hipStream_t s,
+                                      DevBuf *basis) {
+  const int64_t D = h.dim();
+  const HvTables t = tables_of(h);
+  const int64_t nparts = hv_partials(t, 2);
+  const int kk = std::max(k, 1);
+  DevBuf V((size_t)(kk + 1) * 16 * (size_t)D);
+  DevBuf scal(sizeof(double) * (2 * (size_t)kk + 2 + (size_t)std::max<int64_t>(kRedBlocks, nparts)) + 16);
+  DevBuf pp((size_t)2 * (kk + 1) * kProjBlocks * sizeof(double));
+  DevBuf coef((size_t)2 * (kk + 1) * sizeof(double));
+  LanczosState st;
+  double *base = scal.as<double>();
+  st.alpha = base;
+  st.beta = base + kk;
+  st.bnorm = base + 2 * kk;
+  st.partial = base + 2 * kk + 2;
+  st.flags = reinterpret_cast<int *>(base + 2 * kk + 1);
+  LanczosResult res;
+  launch_norm((const double *)b, 2 * D, st.partial, st.bnorm, s);
+  MB_HIP(hipMemcpyAsync(&res.b_norm, st.bnorm, sizeof(double), hipMemcpyDeviceToHost, s));
+  MB_HIP(hipStreamSynchronize(s));
+  if (res.b_norm <= kTol) throw std::runtime_error("Input vector has zero norm");
+  if (k == 0) return res;
+  double *Vb = V.as<double>();
+  launch_scale_div((const double *)b, st.bnorm, Vb, 2 * D, s);
+  MB_HIP(hipMemsetAsync(st.flags, 0, 2 * sizeof(int), s));
+  for (int i = 0; i < k; ++i) {
+    double *v = Vb + (int64_t)i * 2 * D, *w = v + 2 * D;
+    EpiArgs a;
+    a.prev = i > 0 ? v - 2 * D : v;
+    a.beta_dev = st.beta;
+    a.flags = st.flags;
+    a.step = i;
+    a.partial = st.partial;
+    const int64_t slots = launch_hv<double>(t, 2, v, w, Epi::Lanczos, a, s);
+    launch_lanczos_finish_alpha(st, i, 2 * slots, s);
+    launch_lanczos_axpy_norm(v, w, 2 * D, st, i, s);
+    for (int pass = 0; pass < 2; ++pass)
+      launch_reorth(Vb, i + 1, w, D, pp.as<double>(), coef.as<double>(), st.flags, s);
+    launch_sumsq_partial(w, 2 * D, st.partial, s);
+    launch_lanczos_finish_beta(st, i, s);
+    launch_lanczos_normalize(w, 2 * D, st, i, s);
+  }
+  std::vector<double> al(k), be(k);
+  int flags[2];
+  MB_HIP(hipMemcpyAsync(al.data(), st.alpha, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+  MB_HIP(hipMemcpyAsync(be.data(), st.beta, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+  MB_HIP(hipMemcpyAsync(flags, st.flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+  MB_HIP(hipStreamSynchronize(s));
+  res.steps = flags[1];
+  res.alphas.assign(al.begin(), al.begin() + res.steps);
+  res.betas.assign(be.begin(), be.begin() + std::max(res.steps - 1, 0));
+  if (basis) *basis = std::move(V);
+  return res;
+}
+
+// Lowest eigenpair from the reorthogonalised decomposition: x = |b| sum_j y_j V_j
+// straight from the stored vectors (no second pass).
+double find_min_eigenpair_reorth(const Hubbard &h, const void *b, int k, void *x, hipStream_t s) {
+  const int64_t D = h.dim();
+  DevBuf V;
+  const LanczosResult dec = lanczos_pass_one_reorth(h, b, k, s, &V);
+  const int m = dec.steps;
+  if (m == 0) {
+    if (x) MB_HIP(hipMemsetAsync(x, 0, (size_t)D * 16, s));
+    MB_HIP(hipStreamSynchronize(s));
+    return 0.0;
+  }
+  if (!x) return tridiag_min_eig(m, dec.alphas.data(), dec.betas.data());
+  std::vector<double> wv(m), Q((size_t)m * m);
+  tridiag_eig(m, dec.alphas.data(), dec.betas.data(), wv.data(), Q.data());
+  std::vector<double> c(2 * (size_t)m, 0.0);
+  for (int j = 0; j < m; ++j) c[2 * j] = -Q[j] * dec.b_norm;  // x -= c_j V_j with x = 0
+  DevBuf dc = upload(c);
+  DevBuf zero_flag = upload(std::vector<int>(2, 0));
+  MB_HIP(hipMemsetAsync(x, 0, (size_t)D * 16, s));
+  launch_reorth_sub(V.as<double>(), m, dc.as<double>(), (double *)x, D, zero_flag.as<int>(), s);
+  MB_HIP(hipStreamSynchronize(s));
+  return wv[0];
+}
+
 void lanczos_pass_two(const Hubbard &h, const void *b, const LanczosResult &dec,
                       const std::vector<std::complex<double>> &y, void *x, hipStream_t s,
                       LanczosWork *work) {

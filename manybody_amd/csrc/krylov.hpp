@@ -33,6 +33,10 @@ void lanczos_pass_two(const Hubbard &h, const void *b, const LanczosResult &dec,
                       LanczosWork *work = nullptr);
 double tridiag_min_eig(int m, const double *a, const double *b);
 double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipStream_t s);
+// Reorthogonalised variants (all Lanczos vectors kept in HBM; see krylov.cpp).
+LanczosResult lanczos_pass_one_reorth(const Hubbard &h, const void *b, int k, hipStream_t s,
+                                      DevBuf *basis = nullptr);
+double find_min_eigenpair_reorth(const Hubbard &h, const void *b, int k, void *x, hipStream_t s);
 std::vector<std::complex<double>> expm_first_column(const LanczosResult &dec, double t);
 void time_evolve(const Hubbard &h, const void *psi, double t, int krylov_steps, void *out,
                  hipStream_t s);

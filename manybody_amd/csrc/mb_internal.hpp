@@ -254,6 +254,15 @@ struct LanczosState {
   double *bnorm;     // [1]
 };
 void launch_norm(const double *x, int64_t n, double *partial, double *out, hipStream_t s);
+void launch_sumsq_partial(const double *x, int64_t n, double *partial, hipStream_t s);
+// w -= V (V^H w) against the m stored planar vectors of V (stride 2 D); partial
+// holds 2 m kProjBlocks doubles, c 2 m doubles; no-op once flags[0] is set
+constexpr int kProjBlocks = 256;
+void launch_reorth(const double *V, int m, double *w, int64_t D, double *partial, double *c,
+                   const int *flags, hipStream_t s);
+// w -= sum_j c_j V_j only (complex c, 2 m doubles)
+void launch_reorth_sub(const double *V, int m, const double *c, double *w, int64_t D,
+                       const int *flags, hipStream_t s);
 // out[0], out[1] = Re, Im of <a|b> for planar complex vectors of D elements;
 // partial holds 2 kRedBlocks doubles
 void launch_cdot(const double *a, const double *b, int64_t D, double *partial, double *out,
