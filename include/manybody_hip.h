@@ -189,6 +189,15 @@ int mb_time_evolve_steps(const mb_hubbard *h, const void *psi_dev, double t0, do
 /* Symmetric tridiagonal eigen-decomposition on the host (replaces arma::eig_sym
  * of T_k, lanczos.h:258-271): w ascending, V (m*m, column j = eigenvector j,
  * V[i + m*j]) optional. */
+/* Time evolution options (no reference counterpart): MB_EVOLVE_FP32 runs the
+ * Krylov vectors and H.v in FP32 complex; the state psi, the norms, the Lanczos
+ * alpha/beta reductions and the exponential of T_k stay FP64.  flags = 0 is
+ * mb_time_evolve / mb_time_evolve_steps. */
+#define MB_EVOLVE_FP32 1
+int mb_time_evolve_ex(const mb_hubbard *h, const void *psi_dev, double t, int krylov_steps, int flags,
+                      void *out_dev, mb_stream stream);
+int mb_time_evolve_steps_ex(const mb_hubbard *h, const void *psi_dev, double t0, double t1, double dt,
+                            int krylov_steps, int flags, void *out_dev, mb_stream stream);
 int mb_tridiag_eig(int m, const double *alphas, const double *betas, double *w, double *V);
 
 /* KPM density-of-states moments mu_0..mu_{M-1} of Hs = (H - b)/a, averaged

@@ -89,6 +89,9 @@ _SIGS = {
                                          C.POINTER(C.c_double), _vp]),
     "mb_find_min_eigenpair_ex": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_double), _vp,
                                            _vp]),
+    "mb_time_evolve_ex": (C.c_int, [_vp, _vp, C.c_double, C.c_int, C.c_int, _vp, _vp]),
+    "mb_time_evolve_steps_ex": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                          C.c_int, _vp, _vp]),
     "mb_server_create": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.POINTER(_vp)]),
     "mb_server_destroy": (None, [_vp]),
     "mb_server_submit": (C.c_int, [_vp, _vp, C.POINTER(C.c_int64)]),
@@ -401,18 +404,24 @@ def find_min_eigenpair(op: HubbardOperator, k: int, b, want_vector: bool = True,
     return EigenPair(val.value, x)
 
 
+EVOLVE_FP32 = 1
+
+
 def time_evolve_state(op: HubbardOperator, psi, t: float,
-                      opts: EvolutionOptions = EvolutionOptions()):
+                      opts: EvolutionOptions = EvolutionOptions(), fp32: bool = False):
+    """exp(-iHt) psi (evolve_state.h:164-178).  fp32: Krylov vectors and H.v in
+    FP32, state / alpha-beta reductions / exponential in FP64."""
     out = op.empty()
-    _check(lib().mb_time_evolve(op._h, _ptr(psi), t, opts.krylov_steps, _ptr(out), _stream()))
+    _check(lib().mb_time_evolve_ex(op._h, _ptr(psi), t, opts.krylov_steps,
+                                   EVOLVE_FP32 if fp32 else 0, _ptr(out), _stream()))
     return out
 
 
 def time_evolve_state_steps(op: HubbardOperator, psi, t0: float, t1: float, dt: float,
-                            opts: EvolutionOptions = EvolutionOptions()):
+                            opts: EvolutionOptions = EvolutionOptions(), fp32: bool = False):
     out = op.empty()
-    _check(lib().mb_time_evolve_steps(op._h, _ptr(psi), t0, t1, dt, opts.krylov_steps, _ptr(out),
-                                      _stream()))
+    _check(lib().mb_time_evolve_steps_ex(op._h, _ptr(psi), t0, t1, dt, opts.krylov_steps,
+                                         EVOLVE_FP32 if fp32 else 0, _ptr(out), _stream()))
     return out
 
 

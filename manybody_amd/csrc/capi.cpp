@@ -338,6 +338,27 @@ int mb_time_evolve(const mb_hubbard *h, const void *psi_dev, double t, int krylo
   });
 }
 
+int mb_time_evolve_ex(const mb_hubbard *h, const void *psi_dev, double t, int krylov_steps, int flags,
+                      void *out_dev, mb_stream stream) {
+  return guarded([&] {
+    need(psi_dev && out_dev, "null argument");
+    need(krylov_steps >= 0, "krylov_steps must be non-negative");
+    need((flags & ~MB_EVOLVE_FP32) == 0, "unknown time evolution flags");
+    mb::time_evolve(H(h), psi_dev, t, krylov_steps, out_dev, S(stream), (flags & MB_EVOLVE_FP32) != 0);
+  });
+}
+
+int mb_time_evolve_steps_ex(const mb_hubbard *h, const void *psi_dev, double t0, double t1, double dt,
+                            int krylov_steps, int flags, void *out_dev, mb_stream stream) {
+  return guarded([&] {
+    need(psi_dev && out_dev, "null argument");
+    need(krylov_steps >= 0, "krylov_steps must be non-negative");
+    need((flags & ~MB_EVOLVE_FP32) == 0, "unknown time evolution flags");
+    mb::time_evolve_steps(H(h), psi_dev, t0, t1, dt, krylov_steps, out_dev, S(stream),
+                          (flags & MB_EVOLVE_FP32) != 0);
+  });
+}
+
 int mb_time_evolve_steps(const mb_hubbard *h, const void *psi_dev, double t0, double t1,
                          double dt, int krylov_steps, void *out_dev, mb_stream stream) {
   return guarded([&] {

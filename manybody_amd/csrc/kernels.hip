@@ -560,12 +560,14 @@ __global__ __launch_bounds__(kBlock) void final_kernel(const double *partial, in
   if (threadIdx.x == 0) out[0] = take_sqrt ? sqrt(s) : s;
 }
 
+// y = x / den[0], x in FP64, y in T (FP32 Krylov vectors are rounded here).
+template <typename T>
 __global__ __launch_bounds__(kBlock) void scale_div_kernel(const double *__restrict__ x,
                                                            const double *den,
-                                                           double *__restrict__ y, int64_t n) {
+                                                           T *__restrict__ y, int64_t n) {
   const double dv = den[0];
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    y[i] = x[i] / dv;
+    y[i] = (T)(x[i] / dv);
 }
 
 __global__ __launch_bounds__(kBlock) void lanczos_finish_alpha_kernel(LanczosState st, int step,
@@ -580,17 +582,18 @@ __global__ __launch_bounds__(kBlock) void lanczos_finish_alpha_kernel(LanczosSta
 }
 
 // w -= alpha v ; partial = |w|^2
-__global__ __launch_bounds__(kBlock) void lanczos_axpy_norm_kernel(const double *__restrict__ v,
-                                                                   double *__restrict__ w, int64_t n,
+template <typename T>
+__global__ __launch_bounds__(kBlock) void lanczos_axpy_norm_kernel(const T *__restrict__ v,
+                                                                   T *__restrict__ w, int64_t n,
                                                                    LanczosState st, int step) {
   __shared__ double lds[kBlock / kWave];
   if (st.flags[0]) return;
-  const double a = st.alpha[step];
+  const T a = (T)st.alpha[step];
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const double q = w[i] - a * v[i];
+    const T q = w[i] - a * v[i];
     w[i] = q;
-    s += q * q;
+    s += (double)q * (double)q;
   }
   s = wg_sum(s, lds);
   if (threadIdx.x == 0) st.partial[blockIdx.x] = s;
@@ -607,10 +610,11 @@ __global__ __launch_bounds__(kBlock) void lanczos_finish_beta_kernel(LanczosStat
   }
 }
 
-__global__ __launch_bounds__(kBlock) void lanczos_normalize_kernel(double *__restrict__ w, int64_t n,
+template <typename T>
+__global__ __launch_bounds__(kBlock) void lanczos_normalize_kernel(T *__restrict__ w, int64_t n,
                                                                    LanczosState st, int step) {
   if (st.flags[0]) return;
-  const double b = st.beta[step];
+  const T b = (T)st.beta[step];
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
     w[i] = w[i] / b;
 }
@@ -673,20 +677,23 @@ __global__ __launch_bounds__(kBlock) void proj_sub_kernel(const double *__restri
   }
 }
 
-// w /= beta ; x += y * w   (complex y, planar vectors)
-__global__ __launch_bounds__(kBlock) void recon_update_kernel(double *__restrict__ w,
+// w /= beta ; x += y * w   (complex y, planar vectors; w in T, x accumulated in FP64)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void recon_update_kernel(T *__restrict__ w,
                                                               double *__restrict__ x, double beta,
                                                               double yr, double yi, int64_t D) {
+  const T bt = (T)beta;
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
-    const double wr = w[e] / beta, wi = w[D + e] / beta;
+    const T wr = w[e] / bt, wi = w[D + e] / bt;
     w[e] = wr;
     w[D + e] = wi;
-    x[e] += wr * yr - wi * yi;
-    x[D + e] += wr * yi + wi * yr;
+    x[e] += (double)wr * yr - (double)wi * yi;
+    x[D + e] += (double)wr * yi + (double)wi * yr;
   }
 }
 
-__global__ __launch_bounds__(kBlock) void scale_complex_kernel(const double *__restrict__ x,
+template <typename T>
+__global__ __launch_bounds__(kBlock) void scale_complex_kernel(const T *__restrict__ x,
                                                                double yr, double yi,
                                                                double *__restrict__ o, int64_t D) {
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
@@ -892,48 +899,68 @@ void launch_norm(const double *x, int64_t n, double *partial, double *out, hipSt
   launch_sumsq(x, false, n, partial, out, true, s);
 }
 
-void launch_scale_div(const double *x, const double *den, double *y, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(scale_div_kernel, dim3(grid_for(n, 1 << 16)), dim3(kBlock), 0, s, x, den, y, n);
+template <typename T>
+void launch_scale_div(const double *x, const double *den, T *y, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL((scale_div_kernel<T>), dim3(grid_for(n, 1 << 16)), dim3(kBlock), 0, s, x, den, y,
+                     n);
   MB_HIP(hipGetLastError());
 }
+template void launch_scale_div<double>(const double *, const double *, double *, int64_t, hipStream_t);
+template void launch_scale_div<float>(const double *, const double *, float *, int64_t, hipStream_t);
 
 void launch_lanczos_finish_alpha(const LanczosState &st, int step, int64_t nparts, hipStream_t s) {
   hipLaunchKernelGGL(lanczos_finish_alpha_kernel, dim3(1), dim3(kBlock), 0, s, st, step, nparts);
   MB_HIP(hipGetLastError());
 }
 
-void launch_lanczos_axpy_norm(const double *v, double *w, int64_t n, const LanczosState &st,
-                              int step, hipStream_t s) {
-  hipLaunchKernelGGL(lanczos_axpy_norm_kernel, dim3(kRedBlocks), dim3(kBlock), 0, s, v, w, n, st,
+template <typename T>
+void launch_lanczos_axpy_norm(const T *v, T *w, int64_t n, const LanczosState &st, int step,
+                              hipStream_t s) {
+  hipLaunchKernelGGL((lanczos_axpy_norm_kernel<T>), dim3(kRedBlocks), dim3(kBlock), 0, s, v, w, n, st,
                      step);
   MB_HIP(hipGetLastError());
 }
+template void launch_lanczos_axpy_norm<double>(const double *, double *, int64_t, const LanczosState &,
+                                               int, hipStream_t);
+template void launch_lanczos_axpy_norm<float>(const float *, float *, int64_t, const LanczosState &,
+                                              int, hipStream_t);
 
 void launch_lanczos_finish_beta(const LanczosState &st, int step, hipStream_t s) {
   hipLaunchKernelGGL(lanczos_finish_beta_kernel, dim3(1), dim3(kBlock), 0, s, st, step);
   MB_HIP(hipGetLastError());
 }
 
-void launch_lanczos_normalize(double *w, int64_t n, const LanczosState &st, int step,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(lanczos_normalize_kernel, dim3(grid_for(n, 1 << 16)), dim3(kBlock), 0, s, w, n,
-                     st, step);
+template <typename T>
+void launch_lanczos_normalize(T *w, int64_t n, const LanczosState &st, int step, hipStream_t s) {
+  hipLaunchKernelGGL((lanczos_normalize_kernel<T>), dim3(grid_for(n, 1 << 16)), dim3(kBlock), 0, s, w,
+                     n, st, step);
   MB_HIP(hipGetLastError());
 }
+template void launch_lanczos_normalize<double>(double *, int64_t, const LanczosState &, int, hipStream_t);
+template void launch_lanczos_normalize<float>(float *, int64_t, const LanczosState &, int, hipStream_t);
 
-void launch_recon_update(double *w, double *x, double beta, double yr, double yi, int64_t D,
+template <typename T>
+void launch_recon_update(T *w, double *x, double beta, double yr, double yi, int64_t D,
                          hipStream_t s) {
-  hipLaunchKernelGGL(recon_update_kernel, dim3(grid_for(D, 1 << 16)), dim3(kBlock), 0, s, w, x,
+  hipLaunchKernelGGL((recon_update_kernel<T>), dim3(grid_for(D, 1 << 16)), dim3(kBlock), 0, s, w, x,
                      beta, yr, yi, D);
   MB_HIP(hipGetLastError());
 }
+template void launch_recon_update<double>(double *, double *, double, double, double, int64_t,
+                                          hipStream_t);
+template void launch_recon_update<float>(float *, double *, double, double, double, int64_t,
+                                         hipStream_t);
 
-void launch_scale_complex(const double *x, double yr, double yi, double *out, int64_t D,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(scale_complex_kernel, dim3(grid_for(D, 1 << 16)), dim3(kBlock), 0, s, x, yr, yi,
-                     out, D);
+template <typename T>
+void launch_scale_complex(const T *x, double yr, double yi, double *out, int64_t D, hipStream_t s) {
+  hipLaunchKernelGGL((scale_complex_kernel<T>), dim3(grid_for(D, 1 << 16)), dim3(kBlock), 0, s, x, yr,
+                     yi, out, D);
   MB_HIP(hipGetLastError());
 }
+template void launch_scale_complex<double>(const double *, double, double, double *, int64_t,
+                                           hipStream_t);
+template void launch_scale_complex<float>(const float *, double, double, double *, int64_t,
+                                          hipStream_t);
 
 void launch_reduce_cheb(const double *partial, int64_t rows_per_plane, int nvec, double *out,
                         hipStream_t s) {

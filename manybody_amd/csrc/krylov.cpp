@@ -133,8 +133,8 @@ double tridiag_min_eig(int m, const double *a, const double *b) {
 
 // ------------------------------------------------------------ Lanczos
 
-LanczosWork::LanczosWork(int64_t D, int k, int64_t nparts)
-    : D(D), k(k), v(D * 16), vp(D * 16), w(D * 16),
+LanczosWork::LanczosWork(int64_t D, int k, int64_t nparts, size_t esz)
+    : D(D), k(k), esz(esz), v(D * 2 * esz), vp(D * 2 * esz), w(D * 2 * esz),
       scal(sizeof(double) *
                (2 * (size_t)std::max(k, 1) + 2 + (size_t)std::max<int64_t>(kRedBlocks, nparts)) +
            16) {
@@ -146,13 +146,15 @@ LanczosWork::LanczosWork(int64_t D, int k, int64_t nparts)
   st.flags = reinterpret_cast<int *>(base + 2 * std::max(k, 1) + 1);
 }
 
-LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream_t s,
-                               LanczosWork *work) {
+// Pass one with Krylov vectors and H.v in T (double, or float for the FP32
+// Krylov option); b is FP64, the norm and the alpha/beta reductions are FP64.
+template <typename T>
+LanczosResult pass_one_t(const Hubbard &h, const void *b, int k, hipStream_t s, LanczosWork *work) {
   const int64_t D = h.dim();
   std::unique_ptr<LanczosWork> own;
   const HvTables t = tables_of(h);
-  if (!work || work->D != D || work->k < k) {
-    own.reset(new LanczosWork(D, k, hv_partials(t, 2)));
+  if (!work || work->D != D || work->k < k || work->esz != sizeof(T)) {
+    own.reset(new LanczosWork(D, k, hv_partials(t, 2), sizeof(T)));
     work = own.get();
   }
   LanczosWork &W = *work;
@@ -162,10 +164,10 @@ LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream
   MB_HIP(hipStreamSynchronize(s));
   if (res.b_norm <= kTol) throw std::runtime_error("Input vector has zero norm");
   if (k == 0) return res;
-  launch_scale_div((const double *)b, W.st.bnorm, W.v.as<double>(), 2 * D, s);
-  MB_HIP(hipMemsetAsync(W.vp.p, 0, (size_t)D * 16, s));
+  launch_scale_div<T>((const double *)b, W.st.bnorm, W.v.as<T>(), 2 * D, s);
+  MB_HIP(hipMemsetAsync(W.vp.p, 0, (size_t)D * 2 * sizeof(T), s));
   MB_HIP(hipMemsetAsync(W.st.flags, 0, 2 * sizeof(int), s));
-  double *v = W.v.as<double>(), *vp = W.vp.as<double>(), *w = W.w.as<double>();
+  T *v = W.v.as<T>(), *vp = W.vp.as<T>(), *w = W.w.as<T>();
   for (int i = 0; i < k; ++i) {
     EpiArgs a;
     a.prev = vp;
@@ -173,12 +175,12 @@ LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream
     a.flags = W.st.flags;
     a.step = i;
     a.partial = W.st.partial;
-    const int64_t slots = launch_hv<double>(t, 2, v, w, Epi::Lanczos, a, s);
+    const int64_t slots = launch_hv<T>(t, 2, v, w, Epi::Lanczos, a, s);
     launch_lanczos_finish_alpha(W.st, i, 2 * slots, s);
-    launch_lanczos_axpy_norm(v, w, 2 * D, W.st, i, s);
+    launch_lanczos_axpy_norm<T>(v, w, 2 * D, W.st, i, s);
     launch_lanczos_finish_beta(W.st, i, s);
-    launch_lanczos_normalize(w, 2 * D, W.st, i, s);
-    double *tmp = vp;
+    launch_lanczos_normalize<T>(w, 2 * D, W.st, i, s);
+    T *tmp = vp;
     vp = v;
     v = w;
     w = tmp;
@@ -193,6 +195,11 @@ LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream
   res.alphas.assign(al.begin(), al.begin() + res.steps);
   res.betas.assign(be.begin(), be.begin() + std::max(res.steps - 1, 0));
   return res;
+}
+
+LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream_t s,
+                               LanczosWork *work) {
+  return pass_one_t<double>(h, b, k, s, work);
 }
 
 // lanczos_pass_one with full reorthogonalisation (an option the reference does
@@ -282,9 +289,11 @@ double find_min_eigenpair_reorth(const Hubbard &h, const void *b, int k, void *x
   return wv[0];
 }
 
-void lanczos_pass_two(const Hubbard &h, const void *b, const LanczosResult &dec,
-                      const std::vector<std::complex<double>> &y, void *x, hipStream_t s,
-                      LanczosWork *work) {
+// Pass two: regenerates the Lanczos vectors (in T) and accumulates
+// x = sum_j y_j v_j in FP64.
+template <typename T>
+void pass_two_t(const Hubbard &h, const void *b, const LanczosResult &dec,
+                const std::vector<std::complex<double>> &y, void *x, hipStream_t s, LanczosWork *work) {
   const int64_t D = h.dim();
   if ((int)y.size() != dec.steps)
     throw std::runtime_error("Dimension mismatch: y_k size must match decomposition steps");
@@ -295,30 +304,36 @@ void lanczos_pass_two(const Hubbard &h, const void *b, const LanczosResult &dec,
   }
   const HvTables t = tables_of(h);
   std::unique_ptr<LanczosWork> own;
-  if (!work || work->D != D) {
-    own.reset(new LanczosWork(D, 1, hv_partials(t, 2)));
+  if (!work || work->D != D || work->esz != sizeof(T)) {
+    own.reset(new LanczosWork(D, 1, hv_partials(t, 2), sizeof(T)));
     work = own.get();
   }
   LanczosWork &W = *work;
   MB_HIP(hipMemcpyAsync(W.st.bnorm, &dec.b_norm, sizeof(double), hipMemcpyHostToDevice, s));
-  double *v = W.v.as<double>(), *vp = W.vp.as<double>(), *w = W.w.as<double>();
-  launch_scale_div((const double *)b, W.st.bnorm, v, 2 * D, s);
-  MB_HIP(hipMemsetAsync(vp, 0, (size_t)D * 16, s));
-  launch_scale_complex(v, y[0].real(), y[0].imag(), (double *)x, D, s);
+  T *v = W.v.as<T>(), *vp = W.vp.as<T>(), *w = W.w.as<T>();
+  launch_scale_div<T>((const double *)b, W.st.bnorm, v, 2 * D, s);
+  MB_HIP(hipMemsetAsync(vp, 0, (size_t)D * 2 * sizeof(T), s));
+  launch_scale_complex<T>(v, y[0].real(), y[0].imag(), (double *)x, D, s);
   for (int j = 0; j + 1 < dec.steps; ++j) {
     const double aj = dec.alphas[j], bj = dec.betas[j], bp = j == 0 ? 0.0 : dec.betas[j - 1];
     EpiArgs a;
     a.prev = vp;
     a.alpha = aj;
     a.beta = bp;
-    launch_hv<double>(t, 2, v, w, Epi::Recon, a, s);
-    launch_recon_update(w, (double *)x, bj, y[j + 1].real(), y[j + 1].imag(), D, s);
-    double *tmp = vp;
+    launch_hv<T>(t, 2, v, w, Epi::Recon, a, s);
+    launch_recon_update<T>(w, (double *)x, bj, y[j + 1].real(), y[j + 1].imag(), D, s);
+    T *tmp = vp;
     vp = v;
     v = w;
     w = tmp;
   }
   MB_HIP(hipStreamSynchronize(s));
+}
+
+void lanczos_pass_two(const Hubbard &h, const void *b, const LanczosResult &dec,
+                      const std::vector<std::complex<double>> &y, void *x, hipStream_t s,
+                      LanczosWork *work) {
+  pass_two_t<double>(h, b, dec, y, x, s, work);
 }
 
 double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipStream_t s) {
@@ -355,13 +370,13 @@ std::vector<std::complex<double>> expm_first_column(const LanczosResult &dec, do
   return y;
 }
 
-void time_evolve(const Hubbard &h, const void *psi, double t, int krylov_steps, void *out,
-                 hipStream_t s) {
+template <typename T>
+void time_evolve_t(const Hubbard &h, const void *psi, double t, int krylov_steps, void *out,
+                   hipStream_t s) {
   const int64_t D = h.dim();
-  std::vector<double> scratch;
   int k = krylov_steps;
   if (k == 0 || k > D) k = (int)D;
-  LanczosWork work(D, k, hv_partials(tables_of(h), 2));
+  LanczosWork work(D, k, hv_partials(tables_of(h), 2), sizeof(T));
   // norm first: a zero state is returned unchanged (evolve_state.h:131-134)
   double nrm = 0.0;
   launch_norm((const double *)psi, 2 * D, work.st.partial, work.st.bnorm, s);
@@ -372,7 +387,7 @@ void time_evolve(const Hubbard &h, const void *psi, double t, int krylov_steps, 
     MB_HIP(hipStreamSynchronize(s));
     return;
   }
-  LanczosResult dec = lanczos_pass_one(h, psi, k, s, &work);
+  LanczosResult dec = pass_one_t<T>(h, psi, k, s, &work);
   if (dec.steps == 0) {
     MB_HIP(hipMemsetAsync(out, 0, (size_t)D * 16, s));
     MB_HIP(hipStreamSynchronize(s));
@@ -382,16 +397,24 @@ void time_evolve(const Hubbard &h, const void *psi, double t, int krylov_steps, 
   for (auto &v : y) v *= dec.b_norm;
   if (out == psi) {
     DevBuf tmp((size_t)D * 16);
-    lanczos_pass_two(h, psi, dec, y, tmp.p, s, &work);
+    pass_two_t<T>(h, psi, dec, y, tmp.p, s, &work);
     MB_HIP(hipMemcpyAsync(out, tmp.p, (size_t)D * 16, hipMemcpyDeviceToDevice, s));
     MB_HIP(hipStreamSynchronize(s));
   } else {
-    lanczos_pass_two(h, psi, dec, y, out, s, &work);
+    pass_two_t<T>(h, psi, dec, y, out, s, &work);
   }
 }
 
+void time_evolve(const Hubbard &h, const void *psi, double t, int krylov_steps, void *out,
+                 hipStream_t s, bool fp32) {
+  if (fp32)
+    time_evolve_t<float>(h, psi, t, krylov_steps, out, s);
+  else
+    time_evolve_t<double>(h, psi, t, krylov_steps, out, s);
+}
+
 void time_evolve_steps(const Hubbard &h, const void *psi, double t0, double t1, double dt,
-                       int krylov_steps, void *out, hipStream_t s) {
+                       int krylov_steps, void *out, hipStream_t s, bool fp32) {
   const int64_t D = h.dim();
   if (out != psi) MB_HIP(hipMemcpyAsync(out, psi, (size_t)D * 16, hipMemcpyDeviceToDevice, s));
   if (dt == 0.0 || (t1 - t0) == 0.0) {
@@ -403,7 +426,7 @@ void time_evolve_steps(const Hubbard &h, const void *psi, double t0, double t1, 
   double t = t0;
   while ((direction > 0 && t < t1) || (direction < 0 && t > t1)) {
     const double step = (direction > 0) ? std::min(dt, t1 - t) : std::max(dt, t1 - t);
-    time_evolve(h, out, step, krylov_steps, out, s);
+    time_evolve(h, out, step, krylov_steps, out, s, fp32);
     t += step;
   }
 }

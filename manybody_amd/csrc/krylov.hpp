@@ -21,9 +21,10 @@ struct LanczosResult {
 struct LanczosWork {
   int64_t D;
   int k;
+  size_t esz;  // bytes per vector element: 8 (FP64) or 4 (FP32 Krylov vectors)
   DevBuf v, vp, w, scal;
   LanczosState st;
-  LanczosWork(int64_t D, int k, int64_t nparts);
+  LanczosWork(int64_t D, int k, int64_t nparts, size_t esz = sizeof(double));
 };
 
 LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream_t s,
@@ -38,10 +39,12 @@ LanczosResult lanczos_pass_one_reorth(const Hubbard &h, const void *b, int k, hi
                                       DevBuf *basis = nullptr);
 double find_min_eigenpair_reorth(const Hubbard &h, const void *b, int k, void *x, hipStream_t s);
 std::vector<std::complex<double>> expm_first_column(const LanczosResult &dec, double t);
+// fp32: Krylov vectors and H.v in FP32; the state, norms, alpha/beta
+// reductions and the exponential of T_k stay FP64.
 void time_evolve(const Hubbard &h, const void *psi, double t, int krylov_steps, void *out,
-                 hipStream_t s);
+                 hipStream_t s, bool fp32 = false);
 void time_evolve_steps(const Hubbard &h, const void *psi, double t0, double t1, double dt,
-                       int krylov_steps, void *out, hipStream_t s);
+                       int krylov_steps, void *out, hipStream_t s, bool fp32 = false);
 void kpm_dos_moments(const Hubbard &h, const Sector &sec, double a, double b, int M, int nvec,
                      uint64_t seed, bool fp32, double *mu, hipStream_t s);
 

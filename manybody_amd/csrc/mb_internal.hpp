@@ -267,18 +267,21 @@ void launch_reorth_sub(const double *V, int m, const double *c, double *w, int64
 // partial holds 2 kRedBlocks doubles
 void launch_cdot(const double *a, const double *b, int64_t D, double *partial, double *out,
                  hipStream_t s);
-void launch_scale_div(const double *x, const double *den, double *y, int64_t n, hipStream_t s);
+template <typename T>
+void launch_scale_div(const double *x, const double *den, T *y, int64_t n, hipStream_t s);
 void launch_lanczos_finish_alpha(const LanczosState &st, int step, int64_t nparts, hipStream_t s);
-void launch_lanczos_axpy_norm(const double *v, double *w, int64_t n, const LanczosState &st,
-                              int step, hipStream_t s);
-void launch_lanczos_finish_beta(const LanczosState &st, int step, hipStream_t s);
-void launch_lanczos_normalize(double *w, int64_t n, const LanczosState &st, int step,
+template <typename T>
+void launch_lanczos_axpy_norm(const T *v, T *w, int64_t n, const LanczosState &st, int step,
                               hipStream_t s);
+void launch_lanczos_finish_beta(const LanczosState &st, int step, hipStream_t s);
+template <typename T>
+void launch_lanczos_normalize(T *w, int64_t n, const LanczosState &st, int step, hipStream_t s);
 // pass two: w /= beta ; x += y w (complex y, planar vectors of D elements)
-void launch_recon_update(double *w, double *x, double beta, double yr, double yi, int64_t D,
+template <typename T>
+void launch_recon_update(T *w, double *x, double beta, double yr, double yi, int64_t D,
                          hipStream_t s);
-void launch_scale_complex(const double *x, double yr, double yi, double *out, int64_t D,
-                          hipStream_t s);
+template <typename T>
+void launch_scale_complex(const T *x, double yr, double yi, double *out, int64_t D, hipStream_t s);
 // out[r] = sum over row workgroups of planes 2r, 2r+1 of partial[2 wg + j] for j = 0,
 // out[nvec + r] likewise for j = 1.
 void launch_reduce_cheb(const double *partial, int64_t rows_per_plane, int nvec, double *out,
