@@ -123,6 +123,7 @@ def main():
     del y
     batched = None if args.no_batched else bench_batched(args.batched_n)
     kpm = None if args.no_kpm else bench_kpm_c3()
+    krylov = None if args.no_kpm else bench_krylov_fp32()
     serving = None if args.no_serving else bench_serving(args.serving_n, args.serving_rate)
 
     line = {
@@ -147,6 +148,7 @@ def main():
         "lanczos_4x4": lanczos,
         "batched": batched,
         "kpm_l14": kpm,
+        "krylov_l14": krylov,
         "serving": serving,
     }
     print(json.dumps(line))
@@ -298,6 +300,33 @@ def bench_kpm_c3(M=1024, R=16, seed=7):
             "max_abs_dmu_fp32_vs_fp64": float(np.abs(mu32 - mu64).max()),
             "dos_rel_L1_fp32_vs_fp64": l1, "mu0": float(mu64[0]),
             "max_abs_mu": float(np.abs(mu64).max())}
+
+
+def bench_krylov_fp32(t1=1.0, dt=0.05, k=30):
+    """Krylov exp(-iHt) on the 1D ring L = 14, 7 + 7, U = 4 (D = 11,778,624):
+    normalised seeded start, t = 0 -> 1 in steps of 0.05, Krylov dimension 30,
+    Krylov vectors and H.v in FP64 and in FP32 (state, reductions and the
+    exponential stay FP64); the FP32 error against the FP64 run is stated."""
+    import torch
+
+    import manybody_amd as mb
+    op = mb.HubbardModel(1.0, 4.0, 14).operator(mb.FermionBasis.sector(14, 7, 7))
+    psi = op.gaussian(21)
+    psi = psi / torch.linalg.vector_norm(psi)
+    opts = mb.EvolutionOptions(k)
+    out = {}
+    for fp32 in (False, True):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out[fp32] = (mb.time_evolve_state_steps(op, psi, 0.0, t1, dt, opts, fp32=fp32),
+                     time.perf_counter() - t0)
+    err = (torch.linalg.vector_norm(out[True][0] - out[False][0]) /
+           torch.linalg.vector_norm(out[False][0])).item()
+    ov = torch.sum(psi[0] * out[False][0][0] + psi[1] * out[False][0][1]).item()
+    return {"workload": "Krylov exp(-iHt), 1D ring L=14 7+7 U=4, t=1, dt=0.05, k=30",
+            "D": op.dimension(), "fp64_s": round(out[False][1], 3), "fp32_s": round(out[True][1], 3),
+            "rel_state_err_fp32_vs_fp64": err, "re_overlap_psi0_psit": ov,
+            "norm_fp32": torch.linalg.vector_norm(out[True][0]).item()}
 
 
 def bench_serving(n, rate, slots=4, arena=64 << 30, lane=8 << 30):
