@@ -221,14 +221,18 @@ struct LanczosDecomposition {
   double b_norm = 0.0;
 };
 
+// reorthogonalize: full reorthogonalisation (an option; all k + 1 vectors kept in HBM).
 inline LanczosDecomposition lanczos_pass_one(const HubbardOperator &op, const DeviceVector &b,
-                                             size_t k, hipStream_t s = nullptr) {
+                                             size_t k, hipStream_t s = nullptr,
+                                             bool reorthogonalize = false) {
   if (op.dimension() != b.size())
     throw std::runtime_error("Operator dimension does not match input vector");
   std::vector<double> al(k ? k : 1), be(k ? k : 1);
   int steps = 0;
   double bn = 0.0;
-  check(mb_lanczos_pass_one(op.handle(), b.data(), (int)k, al.data(), be.data(), &steps, &bn, s));
+  check(mb_lanczos_pass_one_ex(op.handle(), b.data(), (int)k,
+                               reorthogonalize ? MB_LANCZOS_REORTHOGONALIZE : 0, al.data(), be.data(),
+                               &steps, &bn, s));
   LanczosDecomposition d;
   d.alphas.assign(al.begin(), al.begin() + steps);
   d.betas.assign(be.begin(), be.begin() + (steps > 0 ? steps - 1 : 0));
@@ -245,20 +249,26 @@ struct EigenPair {
 // find_min_eigenpair(op, k) with an explicit start vector (the reference seeds
 // it with arma::randu internally).
 inline EigenPair find_min_eigenpair(const HubbardOperator &op, size_t k, const DeviceVector &b,
-                                    hipStream_t s = nullptr) {
+                                    hipStream_t s = nullptr, bool reorthogonalize = false) {
   EigenPair ep{0.0, DeviceVector(op.dimension())};
-  check(mb_find_min_eigenpair(op.handle(), b.data(), (int)k, &ep.value, ep.vector.data(), s));
+  check(mb_find_min_eigenpair_ex(op.handle(), b.data(), (int)k,
+                                 reorthogonalize ? MB_LANCZOS_REORTHOGONALIZE : 0, &ep.value,
+                                 ep.vector.data(), s));
   return ep;
 }
 
+// EvolutionOptions (evolve_state.h:110-112); fp32 is an option of this library:
+// Krylov vectors and H.v in FP32, state and reductions in FP64.
 struct EvolutionOptions {
   size_t krylov_steps = 30;
+  bool fp32 = false;
 };
 
 inline DeviceVector time_evolve_state(const HubbardOperator &H, const DeviceVector &psi, double t,
                                       EvolutionOptions opts = {}, hipStream_t s = nullptr) {
   DeviceVector out(H.dimension());
-  check(mb_time_evolve(H.handle(), psi.data(), t, (int)opts.krylov_steps, out.data(), s));
+  check(mb_time_evolve_ex(H.handle(), psi.data(), t, (int)opts.krylov_steps,
+                          opts.fp32 ? MB_EVOLVE_FP32 : 0, out.data(), s));
   return out;
 }
 
@@ -266,10 +276,39 @@ inline DeviceVector time_evolve_state_steps(const HubbardOperator &H, const Devi
                                             double t0, double t1, double dt,
                                             EvolutionOptions opts = {}, hipStream_t s = nullptr) {
   DeviceVector out(H.dimension());
-  check(mb_time_evolve_steps(H.handle(), psi.data(), t0, t1, dt, (int)opts.krylov_steps,
-                             out.data(), s));
+  check(mb_time_evolve_steps_ex(H.handle(), psi.data(), t0, t1, dt, (int)opts.krylov_steps,
+                                opts.fp32 ? MB_EVOLVE_FP32 : 0, out.data(), s));
   return out;
 }
+
+// Serving loop: requests admitted as they arrive and as slots and device
+// memory free up (mb_server_*).
+class Server {
+ public:
+  Server(int slots, int64_t arena_bytes, int64_t lane_bytes = 0) {
+    check(mb_server_create(slots, arena_bytes, lane_bytes, &h_));
+  }
+  ~Server() { mb_server_destroy(h_); }
+  Server(const Server &) = delete;
+  Server &operator=(const Server &) = delete;
+  int64_t submit(const mb_request &r) {
+    int64_t id = -1;
+    check(mb_server_submit(h_, &r, &id));
+    return id;
+  }
+  // Blocks until request id has finished; KPM moments go to *moments.
+  mb_result wait(int64_t id, std::vector<double> *moments = nullptr) {
+    mb_result r;
+    std::vector<double> mu(moments ? 4096 : 0);
+    check(mb_server_wait(h_, id, &r, moments ? mu.data() : nullptr, (int)mu.size()));
+    if (moments) moments->assign(mu.begin(), mu.begin() + r.n_moments);
+    return r;
+  }
+  void drain() { check(mb_server_drain(h_)); }
+
+ private:
+  mb_server *h_ = nullptr;
+};
 
 namespace kpm {
 inline std::vector<double> jackson_kernel(size_t N) {

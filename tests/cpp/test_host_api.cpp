@@ -101,5 +101,43 @@ int main() {
     for (double m : mu) CHECK(std::abs(m) <= 1.0 + 1e-6);
   }
   std::printf(g_fail ? "FAILED (%d)\n" : "ALL PASSED\n", g_fail);
+  {  // reorthogonalised Lanczos and FP32 time evolution through the C++ layer
+    auto basis = mbamd::FermionBasis::with_fixed_particle_number_and_spin(8, 8, 0);
+    auto H = mbamd::HubbardModel(1.0, 0.0, 8).op(basis);
+    auto b = H.gaussian(7);
+    auto ep = mbamd::find_min_eigenpair(H, 120, b, nullptr, /*reorthogonalize=*/true);
+    CHECK(std::abs(ep.value + 9.656854249492380) < 1e-9);
+    auto d = mbamd::lanczos_pass_one(H, b, 60, nullptr, true);
+    CHECK(d.steps_taken == 60);
+    mbamd::EvolutionOptions o;
+    o.fp32 = true;
+    auto psi = mbamd::time_evolve_state(H, b, 0.5, o);
+    auto h64 = b.to_host(), h32 = psi.to_host();
+    double n0 = 0.0, n1 = 0.0;
+    for (size_t i = 0; i < h64.size(); ++i) {
+      n0 += std::norm(h64[i]);
+      n1 += std::norm(h32[i]);
+    }
+    CHECK(std::abs(std::sqrt(n1) - std::sqrt(n0)) / std::sqrt(n0) < 1e-6);
+  }
+  {  // serving loop: a ground-state request through the C++ layer
+    mbamd::Server srv(2, int64_t(1) << 30, int64_t(256) << 20);
+    mb_request r{};
+    r.kind = MB_REQ_GROUND;
+    r.ndim = 1;
+    r.sizes[0] = 8;
+    r.n_up = 4;
+    r.n_dn = 4;
+    r.t = 1.0;
+    r.u = 0.0;
+    r.steps = 150;
+    r.seed = 3;
+    r.arrival = -1.0;
+    const mb_result res = srv.wait(srv.submit(r));
+    CHECK(res.status == MB_OK);
+    CHECK(std::abs(res.e0 + 9.656854249492380) < 1e-9);
+    CHECK(res.dim == 4900);
+  }
+
   return g_fail ? 1 : 0;
 }
