@@ -120,6 +120,7 @@ def main():
     if not args.no_lanczos:
         lanczos = bench_lanczos_c2(op, x, basis, (Lx, Ly, nu, nd, t, U), args.c2_cpu_energy,
                                    args.cpu_threads)
+    observables = None if args.no_lanczos else bench_observables(op, x)
     del y
     batched = None if args.no_batched else bench_batched(args.batched_n)
     kpm = None if args.no_kpm else bench_kpm_c3()
@@ -146,6 +147,7 @@ def main():
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "lanczos_4x4": lanczos,
+        "observables_4x4": observables,
         "batched": batched,
         "kpm_l14": kpm,
         "krylov_l14": krylov,
@@ -254,6 +256,32 @@ def bench_lanczos_c2(op, x, basis, model, cpu_energy, cpu_threads, k=300):
         out.update({"E0_cpu_oracle": e_cpu, "cpu_s": round(t_cpu, 1), "cpu_threads": threads,
                     "rel_dE": abs(out["E0"] - e_cpu) / abs(e_cpu)})
     return out
+
+
+def bench_observables(op, x, reps=5):
+    """Host-synchronous observables of the bench vector on 4x4 7+7: all 16
+    opposite-spin correlations G(r) in one pass (16 D algorithmic bytes: x read
+    once) and the (energy, residual) pair (one H.v plus two passes over x and
+    Hx).  Median wall time of `reps` calls each, tables and allocation included."""
+    import numpy as np
+    import torch
+    torch.cuda.synchronize()
+    D = op.dimension()
+    op.opposite_spin_correlation(x)
+    op.energy_residual(x)
+    tc, te = [], []
+    for _ in range(reps):
+        c0 = time.perf_counter()
+        g = op.opposite_spin_correlation(x)
+        tc.append(time.perf_counter() - c0)
+        c0 = time.perf_counter()
+        e, r = op.energy_residual(x)
+        te.append(time.perf_counter() - c0)
+    tc, te = float(np.median(tc)), float(np.median(te))
+    return {"correlation_ms": round(tc * 1e3, 3),
+            "correlation_GBps": round(16.0 * D / tc / 1e9, 1),
+            "sum_rule_error": abs(float(g.sum()) - 49 / 16),
+            "energy_residual_ms": round(te * 1e3, 3), "energy": e, "residual": r}
 
 
 def bench_kpm_c3(M=1024, R=16, seed=7):

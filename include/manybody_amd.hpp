@@ -152,10 +152,12 @@ class FermionBasis {
 class HubbardOperator {
  public:
   using ScalarType = std::complex<double>;
-  HubbardOperator(const FermionBasis &basis, const std::vector<int> &sizes, double t, double u) {
+  HubbardOperator(const FermionBasis &basis, const std::vector<int> &sizes, double t, double u)
+      : sites_(basis.orbitals) {
     check(mb_hubbard_create(basis.handle(), (int)sizes.size(), sizes.data(), t, u, &h_));
   }
-  HubbardOperator(HubbardOperator &&o) noexcept : h_(std::exchange(o.h_, nullptr)) {}
+  HubbardOperator(HubbardOperator &&o) noexcept
+      : h_(std::exchange(o.h_, nullptr)), sites_(o.sites_) {}
   HubbardOperator(const HubbardOperator &) = delete;
   ~HubbardOperator() { mb_hubbard_destroy(h_); }
 
@@ -185,10 +187,25 @@ class HubbardOperator {
     check(mb_fill_gaussian_c128(h_, seed, y.data(), s));
     return y;
   }
+  // HubbardModel*::opposite_spin_correlation (hubbard_model.h:40-53) for every
+  // displacement r at once: g[r] = <x|G_updn(r)|x> / <x|x>, x in device order
+  std::vector<double> opposite_spin_correlation(const DeviceVector &x,
+                                                hipStream_t s = nullptr) const {
+    std::vector<double> g(sites_);
+    check(mb_opposite_spin_correlation(h_, x.data(), g.data(), s));
+    return g;
+  }
+  // (<x|H|x> / <x|x>, |Hx - E x| / |x|)
+  std::pair<double, double> energy_residual(const DeviceVector &x, hipStream_t s = nullptr) const {
+    double e = 0.0, r = 0.0;
+    check(mb_energy_residual(h_, x.data(), &e, &r, s));
+    return {e, r};
+  }
   const mb_hubbard *handle() const { return h_; }
 
  private:
   mb_hubbard *h_ = nullptr;
+  size_t sites_ = 0;
 };
 
 struct HubbardModel {

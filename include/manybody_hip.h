@@ -215,6 +215,20 @@ int mb_jackson_kernel(int N, double *g);
 /* rescaling::from_bounds(E_min, E_max, padding) — rescaling.h:33-38. */
 int mb_rescaling_from_bounds(double e_min, double e_max, double padding, double *a, double *b);
 
+/* ------------------------------------------------------------ observables */
+
+/* <x|G_updn(r)|x> / <x|x> for every lattice displacement r, written to g[r]
+ * (r = rx + Lx ry + Lx Ly rz; one value per site), where
+ * G_updn(r) = (1/N) sum_i n_{i,up} n_{i+r,dn} is the reference's
+ * HubbardModel*::opposite_spin_correlation (hubbard_model.h:40-53, 105-120,
+ * 184-200).  x in device order.  MB_ERR_INVALID for a model made from a bond
+ * list (no lattice). */
+int mb_opposite_spin_correlation(const mb_hubbard *h, const void *x_dev, double *g,
+                                 mb_stream stream);
+/* Rayleigh quotient E = <x|H|x>/<x|x> and residual |Hx - E x| / |x| (x in device order). */
+int mb_energy_residual(const mb_hubbard *h, const void *x_dev, double *energy, double *residual,
+                       mb_stream stream);
+
 /* ------------------------------------------------------------ batched solves */
 
 /* Many independent Lanczos solves advanced in lockstep by one launch per phase

@@ -92,6 +92,8 @@ _SIGS = {
     "mb_time_evolve_ex": (C.c_int, [_vp, _vp, C.c_double, C.c_int, C.c_int, _vp, _vp]),
     "mb_time_evolve_steps_ex": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_int,
                                           C.c_int, _vp, _vp]),
+    "mb_opposite_spin_correlation": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mb_energy_residual": (C.c_int, [_vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_double), _vp]),
     "mb_server_create": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.POINTER(_vp)]),
     "mb_server_destroy": (None, [_vp]),
     "mb_server_submit": (C.c_int, [_vp, _vp, C.POINTER(C.c_int64)]),
@@ -311,6 +313,21 @@ class HubbardOperator:
         work = self.empty(nvec=2)
         _check(lib().mb_apply_ref_c128(self._h, _ptr(x_ref), _ptr(y), _ptr(work), _stream()))
         return y
+
+    def opposite_spin_correlation(self, x):
+        """<x|G_updn(r)|x>/<x|x> for every displacement r (hubbard_model.h:40-53);
+        x in device order.  Returns a numpy array of one value per site."""
+        import numpy as np
+        g = np.zeros(self.basis.orbitals)
+        _check(lib().mb_opposite_spin_correlation(self._h, _ptr(x), g.ctypes.data_as(C.c_void_p),
+                                                  _stream()))
+        return g
+
+    def energy_residual(self, x):
+        """(<x|H|x>/<x|x>, |Hx - E x|/|x|) for x in device order."""
+        e, r = C.c_double(), C.c_double()
+        _check(lib().mb_energy_residual(self._h, _ptr(x), C.byref(e), C.byref(r), _stream()))
+        return e.value, r.value
 
     def gaussian(self, seed: int):
         """Device-order complex vector (2, D), re/im ~ N(0,1) per reference index."""

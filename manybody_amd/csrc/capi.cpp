@@ -7,6 +7,7 @@
 #include "../../include/manybody_hip.h"
 #include "batch.hpp"
 #include "serve.hpp"
+#include "observables.hpp"
 #include "krylov.hpp"
 #include "mb_internal.hpp"
 
@@ -136,7 +137,7 @@ int mb_basis_product_index(const mb_basis *b, int64_t first, int64_t n, int64_t 
 }
 
 static int hubbard_common(const mb_basis *b, std::vector<int> from, std::vector<int> to, double t,
-                          double u, mb_hubbard **out) {
+                          double u, mb_hubbard **out, int ndim = 0, const int *sizes = nullptr) {
   return guarded([&] {
     need(b && out, "null argument");
     *out = nullptr;
@@ -150,6 +151,8 @@ static int hubbard_common(const mb_basis *b, std::vector<int> from, std::vector<
     h->h.U = u;
     h->h.bond_from = std::move(from);
     h->h.bond_to = std::move(to);
+    h->h.ndim = ndim;
+    for (int i = 0; i < ndim; ++i) h->h.lsize[i] = sizes[i];
     const_cast<mb::Sector &>(b->sec).ensure_device();
     mb::build_hubbard(h->h);
     *out = h.release();
@@ -171,7 +174,7 @@ int mb_hubbard_create(const mb_basis *b, int ndim, const int *sizes, double t, d
     mb::lattice_bonds(ndim, sizes, from, to);
   });
   if (st != MB_OK) return st;
-  return hubbard_common(b, std::move(from), std::move(to), t, u, out);
+  return hubbard_common(b, std::move(from), std::move(to), t, u, out, ndim, sizes);
 }
 
 int mb_hubbard_create_bonds(const mb_basis *b, int n_bonds, const int *from, const int *to,
@@ -567,4 +570,23 @@ int64_t mb_request_footprint(const mb_request *r) {
   } catch (...) {
     return -1;
   }
+}
+
+// ------------------------------------------------------------ observables
+
+int mb_opposite_spin_correlation(const mb_hubbard *h, const void *x_dev, double *g,
+                                 mb_stream stream) {
+  return guarded([&] {
+    need(h && x_dev && g, "null argument");
+    const auto v = mb::opposite_spin_correlation(H(h), x_dev, S(stream));
+    for (size_t i = 0; i < v.size(); ++i) g[i] = v[i];
+  });
+}
+
+int mb_energy_residual(const mb_hubbard *h, const void *x_dev, double *energy, double *residual,
+                       mb_stream stream) {
+  return guarded([&] {
+    need(h && x_dev && energy && residual, "null argument");
+    mb::energy_residual(H(h), x_dev, energy, residual, S(stream));
+  });
 }

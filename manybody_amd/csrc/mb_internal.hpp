@@ -162,6 +162,8 @@ struct Hubbard {
   const Sector *sec = nullptr;
   double t = 0, U = 0;
   std::vector<int> bond_from, bond_to;
+  int ndim = 0;                 // periodic lattice the bonds came from (0: a bond list)
+  int lsize[3] = {1, 1, 1};
   SpeciesHops up, dn;
   bool has_onsite = false;
   std::vector<double> diagU;  // interaction energy of k double occupations

@@ -184,6 +184,8 @@ void run_request(const ServeRequest &r, ServeResult &res, hipStream_t s) {
   h.t = r.t;
   h.U = r.u;
   lattice_bonds(r.ndim, r.sizes, h.bond_from, h.bond_to);
+  h.ndim = r.ndim;
+  for (int i = 0; i < r.ndim; ++i) h.lsize[i] = r.sizes[i];
   build_hubbard(h);
   const int64_t D = sec.dim();
   res.dim = D;

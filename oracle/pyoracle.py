@@ -286,3 +286,41 @@ def jackson_kernel(N: int) -> np.ndarray:
     g = np.zeros(N + 1)
     lib().orc_jackson_kernel(N, g)
     return g
+
+
+def lattice_shift_sites(sizes, r: int) -> np.ndarray:
+    """site i -> site i + r on the periodic lattice (site = x + Lx y + Lx Ly z,
+    index.h:82-103; r is the flattened displacement rx + Lx ry + Lx Ly rz)."""
+    dims = list(sizes) + [1] * (3 - len(sizes))
+    Lx, Ly, Lz = dims
+    rx, ry, rz = r % Lx, (r // Lx) % Ly, r // (Lx * Ly)
+    out = np.zeros(Lx * Ly * Lz, np.int64)
+    for z in range(Lz):
+        for y in range(Ly):
+            for x in range(Lx):
+                out[x + Lx * y + Lx * Ly * z] = ((x + rx) % Lx) + Lx * ((y + ry) % Ly) + \
+                    Lx * Ly * ((z + rz) % Lz)
+    return out
+
+
+def opposite_spin_correlation_diag(sizes, n_up: int, n_dn: int) -> np.ndarray:
+    """Diagonal of G_updn(r) = (1/N) sum_i n_{i,up} n_{i+r,dn}
+    (hubbard_model.h:40-53, 105-120, 184-200) for every displacement r, in
+    reference index order: array [N, D]."""
+    N = int(np.prod(sizes))
+    up, dn = basis_states(N, n_up, n_dn)
+    up = np.asarray(up, np.uint64)
+    dn = np.asarray(dn, np.uint64)
+    out = np.zeros((N, len(up)))
+    for r in range(N):
+        m = lattice_shift_sites(sizes, r)
+        for i in range(N):
+            out[r] += ((up >> np.uint64(i)) & np.uint64(1)).astype(np.float64) * \
+                ((dn >> np.uint64(m[i])) & np.uint64(1)).astype(np.float64)
+    return out / N
+
+
+def opposite_spin_correlation(sizes, n_up: int, n_dn: int, x_ref: np.ndarray) -> np.ndarray:
+    """<x|G_updn(r)|x> / <x|x> for every displacement r."""
+    p = np.abs(np.asarray(x_ref)) ** 2
+    return opposite_spin_correlation_diag(sizes, n_up, n_dn) @ p / p.sum()

@@ -56,10 +56,9 @@ void print_states(const FermionBasis& basis) {
   }
 }
 
-template <typename Model>
-void print_matrix(const FermionBasis& basis, const Model& model) {
-  auto H = model.hamiltonian();
-  DenseComplex M = compute_matrix_elements<DenseComplex>(basis, H);
+template <typename Expr>
+void print_operator(const FermionBasis& basis, const Expr& A) {
+  DenseComplex M = compute_matrix_elements<DenseComplex>(basis, A);
   for (size_t i = 0; i < M.n_rows; ++i)
     for (size_t j = 0; j < M.n_cols; ++j) {
       const auto v = M(i, j);
@@ -68,8 +67,13 @@ void print_matrix(const FermionBasis& basis, const Model& model) {
     }
 }
 
+template <typename Model>
+void print_matrix(const FermionBasis& basis, const Model& model) {
+  print_operator(basis, model.hamiltonian());
+}
+
 int usage() {
-  std::fprintf(stderr, "usage: ref_driver basis|h1d|h2d|h3d ...\n");
+  std::fprintf(stderr, "usage: ref_driver basis|h1d|h2d|h3d|g1d|g2d|g3d ...\n");
   return 2;
 }
 
@@ -92,12 +96,19 @@ int main(int argc, char** argv) {
   } else if (mode == "h3d" && a.size() == 7) {
     sites = (size_t)a[0] * (size_t)a[1] * (size_t)a[2];
     nup = (size_t)a[3]; ndn = (size_t)a[4]; t = a[5]; u = a[6];
+  } else if (mode == "g1d" && a.size() == 4) {  // L nup ndn r
+    sites = (size_t)a[0]; nup = (size_t)a[1]; ndn = (size_t)a[2];
+  } else if (mode == "g2d" && a.size() == 6) {  // Lx Ly nup ndn rx ry
+    sites = (size_t)a[0] * (size_t)a[1]; nup = (size_t)a[2]; ndn = (size_t)a[3];
+  } else if (mode == "g3d" && a.size() == 8) {  // Lx Ly Lz nup ndn rx ry rz
+    sites = (size_t)a[0] * (size_t)a[1] * (size_t)a[2]; nup = (size_t)a[3]; ndn = (size_t)a[4];
   } else {
     return usage();
   }
   const size_t n = nup + ndn;
   if (n > 7) { std::fprintf(stderr, "sector exceeds the 7-operator key\n"); return 3; }
-  if (mode != "basis" && n > (u == 0.0 ? 5u : 3u)) {
+  const bool corr = mode[0] == 'g';  // density-density operator: the interaction's range
+  if (mode != "basis" && n > ((u == 0.0 && !corr) ? 5u : 3u)) {
     std::fprintf(stderr, "sector exceeds the exact range of compute_matrix_elements\n");
     return 3;
   }
@@ -108,5 +119,13 @@ int main(int argc, char** argv) {
   if (mode == "h2d") print_matrix(basis, HubbardModel2D(t, u, (size_t)a[0], (size_t)a[1]));
   if (mode == "h3d")
     print_matrix(basis, HubbardModel3D(t, u, (size_t)a[0], (size_t)a[1], (size_t)a[2]));
+  // G_updn(r) = (1/N) sum_i n_{i,up} n_{i+r,dn} (hubbard_model.h:40-53, 105-120, 184-200)
+  if (mode == "g1d") print_operator(basis, HubbardModel(1.0, 0.0, sites).opposite_spin_correlation((size_t)a[3]));
+  if (mode == "g2d")
+    print_operator(basis, HubbardModel2D(1.0, 0.0, (size_t)a[0], (size_t)a[1])
+                              .opposite_spin_correlation((size_t)a[4], (size_t)a[5]));
+  if (mode == "g3d")
+    print_operator(basis, HubbardModel3D(1.0, 0.0, (size_t)a[0], (size_t)a[1], (size_t)a[2])
+                              .opposite_spin_correlation((size_t)a[5], (size_t)a[6], (size_t)a[7]));
   return 0;
 }

@@ -66,6 +66,14 @@ int main() {
       n += std::norm(x[i]);
     }
     CHECK(std::sqrt(r / n) < 1e-6);
+    // the same on the device, and the correlation sum rule sum_r G(r) = N_up N_dn / N
+    auto er = H.energy_residual(ep.vector);
+    CHECK(std::abs(er.first - ep.value) < 1e-10 * std::abs(ep.value));
+    CHECK(std::abs(er.second - std::sqrt(r / n)) < 1e-8);
+    auto g = H.opposite_spin_correlation(ep.vector);
+    double gs = 0;
+    for (double v : g) gs += v;
+    CHECK(g.size() == 8 && std::abs(gs - 2.0) < 1e-12);
   }
   {
     CASE("lanczos_decomposition_shape");

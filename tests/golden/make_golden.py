@@ -11,6 +11,8 @@ dropped long products, expression_base.h:184-186).
 Run in the build container only (the reference does not exist on the GPU box):
     python tests/golden/make_golden.py
 Outputs: tests/golden/ref_basis.npz, tests/golden/ref_hamiltonian.npz,
+tests/golden/ref_correlation.npz (diagonal of the opposite-spin correlation
+operator G_updn(r) of hubbard_model.h, every displacement r, N_up + N_dn <= 3),
 tests/golden/manifest.json.
 """
 from __future__ import annotations
@@ -106,13 +108,42 @@ def main():
         ham_manifest.append({"name": name, "kind": kind, "dims": dims, "n_up": nu, "n_dn": nd,
                              "t": t, "U": U, "dim": int(len(st)), "nnz": int(len(el))})
 
+    # G_updn(r): every displacement of each lattice; the operator is diagonal in
+    # the occupation basis, so its diagonal (reference index order) is stored
+    corr_cases = [("g1d", [4], 2, 1), ("g1d", [5], 1, 2), ("g1d", [6], 2, 1), ("g1d", [7], 1, 1),
+                  ("g2d", [2, 2], 1, 1), ("g2d", [3, 2], 2, 1), ("g2d", [2, 3], 1, 2),
+                  ("g3d", [2, 2, 2], 1, 1)]
+    corr_arrays, corr_manifest = {}, []
+    for kind, dims, nu, nd in corr_cases:
+        N = int(np.prod(dims))
+        diags = []
+        for r in range(N):
+            rr = [r % dims[0]] + ([(r // dims[0]) % dims[1]] if len(dims) > 1 else []) + \
+                 ([r // (dims[0] * dims[1])] if len(dims) > 2 else [])
+            st, el = run([kind] + dims + [nu, nd] + rr)
+            D = len(st)
+            g = np.zeros(D)
+            for i, j, re, im in el:
+                if i != j or im != 0.0:
+                    raise SystemExit(f"{kind} {dims} r={rr}: off-diagonal or complex element")
+                g[int(i)] = re
+            diags.append(g)
+        name = f"{kind}_{'x'.join(map(str, dims))}_{nu}_{nd}"
+        corr_arrays[name + "__states"] = st
+        corr_arrays[name + "__diag"] = np.array(diags)  # [r, reference index]
+        corr_manifest.append({"name": name, "kind": kind, "dims": dims, "n_up": nu, "n_dn": nd,
+                              "dim": int(len(st)), "displacements": N})
+
     np.savez_compressed(os.path.join(HERE, "ref_basis.npz"), **basis_arrays)
+    np.savez_compressed(os.path.join(HERE, "ref_correlation.npz"), **corr_arrays)
     np.savez_compressed(os.path.join(HERE, "ref_hamiltonian.npz"), **ham_arrays)
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump({"generator": "tests/golden/make_golden.py",
                    "source": "reference algebra core compiled by oracle/Makefile (target ref)",
-                   "basis": basis_manifest, "hamiltonian": ham_manifest}, f, indent=1)
-    print(f"{len(basis_manifest)} basis sectors, {len(ham_manifest)} Hamiltonians")
+                   "basis": basis_manifest, "hamiltonian": ham_manifest,
+                   "correlation": corr_manifest}, f, indent=1)
+    print(f"{len(basis_manifest)} basis sectors, {len(ham_manifest)} Hamiltonians, "
+          f"{len(corr_manifest)} correlation operators")
 
 
 if __name__ == "__main__":

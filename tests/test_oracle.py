@@ -275,3 +275,29 @@ def test_jackson_kernel_properties_and_closed_form():
     n = np.arange(N + 1)
     ref = ((N + 1 - n) * np.cos(np.pi * n / (N + 1)) + np.sin(np.pi * n / (N + 1)) / np.tan(np.pi / (N + 1))) / (N + 1)
     assert np.abs(g - ref).max() < 1e-13
+
+
+# ------------------------------------------------------------------ correlation
+
+def test_opposite_spin_correlation_matches_reference_golden():
+    """G_updn(r) of the compiled reference (every displacement, 1D/2D/3D, its
+    exact range N <= 3): the oracle's diagonal equals it exactly."""
+    gold = np.load(os.path.join(GOLD, "ref_correlation.npz"))
+    for c in _manifest()["correlation"]:
+        d = po.opposite_spin_correlation_diag(c["dims"], c["n_up"], c["n_dn"])
+        up, dn = po.basis_states(int(np.prod(c["dims"])), c["n_up"], c["n_dn"])
+        st = gold[c["name"] + "__states"]
+        assert np.array_equal(st[:, 0], np.asarray(up, np.uint64)), c["name"]
+        assert np.array_equal(st[:, 1], np.asarray(dn, np.uint64)), c["name"]
+        assert np.array_equal(d, gold[c["name"] + "__diag"]), c["name"]
+
+
+@pytest.mark.parametrize("sizes,nu,nd", [([8], 4, 3), ([3, 3], 4, 4), ([2, 2, 2], 3, 2)])
+def test_opposite_spin_correlation_sum_rule(sizes, nu, nd):
+    """sum_r G(r) = N_up N_dn / N for every state (sum_r n_{i+r,dn} = N_dn)."""
+    N = int(np.prod(sizes))
+    d = po.opposite_spin_correlation_diag(sizes, nu, nd)
+    assert np.allclose(d.sum(axis=0), nu * nd / N, rtol=0, atol=1e-14)
+    x = np.random.default_rng(1).normal(size=d.shape[1])
+    g = po.opposite_spin_correlation(sizes, nu, nd, x)
+    assert abs(g.sum() - nu * nd / N) < 1e-12
