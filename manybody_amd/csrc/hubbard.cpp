@@ -83,6 +83,62 @@ void species_hops(const std::vector<uint32_t> &strings, const std::vector<Mono> 
   }
 }
 
+// The compact table in sorted slot order with one sign pattern per wave.
+// Strings are sorted (stable) by hop count, then by positive hop count, cut
+// into groups of 64 (one wave), and the groups of each round of kSortedRound
+// slots are dealt to the waves in serpentine order (round j gives wave w the
+// sorted group 16 j + w, or 16 j + 15 - w for odd j), which evens out the
+// total work per wave.  Per group P = the most positive and N = the most
+// negative hops of its strings; a slot lists its positive targets first.
+void build_sorted(SpeciesHops &h) {
+  const size_t n = h.row_ptr.size() - 1;
+  h.srt_ok = false;
+  h.sperm.clear();
+  h.scomp.clear();
+  h.spn.clear();
+  const size_t np = (n + kSortedRound - 1) / kSortedRound * kSortedRound;
+  if (!h.comp_ok || n < (size_t)kSortedRound || np > (size_t)kSortedRound * kSortedRounds) return;
+  std::vector<int> cpos(n, 0), cneg(n, 0);
+  for (size_t d = 0; d < n; ++d)
+    for (int e = h.row_ptr[d]; e < h.row_ptr[d + 1]; ++e) (h.amp[e] < 0 ? cneg : cpos)[d]++;
+  std::vector<uint32_t> order(n);
+  for (size_t d = 0; d < n; ++d) order[d] = (uint32_t)d;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    const int ta = cpos[a] + cneg[a], tb = cpos[b] + cneg[b];
+    return ta != tb ? ta < tb : cpos[a] < cpos[b];
+  });
+  order.resize(np, (uint32_t)n);  // pad slots at the end
+  constexpr int kGroups = kSortedRound / 64;
+  h.sperm.assign(np, (uint16_t)n);
+  h.scomp.assign(np * 32, (uint16_t)n);
+  h.spn.assign(np / 64, 0);
+  for (size_t j = 0; j < np / kSortedRound; ++j)
+    for (int w = 0; w < kGroups; ++w) {
+      const size_t src = j * kGroups + ((j & 1) ? kGroups - 1 - w : w);
+      int P = 0, N = 0;
+      for (int l = 0; l < 64; ++l) {
+        const uint32_t d = order[src * 64 + l];
+        if (d < n) {
+          P = std::max(P, cpos[d]);
+          N = std::max(N, cneg[d]);
+        }
+      }
+      if (P + N > 32) return;  // srt_ok stays false
+      for (int l = 0; l < 64; ++l) {
+        const size_t slot = j * kSortedRound + (size_t)w * 64 + l;
+        const uint32_t d = order[src * 64 + l];
+        h.sperm[slot] = (uint16_t)d;
+        if (d >= n) continue;
+        int kp = 0, kn = P;
+        for (int e = h.row_ptr[d]; e < h.row_ptr[d + 1]; ++e)
+          h.scomp[slot * 32 + (h.amp[e] < 0 ? kn++ : kp++)] = (uint16_t)h.tgt[e];
+      }
+      h.spn[(j * kSortedRound + (size_t)w * 64) / 64] = (uint16_t)(P | N << 8);
+    }
+  h.srt_np = (int64_t)np;
+  h.srt_ok = true;
+}
+
 // Column-major packed copy of the hop table for the row kernel: entry k of
 // string d at ell[k * D + d] = target << 8 | (amplitude < 0) << 7 | magnitude
 // index.  Needs D < 2^24 and at most 128 distinct magnitudes.
@@ -135,6 +191,7 @@ void pack_ell(SpeciesHops &h) {
       h.cgmax[d / 64] = std::max<uint8_t>(h.cgmax[d / 64], (uint8_t)c);
     }
   }
+  build_sorted(h);
 }
 
 void upload_species(SpeciesHops &h) {
@@ -156,6 +213,11 @@ void upload_species(SpeciesHops &h) {
     h.d_comp = upload(h.comp);
     h.d_csign = upload(h.csign);
     h.d_cgmax = upload(h.cgmax);
+  }
+  if (h.srt_ok) {
+    h.d_sperm = upload(h.sperm);
+    h.d_scomp = upload(h.scomp);
+    h.d_spn = upload(h.spn);
   }
 }
 
@@ -249,6 +311,12 @@ HvTables tables_of(const Hubbard &h) {
   t.dn_comp = h.dn.d_comp.as<uint16_t>();
   t.dn_csign = h.dn.d_csign.as<uint32_t>();
   t.dn_cgmax = h.dn.d_cgmax.as<uint8_t>();
+  t.srt_ok = h.dn.srt_ok ? 1 : 0;
+  t.srt_np = h.dn.srt_np;
+  t.srt_perm = h.dn.d_sperm.as<uint16_t>();
+  t.srt_comp = h.dn.d_scomp.as<uint16_t>();
+  t.srt_pn = h.dn.d_spn.as<uint16_t>();
+  t.n_diag = std::min(h.sec->n_up, h.sec->n_dn) + 1;
   t.max_up_row = h.up.max_row;
   return t;
 }

@@ -17,12 +17,14 @@
 //     rows are staged in LDS as 32-bit byte offsets of the target rows (a
 //     plane is < 4 GiB) and amplitudes, so a gather is one add on a uniform
 //     base.  Writes y = (H_up x).
-//  2. row_kernel: one workgroup per plane row stages x[u, :] in LDS (91.5 KB
-//     for D_dn = 11,440 doubles), applies the diagonal and the down hops from
-//     LDS (compact table: padded target indices and sign masks, one trip count
-//     per wave), adds the strip result and applies the epilogue (plain,
-//     Lanczos, reconstruction, Chebyshev) with a per-workgroup partial sum for
-//     the fused inner products.
+//  2. row kernel: a plane row x[u, :] is staged in LDS (91.5 KB for D_dn =
+//     11,440 doubles), the diagonal and the down hops are applied from LDS, the
+//     strip result is added and the epilogue (plain, Lanczos, reconstruction,
+//     Chebyshev) applied, with one partial sum per row for the fused inner
+//     products.  row_kernel_sorted (persistent, one workgroup per CU, strings in
+//     hop-count order so that a wave's lanes share trip count and sign
+//     pattern) is used where its tables exist; row_kernel_compact (one
+//     workgroup per row, padded trip counts, per-lane sign masks) otherwise.
 //
 // Reductions are two-level and atomic-free (fixed partial slots, fixed tree),
 // so results do not depend on scheduling.  A one-kernel generic path (thread
@@ -45,6 +47,10 @@ constexpr int kBlock = 256;
 constexpr int kMaxRowLds = 160 * 1024 - 1024;  // bytes of LDS a staged row may use
 constexpr int kGenericGrid = 2048;              // blocks per plane of the generic path
 constexpr int kCompactK = 32;                   // entries per string of the compact down table
+#ifndef MB_STRIP_LANES
+#define MB_STRIP_LANES 16
+#endif
+constexpr int kStripLanes = MB_STRIP_LANES;     // lanes per row of the strip kernel (16 B each)
 
 __device__ __forceinline__ int popc(uint32_t v) { return __builtin_popcount(v); }
 
@@ -142,7 +148,7 @@ template <typename T, int VEC, int MAXH>
 __global__ __launch_bounds__(kBlock) void up_strip_kernel(HvTables t, int P, const T *__restrict__ x,
                                                           T *__restrict__ y, int nJ, int nC,
                                                           const int *flags) {
-  constexpr int LPR = 16, RPW = kWave / LPR, RC = 32, W = LPR * VEC;
+  constexpr int LPR = kStripLanes, RPW = kWave / LPR, RC = 32, W = LPR * VEC;
   __shared__ uint32_t s_off[RC][MAXH];  // byte offset of the target row in the plane
   __shared__ T s_amp[RC][MAXH];
   __shared__ int s_n[RC];
@@ -309,6 +315,192 @@ __global__ __launch_bounds__(1024) void row_kernel_compact(HvTables t, const T *
     }
   }
   row_reduce<T, E>(acc2, a, wg, red);
+}
+
+// Epilogue with the values it reads from memory loaded beforehand: pv = prev[i]
+// (ignored where the epilogue does not read it).
+template <typename T, Epi E>
+__device__ __forceinline__ void epilogue_pv(T r, T xv, T pv, int64_t i, T *__restrict__ y,
+                                            const EpiArgs &a, T bp, Partial2 &acc) {
+  if constexpr (E == Epi::Plain) {
+    y[i] = r;
+  } else if constexpr (E == Epi::Lanczos) {
+    if (bp != T(0)) r -= bp * pv;
+    y[i] = r;
+    acc.s0 += (double)xv * (double)r;
+  } else if constexpr (E == Epi::Recon) {
+    if (bp != T(0)) r -= bp * pv;
+    r -= (T)a.alpha * xv;
+    y[i] = r;
+  } else {  // Cheb
+    T o = (T)a.scale2 * (r - (T)a.shift * xv);
+    if (!a.first) o -= pv;
+    y[i] = o;
+    acc.s0 += (double)o * (double)xv;
+    acc.s1 += (double)o * (double)o;
+  }
+}
+
+template <typename T, Epi E>
+__device__ __forceinline__ bool reads_prev(const EpiArgs &a, T bp) {
+  if constexpr (E == Epi::Lanczos || E == Epi::Recon) return bp != T(0);
+  if constexpr (E == Epi::Cheb) return !a.first;
+  return false;
+}
+
+// Persistent row kernel over the sorted compact table.  One workgroup of 1024
+// threads per CU walks plane rows w, w + gridDim.x, ...  Per row:
+//  A. every thread takes one slot per round (kSortedRound slots): the string of
+//     the slot, its hop targets and signs; the trip count is the wave's shared
+//     hop count, so lanes do no padded work beyond rounding to even; the
+//     diagonal comes from a copy of diagU in LDS.  Results stay in registers.
+//     Then the next row is requested into registers; it lands during B/C.
+//  B/C. in two halves of kSortedHalf strings (the LDS left over holds half a
+//     row): results are written to an LDS buffer at their string, then one
+//     coalesced sweep adds the strip result and applies the epilogue (y, and
+//     prev for the fused epilogues, loaded before the half's barrier).
+//  D. the prefetched next row replaces the staged one.
+constexpr int kSortedHalf = kSortedRound * (kSortedRounds / 2);
+
+template <typename T, Epi E>
+__global__ __launch_bounds__(1024) void row_kernel_sorted(HvTables t, const T *__restrict__ x,
+                                                          T *__restrict__ y, EpiArgs a,
+                                                          int64_t nrows) {
+  constexpr int K = kCompactK, G = 4, NT = kSortedRound, R = kSortedRounds, H = R / 2;
+  extern __shared__ double smem[];
+  __shared__ double red[2][NT / kWave];
+  __shared__ T s_diag[kMaxSites + 1];
+  const int Ddn = (int)t.d_dn;
+  const int64_t Dup = t.d_up;
+  T *row = reinterpret_cast<T *>(smem);  // Ddn + 1 entries, zero slot at Ddn
+  T *scr = row + Ddn + 1;                // kSortedHalf entries
+  if constexpr (E == Epi::Lanczos) {
+    if (a.flags[0]) return;
+  }
+  int64_t w = blockIdx.x;
+  if (w >= nrows) return;
+  const int tid = threadIdx.x;
+  const int nr = (int)(t.srt_np / NT);  // slot rounds (<= R)
+  if (tid < t.n_diag) s_diag[tid] = Tabs<T>::diag(t)[tid];
+  {
+    const T *__restrict__ xr = x + w * Ddn;  // uniform row base; lanes index it with tid
+    T v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = j * NT + tid < Ddn ? xr[j * NT + tid] : T(0);
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (j * NT + tid < Ddn) row[j * NT + tid] = v[j];
+    if (tid == 0) row[Ddn] = T(0);
+  }
+  __syncthreads();
+  const T h = Tabs<T>::mags(t)[0];
+  const T bp = beta_prev<T, E>(a);
+  const bool rp = reads_prev<T, E>(a, bp);
+  for (; w < nrows; w += gridDim.x) {
+    const int64_t nx = w + gridDim.x;
+    const int64_t u = w % Dup;
+    const uint32_t su = t.s_up[u];
+    const T on_u = t.has_onsite ? Tabs<T>::up_on(t)[u] : T(0);
+    // An opaque zero added to the table bases per row: otherwise the per-round
+    // addresses derived from them are hoisted out of the row loop and spilled.
+    int zero = 0;
+    asm volatile("" : "+s"(zero));
+    const uint16_t *perm_b = t.srt_perm + zero, *comp_b = t.srt_comp + zero, *pn_b = t.srt_pn + zero;
+    // A: hop sums of every slot of this thread; slot strings packed two per word
+    T res[R];
+    uint32_t dd[R / 2];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      res[j] = T(0);
+      if (j % 2 == 0) dd[j / 2] = (uint32_t)Ddn | ((uint32_t)Ddn << 16);
+      if (j < nr) {
+        // uniform bases of this round's tables; lanes index them with tid
+        const uint16_t *__restrict__ perm = perm_b + j * NT;
+        const uint4 *__restrict__ comp = reinterpret_cast<const uint4 *>(comp_b + (int64_t)j * NT * K);
+        const int d = perm[tid];
+        dd[j / 2] = (j & 1) ? ((dd[j / 2] & 0xFFFFu) | ((uint32_t)d << 16))
+                            : ((dd[j / 2] & 0xFFFF0000u) | (uint32_t)d);
+        // wave-uniform sign pattern: slots [0, np_) positive, [np_, nt) negative
+        const int pn = __builtin_amdgcn_readfirstlane((int)pn_b[(j * NT + tid) >> 6]);
+        const int np_ = pn & 0xFF, nt = np_ + (pn >> 8);
+        uint4 q[K / 8];
+#pragma unroll
+        for (int k = 0; k < K / 8; ++k)
+          q[k] = k * 8 < nt ? comp[tid * (K / 8) + k] : make_uint4(0, 0, 0, 0);
+        const int dc = d < Ddn ? d : 0;
+        const uint32_t sd = t.s_dn[dc];
+        T h0 = T(0), h1 = T(0);
+#pragma unroll
+        for (int g = 0; g < K / G; ++g) {
+          if (g * G < nt) {
+            T v[G];
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+              const int k = g * G + i;
+              const uint32_t e = reinterpret_cast<const uint32_t *>(&q[k / 8])[(k % 8) / 2];
+              v[i] = row[(k & 1) ? (e >> 16) : (e & 0xFFFFu)];
+            }
+#pragma unroll
+            for (int i = 0; i < G; ++i) {
+              const int k = g * G + i;
+              const T sg = k < np_ ? T(1) : T(-1);  // uniform: a scalar operand of the fma
+              if (i & 1) h1 = fma(v[i], sg, h1); else h0 = fma(v[i], sg, h0);
+            }
+          }
+        }
+        T dg = s_diag[popc(su & sd)];
+        if (t.has_onsite) dg = dg + on_u + Tabs<T>::dn_on(t)[dc];
+        res[j] = dg * row[d] + h * (h0 + h1);
+      }
+      // keep the rounds apart: hoisting the next round's loads only adds live registers
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // next row into registers (consumed in D).  Issued after A: vmcnt retires in
+    // order, so loads in flight during A would hold up its table loads.
+    T pre[R];
+    {
+      const T *__restrict__ xn = x + (nx < nrows ? nx : w) * Ddn;
+#pragma unroll
+      for (int j = 0; j < R; ++j) pre[j] = j * NT + tid < Ddn ? xn[j * NT + tid] : T(0);
+    }
+    T *__restrict__ yr = y + w * Ddn;
+    const T *__restrict__ pr = rp ? static_cast<const T *>(a.prev) + w * Ddn : nullptr;
+    Partial2 acc2;
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf) {
+      const int lo = hf * kSortedHalf;
+      T yin[H], pv[H];
+#pragma unroll
+      for (int j = 0; j < H; ++j) {
+        const int d = (hf * H + j) * NT + tid;
+        yin[j] = d < Ddn ? yr[d] : T(0);
+        pv[j] = (rp && d < Ddn) ? pr[d] : T(0);
+      }
+      // B: results of strings [lo, lo + kSortedHalf) into the LDS buffer
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int d = (int)((dd[j / 2] >> (16 * (j & 1))) & 0xFFFFu);
+        if (d < Ddn && d >= lo && d < lo + kSortedHalf) scr[d - lo] = res[j];
+      }
+      __syncthreads();
+      // C: coalesced sweep over the half
+#pragma unroll
+      for (int j = 0; j < H; ++j) {
+        const int d = (hf * H + j) * NT + tid;
+        if (d < Ddn)
+          epilogue_pv<T, E>(yin[j] + scr[d - lo], row[d], pv[j], w * Ddn + d, y, a, bp, acc2);
+      }
+      __syncthreads();
+    }
+    row_reduce<T, E>(acc2, a, w, red);
+    // D: next row
+    if (nx < nrows) {
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (j * NT + tid < Ddn) row[j * NT + tid] = pre[j];
+    }
+    __syncthreads();
+  }
 }
 
 // General down table (any number of amplitude magnitudes): column-major uint32
@@ -738,18 +930,56 @@ int row_block(int64_t Ddn) {
   return (int)((Ddn + kWave - 1) / kWave) * kWave;
 }
 
+bool force_compact_row() {
+  const char *e = std::getenv("MANYBODY_AMD_ROW_KERNEL");
+  return e && std::strcmp(e, "compact") == 0;
+}
+
+// Compute units of the current device (cached per device).
+int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  MB_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) dev = 0;
+  if (cus[dev] == 0) {
+    int n = 0;
+    MB_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    cus[dev] = n > 0 ? n : 1;
+  }
+  return cus[dev];
+}
+
+// LDS of the sorted row kernel: the row with its zero slot, then half a row.
+template <typename T>
+size_t sorted_row_lds(const HvTables &t) {
+  return (size_t)(t.d_dn + 1 + kSortedHalf) * sizeof(T);
+}
+
 template <typename T, Epi E>
 void launch_row(const HvTables &t, int P, const T *x, T *y, const EpiArgs &a, hipStream_t s) {
   const size_t lds = (size_t)(t.d_dn + 1) * sizeof(T);  // + the zero slot of the compact table
   static bool attr_set = false;  // once per template instance
   const void *kc = reinterpret_cast<const void *>(&row_kernel_compact<T, E>);
   const void *ke = reinterpret_cast<const void *>(&row_kernel_ell<T, E>);
+  const void *ks = reinterpret_cast<const void *>(&row_kernel_sorted<T, E>);
   if (!attr_set) {
     MB_HIP(hipFuncSetAttribute(kc, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxRowLds));
     MB_HIP(hipFuncSetAttribute(ke, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxRowLds));
+    MB_HIP(hipFuncSetAttribute(ks, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxRowLds));
     attr_set = true;
   }
-  const dim3 g((unsigned)(P * t.d_up)), b(row_block(t.d_dn));
+  const int64_t nrows = (int64_t)P * t.d_up;
+  // the sorted kernel: slots of at most kSortedRounds rounds, 2 halves of kSortedHalf strings
+  if (t.srt_ok && t.srt_np <= (int64_t)kSortedRound * kSortedRounds &&
+      t.d_dn <= 2 * (int64_t)kSortedHalf && sorted_row_lds<T>(t) <= (size_t)kMaxRowLds &&
+      !force_compact_row()) {
+    const int64_t cus = device_cus();
+    const dim3 g((unsigned)(nrows < cus ? nrows : cus)), b(kSortedRound);
+    hipLaunchKernelGGL((row_kernel_sorted<T, E>), g, b, sorted_row_lds<T>(t), s, t, x, y, a, nrows);
+    MB_HIP(hipGetLastError());
+    return;
+  }
+  const dim3 g((unsigned)nrows), b(row_block(t.d_dn));
   if (t.comp_ok)
     hipLaunchKernelGGL((row_kernel_compact<T, E>), g, b, lds, s, t, x, y, a);
   else
@@ -759,7 +989,7 @@ void launch_row(const HvTables &t, int P, const T *x, T *y, const EpiArgs &a, hi
 
 template <typename T, int VEC>
 void launch_strip(const HvTables &t, int P, const T *x, T *y, const int *flags, hipStream_t s) {
-  constexpr int W = 16 * VEC, RC = 32;
+  constexpr int W = kStripLanes * VEC, RC = 32;
   const int nJ = (int)((t.d_dn + W - 1) / W);
   const int nC = (int)((t.d_up + RC - 1) / RC);
   const int64_t S8 = ((int64_t)P * nJ + 7) / 8;

@@ -156,7 +156,19 @@ struct SpeciesHops {
   std::vector<uint32_t> csign;
   std::vector<uint8_t> cgmax;
   DevBuf d_comp, d_csign, d_cgmax;
+  // the compact table again in sorted slot order, with one sign pattern per
+  // wave: strings sorted by (hop count, positive hop count) share a wave of 64
+  // slots; per group P = most positive and N = most negative hops, and every
+  // slot lists its positive targets in [0, P) and negative ones in [P, P + N),
+  // padded with the zero slot n.  slot -> string, targets, per-group P | N << 8;
+  // padded to rounds of kSortedRound slots, pad slots -> string n
+  bool srt_ok = false;
+  int64_t srt_np = 0;
+  std::vector<uint16_t> sperm, scomp, spn;
+  DevBuf d_sperm, d_scomp, d_spn;
 };
+constexpr int kSortedRound = 1024;  // slots per round (one per thread of the row kernel)
+constexpr int kSortedRounds = 12;   // rounds per row the sorted row kernel unrolls
 
 struct Hubbard {
   const Sector *sec = nullptr;
@@ -199,6 +211,11 @@ struct HvTables {
   const uint16_t *dn_comp;
   const uint32_t *dn_csign;
   const uint8_t *dn_cgmax;
+  // the compact table in sorted slot order (see SpeciesHops)
+  int srt_ok;
+  int64_t srt_np;
+  const uint16_t *srt_perm, *srt_comp, *srt_pn;
+  int n_diag;  // entries of diagU that can occur: min(n_up, n_dn) + 1
   int max_up_row;
 };
 HvTables tables_of(const Hubbard &h);
