@@ -1,0 +1,121 @@
+"""The two row kernels of H.v on sectors with D_dn = 11,440 (4 x 4, 7 down
+electrons, 1 or 2 up): the persistent hop-count-sorted kernel (the default at
+this row length) against the per-row compact kernel (selected with
+MANYBODY_AMD_ROW_KERNEL=compact) and the oracle, through every epilogue: plain
+H.v, Lanczos (pass one), reconstruction (the eigenvector pass) and Chebyshev
+(KPM moments, FP64 and FP32).
+
+Tolerances: integer-valued inputs are exact in FP64, so H.v is compared bit
+for bit with the oracle; Gaussian inputs 1e-14 relative between the kernels
+(they add a string's hops in a different order); Lanczos alpha/beta 1e-12
+relative over 30 steps; eigenvalue 1e-12 relative, eigenvector 1e-9 relative;
+FP64 KPM moments 1e-12 absolute (mu_0 = 1 to 1e-12), FP32 KPM moments 1e-5
+absolute between the kernels and 1e-4 against FP64 (mu_0 = 1 to 1e-6).
+"""
+import contextlib
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import manybody_amd as mb
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+T_HOP, U = 1.0, 8.0
+
+
+@contextlib.contextmanager
+def compact_rows():
+    old = os.environ.get("MANYBODY_AMD_ROW_KERNEL")
+    os.environ["MANYBODY_AMD_ROW_KERNEL"] = "compact"
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ["MANYBODY_AMD_ROW_KERNEL"]
+        else:
+            os.environ["MANYBODY_AMD_ROW_KERNEL"] = old
+
+
+def make_op(nu, nd=7):
+    basis = mb.FermionBasis.sector(16, nu, nd)
+    return mb.HubbardModel2D(T_HOP, U, 4, 4).operator(basis)
+
+
+def test_hv_integer_inputs_bit_exact_vs_oracle(cuda):
+    nu, nd = 1, 7
+    op = make_op(nu, nd)
+    D = op.dimension()
+    assert (op.d_up, op.d_dn) == (16, 11440)
+    rng = np.random.default_rng(7)
+    x_ref = (rng.integers(-4, 5, D) + 1j * rng.integers(-4, 5, D)).astype(np.complex128)
+    x = op.to_device_order(torch.from_numpy(x_ref).to(cuda))
+    y = op.to_reference_order(op.hv(x)).cpu().numpy()
+    A = po.build_csc([4, 4], nu, nd, T_HOP, U)
+    assert np.array_equal(y, A.matvec(x_ref))
+
+
+@pytest.mark.parametrize("nu", [1, 2])
+def test_hv_sorted_matches_compact(cuda, nu):
+    op = make_op(nu)
+    x = op.gaussian(seed=11 + nu)
+    y_s = op.hv(x)
+    with compact_rows():
+        y_c = op.hv(x)
+    scale = y_c.abs().max().item()
+    assert (y_s - y_c).abs().max().item() <= 1e-14 * scale
+    # FP32, two complex vectors (four planes)
+    xf = torch.cat([x, op.gaussian(seed=99)]).float()
+    yf_s = op.hv(xf)
+    with compact_rows():
+        yf_c = op.hv(xf)
+    assert (yf_s - yf_c).abs().max().item() <= 1e-5 * yf_c.abs().max().item()
+
+
+def test_lanczos_sorted_matches_compact(cuda):
+    op = make_op(2)
+    b = op.gaussian(seed=3)
+    d_s = mb.lanczos_pass_one(op, b, 30)
+    with compact_rows():
+        d_c = mb.lanczos_pass_one(op, b, 30)
+    assert d_s.steps_taken == d_c.steps_taken == 30
+    a_s, a_c = np.array(d_s.alphas), np.array(d_c.alphas)
+    b_s, b_c = np.array(d_s.betas), np.array(d_c.betas)
+    assert np.abs(a_s - a_c).max() <= 1e-12 * np.abs(a_c).max()
+    assert np.abs(b_s - b_c).max() <= 1e-12 * np.abs(b_c).max()
+
+
+def test_eigenpair_sorted_matches_compact(cuda):
+    """find_min_eigenpair: pass one (Lanczos epilogue) and the eigenvector pass
+    (reconstruction epilogue)."""
+    op = make_op(1)
+    b = op.gaussian(seed=5)
+    ep_s = mb.find_min_eigenpair(op, 80, b)
+    with compact_rows():
+        ep_c = mb.find_min_eigenpair(op, 80, b)
+    assert abs(ep_s.value - ep_c.value) <= 1e-12 * abs(ep_c.value)
+    diff = (ep_s.vector - ep_c.vector).norm().item()
+    assert diff <= 1e-9 * ep_c.vector.norm().item()
+    e, r = op.energy_residual(ep_s.vector)
+    assert abs(e - ep_s.value) <= 1e-10 * abs(ep_s.value)
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_kpm_sorted_matches_compact(cuda, fp32):
+    """Chebyshev epilogue; the spectrum of 4 x 4 (2 up, 7 down) at U = 8 lies in
+    [-36, 52], inside [b - a, b + a] = [-38, 54]."""
+    op = make_op(2)
+    a, bb = 46.0, 8.0
+    mu_s = np.array(mb.kpm_dos_moments(op, a, bb, 64, 2, seed=21, fp32=fp32))
+    with compact_rows():
+        mu_c = np.array(mb.kpm_dos_moments(op, a, bb, 64, 2, seed=21, fp32=fp32))
+    # mu_0 = <r|r> / <r|r>: exact in FP64, to FP32 rounding of the vectors in FP32
+    assert abs(mu_s[0] - 1.0) < (1e-6 if fp32 else 1e-12)
+    tol = 1e-5 if fp32 else 1e-12
+    assert np.abs(mu_s - mu_c).max() <= tol
+    if fp32:
+        mu_64 = np.array(mb.kpm_dos_moments(op, a, bb, 64, 2, seed=21, fp32=False))
+        assert np.abs(mu_s - mu_64).max() <= 1e-4
