@@ -213,7 +213,7 @@ def write_traffic(hv_fetch, hv_write, copy_fetch, copy_write):
     write = _pmc_per_kernel(hv_write, "WRITE_SIZE")
     kernels = {}
     for name in fetch:
-        for short in ("up_strip_kernel", "row_kernel_compact", "row_kernel_ell"):
+        for short in ("up_strip_kernel", "row_kernel_sorted", "row_kernel_compact", "row_kernel_ell"):
             if short + "<" in name:
                 kernels[short] = {"fetch_bytes": round(fetch[name] * 1024.0 * ff),
                                   "write_bytes": round(write.get(name, 0.0) * 1024.0 * wf)}

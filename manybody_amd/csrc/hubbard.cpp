@@ -194,21 +194,6 @@ void pack_ell(SpeciesHops &h) {
   build_sorted(h);
 }
 
-void pad_lists(SpeciesHops &h) {
-  const size_t n = h.row_ptr.size() - 1;
-  h.pad_w = h.max_row <= 32 ? 32 : h.max_row <= 64 ? 64 : 0;
-  h.pad_tgt.clear();
-  h.pad_amp.clear();
-  if (!h.pad_w) return;
-  h.pad_tgt.assign(n * h.pad_w, 0);
-  h.pad_amp.assign(n * h.pad_w, 0.0);
-  for (size_t r = 0; r < n; ++r)
-    for (int e = h.row_ptr[r]; e < h.row_ptr[r + 1]; ++e) {
-      h.pad_tgt[r * h.pad_w + (e - h.row_ptr[r])] = h.tgt[e];
-      h.pad_amp[r * h.pad_w + (e - h.row_ptr[r])] = h.amp[e];
-    }
-}
-
 void upload_species(SpeciesHops &h) {
   h.d_row_ptr = upload(h.row_ptr);
   h.d_tgt = upload(h.tgt);
@@ -228,12 +213,6 @@ void upload_species(SpeciesHops &h) {
     h.d_comp = upload(h.comp);
     h.d_csign = upload(h.csign);
     h.d_cgmax = upload(h.cgmax);
-  }
-  if (h.pad_w) {
-    h.d_pad_tgt = upload(h.pad_tgt);
-    h.d_pad_amp = upload(h.pad_amp);
-    std::vector<float> pf(h.pad_amp.begin(), h.pad_amp.end());
-    h.d_pad_ampf = upload(pf);
   }
   if (h.srt_ok) {
     h.d_sperm = upload(h.sperm);
@@ -279,7 +258,6 @@ void build_hubbard(Hubbard &h) {
   species_hops(s.s_up, mono, h.up);
   species_hops(s.s_dn, mono, h.dn);
   pack_ell(h.dn);
-  pad_lists(h.up);
   h.has_onsite = false;
   for (double v : h.up.onsite) h.has_onsite |= (v != 0.0);
   for (double v : h.dn.onsite) h.has_onsite |= (v != 0.0);
@@ -340,10 +318,6 @@ HvTables tables_of(const Hubbard &h) {
   t.srt_pn = h.dn.d_spn.as<uint16_t>();
   t.n_diag = std::min(h.sec->n_up, h.sec->n_dn) + 1;
   t.max_up_row = h.up.max_row;
-  t.up_pw = h.up.pad_w;
-  t.up_ptgt = h.up.d_pad_tgt.as<int32_t>();
-  t.up_pamp = h.up.d_pad_amp.as<double>();
-  t.up_pampf = h.up.d_pad_ampf.as<float>();
   return t;
 }
 

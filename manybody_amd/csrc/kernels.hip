@@ -72,7 +72,6 @@ struct Tabs;
 template <>
 struct Tabs<double> {
   __device__ static const double *up_amp(const HvTables &t) { return t.up_amp; }
-  __device__ static const double *up_pamp(const HvTables &t) { return t.up_pamp; }
   __device__ static const double *dn_amp(const HvTables &t) { return t.dn_amp; }
   __device__ static const double *up_on(const HvTables &t) { return t.up_on; }
   __device__ static const double *dn_on(const HvTables &t) { return t.dn_on; }
@@ -82,7 +81,6 @@ struct Tabs<double> {
 template <>
 struct Tabs<float> {
   __device__ static const float *up_amp(const HvTables &t) { return t.up_ampf; }
-  __device__ static const float *up_pamp(const HvTables &t) { return t.up_pampf; }
   __device__ static const float *dn_amp(const HvTables &t) { return t.dn_ampf; }
   __device__ static const float *up_on(const HvTables &t) { return t.up_onf; }
   __device__ static const float *dn_on(const HvTables &t) { return t.dn_onf; }
@@ -164,33 +162,20 @@ __global__ __launch_bounds__(kBlock) void up_strip_kernel(HvTables t, int P, con
   const int p = s / nJ, J = s - p * nJ;
   const int u_begin = c * RC;
   const int u_end = (int)min((int64_t)u_begin + RC, t.d_up);
-  // the hop lists of rows [u_begin, u_end) at stride MAXH: all loads issued
-  // before the LDS writes (one round trip)
-  {
-    constexpr int PER = RC * MAXH / kBlock;
-    const int32_t *__restrict__ ptg = t.up_ptgt + (int64_t)u_begin * MAXH;
-    const T *__restrict__ pam = Tabs<T>::up_pamp(t) + (int64_t)u_begin * MAXH;
-    const int lim = (u_end - u_begin) * MAXH;
-    int32_t tg[PER];
-    T am[PER];
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = threadIdx.x + i * kBlock;
-      tg[i] = e < lim ? ptg[e] : 0;
-      am[i] = e < lim ? pam[e] : T(0);
+  const T *__restrict__ amp = Tabs<T>::up_amp(t);
+  for (int e = threadIdx.x; e < RC * MAXH; e += kBlock) {
+    const int r = e / MAXH, k = e % MAXH;
+    const int u = u_begin + r;
+    if (u < u_end) {
+      const int b = t.up_ptr[u], n = t.up_ptr[u + 1] - b;
+      if (k == 0) s_n[r] = n;
+      if (k < n) {
+        s_off[r][k] = (uint32_t)((uint64_t)t.up_tgt[b + k] * (uint64_t)t.d_dn * sizeof(T));
+        s_amp[r][k] = amp[b + k];
+      }
+    } else if (k == 0) {
+      s_n[r] = 0;
     }
-    int nrow = 0;
-    if (threadIdx.x < RC) {
-      const int u = u_begin + (int)threadIdx.x;
-      nrow = u < u_end ? t.up_ptr[u + 1] - t.up_ptr[u] : 0;
-    }
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = threadIdx.x + i * kBlock;
-      s_off[e / MAXH][e % MAXH] = (uint32_t)((uint64_t)tg[i] * (uint64_t)t.d_dn * sizeof(T));
-      s_amp[e / MAXH][e % MAXH] = am[i];
-    }
-    if (threadIdx.x < RC) s_n[threadIdx.x] = nrow;
   }
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
@@ -933,7 +918,7 @@ template <typename T>
 bool use_two_pass(const HvTables &t) {
   // the strip kernel addresses a plane with 32-bit byte offsets
   const bool plane_fits = t.d_up * t.d_dn * (int64_t)sizeof(T) < ((int64_t)1 << 32);
-  return t.ell_ok && (t.up_pw == 32 || t.up_pw == 64) && (t.d_dn + 1) * (int64_t)sizeof(T) <= kMaxRowLds &&
+  return t.ell_ok && t.max_up_row <= 64 && (t.d_dn + 1) * (int64_t)sizeof(T) <= kMaxRowLds &&
          plane_fits &&
          !force_generic();
 }
@@ -1011,7 +996,7 @@ void launch_strip(const HvTables &t, int P, const T *x, T *y, const int *flags, 
   const int64_t S8 = ((int64_t)P * nJ + 7) / 8;
   const int64_t grid = 8 * (int64_t)nC * S8;
   if (grid >= (1ll << 31)) throw std::invalid_argument("sector too large for the strip kernel");
-  if (t.up_pw == 32)
+  if (t.max_up_row <= 32)
     hipLaunchKernelGGL((up_strip_kernel<T, VEC, 32>), dim3((unsigned)grid), dim3(kBlock), 0, s, t, P,
                        x, y, nJ, nC, flags);
   else

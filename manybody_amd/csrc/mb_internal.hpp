@@ -142,12 +142,6 @@ struct SpeciesHops {
   std::vector<double> onsite;  // one-body diagonal of self bonds (all zero unless a size is 1)
   int max_row = 0;
   DevBuf d_row_ptr, d_tgt, d_amp, d_ampf, d_onsite, d_onsitef;
-  // the hop lists again at a fixed stride pad_w (32 or 64, 0 if longer rows),
-  // unused entries zero: a strip workgroup fetches its rows' lists in one pass
-  int pad_w = 0;
-  std::vector<int32_t> pad_tgt;
-  std::vector<double> pad_amp;
-  DevBuf d_pad_tgt, d_pad_amp, d_pad_ampf;
   // packed column-major copy
   bool ell_ok = false;
   int ell_k = 0;
@@ -223,11 +217,6 @@ struct HvTables {
   const uint16_t *srt_perm, *srt_comp, *srt_pn;
   int n_diag;  // entries of diagU that can occur: min(n_up, n_dn) + 1
   int max_up_row;
-  // up hop lists at a fixed stride up_pw (see SpeciesHops::pad_w)
-  int up_pw;
-  const int32_t *up_ptgt;
-  const double *up_pamp;
-  const float *up_pampf;
 };
 HvTables tables_of(const Hubbard &h);
 
