@@ -194,6 +194,19 @@ void pack_ell(SpeciesHops &h) {
   build_sorted(h);
 }
 
+void pack_c16(SpeciesHops &h) {
+  h.c16_ok = false;
+  h.c16.clear();
+  if (h.tgt.empty() || h.row_ptr.size() - 1 > 32768) return;
+  h.mag16 = std::fabs(h.amp[0]);
+  for (double a : h.amp)
+    if (std::fabs(a) != h.mag16) return;
+  h.c16.resize(h.tgt.size());
+  for (size_t e = 0; e < h.tgt.size(); ++e)
+    h.c16[e] = (uint16_t)((uint32_t)h.tgt[e] | (h.amp[e] < 0 ? 0x8000u : 0u));
+  h.c16_ok = true;
+}
+
 void upload_species(SpeciesHops &h) {
   h.d_row_ptr = upload(h.row_ptr);
   h.d_tgt = upload(h.tgt);
@@ -214,6 +227,7 @@ void upload_species(SpeciesHops &h) {
     h.d_csign = upload(h.csign);
     h.d_cgmax = upload(h.cgmax);
   }
+  if (h.c16_ok) h.d_c16 = upload(h.c16);
   if (h.srt_ok) {
     h.d_sperm = upload(h.sperm);
     h.d_scomp = upload(h.scomp);
@@ -258,6 +272,7 @@ void build_hubbard(Hubbard &h) {
   species_hops(s.s_up, mono, h.up);
   species_hops(s.s_dn, mono, h.dn);
   pack_ell(h.dn);
+  pack_c16(h.up);
   h.has_onsite = false;
   for (double v : h.up.onsite) h.has_onsite |= (v != 0.0);
   for (double v : h.dn.onsite) h.has_onsite |= (v != 0.0);
@@ -318,6 +333,9 @@ HvTables tables_of(const Hubbard &h) {
   t.srt_pn = h.dn.d_spn.as<uint16_t>();
   t.n_diag = std::min(h.sec->n_up, h.sec->n_dn) + 1;
   t.max_up_row = h.up.max_row;
+  t.up_c16_ok = h.up.c16_ok ? 1 : 0;
+  t.up_c16 = h.up.d_c16.as<uint16_t>();
+  t.up_mag = h.up.mag16;
   return t;
 }
 

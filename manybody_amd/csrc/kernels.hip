@@ -162,7 +162,11 @@ __global__ __launch_bounds__(kBlock) void up_strip_kernel(HvTables t, int P, con
   const int p = s / nJ, J = s - p * nJ;
   const int u_begin = c * RC;
   const int u_end = (int)min((int64_t)u_begin + RC, t.d_up);
+  // hop lists of the workgroup's rows; from the 16-bit table where there is
+  // one magnitude (every strip re-reads the lists: 2 bytes a hop instead of 12
+  // keeps them from displacing the strip in L2)
   const T *__restrict__ amp = Tabs<T>::up_amp(t);
+  const T mag = (T)t.up_mag;
   for (int e = threadIdx.x; e < RC * MAXH; e += kBlock) {
     const int r = e / MAXH, k = e % MAXH;
     const int u = u_begin + r;
@@ -170,8 +174,18 @@ __global__ __launch_bounds__(kBlock) void up_strip_kernel(HvTables t, int P, con
       const int b = t.up_ptr[u], n = t.up_ptr[u + 1] - b;
       if (k == 0) s_n[r] = n;
       if (k < n) {
-        s_off[r][k] = (uint32_t)((uint64_t)t.up_tgt[b + k] * (uint64_t)t.d_dn * sizeof(T));
-        s_amp[r][k] = amp[b + k];
+        uint32_t tg;
+        T a;
+        if (t.up_c16_ok) {
+          const uint32_t c = t.up_c16[b + k];
+          tg = c & 0x7FFFu;
+          a = (c & 0x8000u) ? -mag : mag;
+        } else {
+          tg = (uint32_t)t.up_tgt[b + k];
+          a = amp[b + k];
+        }
+        s_off[r][k] = (uint32_t)((uint64_t)tg * (uint64_t)t.d_dn * sizeof(T));
+        s_amp[r][k] = a;
       }
     } else if (k == 0) {
       s_n[r] = 0;

@@ -142,6 +142,12 @@ struct SpeciesHops {
   std::vector<double> onsite;  // one-body diagonal of self bonds (all zero unless a size is 1)
   int max_row = 0;
   DevBuf d_row_ptr, d_tgt, d_amp, d_ampf, d_onsite, d_onsitef;
+  // the CSR entries packed in 16 bits, target | (amplitude < 0) << 15, when
+  // every amplitude has the magnitude mag16 and targets are below 2^15
+  bool c16_ok = false;
+  double mag16 = 0.0;
+  std::vector<uint16_t> c16;
+  DevBuf d_c16;
   // packed column-major copy
   bool ell_ok = false;
   int ell_k = 0;
@@ -217,6 +223,10 @@ struct HvTables {
   const uint16_t *srt_perm, *srt_comp, *srt_pn;
   int n_diag;  // entries of diagU that can occur: min(n_up, n_dn) + 1
   int max_up_row;
+  // up hops packed in 16 bits (see SpeciesHops::c16)
+  int up_c16_ok;
+  const uint16_t *up_c16;
+  double up_mag;
 };
 HvTables tables_of(const Hubbard &h);
 
