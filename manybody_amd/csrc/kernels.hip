@@ -909,6 +909,28 @@ __global__ __launch_bounds__(kBlock) void scale_complex_kernel(const T *__restri
   }
 }
 
+// x = sum_{j < m} y_j V_j for Krylov vectors V_j kept in HBM (planar, stride
+// 2 D, in T) and complex coefficients y (re, im pairs on the device), summed
+// in FP64 in the order j = 0, 1, ...: one pass over the m vectors in place of
+// regenerating them with m - 1 more H.v.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void basis_combine_kernel(const T *__restrict__ V, int m,
+                                                               int64_t D,
+                                                               const double *__restrict__ y,
+                                                               double *__restrict__ x) {
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
+    double xr = 0.0, xi = 0.0;
+    for (int j = 0; j < m; ++j) {
+      const double vr = V[(int64_t)j * 2 * D + e], vi = V[(int64_t)j * 2 * D + D + e];
+      const double yr = y[2 * j], yi = y[2 * j + 1];
+      xr += vr * yr - vi * yi;
+      xi += vr * yi + vi * yr;
+    }
+    x[e] = xr;
+    x[D + e] = xi;
+  }
+}
+
 // out[r] = sum over the slots of planes 2r, 2r+1 of partial[2 slot]; out[nvec + r] the odd entries
 __global__ __launch_bounds__(kBlock) void reduce_cheb_kernel(const double *partial, int64_t rows,
                                                              int nvec, double *out) {
@@ -1191,6 +1213,17 @@ void launch_recon_update(T *w, double *x, double beta, double yr, double yi, int
                      beta, yr, yi, D);
   MB_HIP(hipGetLastError());
 }
+template <typename T>
+void launch_basis_combine(const T *V, int m, int64_t D, const double *y, double *x, hipStream_t s) {
+  hipLaunchKernelGGL((basis_combine_kernel<T>), dim3(grid_for(D, 1 << 16)), dim3(kBlock), 0, s, V, m,
+                     D, y, x);
+  MB_HIP(hipGetLastError());
+}
+template void launch_basis_combine<double>(const double *, int, int64_t, const double *, double *,
+                                           hipStream_t);
+template void launch_basis_combine<float>(const float *, int, int64_t, const double *, double *,
+                                          hipStream_t);
+
 template void launch_recon_update<double>(double *, double *, double, double, double, int64_t,
                                           hipStream_t);
 template void launch_recon_update<float>(float *, double *, double, double, double, int64_t,

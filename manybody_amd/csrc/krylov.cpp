@@ -133,11 +133,16 @@ double tridiag_min_eig(int m, const double *a, const double *b) {
 
 // ------------------------------------------------------------ Lanczos
 
-LanczosWork::LanczosWork(int64_t D, int k, int64_t nparts, size_t esz)
-    : D(D), k(k), esz(esz), v(D * 2 * esz), vp(D * 2 * esz), w(D * 2 * esz),
+LanczosWork::LanczosWork(int64_t D, int k, int64_t nparts, size_t esz, bool vectors)
+    : D(D), k(k), esz(esz),
       scal(sizeof(double) *
                (2 * (size_t)std::max(k, 1) + 2 + (size_t)std::max<int64_t>(kRedBlocks, nparts)) +
            16) {
+  if (vectors) {
+    v.alloc(D * 2 * esz);
+    vp.alloc(D * 2 * esz);
+    w.alloc(D * 2 * esz);
+  }
   double *base = scal.as<double>();
   st.alpha = base;
   st.beta = base + std::max(k, 1);
@@ -146,15 +151,29 @@ LanczosWork::LanczosWork(int64_t D, int k, int64_t nparts, size_t esz)
   st.flags = reinterpret_cast<int *>(base + 2 * std::max(k, 1) + 1);
 }
 
+bool keep_krylov_basis(size_t bytes) {
+  if (thread_dev_alloc()) return true;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return bytes <= free_b / 2;
+}
+
 // Pass one with Krylov vectors and H.v in T (double, or float for the FP32
 // Krylov option); b is FP64, the norm and the alpha/beta reductions are FP64.
+// With `basis` (k + 1 vectors of 2 D values of T) the recurrence runs in it:
+// step i reads v_i = basis[i] and v_{i-1}, and writes v_{i+1} into basis[i + 1],
+// so the Krylov vectors stay in HBM for the reconstruction.
 template <typename T>
-LanczosResult pass_one_t(const Hubbard &h, const void *b, int k, hipStream_t s, LanczosWork *work) {
+LanczosResult pass_one_t(const Hubbard &h, const void *b, int k, hipStream_t s, LanczosWork *work,
+                         T *basis = nullptr) {
   const int64_t D = h.dim();
   std::unique_ptr<LanczosWork> own;
   const HvTables t = tables_of(h);
-  if (!work || work->D != D || work->k < k || work->esz != sizeof(T)) {
-    own.reset(new LanczosWork(D, k, hv_partials(t, 2), sizeof(T)));
+  if (!work || work->D != D || work->k < k || work->esz != sizeof(T) || (!basis && !work->v.p)) {
+    own.reset(new LanczosWork(D, k, hv_partials(t, 2), sizeof(T), basis == nullptr));
     work = own.get();
   }
   LanczosWork &W = *work;
@@ -164,11 +183,18 @@ LanczosResult pass_one_t(const Hubbard &h, const void *b, int k, hipStream_t s, 
   MB_HIP(hipStreamSynchronize(s));
   if (res.b_norm <= kTol) throw std::runtime_error("Input vector has zero norm");
   if (k == 0) return res;
-  launch_scale_div<T>((const double *)b, W.st.bnorm, W.v.as<T>(), 2 * D, s);
-  MB_HIP(hipMemsetAsync(W.vp.p, 0, (size_t)D * 2 * sizeof(T), s));
+  T *v = basis ? basis : W.v.as<T>(), *vp = basis ? basis : W.vp.as<T>(),
+    *w = basis ? basis + 2 * D : W.w.as<T>();
+  launch_scale_div<T>((const double *)b, W.st.bnorm, v, 2 * D, s);
+  // step 0 does not read v_prev (beta_{-1} = 0); the scratch copy is cleared anyway
+  if (!basis) MB_HIP(hipMemsetAsync(vp, 0, (size_t)D * 2 * sizeof(T), s));
   MB_HIP(hipMemsetAsync(W.st.flags, 0, 2 * sizeof(int), s));
-  T *v = W.v.as<T>(), *vp = W.vp.as<T>(), *w = W.w.as<T>();
   for (int i = 0; i < k; ++i) {
+    if (basis) {
+      v = basis + (int64_t)i * 2 * D;
+      vp = i > 0 ? v - 2 * D : v;
+      w = v + 2 * D;
+    }
     EpiArgs a;
     a.prev = vp;
     a.beta_dev = W.st.beta;
@@ -180,10 +206,12 @@ LanczosResult pass_one_t(const Hubbard &h, const void *b, int k, hipStream_t s, 
     launch_lanczos_axpy_norm<T>(v, w, 2 * D, W.st, i, s);
     launch_lanczos_finish_beta(W.st, i, s);
     launch_lanczos_normalize<T>(w, 2 * D, W.st, i, s);
-    T *tmp = vp;
-    vp = v;
-    v = w;
-    w = tmp;
+    if (!basis) {
+      T *tmp = vp;
+      vp = v;
+      v = w;
+      w = tmp;
+    }
   }
   std::vector<double> al(k), be(k);
   int flags[2];
@@ -336,9 +364,39 @@ void lanczos_pass_two(const Hubbard &h, const void *b, const LanczosResult &dec,
   pass_two_t<double>(h, b, dec, y, x, s, work);
 }
 
+// x = sum_j y_j V_j from the stored Krylov vectors (coefficients uploaded on s)
+template <typename T>
+void combine_basis(const T *V, const std::vector<std::complex<double>> &y, int64_t D, void *x,
+                   hipStream_t s) {
+  std::vector<double> yc(2 * y.size());
+  for (size_t j = 0; j < y.size(); ++j) {
+    yc[2 * j] = y[j].real();
+    yc[2 * j + 1] = y[j].imag();
+  }
+  DevBuf yd(yc.size() * sizeof(double));
+  MB_HIP(hipMemcpyAsync(yd.p, yc.data(), yc.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_basis_combine<T>(V, (int)y.size(), D, yd.as<double>(), (double *)x, s);
+  MB_HIP(hipStreamSynchronize(s));
+}
+
 double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipStream_t s) {
-  LanczosWork work(h.dim(), k, hv_partials(tables_of(h), 2));
-  LanczosResult dec = lanczos_pass_one(h, b, k, s, &work);
+  const int64_t D = h.dim();
+  // the eigenvector from the k + 1 stored Krylov vectors where they fit in HBM,
+  // else by regenerating them (lanczos_pass_two)
+  DevBuf V;
+  if (x && k > 0) {
+    const size_t bytes = (size_t)(k + 1) * 2 * D * sizeof(double);
+    if (keep_krylov_basis(bytes)) {
+      try {
+        V.alloc(bytes);
+      } catch (const NoMemError &) {
+        V.release();
+      }
+    }
+  }
+  LanczosWork work(D, k, hv_partials(tables_of(h), 2), sizeof(double), V.p == nullptr);
+  LanczosResult dec = V.p ? pass_one_t<double>(h, b, k, s, &work, V.as<double>())
+                          : lanczos_pass_one(h, b, k, s, &work);
   const int m = dec.steps;
   if (m == 0) {
     if (x) MB_HIP(hipMemsetAsync(x, 0, (size_t)h.dim() * 16, s));
@@ -346,12 +404,15 @@ double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipSt
     return 0.0;
   }
   if (!x) return tridiag_min_eig(m, dec.alphas.data(), dec.betas.data());
-  std::vector<double> wv(m), V((size_t)m * m);
-  tridiag_eig(m, dec.alphas.data(), dec.betas.data(), wv.data(), V.data());
+  std::vector<double> wv(m), Z((size_t)m * m);
+  tridiag_eig(m, dec.alphas.data(), dec.betas.data(), wv.data(), Z.data());
   {
     std::vector<std::complex<double>> y(m);
-    for (int i = 0; i < m; ++i) y[i] = V[i] * dec.b_norm;  // column 0: lowest
-    lanczos_pass_two(h, b, dec, y, x, s, &work);
+    for (int i = 0; i < m; ++i) y[i] = Z[i] * dec.b_norm;  // column 0: lowest
+    if (V.p)
+      combine_basis<double>(V.as<double>(), y, D, x, s);
+    else
+      lanczos_pass_two(h, b, dec, y, x, s, &work);
   }
   return wv[0];
 }
@@ -370,13 +431,28 @@ std::vector<std::complex<double>> expm_first_column(const LanczosResult &dec, do
   return y;
 }
 
+// `cache` (optional) keeps the stored Krylov basis between the steps of
+// time_evolve_steps; without one a basis is allocated for this call.
 template <typename T>
 void time_evolve_t(const Hubbard &h, const void *psi, double t, int krylov_steps, void *out,
-                   hipStream_t s) {
+                   hipStream_t s, DevBuf *cache = nullptr) {
   const int64_t D = h.dim();
   int k = krylov_steps;
   if (k == 0 || k > D) k = (int)D;
-  LanczosWork work(D, k, hv_partials(tables_of(h), 2), sizeof(T));
+  // the k + 1 Krylov vectors stay in HBM where they fit: psi(t) is then one
+  // pass over them instead of k - 1 more H.v (evolve_state.h:120-151 recomputes)
+  DevBuf own_basis;
+  DevBuf *basis = cache ? cache : &own_basis;
+  const size_t bytes = (size_t)(k + 1) * 2 * D * sizeof(T);
+  if (basis->bytes < bytes && keep_krylov_basis(bytes)) {
+    try {
+      basis->alloc(bytes);
+    } catch (const NoMemError &) {
+      basis->release();
+    }
+  }
+  const bool stored = basis->p && basis->bytes >= bytes;
+  LanczosWork work(D, k, hv_partials(tables_of(h), 2), sizeof(T), !stored);
   // norm first: a zero state is returned unchanged (evolve_state.h:131-134)
   double nrm = 0.0;
   launch_norm((const double *)psi, 2 * D, work.st.partial, work.st.bnorm, s);
@@ -387,7 +463,7 @@ void time_evolve_t(const Hubbard &h, const void *psi, double t, int krylov_steps
     MB_HIP(hipStreamSynchronize(s));
     return;
   }
-  LanczosResult dec = pass_one_t<T>(h, psi, k, s, &work);
+  LanczosResult dec = pass_one_t<T>(h, psi, k, s, &work, stored ? basis->as<T>() : nullptr);
   if (dec.steps == 0) {
     MB_HIP(hipMemsetAsync(out, 0, (size_t)D * 16, s));
     MB_HIP(hipStreamSynchronize(s));
@@ -395,7 +471,9 @@ void time_evolve_t(const Hubbard &h, const void *psi, double t, int krylov_steps
   }
   auto y = expm_first_column(dec, t);
   for (auto &v : y) v *= dec.b_norm;
-  if (out == psi) {
+  if (stored) {
+    combine_basis<T>(basis->as<T>(), y, D, out, s);  // psi is no longer read: out may be psi
+  } else if (out == psi) {
     DevBuf tmp((size_t)D * 16);
     pass_two_t<T>(h, psi, dec, y, tmp.p, s, &work);
     MB_HIP(hipMemcpyAsync(out, tmp.p, (size_t)D * 16, hipMemcpyDeviceToDevice, s));
@@ -424,9 +502,13 @@ void time_evolve_steps(const Hubbard &h, const void *psi, double t0, double t1, 
   const double direction = (t1 >= t0) ? 1.0 : -1.0;
   dt = std::fabs(dt) * direction;
   double t = t0;
+  DevBuf cache;  // the stored Krylov basis, allocated once for all steps
   while ((direction > 0 && t < t1) || (direction < 0 && t > t1)) {
     const double step = (direction > 0) ? std::min(dt, t1 - t) : std::max(dt, t1 - t);
-    time_evolve(h, out, step, krylov_steps, out, s, fp32);
+    if (fp32)
+      time_evolve_t<float>(h, out, step, krylov_steps, out, s, &cache);
+    else
+      time_evolve_t<double>(h, out, step, krylov_steps, out, s, &cache);
     t += step;
   }
 }

@@ -24,8 +24,14 @@ struct LanczosWork {
   size_t esz;  // bytes per vector element: 8 (FP64) or 4 (FP32 Krylov vectors)
   DevBuf v, vp, w, scal;
   LanczosState st;
-  LanczosWork(int64_t D, int k, int64_t nparts, size_t esz = sizeof(double));
+  // vectors = false: scalars only (the recurrence then runs in a stored basis)
+  LanczosWork(int64_t D, int k, int64_t nparts, size_t esz = sizeof(double), bool vectors = true);
 };
+
+// Whether k + 1 Krylov vectors of `bytes` should be kept in HBM: under a
+// request region always (the allocation itself decides), otherwise when they
+// take at most half of the free device memory.
+bool keep_krylov_basis(size_t bytes);
 
 LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream_t s,
                                LanczosWork *work = nullptr);

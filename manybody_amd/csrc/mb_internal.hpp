@@ -311,6 +311,9 @@ void launch_recon_update(T *w, double *x, double beta, double yr, double yi, int
                          hipStream_t s);
 template <typename T>
 void launch_scale_complex(const T *x, double yr, double yi, double *out, int64_t D, hipStream_t s);
+// x = sum_{j < m} y_j V_j over Krylov vectors kept in HBM (stride 2 D), y on the device
+template <typename T>
+void launch_basis_combine(const T *V, int m, int64_t D, const double *y, double *x, hipStream_t s);
 // out[r] = sum over row workgroups of planes 2r, 2r+1 of partial[2 wg + j] for j = 0,
 // out[nvec + r] likewise for j = 1.
 void launch_reduce_cheb(const double *partial, int64_t rows_per_plane, int nvec, double *out,

@@ -139,9 +139,13 @@ Footprint footprint_parts(const ServeRequest &r) {
   auto A = [](int64_t n) { return round_up(n + 16, kAlign); };
   auto species = [&](int64_t ds) {
     const int64_t nnz = ds * maxrow, dp = round_up(ds, 64);
+    const int64_t ds_sorted = (ds >= kSortedRound && round_up(ds, kSortedRound) <=
+                               (int64_t)kSortedRound * kSortedRounds) ? round_up(ds, kSortedRound) : 0;
     return A(4 * ds) /*strings*/ + A(4 * (ds + 1)) + A(4 * nnz) + A(8 * nnz) + A(4 * nnz) +
            A(8 * ds) + A(4 * ds) /*CSR, onsite*/ + A(4 * maxrow * ds) + A(ds) + A(8 * 128) +
-           A(4 * 128) /*ELL*/ + A(64 * dp) + A(4 * dp) + A(dp / 64) /*compact*/;
+           A(4 * 128) /*ELL*/ + A(64 * dp) + A(4 * dp) + A(dp / 64) /*compact*/ +
+           A(2 * nnz) /*16-bit*/ +
+           (ds_sorted ? A(2 * ds_sorted) + A(64 * ds_sorted) + A(2 * ds_sorted / 64) : 0) /*sorted*/;
   };
   const int64_t tables = species(du) + species(dd) + A(8 * 33 * 33) + 2 * A(8 * (L + 1));
   const int64_t vec = A(16 * D);
@@ -160,7 +164,13 @@ Footprint footprint_parts(const ServeRequest &r) {
                          A(8 * (int64_t)(r.moments / 2 + 1) * P);
     work = vec + std::max(lanczos_work(r.steps), cheb);
   } else if (r.kind == kReqEvolve) {
-    work = 2 * vec + std::max(lanczos_work(r.steps) + vec, A(8 * (2 * 512 + 8)));
+    // start + output vectors, then either the k + 1 stored Krylov vectors (kept
+    // between the time steps) with the Lanczos scalars, or the regenerating
+    // path's three Lanczos vectors and a temporary
+    const int64_t k = (r.steps <= 0 || r.steps > D) ? D : r.steps;
+    const int64_t scalars = A(8 * (2 * k + 2 + std::max<int64_t>(512, rows2))) + A(16 * k);
+    work = 2 * vec + std::max(A(16 * D * (k + 1)) + scalars,
+                              std::max(lanczos_work(r.steps) + vec, A(8 * (2 * 512 + 8))));
   } else {
     throw std::invalid_argument("unknown request kind");
   }

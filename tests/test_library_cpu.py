@@ -107,14 +107,14 @@ def test_jackson_kernel_and_rescaling_match_oracle():
 def test_request_footprint_covers_vectors_and_rejects_invalid():
     """Admission sizes (host arithmetic only): ground = start vector + three
     Lanczos vectors; KPM = start vector + max(Lanczos, three FP32 Chebyshev
-    plane sets); evolution = start + output + Lanczos + one temporary vector."""
+    plane sets); evolution = start + output + the k + 1 stored Krylov vectors."""
     D = 3432 ** 2  # L = 14, 7 + 7
     g = mb.Request(mb.REQ_GROUND, (14,), 7, 7, u=4.0, steps=150).footprint()
     assert 64 * D <= g <= 64 * D + (64 << 20)
     k = mb.Request(mb.REQ_KPM, (14,), 7, 7, steps=40, moments=256, nvec=4).footprint()
     assert 16 * D + 3 * 8 * 4 * D <= k <= 16 * D + 3 * 8 * 4 * D + (64 << 20)
     e = mb.Request(mb.REQ_EVOLVE, (14,), 7, 7, steps=30, t_final=1.0).footprint()
-    assert 96 * D <= e <= 96 * D + (64 << 20)
+    assert 16 * D * (2 + 31) <= e <= 16 * D * (2 + 31) + (64 << 20)
     assert mb.Request(mb.REQ_GROUND, (40,), 1, 1).footprint() == -1
     assert mb.Request(mb.REQ_GROUND, (8,), 9, 1).footprint() == -1
     assert mb.Request(7, (8,), 2, 2).footprint() == -1
