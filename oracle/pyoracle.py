@@ -324,3 +324,107 @@ def opposite_spin_correlation(sizes, n_up: int, n_dn: int, x_ref: np.ndarray) ->
     """<x|G_updn(r)|x> / <x|x> for every displacement r."""
     p = np.abs(np.asarray(x_ref)) ** 2
     return opposite_spin_correlation_diag(sizes, n_up, n_dn) @ p / p.sum()
+
+
+# ------------------------------------------------------------ momentum sectors
+# Fixed total momentum K (basis.cpp:128-185, 304-356; the DFS of the real-space
+# basis with a filter at the leaves, so the sector index is the rank among the
+# K-states in the real-space reference order) and the factorized momentum-space
+# Hubbard operator of hubbard_momentum_factorized_fixed_k.h:92-117:
+#   H = t sum_{k,s} eps(k) n_{k,s} + (U/N) sum_q rho_{q,up} rho_{-q,dn},
+#   eps(k) = -2 sum_d cos(2 pi k_d / L_d), rho_{q,s} = sum_k c+_{k+q,s} c_{k,s}.
+# Orbital k is the momentum with coordinates k = k_x + L_x k_y + L_x L_y k_z.
+
+def _dims3(sizes):
+    return list(sizes) + [1] * (3 - len(sizes))
+
+
+def momentum_add(sizes, k: int, q: int, sign: int = 1) -> int:
+    """Flat index of k + sign * q (component-wise, periodic)."""
+    Lx, Ly, Lz = _dims3(sizes)
+    kc = (k % Lx, (k // Lx) % Ly, k // (Lx * Ly))
+    qc = (q % Lx, (q // Lx) % Ly, q // (Lx * Ly))
+    c = [(kc[d] + sign * qc[d]) % (Lx, Ly, Lz)[d] for d in range(3)]
+    return c[0] + Lx * c[1] + Lx * Ly * c[2]
+
+
+def string_momentum(sizes, bits: int) -> int:
+    """Total momentum (flat) of an occupation string."""
+    tot = 0
+    k = 0
+    while bits:
+        if bits & 1:
+            tot = momentum_add(sizes, tot, k)
+        bits >>= 1
+        k += 1
+    return tot
+
+
+def dispersion(sizes, k: int) -> float:
+    """eps(k) = -2 sum_d cos(2 pi k_d / L_d) (hubbard_momentum_factorized_fixed_k.h:201-215)."""
+    Lx, Ly, Lz = _dims3(sizes)
+    kc = (k % Lx, (k // Lx) % Ly, k // (Lx * Ly))
+    e = 0.0
+    for d, L in enumerate(list(sizes)):
+        e += -2.0 * np.cos(2.0 * np.pi * kc[d] / L)
+    return e
+
+
+def momentum_sector_states(sizes, n_up: int, n_dn: int, K: int):
+    """(up, dn) bit strings of the sector, in reference index order."""
+    N = int(np.prod(sizes))
+    up, dn = basis_states(N, n_up, n_dn)
+    keep = [i for i in range(len(up))
+            if momentum_add(sizes, string_momentum(sizes, int(up[i])),
+                            string_momentum(sizes, int(dn[i]))) == K]
+    return [int(up[i]) for i in keep], [int(dn[i]) for i in keep]
+
+
+def _move(bits: int, k: int, kq: int):
+    """c+_{kq} c_k on one species' string: (new bits, sign) or None if zero.
+    sign = (-1)^(occupied orbitals strictly between k and kq)."""
+    if not (bits >> k) & 1:
+        return None
+    if kq == k:
+        return bits, 1
+    if (bits >> kq) & 1:
+        return None
+    lo, hi = min(k, kq), max(k, kq)
+    between = bin(bits & (((1 << hi) - 1) ^ ((1 << (lo + 1)) - 1))).count("1")
+    return (bits ^ (1 << k)) | (1 << kq), (-1) ** between
+
+
+def momentum_hamiltonian(sizes, n_up: int, n_dn: int, K: int, t: float, U: float):
+    """Dense matrix of the fixed-K factorized operator on the sector (reference order):
+    column j = H |j>, rho_{-q,dn} applied first, then rho_{q,up}."""
+    N = int(np.prod(sizes))
+    ups, dns = momentum_sector_states(sizes, n_up, n_dn, K)
+    idx = {(u, d): i for i, (u, d) in enumerate(zip(ups, dns))}
+    D = len(ups)
+    H = np.zeros((D, D), dtype=np.complex128)
+    eps = [dispersion(sizes, k) for k in range(N)]
+    for j, (u, d) in enumerate(zip(ups, dns)):
+        e = 0.0
+        for k in range(N):
+            if (u >> k) & 1:
+                e += eps[k]
+        for k in range(N):
+            if (d >> k) & 1:
+                e += eps[k]
+        H[j, j] += t * e
+        if abs(U) <= 1e-15:
+            continue
+        for q in range(N):
+            mq = momentum_add(sizes, 0, q, -1)
+            for kd in range(N):
+                rd = _move(d, kd, momentum_add(sizes, kd, mq))
+                if rd is None:
+                    continue
+                for ku in range(N):
+                    ru = _move(u, ku, momentum_add(sizes, ku, q))
+                    if ru is None:
+                        continue
+                    i = idx.get((ru[0], rd[0]))
+                    if i is not None:
+                        H[i, j] += U / N * ru[1] * rd[1]
+    return H

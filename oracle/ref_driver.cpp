@@ -9,11 +9,16 @@
 //   ref_driver h1d    L  Nup Ndn t U        -> states, then "E i j re im" per nonzero
 //   ref_driver h2d    Lx Ly Nup Ndn t U
 //   ref_driver h3d    Lx Ly Lz Nup Ndn t U
+//   ref_driver mbasis ndim L.. Nup Ndn K..          -> fixed-momentum basis states
+//   ref_driver mh     ndim L.. Nup Ndn K.. t U      -> states, then the momentum-space
+//                                                      Hamiltonian's nonzeros
 //
 // Reference entry points exercised:
 //   FermionBasis::with_fixed_particle_number_and_spin   src/algebra/basis.cpp:91-126
 //   HubbardModel / HubbardModel2D / HubbardModel3D      src/algebra/fermion/model/hubbard_model.h:10-212
 //   compute_matrix_elements<MatrixType>                  src/algebra/matrix_elements.h:74-105
+//   FermionBasis::with_fixed_particle_number_spin_momentum src/algebra/basis.cpp:128-185, 304-356
+//   HubbardModelMomentum::hamiltonian                    src/algebra/fermion/model/hubbard_model_momentum.h:119-175
 //
 // The reference key holds at most 7 operators and silently drops longer
 // products (src/algebra/fermion/term.h:11-19, src/algebra/expression_base.h:184-186),
@@ -29,6 +34,8 @@
 
 #include "algebra/fermion/basis.h"
 #include "algebra/fermion/model/hubbard_model.h"
+#include "algebra/fermion/model/hubbard_model_momentum.h"
+#include "utils/index.h"
 #include "algebra/matrix_elements.h"
 
 namespace {
@@ -73,17 +80,52 @@ void print_matrix(const FermionBasis& basis, const Model& model) {
 }
 
 int usage() {
-  std::fprintf(stderr, "usage: ref_driver basis|h1d|h2d|h3d|g1d|g2d|g3d ...\n");
+  std::fprintf(stderr, "usage: ref_driver basis|h1d|h2d|h3d|g1d|g2d|g3d|mbasis|mh ...\n");
   return 2;
 }
 
 }  // namespace
+
+// mbasis / mh: ndim L.. Nup Ndn K.. [t U]
+int momentum_mode(const std::string& mode, const std::vector<double>& a) {
+  if (a.empty()) return usage();
+  const size_t nd = (size_t)a[0];
+  const size_t need = 1 + nd + 2 + nd + (mode == "mh" ? 2 : 0);
+  if (nd < 1 || nd > 3 || a.size() != need) return usage();
+  std::vector<size_t> dims, K;
+  size_t sites = 1;
+  for (size_t d = 0; d < nd; ++d) {
+    dims.push_back((size_t)a[1 + d]);
+    sites *= dims.back();
+  }
+  const size_t nup = (size_t)a[1 + nd], ndn = (size_t)a[2 + nd];
+  for (size_t d = 0; d < nd; ++d) K.push_back((size_t)a[3 + nd + d]);
+  const size_t n = nup + ndn;
+  if (n > 7) { std::fprintf(stderr, "sector exceeds the 7-operator key\n"); return 3; }
+  double t = 0, u = 0;
+  if (mode == "mh") {
+    t = a[3 + 2 * nd];
+    u = a[4 + 2 * nd];
+    // the interaction adds 4 operators to a state's key, the kinetic term 2
+    if (n > (u == 0.0 ? 5u : 3u)) {
+      std::fprintf(stderr, "sector exceeds the exact range of compute_matrix_elements\n");
+      return 3;
+    }
+  }
+  const Index index(dims);
+  auto basis = FermionBasis::with_fixed_particle_number_spin_momentum(sites, n, (int)nup - (int)ndn,
+                                                                     index, K);
+  print_states(basis);
+  if (mode == "mh") print_operator(basis, HubbardModelMomentum(t, u, dims).hamiltonian());
+  return 0;
+}
 
 int main(int argc, char** argv) {
   if (argc < 2) return usage();
   const std::string mode = argv[1];
   std::vector<double> a;
   for (int i = 2; i < argc; ++i) a.push_back(std::atof(argv[i]));
+  if (mode == "mbasis" || mode == "mh") return momentum_mode(mode, a);
 
   size_t sites = 0, nup = 0, ndn = 0;
   double t = 0, u = 0;

@@ -13,6 +13,9 @@ Run in the build container only (the reference does not exist on the GPU box):
 Outputs: tests/golden/ref_basis.npz, tests/golden/ref_hamiltonian.npz,
 tests/golden/ref_correlation.npz (diagonal of the opposite-spin correlation
 operator G_updn(r) of hubbard_model.h, every displacement r, N_up + N_dn <= 3),
+tests/golden/ref_momentum.npz (fixed-momentum basis state lists for every K of
+ten lattices, and the momentum-space Hamiltonian HubbardModelMomentum on eleven
+sectors inside its exact range),
 tests/golden/manifest.json.
 """
 from __future__ import annotations
@@ -134,6 +137,40 @@ def main():
         corr_manifest.append({"name": name, "kind": kind, "dims": dims, "n_up": nu, "n_dn": nd,
                               "dim": int(len(st)), "displacements": N})
 
+    # fixed total momentum K: basis state lists (every K of each lattice) and the
+    # momentum-space Hamiltonian (HubbardModelMomentum) inside its exact range
+    mom_arrays, mom_manifest = {}, []
+    mbasis_cases = [([4], 2, 1), ([5], 2, 2), ([6], 3, 2), ([7], 2, 3), ([2, 2], 2, 1),
+                    ([3, 2], 2, 2), ([2, 3], 3, 1), ([3, 3], 2, 2), ([4, 2], 3, 2),
+                    ([2, 2, 2], 2, 2)]
+    for dims, nu, nd in mbasis_cases:
+        N = int(np.prod(dims))
+        for K in range(N):
+            kc = [K % dims[0]] + ([(K // dims[0]) % dims[1]] if len(dims) > 1 else []) + \
+                 ([K // (dims[0] * dims[1])] if len(dims) > 2 else [])
+            st, _ = run(["mbasis", len(dims)] + dims + [nu, nd] + kc)
+            name = f"mb_{'x'.join(map(str, dims))}_{nu}_{nd}_K{K}"
+            mom_arrays[name + "__states"] = st
+            mom_manifest.append({"name": name, "kind": "basis", "dims": dims, "n_up": nu,
+                                 "n_dn": nd, "K": K, "dim": int(len(st))})
+    mh_cases = [([2], 1, 1, 0, 0.0, 3.0), ([4], 1, 1, 1, 1.0, 4.0), ([4], 2, 1, 2, 1.0, 4.0),
+                ([5], 1, 2, 3, 0.7, 2.5), ([6], 2, 1, 0, 1.0, 8.0), ([4], 2, 2, 1, 1.5, 0.0),
+                ([6], 3, 2, 4, 1.0, 0.0), ([2, 2], 2, 1, 1, 1.0, 4.0), ([3, 2], 1, 2, 5, 1.0, 6.0),
+                ([3, 3], 2, 1, 4, 1.0, 8.0), ([2, 2, 2], 1, 2, 3, 1.0, 4.0)]
+    for dims, nu, nd, K, t, U in mh_cases:
+        kc = [K % dims[0]] + ([(K // dims[0]) % dims[1]] if len(dims) > 1 else []) + \
+             ([K // (dims[0] * dims[1])] if len(dims) > 2 else [])
+        st, el = run(["mh", len(dims)] + dims + [nu, nd] + kc + [t, U])
+        name = f"mh_{'x'.join(map(str, dims))}_{nu}_{nd}_K{K}_t{t}_U{U}"
+        el = np.array(el, np.float64).reshape(-1, 4)
+        mom_arrays[name + "__states"] = st
+        mom_arrays[name + "__ij"] = el[:, :2].astype(np.int64)
+        mom_arrays[name + "__val"] = el[:, 2] + 1j * el[:, 3]
+        mom_manifest.append({"name": name, "kind": "hamiltonian", "dims": dims, "n_up": nu,
+                             "n_dn": nd, "K": K, "t": t, "U": U, "dim": int(len(st)),
+                             "nnz": int(len(el))})
+
+    np.savez_compressed(os.path.join(HERE, "ref_momentum.npz"), **mom_arrays)
     np.savez_compressed(os.path.join(HERE, "ref_basis.npz"), **basis_arrays)
     np.savez_compressed(os.path.join(HERE, "ref_correlation.npz"), **corr_arrays)
     np.savez_compressed(os.path.join(HERE, "ref_hamiltonian.npz"), **ham_arrays)
@@ -141,7 +178,7 @@ def main():
         json.dump({"generator": "tests/golden/make_golden.py",
                    "source": "reference algebra core compiled by oracle/Makefile (target ref)",
                    "basis": basis_manifest, "hamiltonian": ham_manifest,
-                   "correlation": corr_manifest}, f, indent=1)
+                   "correlation": corr_manifest, "momentum": mom_manifest}, f, indent=1)
     print(f"{len(basis_manifest)} basis sectors, {len(ham_manifest)} Hamiltonians, "
           f"{len(corr_manifest)} correlation operators")
 

@@ -301,3 +301,59 @@ def test_opposite_spin_correlation_sum_rule(sizes, nu, nd):
     x = np.random.default_rng(1).normal(size=d.shape[1])
     g = po.opposite_spin_correlation(sizes, nu, nd, x)
     assert abs(g.sum() - nu * nd / N) < 1e-12
+
+
+# ------------------------------------------------------------ momentum sectors
+
+def test_momentum_basis_matches_reference_golden():
+    """Fixed-momentum bases of the compiled reference (every K of ten 1D/2D/3D
+    lattices): the oracle's filtered DFS order equals them, and the sectors of
+    one lattice partition the real-space basis."""
+    gold = np.load(os.path.join(GOLD, "ref_momentum.npz"))
+    totals = {}
+    for c in _manifest()["momentum"]:
+        if c["kind"] != "basis":
+            continue
+        up, dn = po.momentum_sector_states(c["dims"], c["n_up"], c["n_dn"], c["K"])
+        st = gold[c["name"] + "__states"]
+        assert np.array_equal(st[:, 0], np.asarray(up, np.uint64)), c["name"]
+        assert np.array_equal(st[:, 1], np.asarray(dn, np.uint64)), c["name"]
+        key = (tuple(c["dims"]), c["n_up"], c["n_dn"])
+        totals[key] = totals.get(key, 0) + len(up)
+    for (dims, nu, nd), tot in totals.items():
+        N = int(np.prod(dims))
+        assert tot == po.choose(N, nu) * po.choose(N, nd)
+
+
+def test_momentum_hamiltonian_matches_reference_golden():
+    """The oracle's factorized operator (rho_{-q,dn} then rho_{q,up}) against the
+    reference's symbolic HubbardModelMomentum matrix, every element (1e-12)."""
+    gold = np.load(os.path.join(GOLD, "ref_momentum.npz"))
+    n = 0
+    for c in _manifest()["momentum"]:
+        if c["kind"] != "hamiltonian":
+            continue
+        H = po.momentum_hamiltonian(c["dims"], c["n_up"], c["n_dn"], c["K"], c["t"], c["U"])
+        G = np.zeros_like(H)
+        ij, val = gold[c["name"] + "__ij"], gold[c["name"] + "__val"]
+        G[ij[:, 0], ij[:, 1]] = val
+        assert H.shape == (c["dim"], c["dim"]), c["name"]
+        assert np.abs(H - G).max() < 1e-12, c["name"]
+        n += 1
+    assert n == 11
+
+
+@pytest.mark.parametrize("sizes,nu,nd,U", [([6], 3, 2, 4.0), ([4], 2, 2, 6.0),
+                                           ([3, 2], 2, 2, 8.0), ([2, 2, 2], 2, 2, 4.0),
+                                           ([5], 2, 3, 3.0)])
+def test_momentum_sectors_reproduce_real_space_spectrum(sizes, nu, nd, U):
+    """The momentum-space operator is the real-space Hubbard model in another
+    basis: the union of the spectra of all K sectors equals the real-space
+    spectrum (pins the interaction's signs beyond the reference's exact range)."""
+    N = int(np.prod(sizes))
+    A = po.build_csc(sizes, nu, nd, 1.0, U).to_scipy().toarray()
+    w_real = np.linalg.eigvalsh(A)
+    w_k = np.concatenate([np.linalg.eigvalsh(po.momentum_hamiltonian(sizes, nu, nd, K, 1.0, U))
+                          for K in range(N)])
+    assert len(w_k) == len(w_real)
+    assert np.abs(np.sort(w_k) - w_real).max() < 1e-10
