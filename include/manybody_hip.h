@@ -215,6 +215,29 @@ int mb_jackson_kernel(int N, double *g);
 /* rescaling::from_bounds(E_min, E_max, padding) — rescaling.h:33-38. */
 int mb_rescaling_from_bounds(double e_min, double e_max, double padding, double *a, double *b);
 
+/* ------------------------------------------------------------ momentum sectors */
+
+/* Basis::with_fixed_particle_number_spin_momentum (basis.h:27-29,
+ * basis.cpp:128-185, 304-356): the states of total momentum `momentum`
+ * (ndim components) on the periodic lattice `sizes` (orbital k is the momentum
+ * k_x + L_x k_y + L_x L_y k_z), indexed in the reference's order (its real-space
+ * DFS order restricted to the sector).  The handle works with mb_basis_size,
+ * mb_basis_state and mb_basis_index_of; MB_ERR_INVALID for the real-space model
+ * constructors. */
+int mb_momentum_basis_create(int orbitals, int particles, int spin_projection, int ndim,
+                             const int *sizes, const int *momentum, mb_basis **out);
+/* HubbardMomentumFactorizedFixedK (hubbard_momentum_factorized_fixed_k.h:41-117):
+ * H = t sum_{k,s} eps(k) n_{k,s} + (U/N) sum_q rho_{q,up} rho_{-q,dn},
+ * eps(k) = -2 sum_d cos(2 pi k_d / L_d), on a fixed-momentum basis (sizes and
+ * momentum must be the basis's).  The handle works with the FP64 entry points:
+ * mb_hubbard_dimension, mb_hv_c128, mb_apply_ref_c128, the order conversions,
+ * mb_fill_gaussian_c128, Lanczos (plain and reorthogonalised), the eigenpair,
+ * FP64 time evolution and mb_energy_residual; the others return MB_ERR_INVALID.
+ * Device order: blocks of (up strings of momentum p) x (down strings of
+ * momentum K - p); mb_hubbard_shape reports (D, 1). */
+int mb_momentum_hubbard_create(const mb_basis *b, int ndim, const int *sizes, double t, double u,
+                               const int *momentum, mb_hubbard **out);
+
 /* ------------------------------------------------------------ observables */
 
 /* <x|G_updn(r)|x> / <x|x> for every lattice displacement r, written to g[r]

@@ -49,6 +49,10 @@ _SIGS = {
     "mb_basis_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     "mb_basis_destroy": (None, [_vp]),
     "mb_basis_size": (C.c_int64, [_vp]),
+    "mb_momentum_basis_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp,
+                                           C.POINTER(_vp)]),
+    "mb_momentum_hubbard_create": (C.c_int, [_vp, C.c_int, _vp, C.c_double, C.c_double, _vp,
+                                             C.POINTER(_vp)]),
     "mb_basis_state": (C.c_int, [_vp, C.c_int64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "mb_basis_index_of": (C.c_int, [_vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_int64)]),
     "mb_basis_product_index": (C.c_int, [_vp, C.c_int64, C.c_int64, _vp]),
@@ -170,6 +174,25 @@ class FermionBasis:
     def sector(orbitals: int, n_up: int, n_dn: int) -> "FermionBasis":
         return FermionBasis.with_fixed_particle_number_and_spin(orbitals, n_up + n_dn, n_up - n_dn)
 
+    @staticmethod
+    def with_fixed_particle_number_spin_momentum(orbitals: int, particles: int,
+                                                 spin_projection: int, lattice_size,
+                                                 momentum) -> "FermionBasis":
+        """Basis::with_fixed_particle_number_spin_momentum (basis.cpp:128-185): the
+        states of total momentum `momentum` on the periodic lattice `lattice_size`
+        (orbital k = the momentum k_x + L_x k_y + L_x L_y k_z), in the reference's
+        order (its real-space order restricted to the sector)."""
+        sizes = tuple(int(v) for v in lattice_size)
+        K = tuple(int(v) for v in momentum)
+        h = C.c_void_p()
+        _check(lib().mb_momentum_basis_create(orbitals, particles, spin_projection, len(sizes),
+                                              (C.c_int * len(sizes))(*sizes),
+                                              (C.c_int * len(K))(*K), C.byref(h)))
+        up = (particles + spin_projection) // 2
+        b = FermionBasis(h, orbitals, up, particles - up)
+        b.lattice_size, b.momentum = sizes, K
+        return b
+
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.mb_basis_destroy(self._h)
@@ -242,6 +265,20 @@ class HubbardModel3D(_HubbardBase):
     def __init__(self, t: float, u: float, size_x: int, size_y: int, size_z: int):
         self.t, self.u = float(t), float(u)
         self.sizes = (int(size_x), int(size_y), int(size_z))
+
+
+def HubbardMomentumFactorizedFixedK(basis: FermionBasis, lattice_size, t: float, u: float,
+                                    target_momentum) -> "HubbardOperator":
+    """HubbardMomentumFactorizedFixedK (hubbard_momentum_factorized_fixed_k.h:41-90):
+    H = t sum eps(k) n_k + (U/N) sum_q rho_{q,up} rho_{-q,dn} on a fixed-momentum
+    basis.  Returns an operator with the same methods as the real-space one
+    (FP64 H.v, orders, Lanczos, eigenpair, time evolution, energy/residual)."""
+    sizes = tuple(int(v) for v in lattice_size)
+    K = tuple(int(v) for v in target_momentum)
+    h = C.c_void_p()
+    _check(lib().mb_momentum_hubbard_create(basis._h, len(sizes), (C.c_int * len(sizes))(*sizes),
+                                            float(t), float(u), (C.c_int * len(K))(*K), C.byref(h)))
+    return HubbardOperator(h, basis, None)
 
 
 class HubbardOperator:

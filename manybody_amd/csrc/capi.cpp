@@ -10,13 +10,21 @@
 #include "observables.hpp"
 #include "krylov.hpp"
 #include "mb_internal.hpp"
+#include "momentum.hpp"
 
 struct mb_basis {
   mb::Sector sec;
+  std::unique_ptr<mb::MomentumSector> ms;  // set for a fixed-momentum basis
 };
 struct mb_hubbard {
-  mb::Hubbard h;
+  mb::Hubbard h;  // real-space model (unused for a momentum-sector operator)
   const mb_basis *basis;
+  std::unique_ptr<mb::RealHvOp> real_op;
+  std::unique_ptr<mb::MomentumHubbard> mom;
+  const mb::HvOp &op() const {
+    if (mom) return *mom;
+    return *real_op;
+  }
 };
 struct mb_batch {
   std::unique_ptr<mb::Batch> b;
@@ -68,9 +76,24 @@ void need(bool cond, const char *msg) {
 
 hipStream_t S(mb_stream s) { return reinterpret_cast<hipStream_t>(s); }
 
+// the real-space model; entry points without a momentum-sector form reject those
 const mb::Hubbard &H(const mb_hubbard *h) {
   need(h != nullptr, "null model handle");
+  need(!h->mom, "not available for momentum-sector operators");
   return h->h;
+}
+
+// the operator the Krylov layer runs on (either kind)
+const mb::HvOp &OP(const mb_hubbard *h) {
+  need(h != nullptr, "null model handle");
+  return h->op();
+}
+
+// the real-space sector of a basis; momentum bases are rejected
+const mb::Sector &SEC(const mb_basis *b) {
+  need(b != nullptr, "null basis handle");
+  need(!b->ms, "not available for momentum-sector bases");
+  return b->sec;
 }
 
 }  // namespace
@@ -98,14 +121,18 @@ int mb_basis_create(int orbitals, int particles, int spin_projection, mb_basis *
 
 void mb_basis_destroy(mb_basis *b) { delete b; }
 
-int64_t mb_basis_size(const mb_basis *b) { return b ? b->sec.dim() : -1; }
+int64_t mb_basis_size(const mb_basis *b) { return b ? (b->ms ? b->ms->D : b->sec.dim()) : -1; }
 
 int mb_basis_state(const mb_basis *b, int64_t index, uint64_t *up_bits, uint64_t *dn_bits) {
   return guarded([&] {
     need(b && up_bits && dn_bits, "null argument");
-    if (index < 0 || index >= b->sec.dim()) throw mb::RangeError("basis index out of range");
+    const int64_t D = b->ms ? b->ms->D : b->sec.dim();
+    if (index < 0 || index >= D) throw mb::RangeError("basis index out of range");
     uint32_t u, d;
-    b->sec.ref_state(index, &u, &d);
+    if (b->ms)
+      b->ms->ref_state(index, &u, &d);
+    else
+      b->sec.ref_state(index, &u, &d);
     *up_bits = u;
     *dn_bits = d;
   });
@@ -114,6 +141,16 @@ int mb_basis_state(const mb_basis *b, int64_t index, uint64_t *up_bits, uint64_t
 int mb_basis_index_of(const mb_basis *b, uint64_t up_bits, uint64_t dn_bits, int64_t *index) {
   return guarded([&] {
     need(b && index, "null argument");
+    if (b->ms) {
+      const auto &m = *b->ms;
+      const uint64_t mask = (m.L == 64) ? ~0ull : ((1ull << m.L) - 1);
+      const int64_t i = ((up_bits & ~mask) || (dn_bits & ~mask))
+                            ? -1
+                            : m.ref_index((uint32_t)up_bits, (uint32_t)dn_bits);
+      if (i < 0) throw mb::RangeError("state not in basis");
+      *index = i;
+      return;
+    }
     const auto &s = b->sec;
     const uint64_t mask = (s.L == 64) ? ~0ull : ((1ull << s.L) - 1);
     if ((up_bits & ~mask) || (dn_bits & ~mask) || __builtin_popcountll(up_bits) != s.n_up ||
@@ -126,7 +163,7 @@ int mb_basis_index_of(const mb_basis *b, uint64_t up_bits, uint64_t dn_bits, int
 int mb_basis_product_index(const mb_basis *b, int64_t first, int64_t n, int64_t *product_index) {
   return guarded([&] {
     need(b && product_index, "null argument");
-    const auto &s = b->sec;
+    const auto &s = SEC(b);
     if (first < 0 || n < 0 || first + n > s.dim()) throw mb::RangeError("index range out of bounds");
     for (int64_t i = 0; i < n; ++i) {
       uint32_t u, d;
@@ -141,6 +178,7 @@ static int hubbard_common(const mb_basis *b, std::vector<int> from, std::vector<
   return guarded([&] {
     need(b && out, "null argument");
     *out = nullptr;
+    SEC(b);
     for (size_t k = 0; k < from.size(); ++k)
       need(from[k] >= 0 && from[k] < b->sec.L && to[k] >= 0 && to[k] < b->sec.L,
            "bond endpoint outside the lattice");
@@ -155,6 +193,7 @@ static int hubbard_common(const mb_basis *b, std::vector<int> from, std::vector<
     for (int i = 0; i < ndim; ++i) h->h.lsize[i] = sizes[i];
     const_cast<mb::Sector &>(b->sec).ensure_device();
     mb::build_hubbard(h->h);
+    h->real_op = std::make_unique<mb::RealHvOp>(h->h);
     *out = h.release();
   });
 }
@@ -164,6 +203,7 @@ int mb_hubbard_create(const mb_basis *b, int ndim, const int *sizes, double t, d
   std::vector<int> from, to;
   int st = guarded([&] {
     need(b && sizes, "null argument");
+    SEC(b);
     need(ndim >= 1 && ndim <= 3, "ndim must be 1, 2 or 3");
     int prod = 1;
     for (int i = 0; i < ndim; ++i) {
@@ -189,11 +229,16 @@ int mb_hubbard_create_bonds(const mb_basis *b, int n_bonds, const int *from, con
 
 void mb_hubbard_destroy(mb_hubbard *h) { delete h; }
 
-int64_t mb_hubbard_dimension(const mb_hubbard *h) { return h ? h->h.dim() : -1; }
+int64_t mb_hubbard_dimension(const mb_hubbard *h) { return h ? h->op().dim() : -1; }
 
 int mb_hubbard_shape(const mb_hubbard *h, int64_t *d_up, int64_t *d_dn) {
   return guarded([&] {
     need(h && d_up && d_dn, "null argument");
+    if (h->mom) {  // a momentum sector is not one product block
+      *d_up = h->mom->dim();
+      *d_dn = 1;
+      return;
+    }
     *d_up = h->h.sec->d_up;
     *d_dn = h->h.sec->d_dn;
   });
@@ -202,8 +247,7 @@ int mb_hubbard_shape(const mb_hubbard *h, int64_t *d_up, int64_t *d_dn) {
 int mb_hv_c128(const mb_hubbard *h, const void *x_dev, void *y_dev, mb_stream stream) {
   return guarded([&] {
     need(x_dev && y_dev && x_dev != y_dev, "x and y must be distinct device pointers");
-    mb::launch_hv<double>(mb::tables_of(H(h)), 2, (const double *)x_dev, (double *)y_dev,
-                          mb::Epi::Plain, mb::EpiArgs{}, S(stream));
+    OP(h).hv((const double *)x_dev, (double *)y_dev, mb::Epi::Plain, mb::EpiArgs{}, S(stream));
   });
 }
 
@@ -225,10 +269,23 @@ int mb_hv_c64_multi(const mb_hubbard *h, int nvec, const void *x_dev, void *y_de
   });
 }
 
+namespace {
+const int64_t *mom_map(const mb_hubbard *h) {
+  auto &m = const_cast<mb::MomentumSector &>(h->mom->sector());
+  m.ensure_device();
+  return m.d_ref_of_pos.as<int64_t>();
+}
+}  // namespace
+
 int mb_to_device_order_c128(const mb_hubbard *h, const void *x_ref, void *x_dev, mb_stream stream) {
   return guarded([&] {
     need(x_ref && x_dev && x_ref != x_dev, "in and out must be distinct device pointers");
-    mb::launch_gather_c128(mb::perm_of(*H(h).sec), x_ref, x_dev, S(stream));
+    need(h != nullptr, "null model handle");
+    if (h->mom)
+      mb::launch_mom_perm(mom_map(h), h->mom->dim(), (const double *)x_ref, (double *)x_dev, true,
+                          S(stream));
+    else
+      mb::launch_gather_c128(mb::perm_of(*H(h).sec), x_ref, x_dev, S(stream));
   });
 }
 
@@ -236,7 +293,12 @@ int mb_to_reference_order_c128(const mb_hubbard *h, const void *x_dev, void *x_r
                                mb_stream stream) {
   return guarded([&] {
     need(x_ref && x_dev && x_ref != x_dev, "in and out must be distinct device pointers");
-    mb::launch_scatter_c128(mb::perm_of(*H(h).sec), x_dev, x_ref, S(stream));
+    need(h != nullptr, "null model handle");
+    if (h->mom)
+      mb::launch_mom_perm(mom_map(h), h->mom->dim(), (const double *)x_dev, (double *)x_ref, false,
+                          S(stream));
+    else
+      mb::launch_scatter_c128(mb::perm_of(*H(h).sec), x_dev, x_ref, S(stream));
   });
 }
 
@@ -244,6 +306,15 @@ int mb_apply_ref_c128(const mb_hubbard *h, const void *x_ref, void *y_ref, void 
                       mb_stream stream) {
   return guarded([&] {
     need(x_ref && y_ref && work, "null argument");
+    need(h != nullptr, "null model handle");
+    if (h->mom) {
+      const int64_t D = h->mom->dim();
+      double *w0 = (double *)work, *w1 = w0 + 2 * D;
+      mb::launch_mom_perm(mom_map(h), D, (const double *)x_ref, w0, true, S(stream));
+      h->mom->hv(w0, w1, mb::Epi::Plain, mb::EpiArgs{}, S(stream));
+      mb::launch_mom_perm(mom_map(h), D, w1, (double *)y_ref, false, S(stream));
+      return;
+    }
     const auto &hh = H(h);
     char *w0 = (char *)work, *w1 = w0 + (size_t)hh.dim() * 16;
     mb::launch_gather_c128(mb::perm_of(*hh.sec), x_ref, w0, S(stream));
@@ -256,7 +327,11 @@ int mb_apply_ref_c128(const mb_hubbard *h, const void *x_ref, void *y_ref, void 
 int mb_fill_gaussian_c128(const mb_hubbard *h, uint64_t seed, void *x_dev, mb_stream stream) {
   return guarded([&] {
     need(x_dev, "null argument");
-    mb::launch_fill_gaussian_c128(mb::perm_of(*H(h).sec), seed, x_dev, S(stream));
+    need(h != nullptr, "null model handle");
+    if (h->mom)
+      mb::launch_mom_fill_gaussian(mom_map(h), h->mom->dim(), seed, (double *)x_dev, S(stream));
+    else
+      mb::launch_fill_gaussian_c128(mb::perm_of(*H(h).sec), seed, x_dev, S(stream));
   });
 }
 
@@ -283,7 +358,7 @@ int mb_lanczos_pass_one(const mb_hubbard *h, const void *b_dev, int k, double *a
   return guarded([&] {
     need(b_dev && alphas && betas && steps && b_norm, "null argument");
     need(k >= 0, "k must be non-negative");
-    auto r = mb::lanczos_pass_one(H(h), b_dev, k, S(stream));
+    auto r = mb::lanczos_pass_one(OP(h), b_dev, k, S(stream));
     *steps = r.steps;
     *b_norm = r.b_norm;
     for (int i = 0; i < k; ++i) {
@@ -298,7 +373,7 @@ int mb_find_min_eigenpair(const mb_hubbard *h, const void *b_dev, int k, double 
   return guarded([&] {
     need(b_dev && value, "null argument");
     need(k >= 0, "k must be non-negative");
-    *value = mb::find_min_eigenpair(H(h), b_dev, k, x_dev, S(stream));
+    *value = mb::find_min_eigenpair(OP(h), b_dev, k, x_dev, S(stream));
   });
 }
 
@@ -309,8 +384,8 @@ int mb_lanczos_pass_one_ex(const mb_hubbard *h, const void *b_dev, int k, int fl
     need(k >= 0, "k must be non-negative");
     need((flags & ~MB_LANCZOS_REORTHOGONALIZE) == 0, "unknown Lanczos flags");
     auto r = (flags & MB_LANCZOS_REORTHOGONALIZE)
-                 ? mb::lanczos_pass_one_reorth(H(h), b_dev, k, S(stream))
-                 : mb::lanczos_pass_one(H(h), b_dev, k, S(stream));
+                 ? mb::lanczos_pass_one_reorth(OP(h), b_dev, k, S(stream))
+                 : mb::lanczos_pass_one(OP(h), b_dev, k, S(stream));
     *steps = r.steps;
     *b_norm = r.b_norm;
     for (int i = 0; i < k; ++i) {
@@ -327,8 +402,8 @@ int mb_find_min_eigenpair_ex(const mb_hubbard *h, const void *b_dev, int k, int 
     need(k >= 0, "k must be non-negative");
     need((flags & ~MB_LANCZOS_REORTHOGONALIZE) == 0, "unknown Lanczos flags");
     *value = (flags & MB_LANCZOS_REORTHOGONALIZE)
-                 ? mb::find_min_eigenpair_reorth(H(h), b_dev, k, x_dev, S(stream))
-                 : mb::find_min_eigenpair(H(h), b_dev, k, x_dev, S(stream));
+                 ? mb::find_min_eigenpair_reorth(OP(h), b_dev, k, x_dev, S(stream))
+                 : mb::find_min_eigenpair(OP(h), b_dev, k, x_dev, S(stream));
   });
 }
 
@@ -337,7 +412,7 @@ int mb_time_evolve(const mb_hubbard *h, const void *psi_dev, double t, int krylo
   return guarded([&] {
     need(psi_dev && out_dev, "null argument");
     need(krylov_steps >= 0, "krylov_steps must be non-negative");
-    mb::time_evolve(H(h), psi_dev, t, krylov_steps, out_dev, S(stream));
+    mb::time_evolve(OP(h), psi_dev, t, krylov_steps, out_dev, S(stream));
   });
 }
 
@@ -347,7 +422,7 @@ int mb_time_evolve_ex(const mb_hubbard *h, const void *psi_dev, double t, int kr
     need(psi_dev && out_dev, "null argument");
     need(krylov_steps >= 0, "krylov_steps must be non-negative");
     need((flags & ~MB_EVOLVE_FP32) == 0, "unknown time evolution flags");
-    mb::time_evolve(H(h), psi_dev, t, krylov_steps, out_dev, S(stream), (flags & MB_EVOLVE_FP32) != 0);
+    mb::time_evolve(OP(h), psi_dev, t, krylov_steps, out_dev, S(stream), (flags & MB_EVOLVE_FP32) != 0);
   });
 }
 
@@ -357,7 +432,7 @@ int mb_time_evolve_steps_ex(const mb_hubbard *h, const void *psi_dev, double t0,
     need(psi_dev && out_dev, "null argument");
     need(krylov_steps >= 0, "krylov_steps must be non-negative");
     need((flags & ~MB_EVOLVE_FP32) == 0, "unknown time evolution flags");
-    mb::time_evolve_steps(H(h), psi_dev, t0, t1, dt, krylov_steps, out_dev, S(stream),
+    mb::time_evolve_steps(OP(h), psi_dev, t0, t1, dt, krylov_steps, out_dev, S(stream),
                           (flags & MB_EVOLVE_FP32) != 0);
   });
 }
@@ -367,7 +442,7 @@ int mb_time_evolve_steps(const mb_hubbard *h, const void *psi_dev, double t0, do
   return guarded([&] {
     need(psi_dev && out_dev, "null argument");
     need(krylov_steps >= 0, "krylov_steps must be non-negative");
-    mb::time_evolve_steps(H(h), psi_dev, t0, t1, dt, krylov_steps, out_dev, S(stream));
+    mb::time_evolve_steps(OP(h), psi_dev, t0, t1, dt, krylov_steps, out_dev, S(stream));
   });
 }
 
@@ -587,6 +662,56 @@ int mb_energy_residual(const mb_hubbard *h, const void *x_dev, double *energy, d
                        mb_stream stream) {
   return guarded([&] {
     need(h && x_dev && energy && residual, "null argument");
-    mb::energy_residual(H(h), x_dev, energy, residual, S(stream));
+    mb::energy_residual(OP(h), x_dev, energy, residual, S(stream));
+  });
+}
+
+// ------------------------------------------------------------ momentum sectors
+
+int mb_momentum_basis_create(int orbitals, int particles, int spin_projection, int ndim,
+                             const int *sizes, const int *momentum, mb_basis **out) {
+  return guarded([&] {
+    need(out && sizes && momentum, "null argument");
+    *out = nullptr;
+    need(ndim >= 1 && ndim <= 3, "ndim must be 1, 2 or 3");
+    int prod = 1;
+    for (int i = 0; i < ndim; ++i) {
+      need(sizes[i] >= 1, "Dimension with size zero is invalid.");
+      prod *= sizes[i];
+    }
+    need(prod == orbitals, "index.size() must equal orbitals");
+    need(particles >= 0 && particles <= 2 * orbitals, "particles must be in [0, 2*orbitals]");
+    need(std::abs(spin_projection) <= particles, "|spin_projection| must not exceed particles");
+    need((particles + spin_projection) % 2 == 0, "particles + spin_projection must be even");
+    const int up = (particles + spin_projection) / 2, dn = particles - up;
+    auto b = std::make_unique<mb_basis>();
+    b->ms = std::make_unique<mb::MomentumSector>();
+    mb::build_momentum_sector(*b->ms, ndim, sizes, up, dn, momentum);
+    *out = b.release();
+  });
+}
+
+int mb_momentum_hubbard_create(const mb_basis *b, int ndim, const int *sizes, double t, double u,
+                               const int *momentum, mb_hubbard **out) {
+  return guarded([&] {
+    need(b && sizes && momentum && out, "null argument");
+    *out = nullptr;
+    need(b->ms != nullptr, "the basis is not a fixed-momentum basis");
+    const auto &m = *b->ms;
+    need(ndim == m.ndim, "Target momentum dimensionality must match lattice dimensionality.");
+    int K = 0, stride = 1;
+    for (int i = 0; i < ndim; ++i) {
+      need(sizes[i] == m.dims[i], "lattice size does not match the basis");
+      need(momentum[i] >= 0 && momentum[i] < sizes[i],
+           "Target momentum components must be less than lattice size.");
+      K += stride * momentum[i];
+      stride *= sizes[i];
+    }
+    need(K == m.K, "target momentum does not match the basis");
+    auto h = std::make_unique<mb_hubbard>();
+    h->basis = b;
+    h->mom = std::make_unique<mb::MomentumHubbard>(m, t, u);
+    const_cast<mb::MomentumSector &>(m).ensure_device();
+    *out = h.release();
   });
 }

@@ -37,6 +37,7 @@
 
 #include "krylov.hpp"
 #include "mb_internal.hpp"
+#include "momentum.hpp"
 
 namespace mb {
 
@@ -945,6 +946,123 @@ __global__ __launch_bounds__(kBlock) void reduce_cheb_kernel(const double *parti
   }
 }
 
+// ------------------------------------------------------------ momentum sectors
+
+// H.v of the factorized momentum-space operator (momentum.cpp).  One workgroup
+// per output up string g (block p = its momentum, row r inside the block); its
+// columns are the down strings of momentum K - p.  For each q != 0 every source
+// up string of g under rho_{q,up} (block p - q) has its row staged in LDS (both
+// planes), and each column gathers its rho_{-q,dn} sources from it; the sum is
+// scaled by U/N and the diagonal t (eps_up + eps_dn) + (U/N) N_up N_dn added,
+// then the epilogue runs with one partial sum per plane.
+template <Epi E, int CPT>
+__global__ __launch_bounds__(kBlock) void mom_hv_kernel(MomTables t, const double *__restrict__ x,
+                                                        double *__restrict__ y, EpiArgs a) {
+  extern __shared__ double smem[];  // 2 * max_cols: re then im of the staged source row
+  __shared__ double red[2][kBlock / kWave];
+  if constexpr (E == Epi::Lanczos) {
+    if (a.flags[0]) return;
+  }
+  const int g = blockIdx.x, N = t.N, tid = threadIdx.x;
+  const int p = t.up_grp[g];
+  const int nc = t.ncols[p];
+  const int64_t D = t.D;
+  const int64_t base = t.blk_off[p] + (int64_t)t.up_loc[g] * nc;
+  const int dg0 = t.dn_off[t.msub[t.K * N + p]];  // first down string of the columns
+  double ar[CPT], ai[CPT];
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) ar[j] = ai[j] = 0.0;
+  for (int q = 1; q < N; ++q) {
+    const int ps = t.msub[p * N + q];
+    const int ncs = t.ncols[ps];
+    const int64_t gq = (int64_t)g * N + q;
+    const int cu = t.up_cnt[gq];
+    for (int i = 0; i < cu; ++i) {
+      const uint32_t e = t.up_src[gq * t.mu + i];
+      const double su = (e & 0x8000u) ? -1.0 : 1.0;
+      const int64_t src = t.blk_off[ps] + (int64_t)(e & 0x7FFFu) * ncs;
+      __syncthreads();  // the previous row is no longer read
+      for (int c = tid; c < ncs; c += kBlock) {
+        smem[c] = x[src + c];
+        smem[ncs + c] = x[D + src + c];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        const int c = tid + j * kBlock;
+        if (c < nc) {
+          const int64_t hq = (int64_t)(dg0 + c) * N + q;
+          const int cd = t.dn_cnt[hq];
+          double sr = 0.0, si = 0.0;
+          for (int l = 0; l < cd; ++l) {
+            const uint32_t e2 = t.dn_src[hq * t.md + l];
+            const int lc = (int)(e2 & 0x7FFFu);
+            const double sg = (e2 & 0x8000u) ? -1.0 : 1.0;
+            sr += sg * smem[lc];
+            si += sg * smem[ncs + lc];
+          }
+          ar[j] += su * sr;
+          ai[j] += su * si;
+        }
+      }
+    }
+  }
+  const double uo = t.U / N, eu = t.eps_up[g];
+  const double bp = beta_prev<double, E>(a);
+  Partial2 accr, acci;
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    const int c = tid + j * kBlock;
+    if (c < nc) {
+      const int64_t i = base + c;
+      const double xr = x[i], xi = x[D + i];
+      const double dg = t.t * (eu + t.eps_dn[dg0 + c]) + t.q0;
+      epilogue<double, E>(dg * xr + uo * ar[j], xr, i, y, a, bp, accr);
+      epilogue<double, E>(dg * xi + uo * ai[j], xi, D + i, y, a, bp, acci);
+    }
+  }
+  if constexpr (E == Epi::Lanczos) {
+    const double sr = wg_sum(accr.s0, red[0]);
+    const double si = wg_sum(acci.s0, red[1]);
+    if (tid == 0) {
+      a.partial[g] = sr;
+      a.partial[t.nrows + g] = si;
+    }
+  }
+}
+
+// planar device order <-> interleaved reference order through the sector's map
+template <bool TO_DEVICE>
+__global__ __launch_bounds__(kBlock) void mom_perm_kernel(const int64_t *__restrict__ ref_of_pos,
+                                                          int64_t D, const double *__restrict__ src,
+                                                          double *__restrict__ dst) {
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
+    const int64_t r = ref_of_pos[e];
+    if constexpr (TO_DEVICE) {
+      dst[e] = src[2 * r];
+      dst[D + e] = src[2 * r + 1];
+    } else {
+      dst[2 * r] = src[e];
+      dst[2 * r + 1] = src[D + e];
+    }
+  }
+}
+
+// seeded N(0,1) per reference index, the generator of fill_gaussian_kernel
+__global__ __launch_bounds__(kBlock) void mom_fill_gaussian_kernel(const int64_t *__restrict__ ref_of_pos,
+                                                                   int64_t D, uint64_t seed,
+                                                                   double *__restrict__ x) {
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
+    const uint64_t i = (uint64_t)ref_of_pos[e];
+    const double u1 = unit_open(mix64(seed, 2 * i));
+    const double u2 = unit_half(mix64(seed, 2 * i + 1));
+    const double rad = sqrt(-2.0 * log(u1));
+    const double th = 6.283185307179586 * u2;
+    x[e] = rad * cos(th);
+    x[D + e] = rad * sin(th);
+  }
+}
+
 bool force_generic() {
   const char *e = std::getenv("MANYBODY_AMD_HV_PATH");
   return e && std::strcmp(e, "generic") == 0;
@@ -1079,6 +1197,60 @@ int64_t launch_hv(const HvTables &t, int P, const T *x, T *y, Epi epi, const Epi
   }
   MB_HIP(hipGetLastError());
   return g.x;
+}
+
+template <Epi E, int CPT>
+void launch_mom_cpt(const MomTables &t, const double *x, double *y, const EpiArgs &a, hipStream_t s) {
+  const size_t lds = (size_t)2 * t.max_cols * sizeof(double);
+  hipLaunchKernelGGL((mom_hv_kernel<E, CPT>), dim3((unsigned)t.nrows), dim3(kBlock), lds, s, t, x, y, a);
+}
+
+template <Epi E>
+void launch_mom_e(const MomTables &t, const double *x, double *y, const EpiArgs &a, hipStream_t s) {
+  const int cpt = (t.max_cols + kBlock - 1) / kBlock;
+  if (cpt <= 1)
+    launch_mom_cpt<E, 1>(t, x, y, a, s);
+  else if (cpt <= 2)
+    launch_mom_cpt<E, 2>(t, x, y, a, s);
+  else if (cpt <= 4)
+    launch_mom_cpt<E, 4>(t, x, y, a, s);
+  else if (cpt <= 8)
+    launch_mom_cpt<E, 8>(t, x, y, a, s);
+  else if (cpt <= 16)
+    launch_mom_cpt<E, 16>(t, x, y, a, s);
+  else
+    throw std::invalid_argument("momentum group too large for the H.v kernel (more than 4096 strings)");
+}
+
+int64_t launch_mom_hv(const MomTables &t, const double *x, double *y, Epi epi, const EpiArgs &a,
+                      hipStream_t s) {
+  if (t.nrows >= (1ll << 31)) throw std::invalid_argument("sector too large");
+  switch (epi) {
+    case Epi::Plain: launch_mom_e<Epi::Plain>(t, x, y, a, s); break;
+    case Epi::Lanczos: launch_mom_e<Epi::Lanczos>(t, x, y, a, s); break;
+    case Epi::Recon: launch_mom_e<Epi::Recon>(t, x, y, a, s); break;
+    case Epi::Cheb: throw std::invalid_argument("KPM is not available for momentum-sector operators");
+  }
+  MB_HIP(hipGetLastError());
+  return t.nrows;  // partial slots per plane
+}
+
+void launch_mom_perm(const int64_t *map, int64_t D, const double *src, double *dst, bool to_device,
+                     hipStream_t s) {
+  if (to_device)
+    hipLaunchKernelGGL((mom_perm_kernel<true>), dim3(grid_for(D, 1 << 16)), dim3(kBlock), 0, s, map, D,
+                       src, dst);
+  else
+    hipLaunchKernelGGL((mom_perm_kernel<false>), dim3(grid_for(D, 1 << 16)), dim3(kBlock), 0, s, map, D,
+                       src, dst);
+  MB_HIP(hipGetLastError());
+}
+
+void launch_mom_fill_gaussian(const int64_t *ref_of_pos, int64_t D, uint64_t seed, double *x,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(mom_fill_gaussian_kernel, dim3(grid_for(D, 1 << 16)), dim3(kBlock), 0, s,
+                     ref_of_pos, D, seed, x);
+  MB_HIP(hipGetLastError());
 }
 
 template int64_t launch_hv<double>(const HvTables &, int, const double *, double *, Epi,

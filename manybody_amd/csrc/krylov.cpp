@@ -167,13 +167,12 @@ bool keep_krylov_basis(size_t bytes) {
 // step i reads v_i = basis[i] and v_{i-1}, and writes v_{i+1} into basis[i + 1],
 // so the Krylov vectors stay in HBM for the reconstruction.
 template <typename T>
-LanczosResult pass_one_t(const Hubbard &h, const void *b, int k, hipStream_t s, LanczosWork *work,
+LanczosResult pass_one_t(const HvOp &op, const void *b, int k, hipStream_t s, LanczosWork *work,
                          T *basis = nullptr) {
-  const int64_t D = h.dim();
+  const int64_t D = op.dim();
   std::unique_ptr<LanczosWork> own;
-  const HvTables t = tables_of(h);
   if (!work || work->D != D || work->k < k || work->esz != sizeof(T) || (!basis && !work->v.p)) {
-    own.reset(new LanczosWork(D, k, hv_partials(t, 2), sizeof(T), basis == nullptr));
+    own.reset(new LanczosWork(D, k, op.partials(), sizeof(T), basis == nullptr));
     work = own.get();
   }
   LanczosWork &W = *work;
@@ -201,7 +200,7 @@ LanczosResult pass_one_t(const Hubbard &h, const void *b, int k, hipStream_t s, 
     a.flags = W.st.flags;
     a.step = i;
     a.partial = W.st.partial;
-    const int64_t slots = launch_hv<T>(t, 2, v, w, Epi::Lanczos, a, s);
+    const int64_t slots = op.hv(v, w, Epi::Lanczos, a, s);
     launch_lanczos_finish_alpha(W.st, i, 2 * slots, s);
     launch_lanczos_axpy_norm<T>(v, w, 2 * D, W.st, i, s);
     launch_lanczos_finish_beta(W.st, i, s);
@@ -225,9 +224,9 @@ LanczosResult pass_one_t(const Hubbard &h, const void *b, int k, hipStream_t s, 
   return res;
 }
 
-LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream_t s,
+LanczosResult lanczos_pass_one(const HvOp &op, const void *b, int k, hipStream_t s,
                                LanczosWork *work) {
-  return pass_one_t<double>(h, b, k, s, work);
+  return pass_one_t<double>(op, b, k, s, work);
 }
 
 // lanczos_pass_one with full reorthogonalisation (an option the reference does
@@ -236,11 +235,10 @@ LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream
 // (classical Gram-Schmidt twice, FP64 on the device), then beta = |w|.  All
 // k + 1 vectors stay in HBM (16 D (k + 1) bytes); they are handed to `basis`
 // when it is given.
-LanczosResult lanczos_pass_one_reorth(const Hubbard &h, const void *b, int k, hipStream_t s,
+LanczosResult lanczos_pass_one_reorth(const HvOp &op, const void *b, int k, hipStream_t s,
                                       DevBuf *basis) {
-  const int64_t D = h.dim();
-  const HvTables t = tables_of(h);
-  const int64_t nparts = hv_partials(t, 2);
+  const int64_t D = op.dim();
+  const int64_t nparts = op.partials();
   const int kk = std::max(k, 1);
   DevBuf V((size_t)(kk + 1) * 16 * (size_t)D);
   DevBuf scal(sizeof(double) * (2 * (size_t)kk + 2 + (size_t)std::max<int64_t>(kRedBlocks, nparts)) + 16);
@@ -270,7 +268,7 @@ LanczosResult lanczos_pass_one_reorth(const Hubbard &h, const void *b, int k, hi
     a.flags = st.flags;
     a.step = i;
     a.partial = st.partial;
-    const int64_t slots = launch_hv<double>(t, 2, v, w, Epi::Lanczos, a, s);
+    const int64_t slots = op.hv(v, w, Epi::Lanczos, a, s);
     launch_lanczos_finish_alpha(st, i, 2 * slots, s);
     launch_lanczos_axpy_norm(v, w, 2 * D, st, i, s);
     for (int pass = 0; pass < 2; ++pass)
@@ -294,10 +292,10 @@ LanczosResult lanczos_pass_one_reorth(const Hubbard &h, const void *b, int k, hi
 
 // Lowest eigenpair from the reorthogonalised decomposition: x = |b| sum_j y_j V_j
 // straight from the stored vectors (no second pass).
-double find_min_eigenpair_reorth(const Hubbard &h, const void *b, int k, void *x, hipStream_t s) {
-  const int64_t D = h.dim();
+double find_min_eigenpair_reorth(const HvOp &op, const void *b, int k, void *x, hipStream_t s) {
+  const int64_t D = op.dim();
   DevBuf V;
-  const LanczosResult dec = lanczos_pass_one_reorth(h, b, k, s, &V);
+  const LanczosResult dec = lanczos_pass_one_reorth(op, b, k, s, &V);
   const int m = dec.steps;
   if (m == 0) {
     if (x) MB_HIP(hipMemsetAsync(x, 0, (size_t)D * 16, s));
@@ -320,9 +318,9 @@ double find_min_eigenpair_reorth(const Hubbard &h, const void *b, int k, void *x
 // Pass two: regenerates the Lanczos vectors (in T) and accumulates
 // x = sum_j y_j v_j in FP64.
 template <typename T>
-void pass_two_t(const Hubbard &h, const void *b, const LanczosResult &dec,
+void pass_two_t(const HvOp &op, const void *b, const LanczosResult &dec,
                 const std::vector<std::complex<double>> &y, void *x, hipStream_t s, LanczosWork *work) {
-  const int64_t D = h.dim();
+  const int64_t D = op.dim();
   if ((int)y.size() != dec.steps)
     throw std::runtime_error("Dimension mismatch: y_k size must match decomposition steps");
   if (dec.b_norm <= kTol) throw std::runtime_error("Input vector has zero norm");
@@ -330,10 +328,9 @@ void pass_two_t(const Hubbard &h, const void *b, const LanczosResult &dec,
     MB_HIP(hipMemsetAsync(x, 0, (size_t)D * 16, s));
     return;
   }
-  const HvTables t = tables_of(h);
   std::unique_ptr<LanczosWork> own;
-  if (!work || work->D != D || work->esz != sizeof(T)) {
-    own.reset(new LanczosWork(D, 1, hv_partials(t, 2), sizeof(T)));
+  if (!work || work->D != D || work->esz != sizeof(T) || !work->v.p) {
+    own.reset(new LanczosWork(D, 1, op.partials(), sizeof(T)));
     work = own.get();
   }
   LanczosWork &W = *work;
@@ -348,7 +345,7 @@ void pass_two_t(const Hubbard &h, const void *b, const LanczosResult &dec,
     a.prev = vp;
     a.alpha = aj;
     a.beta = bp;
-    launch_hv<T>(t, 2, v, w, Epi::Recon, a, s);
+    op.hv(v, w, Epi::Recon, a, s);
     launch_recon_update<T>(w, (double *)x, bj, y[j + 1].real(), y[j + 1].imag(), D, s);
     T *tmp = vp;
     vp = v;
@@ -358,10 +355,10 @@ void pass_two_t(const Hubbard &h, const void *b, const LanczosResult &dec,
   MB_HIP(hipStreamSynchronize(s));
 }
 
-void lanczos_pass_two(const Hubbard &h, const void *b, const LanczosResult &dec,
+void lanczos_pass_two(const HvOp &op, const void *b, const LanczosResult &dec,
                       const std::vector<std::complex<double>> &y, void *x, hipStream_t s,
                       LanczosWork *work) {
-  pass_two_t<double>(h, b, dec, y, x, s, work);
+  pass_two_t<double>(op, b, dec, y, x, s, work);
 }
 
 // x = sum_j y_j V_j from the stored Krylov vectors (coefficients uploaded on s)
@@ -379,8 +376,8 @@ void combine_basis(const T *V, const std::vector<std::complex<double>> &y, int64
   MB_HIP(hipStreamSynchronize(s));
 }
 
-double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipStream_t s) {
-  const int64_t D = h.dim();
+double find_min_eigenpair(const HvOp &op, const void *b, int k, void *x, hipStream_t s) {
+  const int64_t D = op.dim();
   // the eigenvector from the k + 1 stored Krylov vectors where they fit in HBM,
   // else by regenerating them (lanczos_pass_two)
   DevBuf V;
@@ -394,12 +391,12 @@ double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipSt
       }
     }
   }
-  LanczosWork work(D, k, hv_partials(tables_of(h), 2), sizeof(double), V.p == nullptr);
-  LanczosResult dec = V.p ? pass_one_t<double>(h, b, k, s, &work, V.as<double>())
-                          : lanczos_pass_one(h, b, k, s, &work);
+  LanczosWork work(D, k, op.partials(), sizeof(double), V.p == nullptr);
+  LanczosResult dec = V.p ? pass_one_t<double>(op, b, k, s, &work, V.as<double>())
+                          : lanczos_pass_one(op, b, k, s, &work);
   const int m = dec.steps;
   if (m == 0) {
-    if (x) MB_HIP(hipMemsetAsync(x, 0, (size_t)h.dim() * 16, s));
+    if (x) MB_HIP(hipMemsetAsync(x, 0, (size_t)op.dim() * 16, s));
     MB_HIP(hipStreamSynchronize(s));
     return 0.0;
   }
@@ -412,7 +409,7 @@ double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipSt
     if (V.p)
       combine_basis<double>(V.as<double>(), y, D, x, s);
     else
-      lanczos_pass_two(h, b, dec, y, x, s, &work);
+      lanczos_pass_two(op, b, dec, y, x, s, &work);
   }
   return wv[0];
 }
@@ -434,9 +431,9 @@ std::vector<std::complex<double>> expm_first_column(const LanczosResult &dec, do
 // `cache` (optional) keeps the stored Krylov basis between the steps of
 // time_evolve_steps; without one a basis is allocated for this call.
 template <typename T>
-void time_evolve_t(const Hubbard &h, const void *psi, double t, int krylov_steps, void *out,
+void time_evolve_t(const HvOp &op, const void *psi, double t, int krylov_steps, void *out,
                    hipStream_t s, DevBuf *cache = nullptr) {
-  const int64_t D = h.dim();
+  const int64_t D = op.dim();
   int k = krylov_steps;
   if (k == 0 || k > D) k = (int)D;
   // the k + 1 Krylov vectors stay in HBM where they fit: psi(t) is then one
@@ -452,7 +449,7 @@ void time_evolve_t(const Hubbard &h, const void *psi, double t, int krylov_steps
     }
   }
   const bool stored = basis->p && basis->bytes >= bytes;
-  LanczosWork work(D, k, hv_partials(tables_of(h), 2), sizeof(T), !stored);
+  LanczosWork work(D, k, op.partials(), sizeof(T), !stored);
   // norm first: a zero state is returned unchanged (evolve_state.h:131-134)
   double nrm = 0.0;
   launch_norm((const double *)psi, 2 * D, work.st.partial, work.st.bnorm, s);
@@ -463,7 +460,7 @@ void time_evolve_t(const Hubbard &h, const void *psi, double t, int krylov_steps
     MB_HIP(hipStreamSynchronize(s));
     return;
   }
-  LanczosResult dec = pass_one_t<T>(h, psi, k, s, &work, stored ? basis->as<T>() : nullptr);
+  LanczosResult dec = pass_one_t<T>(op, psi, k, s, &work, stored ? basis->as<T>() : nullptr);
   if (dec.steps == 0) {
     MB_HIP(hipMemsetAsync(out, 0, (size_t)D * 16, s));
     MB_HIP(hipStreamSynchronize(s));
@@ -475,25 +472,25 @@ void time_evolve_t(const Hubbard &h, const void *psi, double t, int krylov_steps
     combine_basis<T>(basis->as<T>(), y, D, out, s);  // psi is no longer read: out may be psi
   } else if (out == psi) {
     DevBuf tmp((size_t)D * 16);
-    pass_two_t<T>(h, psi, dec, y, tmp.p, s, &work);
+    pass_two_t<T>(op, psi, dec, y, tmp.p, s, &work);
     MB_HIP(hipMemcpyAsync(out, tmp.p, (size_t)D * 16, hipMemcpyDeviceToDevice, s));
     MB_HIP(hipStreamSynchronize(s));
   } else {
-    pass_two_t<T>(h, psi, dec, y, out, s, &work);
+    pass_two_t<T>(op, psi, dec, y, out, s, &work);
   }
 }
 
-void time_evolve(const Hubbard &h, const void *psi, double t, int krylov_steps, void *out,
+void time_evolve(const HvOp &op, const void *psi, double t, int krylov_steps, void *out,
                  hipStream_t s, bool fp32) {
   if (fp32)
-    time_evolve_t<float>(h, psi, t, krylov_steps, out, s);
+    time_evolve_t<float>(op, psi, t, krylov_steps, out, s);
   else
-    time_evolve_t<double>(h, psi, t, krylov_steps, out, s);
+    time_evolve_t<double>(op, psi, t, krylov_steps, out, s);
 }
 
-void time_evolve_steps(const Hubbard &h, const void *psi, double t0, double t1, double dt,
+void time_evolve_steps(const HvOp &op, const void *psi, double t0, double t1, double dt,
                        int krylov_steps, void *out, hipStream_t s, bool fp32) {
-  const int64_t D = h.dim();
+  const int64_t D = op.dim();
   if (out != psi) MB_HIP(hipMemcpyAsync(out, psi, (size_t)D * 16, hipMemcpyDeviceToDevice, s));
   if (dt == 0.0 || (t1 - t0) == 0.0) {
     MB_HIP(hipStreamSynchronize(s));
@@ -506,9 +503,9 @@ void time_evolve_steps(const Hubbard &h, const void *psi, double t0, double t1, 
   while ((direction > 0 && t < t1) || (direction < 0 && t > t1)) {
     const double step = (direction > 0) ? std::min(dt, t1 - t) : std::max(dt, t1 - t);
     if (fp32)
-      time_evolve_t<float>(h, out, step, krylov_steps, out, s, &cache);
+      time_evolve_t<float>(op, out, step, krylov_steps, out, s, &cache);
     else
-      time_evolve_t<double>(h, out, step, krylov_steps, out, s, &cache);
+      time_evolve_t<double>(op, out, step, krylov_steps, out, s, &cache);
     t += step;
   }
 }

@@ -9,6 +9,32 @@
 
 namespace mb {
 
+// The operator the Krylov layer runs on (the reference's LinearOperator,
+// linear_operator.h:34-43): one complex H.v over device-order planes with a
+// fused epilogue; returns the partial-sum slots it wrote per plane.
+struct HvOp {
+  virtual ~HvOp() = default;
+  virtual int64_t dim() const = 0;
+  virtual int64_t partials() const = 0;  // partial slots one H.v may write (both planes)
+  virtual int64_t hv(const double *x, double *y, Epi e, const EpiArgs &a, hipStream_t s) const = 0;
+  virtual int64_t hv(const float *x, float *y, Epi e, const EpiArgs &a, hipStream_t s) const = 0;
+};
+
+// The real-space Hubbard operator (two-pass H.v)
+struct RealHvOp final : HvOp {
+  explicit RealHvOp(const Hubbard &h) : D_(h.dim()), t_(tables_of(h)) {}
+  int64_t dim() const override { return D_; }
+  int64_t partials() const override { return hv_partials(t_, 2); }
+  int64_t hv(const double *x, double *y, Epi e, const EpiArgs &a, hipStream_t s) const override {
+    return launch_hv<double>(t_, 2, x, y, e, a, s);
+  }
+  int64_t hv(const float *x, float *y, Epi e, const EpiArgs &a, hipStream_t s) const override {
+    return launch_hv<float>(t_, 2, x, y, e, a, s);
+  }
+  int64_t D_;
+  HvTables t_;
+};
+
 // LanczosDecomposition (lanczos.h:60-68)
 struct LanczosResult {
   std::vector<double> alphas, betas;
@@ -33,24 +59,40 @@ struct LanczosWork {
 // take at most half of the free device memory.
 bool keep_krylov_basis(size_t bytes);
 
-LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream_t s,
+LanczosResult lanczos_pass_one(const HvOp &op, const void *b, int k, hipStream_t s,
                                LanczosWork *work = nullptr);
-void lanczos_pass_two(const Hubbard &h, const void *b, const LanczosResult &dec,
+void lanczos_pass_two(const HvOp &op, const void *b, const LanczosResult &dec,
                       const std::vector<std::complex<double>> &y, void *x, hipStream_t s,
                       LanczosWork *work = nullptr);
 double tridiag_min_eig(int m, const double *a, const double *b);
-double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipStream_t s);
+double find_min_eigenpair(const HvOp &op, const void *b, int k, void *x, hipStream_t s);
 // Reorthogonalised variants (all Lanczos vectors kept in HBM; see krylov.cpp).
-LanczosResult lanczos_pass_one_reorth(const Hubbard &h, const void *b, int k, hipStream_t s,
+LanczosResult lanczos_pass_one_reorth(const HvOp &op, const void *b, int k, hipStream_t s,
                                       DevBuf *basis = nullptr);
-double find_min_eigenpair_reorth(const Hubbard &h, const void *b, int k, void *x, hipStream_t s);
+double find_min_eigenpair_reorth(const HvOp &op, const void *b, int k, void *x, hipStream_t s);
 std::vector<std::complex<double>> expm_first_column(const LanczosResult &dec, double t);
 // fp32: Krylov vectors and H.v in FP32; the state, norms, alpha/beta
 // reductions and the exponential of T_k stay FP64.
-void time_evolve(const Hubbard &h, const void *psi, double t, int krylov_steps, void *out,
+void time_evolve(const HvOp &op, const void *psi, double t, int krylov_steps, void *out,
                  hipStream_t s, bool fp32 = false);
-void time_evolve_steps(const Hubbard &h, const void *psi, double t0, double t1, double dt,
+void time_evolve_steps(const HvOp &op, const void *psi, double t0, double t1, double dt,
                        int krylov_steps, void *out, hipStream_t s, bool fp32 = false);
+// the same on the real-space Hubbard operator
+inline LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream_t s,
+                                      LanczosWork *work = nullptr) {
+  return lanczos_pass_one(RealHvOp(h), b, k, s, work);
+}
+inline double find_min_eigenpair(const Hubbard &h, const void *b, int k, void *x, hipStream_t s) {
+  return find_min_eigenpair(RealHvOp(h), b, k, x, s);
+}
+inline void time_evolve(const Hubbard &h, const void *psi, double t, int krylov_steps, void *out,
+                        hipStream_t s, bool fp32 = false) {
+  time_evolve(RealHvOp(h), psi, t, krylov_steps, out, s, fp32);
+}
+inline void time_evolve_steps(const Hubbard &h, const void *psi, double t0, double t1, double dt,
+                              int krylov_steps, void *out, hipStream_t s, bool fp32 = false) {
+  time_evolve_steps(RealHvOp(h), psi, t0, t1, dt, krylov_steps, out, s, fp32);
+}
 void kpm_dos_moments(const Hubbard &h, const Sector &sec, double a, double b, int M, int nvec,
                      uint64_t seed, bool fp32, double *mu, hipStream_t s);
 

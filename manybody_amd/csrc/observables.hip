@@ -180,14 +180,14 @@ std::vector<double> opposite_spin_correlation(const Hubbard &h, const void *x, h
   return g;
 }
 
-void energy_residual(const Hubbard &h, const void *x, double *energy, double *residual,
+void energy_residual(const HvOp &op, const void *x, double *energy, double *residual,
                      hipStream_t s) {
-  const int64_t D = h.dim();
+  const int64_t D = op.dim();
   DevBuf y((size_t)D * 16);
   DevBuf part((size_t)2 * kObsBlocks * sizeof(double));
   DevBuf sums(4 * sizeof(double));
   EpiArgs a;
-  launch_hv<double>(tables_of(h), 2, (const double *)x, y.as<double>(), Epi::Plain, a, s);
+  op.hv((const double *)x, y.as<double>(), Epi::Plain, a, s);
   hipLaunchKernelGGL(rayleigh_partial_kernel, dim3(kObsBlocks), dim3(kBlock), 0, s, (const double *)x,
                      (const double *)y.as<double>(), 2 * D, part.as<double>());
   hipLaunchKernelGGL(rows_final_kernel, dim3(2), dim3(kBlock), 0, s, (const double *)part.as<double>(),
