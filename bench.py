@@ -41,6 +41,7 @@ def main():
     ap.add_argument("--c2-cpu-energy", action="store_true",
                     help="also run the oracle's 300-step OpenMP Lanczos on the host (minutes)")
     ap.add_argument("--no-kpm", action="store_true")
+    ap.add_argument("--no-momentum", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-n", type=int, default=200)
     ap.add_argument("--serving-rate", type=float, default=40.0)
@@ -122,6 +123,7 @@ def main():
                                    args.cpu_threads)
     observables = None if args.no_lanczos else bench_observables(op, x)
     del y
+    momentum = None if args.no_momentum else bench_momentum()
     batched = None if args.no_batched else bench_batched(args.batched_n)
     kpm = None if args.no_kpm else bench_kpm_c3()
     krylov = None if args.no_kpm else bench_krylov_fp32()
@@ -148,6 +150,7 @@ def main():
         "cpu_baseline": cpu,
         "lanczos_4x4": lanczos,
         "observables_4x4": observables,
+        "momentum_4x4_K00": momentum,
         "batched": batched,
         "kpm_l14": kpm,
         "krylov_l14": krylov,
@@ -282,6 +285,39 @@ def bench_observables(op, x, reps=5):
             "correlation_GBps": round(16.0 * D / tc / 1e9, 1),
             "sum_rule_error": abs(float(g.sum()) - 49 / 16),
             "energy_residual_ms": round(te * 1e3, 3), "energy": e, "residual": r}
+
+
+def bench_momentum(steps=20, k=300):
+    """Fixed-momentum sector K = (0, 0) of the 4x4 7+7 model (D_K ~ D / 16): host
+    table build, FP64 H.v (32 D_K algorithmic bytes) and a 300-step Lanczos."""
+    import torch
+
+    import manybody_amd as mb
+    t0 = time.perf_counter()
+    basis = mb.FermionBasis.with_fixed_particle_number_spin_momentum(16, 14, 0, (4, 4), (0, 0))
+    op = mb.HubbardMomentumFactorizedFixedK(basis, (4, 4), 1.0, 8.0, (0, 0))
+    t_build = time.perf_counter() - t0
+    D = op.dimension()
+    x = op.gaussian(seed=20260101)
+    y = torch.empty_like(x)
+    for _ in range(2):
+        op.hv(x, y)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        op.hv(x, y)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    c0 = time.perf_counter()
+    dec = mb.lanczos_pass_one(op, x, k)
+    t_l = time.perf_counter() - c0
+    w, _ = mb.tridiag_eig(dec.alphas, dec.betas, vectors=False)
+    return {"workload": "4x4 7+7 U=8, total momentum K=(0,0), factorized momentum-space H",
+            "D_K": D, "build_s": round(t_build, 2), "hv_ms": round(ms, 4),
+            "hv_algorithmic_GBps": round(32.0 * D / ms / 1e6, 1),
+            "lanczos_steps": dec.steps_taken, "lanczos_s": round(t_l, 3), "E0_K00": float(w[0])}
 
 
 def bench_kpm_c3(M=1024, R=16, seed=7):

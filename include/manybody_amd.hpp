@@ -117,6 +117,19 @@ class FermionBasis {
     check(mb_basis_create((int)orbitals, (int)particles, spin_projection, &b));
     return FermionBasis(b, orbitals, particles);
   }
+  // Basis::with_fixed_particle_number_spin_momentum (basis.h:27-29): total momentum
+  // `momentum` on the periodic lattice `lattice_size` (the reference's Index)
+  static FermionBasis with_fixed_particle_number_spin_momentum(
+      size_t orbitals, size_t particles, int spin_projection, const std::vector<int> &lattice_size,
+      const std::vector<int> &momentum) {
+    mb_basis *b = nullptr;
+    if (momentum.size() != lattice_size.size())
+      throw std::invalid_argument("momentum and lattice dimensionality differ");
+    check(mb_momentum_basis_create((int)orbitals, (int)particles, spin_projection,
+                                   (int)lattice_size.size(), lattice_size.data(), momentum.data(),
+                                   &b));
+    return FermionBasis(b, orbitals, particles);
+  }
   FermionBasis(FermionBasis &&o) noexcept
       : h_(std::exchange(o.h_, nullptr)), orbitals(o.orbitals), particles(o.particles) {}
   FermionBasis(const FermionBasis &) = delete;
@@ -158,6 +171,20 @@ class HubbardOperator {
   }
   HubbardOperator(HubbardOperator &&o) noexcept
       : h_(std::exchange(o.h_, nullptr)), sites_(o.sites_) {}
+  // HubbardMomentumFactorizedFixedK(basis, lattice_size, t, U, target_momentum)
+  // (hubbard_momentum_factorized_fixed_k.h:41-90) on a fixed-momentum basis
+  static HubbardOperator momentum_factorized_fixed_k(const FermionBasis &basis,
+                                                     const std::vector<int> &lattice_size,
+                                                     double t, double u,
+                                                     const std::vector<int> &target_momentum) {
+    if (target_momentum.size() != lattice_size.size())
+      throw std::invalid_argument(
+          "Target momentum dimensionality must match lattice dimensionality.");
+    mb_hubbard *h = nullptr;
+    check(mb_momentum_hubbard_create(basis.handle(), (int)lattice_size.size(), lattice_size.data(),
+                                     t, u, target_momentum.data(), &h));
+    return HubbardOperator(h, basis.orbitals);
+  }
   HubbardOperator(const HubbardOperator &) = delete;
   ~HubbardOperator() { mb_hubbard_destroy(h_); }
 
@@ -204,9 +231,18 @@ class HubbardOperator {
   const mb_hubbard *handle() const { return h_; }
 
  private:
+  HubbardOperator(mb_hubbard *h, size_t sites) : h_(h), sites_(sites) {}
   mb_hubbard *h_ = nullptr;
   size_t sites_ = 0;
 };
+
+// HubbardMomentumFactorizedFixedK: a HubbardOperator on a fixed-momentum basis
+inline HubbardOperator HubbardMomentumFactorizedFixedK(const FermionBasis &basis,
+                                                       const std::vector<int> &lattice_size,
+                                                       double t, double u,
+                                                       const std::vector<int> &target_momentum) {
+  return HubbardOperator::momentum_factorized_fixed_k(basis, lattice_size, t, u, target_momentum);
+}
 
 struct HubbardModel {
   HubbardModel(double t_val, double u_val, size_t size_val) : t(t_val), u(u_val), size(size_val) {}

@@ -950,15 +950,17 @@ __global__ __launch_bounds__(kBlock) void reduce_cheb_kernel(const double *parti
 
 // H.v of the factorized momentum-space operator (momentum.cpp).  One workgroup
 // per output up string g (block p = its momentum, row r inside the block); its
-// columns are the down strings of momentum K - p.  For each q != 0 every source
-// up string of g under rho_{q,up} (block p - q) has its row staged in LDS (both
-// planes), and each column gathers its rho_{-q,dn} sources from it; the sum is
-// scaled by U/N and the diagonal t (eps_up + eps_dn) + (U/N) N_up N_dn added,
-// then the epilogue runs with one partial sum per plane.
+// columns are the down strings of momentum K - p.  For each q != 0 the source
+// rows of g under rho_{q,up} (block p - q) are first combined in LDS with their
+// signs, comb[d] = sum_u s_u x[u, d] (coalesced loads, both planes), then each
+// column gathers its rho_{-q,dn} sources from comb: sum_u sum_d s_u s_d x[u, d]
+// = sum_d s_d comb[d].  The sum is scaled by U/N and the diagonal t (eps_up +
+// eps_dn) + (U/N) N_up N_dn added, then the epilogue runs with one partial sum
+// per plane.
 template <Epi E, int CPT>
 __global__ __launch_bounds__(kBlock) void mom_hv_kernel(MomTables t, const double *__restrict__ x,
                                                         double *__restrict__ y, EpiArgs a) {
-  extern __shared__ double smem[];  // 2 * max_cols: re then im of the staged source row
+  extern __shared__ double smem[];  // 2 * max_cols: re then im of the combined source row
   __shared__ double red[2][kBlock / kWave];
   if constexpr (E == Epi::Lanczos) {
     if (a.flags[0]) return;
@@ -977,32 +979,45 @@ __global__ __launch_bounds__(kBlock) void mom_hv_kernel(MomTables t, const doubl
     const int ncs = t.ncols[ps];
     const int64_t gq = (int64_t)g * N + q;
     const int cu = t.up_cnt[gq];
-    for (int i = 0; i < cu; ++i) {
-      const uint32_t e = t.up_src[gq * t.mu + i];
-      const double su = (e & 0x8000u) ? -1.0 : 1.0;
-      const int64_t src = t.blk_off[ps] + (int64_t)(e & 0x7FFFu) * ncs;
-      __syncthreads();  // the previous row is no longer read
-      for (int c = tid; c < ncs; c += kBlock) {
-        smem[c] = x[src + c];
-        smem[ncs + c] = x[D + src + c];
+    if (cu == 0) continue;  // uniform over the workgroup
+    const uint16_t *__restrict__ us = t.up_src + gq * t.mu;
+    const double *__restrict__ xb = x + t.blk_off[ps];
+    __syncthreads();  // the previous comb is no longer read
+    for (int c = tid; c < ncs; c += kBlock) {
+      double cr = 0.0, ci = 0.0;
+      for (int i = 0; i < cu; ++i) {
+        const uint32_t e = us[i];
+        const int64_t o = (int64_t)(e & 0x7FFFu) * ncs + c;
+        const double vr = xb[o], vi = xb[D + o];
+        if (e & 0x8000u) {
+          cr -= vr;
+          ci -= vi;
+        } else {
+          cr += vr;
+          ci += vi;
+        }
       }
-      __syncthreads();
+      smem[c] = cr;
+      smem[ncs + c] = ci;
+    }
+    __syncthreads();
 #pragma unroll
-      for (int j = 0; j < CPT; ++j) {
-        const int c = tid + j * kBlock;
-        if (c < nc) {
-          const int64_t hq = (int64_t)(dg0 + c) * N + q;
-          const int cd = t.dn_cnt[hq];
-          double sr = 0.0, si = 0.0;
-          for (int l = 0; l < cd; ++l) {
-            const uint32_t e2 = t.dn_src[hq * t.md + l];
-            const int lc = (int)(e2 & 0x7FFFu);
-            const double sg = (e2 & 0x8000u) ? -1.0 : 1.0;
-            sr += sg * smem[lc];
-            si += sg * smem[ncs + lc];
+    for (int j = 0; j < CPT; ++j) {
+      const int c = tid + j * kBlock;
+      if (c < nc) {
+        const int64_t hq = (int64_t)(dg0 + c) * N + q;
+        const int cd = t.dn_cnt[hq];
+        const uint16_t *__restrict__ ds = t.dn_src + hq * t.md;
+        for (int l = 0; l < cd; ++l) {
+          const uint32_t e2 = ds[l];
+          const int lc = (int)(e2 & 0x7FFFu);
+          if (e2 & 0x8000u) {
+            ar[j] -= smem[lc];
+            ai[j] -= smem[ncs + lc];
+          } else {
+            ar[j] += smem[lc];
+            ai[j] += smem[ncs + lc];
           }
-          ar[j] += su * sr;
-          ai[j] += su * si;
         }
       }
     }

@@ -76,6 +76,36 @@ int main() {
     CHECK(g.size() == 8 && std::abs(gs - 2.0) < 1e-12);
   }
   {
+    CASE("momentum_sector_ground_state");
+    // 1D ring L = 8, 4 + 4, U = 4: the lowest energy over the momentum sectors is
+    // the real-space ground energy
+    auto rbasis = FermionBasis::with_fixed_particle_number_and_spin(8, 8, 0);
+    auto R = HubbardModel(1.0, 4.0, 8).op(rbasis);
+    const double e_real = find_min_eigenpair(R, 200, R.gaussian(5)).value;
+    double e_min = 1e300;
+    size_t dims = 0;
+    for (int K = 0; K < 8; ++K) {
+      auto mbasis = FermionBasis::with_fixed_particle_number_spin_momentum(8, 8, 0, {8}, {K});
+      auto M = HubbardMomentumFactorizedFixedK(mbasis, {8}, 1.0, 4.0, {K});
+      dims += M.dimension();
+      CHECK(mbasis.size() == M.dimension());
+      auto st = mbasis.state(0);
+      CHECK(mbasis.index_of(st.first, st.second) == 0);
+      const int k = (int)std::min<size_t>(200, M.dimension());
+      e_min = std::min(e_min, find_min_eigenpair(M, k, M.gaussian(7 + K)).value);
+    }
+    CHECK(dims == 4900);
+    CHECK(std::abs(e_min - e_real) < 1e-10 * std::abs(e_real));
+    bool thrown = false;
+    try {
+      auto bad = FermionBasis::with_fixed_particle_number_spin_momentum(8, 8, 0, {8}, {0});
+      (void)HubbardMomentumFactorizedFixedK(bad, {8}, 1.0, 4.0, {3});
+    } catch (const std::invalid_argument &) {
+      thrown = true;
+    }
+    CHECK(thrown);
+  }
+  {
     CASE("lanczos_decomposition_shape");
     auto basis = FermionBasis::with_fixed_particle_number_and_spin(6, 5, 1);
     auto H = HubbardModel2D(1.0, 4.0, 3, 2).op(basis);
