@@ -428,3 +428,68 @@ def momentum_hamiltonian(sizes, n_up: int, n_dn: int, K: int, t: float, U: float
                     if i is not None:
                         H[i, j] += U / N * ru[1] * rd[1]
     return H
+
+
+# ------------------------------------------------------------ optical conductivity
+# Current operator J_d(Q) = sum_{k,s} v_d(k) c+_{k+Q,s} c_{k,s}, sector K -> K + Q,
+# v_d(k) = 2 t (2 pi / L_d) sin(2 pi k_d / L_d) (current_operator_fixed_k.h:14-293),
+# and the damped Fourier transform of the current-current correlator
+# (optical_conductivity.cpp:8-80).
+
+def velocity(sizes, t: float, direction: int, k: int) -> float:
+    Lx, Ly, Lz = _dims3(sizes)
+    kc = (k % Lx, (k // Lx) % Ly, k // (Lx * Ly))
+    L = list(sizes)[direction]
+    return 2.0 * t * (2.0 * np.pi / L) * np.sin(2.0 * np.pi * kc[direction] / L)
+
+
+def current_operator(sizes, n_up: int, n_dn: int, K: int, Q: int, direction: int, t: float):
+    """Dense J_d(Q) from sector K (columns) to sector K + Q (rows), reference order."""
+    N = int(np.prod(sizes))
+    src_u, src_d = momentum_sector_states(sizes, n_up, n_dn, K)
+    KQ = momentum_add(sizes, K, Q)
+    tgt_u, tgt_d = momentum_sector_states(sizes, n_up, n_dn, KQ)
+    idx = {(u, d): i for i, (u, d) in enumerate(zip(tgt_u, tgt_d))}
+    J = np.zeros((len(tgt_u), len(src_u)), dtype=np.complex128)
+    v = [velocity(sizes, t, direction, k) for k in range(N)]
+    for j, (u, d) in enumerate(zip(src_u, src_d)):
+        for spin in (0, 1):
+            bits = u if spin == 0 else d
+            for k in range(N):
+                if abs(v[k]) < 1e-15:
+                    continue
+                r = _move(bits, k, momentum_add(sizes, k, Q))
+                if r is None:
+                    continue
+                key = (r[0], d) if spin == 0 else (u, r[0])
+                i = idx.get(key)
+                if i is not None:
+                    J[i, j] += r[1] * v[k]
+    return J
+
+
+def optical_conductivity(correlation, dt: float, beta: float, volume: float):
+    """(frequencies, sigma, sum_rule) of optical_conductivity.cpp:8-80: trapezoid
+    weights, damping exp(-eta t) with eta = 4 / T, omega from 0 to pi/dt in steps
+    of eta / 4, prefactor (1 - exp(-beta omega)) / (omega V) (beta / V at 0)."""
+    c = np.asarray(correlation, dtype=np.complex128)
+    n = len(c)
+    if n < 2 or not dt > 0 or not volume > 0 or not beta >= 0:
+        raise ValueError("invalid arguments")
+    T = (n - 1) * dt
+    eta = 4.0 / T
+    dw = eta / 4.0
+    nf = int(np.floor((np.pi / dt) / dw)) + 1
+    w = np.full(n, dt)
+    w[0] *= 0.5
+    w[-1] *= 0.5
+    series = c * w * np.exp(-eta * dt) ** np.arange(n)
+    freqs = np.arange(nf) * dw
+    sigma = np.zeros(nf, dtype=np.complex128)
+    for i, om in enumerate(freqs):
+        integral = np.sum(series * np.exp(1j * om * dt * np.arange(n)))
+        pref = beta / volume if abs(om) < 1e-12 else -np.expm1(-beta * om) / (om * volume)
+        sigma[i] = pref * integral
+    wts = np.ones(nf)
+    wts[0] = wts[-1] = 0.5
+    return freqs, sigma, float(np.sum(sigma.real * wts) * dw)

@@ -12,6 +12,11 @@
 //   ref_driver mbasis ndim L.. Nup Ndn K..          -> fixed-momentum basis states
 //   ref_driver mh     ndim L.. Nup Ndn K.. t U      -> states, then the momentum-space
 //                                                      Hamiltonian's nonzeros
+//   ref_driver mj     ndim L.. Nup Ndn Q.. dir t    -> states of the full (all-momentum)
+//                                                      basis, then the current operator's
+//                                                      nonzeros
+//   ref_driver sigma  dt beta volume < "re im" lines -> "W omega re im" per frequency,
+//                                                      then "R sum_rule"
 //
 // Reference entry points exercised:
 //   FermionBasis::with_fixed_particle_number_and_spin   src/algebra/basis.cpp:91-126
@@ -19,6 +24,8 @@
 //   compute_matrix_elements<MatrixType>                  src/algebra/matrix_elements.h:74-105
 //   FermionBasis::with_fixed_particle_number_spin_momentum src/algebra/basis.cpp:128-185, 304-356
 //   HubbardModelMomentum::hamiltonian                    src/algebra/fermion/model/hubbard_model_momentum.h:119-175
+//   HubbardModelMomentum::current                        src/algebra/fermion/model/hubbard_model_momentum.h:50-93
+//   compute_optical_conductivity                         src/algorithms/optical_conductivity.cpp:8-80
 //
 // The reference key holds at most 7 operators and silently drops longer
 // products (src/algebra/fermion/term.h:11-19, src/algebra/expression_base.h:184-186),
@@ -36,6 +43,7 @@
 #include "algebra/fermion/model/hubbard_model.h"
 #include "algebra/fermion/model/hubbard_model_momentum.h"
 #include "utils/index.h"
+#include "algorithms/optical_conductivity.h"
 #include "algebra/matrix_elements.h"
 
 namespace {
@@ -80,7 +88,7 @@ void print_matrix(const FermionBasis& basis, const Model& model) {
 }
 
 int usage() {
-  std::fprintf(stderr, "usage: ref_driver basis|h1d|h2d|h3d|g1d|g2d|g3d|mbasis|mh ...\n");
+  std::fprintf(stderr, "usage: ref_driver basis|h1d|h2d|h3d|g1d|g2d|g3d|mbasis|mh|mj|sigma ...\n");
   return 2;
 }
 
@@ -120,12 +128,50 @@ int momentum_mode(const std::string& mode, const std::vector<double>& a) {
   return 0;
 }
 
+// mj: ndim L.. Nup Ndn Q.. dir t -- the current operator on the full basis
+int current_mode(const std::vector<double>& a) {
+  if (a.empty()) return usage();
+  const size_t nd = (size_t)a[0];
+  if (nd < 1 || nd > 3 || a.size() != 1 + nd + 2 + nd + 2) return usage();
+  std::vector<size_t> dims, Q;
+  size_t sites = 1;
+  for (size_t d = 0; d < nd; ++d) {
+    dims.push_back((size_t)a[1 + d]);
+    sites *= dims.back();
+  }
+  const size_t nup = (size_t)a[1 + nd], ndn = (size_t)a[2 + nd];
+  for (size_t d = 0; d < nd; ++d) Q.push_back((size_t)a[3 + nd + d]);
+  const size_t dir = (size_t)a[3 + 2 * nd];
+  const double t = a[4 + 2 * nd];
+  // two operators per term: exact up to 5 particles
+  if (nup + ndn > 5) { std::fprintf(stderr, "sector exceeds the exact range\n"); return 3; }
+  auto basis = FermionBasis::with_fixed_particle_number_and_spin(sites, nup + ndn, (int)nup - (int)ndn);
+  print_states(basis);
+  print_operator(basis, HubbardModelMomentum(t, 0.0, dims).current(Q, dir));
+  return 0;
+}
+
+// sigma: dt beta volume, correlation values "re im" on stdin
+int sigma_mode(const std::vector<double>& a) {
+  if (a.size() != 3) return usage();
+  std::vector<std::complex<double>> c;
+  double re, im;
+  while (std::scanf("%lf %lf", &re, &im) == 2) c.emplace_back(re, im);
+  const OpticalConductivityResult r = compute_optical_conductivity(c, a[0], a[1], a[2]);
+  for (size_t i = 0; i < r.frequencies.size(); ++i)
+    std::printf("W %.17g %.17g %.17g\n", r.frequencies[i], r.sigma[i].real(), r.sigma[i].imag());
+  std::printf("R %.17g\n", r.sum_rule);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) return usage();
   const std::string mode = argv[1];
   std::vector<double> a;
   for (int i = 2; i < argc; ++i) a.push_back(std::atof(argv[i]));
   if (mode == "mbasis" || mode == "mh") return momentum_mode(mode, a);
+  if (mode == "mj") return current_mode(a);
+  if (mode == "sigma") return sigma_mode(a);
 
   size_t sites = 0, nup = 0, ndn = 0;
   double t = 0, u = 0;

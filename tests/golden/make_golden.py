@@ -15,7 +15,9 @@ tests/golden/ref_correlation.npz (diagonal of the opposite-spin correlation
 operator G_updn(r) of hubbard_model.h, every displacement r, N_up + N_dn <= 3),
 tests/golden/ref_momentum.npz (fixed-momentum basis state lists for every K of
 ten lattices, and the momentum-space Hamiltonian HubbardModelMomentum on eleven
-sectors inside its exact range),
+sectors inside its exact range), tests/golden/ref_conductivity.npz (the
+current operator J_d(Q) on seven lattices and compute_optical_conductivity on
+four seeded correlators),
 tests/golden/manifest.json.
 """
 from __future__ import annotations
@@ -170,6 +172,39 @@ def main():
                              "n_dn": nd, "K": K, "t": t, "U": U, "dim": int(len(st)),
                              "nnz": int(len(el))})
 
+    # current operator J_d(Q) of HubbardModelMomentum on the full basis (two
+    # operators per term: exact up to 5 particles), and compute_optical_conductivity
+    # on seeded correlators
+    cond_arrays, cond_manifest = {}, []
+    j_cases = [([4], 1, 1, [1], 0, 1.0), ([5], 2, 1, [2], 0, 0.7), ([6], 2, 2, [1], 0, 1.0),
+               ([3, 2], 2, 1, [1, 1], 0, 1.0), ([3, 2], 1, 2, [0, 1], 1, 1.3),
+               ([2, 2, 2], 2, 1, [1, 0, 1], 2, 1.0), ([4], 2, 2, [0], 0, 1.0)]
+    for dims, nu, nd, Q, dr, t in j_cases:
+        st, el = run(["mj", len(dims)] + dims + [nu, nd] + Q + [dr, t])
+        name = f"mj_{'x'.join(map(str, dims))}_{nu}_{nd}_Q{'-'.join(map(str, Q))}_d{dr}_t{t}"
+        el = np.array(el, np.float64).reshape(-1, 4)
+        cond_arrays[name + "__states"] = st
+        cond_arrays[name + "__ij"] = el[:, :2].astype(np.int64)
+        cond_arrays[name + "__val"] = el[:, 2] + 1j * el[:, 3]
+        cond_manifest.append({"name": name, "kind": "current", "dims": dims, "n_up": nu,
+                              "n_dn": nd, "Q": Q, "direction": dr, "t": t, "dim": int(len(st))})
+    rng = np.random.default_rng(20260101)
+    for n, dt, beta, vol in [(2, 0.5, 1.0, 1.0), (40, 0.1, 2.0, 16.0), (200, 0.05, 10.0, 8.0),
+                             (101, 0.2, 0.0, 12.0)]:
+        c = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        txt = "".join(f"{v.real:.17g} {v.imag:.17g}\n" for v in c)
+        out = subprocess.run([DRIVER, "sigma", str(dt), str(beta), str(vol)], input=txt,
+                             check=True, capture_output=True, text=True).stdout
+        W = np.array([[float(x) for x in l.split()[1:]] for l in out.splitlines() if l[0] == "W"])
+        R = [float(l.split()[1]) for l in out.splitlines() if l[0] == "R"][0]
+        name = f"sigma_n{n}_dt{dt}_b{beta}_v{vol}"
+        cond_arrays[name + "__corr"] = c
+        cond_arrays[name + "__omega"] = W[:, 0]
+        cond_arrays[name + "__sigma"] = W[:, 1] + 1j * W[:, 2]
+        cond_manifest.append({"name": name, "kind": "sigma", "n": n, "dt": dt, "beta": beta,
+                              "volume": vol, "sum_rule": R})
+    np.savez_compressed(os.path.join(HERE, "ref_conductivity.npz"), **cond_arrays)
+
     np.savez_compressed(os.path.join(HERE, "ref_momentum.npz"), **mom_arrays)
     np.savez_compressed(os.path.join(HERE, "ref_basis.npz"), **basis_arrays)
     np.savez_compressed(os.path.join(HERE, "ref_correlation.npz"), **corr_arrays)
@@ -178,7 +213,8 @@ def main():
         json.dump({"generator": "tests/golden/make_golden.py",
                    "source": "reference algebra core compiled by oracle/Makefile (target ref)",
                    "basis": basis_manifest, "hamiltonian": ham_manifest,
-                   "correlation": corr_manifest, "momentum": mom_manifest}, f, indent=1)
+                   "correlation": corr_manifest, "momentum": mom_manifest,
+                   "conductivity": cond_manifest}, f, indent=1)
     print(f"{len(basis_manifest)} basis sectors, {len(ham_manifest)} Hamiltonians, "
           f"{len(corr_manifest)} correlation operators")
 
