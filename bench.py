@@ -42,6 +42,7 @@ def main():
                     help="also run the oracle's 300-step OpenMP Lanczos on the host (minutes)")
     ap.add_argument("--no-kpm", action="store_true")
     ap.add_argument("--no-momentum", action="store_true")
+    ap.add_argument("--no-conductivity", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-n", type=int, default=200)
     ap.add_argument("--serving-rate", type=float, default=40.0)
@@ -124,6 +125,7 @@ def main():
     observables = None if args.no_lanczos else bench_observables(op, x)
     del y
     momentum = None if args.no_momentum else bench_momentum()
+    conductivity = None if args.no_conductivity else bench_conductivity()
     batched = None if args.no_batched else bench_batched(args.batched_n)
     kpm = None if args.no_kpm else bench_kpm_c3()
     krylov = None if args.no_kpm else bench_krylov_fp32()
@@ -151,6 +153,7 @@ def main():
         "lanczos_4x4": lanczos,
         "observables_4x4": observables,
         "momentum_4x4_K00": momentum,
+        "conductivity_4x4": conductivity,
         "batched": batched,
         "kpm_l14": kpm,
         "krylov_l14": krylov,
@@ -318,6 +321,48 @@ def bench_momentum(steps=20, k=300):
             "D_K": D, "build_s": round(t_build, 2), "hv_ms": round(ms, 4),
             "hv_algorithmic_GBps": round(32.0 * D / ms / 1e6, 1),
             "lanczos_steps": dec.steps_taken, "lanczos_s": round(t_l, 3), "E0_K00": float(w[0])}
+
+
+def bench_conductivity(steps=10, samples=1, beta=4.0, dt=0.1, krylov=30, reps=20):
+    """Optical conductivity of the 4x4 7+7 model at U = 8: sectors K = (0,0) and
+    K + Q with Q = (1,0), the x current J_x(Q) (one transfer kernel, 16 D_K read +
+    16 D_KQ written algorithmically), and the stochastic correlator: one sample,
+    exp(-beta H_K / 2) then `steps` time points of two 30-step Krylov evolutions."""
+    import torch
+
+    import manybody_amd as mb
+    t, U = 1.0, 8.0
+    t0 = time.perf_counter()
+    bk = mb.FermionBasis.with_fixed_particle_number_spin_momentum(16, 14, 0, (4, 4), (0, 0))
+    bq = mb.FermionBasis.with_fixed_particle_number_spin_momentum(16, 14, 0, (4, 4), (1, 0))
+    hk = mb.HubbardMomentumFactorizedFixedK(bk, (4, 4), t, U, (0, 0))
+    hq = mb.HubbardMomentumFactorizedFixedK(bq, (4, 4), t, U, (1, 0))
+    J = mb.CurrentOperatorFixedK(bk, bq, (4, 4), t, 1, 0)
+    t_build = time.perf_counter() - t0
+    DK, DQ = hk.dimension(), hq.dimension()
+    x = hk.gaussian(seed=5)
+    y = J.apply(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        J.apply(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del x, y
+    torch.cuda.synchronize()
+    c0 = time.perf_counter()
+    C = mb.optical_correlator(hk, hq, J, samples, beta, dt, steps, seed=20260101,
+                              opts=mb.EvolutionOptions(krylov))
+    t_c = time.perf_counter() - c0
+    r = mb.compute_optical_conductivity(C, dt, beta, 16.0)
+    return {"workload": "4x4 7+7 U=8, J_x(Q=(1,0)) from K=(0,0), beta=4, dt=0.1",
+            "D_K": DK, "D_KQ": DQ, "build_s": round(t_build, 2), "current_ms": round(ms, 4),
+            "current_algorithmic_GBps": round(16.0 * (DK + DQ) / ms / 1e6, 1),
+            "correlator_samples": samples, "correlator_steps": steps, "krylov_steps": krylov,
+            "correlator_s": round(t_c, 3), "C0": [float(C[0].real), float(C[0].imag)],
+            "sum_rule": r.sum_rule}
 
 
 def bench_kpm_c3(M=1024, R=16, seed=7):

@@ -238,6 +238,39 @@ int mb_momentum_basis_create(int orbitals, int particles, int spin_projection, i
 int mb_momentum_hubbard_create(const mb_basis *b, int ndim, const int *sizes, double t, double u,
                                const int *momentum, mb_hubbard **out);
 
+/* ------------------------------------------------------------ optical conductivity */
+
+typedef struct mb_current mb_current;
+/* CurrentOperatorFixedK(source_basis, target_basis, lattice_size, t, Q, direction)
+ * (current_operator_fixed_k.h:28-69): J_d(Q) = sum_{k,s} v_d(k) c+_{k+Q,s} c_{k,s},
+ * v_d(k) = 2 t (2 pi / L_d) sin(2 pi k_d / L_d), from the fixed-momentum basis K
+ * (source) to K + Q (target); Q is the flat momentum index.  MB_ERR_INVALID if
+ * the bases are not fixed-momentum bases of that lattice or target != K + Q. */
+int mb_current_create(const mb_basis *source, const mb_basis *target, int ndim, const int *sizes,
+                      double t, int64_t Q, int direction, mb_current **out);
+void mb_current_destroy(mb_current *j);
+/* y = J x (source -> target) and x = J^dagger y (adjoint_apply, :96-117), device order */
+int mb_current_apply(const mb_current *j, const void *x_src, void *y_tgt, mb_stream stream);
+int mb_current_adjoint_apply(const mb_current *j, const void *y_tgt, void *x_src, mb_stream stream);
+/* imaginary_time_evolve_state (evolve_state.h:184-197): out = exp(-tau H) psi */
+int mb_imaginary_time_evolve(const mb_hubbard *h, const void *psi_dev, double tau, int krylov_steps,
+                             void *out_dev, mb_stream stream);
+/* The stochastic current-current correlator of
+ * hubbard_momentum_optical_conductivity_factorized.cpp:226-283: per sample n a
+ * seeded N(0,1) vector (seed + n, per reference index) in sector K, normalised,
+ * v_b = exp(-beta H_K / 2) v, Z = <v_b|v_b>, then C(i dt) += <phi1|J^dagger phi2>/Z
+ * with phi1 = exp(-i H_K t) v_b, phi2 = exp(-i H_{K+Q} t) J v_b; averaged over the
+ * samples.  corr: 2 * steps doubles (re, im). */
+int mb_optical_correlator(const mb_hubbard *hk, const mb_hubbard *hkq, const mb_current *j,
+                          int samples, double beta, double dt, int steps, int krylov_steps,
+                          uint64_t seed, double *corr, mb_stream stream);
+/* compute_optical_conductivity (optical_conductivity.cpp:8-80) on the host:
+ * corr holds n complex values (re, im); writes *n_freqs (omega and sigma NULL:
+ * size query), omega[n_freqs], sigma[2 n_freqs] (re, im) and *sum_rule. */
+int mb_optical_conductivity(const double *corr, int n, double dt, double beta, double volume,
+                            double *omega, double *sigma, int capacity, int *n_freqs,
+                            double *sum_rule);
+
 /* ------------------------------------------------------------ observables */
 
 /* <x|G_updn(r)|x> / <x|x> for every lattice displacement r, written to g[r]

@@ -97,6 +97,16 @@ _SIGS = {
     "mb_time_evolve_steps_ex": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_int,
                                           C.c_int, _vp, _vp]),
     "mb_opposite_spin_correlation": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mb_current_create": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_double, C.c_int64, C.c_int,
+                                    C.POINTER(_vp)]),
+    "mb_current_destroy": (None, [_vp]),
+    "mb_current_apply": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mb_current_adjoint_apply": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mb_imaginary_time_evolve": (C.c_int, [_vp, _vp, C.c_double, C.c_int, _vp, _vp]),
+    "mb_optical_correlator": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.c_double, C.c_int,
+                                        C.c_int, C.c_uint64, _dp, _vp]),
+    "mb_optical_conductivity": (C.c_int, [_dp, C.c_int, C.c_double, C.c_double, C.c_double, _dp,
+                                          _dp, C.c_int, C.POINTER(C.c_int), _dp]),
     "mb_energy_residual": (C.c_int, [_vp, _vp, C.POINTER(C.c_double), C.POINTER(C.c_double), _vp]),
     "mb_server_create": (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.POINTER(_vp)]),
     "mb_server_destroy": (None, [_vp]),
@@ -477,6 +487,94 @@ def time_evolve_state_steps(op: HubbardOperator, psi, t0: float, t1: float, dt: 
     _check(lib().mb_time_evolve_steps_ex(op._h, _ptr(psi), t0, t1, dt, opts.krylov_steps,
                                          EVOLVE_FP32 if fp32 else 0, _ptr(out), _stream()))
     return out
+
+
+def imaginary_time_evolve_state(op: HubbardOperator, psi, tau: float,
+                                opts: EvolutionOptions = EvolutionOptions()):
+    """exp(-tau H) psi (imaginary_time_evolve_state, evolve_state.h:184-197), device order."""
+    out = op.empty()
+    _check(lib().mb_imaginary_time_evolve(op._h, _ptr(psi), tau, opts.krylov_steps, _ptr(out),
+                                          _stream()))
+    return out
+
+
+class CurrentOperatorFixedK:
+    """CurrentOperatorFixedK(source_basis, target_basis, lattice_size, t, Q, direction)
+    (current_operator_fixed_k.h:28-69): J_d(Q) = sum_{k,s} v_d(k) c+_{k+Q,s} c_{k,s}
+    with v_d(k) = 2 t (2 pi / L_d) sin(2 pi k_d / L_d), from the fixed-momentum
+    basis K to K + Q.  Vectors are device-order planes (2, D) of the two sectors."""
+
+    def __init__(self, source_basis: FermionBasis, target_basis: FermionBasis, lattice_size,
+                 t: float, Q: int, direction: int):
+        sizes = tuple(int(v) for v in lattice_size)
+        h = C.c_void_p()
+        _check(lib().mb_current_create(source_basis._h, target_basis._h, len(sizes),
+                                       (C.c_int * len(sizes))(*sizes), float(t), int(Q),
+                                       int(direction), C.byref(h)))
+        self._h = h
+        self.source_basis, self.target_basis = source_basis, target_basis
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mb_current_destroy(self._h)
+            self._h = None
+
+    def source_dimension(self) -> int:
+        return self.source_basis.size()
+
+    def target_dimension(self) -> int:
+        return self.target_basis.size()
+
+    def apply(self, x):
+        """J x: device order of sector K -> device order of sector K + Q."""
+        import torch
+        y = torch.empty((2, self.target_dimension()), dtype=torch.float64, device="cuda")
+        _check(lib().mb_current_apply(self._h, _ptr(x), _ptr(y), _stream()))
+        return y
+
+    def adjoint_apply(self, y):
+        """J^dagger y: sector K + Q -> sector K."""
+        import torch
+        x = torch.empty((2, self.source_dimension()), dtype=torch.float64, device="cuda")
+        _check(lib().mb_current_adjoint_apply(self._h, _ptr(y), _ptr(x), _stream()))
+        return x
+
+
+def optical_correlator(h_k: HubbardOperator, h_kq: HubbardOperator, J: CurrentOperatorFixedK,
+                       samples: int, beta: float, dt: float, steps: int, seed: int,
+                       opts: EvolutionOptions = EvolutionOptions()):
+    """Stochastic <J^dagger(t) J(0)> of hubbard_momentum_optical_conductivity_
+    factorized.cpp:226-283 at t = i dt, i < steps: sample n starts from
+    h_k.gaussian(seed + n), normalised and evolved by exp(-beta H_K / 2)."""
+    import numpy as np
+    c = np.zeros(2 * steps)
+    _check(lib().mb_optical_correlator(h_k._h, h_kq._h, J._h, samples, beta, dt, steps,
+                                       opts.krylov_steps, seed,
+                                       c.ctypes.data_as(_dp), _stream()))
+    return c[0::2] + 1j * c[1::2]
+
+
+class OpticalConductivityResult:
+    def __init__(self, frequencies, sigma, sum_rule):
+        self.frequencies, self.sigma, self.sum_rule = frequencies, sigma, sum_rule
+
+
+def compute_optical_conductivity(correlation, dt: float, beta: float,
+                                 volume: float) -> OpticalConductivityResult:
+    """compute_optical_conductivity (optical_conductivity.cpp:8-80)."""
+    import numpy as np
+    c = np.ascontiguousarray(correlation, dtype=np.complex128).view(np.float64)
+    n = len(c) // 2
+    nf = C.c_int()
+    _check(lib().mb_optical_conductivity(c.ctypes.data_as(_dp), n, dt, beta, volume, None, None,
+                                         0, C.byref(nf), None))
+    om = np.zeros(nf.value)
+    sg = np.zeros(2 * nf.value)
+    sr = C.c_double()
+    _check(lib().mb_optical_conductivity(c.ctypes.data_as(_dp), n, dt, beta, volume,
+                                         om.ctypes.data_as(_dp), sg.ctypes.data_as(_dp),
+                                         nf.value, C.byref(nf), C.byref(sr)))
+    return OpticalConductivityResult(om, sg[0::2] + 1j * sg[1::2], sr.value)
 
 
 def kpm_dos_moments(op: HubbardOperator, a: float, b: float, M: int, nvec: int, seed: int,

@@ -11,6 +11,7 @@
 #include "krylov.hpp"
 #include "mb_internal.hpp"
 #include "momentum.hpp"
+#include "conductivity.hpp"
 
 struct mb_basis {
   mb::Sector sec;
@@ -28,6 +29,10 @@ struct mb_hubbard {
 };
 struct mb_batch {
   std::unique_ptr<mb::Batch> b;
+};
+struct mb_current {
+  std::unique_ptr<mb::CurrentOperator> j;
+  const mb_basis *source, *target;
 };
 struct mb_server {
   std::unique_ptr<mb::Server> s;
@@ -713,5 +718,90 @@ int mb_momentum_hubbard_create(const mb_basis *b, int ndim, const int *sizes, do
     h->mom = std::make_unique<mb::MomentumHubbard>(m, t, u);
     const_cast<mb::MomentumSector &>(m).ensure_device();
     *out = h.release();
+  });
+}
+
+// ------------------------------------------------------------ optical conductivity
+
+int mb_current_create(const mb_basis *source, const mb_basis *target, int ndim, const int *sizes,
+                      double t, int64_t Q, int direction, mb_current **out) {
+  return guarded([&] {
+    need(source && target && sizes && out, "null argument");
+    *out = nullptr;
+    need(source->ms && target->ms, "the bases must be fixed-momentum bases");
+    const auto &S = *source->ms, &T = *target->ms;
+    need(ndim == S.ndim, "lattice dimensionality does not match the bases");
+    for (int i = 0; i < ndim; ++i) need(sizes[i] == S.dims[i], "lattice size does not match the bases");
+    need(Q >= 0 && Q < S.L, "Q must be a flat momentum index");
+    need(direction >= 0 && direction < ndim, "Direction must be less than lattice dimensionality.");
+    auto j = std::make_unique<mb_current>();
+    j->source = source;
+    j->target = target;
+    j->j = std::make_unique<mb::CurrentOperator>(S, T, t, (int)Q, direction);
+    *out = j.release();
+  });
+}
+
+void mb_current_destroy(mb_current *j) { delete j; }
+
+int mb_current_apply(const mb_current *j, const void *x_src, void *y_tgt, mb_stream stream) {
+  return guarded([&] {
+    need(j && x_src && y_tgt && x_src != y_tgt, "null or aliased argument");
+    j->j->apply((const double *)x_src, (double *)y_tgt, S(stream));
+  });
+}
+
+int mb_current_adjoint_apply(const mb_current *j, const void *y_tgt, void *x_src, mb_stream stream) {
+  return guarded([&] {
+    need(j && x_src && y_tgt && x_src != y_tgt, "null or aliased argument");
+    j->j->adjoint_apply((const double *)y_tgt, (double *)x_src, S(stream));
+  });
+}
+
+int mb_imaginary_time_evolve(const mb_hubbard *h, const void *psi_dev, double tau, int krylov_steps,
+                             void *out_dev, mb_stream stream) {
+  return guarded([&] {
+    need(psi_dev && out_dev, "null argument");
+    need(krylov_steps >= 0, "krylov_steps must be non-negative");
+    mb::imaginary_time_evolve(OP(h), psi_dev, tau, krylov_steps, out_dev, S(stream));
+  });
+}
+
+int mb_optical_correlator(const mb_hubbard *hk, const mb_hubbard *hkq, const mb_current *j,
+                          int samples, double beta, double dt, int steps, int krylov_steps,
+                          uint64_t seed, double *corr, mb_stream stream) {
+  return guarded([&] {
+    need(hk && hkq && j && corr, "null argument");
+    need(hk->mom && hkq->mom, "the Hamiltonians must be momentum-sector operators");
+    need(&hk->mom->sector() == j->source->ms.get() && &hkq->mom->sector() == j->target->ms.get(),
+         "the Hamiltonians must act on the current operator's source and target bases");
+    auto &m = const_cast<mb::MomentumSector &>(hk->mom->sector());
+    m.ensure_device();
+    const auto C = mb::optical_correlator(*hk->mom, *hkq->mom, *j->j, m.d_ref_of_pos.as<int64_t>(),
+                                          samples, beta, dt, steps, krylov_steps, seed, S(stream));
+    for (int i = 0; i < steps; ++i) {
+      corr[2 * i] = C[i].real();
+      corr[2 * i + 1] = C[i].imag();
+    }
+  });
+}
+
+int mb_optical_conductivity(const double *corr, int n, double dt, double beta, double volume,
+                            double *omega, double *sigma, int capacity, int *n_freqs,
+                            double *sum_rule) {
+  return guarded([&] {
+    need(corr && n_freqs, "null argument");
+    std::vector<std::complex<double>> c(n > 0 ? n : 0);
+    for (int i = 0; i < n; ++i) c[i] = {corr[2 * i], corr[2 * i + 1]};
+    const mb::Conductivity r = mb::optical_conductivity(c, dt, beta, volume);
+    *n_freqs = (int)r.omega.size();
+    if (!omega && !sigma) return;  // size query
+    need(omega && sigma && capacity >= *n_freqs, "output arrays too small");
+    for (size_t i = 0; i < r.omega.size(); ++i) {
+      omega[i] = r.omega[i];
+      sigma[2 * i] = r.sigma[i].real();
+      sigma[2 * i + 1] = r.sigma[i].imag();
+    }
+    if (sum_rule) *sum_rule = r.sum_rule;
   });
 }

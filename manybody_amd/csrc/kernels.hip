@@ -38,6 +38,7 @@
 #include "krylov.hpp"
 #include "mb_internal.hpp"
 #include "momentum.hpp"
+#include "conductivity.hpp"
 
 namespace mb {
 
@@ -1046,6 +1047,52 @@ __global__ __launch_bounds__(kBlock) void mom_hv_kernel(MomTables t, const doubl
   }
 }
 
+// One-body momentum transfer sum_s w(s) c+_{s+Q} c_s (both spins), sector S ->
+// T = S + Q (conductivity.cpp).  One workgroup per up string g (momentum p in
+// both sectors): target row g of block p.  Up moves read rows of the source
+// block p - Q at the same column (its column group is the target's); down moves
+// read the source row g of block p, staged in LDS, at the gathered columns.
+__global__ __launch_bounds__(kBlock) void transfer_kernel(TransferTables t, const double *__restrict__ x,
+                                                          double *__restrict__ y) {
+  extern __shared__ double smem[];  // 2 * max_cols: re then im of the source row (down moves)
+  const int g = blockIdx.x, N = t.N, tid = threadIdx.x;
+  const int p = t.up_grp[g];
+  const int ncT = t.nc_t[p];
+  if (ncT == 0) return;  // uniform over the workgroup
+  const int64_t baseT = t.blk_t[p] + (int64_t)t.up_loc[g] * ncT;
+  const int ps = t.msub[p * N + t.Q];
+  const int64_t blk_ps = t.blk_s[ps];
+  const int ncS = t.nc_s[p];
+  const int64_t rowS = t.blk_s[p] + (int64_t)t.up_loc[g] * ncS;
+  for (int c = tid; c < ncS; c += kBlock) {
+    smem[c] = x[rowS + c];
+    smem[ncS + c] = x[t.DS + rowS + c];
+  }
+  __syncthreads();
+  const int cu = t.up_cnt[g];
+  const uint16_t *__restrict__ us = t.up_src + (int64_t)g * t.mu;
+  const double *__restrict__ uw = t.up_w + (int64_t)g * t.mu;
+  const int dgT0 = t.dn_off_t[t.msub[t.KT * N + p]];  // target down group K_T - p
+  for (int c = tid; c < ncT; c += kBlock) {
+    double ar = 0.0, ai = 0.0;
+    for (int i = 0; i < cu; ++i) {
+      const int64_t o = blk_ps + (int64_t)us[i] * ncT + c;
+      ar += uw[i] * x[o];
+      ai += uw[i] * x[t.DS + o];
+    }
+    const int64_t h = dgT0 + c;
+    const int cd = t.dn_cnt[h];
+    for (int l = 0; l < cd; ++l) {
+      const int ld = t.dn_src[h * t.md + l];
+      const double w = t.dn_w[h * t.md + l];
+      ar += w * smem[ld];
+      ai += w * smem[ncS + ld];
+    }
+    y[baseT + c] = ar;
+    y[t.DT + baseT + c] = ai;
+  }
+}
+
 // planar device order <-> interleaved reference order through the sector's map
 template <bool TO_DEVICE>
 __global__ __launch_bounds__(kBlock) void mom_perm_kernel(const int64_t *__restrict__ ref_of_pos,
@@ -1248,6 +1295,13 @@ int64_t launch_mom_hv(const MomTables &t, const double *x, double *y, Epi epi, c
   }
   MB_HIP(hipGetLastError());
   return t.nrows;  // partial slots per plane
+}
+
+void launch_transfer(const TransferTables &t, const double *x_src, double *y_tgt, hipStream_t s) {
+  if (t.rows >= (1ll << 31)) throw std::invalid_argument("sector too large");
+  const size_t lds = (size_t)2 * t.max_cols * sizeof(double);
+  hipLaunchKernelGGL(transfer_kernel, dim3((unsigned)t.rows), dim3(kBlock), lds, s, t, x_src, y_tgt);
+  MB_HIP(hipGetLastError());
 }
 
 void launch_mom_perm(const int64_t *map, int64_t D, const double *src, double *dst, bool to_device,

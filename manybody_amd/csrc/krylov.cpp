@@ -430,9 +430,22 @@ std::vector<std::complex<double>> expm_first_column(const LanczosResult &dec, do
 
 // `cache` (optional) keeps the stored Krylov basis between the steps of
 // time_evolve_steps; without one a basis is allocated for this call.
+// y = exp(-tau T) e1 (expm_tridiagonal_first_column_imaginary, evolve_state.h:64-105)
+std::vector<std::complex<double>> expm_first_column_imag(const LanczosResult &dec, double tau) {
+  const int m = dec.steps;
+  std::vector<double> wv(m), Q((size_t)m * m);
+  tridiag_eig(m, dec.alphas.data(), dec.betas.data(), wv.data(), Q.data());
+  std::vector<std::complex<double>> y(m, 0.0);
+  for (int j = 0; j < m; ++j) {
+    const double f = std::exp(-tau * wv[j]) * Q[0 + (size_t)m * j];
+    for (int i = 0; i < m; ++i) y[i] += Q[i + (size_t)m * j] * f;
+  }
+  return y;
+}
+
 template <typename T>
 void time_evolve_t(const HvOp &op, const void *psi, double t, int krylov_steps, void *out,
-                   hipStream_t s, DevBuf *cache = nullptr) {
+                   hipStream_t s, DevBuf *cache = nullptr, bool imag = false) {
   const int64_t D = op.dim();
   int k = krylov_steps;
   if (k == 0 || k > D) k = (int)D;
@@ -466,7 +479,7 @@ void time_evolve_t(const HvOp &op, const void *psi, double t, int krylov_steps, 
     MB_HIP(hipStreamSynchronize(s));
     return;
   }
-  auto y = expm_first_column(dec, t);
+  auto y = imag ? expm_first_column_imag(dec, t) : expm_first_column(dec, t);
   for (auto &v : y) v *= dec.b_norm;
   if (stored) {
     combine_basis<T>(basis->as<T>(), y, D, out, s);  // psi is no longer read: out may be psi
@@ -478,6 +491,11 @@ void time_evolve_t(const HvOp &op, const void *psi, double t, int krylov_steps, 
   } else {
     pass_two_t<T>(op, psi, dec, y, out, s, &work);
   }
+}
+
+void time_evolve_kind(const HvOp &op, const void *psi, double t, int krylov_steps, void *out,
+                      hipStream_t s, DevBuf *cache, bool imag) {
+  time_evolve_t<double>(op, psi, t, krylov_steps, out, s, cache, imag);
 }
 
 void time_evolve(const HvOp &op, const void *psi, double t, int krylov_steps, void *out,

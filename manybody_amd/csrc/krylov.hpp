@@ -77,6 +77,11 @@ void time_evolve(const HvOp &op, const void *psi, double t, int krylov_steps, vo
                  hipStream_t s, bool fp32 = false);
 void time_evolve_steps(const HvOp &op, const void *psi, double t0, double t1, double dt,
                        int krylov_steps, void *out, hipStream_t s, bool fp32 = false);
+// FP64 exp(-i t H) psi, or exp(-t H) psi with imag (imaginary time), with an
+// optional cache holding the stored Krylov basis between calls
+void time_evolve_kind(const HvOp &op, const void *psi, double t, int krylov_steps, void *out,
+                      hipStream_t s, DevBuf *cache, bool imag);
+std::vector<std::complex<double>> expm_first_column_imag(const LanczosResult &dec, double tau);
 // the same on the real-space Hubbard operator
 inline LanczosResult lanczos_pass_one(const Hubbard &h, const void *b, int k, hipStream_t s,
                                       LanczosWork *work = nullptr) {

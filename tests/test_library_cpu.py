@@ -118,3 +118,33 @@ def test_request_footprint_covers_vectors_and_rejects_invalid():
     assert mb.Request(mb.REQ_GROUND, (40,), 1, 1).footprint() == -1
     assert mb.Request(mb.REQ_GROUND, (8,), 9, 1).footprint() == -1
     assert mb.Request(7, (8,), 2, 2).footprint() == -1
+
+
+def test_optical_conductivity_matches_reference_golden():
+    """compute_optical_conductivity on the host (optical_conductivity.cpp:8-80)
+    against the compiled reference's outputs for four correlators."""
+    import json
+    import os
+    gold_dir = os.path.join(os.path.dirname(__file__), "golden")
+    with open(os.path.join(gold_dir, "manifest.json")) as f:
+        cases = [c for c in json.load(f)["conductivity"] if c["kind"] == "sigma"]
+    gold = np.load(os.path.join(gold_dir, "ref_conductivity.npz"))
+    assert len(cases) == 4
+    for c in cases:
+        r = mb.compute_optical_conductivity(gold[c["name"] + "__corr"], c["dt"], c["beta"],
+                                            c["volume"])
+        w = gold[c["name"] + "__omega"]
+        assert r.frequencies.shape == w.shape
+        assert np.abs(r.frequencies - w).max() <= 1e-12 * w.max()
+        ref = gold[c["name"] + "__sigma"]
+        assert np.abs(r.sigma - ref).max() <= 1e-12 * np.abs(ref).max()
+        assert abs(r.sum_rule - c["sum_rule"]) <= 1e-12 * max(abs(c["sum_rule"]), 1e-300)
+
+
+def test_optical_conductivity_rejects_invalid_arguments():
+    with pytest.raises(ValueError):
+        mb.compute_optical_conductivity(np.ones(1), 0.1, 1.0, 1.0)
+    with pytest.raises(ValueError):
+        mb.compute_optical_conductivity(np.ones(8), 0.0, 1.0, 1.0)
+    with pytest.raises(ValueError):
+        mb.compute_optical_conductivity(np.ones(8), 0.1, 1.0, 0.0)
