@@ -16,6 +16,12 @@
 //   EvolutionOptions, time_evolve_state[_steps]      mbamd::EvolutionOptions, time_evolve_state[_steps]
 //     (src/numerics/evolve_state.h:110-231)
 //   kpm::jackson_kernel, rescaling::from_bounds      mbamd::kpm::jackson_kernel, mbamd::rescaling::from_bounds
+//   CurrentOperatorFixedK (src/numerics/              mbamd::CurrentOperatorFixedK
+//     current_operator_fixed_k.h:28-69)
+//   imaginary_time_evolve_state (evolve_state.h:      mbamd::imaginary_time_evolve_state
+//     184-197)
+//   compute_optical_conductivity (src/algorithms/     mbamd::compute_optical_conductivity
+//     optical_conductivity.h:6-13)
 //
 // Vectors are device buffers (DeviceVector, complex128, device product order);
 // to_device_order / to_reference_order convert from and to the reference's
@@ -332,6 +338,85 @@ inline DeviceVector time_evolve_state_steps(const HubbardOperator &H, const Devi
   check(mb_time_evolve_steps_ex(H.handle(), psi.data(), t0, t1, dt, (int)opts.krylov_steps,
                                 opts.fp32 ? MB_EVOLVE_FP32 : 0, out.data(), s));
   return out;
+}
+
+// exp(-tau H) psi (imaginary_time_evolve_state, evolve_state.h:184-197)
+inline DeviceVector imaginary_time_evolve_state(const HubbardOperator &H, const DeviceVector &psi,
+                                                double tau, EvolutionOptions opts = {},
+                                                hipStream_t s = nullptr) {
+  DeviceVector out(H.dimension());
+  check(mb_imaginary_time_evolve(H.handle(), psi.data(), tau, (int)opts.krylov_steps, out.data(),
+                                 s));
+  return out;
+}
+
+// CurrentOperatorFixedK(source_basis, target_basis, lattice_size, t, Q, direction)
+// (current_operator_fixed_k.h:28-69): sector K -> K + Q, Q a flat momentum index.
+class CurrentOperatorFixedK {
+ public:
+  CurrentOperatorFixedK(const FermionBasis &source_basis, const FermionBasis &target_basis,
+                        const std::vector<int> &lattice_size, double t, size_t Q,
+                        size_t direction)
+      : ns_(source_basis.size()), nt_(target_basis.size()) {
+    if (lattice_size.empty())
+      throw std::invalid_argument("CurrentOperatorFixedK requires at least one dimension.");
+    check(mb_current_create(source_basis.handle(), target_basis.handle(), (int)lattice_size.size(),
+                            lattice_size.data(), t, (int64_t)Q, (int)direction, &j_));
+  }
+  CurrentOperatorFixedK(const CurrentOperatorFixedK &) = delete;
+  ~CurrentOperatorFixedK() { mb_current_destroy(j_); }
+  size_t source_dimension() const { return ns_; }
+  size_t target_dimension() const { return nt_; }
+  DeviceVector apply(const DeviceVector &v, hipStream_t s = nullptr) const {
+    DeviceVector w(nt_);
+    check(mb_current_apply(j_, v.data(), w.data(), s));
+    return w;
+  }
+  DeviceVector adjoint_apply(const DeviceVector &w, hipStream_t s = nullptr) const {
+    DeviceVector v(ns_);
+    check(mb_current_adjoint_apply(j_, w.data(), v.data(), s));
+    return v;
+  }
+  const mb_current *handle() const { return j_; }
+
+ private:
+  mb_current *j_ = nullptr;
+  size_t ns_ = 0, nt_ = 0;
+};
+
+// The stochastic current-current correlator of
+// hubbard_momentum_optical_conductivity_factorized.cpp:226-283 (sample n starts
+// from H_K.gaussian(seed + n)).
+inline std::vector<std::complex<double>> optical_correlator(
+    const HubbardOperator &H_K, const HubbardOperator &H_KQ, const CurrentOperatorFixedK &J,
+    size_t samples, double beta, double dt, size_t steps, uint64_t seed,
+    EvolutionOptions opts = {}, hipStream_t s = nullptr) {
+  std::vector<std::complex<double>> c(steps);
+  check(mb_optical_correlator(H_K.handle(), H_KQ.handle(), J.handle(), (int)samples, beta, dt,
+                              (int)steps, (int)opts.krylov_steps, seed,
+                              reinterpret_cast<double *>(c.data()), s));
+  return c;
+}
+
+// OpticalConductivityResult, compute_optical_conductivity (optical_conductivity.h:6-13)
+struct OpticalConductivityResult {
+  std::vector<double> frequencies{};
+  std::vector<std::complex<double>> sigma{};
+  double sum_rule = 0.0;
+};
+
+inline OpticalConductivityResult compute_optical_conductivity(
+    const std::vector<std::complex<double>> &correlation, double dt, double beta, double volume) {
+  const double *c = reinterpret_cast<const double *>(correlation.data());
+  int nf = 0;
+  check(mb_optical_conductivity(c, (int)correlation.size(), dt, beta, volume, nullptr, nullptr, 0,
+                                &nf, nullptr));
+  OpticalConductivityResult r;
+  r.frequencies.resize(nf);
+  r.sigma.resize(nf);
+  check(mb_optical_conductivity(c, (int)correlation.size(), dt, beta, volume, r.frequencies.data(),
+                                reinterpret_cast<double *>(r.sigma.data()), nf, &nf, &r.sum_rule));
+  return r;
 }
 
 // Serving loop: requests admitted as they arrive and as slots and device

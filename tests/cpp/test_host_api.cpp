@@ -138,7 +138,6 @@ int main() {
     CHECK(std::abs(mu[0] - 1.0) < 1e-6);
     for (double m : mu) CHECK(std::abs(m) <= 1.0 + 1e-6);
   }
-  std::printf(g_fail ? "FAILED (%d)\n" : "ALL PASSED\n", g_fail);
   {  // reorthogonalised Lanczos and FP32 time evolution through the C++ layer
     auto basis = mbamd::FermionBasis::with_fixed_particle_number_and_spin(8, 8, 0);
     auto H = mbamd::HubbardModel(1.0, 0.0, 8).op(basis);
@@ -176,6 +175,34 @@ int main() {
     CHECK(std::abs(res.e0 + 9.656854249492380) < 1e-9);
     CHECK(res.dim == 4900);
   }
+  {
+    CASE("optical_conductivity_pipeline");
+    // ring L = 6, 3 + 2, K = 1 -> K + Q = 2: <y|J x> = <J^dagger y|x>, C(0) real
+    // and positive, and sigma(omega) on pi/dt / (eta/4) + 1 frequencies
+    auto bk = FermionBasis::with_fixed_particle_number_spin_momentum(6, 5, 1, {6}, {1});
+    auto bq = FermionBasis::with_fixed_particle_number_spin_momentum(6, 5, 1, {6}, {2});
+    auto HK = HubbardMomentumFactorizedFixedK(bk, {6}, 1.0, 4.0, {1});
+    auto HQ = HubbardMomentumFactorizedFixedK(bq, {6}, 1.0, 4.0, {2});
+    mbamd::CurrentOperatorFixedK J(bk, bq, {6}, 1.0, 1, 0);
+    CHECK(J.source_dimension() == HK.dimension() && J.target_dimension() == HQ.dimension());
+    auto x = HK.gaussian(1), y = HQ.gaussian(2);
+    auto jx = J.apply(x).to_host(), jty = J.adjoint_apply(y).to_host();
+    auto xh = x.to_host(), yh = y.to_host();
+    std::complex<double> lhs = 0.0, rhs = 0.0;
+    for (size_t i = 0; i < yh.size(); ++i) lhs += std::conj(yh[i]) * jx[i];
+    for (size_t i = 0; i < xh.size(); ++i) rhs += std::conj(jty[i]) * xh[i];
+    CHECK(std::abs(lhs - rhs) < 1e-12 * std::abs(lhs));
+    auto C = mbamd::optical_correlator(HK, HQ, J, 2, 1.0, 0.1, 21, 5);
+    CHECK(C.size() == 21 && C[0].real() > 0.0 && std::abs(C[0].imag()) < 1e-12 * C[0].real());
+    auto r = mbamd::compute_optical_conductivity(C, 0.1, 1.0, 6.0);
+    CHECK(r.frequencies.size() == 63 && r.sigma.size() == 63);  // eta = 4 / 2: floor(10 pi / 0.5) + 1
+    auto z = mbamd::imaginary_time_evolve_state(HK, x, 0.0);
+    auto zh = z.to_host();
+    double dz = 0.0;
+    for (size_t i = 0; i < zh.size(); ++i) dz = std::max(dz, std::abs(zh[i] - xh[i]));
+    CHECK(dz < 1e-12);  // exp(0) = 1
+  }
 
+  std::printf(g_fail ? "FAILED (%d)\n" : "ALL PASSED\n", g_fail);
   return g_fail ? 1 : 0;
 }
