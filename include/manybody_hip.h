@@ -271,6 +271,43 @@ int mb_optical_conductivity(const double *corr, int n, double dt, double beta, d
                             double *omega, double *sigma, int capacity, int *n_freqs,
                             double *sum_rule);
 
+/* ------------------------------------------------------------ generic expressions */
+
+/* compute_matrix_elements(basis, A) (matrix_elements.h:74-105) for any
+ * FermionExpression A, as an operator handle: the expression's terms are
+ * n_terms monomials, coefficient t = coeffs[2t] + i coeffs[2t+1], operators
+ * ops[offsets[t] .. offsets[t+1]) in the reference's OperatorStorage encoding
+ * (operator.h:14-40: bit 15 annihilation, bit 14 spin down, bits 0-13 orbital),
+ * the product o_1 o_2 ... o_n with o_n acting first.  Terms are merged by key and
+ * dropped below 1000 eps as FermionExpression does (expression_base.h:85-120);
+ * outputs outside the basis are dropped.  Works on fixed-N bases (real space)
+ * and fixed-momentum bases (at most 24 orbitals).  The handle works with the
+ * FP64 H.v, order, fill, apply_ref, Lanczos, eigenpair, time-evolution and
+ * energy/residual entry points; MB_ERR_INVALID for an operator on an orbital
+ * >= the basis orbitals. */
+int mb_expression_create(const mb_basis *basis, int64_t n_terms, const double *coeffs,
+                         const int64_t *offsets, const uint16_t *ops, mb_hubbard **out);
+
+/* The sparse matrix of compute_matrix_elements (arma::sp_cx_mat): nonzeros in
+ * the reference's basis index order, column-major with rows ascending (SpMat
+ * iteration order), built on the device from an expression operator. */
+typedef struct mb_sparse mb_sparse;
+int mb_compute_matrix_elements(const mb_hubbard *h, mb_sparse **out, mb_stream stream);
+/* from entries already column-major with rows ascending and unique */
+int mb_sparse_create(uint64_t n_rows, uint64_t n_cols, int64_t nnz, const uint64_t *rows,
+                     const uint64_t *cols, const double *vals, mb_sparse **out);
+void mb_sparse_destroy(mb_sparse *sp);
+int mb_sparse_shape(const mb_sparse *sp, uint64_t *n_rows, uint64_t *n_cols, int64_t *nnz);
+/* rows[nnz], cols[nnz], vals[2 nnz] (re, im) */
+int mb_sparse_entries(const mb_sparse *sp, uint64_t *rows, uint64_t *cols, double *vals);
+/* save / load in the reference's cereal binary format (arma_cereal.h:9-30 through
+ * cereal::BinaryOutputArchive, as save_sparse_matrix / load_sparse_matrix of
+ * examples/hubbard_momentum_optical_conductivity.cpp:185-203 write them):
+ * uint32 class version 0, uint64 n_rows, n_cols, n_nonzero, then per nonzero
+ * uint64 row, uint64 col, float64 re, float64 im, little endian. */
+int mb_sparse_save_cereal(const mb_sparse *sp, const char *path);
+int mb_sparse_load_cereal(const char *path, mb_sparse **out);
+
 /* ------------------------------------------------------------ observables */
 
 /* <x|G_updn(r)|x> / <x|x> for every lattice displacement r, written to g[r]

@@ -97,6 +97,16 @@ _SIGS = {
     "mb_time_evolve_steps_ex": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_int,
                                           C.c_int, _vp, _vp]),
     "mb_opposite_spin_correlation": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mb_expression_create": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "mb_compute_matrix_elements": (C.c_int, [_vp, C.POINTER(_vp), _vp]),
+    "mb_sparse_create": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, _vp, _vp, _vp,
+                                   C.POINTER(_vp)]),
+    "mb_sparse_destroy": (None, [_vp]),
+    "mb_sparse_shape": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                  C.POINTER(C.c_int64)]),
+    "mb_sparse_entries": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mb_sparse_save_cereal": (C.c_int, [_vp, C.c_char_p]),
+    "mb_sparse_load_cereal": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
     "mb_current_create": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_double, C.c_int64, C.c_int,
                                     C.POINTER(_vp)]),
     "mb_current_destroy": (None, [_vp]),
@@ -245,6 +255,11 @@ class _HubbardBase:
     def _bonds_args(self):
         raise NotImplementedError
 
+    def hamiltonian(self):
+        """kinetic() + interaction() as a FermionExpression (hubbard_model.h)."""
+        from .expression import hubbard_hamiltonian
+        return hubbard_hamiltonian(self.t, self.u, self.sizes)
+
     def operator(self, basis: FermionBasis) -> "HubbardOperator":
         sizes = (C.c_int * len(self.sizes))(*self.sizes)
         h = C.c_void_p()
@@ -289,6 +304,32 @@ def HubbardMomentumFactorizedFixedK(basis: FermionBasis, lattice_size, t: float,
     _check(lib().mb_momentum_hubbard_create(basis._h, len(sizes), (C.c_int * len(sizes))(*sizes),
                                             float(t), float(u), (C.c_int * len(K))(*K), C.byref(h)))
     return HubbardOperator(h, basis, None)
+
+
+def FermionExpressionOperator(basis: FermionBasis, expression) -> "HubbardOperator":
+    """compute_matrix_elements(basis, A) (matrix_elements.h:74-105) of any
+    FermionExpression A as a device operator: FP64 H.v, orders, fills, Lanczos,
+    eigenpair, time evolution and energy/residual, on a fixed-N or
+    fixed-momentum basis."""
+    coeffs, offsets, ops = expression.arrays()
+    h = C.c_void_p()
+    _check(lib().mb_expression_create(basis._h, len(offsets) - 1,
+                                      coeffs.ctypes.data_as(C.c_void_p),
+                                      offsets.ctypes.data_as(C.c_void_p),
+                                      ops.ctypes.data_as(C.c_void_p), C.byref(h)))
+    op = HubbardOperator(h, basis, None)
+    op.expression = expression
+    return op
+
+
+def compute_matrix_elements(basis_or_op, expression=None):
+    """The sparse matrix <i|A|j> of compute_matrix_elements over the basis, in
+    the reference's index order (built on the device): a SparseMatrix."""
+    from .expression import SparseMatrix
+    op = basis_or_op if expression is None else FermionExpressionOperator(basis_or_op, expression)
+    h = C.c_void_p()
+    _check(lib().mb_compute_matrix_elements(op._h, C.byref(h), _stream()))
+    return SparseMatrix(h)
 
 
 class HubbardOperator:
@@ -778,3 +819,20 @@ class Server:
 
     def now(self) -> float:
         return float(lib().mb_server_now(self._h))
+
+
+class HubbardModelMomentum:
+    """HubbardModelMomentum(t, u, lattice_size) (hubbard_model_momentum.h): its
+    hamiltonian() expression in momentum space."""
+
+    def __init__(self, t: float, u: float, lattice_size):
+        self.t, self.u = float(t), float(u)
+        self.sizes = tuple(int(v) for v in lattice_size)
+
+    def hamiltonian(self):
+        from .expression import hubbard_momentum_hamiltonian
+        return hubbard_momentum_hamiltonian(self.t, self.u, self.sizes)
+
+
+from .expression import (FermionExpression, SparseMatrix, Spin, annihilation,  # noqa: E402
+                         creation, density, density_density, hopping, one_body, two_body)

@@ -12,6 +12,11 @@
 #include "mb_internal.hpp"
 #include "momentum.hpp"
 #include "conductivity.hpp"
+#include "expression.hpp"
+
+#include <cstdio>
+#include <map>
+#include <fstream>
 
 struct mb_basis {
   mb::Sector sec;
@@ -22,10 +27,33 @@ struct mb_hubbard {
   const mb_basis *basis;
   std::unique_ptr<mb::RealHvOp> real_op;
   std::unique_ptr<mb::MomentumHubbard> mom;
+  std::unique_ptr<mb::ExpressionOperator> expr;  // a compiled FermionExpression
   const mb::HvOp &op() const {
     if (mom) return *mom;
+    if (expr) return *expr;
     return *real_op;
   }
+  // the fixed-momentum sector the operator acts on, or null (real space)
+  const mb::MomentumSector *msec() const {
+    if (mom) return &mom->sector();
+    if (expr && basis->ms) return basis->ms.get();
+    return nullptr;
+  }
+  // the real-space sector the operator acts on, or null (momentum)
+  const mb::Sector *rsec() const {
+    if (mom || (expr && basis->ms)) return nullptr;
+    if (expr) {
+      auto &sec = const_cast<mb::Sector &>(basis->sec);
+      sec.ensure_device();
+      return &sec;
+    }
+    return h.sec;
+  }
+};
+struct mb_sparse {
+  uint64_t n_rows = 0, n_cols = 0;
+  std::vector<uint64_t> rows, cols;
+  std::vector<std::complex<double>> vals;
 };
 struct mb_batch {
   std::unique_ptr<mb::Batch> b;
@@ -85,6 +113,7 @@ hipStream_t S(mb_stream s) { return reinterpret_cast<hipStream_t>(s); }
 const mb::Hubbard &H(const mb_hubbard *h) {
   need(h != nullptr, "null model handle");
   need(!h->mom, "not available for momentum-sector operators");
+  need(!h->expr, "not available for expression operators");
   return h->h;
 }
 
@@ -239,13 +268,13 @@ int64_t mb_hubbard_dimension(const mb_hubbard *h) { return h ? h->op().dim() : -
 int mb_hubbard_shape(const mb_hubbard *h, int64_t *d_up, int64_t *d_dn) {
   return guarded([&] {
     need(h && d_up && d_dn, "null argument");
-    if (h->mom) {  // a momentum sector is not one product block
-      *d_up = h->mom->dim();
+    if (h->msec()) {  // a momentum sector is not one product block
+      *d_up = h->op().dim();
       *d_dn = 1;
       return;
     }
-    *d_up = h->h.sec->d_up;
-    *d_dn = h->h.sec->d_dn;
+    *d_up = h->rsec()->d_up;
+    *d_dn = h->rsec()->d_dn;
   });
 }
 
@@ -276,7 +305,7 @@ int mb_hv_c64_multi(const mb_hubbard *h, int nvec, const void *x_dev, void *y_de
 
 namespace {
 const int64_t *mom_map(const mb_hubbard *h) {
-  auto &m = const_cast<mb::MomentumSector &>(h->mom->sector());
+  auto &m = const_cast<mb::MomentumSector &>(*h->msec());
   m.ensure_device();
   return m.d_ref_of_pos.as<int64_t>();
 }
@@ -286,11 +315,11 @@ int mb_to_device_order_c128(const mb_hubbard *h, const void *x_ref, void *x_dev,
   return guarded([&] {
     need(x_ref && x_dev && x_ref != x_dev, "in and out must be distinct device pointers");
     need(h != nullptr, "null model handle");
-    if (h->mom)
-      mb::launch_mom_perm(mom_map(h), h->mom->dim(), (const double *)x_ref, (double *)x_dev, true,
+    if (h->msec())
+      mb::launch_mom_perm(mom_map(h), h->op().dim(), (const double *)x_ref, (double *)x_dev, true,
                           S(stream));
     else
-      mb::launch_gather_c128(mb::perm_of(*H(h).sec), x_ref, x_dev, S(stream));
+      mb::launch_gather_c128(mb::perm_of(*h->rsec()), x_ref, x_dev, S(stream));
   });
 }
 
@@ -299,11 +328,11 @@ int mb_to_reference_order_c128(const mb_hubbard *h, const void *x_dev, void *x_r
   return guarded([&] {
     need(x_ref && x_dev && x_ref != x_dev, "in and out must be distinct device pointers");
     need(h != nullptr, "null model handle");
-    if (h->mom)
-      mb::launch_mom_perm(mom_map(h), h->mom->dim(), (const double *)x_dev, (double *)x_ref, false,
+    if (h->msec())
+      mb::launch_mom_perm(mom_map(h), h->op().dim(), (const double *)x_dev, (double *)x_ref, false,
                           S(stream));
     else
-      mb::launch_scatter_c128(mb::perm_of(*H(h).sec), x_dev, x_ref, S(stream));
+      mb::launch_scatter_c128(mb::perm_of(*h->rsec()), x_dev, x_ref, S(stream));
   });
 }
 
@@ -312,12 +341,20 @@ int mb_apply_ref_c128(const mb_hubbard *h, const void *x_ref, void *y_ref, void 
   return guarded([&] {
     need(x_ref && y_ref && work, "null argument");
     need(h != nullptr, "null model handle");
-    if (h->mom) {
-      const int64_t D = h->mom->dim();
+    if (h->msec()) {
+      const int64_t D = h->op().dim();
       double *w0 = (double *)work, *w1 = w0 + 2 * D;
       mb::launch_mom_perm(mom_map(h), D, (const double *)x_ref, w0, true, S(stream));
-      h->mom->hv(w0, w1, mb::Epi::Plain, mb::EpiArgs{}, S(stream));
+      h->op().hv(w0, w1, mb::Epi::Plain, mb::EpiArgs{}, S(stream));
       mb::launch_mom_perm(mom_map(h), D, w1, (double *)y_ref, false, S(stream));
+      return;
+    }
+    if (h->expr) {
+      const int64_t D = h->op().dim();
+      double *w0 = (double *)work, *w1 = w0 + 2 * D;
+      mb::launch_gather_c128(mb::perm_of(*h->rsec()), x_ref, w0, S(stream));
+      h->op().hv(w0, w1, mb::Epi::Plain, mb::EpiArgs{}, S(stream));
+      mb::launch_scatter_c128(mb::perm_of(*h->rsec()), w1, y_ref, S(stream));
       return;
     }
     const auto &hh = H(h);
@@ -333,10 +370,10 @@ int mb_fill_gaussian_c128(const mb_hubbard *h, uint64_t seed, void *x_dev, mb_st
   return guarded([&] {
     need(x_dev, "null argument");
     need(h != nullptr, "null model handle");
-    if (h->mom)
-      mb::launch_mom_fill_gaussian(mom_map(h), h->mom->dim(), seed, (double *)x_dev, S(stream));
+    if (h->msec())
+      mb::launch_mom_fill_gaussian(mom_map(h), h->op().dim(), seed, (double *)x_dev, S(stream));
     else
-      mb::launch_fill_gaussian_c128(mb::perm_of(*H(h).sec), seed, x_dev, S(stream));
+      mb::launch_fill_gaussian_c128(mb::perm_of(*h->rsec()), seed, x_dev, S(stream));
   });
 }
 
@@ -803,5 +840,152 @@ int mb_optical_conductivity(const double *corr, int n, double dt, double beta, d
       sigma[2 * i + 1] = r.sigma[i].imag();
     }
     if (sum_rule) *sum_rule = r.sum_rule;
+  });
+}
+
+// ------------------------------------------------------------ generic expressions
+
+int mb_expression_create(const mb_basis *b, int64_t n_terms, const double *coeffs,
+                         const int64_t *offsets, const uint16_t *ops, mb_hubbard **out) {
+  return guarded([&] {
+    need(b && out && n_terms >= 0, "null argument");
+    need(n_terms == 0 || (coeffs && offsets), "null term arrays");
+    *out = nullptr;
+    need(n_terms == 0 || offsets[0] == 0, "offsets[0] must be 0");
+    need(n_terms == 0 || offsets[n_terms] == 0 || ops, "null operator array");
+    auto h = std::make_unique<mb_hubbard>();
+    h->basis = b;
+    const int64_t zero = 0;
+    if (b->ms) {
+      auto &m = const_cast<mb::MomentumSector &>(*b->ms);
+      h->expr = std::make_unique<mb::ExpressionOperator>(nullptr, &m, n_terms, coeffs,
+                                                         n_terms ? offsets : &zero, ops);
+    } else {
+      h->expr = std::make_unique<mb::ExpressionOperator>(&b->sec, nullptr, n_terms, coeffs,
+                                                         n_terms ? offsets : &zero, ops);
+    }
+    *out = h.release();
+  });
+}
+
+int mb_compute_matrix_elements(const mb_hubbard *h, mb_sparse **out, mb_stream stream) {
+  return guarded([&] {
+    need(h && out, "null argument");
+    *out = nullptr;
+    need(h->expr != nullptr, "the sparse export needs an expression operator");
+    auto sp = std::make_unique<mb_sparse>();
+    sp->n_rows = sp->n_cols = (uint64_t)h->expr->dim();
+    h->expr->export_coo(sp->rows, sp->cols, sp->vals, S(stream));
+    *out = sp.release();
+  });
+}
+
+int mb_sparse_create(uint64_t n_rows, uint64_t n_cols, int64_t nnz, const uint64_t *rows,
+                     const uint64_t *cols, const double *vals, mb_sparse **out) {
+  return guarded([&] {
+    need(out && nnz >= 0 && (nnz == 0 || (rows && cols && vals)), "null argument");
+    *out = nullptr;
+    auto sp = std::make_unique<mb_sparse>();
+    sp->n_rows = n_rows;
+    sp->n_cols = n_cols;
+    for (int64_t i = 0; i < nnz; ++i) {
+      need(rows[i] < n_rows && cols[i] < n_cols, "entry outside the matrix");
+      if (i > 0)
+        need(cols[i] > cols[i - 1] || (cols[i] == cols[i - 1] && rows[i] > rows[i - 1]),
+             "entries must be column-major with rows ascending and unique");
+      sp->rows.push_back(rows[i]);
+      sp->cols.push_back(cols[i]);
+      sp->vals.emplace_back(vals[2 * i], vals[2 * i + 1]);
+    }
+    *out = sp.release();
+  });
+}
+
+void mb_sparse_destroy(mb_sparse *sp) { delete sp; }
+
+int mb_sparse_shape(const mb_sparse *sp, uint64_t *n_rows, uint64_t *n_cols, int64_t *nnz) {
+  return guarded([&] {
+    need(sp && n_rows && n_cols && nnz, "null argument");
+    *n_rows = sp->n_rows;
+    *n_cols = sp->n_cols;
+    *nnz = (int64_t)sp->vals.size();
+  });
+}
+
+int mb_sparse_entries(const mb_sparse *sp, uint64_t *rows, uint64_t *cols, double *vals) {
+  return guarded([&] {
+    need(sp && (sp->vals.empty() || (rows && cols && vals)), "null argument");
+    for (size_t i = 0; i < sp->vals.size(); ++i) {
+      rows[i] = sp->rows[i];
+      cols[i] = sp->cols[i];
+      vals[2 * i] = sp->vals[i].real();
+      vals[2 * i + 1] = sp->vals[i].imag();
+    }
+  });
+}
+
+namespace {
+// little-endian fixed-width fields of cereal's BinaryOutputArchive
+template <typename T>
+void put(std::string &b, T v) {
+  b.append(reinterpret_cast<const char *>(&v), sizeof(T));
+}
+template <typename T>
+T get(std::ifstream &f) {
+  T v;
+  f.read(reinterpret_cast<char *>(&v), sizeof(T));
+  if (!f) throw std::runtime_error("truncated sparse-matrix archive");
+  return v;
+}
+}  // namespace
+
+int mb_sparse_save_cereal(const mb_sparse *sp, const char *path) {
+  return guarded([&] {
+    need(sp && path, "null argument");
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error(std::string("Failed to open file for writing: ") + path);
+    std::string b;
+    b.reserve(28 + 32 * sp->vals.size());
+    put<uint32_t>(b, 0);  // cereal class version of a versioned save
+    put<uint64_t>(b, sp->n_rows);
+    put<uint64_t>(b, sp->n_cols);
+    put<uint64_t>(b, (uint64_t)sp->vals.size());
+    for (size_t i = 0; i < sp->vals.size(); ++i) {
+      put<uint64_t>(b, sp->rows[i]);
+      put<uint64_t>(b, sp->cols[i]);
+      put<double>(b, sp->vals[i].real());
+      put<double>(b, sp->vals[i].imag());
+    }
+    f.write(b.data(), (std::streamsize)b.size());
+    if (!f) throw std::runtime_error(std::string("write failed: ") + path);
+  });
+}
+
+int mb_sparse_load_cereal(const char *path, mb_sparse **out) {
+  return guarded([&] {
+    need(path && out, "null argument");
+    *out = nullptr;
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error(std::string("Failed to open file for reading: ") + path);
+    auto sp = std::make_unique<mb_sparse>();
+    (void)get<uint32_t>(f);  // class version
+    sp->n_rows = get<uint64_t>(f);
+    sp->n_cols = get<uint64_t>(f);
+    const uint64_t nz = get<uint64_t>(f);
+    // arma_cereal.h load(): t(row, col) = v per triplet, later ones overwrite
+    std::map<std::pair<uint64_t, uint64_t>, std::complex<double>> m;
+    for (uint64_t i = 0; i < nz; ++i) {
+      const uint64_t r = get<uint64_t>(f), c = get<uint64_t>(f);
+      const double re = get<double>(f), im = get<double>(f);
+      if (r >= sp->n_rows || c >= sp->n_cols) throw std::out_of_range("entry outside the matrix");
+      m[{c, r}] = {re, im};
+    }
+    for (const auto &[k, v] : m) {
+      if (v == 0.0) continue;  // SpMat does not store explicit zeros
+      sp->cols.push_back(k.first);
+      sp->rows.push_back(k.second);
+      sp->vals.push_back(v);
+    }
+    *out = sp.release();
   });
 }

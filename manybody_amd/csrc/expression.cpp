@@ -1,0 +1,413 @@
+// Generic fermionic expressions (FermionExpression, fermion/expression.h) compiled
+// to bit operations on the basis words, for H.v on the device.
+//
+// A basis state is the word s = up | dn << L: mode m = spin L + orbital, the
+// state c+_{m1} c+_{m2} ... |0> with m1 < m2 < ... (the reference's key order:
+// up creators ascending, then down, basis.cpp:273-276).  c_m or c+_m on s gives
+// (-1)^popcount(s & (2^m - 1)) and flips bit m, or zero.  Applied operator by
+// operator, a monomial's action on any s depends only on the occupancy of the
+// modes it touches, which it requires (in) and leaves (out); its sign is a
+// constant times the parity of the occupied untouched modes below each of its
+// operators, i.e. (-1)^popcount(s & parity) with parity the XOR of those masks.
+// This is the same algebra as the reference's canonicalisation of A |s>
+// (canonicalize.h:18-94, matrix_elements.h:74-105) with the output states
+// outside the basis dropped.
+//
+// Compiled terms are split by the species they touch:
+//   up only    H_up (x) 1: per up string, its source up strings and amplitudes
+//              (merged); diagonal ones summed into a per-row value;
+//   down only  1 (x) H_dn: per down string likewise;
+//   mixed      grouped by their up part: per row the up parts whose condition
+//              the row's up string meets (source row and sign), and per up part
+//              its down parts, tested per column on the device.
+// Terms that change N_up or N_dn (or, in a momentum sector, the momentum of a
+// species part) map the basis outside itself and are dropped, as the reference
+// drops output states not in basis.set.
+#include "expression.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <stdexcept>
+#include <tuple>
+
+namespace mb {
+
+namespace {
+
+constexpr double kTol = 1000.0 * 2.220446049250313e-16;  // tolerances::tolerance<double>()
+
+bool tiny(std::complex<double> c) { return std::norm(c) < kTol * kTol; }
+
+int parity64(uint64_t v) { return __builtin_popcountll(v) & 1; }
+
+// ExpressionBase::add_to (expression_base.h:85-97): drop below 1000 eps, merge by
+// key, erase a merged sum below 1000 eps
+template <typename K>
+void add_to(std::map<K, std::complex<double>> &m, const K &k, std::complex<double> c) {
+  if (tiny(c)) return;
+  auto it = m.find(k);
+  if (it == m.end()) {
+    m.emplace(k, c);
+  } else {
+    it->second += c;
+    if (tiny(it->second)) m.erase(it);
+  }
+}
+
+}  // namespace
+
+bool compile_monomial(const uint16_t *ops, int n, int L, std::complex<double> c,
+                      MonomialAction &a) {
+  uint64_t touched = 0;
+  for (int k = 0; k < n; ++k) {
+    const int orb = ops[k] & kOpOrbital;
+    if (orb >= L) throw std::invalid_argument("operator orbital outside the basis orbitals");
+    touched |= uint64_t{1} << ((ops[k] & kOpDown ? L : 0) + orb);
+  }
+  // the first operator to act on a mode (the rightmost) fixes its required
+  // occupancy: occupied for c, empty for c+
+  uint64_t known = 0, in = 0;
+  for (int k = n - 1; k >= 0; --k) {
+    const uint64_t bit = uint64_t{1} << ((ops[k] & kOpDown ? L : 0) + (ops[k] & kOpOrbital));
+    if (!(known & bit)) {
+      known |= bit;
+      if (ops[k] & kOpAnnihilation) in |= bit;
+    }
+  }
+  uint64_t cur = in, par = 0;  // occupancy of the touched modes as the operators act
+  bool neg = false;
+  for (int k = n - 1; k >= 0; --k) {  // rightmost operator acts first
+    const int m = (ops[k] & kOpDown ? L : 0) + (ops[k] & kOpOrbital);
+    const uint64_t bit = uint64_t{1} << m, low = bit - 1;
+    const bool ann = ops[k] & kOpAnnihilation;
+    const bool occ = cur & bit;
+    if (ann != occ) return false;  // c on an empty mode or c+ on an occupied one
+    if (parity64(cur & low)) neg = !neg;  // touched modes below m: known occupancy
+    par ^= low & ~touched;                // untouched modes below m: from the state
+    cur ^= bit;
+  }
+  a.touched = touched;
+  a.in = in;
+  a.out = cur;
+  a.parity = par;
+  a.c = neg ? -c : c;
+  return true;
+}
+
+template <typename V>
+const typename V::value_type *ExpressionOperator::keep(const V &v) {
+  bufs_.push_back(upload(v));
+  return bufs_.back().template as<typename V::value_type>();
+}
+
+ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector *mom,
+                                       int64_t n_terms, const double *coeffs,
+                                       const int64_t *offsets, const uint16_t *ops)
+    : real_(real), mom_(mom) {
+  if ((real == nullptr) == (mom == nullptr)) throw std::invalid_argument("one basis required");
+  if (n_terms < 0) throw std::invalid_argument("negative term count");
+  const int L = real ? real->L : mom->L;
+  const int n_up = real ? real->n_up : mom->n_up, n_dn = real ? real->n_dn : mom->n_dn;
+  const uint64_t U = (uint64_t{1} << L) - 1;
+
+  // the expression's own terms, merged by operator key (FermionExpression's map)
+  std::map<std::vector<uint16_t>, std::complex<double>> expr;
+  for (int64_t t = 0; t < n_terms; ++t) {
+    if (offsets[t + 1] < offsets[t]) throw std::invalid_argument("term offsets must not decrease");
+    std::vector<uint16_t> key(ops + offsets[t], ops + offsets[t + 1]);
+    add_to(expr, key, {coeffs[2 * t], coeffs[2 * t + 1]});
+  }
+
+  // compiled actions by class
+  std::vector<MonomialAction> up_diag, up_off, dn_diag, dn_off, mixed;
+  for (const auto &[key, c] : expr) {
+    MonomialAction a;
+    if (!compile_monomial(key.data(), (int)key.size(), L, c, a)) continue;
+    const uint64_t D_ = ~U;
+    if (__builtin_popcountll(a.in & U) != __builtin_popcountll(a.out & U) ||
+        __builtin_popcountll(a.in & D_) != __builtin_popcountll(a.out & D_))
+      continue;  // changes N_up or N_dn: no output in the basis
+    ++kept_;
+    const bool diag = a.in == a.out;
+    if ((a.touched & ~U) == 0)
+      (diag ? up_diag : up_off).push_back(a);
+    else if ((a.touched & U) == 0)
+      (diag ? dn_diag : dn_off).push_back(a);
+    else
+      mixed.push_back(a);
+  }
+
+  // rows (up strings) and columns (down strings) of the layout
+  std::vector<uint32_t> rows_bits, dn_bits;
+  std::vector<int64_t> row_base;
+  std::vector<int32_t> row_nc, row_dg0;
+  int64_t D = 0;
+  if (real) {
+    rows_bits = real->s_up;
+    dn_bits = real->s_dn;
+    D = real->dim();
+    for (int64_t g = 0; g < real->d_up; ++g) {
+      row_base.push_back(g * real->d_dn);
+      row_nc.push_back((int32_t)real->d_dn);
+      row_dg0.push_back(0);
+    }
+  } else {
+    rows_bits = mom->up_str;
+    dn_bits = mom->dn_str;
+    D = mom->D;
+    for (int p = 0; p < mom->L; ++p)
+      for (int g = mom->up_off[p]; g < mom->up_off[p + 1]; ++g) {
+        const int nc = mom->ncols[p];
+        row_base.push_back(mom->blk_off[p] + (int64_t)(g - mom->up_off[p]) * nc);
+        row_nc.push_back(nc);
+        row_dg0.push_back(mom->dn_off[mom->msub(mom->K, p)]);
+      }
+  }
+  const int64_t nrows = (int64_t)rows_bits.size(), ndn = (int64_t)dn_bits.size();
+
+  // string -> (row base | column) lookups of the source side
+  std::vector<int32_t> up_loc, dn_loc, up_row;
+  std::vector<uint8_t> up_mom, dn_mom;
+  if (mom) {
+    if (L > 24) throw std::invalid_argument("momentum-sector expressions support at most 24 orbitals");
+    const size_t n = size_t{1} << L;
+    dn_loc.assign(n, -1);
+    dn_mom.assign(n, 0);
+    up_row.assign(n, -1);
+    for (int p = 0; p < L; ++p) {
+      for (int h = mom->dn_off[p]; h < mom->dn_off[p + 1]; ++h) {
+        dn_loc[mom->dn_str[h]] = h - mom->dn_off[p];
+        dn_mom[mom->dn_str[h]] = (uint8_t)p;
+      }
+    }
+    for (int64_t g = 0; g < nrows; ++g) up_row[rows_bits[g]] = (int32_t)g;
+  }
+  auto source_row = [&](uint32_t u2, int32_t *grp) -> int64_t {  // row index or -1
+    if (real) {
+      *grp = -1;
+      return Sector::colex_rank(u2);
+    }
+    const int32_t g2 = up_row[u2];
+    if (g2 < 0) return -1;
+    *grp = mom->msub(mom->K, mom->momentum_of(u2));  // momentum of the columns
+    return row_nc[g2] > 0 ? g2 : -1;
+  };
+  auto source_col = [&](uint32_t d2, int32_t grp) -> int64_t {
+    if (real) return Sector::colex_rank(d2);
+    return mom->momentum_of(d2) == grp ? dn_loc[d2] : -1;
+  };
+
+  // diagonal parts of one species
+  std::vector<double> diag_up(2 * nrows, 0.0), diag_dn(2 * ndn, 0.0);
+  for (int64_t g = 0; g < nrows; ++g) {
+    std::complex<double> v = 0.0;
+    for (const auto &a : up_diag)
+      if (((uint64_t)rows_bits[g] & a.touched) == a.in) v += a.c;
+    diag_up[2 * g] = v.real();
+    diag_up[2 * g + 1] = v.imag();
+  }
+  for (int64_t h = 0; h < ndn; ++h) {
+    std::complex<double> v = 0.0;
+    const uint64_t w = (uint64_t)dn_bits[h] << L;
+    for (const auto &a : dn_diag)
+      if ((w & a.touched) == a.in) v += a.c;
+    diag_dn[2 * h] = v.real();
+    diag_dn[2 * h + 1] = v.imag();
+  }
+
+  // up-only off-diagonal: per row, source row base -> amplitude (gather form:
+  // the output row must hold `out`, its source is row ^ (in ^ out))
+  std::vector<int64_t> uo_ptr{0}, uo_base;
+  std::vector<double> uo_amp;
+  for (int64_t g = 0; g < nrows; ++g) {
+    std::map<int64_t, std::complex<double>> acc;
+    const uint64_t u = rows_bits[g];
+    for (const auto &a : up_off) {
+      if ((u & a.touched) != a.out) continue;
+      const uint32_t u2 = (uint32_t)(u ^ (a.in ^ a.out));
+      int32_t grp;
+      const int64_t r2 = source_row(u2, &grp);
+      if (r2 < 0) continue;
+      if (mom && grp != mom->msub(mom->K, mom->momentum_of((uint32_t)u))) continue;
+      acc[row_base[r2]] += parity64(u & a.parity) ? -a.c : a.c;
+    }
+    for (const auto &[b, c] : acc) {
+      if (c == 0.0) continue;
+      uo_base.push_back(b);
+      uo_amp.push_back(c.real());
+      uo_amp.push_back(c.imag());
+    }
+    uo_ptr.push_back((int64_t)uo_base.size());
+  }
+  // down-only off-diagonal: per down string, source column -> amplitude
+  std::vector<int64_t> do_ptr{0};
+  std::vector<int32_t> do_col;
+  std::vector<double> do_amp;
+  for (int64_t h = 0; h < ndn; ++h) {
+    std::map<int64_t, std::complex<double>> acc;
+    const uint64_t w = (uint64_t)dn_bits[h] << L;
+    const int32_t grp = mom ? mom->momentum_of(dn_bits[h]) : -1;
+    for (const auto &a : dn_off) {
+      if ((w & a.touched) != a.out) continue;
+      const uint32_t d2 = (uint32_t)((w ^ (a.in ^ a.out)) >> L);
+      const int64_t c2 = source_col(d2, grp);
+      if (c2 < 0) continue;
+      acc[c2] += parity64(w & a.parity) ? -a.c : a.c;
+    }
+    for (const auto &[col, c] : acc) {
+      if (c == 0.0) continue;
+      do_col.push_back((int32_t)col);
+      do_amp.push_back(c.real());
+      do_amp.push_back(c.imag());
+    }
+    do_ptr.push_back((int64_t)do_col.size());
+  }
+
+  // mixed terms grouped by their up part; per group its down parts (merged)
+  using UpKey = std::tuple<uint64_t, uint64_t, uint64_t, uint64_t>;  // touched, in, out, parity
+  using DnKey = std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>;
+  std::map<UpKey, std::map<DnKey, std::complex<double>>> groups;
+  for (const auto &a : mixed) {
+    const UpKey uk{a.touched & U, a.in & U, a.out & U, a.parity & U};
+    const DnKey dk{(uint32_t)(a.touched >> L), (uint32_t)(a.in >> L), (uint32_t)(a.out >> L),
+                   (uint32_t)(a.parity >> L)};
+    groups[uk][dk] += a.c;
+  }
+  std::vector<UpKey> gkeys;
+  std::vector<int32_t> dp_ptr{0};
+  std::vector<uint32_t> dp_t, dp_o, dp_f, dp_p;
+  std::vector<double> dp_c;
+  for (const auto &[uk, parts] : groups) {
+    gkeys.push_back(uk);
+    for (const auto &[dk, c] : parts) {
+      if (c == 0.0) continue;
+      dp_t.push_back(std::get<0>(dk));
+      dp_o.push_back(std::get<2>(dk));
+      dp_f.push_back(std::get<1>(dk) ^ std::get<2>(dk));
+      dp_p.push_back(std::get<3>(dk));
+      dp_c.push_back(c.real());
+      dp_c.push_back(c.imag());
+    }
+    dp_ptr.push_back((int32_t)dp_t.size());
+  }
+  std::vector<int64_t> mx_ptr{0}, mx_base;
+  std::vector<int32_t> mx_grp, mx_dgrp;
+  std::vector<double> mx_sign;
+  for (int64_t g = 0; g < nrows; ++g) {
+    const uint64_t u = rows_bits[g];
+    for (size_t k = 0; k < gkeys.size(); ++k) {
+      const auto &[t, in, out, par] = gkeys[k];
+      if ((u & t) != out || dp_ptr[k + 1] == dp_ptr[k]) continue;
+      int32_t grp;
+      const int64_t r2 = source_row((uint32_t)(u ^ (in ^ out)), &grp);
+      if (r2 < 0) continue;
+      mx_grp.push_back((int32_t)k);
+      mx_base.push_back(row_base[r2]);
+      mx_dgrp.push_back(grp);
+      mx_sign.push_back(parity64(u & par) ? -1.0 : 1.0);
+    }
+    mx_ptr.push_back((int64_t)mx_base.size());
+  }
+
+  // work units: (row, chunk of columns)
+  std::vector<int32_t> wg_row, wg_c0;
+  for (int64_t g = 0; g < nrows; ++g)
+    for (int c0 = 0; c0 < row_nc[g]; c0 += kExprChunk) {
+      wg_row.push_back((int32_t)g);
+      wg_c0.push_back(c0);
+    }
+  if (wg_row.size() >= (size_t{1} << 31)) throw std::invalid_argument("sector too large");
+
+  auto nonempty = [](auto v) {
+    if (v.empty()) v.resize(1);
+    return v;
+  };
+  tb_.D = D;
+  tb_.nrows = nrows;
+  tb_.nwg = (int64_t)wg_row.size();
+  tb_.L = L;
+  tb_.n_dn = n_dn;
+  tb_.momentum = mom ? 1 : 0;
+  tb_.wg_row = keep(nonempty(wg_row));
+  tb_.wg_c0 = keep(nonempty(wg_c0));
+  tb_.row_base = keep(nonempty(row_base));
+  tb_.row_ncols = keep(nonempty(row_nc));
+  tb_.row_dg0 = keep(nonempty(row_dg0));
+  tb_.diag_up = keep(nonempty(diag_up));
+  tb_.dn_bits = keep(nonempty(dn_bits));
+  tb_.diag_dn = keep(nonempty(diag_dn));
+  tb_.uo_ptr = keep(uo_ptr);
+  tb_.uo_base = keep(nonempty(uo_base));
+  tb_.uo_amp = keep(nonempty(uo_amp));
+  tb_.do_ptr = keep(do_ptr);
+  tb_.do_col = keep(nonempty(do_col));
+  tb_.do_amp = keep(nonempty(do_amp));
+  tb_.mx_ptr = keep(mx_ptr);
+  tb_.mx_base = keep(nonempty(mx_base));
+  tb_.mx_grp = keep(nonempty(mx_grp));
+  tb_.mx_dgrp = keep(nonempty(mx_dgrp));
+  tb_.mx_sign = keep(nonempty(mx_sign));
+  tb_.dp_ptr = keep(dp_ptr);
+  tb_.dp_touched = keep(nonempty(dp_t));
+  tb_.dp_out = keep(nonempty(dp_o));
+  tb_.dp_flip = keep(nonempty(dp_f));
+  tb_.dp_parity = keep(nonempty(dp_p));
+  tb_.dp_c = keep(nonempty(dp_c));
+  if (real) {
+    const Binom &B = binom();
+    std::vector<uint64_t> bt((kMaxSites + 1) * (kMaxSites + 1));
+    for (int n = 0; n <= kMaxSites; ++n)
+      for (int k = 0; k <= kMaxSites; ++k) bt[n * (kMaxSites + 1) + k] = B.c[n][k];
+    tb_.binom = keep(bt);
+  } else {
+    tb_.dn_loc = keep(dn_loc);
+    tb_.dn_mom = keep(dn_mom);
+  }
+  (void)n_up;
+}
+
+int64_t ExpressionOperator::hv(const double *x, double *y, Epi e, const EpiArgs &a,
+                               hipStream_t s) const {
+  return launch_expr_hv(tb_, x, y, e, a, s);
+}
+
+int64_t ExpressionOperator::hv(const float *, float *, Epi, const EpiArgs &, hipStream_t) const {
+  throw std::invalid_argument("FP32 H.v is not available for expression operators");
+}
+
+void ExpressionOperator::export_coo(std::vector<uint64_t> &rows, std::vector<uint64_t> &cols,
+                                    std::vector<std::complex<double>> &vals,
+                                    hipStream_t s) const {
+  const int64_t D = tb_.D;
+  DevBuf refmap((size_t)std::max<int64_t>(D, 1) * sizeof(int64_t));
+  if (real_) {
+    auto &sec = const_cast<Sector &>(*real_);
+    sec.ensure_device();
+    launch_ref_of_pos(perm_of(sec), refmap.as<int64_t>(), s);
+  } else {
+    auto &m = const_cast<MomentumSector &>(*mom_);
+    m.ensure_device();
+    MB_HIP(hipMemcpyAsync(refmap.p, m.d_ref_of_pos.p, (size_t)D * sizeof(int64_t),
+                          hipMemcpyDeviceToDevice, s));
+  }
+  DevBuf keys, vbuf;
+  const int64_t n = expr_export(tb_, refmap.as<int64_t>(), keys, vbuf, s);
+  std::vector<uint64_t> k(n);
+  std::vector<double> v(2 * n);
+  if (n > 0) {
+    MB_HIP(hipMemcpyAsync(k.data(), keys.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+    MB_HIP(hipMemcpyAsync(v.data(), vbuf.p, (size_t)n * 16, hipMemcpyDeviceToHost, s));
+  }
+  MB_HIP(hipStreamSynchronize(s));
+  rows.resize(n);
+  cols.resize(n);
+  vals.resize(n);
+  for (int64_t i = 0; i < n; ++i) {
+    cols[i] = k[i] / (uint64_t)D;
+    rows[i] = k[i] % (uint64_t)D;
+    vals[i] = {v[2 * i], v[2 * i + 1]};
+  }
+}
+
+}  // namespace mb

@@ -39,6 +39,7 @@
 #include "mb_internal.hpp"
 #include "momentum.hpp"
 #include "conductivity.hpp"
+#include "expression.hpp"
 
 namespace mb {
 
@@ -1125,6 +1126,113 @@ __global__ __launch_bounds__(kBlock) void mom_fill_gaussian_kernel(const int64_t
   }
 }
 
+// ------------------------------------------------------------ compiled expressions
+
+static_assert(kExprChunk == kBlock, "one thread per column of an expression work unit");
+
+// colex rank of a string: sum over its set bits (ascending, i = 0, 1, ...) of
+// C(position, i + 1), the rank Sector::colex_rank gives
+__device__ __forceinline__ int64_t colex_rank_dev(const BinomLds &B, uint32_t bits) {
+  int64_t r = 0;
+  int k = 0;
+  while (bits) {
+    const int p = __builtin_ctz(bits);
+    r += (int64_t)B.c[p][k + 1];
+    ++k;
+    bits &= bits - 1;
+  }
+  return r;
+}
+
+__device__ __forceinline__ void cfma(double ar, double ai, double xr, double xi, double &yr,
+                                     double &yi) {
+  yr += ar * xr - ai * xi;
+  yi += ar * xi + ai * xr;
+}
+
+// H.v of a compiled expression (expression.cpp).  One workgroup per (row,
+// chunk of kExprChunk columns); row = up string u, column = down string d of
+// the row's group.  Per element: the diagonal (per-row + per-column values),
+// the up-only sources (row bases shared by the workgroup, same column), the
+// down-only sources (columns of the same row), and the mixed terms: for each up
+// part the row meets (uniform), its down parts tested on d.  One partial sum
+// per workgroup and plane for the Lanczos epilogue.
+template <Epi E>
+__global__ __launch_bounds__(kBlock) void expr_hv_kernel(ExprTables t, const double *__restrict__ x,
+                                                         double *__restrict__ y, EpiArgs a) {
+  __shared__ BinomLds B;
+  __shared__ double red[2][kBlock / kWave];
+  if constexpr (E == Epi::Lanczos) {
+    if (a.flags[0]) return;
+  }
+  if (!t.momentum) load_binom(B, t.binom);
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int g = t.wg_row[w];
+  const int c = t.wg_c0[w] + tid;
+  const int nc = t.row_ncols[g];
+  const int64_t D = t.D, base = t.row_base[g];
+  const double bp = beta_prev<double, E>(a);
+  Partial2 accr, acci;
+  if (c < nc) {
+    const int h = t.row_dg0[g] + c;
+    const uint32_t d = t.dn_bits[h];
+    const int64_t i = base + c;
+    const double xr = x[i], xi = x[D + i];
+    double yr = 0.0, yi = 0.0;
+    cfma(t.diag_up[2 * g] + t.diag_dn[2 * h], t.diag_up[2 * g + 1] + t.diag_dn[2 * h + 1], xr,
+         xi, yr, yi);
+    for (int64_t e = t.uo_ptr[g]; e < t.uo_ptr[g + 1]; ++e) {
+      const int64_t o = t.uo_base[e] + c;
+      cfma(t.uo_amp[2 * e], t.uo_amp[2 * e + 1], x[o], x[D + o], yr, yi);
+    }
+    for (int64_t e = t.do_ptr[h]; e < t.do_ptr[h + 1]; ++e) {
+      const int64_t o = base + t.do_col[e];
+      cfma(t.do_amp[2 * e], t.do_amp[2 * e + 1], x[o], x[D + o], yr, yi);
+    }
+    for (int64_t e = t.mx_ptr[g]; e < t.mx_ptr[g + 1]; ++e) {
+      const int k = t.mx_grp[e];
+      const int64_t b = t.mx_base[e];
+      const int grp = t.mx_dgrp[e];
+      const double su = t.mx_sign[e];
+      for (int l = t.dp_ptr[k]; l < t.dp_ptr[k + 1]; ++l) {
+        if ((d & t.dp_touched[l]) != t.dp_out[l]) continue;
+        const uint32_t d2 = d ^ t.dp_flip[l];
+        int64_t col;
+        if (t.momentum) {
+          if (t.dn_mom[d2] != grp) continue;
+          col = t.dn_loc[d2];
+        } else {
+          col = colex_rank_dev(B, d2);
+        }
+        const double sg = (popc(d & t.dp_parity[l]) & 1) ? -su : su;
+        const int64_t o = b + col;
+        cfma(sg * t.dp_c[2 * l], sg * t.dp_c[2 * l + 1], x[o], x[D + o], yr, yi);
+      }
+    }
+    epilogue<double, E>(yr, xr, i, y, a, bp, accr);
+    epilogue<double, E>(yi, xi, D + i, y, a, bp, acci);
+  }
+  if constexpr (E == Epi::Lanczos) {
+    const double sr = wg_sum(accr.s0, red[0]);
+    const double si = wg_sum(acci.s0, red[1]);
+    if (tid == 0) {
+      a.partial[w] = sr;
+      a.partial[t.nwg + w] = si;
+    }
+  }
+}
+
+// reference index of every device position of a real-space sector
+__global__ __launch_bounds__(kBlock) void ref_of_pos_kernel(PermTables p, int64_t *__restrict__ out) {
+  __shared__ BinomLds B;
+  load_binom(B, p.binom);
+  const int64_t D = p.d_up * p.d_dn;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
+    const int64_t u = e / p.d_dn, d = e - u * p.d_dn;
+    out[e] = ref_rank(B, p.L, p.n_up, p.n_dn, p.s_up[u], p.s_dn[d]);
+  }
+}
+
 bool force_generic() {
   const char *e = std::getenv("MANYBODY_AMD_HV_PATH");
   return e && std::strcmp(e, "generic") == 0;
@@ -1301,6 +1409,26 @@ void launch_transfer(const TransferTables &t, const double *x_src, double *y_tgt
   if (t.rows >= (1ll << 31)) throw std::invalid_argument("sector too large");
   const size_t lds = (size_t)2 * t.max_cols * sizeof(double);
   hipLaunchKernelGGL(transfer_kernel, dim3((unsigned)t.rows), dim3(kBlock), lds, s, t, x_src, y_tgt);
+  MB_HIP(hipGetLastError());
+}
+
+int64_t launch_expr_hv(const ExprTables &t, const double *x, double *y, Epi epi, const EpiArgs &a,
+                       hipStream_t s) {
+  if (t.nwg == 0) return 0;
+  const dim3 g((unsigned)t.nwg);
+  switch (epi) {
+    case Epi::Plain: hipLaunchKernelGGL((expr_hv_kernel<Epi::Plain>), g, dim3(kBlock), 0, s, t, x, y, a); break;
+    case Epi::Lanczos: hipLaunchKernelGGL((expr_hv_kernel<Epi::Lanczos>), g, dim3(kBlock), 0, s, t, x, y, a); break;
+    case Epi::Recon: hipLaunchKernelGGL((expr_hv_kernel<Epi::Recon>), g, dim3(kBlock), 0, s, t, x, y, a); break;
+    case Epi::Cheb: throw std::invalid_argument("KPM is not available for expression operators");
+  }
+  MB_HIP(hipGetLastError());
+  return t.nwg;  // partial slots per plane
+}
+
+void launch_ref_of_pos(const PermTables &p, int64_t *out, hipStream_t s) {
+  hipLaunchKernelGGL(ref_of_pos_kernel, dim3(grid_for(p.d_up * p.d_dn, 1 << 16)), dim3(kBlock), 0,
+                     s, p, out);
   MB_HIP(hipGetLastError());
 }
 

@@ -493,3 +493,92 @@ def optical_conductivity(correlation, dt: float, beta: float, volume: float):
     wts = np.ones(nf)
     wts[0] = wts[-1] = 0.5
     return freqs, sigma, float(np.sum(sigma.real * wts) * dw)
+
+
+# ------------------------------------------------------------ generic expressions
+
+_TOL = 1000.0 * np.finfo(np.float64).eps   # tolerances::tolerance<double>()
+
+
+def _add_to(target: dict, key, c: complex):
+    """ExpressionBase::add_to (expression_base.h:85-97): drop |c| < 1000 eps,
+    merge by key, erase a merged sum below 1000 eps."""
+    if abs(c) ** 2 < _TOL * _TOL:
+        return
+    if key in target:
+        target[key] += c
+        if abs(target[key]) ** 2 < _TOL * _TOL:
+            del target[key]
+    else:
+        target[key] = c
+
+
+def expression_terms(terms):
+    """[(coeff, [(type, spin, orbital), ...]), ...] -> {ops tuple: coeff} merged in
+    the order given (FermionExpression += FermionExpression(FermionMonomial))."""
+    out: dict = {}
+    for c, ops in terms:
+        _add_to(out, tuple(tuple(int(v) for v in op) for op in ops), complex(c))
+    return out
+
+
+def apply_monomial(ops, L: int, up: int, dn: int):
+    """The product ops[0] ops[1] ... ops[-1] applied to the state
+    c+_{up i1} ... c+_{up ia} c+_{dn j1} ... c+_{dn jb} |0> (ascending, up first: the
+    reference's state key order, basis.cpp:273-276) one operator at a time, last
+    first: c_m / c+_m on mode m = spin L + orbital gives (-1)^(occupied modes below
+    m).  Returns (up, dn, sign) or None."""
+    s = up | (dn << L)
+    sign = 1
+    for typ, spin, orb in reversed(ops):
+        m = spin * L + orb
+        occ = (s >> m) & 1
+        if typ == 1 and not occ or typ == 0 and occ:
+            return None
+        if bin(s & ((1 << m) - 1)).count("1") & 1:
+            sign = -sign
+        s ^= 1 << m
+    return s & ((1 << L) - 1), s >> L, sign
+
+
+def expression_matrix(L: int, states, terms):
+    """Dense <i|A|j> of compute_matrix_elements (matrix_elements.h:74-105) over
+    `states` [(up, dn), ...] in basis order: each column from A|j>, outputs outside
+    the basis dropped, contributions to one output merged by add_to."""
+    expr = expression_terms(terms)
+    idx = {(int(u), int(d)): i for i, (u, d) in enumerate(states)}
+    D = len(states)
+    M = np.zeros((D, D), dtype=np.complex128)
+    for j, (u, d) in enumerate(states):
+        col: dict = {}
+        for ops, c in expr.items():
+            r = apply_monomial(ops, L, int(u), int(d))
+            if r is None:
+                continue
+            i = idx.get((r[0], r[1]))
+            if i is not None:
+                _add_to(col, i, c * r[2])
+        for i, v in col.items():
+            M[i, j] = v
+    return M
+
+
+def sparse_column_major(M):
+    """(rows, cols, values) of the nonzeros of M in arma::SpMat iteration order:
+    column by column, rows ascending."""
+    cols, rows = np.nonzero(M.T)
+    return rows.astype(np.uint64), cols.astype(np.uint64), M[rows, cols]
+
+
+def cereal_sparse_bytes(n_rows: int, n_cols: int, rows, cols, vals) -> bytes:
+    """The cereal BinaryOutputArchive bytes of arma_cereal.h's save()
+    (src/utils/arma_cereal.h:9-16): the class version (uint32 0, written by cereal
+    for a versioned save the first time the type is seen), n_rows, n_cols,
+    n_nonzero (arma::uword, 64-bit), then per nonzero row, col (uint64) and the
+    value as real, imag (float64), little endian."""
+    import struct
+    out = [struct.pack("<I", 0), struct.pack("<QQQ", n_rows, n_cols, len(vals))]
+    for r, c, v in zip(rows, cols, vals):
+        v = complex(v)
+        out.append(struct.pack("<QQdd", int(r), int(c), v.real, v.imag))
+    return b"".join(out)

@@ -17,6 +17,14 @@
 //                                                      nonzeros
 //   ref_driver sigma  dt beta volume < "re im" lines -> "W omega re im" per frequency,
 //                                                      then "R sum_rule"
+//   ref_driver expr   L Nup Ndn < terms             -> states, then the nonzeros of the
+//                                                      FermionExpression read from stdin
+//   ref_driver mexpr  ndim L.. Nup Ndn K.. < terms  -> the same on a fixed-momentum basis
+//        terms: one per line, "re im n  type spin orbital  ..." (n operators,
+//        type 0 creation / 1 annihilation, spin 0 up / 1 down), added in order
+//   ref_driver cereal path < "rows cols" then "row col re im" lines
+//                                                   -> the cereal binary archive of a
+//                                                      sparse matrix, written to path
 //
 // Reference entry points exercised:
 //   FermionBasis::with_fixed_particle_number_and_spin   src/algebra/basis.cpp:91-126
@@ -26,6 +34,8 @@
 //   HubbardModelMomentum::hamiltonian                    src/algebra/fermion/model/hubbard_model_momentum.h:119-175
 //   HubbardModelMomentum::current                        src/algebra/fermion/model/hubbard_model_momentum.h:50-93
 //   compute_optical_conductivity                         src/algorithms/optical_conductivity.cpp:8-80
+//   FermionExpression / FermionMonomial (+=, add_to)     src/algebra/expression_base.h:85-120
+//   cereal::BinaryOutputArchive (vendored third-party/cereal)
 //
 // The reference key holds at most 7 operators and silently drops longer
 // products (src/algebra/fermion/term.h:11-19, src/algebra/expression_base.h:184-186),
@@ -45,6 +55,10 @@
 #include "utils/index.h"
 #include "algorithms/optical_conductivity.h"
 #include "algebra/matrix_elements.h"
+
+#include <cereal/archives/binary.hpp>
+#include <cereal/types/complex.hpp>
+#include <fstream>
 
 namespace {
 
@@ -88,11 +102,106 @@ void print_matrix(const FermionBasis& basis, const Model& model) {
 }
 
 int usage() {
-  std::fprintf(stderr, "usage: ref_driver basis|h1d|h2d|h3d|g1d|g2d|g3d|mbasis|mh|mj|sigma ...\n");
+  std::fprintf(stderr,
+               "usage: ref_driver basis|h1d|h2d|h3d|g1d|g2d|g3d|mbasis|mh|mj|sigma|expr|mexpr|"
+               "cereal ...\n");
   return 2;
 }
 
+// Terms from stdin into a FermionExpression, each added as the reference's
+// FermionExpression(FermionMonomial) does (merge by key, drop below 1000 eps,
+// drop keys longer than the monomial's capacity).  Returns the longest term.
+size_t read_expression(FermionExpression& e) {
+  double re, im;
+  int n;
+  size_t longest = 0;
+  while (std::scanf("%lf %lf %d", &re, &im, &n) == 3) {
+    FermionMonomial m(FermionMonomial::complex_type(re, im));
+    for (int k = 0; k < n; ++k) {
+      int type, spin, orb;
+      if (std::scanf("%d %d %d", &type, &spin, &orb) != 3) return (size_t)-1;
+      const auto sp = spin ? FermionOperator::Spin::Down : FermionOperator::Spin::Up;
+      m.operators.push_back(type ? FermionOperator::annihilation(sp, (size_t)orb)
+                                 : FermionOperator::creation(sp, (size_t)orb));
+    }
+    longest = std::max(longest, (size_t)n);
+    e += FermionExpression(m);
+  }
+  return longest;
+}
+
+// A sparse matrix as the reference's arma_cereal.h save() writes it
+// (src/utils/arma_cereal.h:9-16): a versioned non-member save emitting
+// (n_rows, n_cols, n_nonzero) then (row, col, value) per nonzero in the
+// column-major iteration order of arma::SpMat; arma::uword is 64-bit.
+struct TripletMatrix {
+  uint64_t n_rows = 0, n_cols = 0;
+  std::vector<uint64_t> r, c;
+  std::vector<std::complex<double>> v;
+};
+
 }  // namespace
+
+namespace cereal {
+template <class Archive>
+void save(Archive& ar, const TripletMatrix& t, unsigned int /* version */) {
+  ar(t.n_rows, t.n_cols, (uint64_t)t.v.size());
+  for (size_t i = 0; i < t.v.size(); ++i) ar(t.r[i], t.c[i], t.v[i]);
+}
+}  // namespace cereal
+
+int expr_mode(const std::string& mode, const std::vector<double>& a) {
+  FermionBasis basis = [&] {
+    if (mode == "expr") {
+      if (a.size() != 3) throw std::invalid_argument("expr L Nup Ndn");
+      const size_t n = (size_t)a[1] + (size_t)a[2];
+      return FermionBasis::with_fixed_particle_number_and_spin((size_t)a[0], n,
+                                                               (int)a[1] - (int)a[2]);
+    }
+    const size_t nd = a.empty() ? 0 : (size_t)a[0];
+    if (nd < 1 || nd > 3 || a.size() != 1 + nd + 2 + nd) throw std::invalid_argument("mexpr");
+    std::vector<size_t> dims, K;
+    size_t sites = 1;
+    for (size_t d = 0; d < nd; ++d) {
+      dims.push_back((size_t)a[1 + d]);
+      sites *= dims.back();
+    }
+    const size_t nup = (size_t)a[1 + nd], ndn = (size_t)a[2 + nd];
+    for (size_t d = 0; d < nd; ++d) K.push_back((size_t)a[3 + nd + d]);
+    return FermionBasis::with_fixed_particle_number_spin_momentum(
+        sites, nup + ndn, (int)nup - (int)ndn, Index(dims), K);
+  }();
+  FermionExpression e;
+  const size_t longest = read_expression(e);
+  if (longest == (size_t)-1) return usage();
+  const size_t n = basis.set.empty() ? 0 : basis.set[0].size();
+  if (longest + n > 7) {
+    std::fprintf(stderr, "terms times states exceed the 7-operator key\n");
+    return 3;
+  }
+  print_states(basis);
+  print_operator(basis, e);
+  return 0;
+}
+
+int cereal_mode(int argc, char** argv) {
+  if (argc != 3) return usage();
+  TripletMatrix t;
+  unsigned long long r, c;
+  if (std::scanf("%llu %llu", &r, &c) != 2) return usage();
+  t.n_rows = r;
+  t.n_cols = c;
+  double re, im;
+  while (std::scanf("%llu %llu %lf %lf", &r, &c, &re, &im) == 4) {
+    t.r.push_back(r);
+    t.c.push_back(c);
+    t.v.emplace_back(re, im);
+  }
+  std::ofstream ofs(argv[2], std::ios::binary);
+  cereal::BinaryOutputArchive archive(ofs);
+  archive(t);
+  return 0;
+}
 
 // mbasis / mh: ndim L.. Nup Ndn K.. [t U]
 int momentum_mode(const std::string& mode, const std::vector<double>& a) {
@@ -172,6 +281,8 @@ int main(int argc, char** argv) {
   if (mode == "mbasis" || mode == "mh") return momentum_mode(mode, a);
   if (mode == "mj") return current_mode(a);
   if (mode == "sigma") return sigma_mode(a);
+  if (mode == "expr" || mode == "mexpr") return expr_mode(mode, a);
+  if (mode == "cereal") return cereal_mode(argc, argv);
 
   size_t sites = 0, nup = 0, ndn = 0;
   double t = 0, u = 0;

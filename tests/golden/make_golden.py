@@ -17,7 +17,11 @@ tests/golden/ref_momentum.npz (fixed-momentum basis state lists for every K of
 ten lattices, and the momentum-space Hamiltonian HubbardModelMomentum on eleven
 sectors inside its exact range), tests/golden/ref_conductivity.npz (the
 current operator J_d(Q) on seven lattices and compute_optical_conductivity on
-four seeded correlators),
+four seeded correlators), tests/golden/ref_expression.npz (compute_matrix_elements
+of generic FermionExpressions read by the driver: random Hermitian one- and
+two-body terms with spin flips, odd and non-conserving monomials, constants,
+cancelling and sub-tolerance terms, on real-space and fixed-momentum bases; and
+cereal binary archives of sparse matrices from the vendored cereal),
 tests/golden/manifest.json.
 """
 from __future__ import annotations
@@ -50,6 +54,134 @@ def run(args):
 
 def digest(states: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(states, np.uint64).tobytes()).hexdigest()
+
+
+def _random_expression(rng, L, n_one, n_two, hermitian=True, odd=0):
+    """Terms [(coeff, [(type, spin, orbital), ...])]: one-body c+_{a s} c_{b s'} and
+    two-body c+ c+ c c with random modes (spin flips included) and complex
+    coefficients, each followed by its adjoint when hermitian; `odd` extra
+    monomials of 1 to 3 random operators."""
+    def op(t):
+        return (t, int(rng.integers(2)), int(rng.integers(L)))
+
+    def adj(ops):
+        return [(1 - t, s, o) for t, s, o in reversed(ops)]
+
+    terms = []
+    for n_ops, count in ((2, n_one), (4, n_two)):
+        for _ in range(count):
+            ops = [op(0) for _ in range(n_ops // 2)] + [op(1) for _ in range(n_ops // 2)]
+            c = complex(rng.standard_normal(), rng.standard_normal())
+            terms.append((c, ops))
+            if hermitian:
+                terms.append((c.conjugate(), adj(ops)))
+    for _ in range(odd):
+        ops = [op(int(rng.integers(2))) for _ in range(int(rng.integers(1, 4)))]
+        terms.append((complex(rng.standard_normal(), rng.standard_normal()), ops))
+    return terms
+
+
+def _momentum_expression(rng, dims, n_one, n_two):
+    """Momentum-conserving terms on the lattice `dims` (orbital = flat momentum):
+    c+_{k s} c_{k s'} (spin flips included) and c+_{k1+q s} c+_{k2-q s'} c_{k2 s'} c_{k1 s}
+    with complex coefficients and their adjoints, plus two non-conserving terms."""
+    sys.path.insert(0, REPO)
+    from oracle import pyoracle as po
+    N = int(np.prod(dims))
+    terms = []
+
+    def add(c, ops):
+        terms.append((c, ops))
+        terms.append((c.conjugate(), [(1 - t, s, o) for t, s, o in reversed(ops)]))
+
+    for _ in range(n_one):
+        k = int(rng.integers(N))
+        add(complex(rng.standard_normal(), rng.standard_normal()),
+            [(0, int(rng.integers(2)), k), (1, int(rng.integers(2)), k)])
+    for _ in range(n_two):
+        k1, k2, q = (int(v) for v in rng.integers(N, size=3))
+        s1, s2 = int(rng.integers(2)), int(rng.integers(2))
+        add(complex(rng.standard_normal(), rng.standard_normal()),
+            [(0, s1, po.momentum_add(dims, k1, q)), (0, s2, po.momentum_add(dims, k2, q, -1)),
+             (1, s2, k2), (1, s1, k1)])
+    terms.append((0.4, [(0, 0, 1 % N), (1, 0, 0)]))
+    terms.append((0.2j, [(0, 1, 0), (1, 1, 0), (0, 0, 1 % N), (1, 1, 1 % N)]))
+    return terms
+
+
+def _special_terms(L):
+    """Monomials whose action needs care: 1 - n as c c+, c+ c c+ c, a repeated
+    creator (zero), a constant, a coefficient below 1000 eps, a key added twice
+    (merged) and a pair that cancels (erased)."""
+    return [(0.75, [(1, 0, 1), (0, 0, 1)]),
+            (-1.25 + 0.5j, [(0, 0, 0), (1, 0, 0), (0, 1, L - 1), (1, 1, 0)]),
+            (2.0, [(0, 1, 2), (0, 1, 2), (1, 1, 1), (1, 1, 0)]),
+            (0.3 - 0.1j, []),
+            (1e-14, [(0, 0, 1), (1, 0, 0)]),
+            (0.5, [(0, 1, 1), (1, 1, 0)]), (0.25j, [(0, 1, 1), (1, 1, 0)]),
+            (1.5, [(0, 0, 2), (1, 1, 2)]), (-1.5, [(0, 0, 2), (1, 1, 2)]),
+            (0.6, [(0, 1, 0), (1, 0, 1), (0, 0, 1), (1, 1, 2)])]
+
+
+def _terms_text(terms):
+    return "".join(f"{complex(c).real:.17g} {complex(c).imag:.17g} {len(ops)} "
+                   + " ".join(f"{t} {s} {o}" for t, s, o in ops) + "\n" for c, ops in terms)
+
+
+def expression_cases():
+    """compute_matrix_elements of generic expressions through the driver, and
+    cereal archives of three sparse matrices."""
+    rng = np.random.default_rng(77)
+    arrays, manifest = {}, []
+    cases = [("expr", [4], 2, 1, None, _random_expression(rng, 4, 6, 4)),
+             ("expr", [5], 1, 1, None, _random_expression(rng, 5, 5, 3)),
+             ("expr", [3], 2, 2, None, _random_expression(rng, 3, 4, 0, odd=4)),
+             ("expr", [4], 2, 1, None, _special_terms(4)),
+             ("expr", [6], 1, 2, None, _random_expression(rng, 6, 6, 3, hermitian=False, odd=3)),
+             ("mexpr", [4], 2, 1, [1], _momentum_expression(rng, [4], 4, 6)),
+             ("mexpr", [2, 2], 1, 2, [1, 0], _momentum_expression(rng, [2, 2], 4, 6)),
+             ("mexpr", [5], 1, 2, [2], _momentum_expression(rng, [5], 5, 8)),
+             ("mexpr", [7], 2, 1, [3], _momentum_expression(rng, [7], 6, 16)),
+             ("mexpr", [2, 2], 2, 1, [0, 1], _special_terms(4))]
+    for k, (mode, dims, nu, nd, K, terms) in enumerate(cases):
+        args = [mode] + ([dims[0], nu, nd] if mode == "expr" else
+                         [len(dims)] + dims + [nu, nd] + K)
+        out = subprocess.run([DRIVER] + [str(a) for a in args], input=_terms_text(terms),
+                             check=True, capture_output=True, text=True).stdout
+        st = np.array([(int(l.split()[1]), int(l.split()[2])) for l in out.splitlines()
+                       if l[0] == "S"], np.uint64).reshape(-1, 2)
+        el = np.array([[float(x) for x in l.split()[1:]] for l in out.splitlines()
+                       if l[0] == "E"], np.float64).reshape(-1, 4)
+        name = f"{mode}{k}_{'x'.join(map(str, dims))}_{nu}_{nd}"
+        arrays[name + "__states"] = st
+        arrays[name + "__ij"] = el[:, :2].astype(np.int64)
+        arrays[name + "__val"] = el[:, 2] + 1j * el[:, 3]
+        arrays[name + "__coef"] = np.array([complex(c) for c, _ in terms], np.complex128)
+        arrays[name + "__nops"] = np.array([len(o) for _, o in terms], np.int64)
+        arrays[name + "__ops"] = np.array([v for _, o in terms for op in o for v in op],
+                                          np.int64).reshape(-1, 3)
+        manifest.append({"name": name, "kind": mode, "dims": dims, "n_up": nu, "n_dn": nd,
+                         "K": K, "dim": int(len(st)), "nnz": int(len(el))})
+    for k, (nr, nc, nnz) in enumerate([(0, 0, 0), (4, 5, 3), (37, 37, 200)]):
+        r = rng.integers(0, max(nr, 1), nnz)
+        c = rng.integers(0, max(nc, 1), nnz)
+        key = sorted(set(zip(c.tolist(), r.tolist())))   # column-major, unique
+        v = rng.standard_normal(len(key)) + 1j * rng.standard_normal(len(key))
+        txt = f"{nr} {nc}\n" + "".join(f"{rr} {cc} {x.real:.17g} {x.imag:.17g}\n"
+                                       for (cc, rr), x in zip(key, v))
+        path = os.path.join(HERE, "_cereal.bin")
+        subprocess.run([DRIVER, "cereal", path], input=txt, check=True, text=True)
+        with open(path, "rb") as f:
+            blob = f.read()
+        os.remove(path)
+        name = f"cereal{k}_{nr}x{nc}"
+        arrays[name + "__rows"] = np.array([rr for _, rr in key], np.uint64)
+        arrays[name + "__cols"] = np.array([cc for cc, _ in key], np.uint64)
+        arrays[name + "__vals"] = v
+        arrays[name + "__bytes"] = np.frombuffer(blob, np.uint8)
+        manifest.append({"name": name, "kind": "cereal", "n_rows": nr, "n_cols": nc,
+                         "nnz": len(key)})
+    return arrays, manifest
 
 
 def main():
@@ -205,6 +337,9 @@ def main():
                               "volume": vol, "sum_rule": R})
     np.savez_compressed(os.path.join(HERE, "ref_conductivity.npz"), **cond_arrays)
 
+    expr_arrays, expr_manifest = expression_cases()
+    np.savez_compressed(os.path.join(HERE, "ref_expression.npz"), **expr_arrays)
+
     np.savez_compressed(os.path.join(HERE, "ref_momentum.npz"), **mom_arrays)
     np.savez_compressed(os.path.join(HERE, "ref_basis.npz"), **basis_arrays)
     np.savez_compressed(os.path.join(HERE, "ref_correlation.npz"), **corr_arrays)
@@ -214,7 +349,7 @@ def main():
                    "source": "reference algebra core compiled by oracle/Makefile (target ref)",
                    "basis": basis_manifest, "hamiltonian": ham_manifest,
                    "correlation": corr_manifest, "momentum": mom_manifest,
-                   "conductivity": cond_manifest}, f, indent=1)
+                   "conductivity": cond_manifest, "expression": expr_manifest}, f, indent=1)
     print(f"{len(basis_manifest)} basis sectors, {len(ham_manifest)} Hamiltonians, "
           f"{len(corr_manifest)} correlation operators")
 

@@ -148,3 +148,96 @@ def test_optical_conductivity_rejects_invalid_arguments():
         mb.compute_optical_conductivity(np.ones(8), 0.0, 1.0, 1.0)
     with pytest.raises(ValueError):
         mb.compute_optical_conductivity(np.ones(8), 0.1, 1.0, 0.0)
+
+
+def _gold(name):
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", name))
+
+
+def _manifest():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "manifest.json")) as f:
+        return json.load(f)
+
+
+def _oracle_matrix(L, states, expr):
+    terms = [(c, [(o >> 15, (o >> 14) & 1, o & 0x3FFF) for o in k]) for k, c in expr.terms.items()]
+    return po.expression_matrix(L, states, terms)
+
+
+def test_model_expressions_match_reference_golden():
+    """HubbardModel{,2D,3D}.hamiltonian() and HubbardModelMomentum.hamiltonian()
+    built by the Python mirror give the compiled reference's matrices (28
+    real-space and the momentum-space golden sectors)."""
+    H = _gold("ref_hamiltonian.npz")
+    for e in _manifest()["hamiltonian"]:
+        dims = e["dims"]
+        model = (mb.HubbardModel(e["t"], e["U"], dims[0]) if len(dims) == 1 else
+                 mb.HubbardModel2D(e["t"], e["U"], *dims) if len(dims) == 2 else
+                 mb.HubbardModel3D(e["t"], e["U"], *dims))
+        L = int(np.prod(dims))
+        up, dn = po.basis_states(L, e["n_up"], e["n_dn"])
+        M = _oracle_matrix(L, list(zip(up, dn)), model.hamiltonian())
+        R = np.zeros_like(M)
+        ij = H[e["name"] + "__ij"]
+        R[ij[:, 0], ij[:, 1]] = H[e["name"] + "__val"]
+        assert np.abs(M - R).max() <= 1e-14, e["name"]
+    G = _gold("ref_momentum.npz")
+    n = 0
+    for e in _manifest()["momentum"]:
+        if e["kind"] != "hamiltonian":
+            continue
+        dims = e["dims"]
+        st = G[e["name"] + "__states"]
+        M = _oracle_matrix(int(np.prod(dims)), st,
+                           mb.HubbardModelMomentum(e["t"], e["U"], dims).hamiltonian())
+        R = np.zeros_like(M)
+        ij = G[e["name"] + "__ij"]
+        R[ij[:, 0], ij[:, 1]] = G[e["name"] + "__val"]
+        assert np.abs(M - R).max() <= 1e-12, e["name"]
+        n += 1
+    assert n >= 10
+
+
+def test_fermion_expression_algebra_follows_add_to():
+    e = mb.hopping(0.5 + 0.25j, 0, 2, mb.Spin.Down)
+    assert len(e) == 2 and e.adjoint().terms == e.terms  # hopping is Hermitian
+    e += mb.density(mb.Spin.Up, 1) * 1e-14            # below 1000 eps: dropped
+    assert len(e) == 2
+    n = mb.density(mb.Spin.Up, 1)
+    z = n * 1.5 - n * 1.5                              # cancels: erased
+    assert len(z) == 0
+    dd = mb.density_density(mb.Spin.Up, 0, mb.Spin.Down, 0)
+    assert list(dd.terms) == [(0x0000, 0x8000, 0x4000, 0xC000)]
+    two = mb.two_body(0, 1, 1, 2, 1, 3, 0, 4)
+    assert list(two.terms) == [(0x0001, 0x4002, 0xC003, 0x8004)]
+    prod = (mb.creation(0, 0) + 1.0) * mb.annihilation(1, 3)
+    assert prod.terms == {(0x0000, 0xC003): 1.0, (0xC003,): 1.0}
+    assert (2.0 * prod).terms[(0xC003,)] == 2.0 and len(prod * 0.0) == 0
+
+
+def test_sparse_matrix_cereal_matches_reference_golden(tmp_path):
+    """SparseMatrix.save writes the bytes the vendored cereal wrote for the same
+    matrices; load reads them back."""
+    gold = _gold("ref_expression.npz")
+    n = 0
+    for c in _manifest()["expression"]:
+        if c["kind"] != "cereal":
+            continue
+        rows, cols, vals = (gold[c["name"] + s] for s in ("__rows", "__cols", "__vals"))
+        sp = mb.SparseMatrix.from_entries((c["n_rows"], c["n_cols"]), rows, cols, vals)
+        path = tmp_path / (c["name"] + ".bin")
+        sp.save(str(path))
+        assert path.read_bytes() == gold[c["name"] + "__bytes"].tobytes(), c["name"]
+        back = mb.SparseMatrix.load(str(path))
+        r2, c2, v2 = back.entries()
+        assert back.shape == (c["n_rows"], c["n_cols"]) and back.nnz == c["nnz"]
+        assert np.array_equal(r2, rows) and np.array_equal(c2, cols) and np.array_equal(v2, vals)
+        n += 1
+    assert n == 3
+    with pytest.raises(ValueError):  # duplicate entry
+        mb.SparseMatrix.from_entries((2, 2), [0, 0], [1, 1], [1.0, 2.0])
+    with pytest.raises(RuntimeError):
+        mb.SparseMatrix.load(str(tmp_path / "missing.bin"))

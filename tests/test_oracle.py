@@ -422,3 +422,115 @@ def test_optical_conductivity_of_a_constant_correlator_is_a_geometric_sum():
     integral = dt * ((1 - r ** n) / (1 - r) - (1 + r ** (n - 1)) / 2)
     pref = np.where(w < 1e-12, beta / vol, -np.expm1(-beta * w) / np.where(w < 1e-12, 1, w) / vol)
     assert np.abs(sg - pref * integral).max() <= 1e-10 * np.abs(sg).max()
+
+
+# ------------------------------------------------------------ generic expressions
+
+def _golden_terms(gold, name):
+    coef, nops, ops = gold[name + "__coef"], gold[name + "__nops"], gold[name + "__ops"]
+    terms, k = [], 0
+    for c, n in zip(coef, nops):
+        terms.append((complex(c), [tuple(int(v) for v in ops[k + i]) for i in range(int(n))]))
+        k += int(n)
+    return terms
+
+
+def test_expression_matrix_matches_reference_golden():
+    """compute_matrix_elements of ten generic expressions (random Hermitian one-
+    and two-body terms with spin flips, odd and non-conserving monomials,
+    constants, merged, cancelling and sub-tolerance terms; real-space and
+    fixed-momentum bases) against the compiled reference, and the basis order
+    the driver printed against the oracle's enumeration."""
+    gold = np.load(os.path.join(GOLD, "ref_expression.npz"))
+    n = 0
+    for c in _manifest()["expression"]:
+        if c["kind"] not in ("expr", "mexpr"):
+            continue
+        dims, nu, nd = c["dims"], c["n_up"], c["n_dn"]
+        L = int(np.prod(dims))
+        st = gold[c["name"] + "__states"]
+        if c["kind"] == "expr":
+            up, dn = po.basis_states(L, nu, nd)
+        else:
+            Kc = c["K"] + [0] * (3 - len(dims))
+            dd = dims + [1] * (3 - len(dims))
+            up, dn = po.momentum_sector_states(dims, nu, nd,
+                                               Kc[0] + dd[0] * (Kc[1] + dd[1] * Kc[2]))
+        assert np.array_equal(np.stack([up, dn], 1).astype(np.uint64), st), c["name"]
+        M = po.expression_matrix(L, st, _golden_terms(gold, c["name"]))
+        R = np.zeros_like(M)
+        ij = gold[c["name"] + "__ij"]
+        R[ij[:, 0], ij[:, 1]] = gold[c["name"] + "__val"]
+        assert np.count_nonzero(M) == len(ij), c["name"]
+        assert np.abs(M - R).max() <= 1e-14 * max(1.0, np.abs(R).max()), c["name"]
+        n += 1
+    assert n == 10
+
+
+def test_cereal_bytes_match_reference_golden():
+    """The byte layout of arma_cereal.h's save through cereal::BinaryOutputArchive,
+    against archives the vendored cereal wrote (empty, small and 37 x 37)."""
+    gold = np.load(os.path.join(GOLD, "ref_expression.npz"))
+    n = 0
+    for c in _manifest()["expression"]:
+        if c["kind"] != "cereal":
+            continue
+        b = po.cereal_sparse_bytes(c["n_rows"], c["n_cols"], gold[c["name"] + "__rows"],
+                                   gold[c["name"] + "__cols"], gold[c["name"] + "__vals"])
+        assert b == gold[c["name"] + "__bytes"].tobytes(), c["name"]
+        assert len(b) == 28 + 32 * c["nnz"]
+        n += 1
+    assert n == 3
+
+
+def test_expression_one_body_commutator_identity():
+    """[c+_a c_b, c+_b c_a] = n_a - n_b within a fixed (N_up, N_dn) sector (same
+    spin, so the products stay in the sector): pins the fermionic signs beyond
+    the reference's 7-operator range (L = 7, 3 + 3, every pair)."""
+    L, nu, nd = 7, 3, 3
+    up, dn = po.basis_states(L, nu, nd)
+    st = list(zip(up, dn))
+    for s in (0, 1):
+        for a in range(L):
+            for b in range(L):
+                if a == b:
+                    continue
+                A = po.expression_matrix(L, st, [(1.0, [(0, s, a), (1, s, b)])])
+                B = po.expression_matrix(L, st, [(1.0, [(0, s, b), (1, s, a)])])
+                N = po.expression_matrix(L, st, [(1.0, [(0, s, a), (1, s, a)]),
+                                                 (-1.0, [(0, s, b), (1, s, b)])])
+                assert np.array_equal(A @ B - B @ A, N)
+
+
+def test_expression_two_body_reduces_to_one_body_products():
+    """c+_a c+_b c_c c_d = (c+_a c_d)(c+_b c_c) - delta_bd c+_a c_c (anticommutation,
+    all one spin), on L = 6, 3 + 2 for random index sets including coincident ones."""
+    L, nu, nd = 6, 3, 2
+    up, dn = po.basis_states(L, nu, nd)
+    st = list(zip(up, dn))
+    rng = np.random.default_rng(3)
+    for _ in range(40):
+        s = int(rng.integers(2))
+        a, b, c, d = (int(v) for v in rng.integers(L, size=4))
+        T = po.expression_matrix(L, st, [(1.0, [(0, s, a), (0, s, b), (1, s, c), (1, s, d)])])
+        P = (po.expression_matrix(L, st, [(1.0, [(0, s, a), (1, s, d)])])
+             @ po.expression_matrix(L, st, [(1.0, [(0, s, b), (1, s, c)])]))
+        if b == d:
+            P = P - po.expression_matrix(L, st, [(1.0, [(0, s, a), (1, s, c)])])
+        assert np.array_equal(T, P), (s, a, b, c, d)
+
+
+@pytest.mark.parametrize("sizes,nu,nd,t,U", [([6], 3, 3, 1.0, 4.0), ([3, 2], 3, 2, 0.7, 6.0),
+                                             ([2, 2, 2], 2, 2, 1.0, 3.0), ([2], 1, 1, 1.0, 2.0)])
+def test_hubbard_expression_equals_the_hubbard_oracle(sizes, nu, nd, t, U):
+    """The Hubbard Hamiltonian written as an expression (bond loops of
+    hubbard_model.h, merged bonds at size 2) through expression_matrix equals the
+    oracle's Hubbard CSC (a separate construction), beyond the reference's range."""
+    import manybody_amd.expression as ex
+    L = int(np.prod(sizes))
+    up, dn = po.basis_states(L, nu, nd)
+    e = ex.hubbard_hamiltonian(t, U, sizes)
+    terms = [(c, [(o >> 15, (o >> 14) & 1, o & 0x3FFF) for o in k]) for k, c in e.terms.items()]
+    M = po.expression_matrix(L, list(zip(up, dn)), terms)
+    R = po.build_csc(sizes, nu, nd, t, U).to_scipy().toarray()
+    assert np.abs(M - R).max() <= 1e-13
