@@ -22,6 +22,10 @@
 //     184-197)
 //   compute_optical_conductivity (src/algorithms/     mbamd::compute_optical_conductivity
 //     optical_conductivity.h:6-13)
+//   compute_matrix_elements(basis, FermionExpression)  mbamd::expression_operator (device
+//     (src/algebra/matrix_elements.h:74-105)          operator), mbamd::compute_matrix_elements
+//   save/load of arma::sp_cx_mat with cereal         mbamd::SparseMatrix::save / load
+//     (src/utils/arma_cereal.h:9-30)
 //
 // Vectors are device buffers (DeviceVector, complex128, device product order);
 // to_device_order / to_reference_order convert from and to the reference's
@@ -235,6 +239,8 @@ class HubbardOperator {
     return {e, r};
   }
   const mb_hubbard *handle() const { return h_; }
+  // takes ownership of a handle made by another factory entry point
+  static HubbardOperator adopt(mb_hubbard *h, size_t sites) { return HubbardOperator(h, sites); }
 
  private:
   HubbardOperator(mb_hubbard *h, size_t sites) : h_(h), sites_(sites) {}
@@ -417,6 +423,82 @@ inline OpticalConductivityResult compute_optical_conductivity(
   check(mb_optical_conductivity(c, (int)correlation.size(), dt, beta, volume, r.frequencies.data(),
                                 reinterpret_cast<double *>(r.sigma.data()), nf, &nf, &r.sum_rule));
   return r;
+}
+
+// One term of a FermionExpression: coefficient and operator string in the
+// reference's OperatorStorage encoding (bit 15 annihilation, bit 14 spin down,
+// bits 0-13 orbital), o_1 o_2 ... o_n with o_n acting first.
+struct ExpressionTerm {
+  std::complex<double> c;
+  std::vector<uint16_t> ops;
+};
+
+// compute_matrix_elements(basis, A) as a device operator (H.v, Lanczos,
+// eigenpair, time evolution, energy/residual); terms merged by key as the
+// reference's FermionExpression does.
+inline HubbardOperator expression_operator(const FermionBasis &basis,
+                                           const std::vector<ExpressionTerm> &terms) {
+  std::vector<double> coeffs;
+  std::vector<int64_t> offsets{0};
+  std::vector<uint16_t> ops;
+  for (const auto &t : terms) {
+    coeffs.push_back(t.c.real());
+    coeffs.push_back(t.c.imag());
+    ops.insert(ops.end(), t.ops.begin(), t.ops.end());
+    offsets.push_back((int64_t)ops.size());
+  }
+  mb_hubbard *h = nullptr;
+  check(mb_expression_create(basis.handle(), (int64_t)terms.size(), coeffs.data(), offsets.data(),
+                             ops.data(), &h));
+  return HubbardOperator::adopt(h, basis.orbitals);
+}
+
+// arma::sp_cx_mat as compute_matrix_elements returns it (entries column-major,
+// rows ascending), with the reference's cereal save / load.
+class SparseMatrix {
+ public:
+  explicit SparseMatrix(mb_sparse *s) : s_(s) {}
+  SparseMatrix(SparseMatrix &&o) noexcept : s_(std::exchange(o.s_, nullptr)) {}
+  SparseMatrix(const SparseMatrix &) = delete;
+  ~SparseMatrix() { mb_sparse_destroy(s_); }
+  uint64_t n_rows() const { return shape()[0]; }
+  uint64_t n_cols() const { return shape()[1]; }
+  int64_t n_nonzero() const {
+    uint64_t r, c;
+    int64_t n;
+    check(mb_sparse_shape(s_, &r, &c, &n));
+    return n;
+  }
+  void entries(std::vector<uint64_t> &rows, std::vector<uint64_t> &cols,
+               std::vector<std::complex<double>> &vals) const {
+    const int64_t n = n_nonzero();
+    rows.resize(n);
+    cols.resize(n);
+    vals.resize(n);
+    check(mb_sparse_entries(s_, rows.data(), cols.data(), reinterpret_cast<double *>(vals.data())));
+  }
+  void save(const std::string &path) const { check(mb_sparse_save_cereal(s_, path.c_str())); }
+  static SparseMatrix load(const std::string &path) {
+    mb_sparse *s = nullptr;
+    check(mb_sparse_load_cereal(path.c_str(), &s));
+    return SparseMatrix(s);
+  }
+
+ private:
+  std::vector<uint64_t> shape() const {
+    uint64_t r, c;
+    int64_t n;
+    check(mb_sparse_shape(s_, &r, &c, &n));
+    return {r, c};
+  }
+  mb_sparse *s_ = nullptr;
+};
+
+inline SparseMatrix compute_matrix_elements(const HubbardOperator &expression_op,
+                                            hipStream_t s = nullptr) {
+  mb_sparse *sp = nullptr;
+  check(mb_compute_matrix_elements(expression_op.handle(), &sp, s));
+  return SparseMatrix(sp);
 }
 
 // Serving loop: requests admitted as they arrive and as slots and device

@@ -203,6 +203,53 @@ int main() {
     CHECK(dz < 1e-12);  // exp(0) = 1
   }
 
+  {
+    CASE("expression_operator_and_sparse_export");
+    // HubbardModel(1, 4, 6).hamiltonian() written as terms: H.v equals the native
+    // operator, the exported matrix is Hermitian and survives a cereal round trip
+    auto basis = FermionBasis::with_fixed_particle_number_and_spin(6, 5, 1);
+    auto H = HubbardModel(1.0, 4.0, 6).op(basis);
+    std::vector<mbamd::ExpressionTerm> terms;
+    auto op = [](bool ann, int spin, int orb) {
+      return (uint16_t)((ann ? 0x8000 : 0) | (spin ? 0x4000 : 0) | orb);
+    };
+    for (int i = 0; i < 6; ++i)
+      for (int s = 0; s < 2; ++s) {
+        const int j = (i + 1) % 6;
+        terms.push_back({-1.0, {op(false, s, i), op(true, s, j)}});
+        terms.push_back({-1.0, {op(false, s, j), op(true, s, i)}});
+      }
+    for (int i = 0; i < 6; ++i)
+      terms.push_back({4.0, {op(false, 0, i), op(true, 0, i), op(false, 1, i), op(true, 1, i)}});
+    auto E = mbamd::expression_operator(basis, terms);
+    CHECK(E.dimension() == H.dimension());
+    auto x = H.gaussian(4);
+    auto y0 = H.apply(x).to_host(), y1 = E.apply(x).to_host();
+    double d = 0.0, m = 0.0;
+    for (size_t i = 0; i < y0.size(); ++i) {
+      d = std::max(d, std::abs(y0[i] - y1[i]));
+      m = std::max(m, std::abs(y0[i]));
+    }
+    CHECK(d <= 1e-13 * m);
+    auto sp = mbamd::compute_matrix_elements(E);
+    std::vector<uint64_t> r, c;
+    std::vector<std::complex<double>> v;
+    sp.entries(r, c, v);
+    CHECK(sp.n_rows() == H.dimension() && (int64_t)v.size() == sp.n_nonzero());
+    bool sorted = true;
+    for (size_t i = 1; i < v.size(); ++i)
+      sorted &= c[i] > c[i - 1] || (c[i] == c[i - 1] && r[i] > r[i - 1]);
+    CHECK(sorted);
+    const std::string path = "/tmp/mbamd_host_test_matrix.bin";
+    sp.save(path);
+    auto back = mbamd::SparseMatrix::load(path);
+    std::vector<uint64_t> r2, c2;
+    std::vector<std::complex<double>> v2;
+    back.entries(r2, c2, v2);
+    CHECK(r2 == r && c2 == c && v2 == v);
+    std::remove(path.c_str());
+  }
+
   std::printf(g_fail ? "FAILED (%d)\n" : "ALL PASSED\n", g_fail);
   return g_fail ? 1 : 0;
 }
