@@ -329,6 +329,12 @@ ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector 
   tb_.L = L;
   tb_.n_dn = n_dn;
   tb_.momentum = mom ? 1 : 0;
+  for (int64_t g = 0; g < nrows; ++g)
+    tb_.max_uo = std::max<int>(tb_.max_uo, (int)(uo_ptr[g + 1] - uo_ptr[g]));
+  if (real) {
+    tb_.d_up = real->d_up;
+    tb_.d_dn = real->d_dn;
+  }
   tb_.wg_row = keep(nonempty(wg_row));
   tb_.wg_c0 = keep(nonempty(wg_c0));
   tb_.row_base = keep(nonempty(row_base));
@@ -340,6 +346,53 @@ ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector 
   tb_.uo_ptr = keep(uo_ptr);
   tb_.uo_base = keep(nonempty(uo_base));
   tb_.uo_amp = keep(nonempty(uo_amp));
+  // column-major copy of the down-only lists (padding bounded at 4x the entries
+  // plus one column, and at most 255 entries a string)
+  int max_do = 0;
+  for (int64_t h = 0; h < ndn; ++h) max_do = std::max<int>(max_do, (int)(do_ptr[h + 1] - do_ptr[h]));
+  tb_.max_do = max_do;
+  tb_.n_dstr = ndn;
+  if (max_do > 0 && max_do <= 255 &&
+      (int64_t)max_do * ndn <= 4 * (int64_t)do_col.size() + ndn) {
+    std::vector<int32_t> ec((size_t)max_do * ndn, 0);
+    std::vector<double> ea((size_t)2 * max_do * ndn, 0.0);
+    std::vector<uint8_t> cnt(ndn);
+    for (int64_t h = 0; h < ndn; ++h) {
+      const int64_t b = do_ptr[h];
+      cnt[h] = (uint8_t)(do_ptr[h + 1] - b);
+      for (int k = 0; k < cnt[h]; ++k) {
+        ec[(size_t)k * ndn + h] = do_col[b + k];
+        ea[2 * ((size_t)k * ndn + h)] = do_amp[2 * (b + k)];
+        ea[2 * ((size_t)k * ndn + h) + 1] = do_amp[2 * (b + k) + 1];
+      }
+    }
+    tb_.do_cnt = keep(cnt);
+    std::map<std::pair<double, double>, uint32_t> amp_index;
+    bool packable = ndn <= (int64_t{1} << 20);
+    for (size_t e = 0; packable && e < do_col.size(); ++e) {
+      const auto key = std::make_pair(do_amp[2 * e], do_amp[2 * e + 1]);
+      if (!amp_index.count(key)) {
+        const uint32_t id = (uint32_t)amp_index.size();
+        if (id >= 4096) packable = false;
+        amp_index.emplace(key, id);
+      }
+    }
+    if (packable) {
+      std::vector<uint32_t> pk((size_t)max_do * ndn, 0);
+      std::vector<double> amps(2 * std::max<size_t>(amp_index.size(), 1), 0.0);
+      for (const auto &[key, id] : amp_index) {
+        amps[2 * id] = key.first;
+        amps[2 * id + 1] = key.second;
+      }
+      for (size_t q = 0; q < pk.size(); ++q)
+        pk[q] = ((uint32_t)ec[q] << 12) | amp_index[std::make_pair(ea[2 * q], ea[2 * q + 1])];
+      tb_.do_pack = keep(pk);
+      tb_.do_amps = keep(amps);
+    } else {
+      tb_.do_ell_col = keep(ec);
+      tb_.do_ell_amp = keep(ea);
+    }
+  }
   tb_.do_ptr = keep(do_ptr);
   tb_.do_col = keep(nonempty(do_col));
   tb_.do_amp = keep(nonempty(do_amp));

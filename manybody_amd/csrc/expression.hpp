@@ -35,6 +35,8 @@ bool compile_monomial(const uint16_t *ops, int n, int L, std::complex<double> c,
 struct ExprTables {
   int64_t D = 0, nrows = 0, nwg = 0;
   int L = 0, n_dn = 0, momentum = 0;
+  int max_uo = 0;                                     // most up-only sources of a row
+  int64_t d_up = 0, d_dn = 0;                         // real space: the product shape
   const int32_t *wg_row = nullptr, *wg_c0 = nullptr;  // per workgroup
   const int64_t *row_base = nullptr;                 // element offset of column 0
   const int32_t *row_ncols = nullptr, *row_dg0 = nullptr;
@@ -46,6 +48,18 @@ struct ExprTables {
   const int64_t *do_ptr = nullptr;                   // down-only: source column
   const int32_t *do_col = nullptr;
   const double *do_amp = nullptr;
+  // the down-only lists again, column-major (entry k of string h at k * n_dstr + h,
+  // amplitudes interleaved re, im) so that lanes of consecutive strings load
+  // consecutive words; null when padding would exceed the limit
+  int max_do = 0;
+  int64_t n_dstr = 0;
+  const int32_t *do_ell_col = nullptr;
+  const double *do_ell_amp = nullptr;
+  const uint8_t *do_cnt = nullptr;
+  // packed form of the same lists when the columns fit 20 bits and the
+  // amplitudes take at most 4096 values: column << 12 | amplitude index
+  const uint32_t *do_pack = nullptr;
+  const double *do_amps = nullptr;  // (re, im) per amplitude index
   const int64_t *mx_ptr = nullptr, *mx_base = nullptr;  // mixed: per row, applicable up parts
   const int32_t *mx_grp = nullptr, *mx_dgrp = nullptr;
   const double *mx_sign = nullptr;

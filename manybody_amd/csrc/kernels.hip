@@ -1150,14 +1150,86 @@ __device__ __forceinline__ void cfma(double ar, double ai, double xr, double xi,
   yi += ar * xi + ai * xr;
 }
 
+// Up-only part of a compiled expression on a real-space sector, strip by strip
+// (the layout of up_strip_kernel, both planes at once because the amplitudes
+// are complex): a strip is 16 columns of all D_up rows of both planes (2.9 MB
+// at D_up = 11,440), resident in one XCD's L2 while all its row chunks are
+// produced; 16 lanes per row, 32 rows per workgroup, the rows' source lists
+// (byte offset of the source row, amplitude) staged in LDS.  Writes y = the
+// up-only sum with non-temporal stores.
+template <int MAXH>
+__global__ __launch_bounds__(kBlock) void expr_strip_kernel(ExprTables t, const double *__restrict__ x,
+                                                            double *__restrict__ y, int nJ, int nC,
+                                                            const int *flags) {
+  constexpr int LPR = 16, RPW = kWave / LPR, RC = 32;
+  __shared__ uint32_t s_off[RC][MAXH];
+  __shared__ double s_ar[RC][MAXH], s_ai[RC][MAXH];
+  __shared__ int s_n[RC];
+  if (flags && flags[0]) return;
+  const int i = blockIdx.x;
+  const int xcd = i & 7, q = i >> 3;
+  const int s = xcd + 8 * (q / nC);
+  const int c = q % nC;
+  if (s >= nJ) return;
+  const int u_begin = c * RC;
+  const int u_end = (int)min((int64_t)u_begin + RC, t.d_up);
+  for (int e = threadIdx.x; e < RC * MAXH; e += kBlock) {
+    const int r = e / MAXH, k = e % MAXH;
+    const int u = u_begin + r;
+    if (u < u_end) {
+      const int64_t b = t.uo_ptr[u];
+      const int n = (int)(t.uo_ptr[u + 1] - b);
+      if (k == 0) s_n[r] = n;
+      if (k < n) {
+        s_off[r][k] = (uint32_t)(t.uo_base[b + k] * (int64_t)sizeof(double));
+        s_ar[r][k] = t.uo_amp[2 * (b + k)];
+        s_ai[r][k] = t.uo_amp[2 * (b + k) + 1];
+      }
+    } else if (k == 0) {
+      s_n[r] = 0;
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  const int64_t Ddn = t.d_dn, D = t.D;
+  const int64_t col = (int64_t)s * LPR + (lane % LPR);
+  const bool col_ok = col < Ddn;
+  const char *__restrict__ xr = reinterpret_cast<const char *>(x);
+  const char *__restrict__ xi = reinterpret_cast<const char *>(x + D);
+  const uint32_t colb = (uint32_t)(col * (int64_t)sizeof(double));
+  const int sub = lane / LPR;
+  for (int rb = wv * RPW; rb < RC; rb += (kBlock / kWave) * RPW) {
+    const int r = rb + sub;
+    const int u = u_begin + r;
+    const bool ok = col_ok && u < u_end;
+    const int n = ok ? s_n[r] : 0;
+    double ar = 0.0, ai = 0.0;
+#pragma unroll 4
+    for (int k = 0; k < n; ++k) {
+      const uint32_t o = s_off[r][k] + colb;
+      const double vr = *reinterpret_cast<const double *>(xr + o);
+      const double vi = *reinterpret_cast<const double *>(xi + o);
+      cfma(s_ar[r][k], s_ai[r][k], vr, vi, ar, ai);
+    }
+    if (ok) {
+      const int64_t e = (int64_t)u * Ddn + col;
+      __builtin_nontemporal_store(ar, y + e);
+      __builtin_nontemporal_store(ai, y + D + e);
+    }
+  }
+}
+
 // H.v of a compiled expression (expression.cpp).  One workgroup per (row,
 // chunk of kExprChunk columns); row = up string u, column = down string d of
-// the row's group.  Per element: the diagonal (per-row + per-column values),
-// the up-only sources (row bases shared by the workgroup, same column), the
-// down-only sources (columns of the same row), and the mixed terms: for each up
-// part the row meets (uniform), its down parts tested on d.  One partial sum
-// per workgroup and plane for the Lanczos epilogue.
-template <Epi E>
+// the row's group.  Work units are spread so that the chunks of one row run on
+// one XCD (unit = xcd * ceil(nwg / 8) + blockIdx / 8) and its x row stays in
+// that L2.  Per element: the diagonal (per-row + per-column values), the
+// up-only sources (row bases shared by the workgroup, same column; or, with
+// STRIP, the sum expr_strip_kernel left in y), the down-only sources (columns
+// of the same row), and the mixed terms: for each up part the row meets
+// (uniform), its down parts tested on d.  One partial sum per work unit and
+// plane for the Lanczos epilogue.
+template <Epi E, bool STRIP>
 __global__ __launch_bounds__(kBlock) void expr_hv_kernel(ExprTables t, const double *__restrict__ x,
                                                          double *__restrict__ y, EpiArgs a) {
   __shared__ BinomLds B;
@@ -1165,8 +1237,11 @@ __global__ __launch_bounds__(kBlock) void expr_hv_kernel(ExprTables t, const dou
   if constexpr (E == Epi::Lanczos) {
     if (a.flags[0]) return;
   }
+  const int per = (int)((t.nwg + 7) / 8);
+  const int w = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (w >= t.nwg) return;  // uniform over the workgroup
   if (!t.momentum) load_binom(B, t.binom);
-  const int w = blockIdx.x, tid = threadIdx.x;
+  const int tid = threadIdx.x;
   const int g = t.wg_row[w];
   const int c = t.wg_c0[w] + tid;
   const int nc = t.row_ncols[g];
@@ -1179,36 +1254,68 @@ __global__ __launch_bounds__(kBlock) void expr_hv_kernel(ExprTables t, const dou
     const int64_t i = base + c;
     const double xr = x[i], xi = x[D + i];
     double yr = 0.0, yi = 0.0;
+    if constexpr (STRIP) {
+      yr = y[i];
+      yi = y[D + i];
+    }
     cfma(t.diag_up[2 * g] + t.diag_dn[2 * h], t.diag_up[2 * g + 1] + t.diag_dn[2 * h + 1], xr,
          xi, yr, yi);
-    for (int64_t e = t.uo_ptr[g]; e < t.uo_ptr[g + 1]; ++e) {
-      const int64_t o = t.uo_base[e] + c;
-      cfma(t.uo_amp[2 * e], t.uo_amp[2 * e + 1], x[o], x[D + o], yr, yi);
+    if constexpr (!STRIP) {
+      for (int64_t e = t.uo_ptr[g]; e < t.uo_ptr[g + 1]; ++e) {
+        const int64_t o = t.uo_base[e] + c;
+        cfma(t.uo_amp[2 * e], t.uo_amp[2 * e + 1], x[o], x[D + o], yr, yi);
+      }
     }
-    for (int64_t e = t.do_ptr[h]; e < t.do_ptr[h + 1]; ++e) {
-      const int64_t o = base + t.do_col[e];
-      cfma(t.do_amp[2 * e], t.do_amp[2 * e + 1], x[o], x[D + o], yr, yi);
+    if (t.do_pack) {  // column-major packed lists: one coalesced word an entry
+      const int n = t.do_cnt[h];
+      for (int k = 0; k < n; ++k) {
+        const uint32_t w2 = t.do_pack[(int64_t)k * t.n_dstr + h];
+        const int64_t o = base + (w2 >> 12);
+        const uint32_t id = w2 & 0xFFFu;
+        cfma(t.do_amps[2 * id], t.do_amps[2 * id + 1], x[o], x[D + o], yr, yi);
+      }
+    } else if (t.do_ell_col) {  // column-major lists: coalesced over the lanes
+      const int n = t.do_cnt[h];
+      for (int k = 0; k < n; ++k) {
+        const int64_t q = (int64_t)k * t.n_dstr + h;
+        const int64_t o = base + t.do_ell_col[q];
+        cfma(t.do_ell_amp[2 * q], t.do_ell_amp[2 * q + 1], x[o], x[D + o], yr, yi);
+      }
+    } else {
+      for (int64_t e = t.do_ptr[h]; e < t.do_ptr[h + 1]; ++e) {
+        const int64_t o = base + t.do_col[e];
+        cfma(t.do_amp[2 * e], t.do_amp[2 * e + 1], x[o], x[D + o], yr, yi);
+      }
     }
+    double mr = 0.0, mi = 0.0;  // mixed terms that map the element onto itself
     for (int64_t e = t.mx_ptr[g]; e < t.mx_ptr[g + 1]; ++e) {
       const int k = t.mx_grp[e];
       const int64_t b = t.mx_base[e];
       const int grp = t.mx_dgrp[e];
       const double su = t.mx_sign[e];
+      const bool same_row = b == base;
       for (int l = t.dp_ptr[k]; l < t.dp_ptr[k + 1]; ++l) {
         if ((d & t.dp_touched[l]) != t.dp_out[l]) continue;
-        const uint32_t d2 = d ^ t.dp_flip[l];
+        const uint32_t fl = t.dp_flip[l];
+        const double sg = (popc(d & t.dp_parity[l]) & 1) ? -su : su;
+        if (fl == 0 && same_row) {
+          mr += sg * t.dp_c[2 * l];
+          mi += sg * t.dp_c[2 * l + 1];
+          continue;
+        }
+        const uint32_t d2 = d ^ fl;
         int64_t col;
         if (t.momentum) {
           if (t.dn_mom[d2] != grp) continue;
           col = t.dn_loc[d2];
         } else {
-          col = colex_rank_dev(B, d2);
+          col = fl ? colex_rank_dev(B, d2) : c;  // an unchanged down string keeps its column
         }
-        const double sg = (popc(d & t.dp_parity[l]) & 1) ? -su : su;
         const int64_t o = b + col;
         cfma(sg * t.dp_c[2 * l], sg * t.dp_c[2 * l + 1], x[o], x[D + o], yr, yi);
       }
     }
+    cfma(mr, mi, xr, xi, yr, yi);
     epilogue<double, E>(yr, xr, i, y, a, bp, accr);
     epilogue<double, E>(yi, xi, D + i, y, a, bp, acci);
   }
@@ -1412,14 +1519,38 @@ void launch_transfer(const TransferTables &t, const double *x_src, double *y_tgt
   MB_HIP(hipGetLastError());
 }
 
+template <Epi E>
+void launch_expr_e(const ExprTables &t, const double *x, double *y, const EpiArgs &a, hipStream_t s) {
+  // the strip pass: real space, up-only terms present, lists that fit the LDS
+  // staging, planes addressable with 32-bit byte offsets
+  const bool strip = !t.momentum && t.max_uo > 0 && t.max_uo <= 64 &&
+                     t.D * (int64_t)sizeof(double) < ((int64_t)1 << 32) && !force_generic();
+  const int per = (int)((t.nwg + 7) / 8);
+  const dim3 g((unsigned)(8 * per));
+  if (strip) {
+    const int nJ = (int)((t.d_dn + 15) / 16), nC = (int)((t.d_up + 31) / 32);
+    const int64_t grid = (int64_t)8 * ((nJ + 7) / 8) * nC;
+    if (grid >= ((int64_t)1 << 31)) throw std::invalid_argument("sector too large");
+    const int *flags = (E == Epi::Lanczos) ? a.flags : nullptr;
+    if (t.max_uo <= 32)
+      hipLaunchKernelGGL((expr_strip_kernel<32>), dim3((unsigned)grid), dim3(kBlock), 0, s, t, x, y,
+                         nJ, nC, flags);
+    else
+      hipLaunchKernelGGL((expr_strip_kernel<64>), dim3((unsigned)grid), dim3(kBlock), 0, s, t, x, y,
+                         nJ, nC, flags);
+    hipLaunchKernelGGL((expr_hv_kernel<E, true>), g, dim3(kBlock), 0, s, t, x, y, a);
+  } else {
+    hipLaunchKernelGGL((expr_hv_kernel<E, false>), g, dim3(kBlock), 0, s, t, x, y, a);
+  }
+}
+
 int64_t launch_expr_hv(const ExprTables &t, const double *x, double *y, Epi epi, const EpiArgs &a,
                        hipStream_t s) {
   if (t.nwg == 0) return 0;
-  const dim3 g((unsigned)t.nwg);
   switch (epi) {
-    case Epi::Plain: hipLaunchKernelGGL((expr_hv_kernel<Epi::Plain>), g, dim3(kBlock), 0, s, t, x, y, a); break;
-    case Epi::Lanczos: hipLaunchKernelGGL((expr_hv_kernel<Epi::Lanczos>), g, dim3(kBlock), 0, s, t, x, y, a); break;
-    case Epi::Recon: hipLaunchKernelGGL((expr_hv_kernel<Epi::Recon>), g, dim3(kBlock), 0, s, t, x, y, a); break;
+    case Epi::Plain: launch_expr_e<Epi::Plain>(t, x, y, a, s); break;
+    case Epi::Lanczos: launch_expr_e<Epi::Lanczos>(t, x, y, a, s); break;
+    case Epi::Recon: launch_expr_e<Epi::Recon>(t, x, y, a, s); break;
     case Epi::Cheb: throw std::invalid_argument("KPM is not available for expression operators");
   }
   MB_HIP(hipGetLastError());

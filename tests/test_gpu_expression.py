@@ -166,6 +166,27 @@ def test_hubbard_expression_equals_native_operator(cuda, dims, nu, nd, U):
     assert abs(e1 - e0) <= 1e-10 * abs(e0)
 
 
+@pytest.mark.parametrize("dims,nu,nd", [([3, 4], 3, 3), ([10], 5, 4)])
+def test_expression_strip_and_one_pass_paths_agree(cuda, monkeypatch, dims, nu, nd):
+    """Real-space sectors run the up-only terms in a strip pass; the generic-path
+    switch selects the one-pass kernel instead.  Both give the same H.v and the
+    same Lanczos coefficients (fused epilogue on both)."""
+    rng = np.random.default_rng(4)
+    L = int(np.prod(dims))
+    e = random_expression(rng, L, 10, 8, n_odd=3) + mb.HubbardModel(1.0, 4.0, L).hamiltonian()
+    op = mb.FermionExpressionOperator(mb.FermionBasis.sector(L, nu, nd), e)
+    x = op.gaussian(seed=6)
+    y_strip = op.hv(x)
+    d_strip = mb.lanczos_pass_one(op, x, 40)
+    monkeypatch.setenv("MANYBODY_AMD_HV_PATH", "generic")
+    y_one = op.hv(x)
+    d_one = mb.lanczos_pass_one(op, x, 40)
+    monkeypatch.delenv("MANYBODY_AMD_HV_PATH")
+    assert torch.abs(y_strip - y_one).max().item() <= 1e-13 * torch.abs(y_one).max().item()
+    assert np.allclose(d_strip.alphas, d_one.alphas, rtol=1e-12, atol=1e-12)
+    assert np.allclose(d_strip.betas, d_one.betas, rtol=1e-12, atol=1e-12)
+
+
 def test_hubbard_expression_on_4x4_7p7_equals_native_operator(cuda):
     """The full 4 x 4, 7 + 7 sector (D = 130,873,600): the compiled expression of
     HubbardModel2D.hamiltonian() gives the native two-pass H.v."""
