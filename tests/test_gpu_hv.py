@@ -77,6 +77,40 @@ def test_apply_reference_order_bitexact_vs_csc_oracle(cuda, name, sizes, nu, nd,
     assert np.array_equal(y, A.matvec(x))
 
 
+# edge sectors: one site (self bonds on every axis), 32 sites (bit 31 of the
+# strings), a full species, the vacuum, a 3D lattice with a size-1 axis
+EDGE_CASES = [
+    ("1d_L1_1_1_selfbond", [1], 1, 1, 1.0, 3.0),
+    ("1d_L32_1_1_bit31", [32], 1, 1, 1.0, 4.0),
+    ("1d_L32_2_1_bit31", [32], 2, 1, 0.5, 4.0),
+    ("1d_L6_6_3_full_up", [6], 6, 3, 1.0, 4.0),
+    ("1d_L5_0_0_vacuum", [5], 0, 0, 1.0, 4.0),
+    ("3d_2x2x1_2_2", [2, 2, 1], 2, 2, 1.0, 4.0),
+    ("3d_1x1x1_1_0", [1, 1, 1], 1, 0, 1.0, 4.0),
+]
+
+
+@pytest.mark.parametrize("name,sizes,nu,nd,t,U", EDGE_CASES, ids=[c[0] for c in EDGE_CASES])
+def test_hv_edge_sectors_bitexact(cuda, name, sizes, nu, nd, t, U):
+    """Integer inputs, bit for bit against the product oracle, and through the
+    reference-order boundary against the CSC oracle where it applies (L <= 31)."""
+    import torch
+    op = make_op(sizes, nu, nd, t, U)
+    P = po.ProductOperator(sizes, nu, nd, t, U)
+    assert (op.d_up, op.d_dn) == (P.Dup, P.Ddn)
+    x = int_vec(np.random.default_rng(3), P.D)
+    assert np.array_equal(mb.from_planar(op.hv(mb.to_planar(x))), P.matvec(x))
+    if int(np.prod(sizes)) <= 31:
+        A = po.build_csc(sizes, nu, nd, t, U)
+        y = op.apply(torch.from_numpy(x).to(cuda)).cpu().numpy()
+        assert np.array_equal(y, A.matvec(x))
+    # the Krylov layer on the sector: a ground energy (D = 1 breaks down at once)
+    ep = mb.find_min_eigenpair(op, min(60, P.D), op.gaussian(seed=2), want_vector=False)
+    if P.D <= 64:
+        A = po.build_csc(sizes, nu, nd, t, U).to_scipy().toarray()
+        assert abs(ep.value - np.linalg.eigvalsh(A)[0]) <= 1e-10 * max(1.0, abs(ep.value))
+
+
 def test_permutation_matches_oracle_and_roundtrips(cuda):
     import torch
     sizes, nu, nd = [3, 3], 4, 3
