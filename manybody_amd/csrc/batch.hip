@@ -18,6 +18,7 @@
 #include "batch_dev.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <thread>
@@ -240,7 +241,7 @@ Batch::~Batch() {
 template <int BS>
 static void launch_hv_bs(const Batch &b, const double *X, double *V, const double *VP, double *W,
                          int k, int step, hipStream_t s) {
-  static bool attr = false;
+  static std::atomic<bool> attr{false};  // idempotent; host threads may race
   if (!attr) {
     MB_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&bl_hv_kernel<BS>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));

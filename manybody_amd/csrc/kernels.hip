@@ -32,6 +32,8 @@
 // fit in LDS or whose tables cannot be packed.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cstdlib>
 #include <cstring>
 
@@ -1368,16 +1370,18 @@ bool force_compact_row() {
 
 // Compute units of the current device (cached per device).
 int device_cus() {
-  static int cus[64] = {0};
+  static std::atomic<int> cus[64];  // zero-initialised; host threads may race to fill
   int dev = 0;
   MB_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64) dev = 0;
-  if (cus[dev] == 0) {
+  int v = cus[dev].load(std::memory_order_relaxed);
+  if (v == 0) {
     int n = 0;
     MB_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
-    cus[dev] = n > 0 ? n : 1;
+    v = n > 0 ? n : 1;
+    cus[dev].store(v, std::memory_order_relaxed);
   }
-  return cus[dev];
+  return v;
 }
 
 // LDS of the sorted row kernel: the row with its zero slot, then half a row.
@@ -1389,7 +1393,7 @@ size_t sorted_row_lds(const HvTables &t) {
 template <typename T, Epi E>
 void launch_row(const HvTables &t, int P, const T *x, T *y, const EpiArgs &a, hipStream_t s) {
   const size_t lds = (size_t)(t.d_dn + 1) * sizeof(T);  // + the zero slot of the compact table
-  static bool attr_set = false;  // once per template instance
+  static std::atomic<bool> attr_set{false};  // once per template instance (idempotent)
   const void *kc = reinterpret_cast<const void *>(&row_kernel_compact<T, E>);
   const void *ke = reinterpret_cast<const void *>(&row_kernel_ell<T, E>);
   const void *ks = reinterpret_cast<const void *>(&row_kernel_sorted<T, E>);

@@ -17,6 +17,7 @@
 #include "lane.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 #include "batch_dev.hpp"
@@ -190,7 +191,7 @@ LaneEngine::LaneEngine(int64_t buf_doubles, int nslots, int kmax, int tile)
   }
   for (int i = nslots_ - 1; i >= 0; --i) free_slots_.push_back(i);
   free_vec_.push_back({0, buf_});
-  static bool attr = false;
+  static std::atomic<bool> attr{false};  // idempotent; host threads may race
   if (!attr) {
     MB_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&ln_hv_kernel),
                                hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));

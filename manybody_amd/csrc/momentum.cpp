@@ -19,6 +19,8 @@
 // is the constant (U/N) N_up N_dn.
 #include "momentum.hpp"
 
+#include <mutex>
+
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -75,6 +77,8 @@ int64_t MomentumSector::ref_index(uint32_t up, uint32_t dn) const {
 }
 
 void MomentumSector::ensure_device() {
+  static std::mutex mu;  // the first upload is serialised across host threads
+  std::lock_guard<std::mutex> lock(mu);
   int dev = 0;
   MB_HIP(hipGetDevice(&dev));
   if (device == dev) return;

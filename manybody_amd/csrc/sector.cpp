@@ -13,6 +13,8 @@
 
 #include "mb_internal.hpp"
 
+#include <mutex>
+
 namespace mb {
 
 DevAlloc *&thread_dev_alloc() {
@@ -191,6 +193,9 @@ void Sector::ref_state(int64_t index, uint32_t *up_out, uint32_t *dn_out) const 
 }
 
 void Sector::ensure_device() {
+  // operators share a basis across host threads: the first upload is serialised
+  static std::mutex m;
+  std::lock_guard<std::mutex> lock(m);
   int dev = 0;
   MB_HIP(hipGetDevice(&dev));
   if (device == dev && d_s_up.p) return;
