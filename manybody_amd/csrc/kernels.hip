@@ -969,7 +969,12 @@ __global__ __launch_bounds__(kBlock) void mom_hv_kernel(MomTables t, const doubl
   if constexpr (E == Epi::Lanczos) {
     if (a.flags[0]) return;
   }
-  const int g = blockIdx.x, N = t.N, tid = threadIdx.x;
+  // each XCD takes a contiguous range of output rows (workgroups are dispatched
+  // round-robin over the 8 XCDs), so rows of one block share its L2
+  const int per = (int)((t.nrows + 7) / 8);
+  const int g = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (g >= t.nrows) return;  // uniform over the workgroup
+  const int N = t.N, tid = threadIdx.x;
   const int p = t.up_grp[g];
   const int nc = t.ncols[p];
   const int64_t D = t.D;
@@ -1483,7 +1488,8 @@ int64_t launch_hv(const HvTables &t, int P, const T *x, T *y, Epi epi, const Epi
 template <Epi E, int CPT>
 void launch_mom_cpt(const MomTables &t, const double *x, double *y, const EpiArgs &a, hipStream_t s) {
   const size_t lds = (size_t)2 * t.max_cols * sizeof(double);
-  hipLaunchKernelGGL((mom_hv_kernel<E, CPT>), dim3((unsigned)t.nrows), dim3(kBlock), lds, s, t, x, y, a);
+  const unsigned grid = 8u * (unsigned)((t.nrows + 7) / 8);
+  hipLaunchKernelGGL((mom_hv_kernel<E, CPT>), dim3(grid), dim3(kBlock), lds, s, t, x, y, a);
 }
 
 template <Epi E>
