@@ -346,6 +346,27 @@ inline DeviceVector time_evolve_state_steps(const HubbardOperator &H, const Devi
   return out;
 }
 
+// power_method(op, v, iterations) and estimate_bounds(op, power_iterations,
+// spectral_padding) (linear_operator_utils.h:10-78); the seed is explicit.
+template <typename Scalar = std::complex<double>>
+struct Bounds {
+  double alpha = 0.0, beta = 0.0;
+};
+inline double power_method(const HubbardOperator &H, const DeviceVector &seed, size_t iterations,
+                           hipStream_t s = nullptr) {
+  double v = 0.0;
+  check(mb_power_method(H.handle(), seed.data(), (int)iterations, &v, s));
+  return v;
+}
+inline Bounds<> estimate_bounds(const HubbardOperator &H, const DeviceVector &seed,
+                                size_t power_iterations, double spectral_padding,
+                                hipStream_t s = nullptr) {
+  Bounds<> b;
+  check(mb_estimate_bounds(H.handle(), seed.data(), (int)power_iterations, spectral_padding,
+                           &b.alpha, &b.beta, s));
+  return b;
+}
+
 // exp(-tau H) psi (imaginary_time_evolve_state, evolve_state.h:184-197)
 inline DeviceVector imaginary_time_evolve_state(const HubbardOperator &H, const DeviceVector &psi,
                                                 double tau, EvolutionOptions opts = {},

@@ -271,6 +271,21 @@ int mb_optical_conductivity(const double *corr, int n, double dt, double beta, d
                             double *omega, double *sigma, int capacity, int *n_freqs,
                             double *sum_rule);
 
+/* ------------------------------------------------------------ spectral bounds */
+
+/* power_method(op, v, iterations) (linear_operator_utils.h:18-50) from seed_dev
+ * (device order, not modified): the Rayleigh-type estimate
+ * Re(sum v_i (Hv)_i) / Re(sum v_i v_i) of the last iteration (arma::dot, no
+ * conjugation), v renormalised each step; 0 for a zero seed or iterations 0. */
+int mb_power_method(const mb_hubbard *h, const void *seed_dev, int iterations, double *value,
+                    mb_stream stream);
+/* estimate_bounds(op, power_iterations, spectral_padding) (:52-78): dominant
+ * |lambda| by the power method, then the power method on H + shift and
+ * shift - H with shift = dominant (1 + padding) + eps; *alpha <= *beta.  The
+ * reference draws its seed with make_seed_vector; here the caller passes it. */
+int mb_estimate_bounds(const mb_hubbard *h, const void *seed_dev, int power_iterations,
+                       double spectral_padding, double *alpha, double *beta, mb_stream stream);
+
 /* ------------------------------------------------------------ generic expressions */
 
 /* compute_matrix_elements(basis, A) (matrix_elements.h:74-105) for any

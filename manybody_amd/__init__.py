@@ -97,6 +97,9 @@ _SIGS = {
     "mb_time_evolve_steps_ex": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_int,
                                           C.c_int, _vp, _vp]),
     "mb_opposite_spin_correlation": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mb_power_method": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_double), _vp]),
+    "mb_estimate_bounds": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), _vp]),
     "mb_expression_create": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, C.POINTER(_vp)]),
     "mb_compute_matrix_elements": (C.c_int, [_vp, C.POINTER(_vp), _vp]),
     "mb_sparse_create": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, _vp, _vp, _vp,
@@ -616,6 +619,29 @@ def compute_optical_conductivity(correlation, dt: float, beta: float,
                                          om.ctypes.data_as(_dp), sg.ctypes.data_as(_dp),
                                          nf.value, C.byref(nf), C.byref(sr)))
     return OpticalConductivityResult(om, sg[0::2] + 1j * sg[1::2], sr.value)
+
+
+def power_method(op: HubbardOperator, seed, iterations: int) -> float:
+    """power_method(op, v, iterations) (linear_operator_utils.h:18-50), seed in
+    device order (not modified)."""
+    v = C.c_double()
+    _check(lib().mb_power_method(op._h, _ptr(seed), int(iterations), C.byref(v), _stream()))
+    return v.value
+
+
+@dataclass
+class Bounds:
+    alpha: float
+    beta: float
+
+
+def estimate_bounds(op: HubbardOperator, seed, power_iterations: int,
+                    spectral_padding: float) -> Bounds:
+    """estimate_bounds (linear_operator_utils.h:52-78) with an explicit seed."""
+    a, b = C.c_double(), C.c_double()
+    _check(lib().mb_estimate_bounds(op._h, _ptr(seed), int(power_iterations),
+                                    float(spectral_padding), C.byref(a), C.byref(b), _stream()))
+    return Bounds(a.value, b.value)
 
 
 def kpm_dos_moments(op: HubbardOperator, a: float, b: float, M: int, nvec: int, seed: int,

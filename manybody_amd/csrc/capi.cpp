@@ -989,3 +989,25 @@ int mb_sparse_load_cereal(const char *path, mb_sparse **out) {
     *out = sp.release();
   });
 }
+
+// ------------------------------------------------------------ spectral bounds
+
+int mb_power_method(const mb_hubbard *h, const void *seed_dev, int iterations, double *value,
+                    mb_stream stream) {
+  return guarded([&] {
+    need(seed_dev && value, "null argument");
+    *value = mb::power_method(OP(h), seed_dev, iterations, 1.0, 0.0, S(stream));
+  });
+}
+
+int mb_estimate_bounds(const mb_hubbard *h, const void *seed_dev, int power_iterations,
+                       double spectral_padding, double *alpha, double *beta, mb_stream stream) {
+  return guarded([&] {
+    need(seed_dev && alpha && beta, "null argument");
+    need(power_iterations >= 0, "power_iterations must be non-negative");
+    const auto b = mb::estimate_bounds(OP(h), seed_dev, power_iterations, spectral_padding,
+                                       S(stream));
+    *alpha = b.first;
+    *beta = b.second;
+  });
+}

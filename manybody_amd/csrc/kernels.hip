@@ -914,6 +914,36 @@ __global__ __launch_bounds__(kBlock) void scale_complex_kernel(const T *__restri
   }
 }
 
+// One power-method step on the planar vectors v, w (w = H v on entry):
+// w <- sign w + shift v in place, with per-block partial sums of the three
+// quantities power_method reads (linear_operator_utils.h:18-50):
+// Re sum_i v_i w_i and Re sum_i v_i v_i (arma::dot, no conjugation) and
+// sum_i |w_i|^2.
+__global__ __launch_bounds__(kBlock) void power_step_kernel(const double *__restrict__ v,
+                                                            double *__restrict__ w, int64_t D,
+                                                            double sign, double shift,
+                                                            double *__restrict__ partial) {
+  __shared__ double red[kBlock / kWave];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
+    const double vr = v[e], vi = v[D + e];
+    const double wr = sign * w[e] + shift * vr, wi = sign * w[D + e] + shift * vi;
+    w[e] = wr;
+    w[D + e] = wi;
+    a0 += vr * wr - vi * wi;
+    a1 += vr * vr - vi * vi;
+    a2 += wr * wr + wi * wi;
+  }
+  a0 = wg_sum(a0, red);
+  a1 = wg_sum(a1, red);
+  a2 = wg_sum(a2, red);
+  if (threadIdx.x == 0) {
+    partial[3 * blockIdx.x] = a0;
+    partial[3 * blockIdx.x + 1] = a1;
+    partial[3 * blockIdx.x + 2] = a2;
+  }
+}
+
 // x = sum_{j < m} y_j V_j for Krylov vectors V_j kept in HBM (planar, stride
 // 2 D, in T) and complex coefficients y (re, im pairs on the device), summed
 // in FP64 in the order j = 0, 1, ...: one pass over the m vectors in place of
@@ -1738,6 +1768,13 @@ template void launch_recon_update<double>(double *, double *, double, double, do
                                           hipStream_t);
 template void launch_recon_update<float>(float *, double *, double, double, double, int64_t,
                                          hipStream_t);
+
+void launch_power_step(const double *v, double *w, int64_t D, double sign, double shift,
+                       double *partial, hipStream_t s) {
+  hipLaunchKernelGGL(power_step_kernel, dim3(kRedBlocks), dim3(kBlock), 0, s, v, w, D, sign, shift,
+                     partial);
+  MB_HIP(hipGetLastError());
+}
 
 template <typename T>
 void launch_scale_complex(const T *x, double yr, double yi, double *out, int64_t D, hipStream_t s) {

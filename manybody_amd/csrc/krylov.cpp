@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <complex>
 
 namespace mb {
@@ -594,6 +595,62 @@ void kpm_dos_moments(const Hubbard &h, const Sector &sec, double a, double b, in
   }
   const double norm = 1.0 / ((double)nvec * (double)D);
   for (int n = 0; n < M; ++n) mu[n] = acc[n] * norm;
+}
+
+}  // namespace mb
+
+namespace mb {
+
+// ------------------------------------------------------------ power method
+
+double power_method(const HvOp &op, const void *seed, int iterations, double sign, double shift,
+                    hipStream_t s) {
+  if (iterations < 0) throw std::invalid_argument("iterations must be non-negative");
+  if (iterations == 0) return 0.0;
+  const int64_t D = op.dim();
+  DevBuf v((size_t)D * 16), w((size_t)D * 16);
+  DevBuf red(sizeof(double) * (size_t)std::max<int64_t>(3 * kRedBlocks, 2 * kRedBlocks + 4));
+  double *partial = red.as<double>();
+  std::vector<double> host(3 * kRedBlocks);
+  double nrm = 0.0;
+  launch_norm((const double *)seed, 2 * D, partial, partial + 2 * kRedBlocks, s);
+  MB_HIP(hipMemcpyAsync(&nrm, partial + 2 * kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
+  MB_HIP(hipStreamSynchronize(s));
+  if (nrm == 0.0) return 0.0;
+  launch_scale_complex<double>((const double *)seed, 1.0 / nrm, 0.0, v.as<double>(), D, s);
+  double eigenvalue = 0.0;
+  for (int i = 0; i < iterations; ++i) {
+    op.hv(v.as<double>(), w.as<double>(), Epi::Plain, EpiArgs{}, s);
+    launch_power_step(v.as<double>(), w.as<double>(), D, sign, shift, partial, s);
+    MB_HIP(hipMemcpyAsync(host.data(), partial, sizeof(double) * 3 * kRedBlocks,
+                          hipMemcpyDeviceToHost, s));
+    MB_HIP(hipStreamSynchronize(s));
+    double num = 0.0, den = 0.0, ww = 0.0;  // fixed order over the blocks
+    for (int b = 0; b < kRedBlocks; ++b) {
+      num += host[3 * b];
+      den += host[3 * b + 1];
+      ww += host[3 * b + 2];
+    }
+    if (den == 0.0) return 0.0;
+    eigenvalue = num / den;
+    const double n2 = std::sqrt(ww);
+    if (n2 == 0.0) return 0.0;
+    launch_scale_complex<double>(w.as<double>(), 1.0 / n2, 0.0, v.as<double>(), D, s);
+  }
+  MB_HIP(hipStreamSynchronize(s));
+  return eigenvalue;
+}
+
+std::pair<double, double> estimate_bounds(const HvOp &op, const void *seed, int power_iterations,
+                                          double spectral_padding, hipStream_t s) {
+  const double dominant = std::fabs(power_method(op, seed, power_iterations, 1.0, 0.0, s));
+  if (dominant == 0.0) return {0.0, 0.0};
+  const double shift =
+      dominant * (1.0 + spectral_padding) + std::numeric_limits<double>::epsilon();
+  const double beta = power_method(op, seed, power_iterations, 1.0, shift, s) - shift;
+  const double alpha = shift - power_method(op, seed, power_iterations, -1.0, shift, s);
+  if (alpha <= beta) return {alpha, beta};
+  return {beta, alpha};
 }
 
 }  // namespace mb

@@ -3,6 +3,7 @@
 
 #include <complex>
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "mb_internal.hpp"
@@ -102,6 +103,17 @@ void kpm_dos_moments(const Hubbard &h, const Sector &sec, double a, double b, in
                      uint64_t seed, bool fp32, double *mu, hipStream_t s);
 
 // out[0] = sum x_i^2 over n values (FP64 accumulation), or its square root
+// w <- sign w + shift v; partial[3 b + {0, 1, 2}] = Re sum v w, Re sum v v (no
+// conjugation), sum |w|^2 per block b < kRedBlocks
+void launch_power_step(const double *v, double *w, int64_t D, double sign, double shift,
+                       double *partial, hipStream_t s);
+// power_method(op, v, iterations) (linear_operator_utils.h:18-50) on
+// sign H + shift I from seed (device order, not modified)
+double power_method(const HvOp &op, const void *seed, int iterations, double sign, double shift,
+                    hipStream_t s);
+// estimate_bounds(op, power_iterations, spectral_padding) (:52-78): {alpha, beta}
+std::pair<double, double> estimate_bounds(const HvOp &op, const void *seed, int power_iterations,
+                                          double spectral_padding, hipStream_t s);
 void launch_sumsq(const void *x, bool fp32, int64_t n, double *partial, double *out,
                   bool take_sqrt, hipStream_t s);
 

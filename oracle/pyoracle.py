@@ -582,3 +582,42 @@ def cereal_sparse_bytes(n_rows: int, n_cols: int, rows, cols, vals) -> bytes:
         v = complex(v)
         out.append(struct.pack("<QQdd", int(r), int(c), v.real, v.imag))
     return b"".join(out)
+
+
+# ------------------------------------------------------------ spectral bounds
+
+def power_method(matvec, v, iterations: int) -> float:
+    """power_method (linear_operator_utils.h:18-50): v normalised, then per
+    iteration w = A v, estimate Re(sum v w) / Re(sum v v) with arma::dot (no
+    conjugation), v = w / |w|; 0 for a zero vector or a zero denominator."""
+    if iterations == 0:
+        return 0.0
+    v = np.asarray(v, np.complex128)
+    n = np.linalg.norm(v)
+    if n == 0.0:
+        return 0.0
+    v = v / n
+    eig = 0.0
+    for _ in range(iterations):
+        w = matvec(v)
+        den = np.sum(v * v).real
+        if den == 0.0:
+            return 0.0
+        eig = np.sum(v * w).real / den
+        n = np.linalg.norm(w)
+        if n == 0.0:
+            return 0.0
+        v = w / n
+    return float(eig)
+
+
+def estimate_bounds(matvec, seed, power_iterations: int, spectral_padding: float):
+    """estimate_bounds (linear_operator_utils.h:52-78): (alpha, beta) with
+    alpha <= beta, from power iterations on A, A + shift and shift - A."""
+    dominant = abs(power_method(matvec, seed, power_iterations))
+    if dominant == 0.0:
+        return 0.0, 0.0
+    shift = dominant * (1.0 + spectral_padding) + np.finfo(np.float64).eps
+    beta = power_method(lambda v: matvec(v) + shift * v, seed, power_iterations) - shift
+    alpha = shift - power_method(lambda v: -matvec(v) + shift * v, seed, power_iterations)
+    return (alpha, beta) if alpha <= beta else (beta, alpha)

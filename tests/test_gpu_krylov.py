@@ -230,3 +230,40 @@ def test_time_evolve_fp32_error_vs_fp64_is_stated(cuda):
     ref = po.time_evolve_csc(A, b_ref, 0.7, 30)
     got32 = op8.to_reference_order(mb.time_evolve_state(op8, b, 0.7, opts, fp32=True)).cpu().numpy()
     assert np.linalg.norm(got32 - ref) / np.linalg.norm(ref) < 1e-5
+
+
+# ------------------------------------------------------------ spectral bounds
+
+@pytest.mark.parametrize("kind", ["real", "momentum", "expression"])
+def test_power_method_and_bounds_match_oracle(cuda, kind):
+    """power_method / estimate_bounds (linear_operator_utils.h:18-78) on the
+    device against the oracle's restatement with the same seed and iteration
+    counts (1e-9 relative: the same iterates up to rounding)."""
+    import torch
+    if kind == "real":
+        basis = mb.FermionBasis.sector(8, 4, 3)
+        op = mb.HubbardModel(1.0, 4.0, 8).operator(basis)
+        A = po.build_csc([8], 4, 3, 1.0, 4.0)
+        mat = A.to_scipy()
+    elif kind == "momentum":
+        basis = mb.FermionBasis.with_fixed_particle_number_spin_momentum(9, 5, 1, [3, 3], [1, 0])
+        op = mb.HubbardMomentumFactorizedFixedK(basis, [3, 3], 1.0, 6.0, [1, 0])
+        mat = po.momentum_hamiltonian([3, 3], 3, 2, 1, 1.0, 6.0)
+    else:
+        basis = mb.FermionBasis.sector(7, 3, 3)
+        e = mb.HubbardModel(1.0, 3.0, 7).hamiltonian() + mb.hopping(0.4j, 0, 3, mb.Spin.Down)
+        op = mb.FermionExpressionOperator(basis, e)
+        up, dn = po.basis_states(7, 3, 3)
+        terms = [(c, [(o >> 15, (o >> 14) & 1, o & 0x3FFF) for o in k]) for k, c in e.terms.items()]
+        mat = po.expression_matrix(7, list(zip(up, dn)), terms)
+    seed = op.gaussian(seed=12)
+    seed_ref = op.to_reference_order(seed).cpu().numpy()
+    mv = (lambda v: mat @ v)
+    got = mb.power_method(op, seed, 30)
+    want = po.power_method(mv, seed_ref, 30)
+    assert abs(got - want) <= 1e-9 * abs(want)
+    b = mb.estimate_bounds(op, seed, 30, 0.1)
+    wa, wb = po.estimate_bounds(mv, seed_ref, 30, 0.1)
+    assert abs(b.alpha - wa) <= 1e-9 * abs(wa) and abs(b.beta - wb) <= 1e-9 * abs(wb)
+    # the seed is not modified
+    assert torch.equal(seed, op.gaussian(seed=12))

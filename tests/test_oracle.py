@@ -534,3 +534,39 @@ def test_hubbard_expression_equals_the_hubbard_oracle(sizes, nu, nd, t, U):
     M = po.expression_matrix(L, list(zip(up, dn)), terms)
     R = po.build_csc(sizes, nu, nd, t, U).to_scipy().toarray()
     assert np.abs(M - R).max() <= 1e-13
+
+
+# ------------------------------------------------------------ spectral bounds
+
+def test_power_method_known_answer():
+    """Port of test_linear_operator.cpp:128-135: diag(3, -1, 0.5), seed (1, 1, 1),
+    40 iterations -> 3 within 1e-3."""
+    d = np.array([3.0, -1.0, 0.5])
+    assert abs(po.power_method(lambda v: d * v, np.ones(3), 40) - 3.0) < 1e-3
+    assert po.power_method(lambda v: d * v, np.zeros(3), 40) == 0.0
+    assert po.power_method(lambda v: d * v, np.ones(3), 0) == 0.0
+
+
+def test_estimate_bounds_known_answer():
+    """Port of test_linear_operator.cpp:137-144: diag(-2, 0.5, 4), 40 iterations,
+    padding 0.2 -> (-2, 4) within 1e-3 (the reference seeds with randu; here a
+    seeded U[0,1) complex vector)."""
+    d = np.array([-2.0, 0.5, 4.0])
+    rng = np.random.default_rng(3)
+    seed = rng.random(3) + 1j * rng.random(3)
+    a, b = po.estimate_bounds(lambda v: d * v, seed, 40, 0.2)
+    assert abs(a + 2.0) < 1e-3 and abs(b - 4.0) < 1e-3
+
+
+def test_power_method_unconjugated_dot_converges_for_real_symmetric():
+    """With a complex seed the estimate Re(v.w)/Re(v.v) (no conjugation) still
+    converges to the dominant eigenvalue of a real symmetric matrix; bounds
+    match eigvalsh for a spectrum with clear gaps."""
+    rng = np.random.default_rng(5)
+    Q, _ = np.linalg.qr(rng.standard_normal((30, 30)))
+    lam = np.concatenate([[-9.0], np.linspace(-3, 3, 28), [7.0]])
+    A = (Q * lam) @ Q.T
+    seed = rng.standard_normal(30) + 1j * rng.standard_normal(30)
+    assert abs(po.power_method(lambda v: A @ v, seed, 200) + 9.0) < 1e-8
+    a, b = po.estimate_bounds(lambda v: A @ v, seed, 400, 0.1)
+    assert abs(a + 9.0) < 1e-6 and abs(b - 7.0) < 1e-6
