@@ -348,7 +348,7 @@ inline DeviceVector time_evolve_state_steps(const HubbardOperator &H, const Devi
 
 // power_method(op, v, iterations) and estimate_bounds(op, power_iterations,
 // spectral_padding) (linear_operator_utils.h:10-78); the seed is explicit.
-template <typename Scalar = std::complex<double>>
+// Bounds (linear_operator_utils.h:10-16) for the complex operators: alpha <= beta
 struct Bounds {
   double alpha = 0.0, beta = 0.0;
 };
@@ -358,10 +358,10 @@ inline double power_method(const HubbardOperator &H, const DeviceVector &seed, s
   check(mb_power_method(H.handle(), seed.data(), (int)iterations, &v, s));
   return v;
 }
-inline Bounds<> estimate_bounds(const HubbardOperator &H, const DeviceVector &seed,
+inline Bounds estimate_bounds(const HubbardOperator &H, const DeviceVector &seed,
                                 size_t power_iterations, double spectral_padding,
                                 hipStream_t s = nullptr) {
-  Bounds<> b;
+  Bounds b;
   check(mb_estimate_bounds(H.handle(), seed.data(), (int)power_iterations, spectral_padding,
                            &b.alpha, &b.beta, s));
   return b;
