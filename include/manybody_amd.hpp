@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include <complex>
+#include <functional>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -348,6 +349,60 @@ inline DeviceVector time_evolve_state_steps(const HubbardOperator &H, const Devi
 
 // power_method(op, v, iterations) and estimate_bounds(op, power_iterations,
 // spectral_padding) (linear_operator_utils.h:10-78); the seed is explicit.
+// solve(op, b, k, solver) (lanczos.h:220-237): solver(alphas, betas) gives the
+// first-column coefficients; returns |b| sum_j y_j v_j (device order)
+using TridiagSolver = std::function<std::vector<std::complex<double>>(
+    const std::vector<double> &, const std::vector<double> &)>;
+inline DeviceVector solve(const HubbardOperator &H, const DeviceVector &b, size_t k,
+                          const TridiagSolver &solver, hipStream_t s = nullptr) {
+  struct Ctx {
+    const TridiagSolver *f;
+  } ctx{&solver};
+  auto tramp = [](int m, const double *al, const double *be, double *y, void *user) -> int {
+    try {
+      const auto &f = *static_cast<Ctx *>(user)->f;
+      const auto r = f(std::vector<double>(al, al + m),
+                       std::vector<double>(be, be + (m > 0 ? m - 1 : 0)));
+      if ((int)r.size() != m) return 1;
+      for (int i = 0; i < m; ++i) {
+        y[2 * i] = r[i].real();
+        y[2 * i + 1] = r[i].imag();
+      }
+      return 0;
+    } catch (...) {
+      return 1;
+    }
+  };
+  DeviceVector x(H.dimension());
+  check(mb_lanczos_solve(H.handle(), b.data(), (int)k, tramp, &ctx, x.data(), s));
+  return x;
+}
+
+// find_max_eigenpair(op, k) (lanczos.h:246-281) with an explicit start vector
+inline EigenPair find_max_eigenpair(const HubbardOperator &H, size_t k, const DeviceVector &b,
+                                    hipStream_t s = nullptr) {
+  EigenPair ep{0.0, DeviceVector(H.dimension())};
+  check(mb_find_max_eigenpair(H.handle(), b.data(), (int)k, &ep.value, ep.vector.data(), s));
+  return ep;
+}
+
+// Exp(op, ExpOptions{krylov_steps}) (exp_operator.h:46-89): apply(v) = exp(H) v
+class Exp {
+ public:
+  explicit Exp(const HubbardOperator &op, size_t krylov_steps = 30)
+      : op_(op), k_(krylov_steps) {}
+  DeviceVector apply(const DeviceVector &v, hipStream_t s = nullptr) const {
+    DeviceVector out(op_.dimension());
+    check(mb_exp_apply(op_.handle(), v.data(), (int)k_, out.data(), s));
+    return out;
+  }
+  size_t dimension() const { return op_.dimension(); }
+
+ private:
+  const HubbardOperator &op_;
+  size_t k_;
+};
+
 // Bounds (linear_operator_utils.h:10-16) for the complex operators: alpha <= beta
 struct Bounds {
   double alpha = 0.0, beta = 0.0;

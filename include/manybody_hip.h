@@ -198,6 +198,9 @@ int mb_time_evolve_ex(const mb_hubbard *h, const void *psi_dev, double t, int kr
                       void *out_dev, mb_stream stream);
 int mb_time_evolve_steps_ex(const mb_hubbard *h, const void *psi_dev, double t0, double t1, double dt,
                             int krylov_steps, int flags, void *out_dev, mb_stream stream);
+/* eigen-decomposition of the symmetric tridiagonal T (alphas on the diagonal,
+ * m - 1 betas beside it; arma::eig_sym at lanczos.h:269): eigenvalues ascending
+ * in w[m], eigenvectors as the columns of V[m * m] (column-major; V may be NULL) */
 int mb_tridiag_eig(int m, const double *alphas, const double *betas, double *w, double *V);
 
 /* KPM density-of-states moments mu_0..mu_{M-1} of Hs = (H - b)/a, averaged
@@ -285,6 +288,26 @@ int mb_power_method(const mb_hubbard *h, const void *seed_dev, int iterations, d
  * reference draws its seed with make_seed_vector; here the caller passes it. */
 int mb_estimate_bounds(const mb_hubbard *h, const void *seed_dev, int power_iterations,
                        double spectral_padding, double *alpha, double *beta, mb_stream stream);
+
+/* Exp(op, ExpOptions{krylov_steps}).apply(v) (exp_operator.h:50-89): exp(H) v
+ * by the Krylov solve with exp(T_k) e1 (k = krylov_steps, 0 or > D meaning D);
+ * v returned unchanged when |v| <= 1000 eps. */
+int mb_exp_apply(const mb_hubbard *h, const void *v_dev, int krylov_steps, void *out_dev,
+                 mb_stream stream);
+
+/* solve(op, b, k, solver) (lanczos.h:220-237): Lanczos pass one from b, then
+ * the caller's solver fills y[2m] (re, im) from the m alphas and m - 1 betas
+ * (return 0; nonzero aborts with MB_ERR_RUNTIME), then x = |b| sum_j y_j v_j
+ * (pass two, or the stored Krylov basis).  Eigenpairs, exp and linear solves
+ * of the reference are instances; x is zero when no step was taken. */
+typedef int (*mb_tridiag_solver)(int m, const double *alphas, const double *betas, double *y,
+                                 void *user);
+int mb_lanczos_solve(const mb_hubbard *h, const void *b_dev, int k, mb_tridiag_solver solver,
+                     void *user, void *x_dev, mb_stream stream);
+/* find_max_eigenpair(op, k) (lanczos.h:246-281) with an explicit start vector;
+ * x_dev may be NULL (value only) */
+int mb_find_max_eigenpair(const mb_hubbard *h, const void *b_dev, int k, double *value,
+                          void *x_dev, mb_stream stream);
 
 /* ------------------------------------------------------------ generic expressions */
 

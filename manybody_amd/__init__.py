@@ -97,6 +97,9 @@ _SIGS = {
     "mb_time_evolve_steps_ex": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_int,
                                           C.c_int, _vp, _vp]),
     "mb_opposite_spin_correlation": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mb_exp_apply": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
+    "mb_lanczos_solve": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
+    "mb_find_max_eigenpair": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_double), _vp, _vp]),
     "mb_power_method": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_double), _vp]),
     "mb_estimate_bounds": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), _vp]),
@@ -619,6 +622,62 @@ def compute_optical_conductivity(correlation, dt: float, beta: float,
                                          om.ctypes.data_as(_dp), sg.ctypes.data_as(_dp),
                                          nf.value, C.byref(nf), C.byref(sr)))
     return OpticalConductivityResult(om, sg[0::2] + 1j * sg[1::2], sr.value)
+
+
+_TRIDIAG_SOLVER = C.CFUNCTYPE(C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                              C.POINTER(C.c_double), C.c_void_p)
+
+
+def solve(op: HubbardOperator, b, k: int, solver):
+    """solve(op, b, k, solver) (lanczos.h:220-237): solver(alphas, betas) returns
+    the m first-column coefficients y (real or complex); the result
+    |b| sum_j y_j v_j in device order."""
+    import numpy as np
+
+    def cb(m, al, be, y, _user):
+        try:
+            a = np.ctypeslib.as_array(al, (m,)).copy()
+            bb = np.ctypeslib.as_array(be, (m - 1,)).copy() if m > 1 else np.zeros(0)
+            r = np.asarray(solver(a, bb), dtype=np.complex128).reshape(-1)
+            if r.shape != (m,):
+                return 1
+            out = np.ctypeslib.as_array(y, (2 * m,))
+            out[0::2] = r.real
+            out[1::2] = r.imag
+            return 0
+        except Exception:
+            return 1
+
+    f = _TRIDIAG_SOLVER(cb)
+    x = op.empty()
+    _check(lib().mb_lanczos_solve(op._h, _ptr(b), int(k), C.cast(f, C.c_void_p), None, _ptr(x),
+                                  _stream()))
+    return x
+
+
+def find_max_eigenpair(op: HubbardOperator, k: int, b, want_vector: bool = True) -> "EigenPair":
+    """find_max_eigenpair(op, k) (lanczos.h:246-281) with an explicit start vector."""
+    val = C.c_double()
+    x = op.empty() if want_vector else None
+    _check(lib().mb_find_max_eigenpair(op._h, _ptr(b), int(k), C.byref(val),
+                                       _ptr(x) if x is not None else None, _stream()))
+    return EigenPair(val.value, x)
+
+
+class Exp:
+    """Exp(op, ExpOptions{krylov_steps}) (exp_operator.h:50-89): apply(v) = exp(H) v
+    in device order."""
+
+    def __init__(self, op: HubbardOperator, krylov_steps: int = 30):
+        self.op, self.krylov_steps = op, int(krylov_steps)
+
+    def dimension(self) -> int:
+        return self.op.dimension()
+
+    def apply(self, v):
+        out = self.op.empty()
+        _check(lib().mb_exp_apply(self.op._h, _ptr(v), self.krylov_steps, _ptr(out), _stream()))
+        return out
 
 
 def power_method(op: HubbardOperator, seed, iterations: int) -> float:

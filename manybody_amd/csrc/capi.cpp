@@ -1011,3 +1011,42 @@ int mb_estimate_bounds(const mb_hubbard *h, const void *seed_dev, int power_iter
     *beta = b.second;
   });
 }
+
+int mb_exp_apply(const mb_hubbard *h, const void *v_dev, int krylov_steps, void *out_dev,
+                 mb_stream stream) {
+  return guarded([&] {
+    need(v_dev && out_dev, "null argument");
+    need(krylov_steps >= 0, "krylov_steps must be non-negative");
+    // exp(H) v = exp(-tau H) v at tau = -1 (the same Krylov solve)
+    mb::imaginary_time_evolve(OP(h), v_dev, -1.0, krylov_steps, out_dev, S(stream));
+  });
+}
+
+// ------------------------------------------------------------ generic Krylov solve
+
+int mb_lanczos_solve(const mb_hubbard *h, const void *b_dev, int k, mb_tridiag_solver solver,
+                     void *user, void *x_dev, mb_stream stream) {
+  return guarded([&] {
+    need(b_dev && x_dev && solver, "null argument");
+    need(k >= 0, "k must be non-negative");
+    auto f = [&](const std::vector<double> &al, const std::vector<double> &be) {
+      const int m = (int)al.size();
+      std::vector<double> buf(2 * (size_t)m, 0.0);
+      if (solver(m, al.data(), be.data(), buf.data(), user) != 0)
+        throw std::runtime_error("tridiagonal solver callback failed");
+      std::vector<std::complex<double>> y(m);
+      for (int i = 0; i < m; ++i) y[i] = {buf[2 * i], buf[2 * i + 1]};
+      return y;
+    };
+    mb::krylov_solve(OP(h), b_dev, k, f, x_dev, S(stream));
+  });
+}
+
+int mb_find_max_eigenpair(const mb_hubbard *h, const void *b_dev, int k, double *value,
+                          void *x_dev, mb_stream stream) {
+  return guarded([&] {
+    need(b_dev && value, "null argument");
+    need(k >= 0, "k must be non-negative");
+    *value = mb::find_max_eigenpair(OP(h), b_dev, k, x_dev, S(stream));
+  });
+}

@@ -654,3 +654,66 @@ std::pair<double, double> estimate_bounds(const HvOp &op, const void *seed, int 
 }
 
 }  // namespace mb
+
+namespace mb {
+
+// ------------------------------------------------------------ generic solve
+
+// solve(op, b, k, solver) (lanczos.h:220-237): pass one, y = solver(alphas,
+// betas), y *= |b|, x = V y (stored basis where it fits, else pass two)
+void krylov_solve(const HvOp &op, const void *b, int k, const TridiagSolver &solver, void *x,
+                  hipStream_t s) {
+  const int64_t D = op.dim();
+  DevBuf V;
+  if (k > 0) {
+    const size_t bytes = (size_t)(k + 1) * 2 * D * sizeof(double);
+    if (keep_krylov_basis(bytes)) {
+      try {
+        V.alloc(bytes);
+      } catch (const NoMemError &) {
+        V.release();
+      }
+    }
+  }
+  LanczosWork work(D, k, op.partials(), sizeof(double), V.p == nullptr);
+  LanczosResult dec = V.p ? pass_one_t<double>(op, b, k, s, &work, V.as<double>())
+                          : lanczos_pass_one(op, b, k, s, &work);
+  if (dec.steps == 0) {
+    MB_HIP(hipMemsetAsync(x, 0, (size_t)D * 16, s));
+    MB_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  std::vector<std::complex<double>> y = solver(dec.alphas, dec.betas);
+  if ((int)y.size() != dec.steps)
+    throw std::runtime_error("Dimension mismatch: y_k size must match decomposition steps");
+  for (auto &v : y) v *= dec.b_norm;
+  if (V.p)
+    combine_basis<double>(V.as<double>(), y, D, x, s);
+  else
+    lanczos_pass_two(op, b, dec, y, x, s, &work);
+  MB_HIP(hipStreamSynchronize(s));
+}
+
+// find_max_eigenpair (lanczos.h:246-281): the largest Ritz value and its vector
+double find_max_eigenpair(const HvOp &op, const void *b, int k, void *x, hipStream_t s) {
+  double value = 0.0;
+  auto pick = [&](const std::vector<double> &al, const std::vector<double> &be) {
+    const int m = (int)al.size();
+    std::vector<double> wv(m), Z((size_t)m * m);
+    tridiag_eig(m, al.data(), be.data(), wv.data(), Z.data());
+    value = wv[m - 1];
+    std::vector<std::complex<double>> y(m);
+    for (int i = 0; i < m; ++i) y[i] = Z[i + (size_t)m * (m - 1)];
+    return y;
+  };
+  if (x) {
+    krylov_solve(op, b, k, pick, x, s);
+    return value;
+  }
+  LanczosResult dec = lanczos_pass_one(op, b, k, s);
+  if (dec.steps == 0) return 0.0;
+  pick(dec.alphas, dec.betas);
+  return value;
+}
+
+}  // namespace mb

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <complex>
+#include <functional>
 #include <memory>
 #include <utility>
 #include <vector>
@@ -103,6 +104,14 @@ void kpm_dos_moments(const Hubbard &h, const Sector &sec, double a, double b, in
                      uint64_t seed, bool fp32, double *mu, hipStream_t s);
 
 // out[0] = sum x_i^2 over n values (FP64 accumulation), or its square root
+// solve(op, b, k, solver) (lanczos.h:220-237) with a host solver giving the
+// first-column coefficients y (size = steps taken) from (alphas, betas)
+using TridiagSolver = std::function<std::vector<std::complex<double>>(
+    const std::vector<double> &, const std::vector<double> &)>;
+void krylov_solve(const HvOp &op, const void *b, int k, const TridiagSolver &solver, void *x,
+                  hipStream_t s);
+// find_max_eigenpair (lanczos.h:246-281) with an explicit start vector; x may be null
+double find_max_eigenpair(const HvOp &op, const void *b, int k, void *x, hipStream_t s);
 // w <- sign w + shift v; partial[3 b + {0, 1, 2}] = Re sum v w, Re sum v v (no
 // conjugation), sum |w|^2 per block b < kRedBlocks
 void launch_power_step(const double *v, double *w, int64_t D, double sign, double shift,

@@ -250,6 +250,44 @@ int main() {
     std::remove(path.c_str());
   }
 
+  {
+    CASE("generic_solve_max_eigenpair_exp_bounds");
+    // ring L = 8, 4 + 4, U = 0: spectrum within [-9.657, 9.657] (free fermions)
+    auto basis = FermionBasis::with_fixed_particle_number_and_spin(8, 8, 0);
+    auto H = HubbardModel(1.0, 0.0, 8).op(basis);
+    auto b = H.gaussian(9);
+    auto mx = mbamd::find_max_eigenpair(H, 150, b);
+    CHECK(std::abs(mx.value - 9.656854249492380) < 1e-8);
+    // solve with the lowest-eigenvector callback reproduces find_min_eigenpair
+    auto x = mbamd::solve(H, b, 150, [](const std::vector<double> &al,
+                                         const std::vector<double> &be) {
+      const int m = (int)al.size();
+      std::vector<double> w(m), z((size_t)m * m);
+      mb_tridiag_eig(m, al.data(), be.data(), w.data(), z.data());
+      std::vector<std::complex<double>> y(m);
+      for (int i = 0; i < m; ++i) y[i] = z[i];
+      return y;
+    });
+    auto mn = find_min_eigenpair(H, 150, b);
+    auto xh = x.to_host(), mh = mn.vector.to_host();
+    double d = 0.0, n = 0.0;
+    for (size_t i = 0; i < xh.size(); ++i) {
+      d = std::max(d, std::abs(xh[i] - mh[i]));
+      n = std::max(n, std::abs(mh[i]));
+    }
+    CHECK(d <= 1e-10 * n);
+    auto bounds = mbamd::estimate_bounds(H, b, 60, 0.1);
+    CHECK(bounds.alpha <= bounds.beta && bounds.beta <= 9.656854249492380 + 1e-6);
+    auto e = mbamd::Exp(H, 30).apply(b);  // exp(H) b: norm grows by at most e^{9.66}
+    double ne = 0.0, nb = 0.0;
+    auto eh = e.to_host(), bh = b.to_host();
+    for (size_t i = 0; i < eh.size(); ++i) {
+      ne += std::norm(eh[i]);
+      nb += std::norm(bh[i]);
+    }
+    CHECK(std::sqrt(ne) > std::sqrt(nb) && std::sqrt(ne) < std::exp(9.66) * std::sqrt(nb));
+  }
+
   std::printf(g_fail ? "FAILED (%d)\n" : "ALL PASSED\n", g_fail);
   return g_fail ? 1 : 0;
 }

@@ -267,3 +267,142 @@ def test_power_method_and_bounds_match_oracle(cuda, kind):
     assert abs(b.alpha - wa) <= 1e-9 * abs(wa) and abs(b.beta - wb) <= 1e-9 * abs(wb)
     # the seed is not modified
     assert torch.equal(seed, op.gaussian(seed=12))
+
+
+def diag_operator(d):
+    """diag(d) on the device: one up electron on len(d) orbitals (basis order =
+    orbital order) with sum_i d_i n_i, through the expression compiler."""
+    e = mb.FermionExpression()
+    for i, v in enumerate(d):
+        e += float(v) * mb.density(mb.Spin.Up, i)
+    return mb.FermionExpressionOperator(mb.FermionBasis.sector(len(d), 1, 0), e)
+
+
+def test_reference_linear_operator_known_answers_on_device(cuda):
+    """Ports of test_linear_operator.cpp:128-160 run through the device path:
+    power_method on diag(3, -1, 0.5) from (1, 1, 1) -> 3; estimate_bounds on
+    diag(-2, 0.5, 4) -> (-2, 4) (1e-3); Exp with k = D on diag(-2, 0.5, 1.5)
+    applied to (1, -2, 0.5) -> exp(d) v (1e-6)."""
+    import torch
+    op = diag_operator([3.0, -1.0, 0.5])
+    ones = mb.to_planar(np.ones(3, np.complex128))
+    assert abs(mb.power_method(op, ones, 40) - 3.0) < 1e-3
+    op = diag_operator([-2.0, 0.5, 4.0])
+    b = mb.estimate_bounds(op, op.gaussian(seed=4), 40, 0.2)
+    assert abs(b.alpha + 2.0) < 1e-3 and abs(b.beta - 4.0) < 1e-3
+    d = np.array([-2.0, 0.5, 1.5])
+    op = diag_operator(d)
+    v = np.array([1.0, -2.0, 0.5], np.complex128)
+    out = mb.from_planar(mb.Exp(op, krylov_steps=3).apply(mb.to_planar(v)))
+    assert np.abs(out - np.exp(d) * v).max() < 1e-6
+
+
+def test_exp_matches_expm_on_hubbard(cuda):
+    import scipy.linalg as sla
+    op = mb.HubbardModel(0.3, 0.5, 6).operator(mb.FermionBasis.sector(6, 3, 2))
+    A = po.build_csc([6], 3, 2, 0.3, 0.5).to_scipy().toarray()
+    v = op.gaussian(seed=8)
+    got = op.to_reference_order(mb.Exp(op, 30).apply(v)).cpu().numpy()
+    want = sla.expm(A) @ op.to_reference_order(v).cpu().numpy()
+    assert np.linalg.norm(got - want) <= 1e-10 * np.linalg.norm(want)
+    z = op.empty().zero_()
+    assert np.array_equal(mb.Exp(op).apply(z).cpu().numpy(), z.cpu().numpy())  # |v| = 0
+
+
+def matrix_operator(A):
+    """A real symmetric n x n matrix (n <= 32) as a device operator: one up
+    electron on n orbitals with sum_ij A_ij c+_i c_j (one particle: no fermionic
+    signs, basis order = orbital order), through the expression compiler."""
+    A = np.asarray(A, float)
+    e = mb.FermionExpression()
+    for i in range(A.shape[0]):
+        for j in range(A.shape[1]):
+            if A[i, j] != 0.0:
+                e += A[i, j] * mb.one_body(mb.Spin.Up, i, mb.Spin.Up, j)
+    return mb.FermionExpressionOperator(mb.FermionBasis.sector(A.shape[0], 1, 0), e)
+
+
+def tridiag(al, be):
+    m = len(al)
+    T = np.diag(al)
+    for i in range(m - 1):
+        T[i, i + 1] = T[i + 1, i] = be[i]
+    return T
+
+
+def e1_solve(al, be):
+    e1 = np.zeros(len(al))
+    e1[0] = 1.0
+    return np.linalg.solve(tridiag(al, be), e1)
+
+
+def dev(op, v):
+    import torch
+    return op.to_device_order(torch.from_numpy(np.asarray(v, np.complex128)).to("cuda"))
+
+
+def host(op, x):
+    return op.to_reference_order(x).cpu().numpy()
+
+
+def chain(n, d, h):
+    return np.diag(np.full(n, d)) + np.diag(np.full(n - 1, h), 1) + np.diag(np.full(n - 1, h), -1)
+
+
+def test_reference_lanczos_known_answers_on_device(cuda):
+    """Ports of test_lanczos.cpp:30-236 through the device path."""
+    import scipy.linalg as sla
+    # lanczos_solve_matches_direct_solution (:30-52)
+    A = chain(5, 4.0, -1.0)
+    op = matrix_operator(A)
+    b = np.linspace(1.0, 5.0, 5)
+    x = host(op, mb.solve(op, dev(op, b), 5, e1_solve))
+    assert np.linalg.norm(x - np.linalg.solve(A, b)) / np.linalg.norm(np.linalg.solve(A, b)) < 1e-8
+    # lanczos_{max,min}_eigenpair_matches_diagonal (:54-70)
+    op = matrix_operator(np.diag([1.0, 3.0, -2.0]))
+    seed = op.gaussian(seed=1)
+    mx = mb.find_max_eigenpair(op, 3, seed)
+    assert abs(mx.value - 3.0) < 1e-6
+    v = host(op, mx.vector)
+    w = np.diag([1.0, 3.0, -2.0]) @ v
+    assert abs(np.sum(v * w).real / np.sum(v * v).real - mx.value) < 1e-6  # arma::dot
+    assert abs(mb.find_min_eigenpair(op, 3, seed).value + 2.0) < 1e-6
+    # lanczos_exp_solver_matches_direct_expm (:104-134)
+    A = chain(4, 2.0, -1.0)
+    op = matrix_operator(A)
+    b = np.array([1.0, 0.0, 0.0, 0.0])
+    x = host(op, mb.solve(op, dev(op, b), 4, lambda al, be: sla.expm(tridiag(al, be))[:, 0]))
+    assert np.linalg.norm(x - sla.expm(A) @ b) < 1e-8
+    # lanczos_inverse_approximation_improves_with_k (:136-164)
+    A = np.diag([10.0, 8.0, 6.0, 5.0]) + np.diag([1.0] * 3, 1) + np.diag([1.0] * 3, -1)
+    op = matrix_operator(A)
+    b = np.ones(4)
+    exact = np.linalg.solve(A, b)
+    err = [np.linalg.norm(host(op, mb.solve(op, dev(op, b), k, e1_solve)) - exact)
+           / np.linalg.norm(exact) for k in (2, 3, 4)]
+    assert err[1] <= err[0] and err[2] <= err[1] and err[2] < 1e-10
+    # lanczos_decomposition_tracks_steps_and_norm (:166-186)
+    op = matrix_operator(chain(6, 4.0, 1.0))
+    d = mb.lanczos_pass_one(op, dev(op, np.ones(6)), 5)
+    assert 0 < d.steps_taken <= 5
+    assert len(d.alphas) == d.steps_taken and len(d.betas) + 1 == d.steps_taken
+    assert abs(d.b_norm - np.sqrt(6.0)) < 1e-12 and all(v > 0 for v in d.betas)
+    # lanczos_max_eigenpair_matches_nearest_neighbor_chain (:221-236) at n = 32
+    # (the basis holds at most 32 orbitals; the reference uses n = 80, k = 50)
+    n, k = 32, 20
+    op = matrix_operator(chain(n, 2.0, -1.0))
+    ep = mb.find_max_eigenpair(op, k, op.gaussian(seed=2), want_vector=False)
+    assert abs(ep.value - (2.0 - 2.0 * np.cos(n * np.pi / (n + 1)))) < 2e-2
+
+
+def test_solve_generic_callback_and_errors(cuda):
+    """solve with a complex-valued solver (exp(-i t T) e1) equals time evolution;
+    a solver returning the wrong length is an error."""
+    op = mb.HubbardModel(1.0, 4.0, 6).operator(mb.FermionBasis.sector(6, 3, 2))
+    psi = op.gaussian(seed=5)
+    import scipy.linalg as sla
+    x = mb.solve(op, psi, 30, lambda al, be: sla.expm(-0.3j * tridiag(al, be))[:, 0])
+    y = mb.time_evolve_state(op, psi, 0.3, mb.EvolutionOptions(30))
+    assert (x - y).abs().max().item() <= 1e-12 * y.abs().max().item()
+    with pytest.raises(RuntimeError):
+        mb.solve(op, psi, 10, lambda al, be: np.zeros(3))
