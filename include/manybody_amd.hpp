@@ -347,6 +347,28 @@ inline DeviceVector time_evolve_state_steps(const HubbardOperator &H, const Devi
   return out;
 }
 
+// with callback(t, state_dev) at t0 and after every step (evolve_state.h:199-231);
+// state_dev is the device-order vector being evolved, valid during the call
+inline DeviceVector time_evolve_state_steps(
+    const HubbardOperator &H, const DeviceVector &psi, double t0, double t1, double dt,
+    EvolutionOptions opts, const std::function<void(double, const void *)> &callback,
+    hipStream_t s = nullptr) {
+  DeviceVector out(H.dimension());
+  auto tramp = [](double t, const void *state, void *user) -> int {
+    try {
+      (*static_cast<const std::function<void(double, const void *)> *>(user))(t, state);
+      return 0;
+    } catch (...) {
+      return 1;
+    }
+  };
+  check(mb_time_evolve_steps_cb(H.handle(), psi.data(), t0, t1, dt, (int)opts.krylov_steps,
+                                opts.fp32 ? MB_EVOLVE_FP32 : 0, tramp,
+                                const_cast<std::function<void(double, const void *)> *>(&callback),
+                                out.data(), s));
+  return out;
+}
+
 // power_method(op, v, iterations) and estimate_bounds(op, power_iterations,
 // spectral_padding) (linear_operator_utils.h:10-78); the seed is explicit.
 // solve(op, b, k, solver) (lanczos.h:220-237): solver(alphas, betas) gives the

@@ -198,6 +198,14 @@ int mb_time_evolve_ex(const mb_hubbard *h, const void *psi_dev, double t, int kr
                       void *out_dev, mb_stream stream);
 int mb_time_evolve_steps_ex(const mb_hubbard *h, const void *psi_dev, double t0, double t1, double dt,
                             int krylov_steps, int flags, void *out_dev, mb_stream stream);
+/* time_evolve_state_steps(H, psi, t0, t1, dt, opts, callback) (evolve_state.h:
+ * 199-231): callback(t, state_dev, user) at t0 and after every step, with the
+ * state in out_dev (device order) and the stream synchronised; a nonzero
+ * return aborts with MB_ERR_RUNTIME. */
+typedef int (*mb_step_callback)(double t, const void *state_dev, void *user);
+int mb_time_evolve_steps_cb(const mb_hubbard *h, const void *psi_dev, double t0, double t1,
+                            double dt, int krylov_steps, int flags, mb_step_callback callback,
+                            void *user, void *out_dev, mb_stream stream);
 /* eigen-decomposition of the symmetric tridiagonal T (alphas on the diagonal,
  * m - 1 betas beside it; arma::eig_sym at lanczos.h:269): eigenvalues ascending
  * in w[m], eigenvectors as the columns of V[m * m] (column-major; V may be NULL) */

@@ -98,6 +98,8 @@ _SIGS = {
                                           C.c_int, _vp, _vp]),
     "mb_opposite_spin_correlation": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mb_exp_apply": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
+    "mb_time_evolve_steps_cb": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                          C.c_int, _vp, _vp, _vp, _vp]),
     "mb_lanczos_solve": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
     "mb_find_max_eigenpair": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_double), _vp, _vp]),
     "mb_power_method": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_double), _vp]),
@@ -528,11 +530,32 @@ def time_evolve_state(op: HubbardOperator, psi, t: float,
     return out
 
 
+_STEP_CALLBACK = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p, C.c_void_p)
+
+
 def time_evolve_state_steps(op: HubbardOperator, psi, t0: float, t1: float, dt: float,
-                            opts: EvolutionOptions = EvolutionOptions(), fp32: bool = False):
+                            opts: EvolutionOptions = EvolutionOptions(), fp32: bool = False,
+                            callback=None):
+    """time_evolve_state_steps (evolve_state.h:199-231); callback(t, state) at t0
+    and after every step, state = the device-order tensor being evolved (valid
+    during the call; clone it to keep it)."""
     out = op.empty()
-    _check(lib().mb_time_evolve_steps_ex(op._h, _ptr(psi), t0, t1, dt, opts.krylov_steps,
-                                         EVOLVE_FP32 if fp32 else 0, _ptr(out), _stream()))
+    if callback is None:
+        _check(lib().mb_time_evolve_steps_ex(op._h, _ptr(psi), t0, t1, dt, opts.krylov_steps,
+                                             EVOLVE_FP32 if fp32 else 0, _ptr(out), _stream()))
+        return out
+
+    def cb(t, _state, _user):
+        try:
+            callback(t, out)
+            return 0
+        except Exception:
+            return 1
+
+    f = _STEP_CALLBACK(cb)
+    _check(lib().mb_time_evolve_steps_cb(op._h, _ptr(psi), t0, t1, dt, opts.krylov_steps,
+                                         EVOLVE_FP32 if fp32 else 0, C.cast(f, C.c_void_p), None,
+                                         _ptr(out), _stream()))
     return out
 
 

@@ -1050,3 +1050,20 @@ int mb_find_max_eigenpair(const mb_hubbard *h, const void *b_dev, int k, double 
     *value = mb::find_max_eigenpair(OP(h), b_dev, k, x_dev, S(stream));
   });
 }
+
+int mb_time_evolve_steps_cb(const mb_hubbard *h, const void *psi_dev, double t0, double t1,
+                            double dt, int krylov_steps, int flags, mb_step_callback callback,
+                            void *user, void *out_dev, mb_stream stream) {
+  return guarded([&] {
+    need(psi_dev && out_dev, "null argument");
+    need(krylov_steps >= 0, "krylov_steps must be non-negative");
+    need((flags & ~MB_EVOLVE_FP32) == 0, "unknown time evolution flags");
+    mb::StepCallback cb;
+    if (callback)
+      cb = [&](double t, const void *state) {
+        if (callback(t, state, user) != 0) throw std::runtime_error("step callback failed");
+      };
+    mb::time_evolve_steps(OP(h), psi_dev, t0, t1, dt, krylov_steps, out_dev, S(stream),
+                          (flags & MB_EVOLVE_FP32) != 0, cb);
+  });
+}

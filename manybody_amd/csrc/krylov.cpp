@@ -508,16 +508,24 @@ void time_evolve(const HvOp &op, const void *psi, double t, int krylov_steps, vo
 }
 
 void time_evolve_steps(const HvOp &op, const void *psi, double t0, double t1, double dt,
-                       int krylov_steps, void *out, hipStream_t s, bool fp32) {
+                       int krylov_steps, void *out, hipStream_t s, bool fp32,
+                       const StepCallback &callback) {
   const int64_t D = op.dim();
   if (out != psi) MB_HIP(hipMemcpyAsync(out, psi, (size_t)D * 16, hipMemcpyDeviceToDevice, s));
+  auto report = [&](double t) {
+    if (!callback) return;
+    MB_HIP(hipStreamSynchronize(s));
+    callback(t, out);
+  };
   if (dt == 0.0 || (t1 - t0) == 0.0) {
     MB_HIP(hipStreamSynchronize(s));
+    report(t0);
     return;
   }
   const double direction = (t1 >= t0) ? 1.0 : -1.0;
   dt = std::fabs(dt) * direction;
   double t = t0;
+  report(t);
   DevBuf cache;  // the stored Krylov basis, allocated once for all steps
   while ((direction > 0 && t < t1) || (direction < 0 && t > t1)) {
     const double step = (direction > 0) ? std::min(dt, t1 - t) : std::max(dt, t1 - t);
@@ -526,6 +534,7 @@ void time_evolve_steps(const HvOp &op, const void *psi, double t0, double t1, do
     else
       time_evolve_t<double>(op, out, step, krylov_steps, out, s, &cache);
     t += step;
+    report(t);
   }
 }
 

@@ -77,8 +77,12 @@ std::vector<std::complex<double>> expm_first_column(const LanczosResult &dec, do
 // reductions and the exponential of T_k stay FP64.
 void time_evolve(const HvOp &op, const void *psi, double t, int krylov_steps, void *out,
                  hipStream_t s, bool fp32 = false);
+// callback(t, state) at t0 and after every step (time_evolve_state_steps's
+// callback, evolve_state.h:199-231), the stream synchronised before each call
+using StepCallback = std::function<void(double, const void *)>;
 void time_evolve_steps(const HvOp &op, const void *psi, double t0, double t1, double dt,
-                       int krylov_steps, void *out, hipStream_t s, bool fp32 = false);
+                       int krylov_steps, void *out, hipStream_t s, bool fp32 = false,
+                       const StepCallback &callback = nullptr);
 // FP64 exp(-i t H) psi, or exp(-t H) psi with imag (imaginary time), with an
 // optional cache holding the stored Krylov basis between calls
 void time_evolve_kind(const HvOp &op, const void *psi, double t, int krylov_steps, void *out,

@@ -406,3 +406,40 @@ def test_solve_generic_callback_and_errors(cuda):
     assert (x - y).abs().max().item() <= 1e-12 * y.abs().max().item()
     with pytest.raises(RuntimeError):
         mb.solve(op, psi, 10, lambda al, be: np.zeros(3))
+
+
+def test_reference_evolve_state_known_answers_on_device(cuda):
+    """Ports of test_evolve_state.cpp:43-121 (2 x 2 diagonal Hamiltonians as
+    one-particle expressions, k = 2): stepped exp(-iHt) and its callback times,
+    imaginary time, against the exact exponentials (1e-8)."""
+    import torch
+    H = np.diag([1.0, 2.0])
+    op = matrix_operator(H)
+    psi0 = np.array([1.0, 1j])
+    out = mb.time_evolve_state_steps(op, dev(op, psi0), 0.0, 0.3, 0.1, mb.EvolutionOptions(2))
+    want = np.exp(-1j * 0.3 * np.diag(H)) * psi0
+    assert np.linalg.norm(host(op, out) - want) / np.linalg.norm(want) < 1e-8
+    H = np.diag([0.5, 1.5])
+    op = matrix_operator(H)
+    psi0 = np.array([0.3 - 0.4j, 0.2 + 0.1j])
+    times, norms = [], []
+
+    def cb(t, state):
+        times.append(t)
+        norms.append(torch.linalg.vector_norm(state).item())
+
+    out = mb.time_evolve_state_steps(op, dev(op, psi0), 0.0, 0.3, 0.1, mb.EvolutionOptions(2),
+                                     callback=cb)
+    assert len(times) == 4 and abs(times[0]) < 1e-12 and abs(times[-1] - 0.3) < 1e-12
+    assert max(abs(n - np.linalg.norm(psi0)) for n in norms) < 1e-12
+    want = np.exp(-1j * 0.3 * np.diag(H)) * psi0
+    assert np.linalg.norm(host(op, out) - want) / np.linalg.norm(want) < 1e-8
+    with pytest.raises(RuntimeError):  # a failing callback aborts
+        mb.time_evolve_state_steps(op, dev(op, psi0), 0.0, 0.3, 0.1, mb.EvolutionOptions(2),
+                                   callback=lambda t, s: 1 / 0)
+    H = np.diag([0.2, 1.1])
+    op = matrix_operator(H)
+    psi0 = np.array([0.4 - 0.1j, -0.2 + 0.3j])
+    out = mb.imaginary_time_evolve_state(op, dev(op, psi0), 0.7, mb.EvolutionOptions(2))
+    want = np.exp(-0.7 * np.diag(H)) * psi0
+    assert np.linalg.norm(host(op, out) - want) / np.linalg.norm(want) < 1e-8
