@@ -282,6 +282,21 @@ int mb_optical_conductivity(const double *corr, int n, double dt, double beta, d
                             double *omega, double *sigma, int capacity, int *n_freqs,
                             double *sum_rule);
 
+/* ------------------------------------------------------------ KPM projector */
+
+/* kpm::projector_moments(M, epsilon) (kpm.h:46-59): mu[M + 1] of the step
+ * theta(epsilon - x); MB_ERR_INVALID for |epsilon| > 1. */
+int mb_projector_moments(int M, double epsilon, double *mu);
+/* kpm::lorentz_kernel(N, lambda) (kpm.h:32-43): g[N + 1] */
+int mb_lorentz_kernel(int N, double lambda, double *g);
+/* SparseKPMProjector(H, M, E_min, E_max, E_fermi, padding).apply(v)
+ * (kpm.h:184-241) for any operator: out = sum_m g_m mu_m(eps) T_m(H~) v with
+ * H~ = (H - b) / a, a = (E_max - E_min)(1 + padding) / 2, b = (E_max + E_min) / 2,
+ * eps = (E_fermi - b) / a, Jackson g; M H.v on the device. */
+int mb_kpm_projector_apply(const mb_hubbard *h, int M, double e_min, double e_max,
+                           double e_fermi, double padding, const void *v_dev, void *out_dev,
+                           mb_stream stream);
+
 /* ------------------------------------------------------------ spectral bounds */
 
 /* power_method(op, v, iterations) (linear_operator_utils.h:18-50) from seed_dev

@@ -98,6 +98,10 @@ _SIGS = {
                                           C.c_int, _vp, _vp]),
     "mb_opposite_spin_correlation": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mb_exp_apply": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp]),
+    "mb_projector_moments": (C.c_int, [C.c_int, C.c_double, _dp]),
+    "mb_lorentz_kernel": (C.c_int, [C.c_int, C.c_double, _dp]),
+    "mb_kpm_projector_apply": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, _vp, _vp, _vp]),
     "mb_time_evolve_steps_cb": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_int,
                                           C.c_int, _vp, _vp, _vp, _vp]),
     "mb_lanczos_solve": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
@@ -745,6 +749,41 @@ def tridiag_eig(alphas, betas, vectors: bool = True):
                                 V.ctypes.data_as(C.c_void_p) if vectors else None))
     # V is column-major: V_cm[i + m*j] = component i of vector j
     return w[:m], (V.reshape(max(m, 1), max(m, 1)).T[:m, :m] if vectors else None)
+
+
+def projector_moments(M: int, epsilon: float):
+    """kpm::projector_moments (kpm.h:46-59): M + 1 moments of theta(epsilon - x)."""
+    mu = (C.c_double * (M + 1))()
+    _check(lib().mb_projector_moments(int(M), float(epsilon), mu))
+    return list(mu)
+
+
+def lorentz_kernel(N: int, lam: float = 4.0):
+    """kpm::lorentz_kernel (kpm.h:32-43): N + 1 damping factors."""
+    g = (C.c_double * (N + 1))()
+    _check(lib().mb_lorentz_kernel(int(N), float(lam), g))
+    return list(g)
+
+
+class SparseKPMProjector:
+    """SparseKPMProjector(H, M, E_min, E_max, E_fermi, padding) (kpm.h:184-241) on a
+    device operator: apply(v) = sum_m g_m mu_m T_m(H~) v (device order)."""
+
+    def __init__(self, op: HubbardOperator, expansion_order: int, e_min: float, e_max: float,
+                 e_fermi: float = 0.0, padding: float = 0.01):
+        self.op, self.M = op, int(expansion_order)
+        self.e_min, self.e_max, self.e_fermi, self.padding = (float(e_min), float(e_max),
+                                                              float(e_fermi), float(padding))
+
+    def dimension(self) -> int:
+        return self.op.dimension()
+
+    def apply(self, v):
+        out = self.op.empty()
+        _check(lib().mb_kpm_projector_apply(self.op._h, self.M, self.e_min, self.e_max,
+                                            self.e_fermi, self.padding, _ptr(v), _ptr(out),
+                                            _stream()))
+        return out
 
 
 def jackson_kernel(N: int):

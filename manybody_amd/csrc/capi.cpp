@@ -506,14 +506,8 @@ int mb_kpm_dos_moments(const mb_hubbard *h, double a, double b, int M, int nvec,
 int mb_jackson_kernel(int N, double *g) {
   return guarded([&] {
     need(N >= 0 && g, "invalid argument");
-    // kpm.h:19-30
-    const double pi = 3.14159265358979323846;
-    const double denom = (double)(N + 1);
-    const double cot_val = 1.0 / std::tan(pi / denom);
-    for (int n = 0; n <= N; ++n) {
-      const double nd = (double)n, arg = pi * nd / denom;
-      g[n] = ((denom - nd) * std::cos(arg) + std::sin(arg) * cot_val) / denom;
-    }
+    const std::vector<double> v = mb::jackson_kernel(N);
+    std::copy(v.begin(), v.end(), g);
   });
 }
 
@@ -1065,5 +1059,32 @@ int mb_time_evolve_steps_cb(const mb_hubbard *h, const void *psi_dev, double t0,
       };
     mb::time_evolve_steps(OP(h), psi_dev, t0, t1, dt, krylov_steps, out_dev, S(stream),
                           (flags & MB_EVOLVE_FP32) != 0, cb);
+  });
+}
+
+// ------------------------------------------------------------ KPM projector
+
+int mb_projector_moments(int M, double epsilon, double *mu) {
+  return guarded([&] {
+    need(mu != nullptr, "null argument");
+    const std::vector<double> v = mb::projector_moments(M, epsilon);
+    std::copy(v.begin(), v.end(), mu);
+  });
+}
+
+int mb_lorentz_kernel(int N, double lambda, double *g) {
+  return guarded([&] {
+    need(g != nullptr, "null argument");
+    const std::vector<double> v = mb::lorentz_kernel(N, lambda);
+    std::copy(v.begin(), v.end(), g);
+  });
+}
+
+int mb_kpm_projector_apply(const mb_hubbard *h, int M, double e_min, double e_max,
+                           double e_fermi, double padding, const void *v_dev, void *out_dev,
+                           mb_stream stream) {
+  return guarded([&] {
+    need(v_dev && out_dev && v_dev != out_dev, "null or aliased argument");
+    mb::kpm_projector_apply(OP(h), M, e_min, e_max, e_fermi, padding, v_dev, out_dev, S(stream));
   });
 }

@@ -944,6 +944,23 @@ __global__ __launch_bounds__(kBlock) void power_step_kernel(const double *__rest
   }
 }
 
+// One Chebyshev step of the KPM projector over both planes (n = 2 D doubles):
+// T_next = s (y - b T) (first step) or s (y - b T) - T_prev, with y = H T,
+// written over T_prev; result += c T_next.
+__global__ __launch_bounds__(kBlock) void cheb_accum_kernel(const double *__restrict__ y,
+                                                            const double *__restrict__ t,
+                                                            double *__restrict__ tp,
+                                                            double *__restrict__ res, int64_t n,
+                                                            double sc, double b, int first,
+                                                            double c) {
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
+    double tn = sc * (y[e] - b * t[e]);
+    if (!first) tn -= tp[e];
+    tp[e] = tn;
+    res[e] += c * tn;
+  }
+}
+
 // x = sum_{j < m} y_j V_j for Krylov vectors V_j kept in HBM (planar, stride
 // 2 D, in T) and complex coefficients y (re, im pairs on the device), summed
 // in FP64 in the order j = 0, 1, ...: one pass over the m vectors in place of
@@ -1768,6 +1785,13 @@ template void launch_recon_update<double>(double *, double *, double, double, do
                                           hipStream_t);
 template void launch_recon_update<float>(float *, double *, double, double, double, int64_t,
                                          hipStream_t);
+
+void launch_cheb_accum(const double *y, const double *t, double *tp, double *res, int64_t n,
+                       double sc, double b, bool first, double c, hipStream_t s) {
+  hipLaunchKernelGGL(cheb_accum_kernel, dim3(grid_for(n, 1 << 16)), dim3(kBlock), 0, s, y, t, tp,
+                     res, n, sc, b, first ? 1 : 0, c);
+  MB_HIP(hipGetLastError());
+}
 
 void launch_power_step(const double *v, double *w, int64_t D, double sign, double shift,
                        double *partial, hipStream_t s) {

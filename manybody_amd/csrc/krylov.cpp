@@ -726,3 +726,70 @@ double find_max_eigenpair(const HvOp &op, const void *b, int k, void *x, hipStre
 }
 
 }  // namespace mb
+
+namespace mb {
+
+// ------------------------------------------------------------ KPM projector
+
+// kpm::jackson_kernel (kpm.h:19-30): N + 1 damping factors
+std::vector<double> jackson_kernel(int N) {
+  if (N < 0) throw std::invalid_argument("N must be non-negative");
+  constexpr double pi = 3.14159265358979323846;
+  const double denom = (double)(N + 1);
+  const double cot_val = 1.0 / std::tan(pi / denom);
+  std::vector<double> g(N + 1);
+  for (int n = 0; n <= N; ++n) {
+    const double nd = (double)n, arg = pi * nd / denom;
+    g[n] = ((denom - nd) * std::cos(arg) + std::sin(arg) * cot_val) / denom;
+  }
+  return g;
+}
+
+std::vector<double> projector_moments(int M, double epsilon) {
+  if (M < 0) throw std::invalid_argument("expansion order must be non-negative");
+  if (!(epsilon >= -1.0 && epsilon <= 1.0))
+    throw std::invalid_argument("Fermi energy outside rescaled spectrum");
+  constexpr double pi = 3.14159265358979323846;
+  std::vector<double> mu(M + 1);
+  const double ac = std::acos(epsilon);
+  mu[0] = 1.0 - ac / pi;
+  for (int m = 1; m <= M; ++m) mu[m] = -2.0 * std::sin(m * ac) / (m * pi);
+  return mu;
+}
+
+std::vector<double> lorentz_kernel(int N, double lambda) {
+  if (N < 1) throw std::invalid_argument("N must be positive");
+  std::vector<double> g(N + 1);
+  const double sl = std::sinh(lambda);
+  for (int n = 0; n <= N; ++n) g[n] = std::sinh(lambda * (1.0 - (double)n / N)) / sl;
+  return g;
+}
+
+void kpm_projector_apply(const HvOp &op, int M, double e_min, double e_max, double e_fermi,
+                         double padding, const void *v, void *out, hipStream_t s) {
+  if (!(e_max > e_min)) throw std::invalid_argument("E_max must exceed E_min");
+  const double a = (e_max - e_min) / 2.0 * (1.0 + padding);
+  const double b = (e_max + e_min) / 2.0;
+  const std::vector<double> mu = projector_moments(M, (e_fermi - b) / a);
+  const std::vector<double> g = jackson_kernel(M);
+  const int64_t D = op.dim(), n = 2 * D;
+  launch_scale_complex<double>((const double *)v, g[0] * mu[0], 0.0, (double *)out, D, s);
+  if (M == 0) {
+    MB_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  DevBuf A((size_t)D * 16), B((size_t)D * 16), Y((size_t)D * 16);
+  double *tp = A.as<double>(), *tc = B.as<double>(), *y = Y.as<double>();
+  MB_HIP(hipMemcpyAsync(tp, v, (size_t)D * 16, hipMemcpyDeviceToDevice, s));
+  // T_1 = (H - b) v / a into B
+  op.hv(tp, y, Epi::Plain, EpiArgs{}, s);
+  launch_cheb_accum(y, tp, tc, (double *)out, n, 1.0 / a, b, true, g[1] * mu[1], s);
+  for (int m = 2; m <= M; ++m) {  // T_{m} = 2 H~ T_{m-1} - T_{m-2}, written over T_{m-2}
+    op.hv(tc, y, Epi::Plain, EpiArgs{}, s);
+    launch_cheb_accum(y, tc, tp, (double *)out, n, 2.0 / a, b, false, g[m] * mu[m], s);
+    std::swap(tp, tc);
+  }
+  MB_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace mb

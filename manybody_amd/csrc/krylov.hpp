@@ -116,6 +116,18 @@ void krylov_solve(const HvOp &op, const void *b, int k, const TridiagSolver &sol
                   hipStream_t s);
 // find_max_eigenpair (lanczos.h:246-281) with an explicit start vector; x may be null
 double find_max_eigenpair(const HvOp &op, const void *b, int k, void *x, hipStream_t s);
+// T_next = sc (y - b T) [- T_prev] over T_prev, result += c T_next (n doubles)
+void launch_cheb_accum(const double *y, const double *t, double *tp, double *res, int64_t n,
+                       double sc, double b, bool first, double c, hipStream_t s);
+// kpm::jackson_kernel (kpm.h:19-30), kpm::projector_moments (:46-59),
+// kpm::lorentz_kernel (:32-43)
+std::vector<double> jackson_kernel(int N);
+std::vector<double> projector_moments(int M, double epsilon);
+std::vector<double> lorentz_kernel(int N, double lambda);
+// SparseKPMProjector(H, M, E_min, E_max, E_fermi, padding).apply(v) (kpm.h:184-241):
+// sum_m g_m mu_m(eps) T_m(H~) v with H~ = (H - b) / a, Jackson g
+void kpm_projector_apply(const HvOp &op, int M, double e_min, double e_max, double e_fermi,
+                         double padding, const void *v, void *out, hipStream_t s);
 // w <- sign w + shift v; partial[3 b + {0, 1, 2}] = Re sum v w, Re sum v v (no
 // conjugation), sum |w|^2 per block b < kRedBlocks
 void launch_power_step(const double *v, double *w, int64_t D, double sign, double shift,

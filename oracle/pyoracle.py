@@ -621,3 +621,59 @@ def estimate_bounds(matvec, seed, power_iterations: int, spectral_padding: float
     beta = power_method(lambda v: matvec(v) + shift * v, seed, power_iterations) - shift
     alpha = shift - power_method(lambda v: -matvec(v) + shift * v, seed, power_iterations)
     return (alpha, beta) if alpha <= beta else (beta, alpha)
+
+
+# ------------------------------------------------------------ KPM projector
+
+def projector_moments(M: int, epsilon: float) -> np.ndarray:
+    """kpm::projector_moments (kpm.h:46-59): mu_0 = 1 - acos(eps)/pi,
+    mu_m = -2 sin(m acos eps) / (m pi)."""
+    ac = np.arccos(epsilon)
+    m = np.arange(1, M + 1)
+    return np.concatenate([[1.0 - ac / np.pi], -2.0 * np.sin(m * ac) / (m * np.pi)])
+
+
+def lorentz_kernel(N: int, lam: float = 4.0) -> np.ndarray:
+    """kpm::lorentz_kernel (kpm.h:32-43): sinh(lam (1 - n/N)) / sinh(lam)."""
+    n = np.arange(N + 1)
+    return np.sinh(lam * (1.0 - n / N)) / np.sinh(lam)
+
+
+def kpm_projector_matrix(A, M: int, e_min: float, e_max: float, e_fermi: float = 0.0,
+                         padding: float = 0.01) -> np.ndarray:
+    """The matrix of SparseKPMProjector (kpm.h:184-241) for a dense A: the
+    Chebyshev recurrence on matrices, damped by the Jackson kernel."""
+    A = np.asarray(A, np.complex128)
+    a = (e_max - e_min) / 2.0 * (1.0 + padding)
+    b = (e_max + e_min) / 2.0
+    Hs = (A - b * np.eye(len(A))) / a
+    c = jackson_kernel(M) * projector_moments(M, (e_fermi - b) / a)
+    Tp, P = np.eye(len(A), dtype=np.complex128), c[0] * np.eye(len(A), dtype=np.complex128)
+    if M == 0:
+        return P
+    Tc = Hs.copy()
+    P = P + c[1] * Tc
+    for m in range(2, M + 1):
+        Tn = 2.0 * Hs @ Tc - Tp
+        P = P + c[m] * Tn
+        Tp, Tc = Tc, Tn
+    return P
+
+
+def kpm_projector_apply(matvec, v, M: int, e_min: float, e_max: float, e_fermi: float = 0.0,
+                        padding: float = 0.01) -> np.ndarray:
+    """SparseKPMProjector::apply (kpm.h:219-233): the vector recurrence."""
+    a = (e_max - e_min) / 2.0 * (1.0 + padding)
+    b = (e_max + e_min) / 2.0
+    c = jackson_kernel(M) * projector_moments(M, (e_fermi - b) / a)
+    v = np.asarray(v, np.complex128)
+    res = c[0] * v
+    if M == 0:
+        return res
+    Tp, Tc = v, (matvec(v) - b * v) / a
+    res = res + c[1] * Tc
+    for m in range(2, M + 1):
+        Tn = 2.0 * (matvec(Tc) - b * Tc) / a - Tp
+        res = res + c[m] * Tn
+        Tp, Tc = Tc, Tn
+    return res

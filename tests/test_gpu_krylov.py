@@ -443,3 +443,30 @@ def test_reference_evolve_state_known_answers_on_device(cuda):
     out = mb.imaginary_time_evolve_state(op, dev(op, psi0), 0.7, mb.EvolutionOptions(2))
     want = np.exp(-0.7 * np.diag(H)) * psi0
     assert np.linalg.norm(host(op, out) - want) / np.linalg.norm(want) < 1e-8
+
+
+def test_kpm_projector_on_device_matches_oracle(cuda):
+    """SparseKPMProjector (kpm.h:184-241) applied on the device: ports of
+    test_kpm.cpp:74-121 through one-particle matrix operators, and a Hubbard
+    sector against the oracle's vector recurrence (1e-11 relative)."""
+    H = np.diag([-2.0, -0.5, 0.5, 2.0])
+    H[0, 1] = H[1, 0] = H[2, 3] = H[3, 2] = 0.1
+    w, V = np.linalg.eigh(H)
+    P_exact = V[:, w < 0] @ V[:, w < 0].T
+    op = matrix_operator(H)
+    proj = mb.SparseKPMProjector(op, 200, w.min(), w.max(), 0.0)
+    cols = [host(op, proj.apply(dev(op, np.eye(4)[:, j]))) for j in range(4)]
+    assert np.linalg.norm(np.stack(cols, 1) - P_exact) < 1e-2
+    A = chain(6, 0.0, -1.0)
+    op = matrix_operator(A)
+    v = np.random.default_rng(1).standard_normal(6)
+    got = host(op, mb.SparseKPMProjector(op, 100, -2.0, 2.0, 0.0).apply(dev(op, v)))
+    want = po.kpm_projector_matrix(A, 100, -2.0, 2.0, 0.0) @ v
+    assert np.linalg.norm(got - want) < 1e-10
+    # a Hubbard sector: E_F in the middle of the spectrum
+    hop = mb.HubbardModel(1.0, 4.0, 8).operator(mb.FermionBasis.sector(8, 4, 3))
+    Ac = po.build_csc([8], 4, 3, 1.0, 4.0).to_scipy()
+    x = hop.gaussian(seed=3)
+    got = host(hop, mb.SparseKPMProjector(hop, 64, -9.0, 22.0, 3.0, 0.05).apply(x))
+    want = po.kpm_projector_apply(lambda u: Ac @ u, host(hop, x), 64, -9.0, 22.0, 3.0, 0.05)
+    assert np.linalg.norm(got - want) <= 1e-11 * np.linalg.norm(want)

@@ -241,3 +241,14 @@ def test_sparse_matrix_cereal_matches_reference_golden(tmp_path):
         mb.SparseMatrix.from_entries((2, 2), [0, 0], [1, 1], [1.0, 2.0])
     with pytest.raises(RuntimeError):
         mb.SparseMatrix.load(str(tmp_path / "missing.bin"))
+
+
+def test_kpm_host_functions_match_oracle():
+    """projector_moments, lorentz_kernel and jackson_kernel of the library against
+    the oracle (closed forms), and |epsilon| > 1 rejected."""
+    for M, eps in [(0, 0.3), (10, -0.8), (200, 0.0), (64, 1.0)]:
+        assert np.allclose(mb.projector_moments(M, eps), po.projector_moments(M, eps),
+                           rtol=0, atol=1e-15)
+    assert np.allclose(mb.lorentz_kernel(50, 4.0), po.lorentz_kernel(50, 4.0), rtol=1e-14)
+    with pytest.raises(ValueError):
+        mb.projector_moments(10, 1.5)

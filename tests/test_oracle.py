@@ -570,3 +570,44 @@ def test_power_method_unconjugated_dot_converges_for_real_symmetric():
     assert abs(po.power_method(lambda v: A @ v, seed, 200) + 9.0) < 1e-8
     a, b = po.estimate_bounds(lambda v: A @ v, seed, 400, 0.1)
     assert abs(a + 9.0) < 1e-6 and abs(b - 7.0) < 1e-6
+
+
+# ------------------------------------------------------------ KPM projector
+
+def test_projector_moments_and_lorentz_known_answers():
+    """Ports of test_kpm.cpp:30-58: Lorentz kernel g_0 = 1 and 0 <= g <= 1;
+    projector moment sum rules mu_0(0) = 1/2, mu_0(-1) = 0, mu_0(1) = 1."""
+    g = po.lorentz_kernel(100)
+    assert abs(g[0] - 1.0) < 1e-10 and np.all(g >= -1e-10) and np.all(g <= 1 + 1e-10)
+    assert abs(po.projector_moments(50, 0.0)[0] - 0.5) < 1e-10
+    assert abs(po.projector_moments(50, -1.0)[0]) < 1e-10
+    assert abs(po.projector_moments(50, 1.0)[0] - 1.0) < 1e-10
+
+
+def test_projector_moments_are_chebyshev_coefficients_of_the_step():
+    """mu_m = (2 - delta_m0) / pi * int_{-1}^{eps} T_m(x) / sqrt(1 - x^2) dx, by
+    quadrature in theta = arccos x (closed form pinned independently)."""
+    from scipy.integrate import quad
+    for eps in (-0.7, 0.0, 0.35, 0.9):
+        mu = po.projector_moments(12, eps)
+        for m in range(13):
+            val = quad(lambda th: np.cos(m * th), np.arccos(eps), np.pi)[0]
+            assert abs(mu[m] - (2.0 - (m == 0)) / np.pi * val) < 1e-12
+
+
+def test_kpm_projector_known_answers():
+    """Ports of test_kpm.cpp:74-146 and :188-204 (with the dense projector's exact
+    bounds passed as E_min, E_max): order 200 reproduces the exact projector
+    (Frobenius < 1e-2), is nearly idempotent (< 0.1), counts 3 states below 0."""
+    H = np.diag([-2.0, -0.5, 0.5, 2.0])
+    H[0, 1] = H[1, 0] = H[2, 3] = H[3, 2] = 0.1
+    w, V = np.linalg.eigh(H)
+    P_exact = (V[:, w < 0] @ V[:, w < 0].T)
+    P = po.kpm_projector_matrix(H, 200, w.min(), w.max())
+    assert np.linalg.norm(P - P_exact) < 1e-2
+    H = np.diag([-1.5, -0.5, 0.5, 1.5])
+    P = po.kpm_projector_matrix(H, 200, -1.5, 1.5)
+    assert np.linalg.norm(P @ P - P) < 0.1
+    H = np.diag([-2.0, -1.0, -0.5, 0.5, 1.0, 2.0])
+    P = po.kpm_projector_matrix(H, 200, -2.0, 2.0)
+    assert abs(np.trace(P).real - 3.0) < 0.1
