@@ -1,6 +1,6 @@
 // Checker-only driver around the reference's own algebra core (compiled from the
 // reference sources where they lie; see oracle/Makefile).  It is NOT part of the
-// product and is never built or run on the GPU box: it exists to write the small
+// product; it is built only where the reference tree exists, to write the small
 // golden fixtures under tests/golden/ that pin the oracle's basis order, bond
 // lists, fermionic signs and diagonal against the reference itself.
 //
@@ -40,7 +40,7 @@
 // The reference key holds at most 7 operators and silently drops longer
 // products (src/algebra/fermion/term.h:11-19, src/algebra/expression_base.h:184-186),
 // so the caller must keep Nup+Ndn <= 7 for "basis", and Nup+Ndn <= 3 for full
-// Hamiltonians (<= 5 when U == 0).  The driver refuses larger sectors.
+// Hamiltonians (<= 5 when U == 0).  ref_driver refuses larger sectors.
 #include <complex>
 #include <cstdint>
 #include <cstdio>

@@ -8,7 +8,7 @@ reference represents exactly are used: N_up + N_dn <= 7 for the basis, <= 3 for
 the full Hamiltonian, <= 5 with U = 0 (7-operator key, fermion/term.h:11-19,
 dropped long products, expression_base.h:184-186).
 
-Run in the build container only (the reference does not exist on the GPU box):
+Run where the reference tree exists (it is not part of this repository):
     python tests/golden/make_golden.py
 Outputs: tests/golden/ref_basis.npz, tests/golden/ref_hamiltonian.npz,
 tests/golden/ref_correlation.npz (diagonal of the opposite-spin correlation
