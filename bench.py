@@ -58,7 +58,7 @@ def main():
         write_traffic(*args.traffic)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X and has no "
+        print("bench.py measures one GPU: the library runs on a single MI355X and has no "
               "multi-GPU path (--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
