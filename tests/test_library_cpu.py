@@ -252,3 +252,32 @@ def test_kpm_host_functions_match_oracle():
     assert np.allclose(mb.lorentz_kernel(50, 4.0), po.lorentz_kernel(50, 4.0), rtol=1e-14)
     with pytest.raises(ValueError):
         mb.projector_moments(10, 1.5)
+
+
+def test_every_entry_point_rejects_null_arguments():
+    """Every status-returning entry point, called with null handles and pointers
+    and zero sizes, returns a status code (no crash); all but the empty
+    tridiagonal problem report an error.  Run in a child process so that a
+    crash fails this test instead of the session."""
+    import subprocess
+    import sys
+    code = r'''
+import ctypes as C, sys
+sys.path.insert(0, %r)
+import manybody_amd as mb
+L = mb.lib()
+for name in sorted(mb._SIGS):
+    res, args = mb._SIGS[name]
+    if res is not C.c_int:
+        continue
+    vals = [a(0) if a in (C.c_int, C.c_int64, C.c_uint64, C.c_double) else None for a in args]
+    print(name, getattr(L, name)(*vals), flush=True)
+''' % (str(__import__("os").path.dirname(mb.__path__[0])),)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    status = dict(line.split() for line in r.stdout.splitlines())
+    assert len(status) > 60
+    ok_with_nothing = {"mb_tridiag_eig"}  # m = 0: nothing to do
+    for name, st in status.items():
+        assert int(st) in range(6), name
+        assert (int(st) == 0) == (name in ok_with_nothing), (name, st)
