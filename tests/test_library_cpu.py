@@ -27,6 +27,18 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) == set(mb._SIGS), "ctypes signatures must cover the header"
 
 
+def test_header_declares_every_exported_symbol():
+    """The reverse: no mb_ symbol leaves the library undeclared (nm -D)."""
+    import shutil
+    import subprocess
+    if shutil.which("nm") is None:
+        pytest.skip("nm not available")
+    out = subprocess.run(["nm", "-D", "--defined-only", mb.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if l.split()[-1].startswith("mb_")}
+    assert exported == set(header_symbols())
+
+
 def test_version():
     assert b"gfx950" in mb.lib().mb_version()
 
