@@ -108,7 +108,7 @@ ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector 
   if ((real == nullptr) == (mom == nullptr)) throw std::invalid_argument("one basis required");
   if (n_terms < 0) throw std::invalid_argument("negative term count");
   const int L = real ? real->L : mom->L;
-  const int n_up = real ? real->n_up : mom->n_up, n_dn = real ? real->n_dn : mom->n_dn;
+  const int n_dn = real ? real->n_dn : mom->n_dn;
   const uint64_t U = (uint64_t{1} << L) - 1;
 
   // the expression's own terms, merged by operator key (FermionExpression's map)
@@ -417,7 +417,6 @@ ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector 
     tb_.dn_loc = keep(dn_loc);
     tb_.dn_mom = keep(dn_mom);
   }
-  (void)n_up;
 }
 
 int64_t ExpressionOperator::hv(const double *x, double *y, Epi e, const EpiArgs &a,
