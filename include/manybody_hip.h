@@ -271,7 +271,9 @@ int mb_imaginary_time_evolve(const mb_hubbard *h, const void *psi_dev, double ta
  * seeded N(0,1) vector (seed + n, per reference index) in sector K, normalised,
  * v_b = exp(-beta H_K / 2) v, Z = <v_b|v_b>, then C(i dt) += <phi1|J^dagger phi2>/Z
  * with phi1 = exp(-i H_K t) v_b, phi2 = exp(-i H_{K+Q} t) J v_b; averaged over the
- * samples.  corr: 2 * steps doubles (re, im). */
+ * samples.  corr: 2 * steps doubles (re, im).  hk and hkq may be any operators on
+ * the current operator's source and target bases (the factorized momentum-space
+ * model, or a compiled expression on those fixed-momentum bases). */
 int mb_optical_correlator(const mb_hubbard *hk, const mb_hubbard *hkq, const mb_current *j,
                           int samples, double beta, double dt, int steps, int krylov_steps,
                           uint64_t seed, double *corr, mb_stream stream);

@@ -803,12 +803,14 @@ int mb_optical_correlator(const mb_hubbard *hk, const mb_hubbard *hkq, const mb_
                           uint64_t seed, double *corr, mb_stream stream) {
   return guarded([&] {
     need(hk && hkq && j && corr, "null argument");
-    need(hk->mom && hkq->mom, "the Hamiltonians must be momentum-sector operators");
-    need(&hk->mom->sector() == j->source->ms.get() && &hkq->mom->sector() == j->target->ms.get(),
+    // any operator on the current operator's bases: the factorized momentum-space
+    // Hubbard model or a compiled expression on the same fixed-momentum bases
+    need(hk->msec() && hkq->msec(), "the Hamiltonians must act on fixed-momentum bases");
+    need(hk->msec() == j->source->ms.get() && hkq->msec() == j->target->ms.get(),
          "the Hamiltonians must act on the current operator's source and target bases");
-    auto &m = const_cast<mb::MomentumSector &>(hk->mom->sector());
+    auto &m = const_cast<mb::MomentumSector &>(*hk->msec());
     m.ensure_device();
-    const auto C = mb::optical_correlator(*hk->mom, *hkq->mom, *j->j, m.d_ref_of_pos.as<int64_t>(),
+    const auto C = mb::optical_correlator(hk->op(), hkq->op(), *j->j, m.d_ref_of_pos.as<int64_t>(),
                                           samples, beta, dt, steps, krylov_steps, seed, S(stream));
     for (int i = 0; i < steps; ++i) {
       corr[2 * i] = C[i].real();

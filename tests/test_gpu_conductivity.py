@@ -164,3 +164,21 @@ def test_current_operator_rejects_invalid_arguments(cuda):
         mb.CurrentOperatorFixedK(real, real, [6], 1.0, 0, 0)
     with pytest.raises(ValueError):  # lattice does not match the bases
         mb.CurrentOperatorFixedK(bk, bq, [3, 2], 1.0, 2, 0)
+
+
+def test_correlator_with_expression_hamiltonians(cuda):
+    """The correlator accepts any operator on the current operator's bases: the
+    momentum-space Hubbard model compiled as an expression gives the factorized
+    operator's correlator (1e-10 of C(0))."""
+    sizes, nu, nd, K, Q = [6], 3, 2, 1, 1
+    bk, hk = sector(sizes, nu, nd, K, 1.0, 4.0)
+    bq, hq = sector(sizes, nu, nd, po.momentum_add(sizes, K, Q), 1.0, 4.0)
+    J = mb.CurrentOperatorFixedK(bk, bq, sizes, 1.0, Q, 0)
+    e = mb.HubbardModelMomentum(1.0, 4.0, sizes).hamiltonian()
+    ek = mb.FermionExpressionOperator(bk, e)
+    eq = mb.FermionExpressionOperator(bq, e)
+    C0 = mb.optical_correlator(hk, hq, J, 2, 1.5, 0.1, 8, 11)
+    C1 = mb.optical_correlator(ek, eq, J, 2, 1.5, 0.1, 8, 11)
+    assert np.abs(C1 - C0).max() <= 1e-10 * abs(C0[0])
+    with pytest.raises(ValueError):  # operators on other bases
+        mb.optical_correlator(hq, hk, J, 1, 1.0, 0.1, 2, 1)
