@@ -128,6 +128,12 @@ class FermionBasis {
     check(mb_basis_create((int)orbitals, (int)particles, spin_projection, &b));
     return FermionBasis(b, orbitals, particles);
   }
+  // Basis::with_fixed_particle_number (basis.h:24): every spin split of N particles
+  static FermionBasis with_fixed_particle_number(size_t orbitals, size_t particles) {
+    mb_basis *b = nullptr;
+    check(mb_fixed_n_basis_create((int)orbitals, (int)particles, &b));
+    return FermionBasis(b, orbitals, particles);
+  }
   // Basis::with_fixed_particle_number_spin_momentum (basis.h:27-29): total momentum
   // `momentum` on the periodic lattice `lattice_size` (the reference's Index)
   static FermionBasis with_fixed_particle_number_spin_momentum(

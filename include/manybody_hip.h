@@ -334,6 +334,17 @@ int mb_lanczos_solve(const mb_hubbard *h, const void *b_dev, int k, mb_tridiag_s
 int mb_find_max_eigenpair(const mb_hubbard *h, const void *b_dev, int k, double *value,
                           void *x_dev, mb_stream stream);
 
+/* ------------------------------------------------------------ fixed-N bases */
+
+/* Basis::with_fixed_particle_number(orbitals, particles) (basis.cpp:70-89,
+ * 240-265): every state with that many particles, any N_up, indexed in the
+ * reference's depth-first order (orbitals <= 32, C(2 orbitals, particles) <
+ * 2^31).  The handle works with mb_basis_size/state/index_of, with
+ * mb_expression_create (terms may move particles between spins: spin flips)
+ * and with mb_hubbard_create / mb_hubbard_create_bonds (the Hubbard model is
+ * compiled as an expression on it; at most 24 orbitals for operators). */
+int mb_fixed_n_basis_create(int orbitals, int particles, mb_basis **out);
+
 /* ------------------------------------------------------------ generic expressions */
 
 /* compute_matrix_elements(basis, A) (matrix_elements.h:74-105) for any

@@ -109,6 +109,7 @@ _SIGS = {
     "mb_power_method": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_double), _vp]),
     "mb_estimate_bounds": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), _vp]),
+    "mb_fixed_n_basis_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_vp)]),
     "mb_expression_create": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, C.POINTER(_vp)]),
     "mb_compute_matrix_elements": (C.c_int, [_vp, C.POINTER(_vp), _vp]),
     "mb_sparse_create": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, _vp, _vp, _vp,
@@ -192,7 +193,7 @@ class FermionBasis:
         self.orbitals = orbitals
         self.n_up = n_up
         self.n_dn = n_dn
-        self.particles = n_up + n_dn
+        self.particles = (n_up + n_dn) if n_up is not None else None
 
     @staticmethod
     def with_fixed_particle_number_and_spin(orbitals: int, particles: int,
@@ -201,6 +202,16 @@ class FermionBasis:
         _check(lib().mb_basis_create(orbitals, particles, spin_projection, C.byref(h)))
         up = (particles + spin_projection) // 2
         return FermionBasis(h, orbitals, up, particles - up)
+
+    @staticmethod
+    def with_fixed_particle_number(orbitals: int, particles: int) -> "FermionBasis":
+        """Basis::with_fixed_particle_number (basis.cpp:70-89, 240-265): every state
+        of `particles` particles, any N_up, in the reference's order."""
+        h = C.c_void_p()
+        _check(lib().mb_fixed_n_basis_create(int(orbitals), int(particles), C.byref(h)))
+        b = FermionBasis(h, orbitals, None, None)
+        b.particles = int(particles)
+        return b
 
     @staticmethod
     def sector(orbitals: int, n_up: int, n_dn: int) -> "FermionBasis":

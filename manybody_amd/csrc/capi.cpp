@@ -13,6 +13,7 @@
 #include "momentum.hpp"
 #include "conductivity.hpp"
 #include "expression.hpp"
+#include "fixed_n.hpp"
 
 #include <cstdio>
 #include <map>
@@ -21,6 +22,9 @@
 struct mb_basis {
   mb::Sector sec;
   std::unique_ptr<mb::MomentumSector> ms;  // set for a fixed-momentum basis
+  std::unique_ptr<mb::FixedNBasis> fn;     // set for a fixed-N (any N_up) basis
+  int orbitals() const { return ms ? ms->L : fn ? fn->L : sec.L; }
+  int64_t size() const { return ms ? ms->D : fn ? fn->D : sec.dim(); }
 };
 struct mb_hubbard {
   mb::Hubbard h;  // real-space model (unused for a momentum-sector operator)
@@ -39,9 +43,26 @@ struct mb_hubbard {
     if (expr && basis->ms) return basis->ms.get();
     return nullptr;
   }
-  // the real-space sector the operator acts on, or null (momentum)
+  // the fixed-N basis the operator acts on, or null
+  const mb::FixedNBasis *fsec() const { return (expr && basis->fn) ? basis->fn.get() : nullptr; }
+  // the device position -> reference index map of a grouped basis (fixed K or
+  // fixed N), uploaded on first use; null for a real-space sector
+  const int64_t *ref_map() const {
+    if (auto *m = msec()) {
+      auto &mm = const_cast<mb::MomentumSector &>(*m);
+      mm.ensure_device();
+      return mm.d_ref_of_pos.as<int64_t>();
+    }
+    if (auto *f = fsec()) {
+      auto &ff = const_cast<mb::FixedNBasis &>(*f);
+      ff.ensure_device();
+      return ff.d_ref_of_pos.as<int64_t>();
+    }
+    return nullptr;
+  }
+  // the real-space sector the operator acts on, or null (momentum, fixed N)
   const mb::Sector *rsec() const {
-    if (mom || (expr && basis->ms)) return nullptr;
+    if (mom || (expr && (basis->ms || basis->fn))) return nullptr;
     if (expr) {
       auto &sec = const_cast<mb::Sector &>(basis->sec);
       sec.ensure_device();
@@ -127,6 +148,7 @@ const mb::HvOp &OP(const mb_hubbard *h) {
 const mb::Sector &SEC(const mb_basis *b) {
   need(b != nullptr, "null basis handle");
   need(!b->ms, "not available for momentum-sector bases");
+  need(!b->fn, "not available for fixed-N bases");
   return b->sec;
 }
 
@@ -155,16 +177,18 @@ int mb_basis_create(int orbitals, int particles, int spin_projection, mb_basis *
 
 void mb_basis_destroy(mb_basis *b) { delete b; }
 
-int64_t mb_basis_size(const mb_basis *b) { return b ? (b->ms ? b->ms->D : b->sec.dim()) : -1; }
+int64_t mb_basis_size(const mb_basis *b) { return b ? b->size() : -1; }
 
 int mb_basis_state(const mb_basis *b, int64_t index, uint64_t *up_bits, uint64_t *dn_bits) {
   return guarded([&] {
     need(b && up_bits && dn_bits, "null argument");
-    const int64_t D = b->ms ? b->ms->D : b->sec.dim();
+    const int64_t D = b->size();
     if (index < 0 || index >= D) throw mb::RangeError("basis index out of range");
     uint32_t u, d;
     if (b->ms)
       b->ms->ref_state(index, &u, &d);
+    else if (b->fn)
+      b->fn->ref_state(index, &u, &d);
     else
       b->sec.ref_state(index, &u, &d);
     *up_bits = u;
@@ -175,6 +199,15 @@ int mb_basis_state(const mb_basis *b, int64_t index, uint64_t *up_bits, uint64_t
 int mb_basis_index_of(const mb_basis *b, uint64_t up_bits, uint64_t dn_bits, int64_t *index) {
   return guarded([&] {
     need(b && index, "null argument");
+    if (b->fn) {
+      const uint64_t mask = (1ull << b->fn->L) - 1;
+      const int64_t i = ((up_bits & ~mask) || (dn_bits & ~mask))
+                            ? -1
+                            : b->fn->ref_index((uint32_t)up_bits, (uint32_t)dn_bits);
+      if (i < 0) throw mb::RangeError("state not in basis");
+      *index = i;
+      return;
+    }
     if (b->ms) {
       const auto &m = *b->ms;
       const uint64_t mask = (m.L == 64) ? ~0ull : ((1ull << m.L) - 1);
@@ -207,11 +240,49 @@ int mb_basis_product_index(const mb_basis *b, int64_t first, int64_t n, int64_t 
   });
 }
 
+// HubbardModel*::hamiltonian() (hubbard_model.h:13-35) on a fixed-N basis: the
+// bond list's hoppings (up then down per bond) and the on-site U terms as an
+// expression, compiled like any other
+static void hubbard_fixed_n(const mb_basis *b, const std::vector<int> &from,
+                            const std::vector<int> &to, double t, double u, mb_hubbard *h) {
+  std::vector<double> coeffs;
+  std::vector<int64_t> offsets{0};
+  std::vector<uint16_t> ops;
+  auto term = [&](double c, std::initializer_list<uint16_t> o) {
+    coeffs.push_back(c);
+    coeffs.push_back(0.0);
+    ops.insert(ops.end(), o);
+    offsets.push_back((int64_t)ops.size());
+  };
+  for (size_t k = 0; k < from.size(); ++k)
+    for (uint16_t spin : {uint16_t{0}, mb::kOpDown}) {
+      const uint16_t i = (uint16_t)from[k], j = (uint16_t)to[k];
+      term(-t, {(uint16_t)(spin | i), (uint16_t)(mb::kOpAnnihilation | spin | j)});
+      term(-t, {(uint16_t)(spin | j), (uint16_t)(mb::kOpAnnihilation | spin | i)});
+    }
+  for (int i = 0; i < b->fn->L; ++i)
+    term(u, {(uint16_t)i, (uint16_t)(mb::kOpAnnihilation | i), (uint16_t)(mb::kOpDown | i),
+             (uint16_t)(mb::kOpAnnihilation | mb::kOpDown | i)});
+  h->expr = std::make_unique<mb::ExpressionOperator>(nullptr, nullptr, (int64_t)(offsets.size() - 1),
+                                                     coeffs.data(), offsets.data(), ops.data(),
+                                                     b->fn.get());
+}
+
 static int hubbard_common(const mb_basis *b, std::vector<int> from, std::vector<int> to, double t,
                           double u, mb_hubbard **out, int ndim = 0, const int *sizes = nullptr) {
   return guarded([&] {
     need(b && out, "null argument");
     *out = nullptr;
+    if (b->fn) {
+      for (size_t k = 0; k < from.size(); ++k)
+        need(from[k] >= 0 && from[k] < b->fn->L && to[k] >= 0 && to[k] < b->fn->L,
+             "bond endpoint outside the lattice");
+      auto h = std::make_unique<mb_hubbard>();
+      h->basis = b;
+      hubbard_fixed_n(b, from, to, t, u, h.get());
+      *out = h.release();
+      return;
+    }
     SEC(b);
     for (size_t k = 0; k < from.size(); ++k)
       need(from[k] >= 0 && from[k] < b->sec.L && to[k] >= 0 && to[k] < b->sec.L,
@@ -237,14 +308,14 @@ int mb_hubbard_create(const mb_basis *b, int ndim, const int *sizes, double t, d
   std::vector<int> from, to;
   int st = guarded([&] {
     need(b && sizes, "null argument");
-    SEC(b);
+    need(!b->ms, "use the momentum-space model on a fixed-momentum basis");
     need(ndim >= 1 && ndim <= 3, "ndim must be 1, 2 or 3");
     int prod = 1;
     for (int i = 0; i < ndim; ++i) {
       need(sizes[i] >= 1, "Dimension with size zero is invalid.");
       prod *= sizes[i];
     }
-    need(prod == b->sec.L, "lattice size does not match the basis' orbital count");
+    need(prod == b->orbitals(), "lattice size does not match the basis' orbital count");
     mb::lattice_bonds(ndim, sizes, from, to);
   });
   if (st != MB_OK) return st;
@@ -268,7 +339,7 @@ int64_t mb_hubbard_dimension(const mb_hubbard *h) { return h ? h->op().dim() : -
 int mb_hubbard_shape(const mb_hubbard *h, int64_t *d_up, int64_t *d_dn) {
   return guarded([&] {
     need(h && d_up && d_dn, "null argument");
-    if (h->msec()) {  // a momentum sector is not one product block
+    if (h->msec() || h->fsec()) {  // a grouped basis is not one product block
       *d_up = h->op().dim();
       *d_dn = 1;
       return;
@@ -303,20 +374,13 @@ int mb_hv_c64_multi(const mb_hubbard *h, int nvec, const void *x_dev, void *y_de
   });
 }
 
-namespace {
-const int64_t *mom_map(const mb_hubbard *h) {
-  auto &m = const_cast<mb::MomentumSector &>(*h->msec());
-  m.ensure_device();
-  return m.d_ref_of_pos.as<int64_t>();
-}
-}  // namespace
 
 int mb_to_device_order_c128(const mb_hubbard *h, const void *x_ref, void *x_dev, mb_stream stream) {
   return guarded([&] {
     need(x_ref && x_dev && x_ref != x_dev, "in and out must be distinct device pointers");
     need(h != nullptr, "null model handle");
-    if (h->msec())
-      mb::launch_mom_perm(mom_map(h), h->op().dim(), (const double *)x_ref, (double *)x_dev, true,
+    if (const int64_t *map = h->ref_map())
+      mb::launch_mom_perm(map, h->op().dim(), (const double *)x_ref, (double *)x_dev, true,
                           S(stream));
     else
       mb::launch_gather_c128(mb::perm_of(*h->rsec()), x_ref, x_dev, S(stream));
@@ -328,8 +392,8 @@ int mb_to_reference_order_c128(const mb_hubbard *h, const void *x_dev, void *x_r
   return guarded([&] {
     need(x_ref && x_dev && x_ref != x_dev, "in and out must be distinct device pointers");
     need(h != nullptr, "null model handle");
-    if (h->msec())
-      mb::launch_mom_perm(mom_map(h), h->op().dim(), (const double *)x_dev, (double *)x_ref, false,
+    if (const int64_t *map = h->ref_map())
+      mb::launch_mom_perm(map, h->op().dim(), (const double *)x_dev, (double *)x_ref, false,
                           S(stream));
     else
       mb::launch_scatter_c128(mb::perm_of(*h->rsec()), x_dev, x_ref, S(stream));
@@ -341,12 +405,12 @@ int mb_apply_ref_c128(const mb_hubbard *h, const void *x_ref, void *y_ref, void 
   return guarded([&] {
     need(x_ref && y_ref && work, "null argument");
     need(h != nullptr, "null model handle");
-    if (h->msec()) {
+    if (const int64_t *map = h->ref_map()) {
       const int64_t D = h->op().dim();
       double *w0 = (double *)work, *w1 = w0 + 2 * D;
-      mb::launch_mom_perm(mom_map(h), D, (const double *)x_ref, w0, true, S(stream));
+      mb::launch_mom_perm(map, D, (const double *)x_ref, w0, true, S(stream));
       h->op().hv(w0, w1, mb::Epi::Plain, mb::EpiArgs{}, S(stream));
-      mb::launch_mom_perm(mom_map(h), D, w1, (double *)y_ref, false, S(stream));
+      mb::launch_mom_perm(map, D, w1, (double *)y_ref, false, S(stream));
       return;
     }
     if (h->expr) {
@@ -370,8 +434,8 @@ int mb_fill_gaussian_c128(const mb_hubbard *h, uint64_t seed, void *x_dev, mb_st
   return guarded([&] {
     need(x_dev, "null argument");
     need(h != nullptr, "null model handle");
-    if (h->msec())
-      mb::launch_mom_fill_gaussian(mom_map(h), h->op().dim(), seed, (double *)x_dev, S(stream));
+    if (const int64_t *map = h->ref_map())
+      mb::launch_mom_fill_gaussian(map, h->op().dim(), seed, (double *)x_dev, S(stream));
     else
       mb::launch_fill_gaussian_c128(mb::perm_of(*h->rsec()), seed, x_dev, S(stream));
   });
@@ -856,6 +920,10 @@ int mb_expression_create(const mb_basis *b, int64_t n_terms, const double *coeff
       auto &m = const_cast<mb::MomentumSector &>(*b->ms);
       h->expr = std::make_unique<mb::ExpressionOperator>(nullptr, &m, n_terms, coeffs,
                                                          n_terms ? offsets : &zero, ops);
+    } else if (b->fn) {
+      h->expr = std::make_unique<mb::ExpressionOperator>(nullptr, nullptr, n_terms, coeffs,
+                                                         n_terms ? offsets : &zero, ops,
+                                                         b->fn.get());
     } else {
       h->expr = std::make_unique<mb::ExpressionOperator>(&b->sec, nullptr, n_terms, coeffs,
                                                          n_terms ? offsets : &zero, ops);
@@ -1088,5 +1156,18 @@ int mb_kpm_projector_apply(const mb_hubbard *h, int M, double e_min, double e_ma
   return guarded([&] {
     need(v_dev && out_dev && v_dev != out_dev, "null or aliased argument");
     mb::kpm_projector_apply(OP(h), M, e_min, e_max, e_fermi, padding, v_dev, out_dev, S(stream));
+  });
+}
+
+// ------------------------------------------------------------ fixed-N bases
+
+int mb_fixed_n_basis_create(int orbitals, int particles, mb_basis **out) {
+  return guarded([&] {
+    need(out != nullptr, "null argument");
+    *out = nullptr;
+    auto b = std::make_unique<mb_basis>();
+    b->fn = std::make_unique<mb::FixedNBasis>();
+    mb::build_fixed_n_basis(*b->fn, orbitals, particles);
+    *out = b.release();
   });
 }

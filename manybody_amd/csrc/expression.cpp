@@ -26,6 +26,7 @@
 #include "expression.hpp"
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <map>
 #include <stdexcept>
@@ -101,15 +102,57 @@ const typename V::value_type *ExpressionOperator::keep(const V &v) {
   return bufs_.back().template as<typename V::value_type>();
 }
 
+namespace {
+
+// A layout in groups: rows are the up strings of group p (momentum, or particle
+// number for a fixed-N basis), their columns the down strings of group
+// col_group(p); the element offset of row r of group p is blk_off[p] + r ncols[p].
+struct Grouped {
+  int ngroups = 0;
+  const std::vector<uint32_t> *up_str = nullptr, *dn_str = nullptr;
+  const std::vector<int32_t> *up_off = nullptr, *dn_off = nullptr;
+  const std::vector<int64_t> *blk_off = nullptr;
+  const std::vector<int32_t> *ncols = nullptr;
+  std::function<int(uint32_t)> group_of;
+  std::function<int(int)> col_group;  // -1: no columns
+};
+
+}  // namespace
+
 ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector *mom,
                                        int64_t n_terms, const double *coeffs,
-                                       const int64_t *offsets, const uint16_t *ops)
-    : real_(real), mom_(mom) {
-  if ((real == nullptr) == (mom == nullptr)) throw std::invalid_argument("one basis required");
+                                       const int64_t *offsets, const uint16_t *ops,
+                                       const FixedNBasis *fn)
+    : real_(real), mom_(mom), fn_(fn) {
+  if ((real != nullptr) + (mom != nullptr) + (fn != nullptr) != 1)
+    throw std::invalid_argument("one basis required");
   if (n_terms < 0) throw std::invalid_argument("negative term count");
-  const int L = real ? real->L : mom->L;
-  const int n_dn = real ? real->n_dn : mom->n_dn;
+  const int L = real ? real->L : mom ? mom->L : fn->L;
+  const int n_dn = real ? real->n_dn : mom ? mom->n_dn : 0;
   const uint64_t U = (uint64_t{1} << L) - 1;
+  Grouped G;
+  if (mom) {
+    G.ngroups = mom->L;
+    G.up_str = &mom->up_str;
+    G.dn_str = &mom->dn_str;
+    G.up_off = &mom->up_off;
+    G.dn_off = &mom->dn_off;
+    G.blk_off = &mom->blk_off;
+    G.ncols = &mom->ncols;
+    G.group_of = [mom](uint32_t b) { return mom->momentum_of(b); };
+    G.col_group = [mom](int p) { return mom->msub(mom->K, p); };
+  } else if (fn) {
+    G.ngroups = fn->L + 1;
+    G.up_str = &fn->up_str;
+    G.dn_str = &fn->dn_str;
+    G.up_off = &fn->up_off;
+    G.dn_off = &fn->dn_off;
+    G.blk_off = &fn->blk_off;
+    G.ncols = &fn->ncols;
+    G.group_of = [](uint32_t b) { return __builtin_popcount(b); };
+    const int N = fn->N, Lf = fn->L;
+    G.col_group = [N, Lf](int k) { return (N - k >= 0 && N - k <= Lf) ? N - k : -1; };
+  }
 
   // the expression's own terms, merged by operator key (FermionExpression's map)
   std::map<std::vector<uint16_t>, std::complex<double>> expr;
@@ -125,9 +168,12 @@ ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector 
     MonomialAction a;
     if (!compile_monomial(key.data(), (int)key.size(), L, c, a)) continue;
     const uint64_t D_ = ~U;
-    if (__builtin_popcountll(a.in & U) != __builtin_popcountll(a.out & U) ||
-        __builtin_popcountll(a.in & D_) != __builtin_popcountll(a.out & D_))
+    if (fn) {  // a fixed-N basis holds every N_up: only N must be kept
+      if (__builtin_popcountll(a.in) != __builtin_popcountll(a.out)) continue;
+    } else if (__builtin_popcountll(a.in & U) != __builtin_popcountll(a.out & U) ||
+               __builtin_popcountll(a.in & D_) != __builtin_popcountll(a.out & D_)) {
       continue;  // changes N_up or N_dn: no output in the basis
+    }
     ++kept_;
     const bool diag = a.in == a.out;
     if ((a.touched & ~U) == 0)
@@ -153,32 +199,34 @@ ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector 
       row_dg0.push_back(0);
     }
   } else {
-    rows_bits = mom->up_str;
-    dn_bits = mom->dn_str;
-    D = mom->D;
-    for (int p = 0; p < mom->L; ++p)
-      for (int g = mom->up_off[p]; g < mom->up_off[p + 1]; ++g) {
-        const int nc = mom->ncols[p];
-        row_base.push_back(mom->blk_off[p] + (int64_t)(g - mom->up_off[p]) * nc);
+    rows_bits = *G.up_str;
+    dn_bits = *G.dn_str;
+    D = mom ? mom->D : fn->D;
+    for (int p = 0; p < G.ngroups; ++p)
+      for (int g = (*G.up_off)[p]; g < (*G.up_off)[p + 1]; ++g) {
+        const int nc = (*G.ncols)[p];
+        const int cg = G.col_group(p);
+        row_base.push_back((*G.blk_off)[p] + (int64_t)(g - (*G.up_off)[p]) * nc);
         row_nc.push_back(nc);
-        row_dg0.push_back(mom->dn_off[mom->msub(mom->K, p)]);
+        row_dg0.push_back(cg >= 0 ? (*G.dn_off)[cg] : 0);
       }
   }
   const int64_t nrows = (int64_t)rows_bits.size(), ndn = (int64_t)dn_bits.size();
 
   // string -> (row base | column) lookups of the source side
-  std::vector<int32_t> up_loc, dn_loc, up_row;
-  std::vector<uint8_t> up_mom, dn_mom;
-  if (mom) {
-    if (L > 24) throw std::invalid_argument("momentum-sector expressions support at most 24 orbitals");
+  // (group id 0xFF: a string outside the down table, which matches no group)
+  std::vector<int32_t> dn_loc, up_row;
+  std::vector<uint8_t> dn_mom;
+  if (!real) {
+    if (L > 24) throw std::invalid_argument("grouped-basis expressions support at most 24 orbitals");
     const size_t n = size_t{1} << L;
     dn_loc.assign(n, -1);
-    dn_mom.assign(n, 0);
+    dn_mom.assign(n, 0xFF);
     up_row.assign(n, -1);
-    for (int p = 0; p < L; ++p) {
-      for (int h = mom->dn_off[p]; h < mom->dn_off[p + 1]; ++h) {
-        dn_loc[mom->dn_str[h]] = h - mom->dn_off[p];
-        dn_mom[mom->dn_str[h]] = (uint8_t)p;
+    for (int p = 0; p < G.ngroups; ++p) {
+      for (int h = (*G.dn_off)[p]; h < (*G.dn_off)[p + 1]; ++h) {
+        dn_loc[(*G.dn_str)[h]] = h - (*G.dn_off)[p];
+        dn_mom[(*G.dn_str)[h]] = (uint8_t)p;
       }
     }
     for (int64_t g = 0; g < nrows; ++g) up_row[rows_bits[g]] = (int32_t)g;
@@ -190,12 +238,12 @@ ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector 
     }
     const int32_t g2 = up_row[u2];
     if (g2 < 0) return -1;
-    *grp = mom->msub(mom->K, mom->momentum_of(u2));  // momentum of the columns
-    return row_nc[g2] > 0 ? g2 : -1;
+    *grp = G.col_group(G.group_of(u2));  // group of the row's columns
+    return (row_nc[g2] > 0 && *grp >= 0) ? g2 : -1;
   };
   auto source_col = [&](uint32_t d2, int32_t grp) -> int64_t {
     if (real) return Sector::colex_rank(d2);
-    return mom->momentum_of(d2) == grp ? dn_loc[d2] : -1;
+    return (dn_mom[d2] == grp && dn_loc[d2] >= 0) ? dn_loc[d2] : -1;
   };
 
   // diagonal parts of one species
@@ -229,7 +277,7 @@ ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector 
       int32_t grp;
       const int64_t r2 = source_row(u2, &grp);
       if (r2 < 0) continue;
-      if (mom && grp != mom->msub(mom->K, mom->momentum_of((uint32_t)u))) continue;
+      if (!real && grp != G.col_group(G.group_of((uint32_t)u))) continue;
       acc[row_base[r2]] += parity64(u & a.parity) ? -a.c : a.c;
     }
     for (const auto &[b, c] : acc) {
@@ -247,7 +295,7 @@ ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector 
   for (int64_t h = 0; h < ndn; ++h) {
     std::map<int64_t, std::complex<double>> acc;
     const uint64_t w = (uint64_t)dn_bits[h] << L;
-    const int32_t grp = mom ? mom->momentum_of(dn_bits[h]) : -1;
+    const int32_t grp = real ? -1 : G.group_of(dn_bits[h]);
     for (const auto &a : dn_off) {
       if ((w & a.touched) != a.out) continue;
       const uint32_t d2 = (uint32_t)((w ^ (a.in ^ a.out)) >> L);
@@ -328,7 +376,7 @@ ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector 
   tb_.nwg = (int64_t)wg_row.size();
   tb_.L = L;
   tb_.n_dn = n_dn;
-  tb_.momentum = mom ? 1 : 0;
+  tb_.momentum = real ? 0 : 1;  // grouped layout: source columns through the tables
   for (int64_t g = 0; g < nrows; ++g)
     tb_.max_uo = std::max<int>(tb_.max_uo, (int)(uo_ptr[g + 1] - uo_ptr[g]));
   if (real) {
@@ -437,10 +485,15 @@ void ExpressionOperator::export_coo(std::vector<uint64_t> &rows, std::vector<uin
     auto &sec = const_cast<Sector &>(*real_);
     sec.ensure_device();
     launch_ref_of_pos(perm_of(sec), refmap.as<int64_t>(), s);
-  } else {
+  } else if (mom_) {
     auto &m = const_cast<MomentumSector &>(*mom_);
     m.ensure_device();
     MB_HIP(hipMemcpyAsync(refmap.p, m.d_ref_of_pos.p, (size_t)D * sizeof(int64_t),
+                          hipMemcpyDeviceToDevice, s));
+  } else {
+    auto &f = const_cast<FixedNBasis &>(*fn_);
+    f.ensure_device();
+    MB_HIP(hipMemcpyAsync(refmap.p, f.d_ref_of_pos.p, (size_t)D * sizeof(int64_t),
                           hipMemcpyDeviceToDevice, s));
   }
   DevBuf keys, vbuf;

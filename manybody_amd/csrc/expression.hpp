@@ -5,6 +5,7 @@
 #include <complex>
 #include <vector>
 
+#include "fixed_n.hpp"
 #include "krylov.hpp"
 #include "mb_internal.hpp"
 #include "momentum.hpp"
@@ -34,7 +35,7 @@ bool compile_monomial(const uint16_t *ops, int n, int L, std::complex<double> c,
 // columns); row = up string, its columns = the down strings of its group.
 struct ExprTables {
   int64_t D = 0, nrows = 0, nwg = 0;
-  int L = 0, n_dn = 0, momentum = 0;
+  int L = 0, n_dn = 0, momentum = 0;  // momentum: a grouped layout (fixed K or fixed N)
   int max_uo = 0;                                     // most up-only sources of a row
   int64_t d_up = 0, d_dn = 0;                         // real space: the product shape
   const int32_t *wg_row = nullptr, *wg_c0 = nullptr;  // per workgroup
@@ -68,8 +69,9 @@ struct ExprTables {
                  *dp_parity = nullptr;
   const double *dp_c = nullptr;
   const uint64_t *binom = nullptr;                   // real space: colex ranks of down strings
-  const int32_t *dn_loc = nullptr;                   // momentum: index in its group, by bits
-  const uint8_t *dn_mom = nullptr;                   // momentum: momentum, by bits
+  const int32_t *dn_loc = nullptr;                   // grouped: index in its group, by bits
+  const uint8_t *dn_mom = nullptr;                   // grouped: group (momentum or particle
+                                                     // number) by bits, 0xFF outside
 };
 
 constexpr int kExprChunk = 256;
@@ -79,9 +81,10 @@ constexpr int kExprChunk = 256;
 class ExpressionOperator final : public HvOp {
  public:
   // terms: n_terms coefficients (re, im interleaved), offsets[n_terms + 1] into
-  // ops (reference operator storage); exactly one of real / mom is set
+  // ops (reference operator storage); exactly one of real / mom / fn is set
   ExpressionOperator(const Sector *real, const MomentumSector *mom, int64_t n_terms,
-                     const double *coeffs, const int64_t *offsets, const uint16_t *ops);
+                     const double *coeffs, const int64_t *offsets, const uint16_t *ops,
+                     const FixedNBasis *fn = nullptr);
   int64_t dim() const override { return tb_.D; }
   int64_t partials() const override { return 2 * tb_.nwg + 16; }
   int64_t hv(const double *x, double *y, Epi e, const EpiArgs &a, hipStream_t s) const override;
@@ -96,6 +99,7 @@ class ExpressionOperator final : public HvOp {
  private:
   const Sector *real_;
   const MomentumSector *mom_;
+  const FixedNBasis *fn_;
   ExprTables tb_{};
   int64_t kept_ = 0;
   std::vector<DevBuf> bufs_;

@@ -677,3 +677,29 @@ def kpm_projector_apply(matvec, v, M: int, e_min: float, e_max: float, e_fermi: 
         res = res + c[m] * Tn
         Tp, Tc = Tc, Tn
     return res
+
+
+# ------------------------------------------------------------ fixed-N basis
+
+def fixed_n_states(n_sites: int, n_particles: int):
+    """Basis::with_fixed_particle_number (basis.cpp:70-89, 240-265): depth-first
+    over modes (spin, orbital); at each level the up candidates from the current
+    orbital on, then the down ones, a mode allowed when its orbital is above the
+    last one's or equal with a higher spin; a leaf at N modes.  Returns
+    (up_bits, dn_bits) lists in index order."""
+    ups, dns = [], []
+
+    def rec(cur, first):
+        if len(cur) == n_particles:
+            ups.append(sum(1 << o for s, o in cur if s == 0))
+            dns.append(sum(1 << o for s, o in cur if s == 1))
+            return
+        for s in (0, 1):
+            for i in range(first, n_sites):
+                if not cur or cur[-1][1] < i or (cur[-1][1] == i and s > cur[-1][0]):
+                    cur.append((s, i))
+                    rec(cur, i)
+                    cur.pop()
+
+    rec([], 0)
+    return ups, dns

@@ -20,6 +20,8 @@
 //   ref_driver expr   L Nup Ndn < terms             -> states, then the nonzeros of the
 //                                                      FermionExpression read from stdin
 //   ref_driver mexpr  ndim L.. Nup Ndn K.. < terms  -> the same on a fixed-momentum basis
+//   ref_driver nbasis L N                         -> states of with_fixed_particle_number
+//   ref_driver nexpr  L N < terms                 -> the expression on that basis
 //        terms: one per line, "re im n  type spin orbital  ..." (n operators,
 //        type 0 creation / 1 annihilation, spin 0 up / 1 down), added in order
 //   ref_driver cereal path < "rows cols" then "row col re im" lines
@@ -28,6 +30,7 @@
 //
 // Reference entry points exercised:
 //   FermionBasis::with_fixed_particle_number_and_spin   src/algebra/basis.cpp:91-126
+//   FermionBasis::with_fixed_particle_number            src/algebra/basis.cpp:70-89, 240-265
 //   HubbardModel / HubbardModel2D / HubbardModel3D      src/algebra/fermion/model/hubbard_model.h:10-212
 //   compute_matrix_elements<MatrixType>                  src/algebra/matrix_elements.h:74-105
 //   FermionBasis::with_fixed_particle_number_spin_momentum src/algebra/basis.cpp:128-185, 304-356
@@ -152,6 +155,10 @@ void save(Archive& ar, const TripletMatrix& t, unsigned int /* version */) {
 
 int expr_mode(const std::string& mode, const std::vector<double>& a) {
   FermionBasis basis = [&] {
+    if (mode == "nexpr" || mode == "nbasis") {
+      if (a.size() != 2) throw std::invalid_argument("nexpr L N");
+      return FermionBasis::with_fixed_particle_number((size_t)a[0], (size_t)a[1]);
+    }
     if (mode == "expr") {
       if (a.size() != 3) throw std::invalid_argument("expr L Nup Ndn");
       const size_t n = (size_t)a[1] + (size_t)a[2];
@@ -171,6 +178,10 @@ int expr_mode(const std::string& mode, const std::vector<double>& a) {
     return FermionBasis::with_fixed_particle_number_spin_momentum(
         sites, nup + ndn, (int)nup - (int)ndn, Index(dims), K);
   }();
+  if (mode == "nbasis") {
+    print_states(basis);
+    return 0;
+  }
   FermionExpression e;
   const size_t longest = read_expression(e);
   if (longest == (size_t)-1) return usage();
@@ -281,7 +292,8 @@ int main(int argc, char** argv) {
   if (mode == "mbasis" || mode == "mh") return momentum_mode(mode, a);
   if (mode == "mj") return current_mode(a);
   if (mode == "sigma") return sigma_mode(a);
-  if (mode == "expr" || mode == "mexpr") return expr_mode(mode, a);
+  if (mode == "expr" || mode == "mexpr" || mode == "nexpr" || mode == "nbasis")
+    return expr_mode(mode, a);
   if (mode == "cereal") return cereal_mode(argc, argv);
 
   size_t sites = 0, nup = 0, ndn = 0;

@@ -288,6 +288,25 @@ int main() {
     CHECK(std::sqrt(ne) > std::sqrt(nb) && std::sqrt(ne) < std::exp(9.66) * std::sqrt(nb));
   }
 
+  {
+    CASE("exact_diagonalization_example");
+    // examples/hubbard_exact_diagonalization.cpp: L = 4, N = 2, t = 1, U = 4 on
+    // the full fixed-N basis and on the spin-0 sector; the ground state is a
+    // singlet, so both give the same lowest eigenvalue
+    HubbardModel hubbard(1.0, 4.0, 4);
+    auto full_basis = FermionBasis::with_fixed_particle_number(4, 2);
+    auto spin_zero_basis = FermionBasis::with_fixed_particle_number_and_spin(4, 2, 0);
+    CHECK(full_basis.size() == 28 && spin_zero_basis.size() == 16);
+    auto H = hubbard.op(full_basis);
+    auto H0 = hubbard.op(spin_zero_basis);
+    auto e = find_min_eigenpair(H, 28, H.gaussian(5));
+    auto e0 = find_min_eigenpair(H0, 16, H0.gaussian(5));
+    std::printf("  ground state (full basis) %.12f, (spin 0) %.12f\n", e.value, e0.value);
+    CHECK(std::abs(e.value - e0.value) < 1e-10);
+    auto sp = mbamd::compute_matrix_elements(H);  // H as an arma::sp_cx_mat
+    CHECK(sp.n_rows() == 28 && sp.n_cols() == 28 && sp.n_nonzero() == 100);
+  }
+
   std::printf(g_fail ? "FAILED (%d)\n" : "ALL PASSED\n", g_fail);
   return g_fail ? 1 : 0;
 }

@@ -109,6 +109,20 @@ def _momentum_expression(rng, dims, n_one, n_two):
     return terms
 
 
+def _hubbard_terms(L, t, U):
+    """HubbardModel(t, U, L).hamiltonian() as raw terms, in the reference's loop
+    order (hubbard_model.h:13-35), for the fixed-N example basis."""
+    terms = []
+    for i in range(L):
+        j = (i + 1) % L
+        for s in (0, 1):
+            terms.append((-t, [(0, s, i), (1, s, j)]))
+            terms.append((-t, [(0, s, j), (1, s, i)]))
+    for i in range(L):
+        terms.append((U, [(0, 0, i), (1, 0, i), (0, 1, i), (1, 1, i)]))
+    return terms
+
+
 def _special_terms(L):
     """Monomials whose action needs care: 1 - n as c c+, c+ c c+ c, a repeated
     creator (zero), a constant, a coefficient below 1000 eps, a key added twice
@@ -142,17 +156,21 @@ def expression_cases():
              ("mexpr", [2, 2], 1, 2, [1, 0], _momentum_expression(rng, [2, 2], 4, 6)),
              ("mexpr", [5], 1, 2, [2], _momentum_expression(rng, [5], 5, 8)),
              ("mexpr", [7], 2, 1, [3], _momentum_expression(rng, [7], 6, 16)),
-             ("mexpr", [2, 2], 2, 1, [0, 1], _special_terms(4))]
+             ("mexpr", [2, 2], 2, 1, [0, 1], _special_terms(4)),
+             ("nexpr", [4], 2, None, None, _hubbard_terms(4, 1.0, 4.0)),
+             ("nexpr", [4], 3, None, None, _random_expression(rng, 4, 6, 4, odd=2)),
+             ("nexpr", [5], 2, None, None, _random_expression(rng, 5, 8, 5)),
+             ("nexpr", [3], 3, None, None, _special_terms(3))]
     for k, (mode, dims, nu, nd, K, terms) in enumerate(cases):
-        args = [mode] + ([dims[0], nu, nd] if mode == "expr" else
-                         [len(dims)] + dims + [nu, nd] + K)
+        args = [mode] + ([dims[0], nu, nd] if mode == "expr" else [dims[0], nu] if mode == "nexpr"
+                         else [len(dims)] + dims + [nu, nd] + K)
         out = subprocess.run([DRIVER] + [str(a) for a in args], input=_terms_text(terms),
                              check=True, capture_output=True, text=True).stdout
         st = np.array([(int(l.split()[1]), int(l.split()[2])) for l in out.splitlines()
                        if l[0] == "S"], np.uint64).reshape(-1, 2)
         el = np.array([[float(x) for x in l.split()[1:]] for l in out.splitlines()
                        if l[0] == "E"], np.float64).reshape(-1, 4)
-        name = f"{mode}{k}_{'x'.join(map(str, dims))}_{nu}_{nd}"
+        name = f"{mode}{k}_{'x'.join(map(str, dims))}_{nu}_{nd}"  # nexpr: nu = N
         arrays[name + "__states"] = st
         arrays[name + "__ij"] = el[:, :2].astype(np.int64)
         arrays[name + "__val"] = el[:, 2] + 1j * el[:, 3]
@@ -162,6 +180,12 @@ def expression_cases():
                                           np.int64).reshape(-1, 3)
         manifest.append({"name": name, "kind": mode, "dims": dims, "n_up": nu, "n_dn": nd,
                          "K": K, "dim": int(len(st)), "nnz": int(len(el))})
+    for L in range(1, 7):
+        for N in range(0, min(2 * L, 7) + 1):
+            st, _ = run(["nbasis", L, N])
+            name = f"nbasis_{L}_{N}"
+            arrays[name + "__states"] = st
+            manifest.append({"name": name, "kind": "nbasis", "L": L, "N": N, "dim": int(len(st))})
     for k, (nr, nc, nnz) in enumerate([(0, 0, 0), (4, 5, 3), (37, 37, 200)]):
         r = rng.integers(0, max(nr, 1), nnz)
         c = rng.integers(0, max(nc, 1), nnz)

@@ -51,8 +51,10 @@ def kflat(dims, K):
     return Kc[0] + dd[0] * (Kc[1] + dd[1] * Kc[2])
 
 
-def basis_for(dims, nu, nd, K=None):
+def basis_for(dims, nu, nd, K=None, fixed_n=False):
     L = int(np.prod(dims))
+    if fixed_n:  # with_fixed_particle_number(L, N = nu)
+        return mb.FermionBasis.with_fixed_particle_number(L, nu)
     if K is None:
         return mb.FermionBasis.sector(L, nu, nd)
     return mb.FermionBasis.with_fixed_particle_number_spin_momentum(L, nu + nd, nu - nd, dims, K)
@@ -67,7 +69,7 @@ def test_expression_operator_matches_reference_golden(cuda):
     rng = np.random.default_rng(1)
     n = 0
     for c in _manifest():
-        if c["kind"] not in ("expr", "mexpr"):
+        if c["kind"] not in ("expr", "mexpr", "nexpr"):
             continue
         name = c["name"]
         coef, nops, ops = gold[name + "__coef"], gold[name + "__nops"], gold[name + "__ops"]
@@ -75,7 +77,7 @@ def test_expression_operator_matches_reference_golden(cuda):
         for cc, m in zip(coef, nops):
             terms.append((complex(cc), [tuple(int(v) for v in ops[k + i]) for i in range(int(m))]))
             k += int(m)
-        basis = basis_for(c["dims"], c["n_up"], c["n_dn"], c["K"])
+        basis = basis_for(c["dims"], c["n_up"], c["n_dn"], c["K"], c["kind"] == "nexpr")
         op = mb.FermionExpressionOperator(basis, expr_from_terms(terms))
         D = op.dimension()
         assert D == c["dim"]
@@ -95,7 +97,7 @@ def test_expression_operator_matches_reference_golden(cuda):
         assert np.abs(v - gold[name + "__val"][order]).max(initial=0) <= 1e-14 * max(
             1.0, np.abs(v).max(initial=0)), name
         n += 1
-    assert n == 10
+    assert n == 14
 
 
 def random_expression(rng, L, n_one, n_two, n_odd=0, n_three=0):
@@ -245,3 +247,70 @@ def test_expression_operator_rejects_invalid_use(cuda):
     empty = mb.FermionExpressionOperator(basis, FermionExpression())
     assert torch.count_nonzero(empty.hv(x)).item() == 0
     assert mb.compute_matrix_elements(empty).nnz == 0
+
+
+def test_hubbard_on_fixed_n_basis_reference_example(cuda):
+    """The reference's exact-diagonalisation example (examples/hubbard_exact_
+    diagonalization.cpp): HubbardModel(1, 4, 4) on with_fixed_particle_number(4, 2)
+    and on the spin-0 sector; the full-basis spectrum is the union of the three
+    spin sectors' spectra (oracle CSC), and the model's operator handle (built by
+    mb_hubbard_create) equals the compiled expression."""
+    model = mb.HubbardModel(1.0, 4.0, 4)
+    basis = mb.FermionBasis.with_fixed_particle_number(4, 2)
+    op = model.operator(basis)
+    assert op.dimension() == 28
+    H = mb.compute_matrix_elements(basis, model.hamiltonian()).to_dense()
+    w = np.linalg.eigvalsh(H)
+    sectors = np.concatenate([np.linalg.eigvalsh(po.build_csc([4], nu, 2 - nu, 1.0, 4.0)
+                                                 .to_scipy().toarray()) for nu in range(3)])
+    assert np.abs(np.sort(sectors) - w).max() < 1e-12
+    x = op.gaussian(seed=3)
+    y0 = op.hv(x)
+    y1 = mb.FermionExpressionOperator(basis, model.hamiltonian()).hv(x)
+    assert torch.abs(y1 - y0).max().item() <= 1e-14 * torch.abs(y0).max().item()
+    assert torch.abs(op.to_device_order(op.apply(op.to_reference_order(x))) - y0).max().item() < 1e-12
+
+
+@pytest.mark.parametrize("L,N", [(6, 5), (5, 4), (7, 3)])
+def test_spin_flip_expressions_on_fixed_n_match_oracle(cuda, L, N):
+    """Expressions with spin flips (terms moving particles between spins) on a
+    fixed-N basis against the oracle, beyond the reference's range."""
+    rng = np.random.default_rng(L * 10 + N)
+    e = random_expression(rng, L, 12, 8, n_odd=4) + mb.HubbardModel(1.0, 3.0, L).hamiltonian()
+    for i in range(L - 1):  # transverse field and spin exchange
+        e += 0.4 * (mb.one_body(mb.Spin.Up, i, mb.Spin.Down, i)
+                    + mb.one_body(mb.Spin.Down, i, mb.Spin.Up, i))
+    basis = mb.FermionBasis.with_fixed_particle_number(L, N)
+    op = mb.FermionExpressionOperator(basis, e)
+    up, dn = po.fixed_n_states(L, N)
+    M = po.expression_matrix(L, list(zip(up, dn)), oracle_terms(e))
+    x = rng.standard_normal(len(up)) + 1j * rng.standard_normal(len(up))
+    want = M @ x
+    got = apply_ref(op, x, cuda)
+    assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+    r, c, v = mb.compute_matrix_elements(op).entries()
+    rr, cc, vv = po.sparse_column_major(M)
+    assert np.array_equal(r, rr) and np.array_equal(c, cc)
+    assert np.abs(v - vv).max() <= 1e-13 * np.abs(vv).max()
+
+
+def test_fixed_n_ground_energy_is_the_sector_minimum(cuda):
+    """Lanczos on with_fixed_particle_number(8, 8) (D = 12,870) for the ring
+    Hubbard model gives the lowest ground energy over the nine spin sectors."""
+    model = mb.HubbardModel(1.0, 4.0, 8)
+    op = model.operator(mb.FermionBasis.with_fixed_particle_number(8, 8))
+    e_full = mb.find_min_eigenpair(op, 200, op.gaussian(seed=1), want_vector=False).value
+    e_sec = []
+    for nu in range(0, 9):
+        s = model.operator(mb.FermionBasis.sector(8, nu, 8 - nu))
+        e_sec.append(mb.find_min_eigenpair(s, min(200, s.dimension()), s.gaussian(seed=2),
+                                           want_vector=False).value)
+    assert abs(e_full - min(e_sec)) <= 1e-9 * abs(min(e_sec))
+
+
+def test_matrix_elements_density_on_single_orbital(cuda):
+    """Port of test_matrix_elements.cpp:54-68: n_up0 on with_fixed_particle_number
+    (1, 1) is diag(1, 0)."""
+    basis = mb.FermionBasis.with_fixed_particle_number(1, 1)
+    M = mb.compute_matrix_elements(basis, mb.density(mb.Spin.Up, 0)).to_dense()
+    assert np.array_equal(M, np.array([[1, 0], [0, 0]], np.complex128))

@@ -293,3 +293,34 @@ for name in sorted(mb._SIGS):
     for name, st in status.items():
         assert int(st) in range(6), name
         assert (int(st) == 0) == (name in ok_with_nothing), (name, st)
+
+
+def test_fixed_n_basis_matches_reference_golden():
+    """with_fixed_particle_number through the library (host side): states and
+    indices of the compiled reference's lists for every L <= 6, N <= min(2L, 7)."""
+    gold = _gold("ref_expression.npz")
+    n = 0
+    for c in _manifest()["expression"]:
+        if c["kind"] != "nbasis":
+            continue
+        b = mb.FermionBasis.with_fixed_particle_number(c["L"], c["N"])
+        st = gold[c["name"] + "__states"]
+        assert b.size() == len(st)
+        for i, (u, d) in enumerate(st):
+            assert b.state(i) == (int(u), int(d))
+            assert b.index_of(int(u), int(d)) == i
+        n += 1
+    assert n == 39
+    with pytest.raises(ValueError):
+        mb.FermionBasis.with_fixed_particle_number(3, 7)
+
+
+def test_fixed_n_basis_reference_unit_cases():
+    """Ports of test_basis.cpp:5-19: with_fixed_particle_number(1, 1) holds
+    c+_up0 and c+_dn0 (in that order, the reference's enumeration), (1, 2) the
+    one state c+_up0 c+_dn0."""
+    b = mb.FermionBasis.with_fixed_particle_number(1, 1)
+    assert b.size() == 2
+    assert b.state(0) == (1, 0) and b.state(1) == (0, 1)
+    b = mb.FermionBasis.with_fixed_particle_number(1, 2)
+    assert b.size() == 1 and b.state(0) == (1, 1)

@@ -444,12 +444,14 @@ def test_expression_matrix_matches_reference_golden():
     gold = np.load(os.path.join(GOLD, "ref_expression.npz"))
     n = 0
     for c in _manifest()["expression"]:
-        if c["kind"] not in ("expr", "mexpr"):
+        if c["kind"] not in ("expr", "mexpr", "nexpr"):
             continue
         dims, nu, nd = c["dims"], c["n_up"], c["n_dn"]
         L = int(np.prod(dims))
         st = gold[c["name"] + "__states"]
-        if c["kind"] == "expr":
+        if c["kind"] == "nexpr":  # with_fixed_particle_number(L, N = n_up)
+            up, dn = po.fixed_n_states(L, nu)
+        elif c["kind"] == "expr":
             up, dn = po.basis_states(L, nu, nd)
         else:
             Kc = c["K"] + [0] * (3 - len(dims))
@@ -464,7 +466,7 @@ def test_expression_matrix_matches_reference_golden():
         assert np.count_nonzero(M) == len(ij), c["name"]
         assert np.abs(M - R).max() <= 1e-14 * max(1.0, np.abs(R).max()), c["name"]
         n += 1
-    assert n == 10
+    assert n == 14
 
 
 def test_cereal_bytes_match_reference_golden():
@@ -611,3 +613,21 @@ def test_kpm_projector_known_answers():
     H = np.diag([-2.0, -1.0, -0.5, 0.5, 1.0, 2.0])
     P = po.kpm_projector_matrix(H, 200, -2.0, 2.0)
     assert abs(np.trace(P).real - 3.0) < 0.1
+
+
+def test_fixed_n_basis_matches_reference_golden():
+    """with_fixed_particle_number (basis.cpp:70-89, 240-265): the oracle's DFS
+    against the compiled reference for every L <= 6, N <= min(2L, 7); the size
+    is C(2L, N)."""
+    import math
+    gold = np.load(os.path.join(GOLD, "ref_expression.npz"))
+    n = 0
+    for c in _manifest()["expression"]:
+        if c["kind"] != "nbasis":
+            continue
+        up, dn = po.fixed_n_states(c["L"], c["N"])
+        st = np.stack([up, dn], 1).astype(np.uint64).reshape(-1, 2)
+        assert np.array_equal(st, gold[c["name"] + "__states"]), c["name"]
+        assert len(up) == math.comb(2 * c["L"], c["N"])
+        n += 1
+    assert n == 39
