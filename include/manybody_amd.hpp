@@ -605,6 +605,49 @@ inline SparseMatrix compute_matrix_elements(const HubbardOperator &expression_op
   return SparseMatrix(sp);
 }
 
+namespace detail {
+struct TermArrays {
+  std::vector<double> coeffs;
+  std::vector<int64_t> offsets{0};
+  std::vector<uint16_t> ops;
+  explicit TermArrays(const std::vector<ExpressionTerm> &terms) {
+    for (const auto &t : terms) {
+      coeffs.push_back(t.c.real());
+      coeffs.push_back(t.c.imag());
+      ops.insert(ops.end(), t.ops.begin(), t.ops.end());
+      offsets.push_back((int64_t)ops.size());
+    }
+    if (ops.empty()) ops.push_back(0);
+  }
+  int64_t n() const { return (int64_t)offsets.size() - 1; }
+};
+}  // namespace detail
+
+// compute_rectangular_matrix_elements<arma::sp_cx_mat>(row_basis, col_basis, A)
+// (matrix_elements.h:110-167), built on the device
+inline SparseMatrix compute_rectangular_matrix_elements(const FermionBasis &row_basis,
+                                                        const FermionBasis &col_basis,
+                                                        const std::vector<ExpressionTerm> &terms,
+                                                        hipStream_t s = nullptr) {
+  detail::TermArrays a(terms);
+  mb_sparse *sp = nullptr;
+  check(mb_compute_rectangular_matrix_elements(row_basis.handle(), col_basis.handle(), a.n(),
+                                               a.coeffs.data(), a.offsets.data(), a.ops.data(),
+                                               &sp, s));
+  return SparseMatrix(sp);
+}
+
+// compute_vector_elements<arma::cx_vec>(basis, A) (matrix_elements.h:31-51): <i|A|0>
+inline std::vector<std::complex<double>> compute_vector_elements(
+    const FermionBasis &basis, const std::vector<ExpressionTerm> &terms, hipStream_t s = nullptr) {
+  detail::TermArrays a(terms);
+  std::vector<std::complex<double>> v(std::max<size_t>(basis.size(), 1));
+  check(mb_compute_vector_elements(basis.handle(), a.n(), a.coeffs.data(), a.offsets.data(),
+                                   a.ops.data(), reinterpret_cast<double *>(v.data()), s));
+  v.resize(basis.size());
+  return v;
+}
+
 // Serving loop: requests admitted as they arrive and as slots and device
 // memory free up (mb_server_*).
 class Server {

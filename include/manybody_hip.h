@@ -382,6 +382,24 @@ int mb_sparse_entries(const mb_sparse *sp, uint64_t *rows, uint64_t *cols, doubl
 int mb_sparse_save_cereal(const mb_sparse *sp, const char *path);
 int mb_sparse_load_cereal(const char *path, mb_sparse **out);
 
+/* compute_rectangular_matrix_elements(row_basis, col_basis, A)
+ * (matrix_elements.h:110-167): <i|A|j> for i in row_basis and j in col_basis
+ * (reference indices), as an n_rows x n_cols sparse matrix in SpMat iteration
+ * order.  Terms as for mb_expression_create; A may change the particle numbers
+ * (creators, annihilators, pair creation) and the two bases may be of any kinds
+ * (fixed N_up/N_dn, fixed N, fixed momentum).  Built on the device.
+ * MB_ERR_INVALID if the bases have different orbital counts or more than 24. */
+int mb_compute_rectangular_matrix_elements(const mb_basis *row_basis, const mb_basis *col_basis,
+                                           int64_t n_terms, const double *coeffs,
+                                           const int64_t *offsets, const uint16_t *ops,
+                                           mb_sparse **out, mb_stream stream);
+/* compute_vector_elements(basis, A) (matrix_elements.h:31-51): <i|A|0> for every
+ * state i of the basis, |0> the vacuum, into out[2 * size] (host memory,
+ * re, im interleaved, reference order). */
+int mb_compute_vector_elements(const mb_basis *basis, int64_t n_terms, const double *coeffs,
+                               const int64_t *offsets, const uint16_t *ops, double *out,
+                               mb_stream stream);
+
 /* ------------------------------------------------------------ observables */
 
 /* <x|G_updn(r)|x> / <x|x> for every lattice displacement r, written to g[r]

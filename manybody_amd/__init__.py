@@ -112,6 +112,9 @@ _SIGS = {
     "mb_fixed_n_basis_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_vp)]),
     "mb_expression_create": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, C.POINTER(_vp)]),
     "mb_compute_matrix_elements": (C.c_int, [_vp, C.POINTER(_vp), _vp]),
+    "mb_compute_rectangular_matrix_elements": (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp, _vp,
+                                                         C.POINTER(_vp), _vp]),
+    "mb_compute_vector_elements": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, _vp, _vp]),
     "mb_sparse_create": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, _vp, _vp, _vp,
                                    C.POINTER(_vp)]),
     "mb_sparse_destroy": (None, [_vp]),
@@ -353,6 +356,35 @@ def compute_matrix_elements(basis_or_op, expression=None):
     h = C.c_void_p()
     _check(lib().mb_compute_matrix_elements(op._h, C.byref(h), _stream()))
     return SparseMatrix(h)
+
+
+def compute_rectangular_matrix_elements(row_basis: FermionBasis, col_basis: FermionBasis,
+                                        expression):
+    """compute_rectangular_matrix_elements (matrix_elements.h:110-167): the sparse
+    matrix <i|A|j>, i in row_basis, j in col_basis (reference index order), built
+    on the device.  A may change the particle numbers; the bases may be of any
+    kinds with the same orbital count."""
+    from .expression import SparseMatrix
+    coeffs, offsets, ops = expression.arrays()
+    h = C.c_void_p()
+    _check(lib().mb_compute_rectangular_matrix_elements(
+        row_basis._h, col_basis._h, len(offsets) - 1, coeffs.ctypes.data_as(C.c_void_p),
+        offsets.ctypes.data_as(C.c_void_p), ops.ctypes.data_as(C.c_void_p), C.byref(h),
+        _stream()))
+    return SparseMatrix(h)
+
+
+def compute_vector_elements(basis: FermionBasis, expression):
+    """compute_vector_elements (matrix_elements.h:31-51): <i|A|0> for every state i
+    of the basis (|0> the vacuum), a complex128 numpy vector in reference order."""
+    import numpy as np
+    coeffs, offsets, ops = expression.arrays()
+    out = np.zeros(2 * max(basis.size(), 1), np.float64)
+    _check(lib().mb_compute_vector_elements(
+        basis._h, len(offsets) - 1, coeffs.ctypes.data_as(C.c_void_p),
+        offsets.ctypes.data_as(C.c_void_p), ops.ctypes.data_as(C.c_void_p),
+        out.ctypes.data_as(C.c_void_p), _stream()))
+    return (out[0::2] + 1j * out[1::2])[:basis.size()]
 
 
 class HubbardOperator:

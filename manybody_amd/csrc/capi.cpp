@@ -14,6 +14,7 @@
 #include "conductivity.hpp"
 #include "expression.hpp"
 #include "fixed_n.hpp"
+#include "rectangular.hpp"
 
 #include <cstdio>
 #include <map>
@@ -941,6 +942,57 @@ int mb_compute_matrix_elements(const mb_hubbard *h, mb_sparse **out, mb_stream s
     sp->n_rows = sp->n_cols = (uint64_t)h->expr->dim();
     h->expr->export_coo(sp->rows, sp->cols, sp->vals, S(stream));
     *out = sp.release();
+  });
+}
+
+namespace {
+
+mb::BasisRef basis_ref(const mb_basis *b) {
+  mb::BasisRef r;
+  if (b->ms)
+    r.mom = b->ms.get();
+  else if (b->fn)
+    r.fn = b->fn.get();
+  else
+    r.real = &b->sec;
+  return r;
+}
+
+void check_terms(int64_t n_terms, const double *coeffs, const int64_t *offsets,
+                 const uint16_t *ops) {
+  need(n_terms >= 0, "negative term count");
+  need(n_terms == 0 || (coeffs && offsets), "null term arrays");
+  need(n_terms == 0 || offsets[0] == 0, "offsets[0] must be 0");
+  need(n_terms == 0 || offsets[n_terms] == 0 || ops, "null operator array");
+}
+
+}  // namespace
+
+int mb_compute_rectangular_matrix_elements(const mb_basis *row_basis, const mb_basis *col_basis,
+                                           int64_t n_terms, const double *coeffs,
+                                           const int64_t *offsets, const uint16_t *ops,
+                                           mb_sparse **out, mb_stream stream) {
+  return guarded([&] {
+    need(row_basis && col_basis && out, "null argument");
+    *out = nullptr;
+    check_terms(n_terms, coeffs, offsets, ops);
+    auto sp = std::make_unique<mb_sparse>();
+    sp->n_rows = (uint64_t)row_basis->size();
+    sp->n_cols = (uint64_t)col_basis->size();
+    mb::rectangular_matrix_elements(basis_ref(row_basis), basis_ref(col_basis), n_terms, coeffs,
+                                    offsets, ops, sp->rows, sp->cols, sp->vals, S(stream));
+    *out = sp.release();
+  });
+}
+
+int mb_compute_vector_elements(const mb_basis *basis, int64_t n_terms, const double *coeffs,
+                               const int64_t *offsets, const uint16_t *ops, double *out,
+                               mb_stream stream) {
+  return guarded([&] {
+    need(basis && out, "null argument");
+    check_terms(n_terms, coeffs, offsets, ops);
+    mb::vector_elements(basis_ref(basis), n_terms, coeffs, offsets, ops,
+                        reinterpret_cast<std::complex<double> *>(out), S(stream));
   });
 }
 

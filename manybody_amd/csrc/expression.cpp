@@ -119,6 +119,26 @@ struct Grouped {
 
 }  // namespace
 
+std::vector<MonomialAction> compile_expression(int64_t n_terms, const double *coeffs,
+                                               const int64_t *offsets, const uint16_t *ops,
+                                               int L) {
+  if (n_terms < 0) throw std::invalid_argument("negative term count");
+  if (n_terms > 0 && (!coeffs || !offsets || (offsets[n_terms] > 0 && !ops)))
+    throw std::invalid_argument("null term arrays");
+  std::map<std::vector<uint16_t>, std::complex<double>> expr;
+  for (int64_t t = 0; t < n_terms; ++t) {
+    if (offsets[t + 1] < offsets[t]) throw std::invalid_argument("term offsets must not decrease");
+    std::vector<uint16_t> key(ops + offsets[t], ops + offsets[t + 1]);
+    add_to(expr, key, {coeffs[2 * t], coeffs[2 * t + 1]});
+  }
+  std::vector<MonomialAction> out;
+  for (const auto &[key, c] : expr) {
+    MonomialAction a;
+    if (compile_monomial(key.data(), (int)key.size(), L, c, a)) out.push_back(a);
+  }
+  return out;
+}
+
 ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector *mom,
                                        int64_t n_terms, const double *coeffs,
                                        const int64_t *offsets, const uint16_t *ops,

@@ -140,6 +140,52 @@ __global__ __launch_bounds__(kBlk) void export_compact_kernel(int64_t n, const u
   }
 }
 
+// Rectangular export: the reference index of output word s2 in the row basis,
+// or -1 when it is not a state of that basis.  Words have 2L bits (monomial
+// orbitals are below L), so both halves index the 2^L lookups.
+__device__ __forceinline__ int64_t rect_target(const RectSide &r, uint64_t s2) {
+  const uint32_t u2 = (uint32_t)(s2 & ((uint64_t{1} << r.L) - 1)), d2 = (uint32_t)(s2 >> r.L);
+  const int32_t g = r.up_row[u2];
+  if (g < 0 || r.dn_grp[d2] != r.row_cgrp[g]) return -1;
+  return r.ref[r.row_base[g] + r.dn_loc[d2]];
+}
+
+// One thread per column-basis element (row blockIdx.x, column blockIdx.y * kBlk
+// + tid): counts (EMIT false) or emits its contributions, one per monomial that
+// applies and lands in the row basis.
+template <bool EMIT>
+__global__ __launch_bounds__(kBlk) void rect_kernel(RectSide row, RectSide col,
+                                                    const RectMono *__restrict__ mono, int n_mono,
+                                                    const int64_t *__restrict__ off,
+                                                    int64_t *__restrict__ cnt,
+                                                    uint64_t *__restrict__ keys,
+                                                    int64_t *__restrict__ idx,
+                                                    double *__restrict__ vals) {
+  const int64_t g = blockIdx.x;
+  const int c = (int)blockIdx.y * kBlk + (int)threadIdx.x;
+  if (c >= col.row_nc[g]) return;
+  const int64_t pos = col.row_base[g] + c;
+  const uint64_t s =
+      (uint64_t)col.row_up[g] | ((uint64_t)col.dn_bits[col.row_dg0[g] + c] << col.L);
+  int64_t o = EMIT ? off[pos] : 0;
+  const uint64_t cref = EMIT ? (uint64_t)col.ref[pos] : 0;
+  for (int m = 0; m < n_mono; ++m) {
+    const RectMono a = mono[m];
+    if ((s & a.touched) != a.in) continue;
+    const int64_t rr = rect_target(row, s ^ (a.in ^ a.out));
+    if (rr < 0) continue;
+    if constexpr (EMIT) {
+      const double sg = (__builtin_popcountll(s & a.parity) & 1) ? -1.0 : 1.0;
+      keys[o] = cref * (uint64_t)row.D + (uint64_t)rr;
+      idx[o] = o;
+      vals[2 * o] = sg * a.re;
+      vals[2 * o + 1] = sg * a.im;
+    }
+    ++o;
+  }
+  if constexpr (!EMIT) cnt[pos] = o;
+}
+
 unsigned grid_of(int64_t n) {
   const int64_t g = (n + kBlk - 1) / kBlk;
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 1 << 16));
@@ -161,6 +207,9 @@ int64_t exclusive_sum(const int64_t *in, int64_t *out, int64_t n, hipStream_t s)
 
 }  // namespace
 
+int64_t sort_merge(int64_t n, uint64_t key_max, DevBuf &k0, DevBuf &k1, DevBuf &i0, DevBuf &i1,
+                   DevBuf &v0, DevBuf &keys, DevBuf &vals, hipStream_t s);
+
 int64_t expr_export(const ExprTables &t, const int64_t *ref_of_pos, DevBuf &keys, DevBuf &vals,
                     hipStream_t s) {
   const int64_t D = t.D;
@@ -178,8 +227,16 @@ int64_t expr_export(const ExprTables &t, const int64_t *ref_of_pos, DevBuf &keys
                      off.as<int64_t>(), ref_of_pos, k0.as<uint64_t>(), i0.as<int64_t>(),
                      v0.as<double>());
   MB_HIP(hipGetLastError());
+  return sort_merge(n, (uint64_t)D * (uint64_t)D - 1, k0, k1, i0, i1, v0, keys, vals, s);
+}
+
+// Sorts n raw (key, value) contributions by key (stable: equal keys keep their
+// emission order), sums each run and keeps the sums not below 1000 eps: keys /
+// vals get the kept entries in key order; returns their number.
+int64_t sort_merge(int64_t n, uint64_t key_max, DevBuf &k0, DevBuf &k1, DevBuf &i0, DevBuf &i1,
+                   DevBuf &v0, DevBuf &keys, DevBuf &vals, hipStream_t s) {
   int bits = 1;
-  while (bits < 64 && ((uint64_t)D * (uint64_t)D - 1) >> bits) ++bits;
+  while (bits < 64 && (key_max >> bits)) ++bits;
   size_t tmp = 0;
   MB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k0.as<uint64_t>(), k1.as<uint64_t>(),
                                             i0.as<int64_t>(), i1.as<int64_t>(), (int)n, 0, bits,
@@ -205,6 +262,30 @@ int64_t expr_export(const ExprTables &t, const int64_t *ref_of_pos, DevBuf &keys
   MB_HIP(hipGetLastError());
   MB_HIP(hipStreamSynchronize(s));
   return m;
+}
+
+int64_t rect_export(const RectSide &row, const RectSide &col, const RectMono *monos, int n_mono,
+                    DevBuf &keys, DevBuf &vals, hipStream_t s) {
+  const int64_t Dc = col.D;
+  if (Dc == 0 || row.D == 0 || n_mono == 0 || col.nrows == 0 || col.max_nc == 0) return 0;
+  if (Dc >= (int64_t{1} << 31) || col.nrows >= (int64_t{1} << 31))
+    throw std::invalid_argument("column basis too large to export");
+  DevBuf cnt((size_t)Dc * 8), off((size_t)Dc * 8);
+  MB_HIP(hipMemsetAsync(cnt.p, 0, (size_t)Dc * 8, s));
+  const dim3 grid((unsigned)col.nrows, (unsigned)((col.max_nc + kBlk - 1) / kBlk));
+  hipLaunchKernelGGL(rect_kernel<false>, grid, dim3(kBlk), 0, s, row, col, monos, n_mono,
+                     nullptr, cnt.as<int64_t>(), nullptr, nullptr, nullptr);
+  MB_HIP(hipGetLastError());
+  const int64_t n = exclusive_sum(cnt.as<int64_t>(), off.as<int64_t>(), Dc, s);
+  if (n == 0) return 0;
+  if (n >= (int64_t{1} << 31)) throw std::invalid_argument("too many matrix entries to export");
+  DevBuf k0((size_t)n * 8), k1((size_t)n * 8), i0((size_t)n * 8), i1((size_t)n * 8),
+      v0((size_t)n * 16);
+  hipLaunchKernelGGL(rect_kernel<true>, grid, dim3(kBlk), 0, s, row, col, monos, n_mono,
+                     off.as<int64_t>(), nullptr, k0.as<uint64_t>(), i0.as<int64_t>(),
+                     v0.as<double>());
+  MB_HIP(hipGetLastError());
+  return sort_merge(n, (uint64_t)Dc * (uint64_t)row.D - 1, k0, k1, i0, i1, v0, keys, vals, s);
 }
 
 }  // namespace mb

@@ -107,6 +107,36 @@ class ExpressionOperator final : public HvOp {
   const typename V::value_type *keep(const V &v);
 };
 
+// The terms merged by operator key as FermionExpression does (add_to) and
+// compiled on L orbitals; identically zero monomials are left out.
+std::vector<MonomialAction> compile_expression(int64_t n_terms, const double *coeffs,
+                                               const int64_t *offsets, const uint16_t *ops, int L);
+
+// Device view of one basis for the rectangular export: rows of up strings, row
+// g holding row_nc[g] elements from row_base[g] whose down strings are
+// dn_bits[row_dg0[g] ...], all of group row_cgrp[g]; bit lookups up_row (row of
+// an up string), dn_loc and dn_grp (index in its group and group of a down
+// string; -1 outside); ref maps a device position to the reference index.
+struct RectSide {
+  int L = 0;
+  int64_t D = 0, nrows = 0;
+  int max_nc = 0;
+  const uint32_t *row_up = nullptr;
+  const int64_t *row_base = nullptr;
+  const int32_t *row_nc = nullptr, *row_dg0 = nullptr, *row_cgrp = nullptr;
+  const uint32_t *dn_bits = nullptr;
+  const int32_t *up_row = nullptr, *dn_loc = nullptr, *dn_grp = nullptr;
+  const int64_t *ref = nullptr;
+};
+struct RectMono {
+  uint64_t touched, in, out, parity;
+  double re, im;
+};
+// every (column element, monomial) whose output lies in the row basis emits
+// key col_ref * row.D + row_ref; sorted and merged as expr_export
+int64_t rect_export(const RectSide &row, const RectSide &col, const RectMono *monos, int n_mono,
+                    DevBuf &keys, DevBuf &vals, hipStream_t s);
+
 // kernels (kernels.hip, expression.hip)
 int64_t launch_expr_hv(const ExprTables &t, const double *x, double *y, Epi epi, const EpiArgs &a,
                        hipStream_t s);

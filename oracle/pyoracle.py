@@ -563,6 +563,34 @@ def expression_matrix(L: int, states, terms):
     return M
 
 
+def rectangular_matrix(L: int, row_states, col_states, terms):
+    """Dense <i|A|j> of compute_rectangular_matrix_elements (matrix_elements.h:
+    110-167): i over `row_states`, j over `col_states` (each [(up, dn), ...] in
+    basis order); A|j> canonicalised and merged by add_to, outputs outside the
+    row basis dropped (fermion state normalisation is 1)."""
+    expr = expression_terms(terms)
+    idx = {(int(u), int(d)): i for i, (u, d) in enumerate(row_states)}
+    M = np.zeros((len(row_states), len(col_states)), dtype=np.complex128)
+    for j, (u, d) in enumerate(col_states):
+        col: dict = {}
+        for ops, c in expr.items():
+            r = apply_monomial(ops, L, int(u), int(d))
+            if r is None:
+                continue
+            i = idx.get((r[0], r[1]))
+            if i is not None:
+                _add_to(col, i, c * r[2])
+        for i, v in col.items():
+            M[i, j] = v
+    return M
+
+
+def vector_elements(L: int, states, terms):
+    """compute_vector_elements (matrix_elements.h:31-51): element i is the constant
+    term of canonicalize(adjoint(<i|) A), i.e. <i|A|0> with |0> the vacuum."""
+    return rectangular_matrix(L, states, [(0, 0)], terms)[:, 0]
+
+
 def sparse_column_major(M):
     """(rows, cols, values) of the nonzeros of M in arma::SpMat iteration order:
     column by column, rows ascending."""

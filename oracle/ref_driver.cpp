@@ -107,7 +107,7 @@ void print_matrix(const FermionBasis& basis, const Model& model) {
 int usage() {
   std::fprintf(stderr,
                "usage: ref_driver basis|h1d|h2d|h3d|g1d|g2d|g3d|mbasis|mh|mj|sigma|expr|mexpr|"
-               "cereal ...\n");
+               "cereal|rexpr|vexpr ...\n");
   return 2;
 }
 
@@ -192,6 +192,87 @@ int expr_mode(const std::string& mode, const std::vector<double>& a) {
   }
   print_states(basis);
   print_operator(basis, e);
+  return 0;
+}
+
+// A basis from argv tokens: "s L nup ndn" (with_fixed_particle_number_and_spin),
+// "n L N" (with_fixed_particle_number) or "m nd d_1 .. d_nd nup ndn K_1 .. K_nd"
+// (with_fixed_particle_number_spin_momentum); advances `at`.
+FermionBasis parse_basis(char** argv, int argc, int& at) {
+  auto next = [&]() -> size_t {
+    if (at >= argc) throw std::invalid_argument("basis spec too short");
+    return (size_t)std::atol(argv[at++]);
+  };
+  if (at >= argc) throw std::invalid_argument("basis spec missing");
+  const std::string kind = argv[at++];
+  if (kind == "s") {
+    const size_t L = next(), nu = next(), nd = next();
+    return FermionBasis::with_fixed_particle_number_and_spin(L, nu + nd, (int)nu - (int)nd);
+  }
+  if (kind == "n") {
+    const size_t L = next(), N = next();
+    return FermionBasis::with_fixed_particle_number(L, N);
+  }
+  if (kind != "m") throw std::invalid_argument("basis kind");
+  const size_t nd = next();
+  std::vector<size_t> dims, K;
+  size_t sites = 1;
+  for (size_t d = 0; d < nd; ++d) {
+    dims.push_back(next());
+    sites *= dims.back();
+  }
+  const size_t nu = next(), ndn = next();
+  for (size_t d = 0; d < nd; ++d) K.push_back(next());
+  return FermionBasis::with_fixed_particle_number_spin_momentum(sites, nu + ndn,
+                                                                (int)nu - (int)ndn, Index(dims), K);
+}
+
+struct DenseVector {
+  std::vector<std::complex<double>> a;
+  explicit DenseVector(size_t n) : a(n) {}
+  void zeros() { std::fill(a.begin(), a.end(), std::complex<double>(0.0, 0.0)); }
+  std::complex<double>& operator()(size_t i) { return a[i]; }
+};
+
+// rexpr <row basis> <col basis>: compute_rectangular_matrix_elements
+// (matrix_elements.h:110-167); vexpr <basis>: compute_vector_elements
+// (matrix_elements.h:31-51); terms on stdin.  Prints the row states (S), the
+// column states (C) and the nonzero elements (E i j re im).
+int rect_mode(int argc, char** argv) {
+  const std::string mode = argv[1];
+  int at = 2;
+  FermionBasis row = parse_basis(argv, argc, at);
+  FermionExpression e;
+  const size_t longest = read_expression(e);
+  if (longest == (size_t)-1) return usage();
+  print_states(row);
+  if (mode == "vexpr") {
+    if (longest > 7) return 3;
+    DenseVector v = compute_vector_elements<DenseVector>(row, e);
+    for (size_t i = 0; i < v.a.size(); ++i)
+      if (v.a[i].real() != 0.0 || v.a[i].imag() != 0.0)
+        std::printf("E %zu 0 %.17g %.17g\n", i, v.a[i].real(), v.a[i].imag());
+    return 0;
+  }
+  FermionBasis col = parse_basis(argv, argc, at);
+  const size_t n = col.set.empty() ? 0 : col.set[0].size();
+  if (longest + n > 7) {
+    std::fprintf(stderr, "terms times states exceed the 7-operator key\n");
+    return 3;
+  }
+  for (size_t i = 0; i < col.set.size(); ++i) {
+    uint64_t up = 0, dn = 0;
+    for (const auto& op : col.set[i])
+      (op.spin() == FermionOperator::Spin::Up ? up : dn) |= (uint64_t{1} << op.value());
+    std::printf("C %llu %llu\n", (unsigned long long)up, (unsigned long long)dn);
+  }
+  DenseComplex M = compute_rectangular_matrix_elements<DenseComplex>(row, col, e);
+  for (size_t i = 0; i < M.n_rows; ++i)
+    for (size_t j = 0; j < M.n_cols; ++j) {
+      const auto v = M(i, j);
+      if (v.real() != 0.0 || v.imag() != 0.0)
+        std::printf("E %zu %zu %.17g %.17g\n", i, j, v.real(), v.imag());
+    }
   return 0;
 }
 
@@ -295,6 +376,7 @@ int main(int argc, char** argv) {
   if (mode == "expr" || mode == "mexpr" || mode == "nexpr" || mode == "nbasis")
     return expr_mode(mode, a);
   if (mode == "cereal") return cereal_mode(argc, argv);
+  if (mode == "rexpr" || mode == "vexpr") return rect_mode(argc, argv);
 
   size_t sites = 0, nup = 0, ndn = 0;
   double t = 0, u = 0;

@@ -307,6 +307,30 @@ int main() {
     CHECK(sp.n_rows() == 28 && sp.n_cols() == 28 && sp.n_nonzero() == 100);
   }
 
+  {
+    CASE("rectangular_and_vector_elements");
+    // c+_{up 0} from (1 up, 1 down) to (2 up, 1 down) on 3 orbitals: every column
+    // whose up orbital 0 is empty maps to one row with sign +1 (nothing passed);
+    // c+_{dn 1} c+_{up 0}|0> = -c+_{up 0} c+_{dn 1}|0> (one transposition)
+    auto row = FermionBasis::with_fixed_particle_number_and_spin(3, 3, 1);
+    auto col = FermionBasis::with_fixed_particle_number_and_spin(3, 2, 0);
+    const uint16_t up0 = 0, dn1 = 0x4000 | 1;
+    auto sp = compute_rectangular_matrix_elements(row, col, {{{1.0, 0.0}, {up0}}});
+    CHECK(sp.n_rows() == row.size() && sp.n_cols() == col.size());
+    std::vector<uint64_t> r, c;
+    std::vector<std::complex<double>> v;
+    sp.entries(r, c, v);
+    CHECK(v.size() == 6);  // 2 empty up orbitals 1, 2 (up 0 empty) x 3 down orbitals
+    for (size_t k = 0; k < v.size(); ++k) {
+      const auto cs = col.state(c[k]), rs = row.state(r[k]);
+      CHECK(rs.first == (cs.first | 1u) && rs.second == cs.second && v[k] == 1.0);
+    }
+    auto vec = compute_vector_elements(col, {{{2.0, 0.0}, {dn1, up0}}});
+    CHECK(vec.size() == col.size());
+    for (size_t i = 0; i < vec.size(); ++i)
+      CHECK(vec[i] == (i == col.index_of(1, 2) ? std::complex<double>(-2.0, 0.0) : 0.0));
+  }
+
   std::printf(g_fail ? "FAILED (%d)\n" : "ALL PASSED\n", g_fail);
   return g_fail ? 1 : 0;
 }

@@ -205,6 +205,86 @@ def expression_cases():
         arrays[name + "__bytes"] = np.frombuffer(blob, np.uint8)
         manifest.append({"name": name, "kind": "cereal", "n_rows": nr, "n_cols": nc,
                          "nnz": len(key)})
+    rect_arrays, rect_manifest = rectangular_cases()
+    arrays.update(rect_arrays)
+    return arrays, manifest + rect_manifest
+
+
+def _signature_terms(rng, L, signatures, count):
+    """`count` terms per signature; a signature lists (type, spin) per operator,
+    spin None for a random one; orbitals random, complex coefficients."""
+    terms = []
+    for sig in signatures:
+        for _ in range(count):
+            ops = [(t, int(rng.integers(2)) if sp is None else sp, int(rng.integers(L)))
+                   for t, sp in sig]
+            terms.append((complex(rng.standard_normal(), rng.standard_normal()), ops))
+    return terms
+
+
+def _momentum_current_terms(rng, dims, Q):
+    """sum_{k, s} v_k c+_{k+Q s} c_{k s} with random complex v_k: a J(Q)-shaped
+    operator from sector K to K + Q (orbital = flat momentum)."""
+    sys.path.insert(0, REPO)
+    from oracle import pyoracle as po
+    N = int(np.prod(dims))
+    return [(complex(rng.standard_normal(), rng.standard_normal()),
+             [(0, s, po.momentum_add(dims, k, Q)), (1, s, k)]) for k in range(N) for s in (0, 1)]
+
+
+def rectangular_cases():
+    """compute_rectangular_matrix_elements between two bases (row <- column) and
+    compute_vector_elements through the driver: particle-number changing
+    operators (creators, annihilators, pair creation), the square case, a J(Q)
+    between momentum sectors and bases of different kinds."""
+    rng = np.random.default_rng(78)
+    C, A = 0, 1
+    cases = [
+        ("rexpr", ["s", 4, 2, 1], ["s", 4, 1, 1],
+         _signature_terms(rng, 4, [[(C, 0)], [(C, 0), (C, 1), (A, 1)], [(C, 1)], [(C, 0), (A, 0)]], 3)),
+        ("rexpr", ["n", 4, 3], ["n", 4, 2],
+         _signature_terms(rng, 4, [[(C, None)], [(C, None), (C, None), (A, None)]], 4)),
+        ("rexpr", ["s", 4, 2, 2], ["s", 4, 1, 1],
+         [(complex(rng.standard_normal(), 0.3), [(C, 0, i), (C, 1, i)]) for i in range(4)]
+         + _signature_terms(rng, 4, [[(C, 1), (C, 0)]], 4)),
+        ("rexpr", ["s", 3, 1, 0], ["s", 3, 1, 1],
+         _signature_terms(rng, 3, [[(A, 1)], [(C, 0), (A, 0), (A, 1)]], 3)),
+        ("rexpr", ["s", 4, 1, 1], ["s", 4, 1, 1], _random_expression(rng, 4, 5, 3)),
+        ("rexpr", ["m", 1, 4, 1, 1, 1], ["m", 1, 4, 1, 1, 0], _momentum_current_terms(rng, [4], 1)),
+        ("rexpr", ["m", 2, 2, 2, 1, 1, 1, 0], ["s", 4, 1, 1], _random_expression(rng, 4, 6, 2)),
+        ("rexpr", ["s", 2, 2, 1], ["n", 2, 2],
+         _signature_terms(rng, 2, [[(C, None)], [(C, None), (C, None), (A, None)]], 3)),
+        ("vexpr", ["s", 3, 1, 1], None,
+         _signature_terms(rng, 3, [[(C, 0), (C, 1)], [(C, 1), (C, 0)], [(C, 0)], []], 3)
+         + [(0.5, [(C, 0, 0), (C, 1, 1), (A, 1, 1), (C, 1, 1)])]),
+        ("vexpr", ["n", 3, 2], None, _signature_terms(rng, 3, [[(C, None), (C, None)]], 8)),
+        ("vexpr", ["m", 1, 4, 1, 1, 1], None,
+         [(complex(rng.standard_normal(), rng.standard_normal()), [(C, s0, a), (C, 1 - s0, b)])
+          for s0 in (0, 1) for a in range(4) for b in range(4)]),
+    ]
+    arrays, manifest = {}, []
+    for k, (mode, row, col, terms) in enumerate(cases):
+        args = [mode] + row + (col or [])
+        out = subprocess.run([DRIVER] + [str(a) for a in args], input=_terms_text(terms),
+                             check=True, capture_output=True, text=True).stdout
+        rows = np.array([(int(l.split()[1]), int(l.split()[2])) for l in out.splitlines()
+                         if l[0] == "S"], np.uint64).reshape(-1, 2)
+        cols = np.array([(int(l.split()[1]), int(l.split()[2])) for l in out.splitlines()
+                         if l[0] == "C"], np.uint64).reshape(-1, 2)
+        el = np.array([[float(x) for x in l.split()[1:]] for l in out.splitlines()
+                       if l[0] == "E"], np.float64).reshape(-1, 4)
+        name = f"{mode}{k}"
+        arrays[name + "__rows"] = rows
+        arrays[name + "__cols"] = cols
+        arrays[name + "__ij"] = el[:, :2].astype(np.int64)
+        arrays[name + "__val"] = el[:, 2] + 1j * el[:, 3]
+        arrays[name + "__coef"] = np.array([complex(c) for c, _ in terms], np.complex128)
+        arrays[name + "__nops"] = np.array([len(o) for _, o in terms], np.int64)
+        arrays[name + "__ops"] = np.array([v for _, o in terms for op in o for v in op],
+                                          np.int64).reshape(-1, 3)
+        manifest.append({"name": name, "kind": mode, "row": row, "col": col,
+                         "n_rows": int(len(rows)), "n_cols": int(len(cols)) if col else 1,
+                         "nnz": int(len(el))})
     return arrays, manifest
 
 

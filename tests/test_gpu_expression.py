@@ -314,3 +314,110 @@ def test_matrix_elements_density_on_single_orbital(cuda):
     basis = mb.FermionBasis.with_fixed_particle_number(1, 1)
     M = mb.compute_matrix_elements(basis, mb.density(mb.Spin.Up, 0)).to_dense()
     assert np.array_equal(M, np.array([[1, 0], [0, 0]], np.complex128))
+
+
+def _basis_from_spec(spec):
+    """Basis of a driver spec: ["s", L, nup, ndn], ["n", L, N] or
+    ["m", nd, d.., nup, ndn, K..]."""
+    if spec[0] == "s":
+        return mb.FermionBasis.sector(spec[1], spec[2], spec[3])
+    if spec[0] == "n":
+        return mb.FermionBasis.with_fixed_particle_number(spec[1], spec[2])
+    nd = spec[1]
+    dims = spec[2:2 + nd]
+    nu, ndn = spec[2 + nd], spec[3 + nd]
+    K = spec[4 + nd:4 + 2 * nd]
+    return mb.FermionBasis.with_fixed_particle_number_spin_momentum(
+        int(np.prod(dims)), nu + ndn, nu - ndn, dims, K)
+
+
+def test_rectangular_and_vector_elements_match_reference_golden(cuda):
+    """compute_rectangular_matrix_elements / compute_vector_elements through the
+    device against the compiled reference: creators, annihilators and pair
+    creation between sectors of different particle numbers, the square case, a
+    J(Q) between momentum sectors, bases of different kinds."""
+    gold = _gold()
+    n = 0
+    for c in _manifest():
+        if c["kind"] not in ("rexpr", "vexpr"):
+            continue
+        name = c["name"]
+        coef, nops, ops = gold[name + "__coef"], gold[name + "__nops"], gold[name + "__ops"]
+        terms, k = [], 0
+        for cc, m in zip(coef, nops):
+            terms.append((complex(cc), [tuple(int(v) for v in ops[k + i]) for i in range(int(m))]))
+            k += int(m)
+        e = expr_from_terms(terms)
+        rb = _basis_from_spec(c["row"])
+        assert [rb.state(i) for i in range(rb.size())] == \
+            [tuple(int(v) for v in r) for r in gold[name + "__rows"]], name
+        ij, val = gold[name + "__ij"], gold[name + "__val"]
+        if c["kind"] == "vexpr":
+            v = mb.compute_vector_elements(rb, e)
+            want = np.zeros(rb.size(), np.complex128)
+            want[ij[:, 0]] = val
+            assert np.abs(v - want).max() <= 1e-14 * max(1.0, np.abs(want).max()), name
+        else:
+            cb = _basis_from_spec(c["col"])
+            sp = mb.compute_rectangular_matrix_elements(rb, cb, e)
+            assert sp.shape == (rb.size(), cb.size())
+            r, cl, v = sp.entries()
+            order = np.lexsort((ij[:, 0], ij[:, 1]))
+            assert sp.nnz == len(ij), name
+            assert np.array_equal(r.astype(np.int64), ij[order, 0]), name
+            assert np.array_equal(cl.astype(np.int64), ij[order, 1]), name
+            assert np.abs(v - val[order]).max() <= 1e-14 * max(1.0, np.abs(v).max()), name
+        n += 1
+    assert n == 11
+
+
+@pytest.mark.parametrize("row,col", [(("n", 6, 4), ("n", 6, 3)), (("s", 6, 3, 2), ("s", 6, 2, 2)),
+                                     (("s", 5, 2, 2), ("n", 5, 3)), (("n", 7, 2), ("s", 7, 1, 1))])
+def test_rectangular_matrix_elements_match_oracle_beyond_reference_range(cuda, row, col):
+    """Random creators, c+ c+ c, c+ c and spin flips between two bases against
+    the oracle, and the adjoint relation <i|A|j> = conj(<j|A^dagger|i>) on the
+    device."""
+    L = row[1]
+    rng = np.random.default_rng(sum(row[1:]) * 13 + sum(col[1:]))
+    e = random_expression(rng, L, 6, 4, n_odd=10)
+    for _ in range(6):
+        e += FermionExpression({(int(rng.integers(2)) * 0x4000 | int(rng.integers(L)),):
+                                complex(rng.standard_normal(), rng.standard_normal())})
+    rb, cb = _basis_from_spec(list(row)), _basis_from_spec(list(col))
+    rs = [rb.state(i) for i in range(rb.size())]
+    cs = [cb.state(i) for i in range(cb.size())]
+    M = po.rectangular_matrix(L, rs, cs, oracle_terms(e))
+    sp = mb.compute_rectangular_matrix_elements(rb, cb, e)
+    r, c, v = sp.entries()
+    rr, cc, vv = po.sparse_column_major(M)
+    assert len(vv) > 0
+    assert np.array_equal(r, rr) and np.array_equal(c, cc)
+    assert np.abs(v - vv).max() <= 1e-13 * np.abs(vv).max()
+    adj = mb.compute_rectangular_matrix_elements(cb, rb, e.adjoint()).to_dense()
+    assert np.abs(adj.conj().T - sp.to_dense()).max() <= 1e-13 * np.abs(vv).max()
+
+
+@pytest.mark.parametrize("sizes,nu,nd,K,Q,d", [([4], 2, 2, [1], [1], 0), ([3, 3], 2, 1, [1, 2], [2, 0], 1)])
+def test_rectangular_current_between_momentum_sectors(cuda, sizes, nu, nd, K, Q, d):
+    """J_d(Q) = sum_{k, s} v_d(k) c+_{k+Q s} c_{k s} from sector K to K + Q as
+    compute_rectangular_matrix_elements does it in the reference's example
+    (hubbard_momentum_optical_conductivity.cpp:243), against the oracle's current
+    operator (pinned to the reference's symbolic current)."""
+    N = int(np.prod(sizes))
+    flat = lambda v: v[0] + (sizes[0] * v[1] if len(v) > 1 else 0)  # noqa: E731
+    Kf, Qf = flat(K), flat(Q)
+    terms = []
+    for k in range(N):
+        vk = po.velocity(sizes, 1.0, d, k)
+        if abs(vk) < 1e-15:
+            continue
+        for s in (0, 1):
+            terms.append((vk, [(0, s, po.momentum_add(sizes, k, Qf)), (1, s, k)]))
+    e = expr_from_terms(terms)
+    KQ = [(a + b) % n for a, b, n in zip(K, Q, sizes)]
+    src = mb.FermionBasis.with_fixed_particle_number_spin_momentum(N, nu + nd, nu - nd, sizes, K)
+    tgt = mb.FermionBasis.with_fixed_particle_number_spin_momentum(N, nu + nd, nu - nd, sizes, KQ)
+    got = mb.compute_rectangular_matrix_elements(tgt, src, e).to_dense()
+    want = po.current_operator(sizes, nu, nd, Kf, Qf, d, 1.0)
+    assert np.count_nonzero(want) > 0
+    assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()

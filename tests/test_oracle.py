@@ -631,3 +631,80 @@ def test_fixed_n_basis_matches_reference_golden():
         assert len(up) == math.comb(2 * c["L"], c["N"])
         n += 1
     assert n == 39
+
+
+def _rect_case(gold, c):
+    coef, nops, ops = gold[c["name"] + "__coef"], gold[c["name"] + "__nops"], gold[c["name"] + "__ops"]
+    terms, k = [], 0
+    for cc, m in zip(coef, nops):
+        terms.append((complex(cc), [tuple(int(v) for v in ops[k + i]) for i in range(int(m))]))
+        k += int(m)
+    spec = c["row"]
+    L = spec[1] if spec[0] in "sn" else int(np.prod(spec[2:2 + spec[1]]))
+    return L, terms
+
+
+def test_rectangular_and_vector_elements_match_reference_golden():
+    """rectangular_matrix / vector_elements (matrix_elements.h:31-51, 110-167)
+    against the compiled reference on eleven cases: creators, annihilators and
+    pair creation between sectors of different particle numbers, the square
+    case, a J(Q)-shaped operator between momentum sectors, bases of different
+    kinds, and vector elements on real-space, fixed-N and momentum bases."""
+    gold = np.load(os.path.join(GOLD, "ref_expression.npz"))
+    n = 0
+    for c in _manifest()["expression"]:
+        if c["kind"] not in ("rexpr", "vexpr"):
+            continue
+        L, terms = _rect_case(gold, c)
+        rows = [tuple(int(v) for v in r) for r in gold[c["name"] + "__rows"]]
+        if c["kind"] == "rexpr":
+            cols = [tuple(int(v) for v in r) for r in gold[c["name"] + "__cols"]]
+            M = po.rectangular_matrix(L, rows, cols, terms)
+        else:
+            M = po.vector_elements(L, rows, terms)[:, None]
+        G = np.zeros_like(M)
+        ij = gold[c["name"] + "__ij"]
+        G[ij[:, 0], ij[:, 1]] = gold[c["name"] + "__val"]
+        assert np.abs(M - G).max(initial=0) <= 1e-14 * max(1.0, np.abs(G).max(initial=0)), c["name"]
+        assert np.count_nonzero(M) == c["nnz"] > 0, c["name"]
+        n += 1
+    assert n == 11
+
+
+def test_rectangular_adjoint_and_pair_creation_identities():
+    """<i|A|j> between bases B1 <- B2 equals conj(<j|A^dagger|i>) between B2 <- B1
+    (random creators, annihilators and spin flips, L = 5, (2,1) <- (1,1) and
+    fixed N = 3 <- 2); c+_{a s} c+_{b s'}|0> is one basis state of norm 1 for
+    every pair of distinct modes, with the sign of the reordering."""
+    rng = np.random.default_rng(5)
+    L = 5
+    terms = []
+    for _ in range(12):
+        nc = int(rng.integers(1, 3))
+        ops = [(0, int(rng.integers(2)), int(rng.integers(L))) for _ in range(nc)]
+        ops += [(1, int(rng.integers(2)), int(rng.integers(L))) for _ in range(nc - 1)]
+        terms.append((complex(rng.standard_normal(), rng.standard_normal()), ops))
+    adj = [(c.conjugate(), [(1 - t, s, o) for t, s, o in reversed(ops)]) for c, ops in terms]
+    for b1, b2 in (((2, 1), (1, 1)), ((1, 2), (1, 1))):
+        s1 = list(zip(*po.basis_states(L, *b1)))
+        s2 = list(zip(*po.basis_states(L, *b2)))
+        A = po.rectangular_matrix(L, s1, s2, terms)
+        B = po.rectangular_matrix(L, s2, s1, adj)
+        assert np.count_nonzero(A) > 10 and np.abs(A - B.conj().T).max() < 1e-14
+    f3 = list(zip(*po.fixed_n_states(L, 3)))
+    f2 = list(zip(*po.fixed_n_states(L, 2)))
+    A = po.rectangular_matrix(L, f3, f2, terms)
+    B = po.rectangular_matrix(L, f2, f3, adj)
+    assert np.count_nonzero(A) > 10 and np.abs(A - B.conj().T).max() < 1e-14
+    idx = {st: i for i, st in enumerate(f2)}
+    for m1 in range(2 * L):
+        for m2 in range(2 * L):
+            if m1 == m2:
+                continue
+            (s1, o1), (s2, o2) = divmod(m1, L), divmod(m2, L)
+            v = po.vector_elements(L, f2, [(1.0, [(0, s1, o1), (0, s2, o2)])])
+            up = (1 << o1 if s1 == 0 else 0) | (1 << o2 if s2 == 0 else 0)
+            dn = (1 << o1 if s1 == 1 else 0) | (1 << o2 if s2 == 1 else 0)
+            want = np.zeros(len(f2))
+            want[idx[(up, dn)]] = 1.0 if m1 < m2 else -1.0  # canonical order: ascending modes
+            assert np.array_equal(v, want), (m1, m2)
