@@ -248,6 +248,7 @@ class HubbardOperator {
   const mb_hubbard *handle() const { return h_; }
   // takes ownership of a handle made by another factory entry point
   static HubbardOperator adopt(mb_hubbard *h, size_t sites) { return HubbardOperator(h, sites); }
+  size_t sites() const { return sites_; }
 
  private:
   HubbardOperator(mb_hubbard *h, size_t sites) : h_(h), sites_(sites) {}
@@ -536,6 +537,53 @@ struct ExpressionTerm {
   std::complex<double> c;
   std::vector<uint16_t> ops;
 };
+
+// The reference's operator algebra (linear_operator.h:93-281) on device
+// operators: -A, A + B, A - B, A * B (B applied first), A * c, c * A,
+// Shifted(A, c) = A + c I, NegShift(A, c) = -A + c I, Identity(basis).  The
+// result references its operands (the reference's wrappers copy them): keep
+// them alive while it is used.
+inline HubbardOperator operator-(const HubbardOperator &a) {
+  mb_hubbard *h = nullptr;
+  check(mb_operator_negated(a.handle(), &h));
+  return HubbardOperator::adopt(h, a.sites());
+}
+inline HubbardOperator operator*(const HubbardOperator &a, std::complex<double> c) {
+  mb_hubbard *h = nullptr;
+  check(mb_operator_scaled(a.handle(), c.real(), c.imag(), &h));
+  return HubbardOperator::adopt(h, a.sites());
+}
+inline HubbardOperator operator*(std::complex<double> c, const HubbardOperator &a) { return a * c; }
+inline HubbardOperator Shifted(const HubbardOperator &a, std::complex<double> c) {
+  mb_hubbard *h = nullptr;
+  check(mb_operator_shifted(a.handle(), c.real(), c.imag(), &h));
+  return HubbardOperator::adopt(h, a.sites());
+}
+inline HubbardOperator NegShift(const HubbardOperator &a, std::complex<double> c) {
+  mb_hubbard *h = nullptr;
+  check(mb_operator_neg_shifted(a.handle(), c.real(), c.imag(), &h));
+  return HubbardOperator::adopt(h, a.sites());
+}
+inline HubbardOperator operator+(const HubbardOperator &a, const HubbardOperator &b) {
+  mb_hubbard *h = nullptr;
+  check(mb_operator_sum(a.handle(), b.handle(), &h));
+  return HubbardOperator::adopt(h, a.sites());
+}
+inline HubbardOperator operator-(const HubbardOperator &a, const HubbardOperator &b) {
+  mb_hubbard *h = nullptr;
+  check(mb_operator_difference(a.handle(), b.handle(), &h));
+  return HubbardOperator::adopt(h, a.sites());
+}
+inline HubbardOperator operator*(const HubbardOperator &a, const HubbardOperator &b) {
+  mb_hubbard *h = nullptr;
+  check(mb_operator_composed(a.handle(), b.handle(), &h));
+  return HubbardOperator::adopt(h, a.sites());
+}
+inline HubbardOperator Identity(const FermionBasis &basis) {
+  mb_hubbard *h = nullptr;
+  check(mb_operator_identity(basis.handle(), &h));
+  return HubbardOperator::adopt(h, basis.orbitals);
+}
 
 // compute_matrix_elements(basis, A) as a device operator (H.v, Lanczos,
 // eigenpair, time evolution, energy/residual); terms merged by key as the

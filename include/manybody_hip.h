@@ -362,6 +362,26 @@ int mb_fixed_n_basis_create(int orbitals, int particles, mb_basis **out);
 int mb_expression_create(const mb_basis *basis, int64_t n_terms, const double *coeffs,
                          const int64_t *offsets, const uint16_t *ops, mb_hubbard **out);
 
+/* The reference's LinearOperator algebra (linear_operator.h:93-281) on operator
+ * handles: Negated -A, Scaled c A, Shifted A + c I, NegShift -A + c I, Sum
+ * A + B, Difference A - B, Composed A B (B applied first) and Identity(basis).
+ * c is complex (re, im).  The result is an operator handle that works with the
+ * FP64 H.v, order, fill, apply_ref, Lanczos, eigenpair, time-evolution, Exp,
+ * power-method, bounds and energy/residual entry points; composites nest.
+ * The operands are referenced, not copied: they must outlive the composite
+ * (the reference's wrappers hold copies).  Sum, Difference and Composed need
+ * operands on bases of the same kind and parameters (MB_ERR_INVALID
+ * otherwise).  Each H.v of a composite takes its scratch vectors with
+ * hipMallocAsync on the call's stream. */
+int mb_operator_negated(const mb_hubbard *a, mb_hubbard **out);
+int mb_operator_scaled(const mb_hubbard *a, double re, double im, mb_hubbard **out);
+int mb_operator_shifted(const mb_hubbard *a, double re, double im, mb_hubbard **out);
+int mb_operator_neg_shifted(const mb_hubbard *a, double re, double im, mb_hubbard **out);
+int mb_operator_sum(const mb_hubbard *a, const mb_hubbard *b, mb_hubbard **out);
+int mb_operator_difference(const mb_hubbard *a, const mb_hubbard *b, mb_hubbard **out);
+int mb_operator_composed(const mb_hubbard *a, const mb_hubbard *b, mb_hubbard **out);
+int mb_operator_identity(const mb_basis *basis, mb_hubbard **out);
+
 /* The sparse matrix of compute_matrix_elements (arma::sp_cx_mat): nonzeros in
  * the reference's basis index order, column-major with rows ascending (SpMat
  * iteration order), built on the device from an expression operator. */

@@ -115,6 +115,14 @@ _SIGS = {
     "mb_compute_rectangular_matrix_elements": (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp, _vp,
                                                          C.POINTER(_vp), _vp]),
     "mb_compute_vector_elements": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, _vp, _vp]),
+    "mb_operator_negated": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "mb_operator_scaled": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(_vp)]),
+    "mb_operator_shifted": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(_vp)]),
+    "mb_operator_neg_shifted": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(_vp)]),
+    "mb_operator_sum": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+    "mb_operator_difference": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+    "mb_operator_composed": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
+    "mb_operator_identity": (C.c_int, [_vp, C.POINTER(_vp)]),
     "mb_sparse_create": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, _vp, _vp, _vp,
                                    C.POINTER(_vp)]),
     "mb_sparse_destroy": (None, [_vp]),
@@ -358,6 +366,59 @@ def compute_matrix_elements(basis_or_op, expression=None):
     return SparseMatrix(h)
 
 
+def _composite(fn, operands, *args):
+    h = C.c_void_p()
+    _check(fn(*[o._h for o in operands], *args, C.byref(h)))
+    op = HubbardOperator(h, operands[0].basis, None)
+    op._operands = operands  # the composite references its operands
+    return op
+
+
+def Negated(op: "HubbardOperator") -> "HubbardOperator":
+    """Negated (linear_operator.h:93-105): -A."""
+    return _composite(lib().mb_operator_negated, (op,))
+
+
+def Scaled(op: "HubbardOperator", scale) -> "HubbardOperator":
+    """Scaled (linear_operator.h:107-120): c A, c complex."""
+    c = complex(scale)
+    return _composite(lib().mb_operator_scaled, (op,), c.real, c.imag)
+
+
+def Shifted(op: "HubbardOperator", shift) -> "HubbardOperator":
+    """Shifted (linear_operator.h:122-135): A + c I."""
+    c = complex(shift)
+    return _composite(lib().mb_operator_shifted, (op,), c.real, c.imag)
+
+
+def NegShift(op: "HubbardOperator", shift) -> "HubbardOperator":
+    """NegShift (linear_operator.h:137-150): -A + c I."""
+    c = complex(shift)
+    return _composite(lib().mb_operator_neg_shifted, (op,), c.real, c.imag)
+
+
+def Sum(a: "HubbardOperator", b: "HubbardOperator") -> "HubbardOperator":
+    """Sum (linear_operator.h:152-172): A + B on bases of the same kind."""
+    return _composite(lib().mb_operator_sum, (a, b))
+
+
+def Difference(a: "HubbardOperator", b: "HubbardOperator") -> "HubbardOperator":
+    """Difference (linear_operator.h:174-194): A - B."""
+    return _composite(lib().mb_operator_difference, (a, b))
+
+
+def Composed(a: "HubbardOperator", b: "HubbardOperator") -> "HubbardOperator":
+    """Composed (linear_operator.h:196-216): A B, B applied first."""
+    return _composite(lib().mb_operator_composed, (a, b))
+
+
+def Identity(basis: FermionBasis) -> "HubbardOperator":
+    """Identity (linear_operator.h:218-228) on the basis."""
+    h = C.c_void_p()
+    _check(lib().mb_operator_identity(basis._h, C.byref(h)))
+    return HubbardOperator(h, basis, None)
+
+
 def compute_rectangular_matrix_elements(row_basis: FermionBasis, col_basis: FermionBasis,
                                         expression):
     """compute_rectangular_matrix_elements (matrix_elements.h:110-167): the sparse
@@ -402,6 +463,25 @@ class HubbardOperator:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.mb_hubbard_destroy(self._h)
             self._h = None
+
+    # the reference's operator algebra (linear_operator.h:232-281): -A, A + B,
+    # A - B, A * B (composition), scalar * A and A * scalar
+    def __neg__(self):
+        return Negated(self)
+
+    def __add__(self, other):
+        return Sum(self, other)
+
+    def __sub__(self, other):
+        return Difference(self, other)
+
+    def __mul__(self, other):
+        if isinstance(other, HubbardOperator):
+            return Composed(self, other)
+        return Scaled(self, other)
+
+    def __rmul__(self, other):
+        return Scaled(self, other)
 
     def dimension(self) -> int:
         return int(lib().mb_hubbard_dimension(self._h))

@@ -15,6 +15,7 @@
 #include "expression.hpp"
 #include "fixed_n.hpp"
 #include "rectangular.hpp"
+#include "composite.hpp"
 
 #include <cstdio>
 #include <map>
@@ -33,19 +34,25 @@ struct mb_hubbard {
   std::unique_ptr<mb::RealHvOp> real_op;
   std::unique_ptr<mb::MomentumHubbard> mom;
   std::unique_ptr<mb::ExpressionOperator> expr;  // a compiled FermionExpression
+  std::unique_ptr<mb::CompositeOp> comp;          // operator algebra over other handles
   const mb::HvOp &op() const {
+    if (comp) return *comp;
     if (mom) return *mom;
     if (expr) return *expr;
     return *real_op;
   }
+  // built on the basis handle alone (expression or composite operators)
+  bool on_basis() const { return expr || comp; }
   // the fixed-momentum sector the operator acts on, or null (real space)
   const mb::MomentumSector *msec() const {
     if (mom) return &mom->sector();
-    if (expr && basis->ms) return basis->ms.get();
+    if (on_basis() && basis->ms) return basis->ms.get();
     return nullptr;
   }
   // the fixed-N basis the operator acts on, or null
-  const mb::FixedNBasis *fsec() const { return (expr && basis->fn) ? basis->fn.get() : nullptr; }
+  const mb::FixedNBasis *fsec() const {
+    return (on_basis() && basis->fn) ? basis->fn.get() : nullptr;
+  }
   // the device position -> reference index map of a grouped basis (fixed K or
   // fixed N), uploaded on first use; null for a real-space sector
   const int64_t *ref_map() const {
@@ -63,8 +70,8 @@ struct mb_hubbard {
   }
   // the real-space sector the operator acts on, or null (momentum, fixed N)
   const mb::Sector *rsec() const {
-    if (mom || (expr && (basis->ms || basis->fn))) return nullptr;
-    if (expr) {
+    if (mom || (on_basis() && (basis->ms || basis->fn))) return nullptr;
+    if (on_basis()) {
       auto &sec = const_cast<mb::Sector &>(basis->sec);
       sec.ensure_device();
       return &sec;
@@ -136,7 +143,44 @@ const mb::Hubbard &H(const mb_hubbard *h) {
   need(h != nullptr, "null model handle");
   need(!h->mom, "not available for momentum-sector operators");
   need(!h->expr, "not available for expression operators");
+  need(!h->comp, "not available for composite operators");
   return h->h;
+}
+
+// whether two bases have the same device layout (same kind and parameters)
+bool same_layout(const mb_basis *a, const mb_basis *b) {
+  if (a == b) return true;
+  if ((a->ms != nullptr) != (b->ms != nullptr) || (a->fn != nullptr) != (b->fn != nullptr))
+    return false;
+  if (a->ms) {
+    const auto &p = *a->ms, &q = *b->ms;
+    return p.L == q.L && p.n_up == q.n_up && p.n_dn == q.n_dn && p.K == q.K &&
+           p.ndim == q.ndim && p.dims[0] == q.dims[0] && p.dims[1] == q.dims[1] &&
+           p.dims[2] == q.dims[2];
+  }
+  if (a->fn) return a->fn->L == b->fn->L && a->fn->N == b->fn->N;
+  return a->sec.L == b->sec.L && a->sec.n_up == b->sec.n_up && a->sec.n_dn == b->sec.n_dn;
+}
+
+int make_composite(mb::CompositeOp::Kind k, const mb_hubbard *a, const mb_hubbard *b,
+                   double re, double im, mb_hubbard **out) {
+  return guarded([&] {
+    need(a && out, "null argument");
+    *out = nullptr;
+    const bool binary = k == mb::CompositeOp::Kind::Sum ||
+                        k == mb::CompositeOp::Kind::Difference ||
+                        k == mb::CompositeOp::Kind::Composed;
+    if (binary) {
+      need(b != nullptr, "null argument");
+      need(a->op().dim() == b->op().dim(), "operand dimensions differ");
+      need(same_layout(a->basis, b->basis), "operands act on different bases");
+    }
+    auto h = std::make_unique<mb_hubbard>();
+    h->basis = a->basis;
+    h->comp = std::make_unique<mb::CompositeOp>(k, &a->op(), binary ? &b->op() : nullptr,
+                                                std::complex<double>(re, im), a->op().dim());
+    *out = h.release();
+  });
 }
 
 // the operator the Krylov layer runs on (either kind)
@@ -414,7 +458,7 @@ int mb_apply_ref_c128(const mb_hubbard *h, const void *x_ref, void *y_ref, void 
       mb::launch_mom_perm(map, D, w1, (double *)y_ref, false, S(stream));
       return;
     }
-    if (h->expr) {
+    if (h->on_basis()) {
       const int64_t D = h->op().dim();
       double *w0 = (double *)work, *w1 = w0 + 2 * D;
       mb::launch_gather_c128(mb::perm_of(*h->rsec()), x_ref, w0, S(stream));
@@ -993,6 +1037,40 @@ int mb_compute_vector_elements(const mb_basis *basis, int64_t n_terms, const dou
     check_terms(n_terms, coeffs, offsets, ops);
     mb::vector_elements(basis_ref(basis), n_terms, coeffs, offsets, ops,
                         reinterpret_cast<std::complex<double> *>(out), S(stream));
+  });
+}
+
+int mb_operator_negated(const mb_hubbard *a, mb_hubbard **out) {
+  return make_composite(mb::CompositeOp::Kind::Negated, a, nullptr, 0.0, 0.0, out);
+}
+int mb_operator_scaled(const mb_hubbard *a, double re, double im, mb_hubbard **out) {
+  return make_composite(mb::CompositeOp::Kind::Scaled, a, nullptr, re, im, out);
+}
+int mb_operator_shifted(const mb_hubbard *a, double re, double im, mb_hubbard **out) {
+  return make_composite(mb::CompositeOp::Kind::Shifted, a, nullptr, re, im, out);
+}
+int mb_operator_neg_shifted(const mb_hubbard *a, double re, double im, mb_hubbard **out) {
+  return make_composite(mb::CompositeOp::Kind::NegShift, a, nullptr, re, im, out);
+}
+int mb_operator_sum(const mb_hubbard *a, const mb_hubbard *b, mb_hubbard **out) {
+  return make_composite(mb::CompositeOp::Kind::Sum, a, b, 0.0, 0.0, out);
+}
+int mb_operator_difference(const mb_hubbard *a, const mb_hubbard *b, mb_hubbard **out) {
+  return make_composite(mb::CompositeOp::Kind::Difference, a, b, 0.0, 0.0, out);
+}
+int mb_operator_composed(const mb_hubbard *a, const mb_hubbard *b, mb_hubbard **out) {
+  return make_composite(mb::CompositeOp::Kind::Composed, a, b, 0.0, 0.0, out);
+}
+int mb_operator_identity(const mb_basis *basis, mb_hubbard **out) {
+  return guarded([&] {
+    need(basis && out, "null argument");
+    *out = nullptr;
+    auto h = std::make_unique<mb_hubbard>();
+    h->basis = basis;
+    h->comp = std::make_unique<mb::CompositeOp>(mb::CompositeOp::Kind::Identity, nullptr,
+                                                nullptr, std::complex<double>(0.0, 0.0),
+                                                basis->size());
+    *out = h.release();
   });
 }
 

@@ -1492,9 +1492,79 @@ void launch_strip(const HvTables &t, int P, const T *x, T *y, const int *flags, 
   MB_HIP(hipGetLastError());
 }
 
+// Operator algebra: h = alpha z1 + beta z2 + gamma x per complex element
+// (terms with a null operand left out), then the epilogue on both planes; one
+// partial sum per workgroup (fixed-order reductions).
+template <typename T, Epi E>
+__global__ __launch_bounds__(kBlock) void combine_kernel(int64_t D, const T *__restrict__ z1,
+                                                         const T *__restrict__ z2,
+                                                         const T *__restrict__ x,
+                                                         T *__restrict__ y, CombineCoef c,
+                                                         EpiArgs a) {
+  __shared__ double red[2][1024 / kWave];
+  if constexpr (E == Epi::Lanczos) {
+    if (a.flags[0]) return;
+  }
+  const T bp = beta_prev<T, E>(a);
+  const T ar = (T)c.ar, ai = (T)c.ai, br = (T)c.br, bi = (T)c.bi, gr = (T)c.gr, gi = (T)c.gi;
+  Partial2 acc;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < D;
+       i += (int64_t)gridDim.x * kBlock) {
+    const T xr = x[i], xi = x[D + i];
+    T hr = T(0), hi = T(0);
+    if (z1) {
+      const T zr = z1[i], zi = z1[D + i];
+      hr = ar * zr - ai * zi;
+      hi = ar * zi + ai * zr;
+    }
+    if (z2) {
+      const T zr = z2[i], zi = z2[D + i];
+      hr += br * zr - bi * zi;
+      hi += br * zi + bi * zr;
+    }
+    if (c.with_x) {
+      hr += gr * xr - gi * xi;
+      hi += gr * xi + gi * xr;
+    }
+    epilogue<T, E>(hr, xr, i, y, a, bp, acc);
+    epilogue<T, E>(hi, xi, D + i, y, a, bp, acc);
+  }
+  row_reduce<T, E>(acc, a, blockIdx.x, red);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ launchers
+
+int64_t combine_slots(int64_t D) { return (int64_t)grid_for((D + 1) / 2, 512); }
+
+template <typename T>
+int64_t launch_combine(int64_t D, const T *z1, const T *z2, const T *x, T *y, const CombineCoef &c,
+                       Epi epi, const EpiArgs &a, hipStream_t s) {
+  const int64_t slots = combine_slots(D);
+  const dim3 g((unsigned)(2 * slots));
+  switch (epi) {
+    case Epi::Plain:
+      hipLaunchKernelGGL((combine_kernel<T, Epi::Plain>), g, dim3(kBlock), 0, s, D, z1, z2, x, y, c, a);
+      break;
+    case Epi::Lanczos:
+      hipLaunchKernelGGL((combine_kernel<T, Epi::Lanczos>), g, dim3(kBlock), 0, s, D, z1, z2, x, y, c, a);
+      break;
+    case Epi::Recon:
+      hipLaunchKernelGGL((combine_kernel<T, Epi::Recon>), g, dim3(kBlock), 0, s, D, z1, z2, x, y, c, a);
+      break;
+    case Epi::Cheb:
+      throw std::invalid_argument("the Chebyshev epilogue is not available for composite operators");
+  }
+  MB_HIP(hipGetLastError());
+  return slots;  // partial slots per plane: the launch wrote 2 slots
+}
+template int64_t launch_combine<double>(int64_t, const double *, const double *, const double *,
+                                        double *, const CombineCoef &, Epi, const EpiArgs &,
+                                        hipStream_t);
+template int64_t launch_combine<float>(int64_t, const float *, const float *, const float *,
+                                       float *, const CombineCoef &, Epi, const EpiArgs &,
+                                       hipStream_t);
 
 int64_t hv_partials(const HvTables &t, int P) {
   const int64_t two = (int64_t)P * t.d_up, gen = (int64_t)P * kGenericGrid;

@@ -265,6 +265,19 @@ int64_t launch_hv(const HvTables &t, int P, const T *x, T *y, Epi epi, const Epi
 // Number of row workgroups (= partial slots) for P planes.
 int64_t hv_partials(const HvTables &t, int P);
 
+// h = alpha z1 + beta z2 + gamma x (complex coefficients; a null z1 / z2 or
+// with_x == 0 drops that term) over planar complex vectors of D elements, then
+// the epilogue (Plain, Lanczos or Recon).  Returns the partial slots per plane
+// (the launch writes twice that many); combine_slots(D) of them.
+struct CombineCoef {
+  double ar = 0, ai = 0, br = 0, bi = 0, gr = 0, gi = 0;
+  int with_x = 0;
+};
+int64_t combine_slots(int64_t D);
+template <typename T>
+int64_t launch_combine(int64_t D, const T *z1, const T *z2, const T *x, T *y, const CombineCoef &c,
+                       Epi epi, const EpiArgs &a, hipStream_t s);
+
 // Number of blocks used by the elementwise deterministic reductions.
 constexpr int kRedBlocks = 512;
 

@@ -331,6 +331,24 @@ int main() {
       CHECK(vec[i] == (i == col.index_of(1, 2) ? std::complex<double>(-2.0, 0.0) : 0.0));
   }
 
+  {
+    CASE("operator_algebra");
+    // ring L = 8, 4 + 4, U = 0: E0 = -9.656854249492380; 2 H + 1 has 2 E0 + 1
+    // and H - H is zero
+    auto basis = FermionBasis::with_fixed_particle_number_and_spin(8, 8, 0);
+    auto H = HubbardModel(1.0, 0.0, 8).op(basis);
+    auto twoH = 2.0 * H;
+    auto A = Shifted(twoH, 1.0);
+    auto e = find_min_eigenpair(A, 150, H.gaussian(2));
+    CHECK(std::abs(e.value - (2.0 * -9.656854249492380 + 1.0)) < 1e-9);
+    auto Z = H - H;
+    auto x = H.gaussian(5);
+    auto z = Z.apply(x).to_host();
+    double m = 0.0;
+    for (auto v : z) m = std::max(m, std::abs(v));
+    CHECK(m == 0.0);
+  }
+
   std::printf(g_fail ? "FAILED (%d)\n" : "ALL PASSED\n", g_fail);
   return g_fail ? 1 : 0;
 }

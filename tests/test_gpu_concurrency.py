@@ -59,10 +59,11 @@ def operators():
     mom = mb.HubbardMomentumFactorizedFixedK(mbasis, [3, 3], 1.0, 6.0, [1, 2])
     e = mb.HubbardModel(1.0, 4.0, 10).hamiltonian() + mb.hopping(0.3j, 0, 5, mb.Spin.Up)
     expr = mb.FermionExpressionOperator(mb.FermionBasis.sector(10, 4, 4), e)
-    return {"real": real, "momentum": mom, "expression": expr}
+    comp = mb.Shifted(real, 0.5) + 0.25 * real  # operator algebra: scratch per call
+    return {"real": real, "momentum": mom, "expression": expr, "composite": comp}
 
 
-@pytest.mark.parametrize("kind", ["real", "momentum", "expression"])
+@pytest.mark.parametrize("kind", ["real", "momentum", "expression", "composite"])
 def test_concurrent_calls_match_serial(cuda, kind):
     op = operators()[kind]
     serial = [work(op, k) for k in range(THREADS)]
