@@ -1103,6 +1103,13 @@ class HubbardModelMomentum:
         from .expression import hubbard_momentum_hamiltonian
         return hubbard_momentum_hamiltonian(self.t, self.u, self.sizes)
 
+    def current(self, Q, direction: int):
+        """current(Q, direction) (hubbard_model_momentum.h:52-91): J_d(Q) as an
+        expression, for compute_rectangular_matrix_elements between sectors K and
+        K + Q."""
+        from .expression import hubbard_momentum_current
+        return hubbard_momentum_current(self.t, self.sizes, Q, direction)
+
 
 from .expression import (FermionExpression, SparseMatrix, Spin, annihilation,  # noqa: E402
                          creation, density, density_density, hopping, one_body, two_body)

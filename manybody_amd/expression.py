@@ -231,6 +231,34 @@ def hubbard_kinetic(t: float, sizes) -> FermionExpression:
     return e
 
 
+def hubbard_momentum_current(t: float, sizes, Q, direction: int) -> FermionExpression:
+    """HubbardModelMomentum::current(Q, direction) (hubbard_model_momentum.h:52-91):
+    sum_k v_d(k) (c+_{k+Q,up} c_{k,up} + c+_{k+Q,dn} c_{k,dn}) with the group velocity
+    v_d(k) = 2t (2 pi / L_d) sin(2 pi k_d / L_d) (:38-48); terms with v exactly 0
+    are skipped; k = kx + Lx ky + Lx Ly kz."""
+    sizes = [int(v) for v in sizes]
+    Q = [int(v) for v in Q]
+    if len(Q) != len(sizes):
+        raise ValueError("Momentum transfer Q must have the same dimension as the system.")
+    if direction < 0 or direction >= len(sizes):
+        raise ValueError("Direction index out of bounds.")
+    d3 = sizes + [1] * (3 - len(sizes))
+    q3 = Q + [0] * (3 - len(Q))
+    N = int(np.prod(sizes))
+    e = FermionExpression()
+    for k in range(N):
+        kc = [k % d3[0], (k // d3[0]) % d3[1], k // (d3[0] * d3[1])]
+        L = sizes[direction]
+        v = 2.0 * t * (2.0 * np.pi / L) * np.sin(2.0 * np.pi * kc[direction] / L)
+        if v == 0.0:
+            continue
+        kq = [(kc[i] + q3[i]) % d3[i] for i in range(3)]
+        kqf = kq[0] + d3[0] * (kq[1] + d3[1] * kq[2])
+        e += _mono(complex(v), _op(False, 0, kqf), _op(True, 0, k))
+        e += _mono(complex(v), _op(False, 1, kqf), _op(True, 1, k))
+    return e
+
+
 def hubbard_momentum_hamiltonian(t: float, u: float, sizes) -> FermionExpression:
     """HubbardModelMomentum::hamiltonian() (hubbard_model_momentum.h:119-175):
     sum_k eps(k) (n_k,up + n_k,dn) + (U/N) sum_{k1,k2,q}

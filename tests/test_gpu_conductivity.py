@@ -182,3 +182,84 @@ def test_correlator_with_expression_hamiltonians(cuda):
     assert np.abs(C1 - C0).max() <= 1e-10 * abs(C0[0])
     with pytest.raises(ValueError):  # operators on other bases
         mb.optical_correlator(hq, hk, J, 1, 1.0, 0.1, 2, 1)
+
+
+def _mbasis(sizes, n, K):
+    return mb.FermionBasis.with_fixed_particle_number_spin_momentum(
+        int(np.prod(sizes)), n, 0, sizes, K)
+
+
+def _kflat(sizes, K):
+    return K[0] + (sizes[0] * K[1] if len(K) > 1 else 0)
+
+
+def _check_against_rectangular(cuda, sizes, K, Q, direction, n, rng, t=1.0):
+    """CurrentOperatorFixedK (device order) against compute_rectangular_matrix_
+    elements of the symbolic current, both directions, absolute 1e-10 as the
+    reference checks (test_current_operator_fixed_k.cpp:48-172)."""
+    KQ = [(k + q) % s for k, q, s in zip(K, Q, sizes)]
+    bk, bq = _mbasis(sizes, n, K), _mbasis(sizes, n, KQ)
+    if bk.size() == 0 or bq.size() == 0:
+        return False
+    model = mb.HubbardModelMomentum(t, 4.0, sizes)
+    Jd = mb.compute_rectangular_matrix_elements(bq, bk, model.current(Q, direction)).to_dense()
+    J = mb.CurrentOperatorFixedK(bk, bq, sizes, t, _kflat(sizes, Q), direction)
+    hk = mb.HubbardMomentumFactorizedFixedK(bk, sizes, t, 4.0, K)
+    hq = mb.HubbardMomentumFactorizedFixedK(bq, sizes, t, 4.0, KQ)
+    v = rng.standard_normal(bk.size()) + 1j * rng.standard_normal(bk.size())
+    assert np.abs(ref(hq, J.apply(dev(hk, v, cuda))) - Jd @ v).max() < 1e-10
+    u = rng.standard_normal(bq.size()) + 1j * rng.standard_normal(bq.size())
+    assert np.abs(ref(hk, J.adjoint_apply(dev(hq, u, cuda))) - Jd.conj().T @ u).max() < 1e-10
+    return True
+
+
+def test_reference_current_operator_cases_1d(cuda):
+    """test_current_operator_fixed_k.cpp:26-129: the 4-site ring with 2 particles
+    at S_z = 0, every K and Q, the operator and its adjoint against the
+    rectangular matrix of HubbardModelMomentum::current."""
+    rng = np.random.default_rng(11)
+    bk, bq = _mbasis([4], 2, [0]), _mbasis([4], 2, [1])
+    J = mb.CurrentOperatorFixedK(bk, bq, [4], 1.0, 1, 0)
+    assert J.source_dimension() == bk.size() and J.target_dimension() == bq.size()
+    done = sum(_check_against_rectangular(cuda, [4], [K], [Q], 0, 2, rng)
+               for K in range(4) for Q in range(4))
+    assert done > 8
+
+
+def test_reference_current_operator_cases_2d_and_q0(cuda):
+    """test_current_operator_fixed_k.cpp:131-241: 3x3 with 2 particles, Q = (1, 1),
+    both directions; Q = 0 is diagonal (here: J(0) = sum_k v(k) n_k, the
+    rectangular matrix of one sector with itself); the adjoint identity
+    <u|J v> = <J^dagger u|v> at Q = 2 on the ring."""
+    rng = np.random.default_rng(12)
+    for d in (0, 1):
+        assert _check_against_rectangular(cuda, [3, 3], [1, 0], [1, 1], d, 2, rng)
+    b = _mbasis([4], 2, [1])
+    Jd = mb.compute_rectangular_matrix_elements(
+        b, b, mb.HubbardModelMomentum(1.0, 4.0, [4]).current([0], 0)).to_dense()
+    assert np.abs(Jd - np.diag(np.diag(Jd))).max() < 1e-10
+    assert _check_against_rectangular(cuda, [4], [1], [0], 0, 2, rng)
+    bk, bq = _mbasis([4], 2, [1]), _mbasis([4], 2, [3])
+    J = mb.CurrentOperatorFixedK(bk, bq, [4], 1.0, 2, 0)
+    hk = mb.HubbardMomentumFactorizedFixedK(bk, [4], 1.0, 4.0, [1])
+    hq = mb.HubbardMomentumFactorizedFixedK(bq, [4], 1.0, 4.0, [3])
+    v = rng.standard_normal(bk.size()) + 1j * rng.standard_normal(bk.size())
+    u = rng.standard_normal(bq.size()) + 1j * rng.standard_normal(bq.size())
+    jv = ref(hq, J.apply(dev(hk, v, cuda)))
+    jtu = ref(hk, J.adjoint_apply(dev(hq, u, cuda)))
+    assert abs(np.vdot(u, jv) - np.vdot(jtu, v)) < 1e-10
+
+
+def test_reference_current_velocity_values(cuda):
+    """test_current_operator_fixed_k.cpp:276-300: the symbolic current's
+    coefficients are v_d(k) = 2t (2 pi / L) sin(2 pi k / L), k = 0 and k = L/2
+    skipped (v = 0, or below 1000 eps after sin(pi) rounding)."""
+    t, L = 1.3, 6
+    e = mb.HubbardModelMomentum(t, 2.0, [L]).current([1], 0)
+    for k in range(L):
+        v = 2 * t * (2 * np.pi / L) * np.sin(2 * np.pi * k / L)
+        key = (((k + 1) % L), 0x8000 | k)
+        if abs(v) < 1e-12:
+            assert key not in e.terms
+        else:
+            assert abs(e.terms[key] - v) < 1e-15

@@ -324,3 +324,19 @@ def test_fixed_n_basis_reference_unit_cases():
     assert b.state(0) == (1, 0) and b.state(1) == (0, 1)
     b = mb.FermionBasis.with_fixed_particle_number(1, 2)
     assert b.size() == 1 and b.state(0) == (1, 1)
+
+
+def test_momentum_current_expression_validation():
+    """HubbardModelMomentum::current (hubbard_model_momentum.h:52-60): Q of the
+    wrong dimension and an out-of-range direction are invalid arguments; J(Q)
+    is sum_k v(k) c+_{k+Q} c_k for both spins."""
+    m = mb.HubbardModelMomentum(1.0, 4.0, [4, 2])
+    with pytest.raises(ValueError):
+        m.current([1], 0)
+    with pytest.raises(ValueError):
+        m.current([1, 0], 2)
+    j = m.current([1, 1], 0)
+    assert len(j.terms) == 2 * 4  # kx in {1, 3} (v != 0) x ky in {0, 1} x 2 spins
+    # k = (1, 0) -> k + Q = (2, 1), flat 2 + 4 * 1 = 6; v = 2 (2 pi / 4) sin(pi / 2) = pi
+    assert abs(j.terms[(6, 0x8000 | 1)] - np.pi) < 1e-15
+    assert abs(j.terms[(0x4000 | 6, 0xC000 | 1)] - np.pi) < 1e-15
