@@ -99,18 +99,49 @@ struct Partial2 {
   double s0 = 0.0, s1 = 0.0;
 };
 
+// Per-launch constants of the epilogue: bp multiplies prev; sx scales H x and
+// x when the Lanczos vectors are kept unnormalised (EpiArgs::lazy).
+template <typename T>
+struct EpiK {
+  T bp = T(0), sx = T(1);
+};
+
+template <typename T, Epi E>
+__device__ __forceinline__ EpiK<T> epi_consts(const EpiArgs &a) {
+  EpiK<T> k;
+  if constexpr (E == Epi::Lanczos) {
+    if (a.step > 0) {
+      const double b1 = a.beta_dev[a.step - 1];
+      if (a.lazy) {  // x = b1 v, prev = b2 v_prev (b2 = beta[step - 2], 1 at step 1)
+        const double b2 = a.step > 1 ? a.beta_dev[a.step - 2] : 1.0;
+        k.sx = (T)(1.0 / b1);
+        k.bp = (T)(b1 / b2);
+      } else {
+        k.bp = (T)b1;
+      }
+    }
+  } else if constexpr (E == Epi::Recon) {
+    k.bp = (T)a.beta;
+  }
+  return k;
+}
+
 // Applies the epilogue to r = (H x)[i] at plane offset i; xv = x[i].
 template <typename T, Epi E>
 __device__ __forceinline__ void epilogue(T r, T xv, int64_t i, T *__restrict__ y,
-                                         const EpiArgs &a, T bp, Partial2 &acc) {
+                                         const EpiArgs &a, const EpiK<T> &bp, Partial2 &acc) {
   if constexpr (E == Epi::Plain) {
     y[i] = r;
   } else if constexpr (E == Epi::Lanczos) {
-    if (bp != T(0)) r -= bp * static_cast<const T *>(a.prev)[i];
+    if (a.lazy) {
+      r *= bp.sx;
+      xv *= bp.sx;
+    }
+    if (bp.bp != T(0)) r -= bp.bp * static_cast<const T *>(a.prev)[i];
     y[i] = r;
     acc.s0 += (double)xv * (double)r;
   } else if constexpr (E == Epi::Recon) {
-    if (bp != T(0)) r -= bp * static_cast<const T *>(a.prev)[i];
+    if (bp.bp != T(0)) r -= bp.bp * static_cast<const T *>(a.prev)[i];
     r -= (T)a.alpha * xv;
     y[i] = r;
   } else {  // Cheb
@@ -122,12 +153,6 @@ __device__ __forceinline__ void epilogue(T r, T xv, int64_t i, T *__restrict__ y
   }
 }
 
-template <typename T, Epi E>
-__device__ __forceinline__ T beta_prev(const EpiArgs &a) {
-  if constexpr (E == Epi::Lanczos) return a.step > 0 ? (T)a.beta_dev[a.step - 1] : T(0);
-  if constexpr (E == Epi::Recon) return (T)a.beta;
-  return T(0);
-}
 
 // ------------------------------------------------------------ kernel 1: up hops
 
@@ -291,7 +316,7 @@ __global__ __launch_bounds__(1024) void row_kernel_compact(HvTables t, const T *
   const uint32_t su = t.s_up[u];
   const T *__restrict__ diag = Tabs<T>::diag(t);
   const T h = Tabs<T>::mags(t)[0];
-  const T bp = beta_prev<T, E>(a);
+  const EpiK<T> bp = epi_consts<T, E>(a);
   const T on_u = t.has_onsite ? Tabs<T>::up_on(t)[u] : T(0);
   const int lane = threadIdx.x & (kWave - 1);
   Partial2 acc2;
@@ -340,15 +365,19 @@ __global__ __launch_bounds__(1024) void row_kernel_compact(HvTables t, const T *
 // (ignored where the epilogue does not read it).
 template <typename T, Epi E>
 __device__ __forceinline__ void epilogue_pv(T r, T xv, T pv, int64_t i, T *__restrict__ y,
-                                            const EpiArgs &a, T bp, Partial2 &acc) {
+                                            const EpiArgs &a, const EpiK<T> &bp, Partial2 &acc) {
   if constexpr (E == Epi::Plain) {
     y[i] = r;
   } else if constexpr (E == Epi::Lanczos) {
-    if (bp != T(0)) r -= bp * pv;
+    if (a.lazy) {
+      r *= bp.sx;
+      xv *= bp.sx;
+    }
+    if (bp.bp != T(0)) r -= bp.bp * pv;
     y[i] = r;
     acc.s0 += (double)xv * (double)r;
   } else if constexpr (E == Epi::Recon) {
-    if (bp != T(0)) r -= bp * pv;
+    if (bp.bp != T(0)) r -= bp.bp * pv;
     r -= (T)a.alpha * xv;
     y[i] = r;
   } else {  // Cheb
@@ -361,8 +390,8 @@ __device__ __forceinline__ void epilogue_pv(T r, T xv, T pv, int64_t i, T *__res
 }
 
 template <typename T, Epi E>
-__device__ __forceinline__ bool reads_prev(const EpiArgs &a, T bp) {
-  if constexpr (E == Epi::Lanczos || E == Epi::Recon) return bp != T(0);
+__device__ __forceinline__ bool reads_prev(const EpiArgs &a, const EpiK<T> &bp) {
+  if constexpr (E == Epi::Lanczos || E == Epi::Recon) return bp.bp != T(0);
   if constexpr (E == Epi::Cheb) return !a.first;
   return false;
 }
@@ -413,7 +442,7 @@ __global__ __launch_bounds__(1024) void row_kernel_sorted(HvTables t, const T *_
   }
   __syncthreads();
   const T h = Tabs<T>::mags(t)[0];
-  const T bp = beta_prev<T, E>(a);
+  const EpiK<T> bp = epi_consts<T, E>(a);
   const bool rp = reads_prev<T, E>(a, bp);
   for (; w < nrows; w += gridDim.x) {
     const int64_t nx = w + gridDim.x;
@@ -541,7 +570,7 @@ __global__ __launch_bounds__(1024) void row_kernel_ell(HvTables t, const T *__re
   const uint32_t su = t.s_up[u];
   const T *__restrict__ diag = Tabs<T>::diag(t);
   const T *__restrict__ mags = Tabs<T>::mags(t);
-  const T bp = beta_prev<T, E>(a);
+  const EpiK<T> bp = epi_consts<T, E>(a);
   const T on_u = t.has_onsite ? Tabs<T>::up_on(t)[u] : T(0);
   Partial2 acc2;
   for (int64_t d = threadIdx.x; d < Ddn; d += blockDim.x) {
@@ -574,7 +603,7 @@ __global__ __launch_bounds__(kBlock) void generic_kernel(HvTables t, const T *__
   const int64_t Ddn = t.d_dn, D = t.d_up * Ddn;
   const int64_t p = blockIdx.y;
   const T *__restrict__ xp = x + p * D;
-  const T bp = beta_prev<T, E>(a);
+  const EpiK<T> bp = epi_consts<T, E>(a);
   Partial2 acc2;
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
     const int64_t u = e / Ddn, d = e - u * Ddn;
@@ -792,14 +821,16 @@ __global__ __launch_bounds__(kBlock) void lanczos_finish_alpha_kernel(LanczosSta
   }
 }
 
-// w -= alpha v ; partial = |w|^2
+// w -= alpha v ; partial = |w|^2.  lazy: v is stored unnormalised as
+// beta[step - 1] v (step > 0), so alpha is divided by that norm.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void lanczos_axpy_norm_kernel(const T *__restrict__ v,
                                                                    T *__restrict__ w, int64_t n,
-                                                                   LanczosState st, int step) {
+                                                                   LanczosState st, int step,
+                                                                   int lazy) {
   __shared__ double lds[kBlock / kWave];
   if (st.flags[0]) return;
-  const T a = (T)st.alpha[step];
+  const T a = (lazy && step > 0) ? (T)(st.alpha[step] / st.beta[step - 1]) : (T)st.alpha[step];
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const T q = w[i] - a * v[i];
@@ -1079,7 +1110,7 @@ __global__ __launch_bounds__(kBlock) void mom_hv_kernel(MomTables t, const doubl
     }
   }
   const double uo = t.U / N, eu = t.eps_up[g];
-  const double bp = beta_prev<double, E>(a);
+  const EpiK<double> bp = epi_consts<double, E>(a);
   Partial2 accr, acci;
 #pragma unroll
   for (int j = 0; j < CPT; ++j) {
@@ -1300,7 +1331,7 @@ __global__ __launch_bounds__(kBlock) void expr_hv_kernel(ExprTables t, const dou
   const int c = t.wg_c0[w] + tid;
   const int nc = t.row_ncols[g];
   const int64_t D = t.D, base = t.row_base[g];
-  const double bp = beta_prev<double, E>(a);
+  const EpiK<double> bp = epi_consts<double, E>(a);
   Partial2 accr, acci;
   if (c < nc) {
     const int h = t.row_dg0[g] + c;
@@ -1505,7 +1536,7 @@ __global__ __launch_bounds__(kBlock) void combine_kernel(int64_t D, const T *__r
   if constexpr (E == Epi::Lanczos) {
     if (a.flags[0]) return;
   }
-  const T bp = beta_prev<T, E>(a);
+  const EpiK<T> bp = epi_consts<T, E>(a);
   const T ar = (T)c.ar, ai = (T)c.ai, br = (T)c.br, bi = (T)c.bi, gr = (T)c.gr, gi = (T)c.gi;
   Partial2 acc;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < D;
@@ -1809,15 +1840,15 @@ void launch_lanczos_finish_alpha(const LanczosState &st, int step, int64_t npart
 
 template <typename T>
 void launch_lanczos_axpy_norm(const T *v, T *w, int64_t n, const LanczosState &st, int step,
-                              hipStream_t s) {
+                              hipStream_t s, bool lazy) {
   hipLaunchKernelGGL((lanczos_axpy_norm_kernel<T>), dim3(kRedBlocks), dim3(kBlock), 0, s, v, w, n, st,
-                     step);
+                     step, lazy ? 1 : 0);
   MB_HIP(hipGetLastError());
 }
 template void launch_lanczos_axpy_norm<double>(const double *, double *, int64_t, const LanczosState &,
-                                               int, hipStream_t);
+                                               int, hipStream_t, bool);
 template void launch_lanczos_axpy_norm<float>(const float *, float *, int64_t, const LanczosState &,
-                                              int, hipStream_t);
+                                              int, hipStream_t, bool);
 
 void launch_lanczos_finish_beta(const LanczosState &st, int step, hipStream_t s) {
   hipLaunchKernelGGL(lanczos_finish_beta_kernel, dim3(1), dim3(kBlock), 0, s, st, step);

@@ -189,6 +189,10 @@ LanczosResult pass_one_t(const HvOp &op, const void *b, int k, hipStream_t s, La
   // step 0 does not read v_prev (beta_{-1} = 0); the scratch copy is cleared anyway
   if (!basis) MB_HIP(hipMemsetAsync(vp, 0, (size_t)D * 2 * sizeof(T), s));
   MB_HIP(hipMemsetAsync(W.st.flags, 0, 2 * sizeof(int), s));
+  // Without a stored basis the vectors are kept unnormalised (w is not divided
+  // by beta; the next step's epilogue and axpy apply 1 / beta instead), which
+  // saves one pass over w per step.  A stored basis is normalised as it is built.
+  const bool lazy = basis == nullptr;
   for (int i = 0; i < k; ++i) {
     if (basis) {
       v = basis + (int64_t)i * 2 * D;
@@ -201,11 +205,12 @@ LanczosResult pass_one_t(const HvOp &op, const void *b, int k, hipStream_t s, La
     a.flags = W.st.flags;
     a.step = i;
     a.partial = W.st.partial;
+    a.lazy = lazy ? 1 : 0;
     const int64_t slots = op.hv(v, w, Epi::Lanczos, a, s);
     launch_lanczos_finish_alpha(W.st, i, 2 * slots, s);
-    launch_lanczos_axpy_norm<T>(v, w, 2 * D, W.st, i, s);
+    launch_lanczos_axpy_norm<T>(v, w, 2 * D, W.st, i, s, lazy);
     launch_lanczos_finish_beta(W.st, i, s);
-    launch_lanczos_normalize<T>(w, 2 * D, W.st, i, s);
+    if (!lazy) launch_lanczos_normalize<T>(w, 2 * D, W.st, i, s);
     if (!basis) {
       T *tmp = vp;
       vp = v;

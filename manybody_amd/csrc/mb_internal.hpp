@@ -255,6 +255,10 @@ struct EpiArgs {
   double scale2 = 0, shift = 0;   // Cheb
   int first = 0;                  // Cheb: no t_prev term
   double *partial = nullptr;      // per-row-workgroup partial sums
+  // Lanczos: x and prev are the unnormalised vectors beta[step-1] v and
+  // beta[step-2] v_prev (1 at step 1): the epilogue forms w = H x / beta[step-1]
+  // - (beta[step-1] / beta[step-2]) prev and the partial sum of (x / beta[step-1]) . w
+  int lazy = 0;
 };
 
 // y = epilogue(H x) over P planes; T = double or float.  Returns the number of
@@ -312,9 +316,10 @@ void launch_cdot(const double *a, const double *b, int64_t D, double *partial, d
 template <typename T>
 void launch_scale_div(const double *x, const double *den, T *y, int64_t n, hipStream_t s);
 void launch_lanczos_finish_alpha(const LanczosState &st, int step, int64_t nparts, hipStream_t s);
+// lazy: v is stored as beta[step - 1] v (see EpiArgs::lazy)
 template <typename T>
 void launch_lanczos_axpy_norm(const T *v, T *w, int64_t n, const LanczosState &st, int step,
-                              hipStream_t s);
+                              hipStream_t s, bool lazy = false);
 void launch_lanczos_finish_beta(const LanczosState &st, int step, hipStream_t s);
 template <typename T>
 void launch_lanczos_normalize(T *w, int64_t n, const LanczosState &st, int step, hipStream_t s);
