@@ -752,6 +752,23 @@ inline Rescaling from_bounds(double E_min, double E_max, double padding = 0.01) 
   check(mb_rescaling_from_bounds(E_min, E_max, padding, &r.a, &r.b));
   return r;
 }
+// estimate_rescaling (rescaling.h:21-31) for a device operator: extreme
+// eigenvalues from min(k, D) Lanczos steps from a seeded start (exact up to
+// rounding once k >= D) instead of a dense eig_sym
+inline Rescaling estimate_rescaling(const HubbardOperator &H, double padding = 0.01,
+                                    size_t k = 300, uint64_t seed = 1) {
+  auto b = H.gaussian(seed);
+  const size_t kk = std::min<size_t>(k, H.dimension());
+  const double lo = find_min_eigenpair(H, kk, b).value;
+  const double hi = find_max_eigenpair(H, kk, b).value;
+  return from_bounds(lo, hi, padding);
+}
+// rescale_hamiltonian (rescaling.h:40-48): (H - b I) / a; references H
+inline HubbardOperator rescale_hamiltonian(const HubbardOperator &H, const Rescaling &r) {
+  mb_hubbard *h = nullptr;
+  check(mb_operator_rescaled(H.handle(), r.a, r.b, &h));
+  return HubbardOperator::adopt(h, H.sites());
+}
 }  // namespace rescaling
 
 }  // namespace mbamd

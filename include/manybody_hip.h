@@ -381,6 +381,10 @@ int mb_operator_sum(const mb_hubbard *a, const mb_hubbard *b, mb_hubbard **out);
 int mb_operator_difference(const mb_hubbard *a, const mb_hubbard *b, mb_hubbard **out);
 int mb_operator_composed(const mb_hubbard *a, const mb_hubbard *b, mb_hubbard **out);
 int mb_operator_identity(const mb_basis *basis, mb_hubbard **out);
+/* rescaling::rescale_hamiltonian(H, Rescaling{scale, shift}) (rescaling.h:40-48)
+ * on an operator: (A - shift I) / scale, applied as A x / scale - (shift /
+ * scale) x.  MB_ERR_INVALID for a zero or non-finite scale. */
+int mb_operator_rescaled(const mb_hubbard *a, double scale, double shift, mb_hubbard **out);
 
 /* The sparse matrix of compute_matrix_elements (arma::sp_cx_mat): nonzeros in
  * the reference's basis index order, column-major with rows ascending (SpMat

@@ -123,6 +123,7 @@ _SIGS = {
     "mb_operator_difference": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
     "mb_operator_composed": (C.c_int, [_vp, _vp, C.POINTER(_vp)]),
     "mb_operator_identity": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "mb_operator_rescaled": (C.c_int, [_vp, C.c_double, C.c_double, C.POINTER(_vp)]),
     "mb_sparse_create": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, _vp, _vp, _vp,
                                    C.POINTER(_vp)]),
     "mb_sparse_destroy": (None, [_vp]),
@@ -919,6 +920,43 @@ def rescaling_from_bounds(e_min: float, e_max: float, padding: float = 0.01):
     a, b = C.c_double(), C.c_double()
     _check(lib().mb_rescaling_from_bounds(e_min, e_max, padding, C.byref(a), C.byref(b)))
     return a.value, b.value
+
+
+class Rescaling:
+    """rescaling::Rescaling (rescaling.h:9-19): H_scaled = (H - b) / a."""
+
+    def __init__(self, a: float, b: float):
+        self.a, self.b = float(a), float(b)
+
+    def rescale(self, E: float) -> float:
+        return (E - self.b) / self.a
+
+    def inverse(self, E_scaled: float) -> float:
+        return E_scaled * self.a + self.b
+
+    @staticmethod
+    def from_bounds(e_min: float, e_max: float, padding: float = 0.01) -> "Rescaling":
+        """rescaling::from_bounds (rescaling.h:33-38)."""
+        return Rescaling(*rescaling_from_bounds(e_min, e_max, padding))
+
+
+def estimate_rescaling(op: "HubbardOperator", padding: float = 0.01, k: int = 300,
+                       seed: int = 1) -> Rescaling:
+    """rescaling::estimate_rescaling (rescaling.h:21-31) for a device operator: the
+    extreme eigenvalues from Lanczos (min(k, D) steps from a seeded Gaussian
+    start; exact up to rounding once k >= D) where the reference diagonalises a
+    dense matrix."""
+    b = op.gaussian(seed=seed)
+    k = min(int(k), op.dimension())
+    e_min = find_min_eigenpair(op, k, b, want_vector=False).value
+    e_max = find_max_eigenpair(op, k, b, want_vector=False).value
+    return Rescaling.from_bounds(e_min, e_max, padding)
+
+
+def rescale_hamiltonian(op: "HubbardOperator", r: Rescaling) -> "HubbardOperator":
+    """rescaling::rescale_hamiltonian (rescaling.h:40-48): (H - b I) / a as an
+    operator (composite; it references op)."""
+    return _composite(lib().mb_operator_rescaled, (op,), r.a, r.b)
 
 
 # ------------------------------------------------------------------ batched

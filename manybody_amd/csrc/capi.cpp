@@ -1,6 +1,7 @@
 // extern "C" boundary of libmanybody_hip.so (declarations and reference
 // citations: include/manybody_hip.h).  Every entry point converts C++
 // exceptions into status codes and a thread-local message.
+#include <cmath>
 #include <cstring>
 #include <memory>
 
@@ -1060,6 +1061,14 @@ int mb_operator_difference(const mb_hubbard *a, const mb_hubbard *b, mb_hubbard 
 }
 int mb_operator_composed(const mb_hubbard *a, const mb_hubbard *b, mb_hubbard **out) {
   return make_composite(mb::CompositeOp::Kind::Composed, a, b, 0.0, 0.0, out);
+}
+int mb_operator_rescaled(const mb_hubbard *a, double scale, double shift, mb_hubbard **out) {
+  const int st = guarded([&] {
+    need(scale != 0.0 && std::isfinite(scale) && std::isfinite(shift),
+         "the scale must be finite and nonzero");
+  });
+  if (st != MB_OK) return st;
+  return make_composite(mb::CompositeOp::Kind::Rescaled, a, nullptr, scale, shift, out);
 }
 int mb_operator_identity(const mb_basis *basis, mb_hubbard **out) {
   return guarded([&] {

@@ -1,4 +1,5 @@
-// Operator algebra on device operators.
+// Operator algebra on device operators.  Rescaled is rescale_hamiltonian
+// (rescaling.h:40-48) on an operator: (A - b I) / a.
 //
 // Reference: linear_operator.h:93-281 wraps LinearOperators into Negated,
 // Scaled, Shifted, NegShift, Sum, Difference, Composed and Identity (with the
@@ -63,6 +64,11 @@ int64_t CompositeOp::run(const T *x, T *y, Epi e, const EpiArgs &ea, hipStream_t
     case Kind::Scaled: c.ar = c_.real(); c.ai = c_.imag(); break;
     case Kind::Shifted: c.ar = 1.0; c.gr = c_.real(); c.gi = c_.imag(); c.with_x = 1; break;
     case Kind::NegShift: c.ar = -1.0; c.gr = c_.real(); c.gi = c_.imag(); c.with_x = 1; break;
+    case Kind::Rescaled:  // (A x - b x) / a
+      c.ar = 1.0 / c_.real();
+      c.gr = -c_.imag() / c_.real();
+      c.with_x = 1;
+      break;
     case Kind::Sum:
     case Kind::Difference: {
       Scratch<T> z2(n, s);

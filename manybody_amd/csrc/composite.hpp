@@ -11,10 +11,13 @@ namespace mb {
 class CompositeOp final : public HvOp {
  public:
   // Negated -A, Scaled c A, Shifted A + c I, NegShift -A + c I, Sum A + B,
-  // Difference A - B, Composed A B (B first), Identity I
-  enum class Kind { Negated, Scaled, Shifted, NegShift, Sum, Difference, Composed, Identity };
+  // Difference A - B, Composed A B (B first), Identity I, Rescaled (A - b I) / a
+  enum class Kind {
+    Negated, Scaled, Shifted, NegShift, Sum, Difference, Composed, Identity, Rescaled
+  };
   // a, b: operands (not owned; b only for Sum, Difference, Composed, neither for
-  // Identity); c: the scalar of Scaled, Shifted, NegShift; dim: the dimension
+  // Identity); c: the scalar of Scaled, Shifted, NegShift, (a, b) of Rescaled as
+  // (real, imaginary) parts; dim: the dimension
   CompositeOp(Kind k, const HvOp *a, const HvOp *b, std::complex<double> c, int64_t dim);
   int64_t dim() const override { return dim_; }
   int64_t partials() const override { return 2 * combine_slots(dim_); }

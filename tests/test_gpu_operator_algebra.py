@@ -141,3 +141,51 @@ def test_composite_rejects_mismatched_operands(cuda):
         a + b
     c = mb.HubbardModel(1.0, 4.0, 6).operator(mb.FermionBasis.sector(6, 3, 3))
     assert (a - c).dimension() == a.dimension()
+
+
+def small_op(d, hop=0.0):
+    """sum_i d_i n_{up,i} (+ hop (c+_0 c_1 + h.c.)) on the len(d)-orbital sector with
+    one up particle: the matrix diag(d) with H(0, 1) = H(1, 0) = hop."""
+    basis = mb.FermionBasis.sector(len(d), 1, 0)
+    e = mb.density(mb.Spin.Up, 0) * 0.0
+    for i, v in enumerate(d):
+        e += mb.density(mb.Spin.Up, i) * float(v)
+    if hop:
+        e += mb.hopping(hop, 0, 1, mb.Spin.Up)
+    op = mb.FermionExpressionOperator(basis, e)
+    op._keep = basis
+    return op
+
+
+def dense_of(op, cuda):
+    D = op.dimension()
+    return np.stack([apply(op, np.eye(D)[j], cuda) for j in range(D)], axis=1)
+
+
+def test_reference_rescaling_operator_cases(cuda):
+    """test_rescaling.cpp:46-132 on device operators: estimate_rescaling puts the
+    spectrum inside (-1, 1) and, without padding, maps diag(0, 1, 2) onto
+    [-1, 1] exactly; rescale_hamiltonian keeps the eigenvectors and equals
+    (H - b I) / a."""
+    H = small_op([-2.0, -1.0, 1.0, 3.0])
+    r = mb.estimate_rescaling(H)
+    w = np.linalg.eigvalsh(dense_of(mb.rescale_hamiltonian(H, r), cuda))
+    assert w.min() > -1.0 and w.max() < 1.0
+    H = small_op([0.0, 1.0, 2.0])
+    r0 = mb.estimate_rescaling(H, 0.0)
+    assert abs(r0.a - 1.0) < 1e-10 and abs(r0.b - 1.0) < 1e-10
+    r1 = mb.estimate_rescaling(H, 0.1)
+    assert r1.a > r0.a and abs(r1.b - 1.0) < 1e-10
+    H = small_op([-1.0, 0.0, 2.0], hop=0.5)
+    M = dense_of(H, cuda)
+    assert abs(M[0, 1] - 0.5) < 1e-15 and abs(M[1, 0] - 0.5) < 1e-15
+    _, V0 = np.linalg.eigh(M)
+    _, V1 = np.linalg.eigh(dense_of(mb.rescale_hamiltonian(H, mb.estimate_rescaling(H)), cuda))
+    for i in range(3):
+        assert abs(abs(np.vdot(V0[:, i], V1[:, i])) - 1.0) < 1e-10
+    H = small_op([-2.0, -1.0, 1.0, 3.0], hop=0.5)
+    r = mb.Rescaling.from_bounds(-2.5, 3.5)
+    want = (dense_of(H, cuda) - r.b * np.eye(4)) / r.a
+    assert np.linalg.norm(dense_of(mb.rescale_hamiltonian(H, r), cuda) - want) < 1e-14
+    with pytest.raises(ValueError):
+        mb.rescale_hamiltonian(H, mb.Rescaling(0.0, 1.0))

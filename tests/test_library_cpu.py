@@ -340,3 +340,18 @@ def test_momentum_current_expression_validation():
     # k = (1, 0) -> k + Q = (2, 1), flat 2 + 4 * 1 = 6; v = 2 (2 pi / 4) sin(pi / 2) = pi
     assert abs(j.terms[(6, 0x8000 | 1)] - np.pi) < 1e-15
     assert abs(j.terms[(0x4000 | 6, 0xC000 | 1)] - np.pi) < 1e-15
+
+
+def test_reference_rescaling_host_cases():
+    """test_rescaling.cpp:9-92: rescale / inverse and from_bounds."""
+    r = mb.Rescaling(2.5, 1.0)
+    assert abs(r.inverse(r.rescale(3.5)) - 3.5) < 1e-14
+    r = mb.Rescaling(3.0, 2.0)
+    assert abs(r.rescale(2.0)) < 1e-14 and abs(r.rescale(5.0) - 1.0) < 1e-14
+    assert abs(r.rescale(-1.0) + 1.0) < 1e-14
+    assert abs(r.inverse(0.0) - 2.0) < 1e-14 and abs(r.inverse(1.0) - 5.0) < 1e-14
+    assert abs(r.inverse(-1.0) + 1.0) < 1e-14
+    r = mb.Rescaling.from_bounds(-2.0, 4.0, 0.0)
+    assert abs(r.a - 3.0) < 1e-14 and abs(r.b - 1.0) < 1e-14
+    r = mb.Rescaling.from_bounds(-2.0, 4.0, 0.1)
+    assert abs(r.a - 3.3) < 1e-14 and abs(r.b - 1.0) < 1e-14
