@@ -910,6 +910,46 @@ class SparseKPMProjector:
         return out
 
 
+class KPMProjector(SparseKPMProjector):
+    """KPMProjector(H, M, E_fermi, padding) (kpm.h:67-180) on a device operator:
+    the rescaling is estimate_rescaling(H, padding), whose extreme eigenvalues
+    come from Lanczos (min(k, D) steps) where the reference diagonalises the
+    dense H; apply is SparseKPMProjector's recurrence; build_matrix applies the
+    projector to every unit vector (reference order, small operators)."""
+
+    def __init__(self, op: HubbardOperator, expansion_order: int, e_fermi: float = 0.0,
+                 padding: float = 0.01, k: int = 300, seed: int = 1):
+        b = op.gaussian(seed=seed)
+        kk = min(int(k), op.dimension())
+        e_min = find_min_eigenpair(op, kk, b, want_vector=False).value
+        e_max = find_max_eigenpair(op, kk, b, want_vector=False).value
+        super().__init__(op, expansion_order, e_min, e_max, e_fermi, padding)
+        self._rescaling = Rescaling.from_bounds(e_min, e_max, padding)
+        eps = self._rescaling.rescale(self.e_fermi)
+        if eps < -1.0 or eps > 1.0:
+            raise RuntimeError("Fermi energy outside rescaled spectrum")
+
+    def expansion_order(self) -> int:
+        return self.M
+
+    def rescaling(self) -> Rescaling:
+        return self._rescaling
+
+    def build_matrix(self):
+        """The dense projector (kpm.h:144-170), columns P e_j in reference order."""
+        import numpy as np
+        import torch
+        D = self.dimension()
+        P = np.zeros((D, D), np.complex128)
+        e = torch.zeros(D, dtype=torch.complex128, device="cuda")
+        for j in range(D):
+            e.zero_()
+            e[j] = 1.0
+            col = self.op.to_reference_order(self.apply(self.op.to_device_order(e)))
+            P[:, j] = col.cpu().numpy()
+        return P
+
+
 def jackson_kernel(N: int):
     g = (C.c_double * (N + 1))()
     _check(lib().mb_jackson_kernel(N, g))

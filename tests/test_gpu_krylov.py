@@ -470,3 +470,27 @@ def test_kpm_projector_on_device_matches_oracle(cuda):
     got = host(hop, mb.SparseKPMProjector(hop, 64, -9.0, 22.0, 3.0, 0.05).apply(x))
     want = po.kpm_projector_apply(lambda u: Ac @ u, host(hop, x), 64, -9.0, 22.0, 3.0, 0.05)
     assert np.linalg.norm(got - want) <= 1e-11 * np.linalg.norm(want)
+
+
+def test_kpm_projector_from_operator_matches_oracle(cuda):
+    """KPMProjector(H, M, E_fermi, padding) (kpm.h:67-180) on a device operator:
+    the bounds from Lanczos equal the exact extreme eigenvalues (k >= D), the
+    built matrix equals the oracle's projector matrix with those bounds, and at
+    order 200 with E_fermi between the 3rd and 4th levels it is close to the
+    projector on the three lowest states (trace ~ 3, idempotent), as
+    test_kpm.cpp:100-146 checks for its dense matrices."""
+    sizes, nu, nd = [4], 2, 1
+    A = po.build_csc(sizes, nu, nd, 1.0, 4.0).to_scipy().toarray()
+    w = np.linalg.eigvalsh(A)
+    op = mb.HubbardModel(1.0, 4.0, 4).operator(mb.FermionBasis.sector(4, nu, nd))
+    ef = 0.5 * (w[2] + w[3])
+    P = mb.KPMProjector(op, 200, ef, 0.01)
+    r = P.rescaling()
+    assert abs(r.a - (w[-1] - w[0]) / 2 * 1.01) < 1e-10 and abs(r.b - (w[-1] + w[0]) / 2) < 1e-10
+    Pm = P.build_matrix()
+    want = po.kpm_projector_matrix(A, 200, P.e_min, P.e_max, ef, 0.01)
+    assert np.abs(Pm - want).max() < 1e-12
+    assert abs(np.trace(Pm).real - 3.0) < 0.1
+    assert np.abs(Pm @ Pm - Pm).max() < 0.05
+    with pytest.raises(RuntimeError):
+        mb.KPMProjector(op, 50, w[-1] + 100.0)
