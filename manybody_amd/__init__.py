@@ -856,6 +856,11 @@ def estimate_bounds(op: HubbardOperator, seed, power_iterations: int,
 
 def kpm_dos_moments(op: HubbardOperator, a: float, b: float, M: int, nvec: int, seed: int,
                     fp32: bool = True):
+    """KPM density-of-states moments mu_0 .. mu_{M-1} of (H - b) / a with nvec
+    random-phase vectors (phases keyed by reference index).  Real-space Hubbard
+    operators run the multi-vector FP32 / FP64 path; every other operator
+    (momentum sectors, fixed-N bases, expressions, composites) runs FP64, one
+    vector at a time, with the same phases."""
     mu = (C.c_double * M)()
     _check(lib().mb_kpm_dos_moments(op._h, a, b, M, nvec, seed, int(fp32), mu, _stream()))
     return list(mu)

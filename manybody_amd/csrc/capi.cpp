@@ -148,6 +148,16 @@ const mb::Hubbard &H(const mb_hubbard *h) {
   return h->h;
 }
 
+// device position -> reference index of any operator handle; computed into
+// `own` for a real-space sector (free it only after the work using it)
+const int64_t *ref_positions(const mb_hubbard *h, mb::DevBuf &own, hipStream_t s) {
+  if (const int64_t *m = h->ref_map()) return m;
+  const mb::Sector *sec = h->rsec();
+  own.alloc((size_t)std::max<int64_t>(sec->dim(), 1) * sizeof(int64_t));
+  mb::launch_ref_of_pos(mb::perm_of(*sec), own.as<int64_t>(), s);
+  return own.as<int64_t>();
+}
+
 // whether two bases have the same device layout (same kind and parameters)
 bool same_layout(const mb_basis *a, const mb_basis *b) {
   if (a == b) return true;
@@ -500,8 +510,19 @@ int mb_fill_phase_c128_multi(const mb_hubbard *h, uint64_t seed, int nvec, void 
                              mb_stream stream) {
   return guarded([&] {
     need(x_dev, "null argument");
+    need(h != nullptr, "null model handle");
     need(nvec >= 1 && nvec <= 64, "nvec out of range");
-    mb::launch_fill_phase(mb::perm_of(*H(h).sec), seed, nvec, false, x_dev, S(stream));
+    if (!h->on_basis() && !h->mom) {
+      mb::launch_fill_phase(mb::perm_of(*H(h).sec), seed, nvec, false, x_dev, S(stream));
+      return;
+    }
+    mb::DevBuf own;
+    const int64_t *ref = ref_positions(h, own, S(stream));
+    const int64_t D = h->op().dim();
+    for (int r = 0; r < nvec; ++r)
+      mb::launch_fill_phase_ref(ref, D, seed, nvec, r, false, (double *)x_dev + 2 * r * D,
+                                S(stream));
+    MB_HIP(hipStreamSynchronize(S(stream)));  // own is freed on return
   });
 }
 
@@ -609,7 +630,15 @@ int mb_kpm_dos_moments(const mb_hubbard *h, double a, double b, int M, int nvec,
                        int fp32, double *mu, mb_stream stream) {
   return guarded([&] {
     need(mu != nullptr, "null argument");
-    mb::kpm_dos_moments(H(h), *H(h).sec, a, b, M, nvec, seed, fp32 != 0, mu, S(stream));
+    need(h != nullptr, "null model handle");
+    if (!h->on_basis() && !h->mom) {
+      mb::kpm_dos_moments(H(h), *H(h).sec, a, b, M, nvec, seed, fp32 != 0, mu, S(stream));
+      return;
+    }
+    need(fp32 == 0, "FP32 KPM moments need a real-space Hubbard operator");
+    mb::DevBuf own;
+    const int64_t *ref = ref_positions(h, own, S(stream));
+    mb::kpm_dos_moments_op(h->op(), ref, a, b, M, nvec, seed, mu, S(stream));
   });
 }
 

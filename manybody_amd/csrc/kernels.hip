@@ -755,6 +755,39 @@ __global__ __launch_bounds__(kBlock) void fill_phase_kernel(PermTables p, uint64
   }
 }
 
+// The same phases keyed by a device position -> reference index map (any
+// basis layout): vector r only, planes (re, im) of D values.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void fill_phase_ref_kernel(const int64_t *__restrict__ ref,
+                                                                int64_t D, uint64_t seed,
+                                                                int nvec, int r,
+                                                                T *__restrict__ x) {
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < D; e += (int64_t)gridDim.x * kBlock) {
+    const uint64_t i = (uint64_t)ref[e];
+    const double ph = 6.283185307179586 * unit_half(mix64(seed, i * (uint64_t)nvec + r));
+    x[e] = (T)cos(ph);
+    x[D + e] = (T)sin(ph);
+  }
+}
+
+// out0 = sum of partial[2 w], out1 = sum of partial[2 w + 1], w < n (fixed order)
+__global__ __launch_bounds__(kBlock) void reduce_pairs_kernel(const double *__restrict__ partial,
+                                                              int64_t n, double *out0,
+                                                              double *out1) {
+  __shared__ double lds[kBlock / kWave];
+  double s0 = 0.0, s1 = 0.0;
+  for (int64_t w = threadIdx.x; w < n; w += kBlock) {
+    s0 += partial[2 * w];
+    s1 += partial[2 * w + 1];
+  }
+  s0 = wg_sum(s0, lds);
+  s1 = wg_sum(s1, lds);
+  if (threadIdx.x == 0) {
+    *out0 = s0;
+    *out1 = s1;
+  }
+}
+
 // ------------------------------------------------------------ vector steps
 
 template <typename T>
@@ -1585,7 +1618,8 @@ int64_t launch_combine(int64_t D, const T *z1, const T *z2, const T *x, T *y, co
       hipLaunchKernelGGL((combine_kernel<T, Epi::Recon>), g, dim3(kBlock), 0, s, D, z1, z2, x, y, c, a);
       break;
     case Epi::Cheb:
-      throw std::invalid_argument("the Chebyshev epilogue is not available for composite operators");
+      hipLaunchKernelGGL((combine_kernel<T, Epi::Cheb>), g, dim3(kBlock), 0, s, D, z1, z2, x, y, c, a);
+      break;
   }
   MB_HIP(hipGetLastError());
   return slots;  // partial slots per plane: the launch wrote 2 slots
@@ -1769,6 +1803,22 @@ void launch_fill_phase(const PermTables &p, uint64_t seed, int nvec, bool fp32, 
     hipLaunchKernelGGL((fill_phase_kernel<float>), g, b, 0, s, p, seed, nvec, (float *)x);
   else
     hipLaunchKernelGGL((fill_phase_kernel<double>), g, b, 0, s, p, seed, nvec, (double *)x);
+  MB_HIP(hipGetLastError());
+}
+
+void launch_fill_phase_ref(const int64_t *ref, int64_t D, uint64_t seed, int nvec, int r,
+                           bool fp32, void *x, hipStream_t s) {
+  const dim3 g(grid_for(D, 1 << 16)), b(kBlock);
+  if (fp32)
+    hipLaunchKernelGGL((fill_phase_ref_kernel<float>), g, b, 0, s, ref, D, seed, nvec, r, (float *)x);
+  else
+    hipLaunchKernelGGL((fill_phase_ref_kernel<double>), g, b, 0, s, ref, D, seed, nvec, r, (double *)x);
+  MB_HIP(hipGetLastError());
+}
+
+void launch_reduce_pairs(const double *partial, int64_t n, double *out0, double *out1,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(reduce_pairs_kernel, dim3(1), dim3(kBlock), 0, s, partial, n, out0, out1);
   MB_HIP(hipGetLastError());
 }
 

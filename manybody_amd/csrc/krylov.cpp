@@ -588,6 +588,13 @@ void kpm_dos_moments(const Hubbard &h, const Sector &sec, double a, double b, in
     tn = tnext;
     tnext = old_prev;
   }
+  kpm_accumulate(rawp, M, nvec, D, mu, s);
+}
+
+// Moments from the raw overlaps (row 0: <T0|T0> per vector in the second half
+// of the row; row n + 1: <T_{n+1}|T_n> then <T_{n+1}|T_{n+1}> per vector).
+void kpm_accumulate(const double *rawp, int M, int nvec, int64_t D, double *mu, hipStream_t s) {
+  const int width = 2 * nvec, steps = M / 2;
   std::vector<double> hr((size_t)(steps + 1) * width);
   MB_HIP(hipMemcpyAsync(hr.data(), rawp, hr.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   MB_HIP(hipStreamSynchronize(s));
@@ -609,6 +616,52 @@ void kpm_dos_moments(const Hubbard &h, const Sector &sec, double a, double b, in
   }
   const double norm = 1.0 / ((double)nvec * (double)D);
   for (int n = 0; n < M; ++n) mu[n] = acc[n] * norm;
+}
+
+// The same moments for any operator, FP64, one random-phase vector at a time:
+// per step one plain H.v and one combine pass with the Chebyshev epilogue
+// (T_{n+1} = scale2 (H T_n - b T_n) - T_{n-1} and its two overlaps), reduced on
+// the device; the phases are keyed by reference index, as in the real-space
+// path, so both give the same moments for the same operator.
+void kpm_dos_moments_op(const HvOp &op, const int64_t *ref, double a, double b, int M, int nvec,
+                        uint64_t seed, double *mu, hipStream_t s) {
+  if (M < 2 || (M & 1)) throw std::invalid_argument("M must be even and >= 2");
+  if (nvec < 1 || nvec > 64) throw std::invalid_argument("nvec must be in [1, 64]");
+  if (!(a > 0.0)) throw std::invalid_argument("rescaling factor a must be positive");
+  const int64_t D = op.dim();
+  const size_t bytes = (size_t)D * 2 * sizeof(double);
+  DevBuf T0(bytes), T1(bytes), T2(bytes), z(bytes);
+  const int width = 2 * nvec, steps = M / 2;
+  const int64_t slots = combine_slots(D);
+  DevBuf partial((size_t)std::max<int64_t>(4 * slots, kRedBlocks) * sizeof(double));
+  DevBuf raw((size_t)(steps + 1) * width * sizeof(double));
+  MB_HIP(hipMemsetAsync(raw.p, 0, raw.bytes, s));
+  double *rawp = raw.as<double>();
+  for (int r = 0; r < nvec; ++r) {
+    launch_fill_phase_ref(ref, D, seed, nvec, r, false, T0.p, s);
+    launch_sumsq(T0.p, false, 2 * D, partial.as<double>(), rawp + nvec + r, false, s);
+    double *tn = T0.as<double>(), *tprev = T2.as<double>(), *tnext = T1.as<double>();
+    for (int n = 0; n < steps; ++n) {
+      op.hv(tn, z.as<double>(), Epi::Plain, EpiArgs{}, s);
+      EpiArgs ea;
+      ea.prev = tprev;
+      ea.first = n == 0;
+      ea.scale2 = ea.first ? 1.0 / a : 2.0 / a;
+      ea.shift = b;
+      ea.partial = partial.as<double>();
+      CombineCoef c;
+      c.ar = 1.0;
+      const int64_t per_plane = launch_combine<double>(D, z.as<double>(), nullptr, tn, tnext, c,
+                                                       Epi::Cheb, ea, s);
+      double *row = rawp + (size_t)(n + 1) * width;
+      launch_reduce_pairs(partial.as<double>(), 2 * per_plane, row + r, row + nvec + r, s);
+      double *old_prev = tprev;
+      tprev = tn;
+      tn = tnext;
+      tnext = old_prev;
+    }
+  }
+  kpm_accumulate(rawp, M, nvec, D, mu, s);
 }
 
 }  // namespace mb

@@ -106,6 +106,10 @@ inline void time_evolve_steps(const Hubbard &h, const void *psi, double t0, doub
 }
 void kpm_dos_moments(const Hubbard &h, const Sector &sec, double a, double b, int M, int nvec,
                      uint64_t seed, bool fp32, double *mu, hipStream_t s);
+// The same for any operator (FP64); ref: device position -> reference index
+void kpm_dos_moments_op(const HvOp &op, const int64_t *ref, double a, double b, int M, int nvec,
+                        uint64_t seed, double *mu, hipStream_t s);
+void kpm_accumulate(const double *raw, int M, int nvec, int64_t D, double *mu, hipStream_t s);
 
 // out[0] = sum x_i^2 over n values (FP64 accumulation), or its square root
 // solve(op, b, k, solver) (lanczos.h:220-237) with a host solver giving the

@@ -282,6 +282,14 @@ template <typename T>
 int64_t launch_combine(int64_t D, const T *z1, const T *z2, const T *x, T *y, const CombineCoef &c,
                        Epi epi, const EpiArgs &a, hipStream_t s);
 
+// Random-phase vector r of nvec (the phases of launch_fill_phase) for any
+// layout, keyed by a device position -> reference index map.
+void launch_fill_phase_ref(const int64_t *ref, int64_t D, uint64_t seed, int nvec, int r,
+                           bool fp32, void *x, hipStream_t s);
+// out0 / out1 = the sums of the even / odd entries of partial[0, 2 n)
+void launch_reduce_pairs(const double *partial, int64_t n, double *out0, double *out1,
+                         hipStream_t s);
+
 // Number of blocks used by the elementwise deterministic reductions.
 constexpr int kRedBlocks = 512;
 

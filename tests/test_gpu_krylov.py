@@ -494,3 +494,43 @@ def test_kpm_projector_from_operator_matches_oracle(cuda):
     assert np.abs(Pm @ Pm - Pm).max() < 0.05
     with pytest.raises(RuntimeError):
         mb.KPMProjector(op, 50, w[-1] + 100.0)
+
+
+def test_kpm_moments_on_any_operator(cuda):
+    """KPM DOS moments through the generic path (any operator, FP64): the
+    compiled Hubbard expression gives the native real-space moments (same
+    phases, 1e-12); a fixed-momentum sector gives the oracle's moments for the
+    same vectors (1e-11); Shifted(H, 1) with b + 1 gives H's moments; FP32 is
+    refused off the real-space kernels."""
+    sizes, nu, nd, t, U = [8], 3, 3, 1.0, 4.0
+    native = make_op(sizes, nu, nd, t, U)
+    A = po.build_csc(sizes, nu, nd, t, U)
+    ev = np.linalg.eigvalsh(A.to_scipy().toarray().real)
+    a, b = (ev[-1] - ev[0]) / 2 * 1.02, (ev[-1] + ev[0]) / 2
+    M, nvec, seed = 64, 3, 7
+    mu_n = np.array(mb.kpm_dos_moments(native, a, b, M, nvec, seed, fp32=False))
+    expr = mb.FermionExpressionOperator(native.basis, mb.HubbardModel(t, U, 8).hamiltonian())
+    mu_e = np.array(mb.kpm_dos_moments(expr, a, b, M, nvec, seed, fp32=False))
+    assert np.abs(mu_e - mu_n).max() < 1e-12
+    mu_s = np.array(mb.kpm_dos_moments(mb.Shifted(native, 1.0), a, b + 1.0, M, nvec, seed,
+                                       fp32=False))
+    assert np.abs(mu_s - mu_n).max() < 1e-12
+    with pytest.raises(ValueError):
+        mb.kpm_dos_moments(expr, a, b, M, nvec, seed, fp32=True)
+    # a fixed-momentum sector against the oracle with the same phase vectors
+    import scipy.sparse as sps
+    msz, K = [6], 1
+    basis = mb.FermionBasis.with_fixed_particle_number_spin_momentum(6, 5, 1, msz, [K])
+    H = mb.HubbardMomentumFactorizedFixedK(basis, msz, 1.0, 4.0, [K])
+    Hd = po.momentum_hamiltonian(msz, 3, 2, K, 1.0, 4.0)
+    w = np.linalg.eigvalsh(Hd)
+    a, b = (w[-1] - w[0]) / 2 * 1.05, (w[-1] + w[0]) / 2
+    mu = np.array(mb.kpm_dos_moments(H, a, b, 40, 2, 3, fp32=False))
+    vecs = H.random_phase(3, 2, fp32=False)
+    vref = np.stack([H.to_reference_order(vecs[2 * r:2 * r + 2].contiguous()).cpu().numpy()
+                     for r in range(2)])
+    sp = sps.csc_matrix(Hd)
+    csc = po.CSC(sp.shape[0], sp.indptr.astype(np.int64), sp.indices.astype(np.int64),
+                 sp.data.astype(np.complex128))
+    ref = po.kpm_moments_csc(csc, a, b, 40, vref)
+    assert np.abs(mu - ref).max() < 1e-11 and abs(mu[0] - 1.0) < 1e-14
