@@ -143,7 +143,8 @@ int mb_fill_gaussian_c128(const mb_hubbard *h, uint64_t seed, void *x_dev, mb_st
 int mb_fill_phase_c64_multi(const mb_hubbard *h, uint64_t seed, int nvec, void *x_dev,
                             mb_stream stream);
 
-/* The same random-phase vectors in complex128 (e^{i phi} evaluated in FP64). */
+/* The same random-phase vectors in complex128 (e^{i phi} evaluated in FP64);
+ * any operator handle (the complex64 form needs a real-space Hubbard operator). */
 int mb_fill_phase_c128_multi(const mb_hubbard *h, uint64_t seed, int nvec, void *x_dev,
                              mb_stream stream);
 
@@ -216,7 +217,10 @@ int mb_tridiag_eig(int m, const double *alphas, const double *betas, double *w, 
  * mu_n = (1/(nvec D)) sum_r <r|T_n(Hs)|r>.  The Chebyshev recurrence of
  * SparseKPMProjector::apply (kpm.h:212-229); H.v in FP32 complex (fp32 = 1) or
  * FP64 complex (fp32 = 0); inner products and moments accumulate in FP64.
- * M even, 1 <= nvec <= 64. */
+ * M even, 1 <= nvec <= 64.  Real-space Hubbard operators advance all vectors
+ * in one launch; any other handle (momentum sector, fixed-N basis, expression,
+ * composite) runs FP64 one vector at a time with the same phases
+ * (MB_ERR_INVALID for fp32 = 1 there). */
 int mb_kpm_dos_moments(const mb_hubbard *h, double a, double b, int M, int nvec, uint64_t seed,
                        int fp32, double *mu, mb_stream stream);
 
