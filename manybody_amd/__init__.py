@@ -466,22 +466,33 @@ class HubbardOperator:
             self._h = None
 
     # the reference's operator algebra (linear_operator.h:232-281): -A, A + B,
-    # A - B, A * B (composition), scalar * A and A * scalar
+    # A - B, A * B (composition), scalar * A and A * scalar; numpy scalars defer
+    # to these methods
+    __array_ufunc__ = None
+
     def __neg__(self):
         return Negated(self)
 
     def __add__(self, other):
+        if not isinstance(other, HubbardOperator):
+            return NotImplemented  # A + c I is Shifted(A, c), as in the reference
         return Sum(self, other)
 
     def __sub__(self, other):
+        if not isinstance(other, HubbardOperator):
+            return NotImplemented
         return Difference(self, other)
 
     def __mul__(self, other):
         if isinstance(other, HubbardOperator):
             return Composed(self, other)
+        if not isinstance(other, (int, float, complex)) and not hasattr(other, "__complex__"):
+            return NotImplemented
         return Scaled(self, other)
 
     def __rmul__(self, other):
+        if not isinstance(other, (int, float, complex)) and not hasattr(other, "__complex__"):
+            return NotImplemented
         return Scaled(self, other)
 
     def dimension(self) -> int:

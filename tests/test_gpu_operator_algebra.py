@@ -189,3 +189,16 @@ def test_reference_rescaling_operator_cases(cuda):
     assert np.linalg.norm(dense_of(mb.rescale_hamiltonian(H, r), cuda) - want) < 1e-14
     with pytest.raises(ValueError):
         mb.rescale_hamiltonian(H, mb.Rescaling(0.0, 1.0))
+
+
+def test_operator_overloads_reject_non_operators(cuda):
+    """A + c and A - c are not defined (the reference shifts with Shifted); numpy
+    scalars scale."""
+    H = small_op([1.0, 2.0, 3.0])
+    with pytest.raises(TypeError):
+        H + 1.0
+    with pytest.raises(TypeError):
+        H - 1.0
+    with pytest.raises(TypeError):
+        H * "x"
+    assert np.array_equal(apply(np.float64(2.0) * H, [1.0, 1.0, 1.0], cuda), [2.0, 4.0, 6.0])
