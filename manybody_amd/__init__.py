@@ -160,6 +160,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        # torch carries its own HIP runtime; it has to bring the device up before
+        # this library's runtime does, or torch later finds no device (the
+        # library's tensors are torch's), so initialise it first
+        try:
+            import torch
+            torch.cuda.is_available()
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)

@@ -303,15 +303,16 @@ class SparseMatrix:
     def __init__(self, handle):
         from . import lib, _check
         self._h = handle
+        self._lib = lib()  # kept for __del__, which may run at interpreter shutdown
         r, c, n = C.c_uint64(), C.c_uint64(), C.c_int64()
         _check(lib().mb_sparse_shape(handle, C.byref(r), C.byref(c), C.byref(n)))
         self.shape = (int(r.value), int(c.value))
         self.nnz = int(n.value)
 
     def __del__(self):
-        from . import _lib
-        if getattr(self, "_h", None) and _lib is not None:
-            _lib.mb_sparse_destroy(self._h)
+        h, L = getattr(self, "_h", None), getattr(self, "_lib", None)
+        if h and L is not None:
+            L.mb_sparse_destroy(h)
             self._h = None
 
     def entries(self):

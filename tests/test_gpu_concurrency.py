@@ -87,3 +87,22 @@ def test_first_device_upload_from_threads(cuda):
         if want is None:
             want = got[0]
         assert np.array_equal(got[0], want)
+
+
+def test_library_first_then_torch_in_a_fresh_process(cuda):
+    """A script that builds operators before touching torch.cuda still gets a
+    device for torch (the package brings torch's HIP runtime up first); run in a
+    child process, where nothing has initialised the GPU yet."""
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import manybody_amd as mb\n"
+        "b = mb.FermionBasis.sector(6, 3, 3)\n"
+        "op = mb.FermionExpressionOperator(b, mb.HubbardModel(1.0, 4.0, 6).hamiltonian())\n"
+        "e = mb.find_min_eigenpair(op, 100, op.gaussian(1), want_vector=False).value\n"
+        "print('ok', e)\n")
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], cwd=repo, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout[-1000:] + r.stderr[-2000:]
