@@ -38,8 +38,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-n", type=int, default=512)
     ap.add_argument("--no-lanczos", action="store_true")
-    ap.add_argument("--c2-cpu-energy", action="store_true",
-                    help="also run the oracle's 300-step OpenMP Lanczos on the host (minutes)")
+    ap.add_argument("--no-c2-cpu-energy", action="store_true",
+                    help="skip the oracle's 300-step OpenMP Lanczos on the host (about 2 minutes) "
+                         "that checks the 4x4 ground energy")
+    ap.add_argument("--no-csc-baseline", action="store_true")
     ap.add_argument("--no-kpm", action="store_true")
     ap.add_argument("--no-momentum", action="store_true")
     ap.add_argument("--no-conductivity", action="store_true")
@@ -95,15 +97,18 @@ def main():
     alg_bytes = 32.0 * D
     gbs = alg_bytes / (ms * 1e-3) / 1e9
 
+    copy_gbs = device_copy_rate(x)
+
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
+    host = host_info()
+    threads = args.cpu_threads or min(16, host["affinity"])
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import pyoracle as po
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         P = po.ProductOperator([Lx, Ly], nu, nd, t, U, threads=threads)
         xh = mb.from_planar(x)
         P.matvec(xh)  # warm-up (page faults)
@@ -116,13 +121,14 @@ def main():
         cpu = {"value": round(alg_bytes / tc / 1e9, 3), "unit": "GB/s", "cores": threads,
                "kind": "port",
                "sample": f"median of 3 full-vector H.v (D={D}) by the oracle's OpenMP "
-                         f"matrix-free restatement, {tc * 1e3:.0f} ms each"}
+                         f"matrix-free restatement, {tc * 1e3:.0f} ms each",
+               "host": host}
         del xh
 
     lanczos = None
     if not args.no_lanczos:
-        lanczos = bench_lanczos_c2(op, x, basis, (Lx, Ly, nu, nd, t, U), args.c2_cpu_energy,
-                                   args.cpu_threads)
+        lanczos = bench_lanczos_c2(op, x, basis, (Lx, Ly, nu, nd, t, U),
+                                   not args.no_c2_cpu_energy, threads)
     observables = None if args.no_lanczos else bench_observables(op, x)
     del y
     momentum = None if args.no_momentum else bench_momentum()
@@ -132,6 +138,7 @@ def main():
     kpm = None if args.no_kpm else bench_kpm_c3()
     krylov = None if args.no_kpm else bench_krylov_fp32()
     serving = None if args.no_serving else bench_serving(args.serving_n, args.serving_rate)
+    csc = None if args.no_csc_baseline else bench_cpu_csc_c1(threads)
 
     line = {
         "metric": "Hubbard H·v GB/s (% HBM peak); batched solves/s + p99 latency at |ΔE|<1e-8",
@@ -150,7 +157,8 @@ def main():
                    "algorithmic_bytes_per_step": int(alg_bytes), "parallelism": "single GPU",
                    "wall_s": round(wall, 3)},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "copy_GBps": round(copy_gbs, 1), "frac_vs_copy": round(gbs / copy_gbs, 4)},
         "cpu_baseline": cpu,
         "lanczos_4x4": lanczos,
         "observables_4x4": observables,
@@ -161,11 +169,94 @@ def main():
         "kpm_l14": kpm,
         "krylov_l14": krylov,
         "serving": serving,
+        "cpu_csc_1d_l8": csc,
     }
     print(json.dumps(line))
 
 
 D_4X4_7P7 = 11440 * 11440
+
+
+def host_info():
+    """The host the CPU legs run on: logical CPUs, the ones this process may use,
+    the CPU model (/proc/cpuinfo) and OMP_NUM_THREADS."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def device_copy_rate(x, reps=10):
+    """GB/s of a plain device copy of x (read + write bytes / time): the
+    achievable HBM rate the H.v is also priced against."""
+    import torch
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del y
+    return 2.0 * x.numel() * x.element_size() / (ms * 1e-3) / 1e9
+
+
+def bench_cpu_csc_c1(threads, reps=5, k=200):
+    """The reference's cost structure on the 1D ring L = 8, 4 + 4, U = 4
+    (D = 4,900), restated by the oracle: OpenMP column-parallel CSC build with
+    a hash lookup per nonzero (matrix_elements.h:74-105) on `threads` threads,
+    then a 200-step Lanczos (lanczos.h:130-175) whose CSC product runs on one
+    thread (linear_operator.h:87; ARMA_OPENMP_THREADS=1, src/CMakeLists.txt:20),
+    vector passes on one thread too.  Median of `reps` runs after a warm-up,
+    beside the device's 200-step Lanczos from the same start vector."""
+    import numpy as np
+    import torch
+
+    import manybody_amd as mb
+    from oracle import pyoracle as po
+    L, nu, nd, t, U = 8, 4, 4, 1.0, 4.0
+    op = mb.HubbardModel(t, U, L).operator(mb.FermionBasis.sector(L, nu, nd))
+    b = op.gaussian(1)
+    b_ref = op.to_reference_order(b).cpu().numpy()
+    builds, solves, gpu = [], [], []
+    A = po.build_csc([L], nu, nd, t, U, threads=threads)
+    for _ in range(reps):
+        c0 = time.perf_counter()
+        A = po.build_csc([L], nu, nd, t, U, threads=threads)
+        builds.append(time.perf_counter() - c0)
+    po.lib().orc_set_threads(1)
+    po.lanczos_csc(A, b_ref, k)
+    for _ in range(reps):
+        c0 = time.perf_counter()
+        al, be, _ = po.lanczos_csc(A, b_ref, k)
+        e_cpu = float(po.tridiag_eigvals(al, be).min())
+        solves.append(time.perf_counter() - c0)
+    po.lib().orc_set_threads(threads)
+    mb.lanczos_pass_one(op, b, k)
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        c0 = time.perf_counter()
+        dec = mb.lanczos_pass_one(op, b, k)
+        w, _ = mb.tridiag_eig(dec.alphas, dec.betas, vectors=False)
+        gpu.append(time.perf_counter() - c0)
+    tb, ts, tg = (float(np.median(v)) for v in (builds, solves, gpu))
+    return {"workload": "1D ring L=8 4+4 U=4 (D=4900): CSC build + 200-step Lanczos, "
+                        "oracle restatement of the reference's cost structure",
+            "nnz": int(A.col_ptr[-1]), "build_threads": threads, "build_ms": round(tb * 1e3, 3),
+            "solve_threads": 1, "solve_ms": round(ts * 1e3, 3),
+            "ms_per_step_cpu": round(ts / k * 1e3, 4), "E0_cpu": e_cpu,
+            "gpu_solve_ms": round(tg * 1e3, 3), "E0_gpu": float(w[0]),
+            "rel_dE": abs(float(w[0]) - e_cpu) / abs(e_cpu)}
 
 
 def copy_calibration(steps, warmup):
@@ -235,10 +326,11 @@ def write_traffic(hv_fetch, hv_write, copy_fetch, copy_write):
     print(json.dumps(out))
 
 
-def bench_lanczos_c2(op, x, basis, model, cpu_energy, cpu_threads, k=300):
+def bench_lanczos_c2(op, x, basis, model, cpu_energy, threads, k=300):
     """300-step FP64 Lanczos on the 4x4 7+7 sector from the bench vector: lowest
-    Ritz value and time per step; optionally the oracle's OpenMP Lanczos on the
-    host from the same vector (minutes) for the energy comparison."""
+    Ritz value and time per step; unless switched off, the oracle's OpenMP
+    Lanczos on the host from the same vector (about 2 minutes) for the energy
+    comparison (bar: 1e-10 relative)."""
     import numpy as np
     import torch
 
@@ -255,15 +347,17 @@ def bench_lanczos_c2(op, x, basis, model, cpu_energy, cpu_threads, k=300):
         import scipy.linalg as sla
         from oracle import pyoracle as po
         Lx, Ly, nu, nd, t, U = model
-        threads = cpu_threads or min(16, os.cpu_count() or 1)
         P = po.ProductOperator([Lx, Ly], nu, nd, t, U, threads=threads)
         xh = mb.from_planar(x)
+        po.lib().orc_set_progress(25)  # progress lines on stderr during the host run
         c0 = time.perf_counter()
         al, be, _ = P.lanczos(xh, k)
         t_cpu = time.perf_counter() - c0
+        po.lib().orc_set_progress(0)
         e_cpu = float(sla.eigvalsh_tridiagonal(np.asarray(al), np.asarray(be))[0])
+        rel = abs(out["E0"] - e_cpu) / abs(e_cpu)
         out.update({"E0_cpu_oracle": e_cpu, "cpu_s": round(t_cpu, 1), "cpu_threads": threads,
-                    "rel_dE": abs(out["E0"] - e_cpu) / abs(e_cpu)})
+                    "rel_dE": rel, "rel_dE_below_1e-10": bool(rel < 1e-10)})
     return out
 
 

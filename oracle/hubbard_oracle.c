@@ -10,6 +10,7 @@
 #include "hubbard_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -396,6 +397,18 @@ int64_t orc_build_csc(int n_sites, const uint64_t *up, const uint64_t *dn, int64
   return missing ? -1 : nnz;
 }
 
+static int progress_every = 0;
+
+void orc_set_progress(int every) { progress_every = every; }
+
+void orc_set_threads(int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+}
+
 void orc_spmv_csc(int64_t D, const int64_t *col_ptr, const int64_t *row_idx, const double *val,
                   const double *x, double *y) {
   memset(y, 0, sizeof(double) * 2 * (size_t)D);
@@ -586,6 +599,10 @@ static int lanczos_generic(int64_t D, apply_fn f, void *ctx, const double *b, in
     for (int64_t e = 0; e < 2 * D; ++e) w[e] -= alpha * v[e];
     const double beta = cnorm(D, w);
     alphas[steps++] = alpha;
+    if (progress_every > 0 && steps % progress_every == 0) {
+      fprintf(stderr, "oracle Lanczos: step %d of %d\n", steps, k);
+      fflush(stderr);
+    }
     if (beta <= ORC_TOL) break;
     if (i + 1 < k) betas[i] = beta;
 #pragma omp parallel for schedule(static)
@@ -763,19 +780,18 @@ void orc_lanczos_pass_two_csc(int64_t D, const int64_t *col_ptr, const int64_t *
 }
 
 /* evolve_state (evolve_state.h:120-151) with expm_tridiagonal_first_column_unitary
- * (:22-61) and solve (lanczos.h:220-237). */
-void orc_time_evolve_csc(int64_t D, const int64_t *col_ptr, const int64_t *row_idx,
-                         const double *val, const double *psi, double t, int k, double *out) {
+ * (:22-61) and solve (lanczos.h:220-237), over any operator. */
+static void time_evolve_generic(int64_t D, apply_fn f, void *ctx, const double *psi, double t,
+                                int k, double *out) {
   memcpy(out, psi, sizeof(double) * 2 * (size_t)D);
   if (D == 0) return;
   const double nrm = cnorm(D, psi);
   if (nrm <= ORC_TOL) return;
   if (k == 0 || k > D) k = (int)D;
-  csc_op o = {D, col_ptr, row_idx, val, 0};
   double *al = (double *)malloc(sizeof(double) * (size_t)k);
   double *be = (double *)malloc(sizeof(double) * (size_t)k);
   double bn;
-  int m = lanczos_generic(D, csc_apply, &o, psi, k, al, be, &bn);
+  int m = lanczos_generic(D, f, ctx, psi, k, al, be, &bn);
   if (m <= 0) {
     memset(out, 0, sizeof(double) * 2 * (size_t)D);
     free(al);
@@ -799,7 +815,7 @@ void orc_time_evolve_csc(int64_t D, const int64_t *col_ptr, const int64_t *row_i
     y[2 * i] = re * bn;
     y[2 * i + 1] = im * bn;
   }
-  pass_two_complex(D, csc_apply, &o, psi, m, al, be, bn, y, out);
+  pass_two_complex(D, f, ctx, psi, m, al, be, bn, y, out);
   free(T);
   free(w);
   free(Q);
@@ -808,11 +824,30 @@ void orc_time_evolve_csc(int64_t D, const int64_t *col_ptr, const int64_t *row_i
   free(be);
 }
 
+void orc_time_evolve_csc(int64_t D, const int64_t *col_ptr, const int64_t *row_idx,
+                         const double *val, const double *psi, double t, int k, double *out) {
+  csc_op o = {D, col_ptr, row_idx, val, 0};
+  time_evolve_generic(D, csc_apply, &o, psi, t, k, out);
+}
+
+void orc_time_evolve_product(int64_t Dup, int64_t Ddn, const uint64_t *s_up,
+                             const uint64_t *s_dn, const int64_t *up_ptr, const int32_t *up_cols,
+                             const double *up_vals, const double *up_onsite,
+                             const int64_t *dn_ptr, const int32_t *dn_cols, const double *dn_vals,
+                             const double *dn_onsite, const double *diagU, const double *psi,
+                             double t, int k, double *out, int threads) {
+  prod_op o = {Dup, Ddn, s_up, s_dn, up_ptr, dn_ptr, up_cols, dn_cols, up_vals, dn_vals,
+               up_onsite, dn_onsite, diagU, threads};
+  time_evolve_generic(Dup * Ddn, prod_apply, &o, psi, t, k, out);
+}
+
 /* ---------------------------------------------------------------- KPM */
 
-void orc_kpm_moments_csc(int64_t D, const int64_t *col_ptr, const int64_t *row_idx,
-                         const double *val, double a, double b, int M, int R,
-                         const double *vecs, double *mu) {
+/* Density-of-states moments mu_n = (1 / (R D)) sum_r Re <r|T_n(Hs)|r>, Hs =
+ * (H - b) / a, by the Chebyshev recurrence of kpm.h:212-229 (T_{n+1} =
+ * 2 Hs T_n - T_{n-1}) and a direct overlap per order (no doubling identities). */
+static void kpm_generic(int64_t D, apply_fn f, void *ctx, double a, double b, int M, int R,
+                        const double *vecs, double *mu) {
   double *t0 = (double *)malloc(sizeof(double) * 2 * (size_t)D);
   double *t1 = (double *)malloc(sizeof(double) * 2 * (size_t)D);
   double *t2 = (double *)malloc(sizeof(double) * 2 * (size_t)D);
@@ -824,13 +859,15 @@ void orc_kpm_moments_csc(int64_t D, const int64_t *col_ptr, const int64_t *row_i
       if (n == 0) {
         memcpy(t1, t0, sizeof(double) * 2 * (size_t)D);
       } else if (n == 1) {
-        orc_spmv_csc(D, col_ptr, row_idx, val, t0, t1);
+        f(ctx, t0, t1);
+#pragma omp parallel for schedule(static)
         for (int64_t e = 0; e < 2 * D; ++e) t1[e] = (t1[e] - b * t0[e]) / a;
         /* keep T_0 in t2, T_1 in t1 */
         memcpy(t2, t0, sizeof(double) * 2 * (size_t)D);
       } else {
         /* T_{n} = 2 Hs T_{n-1} - T_{n-2}; t1 = T_{n-1}, t2 = T_{n-2} */
-        orc_spmv_csc(D, col_ptr, row_idx, val, t1, t0);
+        f(ctx, t1, t0);
+#pragma omp parallel for schedule(static)
         for (int64_t e = 0; e < 2 * D; ++e) t0[e] = 2.0 * (t0[e] - b * t1[e]) / a - t2[e];
         double *tmp = t2;
         t2 = t1;
@@ -838,15 +875,31 @@ void orc_kpm_moments_csc(int64_t D, const int64_t *col_ptr, const int64_t *row_i
         t0 = tmp;
       }
       /* <r | T_n r>, real part */
-      double s = 0.0;
-      for (int64_t e = 0; e < 2 * D; ++e) s += v[e] * t1[e];
-      mu[n] += s;
+      mu[n] += dot_fixed(2 * D, v, t1);
     }
   }
   for (int n = 0; n < M; ++n) mu[n] /= ((double)R * (double)D);
   free(t0);
   free(t1);
   free(t2);
+}
+
+void orc_kpm_moments_csc(int64_t D, const int64_t *col_ptr, const int64_t *row_idx,
+                         const double *val, double a, double b, int M, int R,
+                         const double *vecs, double *mu) {
+  csc_op o = {D, col_ptr, row_idx, val, 0};
+  kpm_generic(D, csc_apply, &o, a, b, M, R, vecs, mu);
+}
+
+void orc_kpm_moments_product(int64_t Dup, int64_t Ddn, const uint64_t *s_up,
+                             const uint64_t *s_dn, const int64_t *up_ptr, const int32_t *up_cols,
+                             const double *up_vals, const double *up_onsite,
+                             const int64_t *dn_ptr, const int32_t *dn_cols, const double *dn_vals,
+                             const double *dn_onsite, const double *diagU, double a, double b,
+                             int M, int R, const double *vecs, double *mu, int threads) {
+  prod_op o = {Dup, Ddn, s_up, s_dn, up_ptr, dn_ptr, up_cols, dn_cols, up_vals, dn_vals,
+               up_onsite, dn_onsite, diagU, threads};
+  kpm_generic(Dup * Ddn, prod_apply, &o, a, b, M, R, vecs, mu);
 }
 
 /* kpm.h:19-30 */

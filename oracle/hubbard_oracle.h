@@ -51,6 +51,15 @@ int64_t orc_build_csc(int n_sites, const uint64_t *up, const uint64_t *dn, int64
                       int n_bonds, const int *from, const int *to, double t, double U,
                       int64_t *col_ptr, int64_t *row_idx, double *val, int64_t cap, int threads);
 
+/* Threads of the OpenMP loops that follow (the vector passes of the Lanczos,
+ * Krylov and KPM routines; the CSC product itself is always one thread, as the
+ * reference configures Armadillo, src/CMakeLists.txt:20). */
+void orc_set_threads(int threads);
+
+/* every > 0: the Lanczos routines print a line to stderr every `every` steps
+ * (long host runs show that they progress); 0 (default): silent. */
+void orc_set_progress(int every);
+
 /* y = A x with A in CSC, complex values, column scatter order (Armadillo
  * sp_cx_mat * cx_vec, linear_operator.h:87). */
 void orc_spmv_csc(int64_t D, const int64_t *col_ptr, const int64_t *row_idx, const double *val,
@@ -112,12 +121,28 @@ void orc_lanczos_pass_two_csc(int64_t D, const int64_t *col_ptr, const int64_t *
 void orc_time_evolve_csc(int64_t D, const int64_t *col_ptr, const int64_t *row_idx,
                          const double *val, const double *psi, double t, int k, double *out);
 
+/* The same on the matrix-free product-order operator of orc_hv_product. */
+void orc_time_evolve_product(int64_t Dup, int64_t Ddn, const uint64_t *s_up,
+                             const uint64_t *s_dn, const int64_t *up_ptr, const int32_t *up_cols,
+                             const double *up_vals, const double *up_onsite,
+                             const int64_t *dn_ptr, const int32_t *dn_cols, const double *dn_vals,
+                             const double *dn_onsite, const double *diagU, const double *psi,
+                             double t, int k, double *out, int threads);
+
 /* Chebyshev moments mu_n = (1/R) sum_r <r|T_n(Hs)|r> / D for Hs = (H - b)/a,
  * evaluated by the plain three-term recurrence (kpm.h:212-229) with one
  * inner product per order, FP64.  vecs holds R complex start vectors. */
 void orc_kpm_moments_csc(int64_t D, const int64_t *col_ptr, const int64_t *row_idx,
                          const double *val, double a, double b, int M, int R,
                          const double *vecs, double *mu);
+
+/* The same on the matrix-free product-order operator of orc_hv_product. */
+void orc_kpm_moments_product(int64_t Dup, int64_t Ddn, const uint64_t *s_up,
+                             const uint64_t *s_dn, const int64_t *up_ptr, const int32_t *up_cols,
+                             const double *up_vals, const double *up_onsite,
+                             const int64_t *dn_ptr, const int32_t *dn_cols, const double *dn_vals,
+                             const double *dn_onsite, const double *diagU, double a, double b,
+                             int M, int R, const double *vecs, double *mu, int threads);
 
 /* Jackson kernel g_0..g_N (kpm.h:19-30). */
 void orc_jackson_kernel(int N, double *g);

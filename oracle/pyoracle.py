@@ -43,6 +43,10 @@ def lib():
         L.orc_build_csc.argtypes = [C.c_int, _u64p, _u64p, C.c_int64, C.c_int, _i32p, _i32p,
                                     C.c_double, C.c_double, _i64p, C.c_void_p, C.c_void_p,
                                     C.c_int64, C.c_int]
+        L.orc_set_progress.restype = None
+        L.orc_set_progress.argtypes = [C.c_int]
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_spmv_csc.restype = None
         L.orc_spmv_csc.argtypes = [C.c_int64, _i64p, _i64p, _c128p, _c128p, _c128p]
         L.orc_species_table.restype = C.c_int64
@@ -73,6 +77,12 @@ def lib():
         L.orc_kpm_moments_csc.restype = None
         L.orc_kpm_moments_csc.argtypes = [C.c_int64, _i64p, _i64p, _c128p, C.c_double, C.c_double,
                                           C.c_int, C.c_int, _c128p, _f64p]
+        L.orc_time_evolve_product.restype = None
+        L.orc_time_evolve_product.argtypes = prod_args + [_c128p, C.c_double, C.c_int, _c128p,
+                                                          C.c_int]
+        L.orc_kpm_moments_product.restype = None
+        L.orc_kpm_moments_product.argtypes = prod_args + [C.c_double, C.c_double, C.c_int,
+                                                          C.c_int, _c128p, _f64p, C.c_int]
         L.orc_jackson_kernel.restype = None
         L.orc_jackson_kernel.argtypes = [C.c_int, _f64p]
         _lib = L
@@ -207,6 +217,34 @@ class ProductOperator:
         if m < 0:
             raise RuntimeError("Input vector has zero norm")
         return al[:m].copy(), be[:max(m - 1, 0)].copy(), bn.value
+
+    def time_evolve(self, psi: np.ndarray, t: float, k: int = 30) -> np.ndarray:
+        """exp(-i t H) psi with k Krylov steps (evolve_state.h:120-178), product order."""
+        out = np.zeros(self.D, np.complex128)
+        lib().orc_time_evolve_product(*self._args(), np.ascontiguousarray(psi, np.complex128),
+                                      t, k, out, self.threads)
+        return out
+
+    def time_evolve_steps(self, psi: np.ndarray, t0: float, t1: float, dt: float,
+                          k: int = 30) -> np.ndarray:
+        """time_evolve_state_steps (evolve_state.h:204-231): steps of dt from t0,
+        the last one clipped to t1."""
+        if dt == 0.0 or t1 == t0:
+            return psi
+        t = t0
+        while t < t1:
+            step = min(dt, t1 - t)
+            psi = self.time_evolve(psi, step, k)
+            t += step
+        return psi
+
+    def kpm_moments(self, a: float, b: float, M: int, vecs: np.ndarray) -> np.ndarray:
+        """mu_n = (1 / (R D)) sum_r Re <r|T_n((H - b) / a)|r>, n < M, product order."""
+        vecs = np.ascontiguousarray(np.atleast_2d(vecs), np.complex128)
+        mu = np.zeros(M)
+        lib().orc_kpm_moments_product(*self._args(), a, b, M, vecs.shape[0], vecs, mu,
+                                      self.threads)
+        return mu
 
 
 def product_permutation(sizes, n_up: int, n_dn: int, ref_states=None) -> np.ndarray:

@@ -266,6 +266,36 @@ def test_kpm_moments_match_spectral_sum():
     assert abs(mu[0] - 1.0) < 1e-14
 
 
+def test_product_order_kpm_and_evolution_match_spectral_forms():
+    """The product-order KPM moments and stepped Krylov evolution (used at sizes
+    where no CSC fits) against the spectral sum and expm of the dense H, through
+    the DFS permutation: 2D 3 x 2, 2 + 2, U = 3."""
+    sizes, nu, nd, t, U = [3, 2], 2, 2, 1.0, 3.0
+    A = po.build_csc(sizes, nu, nd, t, U)
+    P = po.ProductOperator(sizes, nu, nd, t, U)
+    perm = po.product_permutation(sizes, nu, nd)
+    H = A.to_scipy().toarray()
+    lam, Q = np.linalg.eigh(H)
+    a, b = (lam[-1] - lam[0]) / 2 * 1.05, (lam[-1] + lam[0]) / 2
+    rng = np.random.default_rng(9)
+    R, M = 2, 48
+    vecs_ref = np.exp(2j * np.pi * rng.random((R, A.D)))
+    vecs_dev = np.zeros_like(vecs_ref)
+    vecs_dev[:, perm] = vecs_ref
+    mu = P.kpm_moments(a, b, M, vecs_dev)
+    x = (lam - b) / a
+    w = (np.abs(Q.conj().T @ vecs_ref.T) ** 2).sum(axis=1)
+    ref = np.array([(w * np.cos(n * np.arccos(x))).sum() for n in range(M)]) / (R * A.D)
+    assert np.abs(mu - ref).max() < 1e-12
+    psi_ref = rng.standard_normal(A.D) + 1j * rng.standard_normal(A.D)
+    psi_dev = np.zeros_like(psi_ref)
+    psi_dev[perm] = psi_ref
+    out = P.time_evolve_steps(psi_dev, 0.0, 0.33, 0.1, k=30)  # 0.1, 0.1, 0.1, 0.03
+    exact = sla.expm(-1j * 0.33 * H) @ psi_ref
+    assert np.linalg.norm(out[perm] - exact) / np.linalg.norm(exact) < 1e-10
+    assert np.array_equal(P.time_evolve_steps(psi_dev, 0.0, 0.0, 0.1), psi_dev)
+
+
 def test_jackson_kernel_properties_and_closed_form():
     g = po.jackson_kernel(50)
     assert abs(g[0] - 1.0) < 1e-10
