@@ -158,6 +158,21 @@ int mb_fill_phase_c128_multi(const mb_hubbard *h, uint64_t seed, int nvec, void 
 int mb_lanczos_pass_one(const mb_hubbard *h, const void *b_dev, int k, double *alphas,
                         double *betas, int *steps, double *b_norm, mb_stream stream);
 
+/* No counterpart in the reference: mb_lanczos_pass_one for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results; every
+ * buffer beyond b (three complex128 vectors, alpha/beta, partial sums, flags)
+ * comes from a static 48 MiB device array that each GPU holds until the
+ * process ends, so it calls no hipMalloc/hipFree.  Limit: 48 D + 16 k bytes
+ * plus a few KB of partial sums must fit, i.e. D <= ~1,040,000 (every sector
+ * with L <= 12) for k up to a few thousand; larger runs -> MB_ERR_INVALID with
+ * the byte counts, before any device work.  Real-space Hubbard operators only
+ * (MB_ERR_INVALID otherwise).  Calls on one GPU share the array; they are
+ * serialised by a host lock held for the whole call, which returns after its
+ * stream has drained. */
+int mb_lanczos_pass_one_noalloc(const mb_hubbard *h, const void *b_dev, int k, double *alphas,
+                                double *betas, int *steps, double *b_norm, mb_stream stream);
+
 /* find_min_eigenpair(op, k) with an explicit start vector b (the reference
  * draws it with arma::randu, linear_operator.h:26-32) — lanczos.h:246-290.
  * *value = lowest Ritz value; if x_dev != NULL it receives the eigenvector

@@ -72,6 +72,8 @@ _SIGS = {
     "mb_fill_phase_c64_multi": (C.c_int, [_vp, C.c_uint64, C.c_int, _vp, _vp]),
     "mb_fill_phase_c128_multi": (C.c_int, [_vp, C.c_uint64, C.c_int, _vp, _vp]),
     "mb_lanczos_pass_one": (C.c_int, [_vp, _vp, C.c_int, _dp, _dp, C.POINTER(C.c_int), _dp, _vp]),
+    "mb_lanczos_pass_one_noalloc": (C.c_int, [_vp, _vp, C.c_int, _dp, _dp, C.POINTER(C.c_int),
+                                              _dp, _vp]),
     "mb_find_min_eigenpair": (C.c_int, [_vp, _vp, C.c_int, _dp, _vp, _vp]),
     "mb_time_evolve": (C.c_int, [_vp, _vp, C.c_double, C.c_int, _vp, _vp]),
     "mb_time_evolve_steps": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_int,
@@ -646,6 +648,21 @@ def lanczos_pass_one(op: HubbardOperator, b, k: int,
     else:
         _check(lib().mb_lanczos_pass_one(op._h, _ptr(b), k, al, be, C.byref(steps), C.byref(bn),
                                          _stream()))
+    m = steps.value
+    return LanczosDecomposition(list(al[:m]), list(be[:max(m - 1, 0)]), m, bn.value)
+
+
+def lanczos_pass_one_noalloc(op: HubbardOperator, b, k: int) -> LanczosDecomposition:
+    """lanczos_pass_one with no device allocation (mb_lanczos_pass_one_noalloc):
+    the work buffers come from the library's static per-GPU scratch array, so
+    D is limited to about 1.04e6 (ValueError beyond); real-space Hubbard
+    operators only."""
+    al = (C.c_double * max(k, 1))()
+    be = (C.c_double * max(k, 1))()
+    steps = C.c_int()
+    bn = C.c_double()
+    _check(lib().mb_lanczos_pass_one_noalloc(op._h, _ptr(b), k, al, be, C.byref(steps),
+                                             C.byref(bn), _stream()))
     m = steps.value
     return LanczosDecomposition(list(al[:m]), list(be[:max(m - 1, 0)]), m, bn.value)
 

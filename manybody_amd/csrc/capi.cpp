@@ -541,6 +541,23 @@ int mb_lanczos_pass_one(const mb_hubbard *h, const void *b_dev, int k, double *a
   });
 }
 
+int mb_lanczos_pass_one_noalloc(const mb_hubbard *h, const void *b_dev, int k, double *alphas,
+                                double *betas, int *steps, double *b_norm, mb_stream stream) {
+  return guarded([&] {
+    need(h && b_dev && alphas && betas && steps && b_norm, "null argument");
+    need(k >= 0, "k must be non-negative");
+    need(h->real_op && !h->comp && !h->mom && !h->expr,
+         "the no-allocation Lanczos runs on real-space Hubbard operators only");
+    auto r = mb::lanczos_pass_one_noalloc(OP(h), b_dev, k, S(stream));
+    *steps = r.steps;
+    *b_norm = r.b_norm;
+    for (int i = 0; i < k; ++i) {
+      alphas[i] = i < (int)r.alphas.size() ? r.alphas[i] : 0.0;
+      betas[i] = i < (int)r.betas.size() ? r.betas[i] : 0.0;
+    }
+  });
+}
+
 int mb_find_min_eigenpair(const mb_hubbard *h, const void *b_dev, int k, double *value,
                           void *x_dev, mb_stream stream) {
   return guarded([&] {

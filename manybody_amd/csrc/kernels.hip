@@ -1870,6 +1870,22 @@ void launch_reorth_sub(const double *V, int m, const double *c, double *w, int64
   MB_HIP(hipGetLastError());
 }
 
+// Clears n 32-bit words (a kernel rather than hipMemsetAsync, so a captured
+// stream records no memset node).
+__global__ __launch_bounds__(kBlock) void zero_words_kernel(uint32_t *__restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    p[i] = 0u;
+}
+
+void launch_zero(void *p, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return;
+  if (bytes % 4 != 0) throw std::invalid_argument("launch_zero: byte count not a multiple of 4");
+  const int64_t n = (int64_t)(bytes / 4);
+  hipLaunchKernelGGL(zero_words_kernel, dim3(grid_for(n, 1 << 14)), dim3(kBlock), 0, s,
+                     static_cast<uint32_t *>(p), n);
+  MB_HIP(hipGetLastError());
+}
+
 void launch_norm(const double *x, int64_t n, double *partial, double *out, hipStream_t s) {
   launch_sumsq(x, false, n, partial, out, true, s);
 }

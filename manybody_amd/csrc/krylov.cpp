@@ -187,8 +187,8 @@ LanczosResult pass_one_t(const HvOp &op, const void *b, int k, hipStream_t s, La
     *w = basis ? basis + 2 * D : W.w.as<T>();
   launch_scale_div<T>((const double *)b, W.st.bnorm, v, 2 * D, s);
   // step 0 does not read v_prev (beta_{-1} = 0); the scratch copy is cleared anyway
-  if (!basis) MB_HIP(hipMemsetAsync(vp, 0, (size_t)D * 2 * sizeof(T), s));
-  MB_HIP(hipMemsetAsync(W.st.flags, 0, 2 * sizeof(int), s));
+  if (!basis) launch_zero(vp, (size_t)D * 2 * sizeof(T), s);
+  launch_zero(W.st.flags, 2 * sizeof(int), s);
   // Without a stored basis the vectors are kept unnormalised (w is not divided
   // by beta; the next step's epilogue and axpy apply 1 / beta instead), which
   // saves one pass over w per step.  A stored basis is normalised as it is built.

@@ -67,6 +67,12 @@ void lanczos_pass_two(const HvOp &op, const void *b, const LanczosResult &dec,
                       const std::vector<std::complex<double>> &y, void *x, hipStream_t s,
                       LanczosWork *work = nullptr);
 double tridiag_min_eig(int m, const double *a, const double *b);
+// lanczos_pass_one with every device buffer carved from a static per-device
+// scratch array (noalloc.hip): no hipMalloc / hipFree; std::invalid_argument
+// when the run needs more scratch than the array holds.
+LanczosResult lanczos_pass_one_noalloc(const HvOp &op, const void *b, int k, hipStream_t s);
+size_t lanczos_noalloc_bytes(int64_t D, int k, int64_t nparts);
+size_t lanczos_noalloc_scratch_bytes();
 double find_min_eigenpair(const HvOp &op, const void *b, int k, void *x, hipStream_t s);
 // Reorthogonalised variants (all Lanczos vectors kept in HBM; see krylov.cpp).
 LanczosResult lanczos_pass_one_reorth(const HvOp &op, const void *b, int k, hipStream_t s,

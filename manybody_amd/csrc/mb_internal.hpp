@@ -308,6 +308,8 @@ struct LanczosState {
   double *bnorm;     // [1]
 };
 void launch_norm(const double *x, int64_t n, double *partial, double *out, hipStream_t s);
+// Zeroes `bytes` (a multiple of 4) at p with a fill kernel.
+void launch_zero(void *p, size_t bytes, hipStream_t s);
 void launch_sumsq_partial(const double *x, int64_t n, double *partial, hipStream_t s);
 // w -= V (V^H w) against the m stored planar vectors of V (stride 2 D); partial
 // holds 2 m kProjBlocks doubles, c 2 m doubles; no-op once flags[0] is set
