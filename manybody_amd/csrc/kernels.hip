@@ -361,6 +361,145 @@ __global__ __launch_bounds__(1024) void row_kernel_compact(HvTables t, const T *
   row_reduce<T, E>(acc2, a, wg, red);
 }
 
+// NP consecutive values of T at a 16-byte (or, for 8 bytes, 8-byte) aligned address.
+template <typename T, int NP>
+__device__ __forceinline__ void load_pack(const T *p, T (&v)[NP]) {
+  constexpr int B = NP * (int)sizeof(T);
+  if constexpr (B % 16 == 0) {
+#pragma unroll
+    for (int j = 0; j < B / 16; ++j) {
+      const uint4 q = reinterpret_cast<const uint4 *>(p)[j];
+      __builtin_memcpy(reinterpret_cast<char *>(v) + 16 * j, &q, 16);
+    }
+  } else {
+    static_assert(B == 8, "pack of 8, 16 or 32 bytes");
+    const uint2 q = *reinterpret_cast<const uint2 *>(p);
+    __builtin_memcpy(v, &q, 8);
+  }
+}
+template <typename T, int NP>
+__device__ __forceinline__ void store_pack(T *p, const T (&v)[NP]) {
+  constexpr int B = NP * (int)sizeof(T);
+  if constexpr (B % 16 == 0) {
+#pragma unroll
+    for (int j = 0; j < B / 16; ++j) {
+      uint4 q;
+      __builtin_memcpy(&q, reinterpret_cast<const char *>(v) + 16 * j, 16);
+      reinterpret_cast<uint4 *>(p)[j] = q;
+    }
+  } else {
+    uint2 q;
+    __builtin_memcpy(&q, v, 8);
+    *reinterpret_cast<uint2 *>(p) = q;
+  }
+}
+
+// row_kernel_compact over NP planes of the same row at once: the rows of NP
+// consecutive planes (for one complex vector its re and im planes, for several
+// vectors several of them) are staged in LDS interleaved, NP values per string,
+// so each compact-table entry is read from L2 once for NP planes and each hop
+// is one LDS read of NP values.  Same sums in the same order as
+// row_kernel_compact, so the results are bit-identical; one partial-sum slot
+// per plane row as there.  Workgroup = plane group * D_up + row.
+template <typename T, Epi E, int NP>
+__global__ __launch_bounds__(1024) void row_kernel_mp(HvTables t, const T *__restrict__ x,
+                                                      T *__restrict__ y, EpiArgs a) {
+  constexpr int K = kCompactK, G = 8, NS = (E == Epi::Cheb) ? 2 : 1;
+  extern __shared__ uint4 smem16[];  // 16-byte aligned for the pack reads
+  __shared__ double red[2 * NP][1024 / kWave];
+  T *row = reinterpret_cast<T *>(smem16);  // (D_dn + 1) * NP, string-major; string D_dn is zero
+  if constexpr (E == Epi::Lanczos) {
+    if (a.flags[0]) return;
+  }
+  const int64_t Ddn = t.d_dn, Dup = t.d_up, D = Dup * Ddn;
+  const int64_t wg = blockIdx.x;
+  const int64_t u = wg % Dup, pg = wg / Dup;
+  const int64_t base0 = pg * NP * D + u * Ddn;  // plane pg * NP, row u
+  for (int64_t d = threadIdx.x; d < Ddn; d += blockDim.x) {
+    T v[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) v[p] = x[base0 + p * D + d];
+    store_pack<T, NP>(row + d * NP, v);
+  }
+  if (threadIdx.x < NP) row[Ddn * NP + threadIdx.x] = T(0);
+  __syncthreads();
+  const uint32_t su = t.s_up[u];
+  const T *__restrict__ diag = Tabs<T>::diag(t);
+  const T h = Tabs<T>::mags(t)[0];
+  const EpiK<T> bp = epi_consts<T, E>(a);
+  const T on_u = t.has_onsite ? Tabs<T>::up_on(t)[u] : T(0);
+  const int lane = threadIdx.x & (kWave - 1);
+  Partial2 acc2[NP];
+  for (int64_t d0 = (int64_t)threadIdx.x - lane; d0 < Ddn; d0 += blockDim.x) {
+    const int64_t d = d0 + lane;  // tables are padded to a multiple of 64 strings
+    const bool ok = d < Ddn;
+    const uint4 *tp = reinterpret_cast<const uint4 *>(t.dn_comp + d * K);
+    uint4 q[K / 8];
+#pragma unroll
+    for (int j = 0; j < K / 8; ++j) q[j] = tp[j];
+    const uint32_t sm = t.dn_csign[d];
+    const int wm = __builtin_amdgcn_readfirstlane((int)t.dn_cgmax[d0 / kWave]);
+    T dg = diag[popc(su & t.s_dn[ok ? d : 0])];
+    if (t.has_onsite) dg = dg + on_u + Tabs<T>::dn_on(t)[ok ? d : 0];
+    T h0[NP], h1[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) h0[p] = h1[p] = T(0);
+#pragma unroll
+    for (int g = 0; g < K / G; ++g) {
+      if (g * G < wm) {
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const int k = g * G + j;
+          const uint32_t w = reinterpret_cast<const uint32_t *>(&q[k / 8])[(k % 8) / 2];
+          const uint32_t tg = (k & 1) ? (w >> 16) : (w & 0xFFFFu);
+          T v[NP];
+          load_pack<T, NP>(row + (int64_t)tg * NP, v);
+          const uint32_t sb = (sm << (31 - k)) & 0x80000000u;
+#pragma unroll
+          for (int p = 0; p < NP; ++p) {
+            const T fv = flip_sign(v[p], sb);
+            if (j & 1) h1[p] += fv; else h0[p] += fv;
+          }
+        }
+      }
+    }
+    if (ok) {
+      T xv[NP];
+      load_pack<T, NP>(row + d * NP, xv);
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const int64_t i = base0 + p * D + d;
+        const T r = y[i] + (dg * xv[p] + h * (h0[p] + h1[p]));
+        epilogue<T, E>(r, xv[p], i, y, a, bp, acc2[p]);
+      }
+    }
+  }
+  if constexpr (E == Epi::Lanczos || E == Epi::Cheb) {
+    // per plane: wave sums, then the waves in order (fixed tree, as wg_sum)
+    const int wv = threadIdx.x / kWave;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+#pragma unroll
+      for (int c = 0; c < NS; ++c) {
+        double v = c == 0 ? acc2[p].s0 : acc2[p].s1;
+        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+        if (lane == 0) red[NS * p + c][wv] = v;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < NS * NP) {
+      double r = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) r += red[threadIdx.x][w];
+      const int p = threadIdx.x / NS, c = threadIdx.x % NS;
+      const int64_t prow = (pg * NP + p) * Dup + u;  // the plane row's slot
+      if constexpr (E == Epi::Lanczos)
+        a.partial[prow] = r;
+      else
+        a.partial[2 * prow + c] = r;
+    }
+  }
+}
+
 // Epilogue with the values it reads from memory loaded beforehand: pv = prev[i]
 // (ignored where the epilogue does not read it).
 template <typename T, Epi E>
@@ -1506,6 +1645,40 @@ size_t sorted_row_lds(const HvTables &t) {
   return (size_t)(t.d_dn + 1 + kSortedHalf) * sizeof(T);
 }
 
+// Planes per workgroup of the multi-plane row kernel: the most of 8, 4, 2 up
+// to the cap that divides P and keeps the interleaved row within kMpRowLds;
+// 1 = the one-plane kernels.  The default cap is 8 bytes per string (two FP32
+// planes): measured on the L = 14, 7 + 7 ring, the FP32 Chebyshev step over 16
+// vectors takes 4.03 ms with one plane per workgroup, 3.22 ms with two and
+// 3.73 ms with four (fewer workgroups fit a CU); FP64 gains nothing (6.08 ms
+// with one plane, 6.16 ms with two).  MANYBODY_AMD_ROW_PLANES=n sets the cap
+// (up to 32 bytes per string; 1 switches the multi-plane kernel off).
+#ifndef MB_MP_ROW_LDS
+#define MB_MP_ROW_LDS (64 * 1024)
+#endif
+constexpr size_t kMpRowLds = MB_MP_ROW_LDS;
+
+template <typename T>
+int row_planes(const HvTables &t, int P) {
+  int cap = 8 / (int)sizeof(T);
+  if (const char *e = std::getenv("MANYBODY_AMD_ROW_PLANES"))
+    cap = std::max(1, std::min(32 / (int)sizeof(T), std::atoi(e)));
+  for (int np = cap; np >= 2; np /= 2)
+    if (P % np == 0 && (size_t)(t.d_dn + 1) * np * sizeof(T) <= kMpRowLds) return np;
+  return 1;
+}
+
+template <typename T, Epi E, int NP>
+void launch_mp(dim3 g, dim3 b, size_t lds, hipStream_t s, const HvTables &t, const T *x, T *y,
+               const EpiArgs &a) {
+  if constexpr (NP * sizeof(T) <= 32) {
+    hipLaunchKernelGGL((row_kernel_mp<T, E, NP>), g, b, lds, s, t, x, y, a);
+    MB_HIP(hipGetLastError());
+  } else {
+    throw std::logic_error("multi-plane row kernel: pack wider than 32 bytes");
+  }
+}
+
 template <typename T, Epi E>
 void launch_row(const HvTables &t, int P, const T *x, T *y, const EpiArgs &a, hipStream_t s) {
   const size_t lds = (size_t)(t.d_dn + 1) * sizeof(T);  // + the zero slot of the compact table
@@ -1531,7 +1704,19 @@ void launch_row(const HvTables &t, int P, const T *x, T *y, const EpiArgs &a, hi
     MB_HIP(hipGetLastError());
     return;
   }
-  const dim3 g((unsigned)nrows), b(row_block(t.d_dn));
+  const dim3 b(row_block(t.d_dn));
+  const int np = t.comp_ok ? row_planes<T>(t, P) : 1;
+  if (np > 1) {
+    const dim3 g((unsigned)(nrows / np));
+    const size_t lm = (size_t)(t.d_dn + 1) * np * sizeof(T);
+    switch (np) {
+      case 2: launch_mp<T, E, 2>(g, b, lm, s, t, x, y, a); break;
+      case 4: launch_mp<T, E, 4>(g, b, lm, s, t, x, y, a); break;
+      default: launch_mp<T, E, 8>(g, b, lm, s, t, x, y, a); break;
+    }
+    return;
+  }
+  const dim3 g((unsigned)nrows);
   if (t.comp_ok)
     hipLaunchKernelGGL((row_kernel_compact<T, E>), g, b, lds, s, t, x, y, a);
   else

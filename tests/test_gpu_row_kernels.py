@@ -119,3 +119,67 @@ def test_kpm_sorted_matches_compact(cuda, fp32):
     if fp32:
         mu_64 = np.array(mb.kpm_dos_moments(op, a, bb, 64, 2, seed=21, fp32=False))
         assert np.abs(mu_s - mu_64).max() <= 1e-4
+
+
+# ------------------------------------------------------------ multi-plane row kernel
+
+@contextlib.contextmanager
+def row_planes(n):
+    old = os.environ.get("MANYBODY_AMD_ROW_PLANES")
+    os.environ["MANYBODY_AMD_ROW_PLANES"] = str(n)
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ["MANYBODY_AMD_ROW_PLANES"]
+        else:
+            os.environ["MANYBODY_AMD_ROW_PLANES"] = old
+
+
+@pytest.mark.parametrize("L,nu,nd", [(14, 7, 7), (12, 6, 6), (10, 5, 4)])
+def test_multi_plane_row_kernel_equals_one_plane_kernel(cuda, L, nu, nd):
+    """The row pass over several planes at once (two FP32 planes by default;
+    2, 4 or 8 with MANYBODY_AMD_ROW_PLANES) against the one-plane compact
+    kernel (MANYBODY_AMD_ROW_PLANES=1): the same
+    sums in the same order (the compiler may contract multiply-adds
+    differently), so H.v agrees to 1e-15 relative, Lanczos coefficients to
+    1e-13, FP64 KPM moments to 1e-13 and FP32 moments to 1e-6; integer inputs
+    bit for bit against the oracle."""
+    op = mb.HubbardModel(T_HOP, 4.0, L).operator(mb.FermionBasis.sector(L, nu, nd))
+    x = op.gaussian(17)
+    w, _ = mb.tridiag_eig(*[getattr(mb.lanczos_pass_one(op, x, 60), f) for f in ("alphas", "betas")],
+                          vectors=False)
+    a, c = (w[-1] - w[0]) / 2 * 1.05, (w[-1] + w[0]) / 2  # spectrum inside [-1, 1]
+    with row_planes(1):
+        y1 = op.hv(x)
+        dec1 = mb.lanczos_pass_one(op, x, 12)
+        mu1 = np.array(mb.kpm_dos_moments(op, a, c, 32, 8, 3, fp32=True))
+        mu1d = np.array(mb.kpm_dos_moments(op, a, c, 32, 4, 3, fp32=False))
+    for cap in (2, 4, 8):
+        with row_planes(cap):
+            y = op.hv(x)
+            dec = mb.lanczos_pass_one(op, x, 12)
+        assert torch.abs(y - y1).max().item() <= 1e-15 * torch.abs(y1).max().item()
+        assert np.abs(np.array(dec.alphas) - dec1.alphas).max() <= 1e-13 * np.abs(dec1.alphas).max()
+        assert np.abs(np.array(dec.betas) - dec1.betas).max() <= 1e-13 * np.abs(dec1.betas).max()
+        with row_planes(cap):
+            mu = np.array(mb.kpm_dos_moments(op, a, c, 32, 8, 3, fp32=True))
+            mud = np.array(mb.kpm_dos_moments(op, a, c, 32, 4, 3, fp32=False))
+        assert np.abs(mu - mu1).max() <= 1e-6, cap
+        assert np.abs(mud - mu1d).max() <= 1e-13, cap
+    # integer inputs: bit for bit against the oracle (FP64: the default kernel)
+    rng = np.random.default_rng(11)
+    D = op.dimension()
+    z = (rng.integers(-4, 5, D) + 1j * rng.integers(-4, 5, D)).astype(np.complex128)
+    P = po.ProductOperator([L], nu, nd, T_HOP, 4.0)
+    yz = op.hv(torch.from_numpy(np.stack([z.real, z.imag])).to(cuda)).cpu().numpy()
+    assert np.array_equal(yz[0] + 1j * yz[1], P.matvec(z))
+    # and the multi-vector FP32 H.v on integer inputs (the default two planes
+    # per workgroup, then eight)
+    zz = torch.from_numpy(np.stack([z.real, z.imag] * 4).astype(np.float32)).to(cuda)
+    ref = P.matvec(z).astype(np.complex64)
+    for cap in (None, 8):
+        with (row_planes(cap) if cap else contextlib.nullcontext()):
+            yy = op.hv(zz).cpu().numpy()
+        for r in range(4):
+            assert np.array_equal(yy[2 * r] + 1j * yy[2 * r + 1], ref), (cap, r)
