@@ -43,6 +43,10 @@ def main():
                          "that checks the 4x4 ground energy")
     ap.add_argument("--no-csc-baseline", action="store_true")
     ap.add_argument("--no-kpm", action="store_true")
+    ap.add_argument("--kpm-moments", type=int, default=1024,
+                    help="Chebyshev moments of the L=14 KPM leg (1024; fewer for profiling runs)")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the one-at-a-time comparison of the batched leg (profiling runs)")
     ap.add_argument("--no-momentum", action="store_true")
     ap.add_argument("--no-conductivity", action="store_true")
     ap.add_argument("--no-expression", action="store_true")
@@ -134,8 +138,8 @@ def main():
     momentum = None if args.no_momentum else bench_momentum()
     conductivity = None if args.no_conductivity else bench_conductivity()
     expression = None if args.no_expression else bench_expression()
-    batched = None if args.no_batched else bench_batched(args.batched_n)
-    kpm = None if args.no_kpm else bench_kpm_c3()
+    batched = None if args.no_batched else bench_batched(args.batched_n, not args.no_single)
+    kpm = None if args.no_kpm else bench_kpm_c3(M=args.kpm_moments)
     krylov = None if args.no_kpm else bench_krylov_fp32()
     serving = None if args.no_serving else bench_serving(args.serving_n, args.serving_rate)
     csc = None if args.no_csc_baseline else bench_cpu_csc_c1(threads)
@@ -617,7 +621,7 @@ def bench_serving(n, rate, slots=4, arena=64 << 30, lane=8 << 30):
             "in_lane": int(sum(1 for x in out if x["slot"] >= slots))}
 
 
-def bench_batched(n):
+def bench_batched(n, single=True):
     """512 independent 1D ring ground states (L ~ U{8..12}, N_up, N_dn ~
     U{1..L/2}, U ~ U[0,12], t = 1), 150 FP64 Lanczos steps each from seeded
     N(0,1) start vectors: batched engine (autotuned mapping, one hipGraph)
@@ -641,6 +645,11 @@ def bench_batched(n):
     t0 = time.perf_counter()
     e_batch = B.ground_energies(k, use_graph=True)
     t_batch = time.perf_counter() - t0
+    dims = np.array([op.dimension() for op in ops])
+    if not single:
+        return {"workload": f"{n} x 1D Hubbard ring ground states, L 8-12, 150 FP64 Lanczos steps",
+                "solves_per_s": round(n / t_batch, 1), "batch_s": round(t_batch, 4),
+                "mapping": {"tile": tile, "block": block}}
     # plain path: one find_min_eigenpair per instance (start vectors filled beforehand)
     starts = [op.gaussian(int(sd)) for op, sd in zip(ops, seeds)]
     torch.cuda.synchronize()
@@ -648,7 +657,6 @@ def bench_batched(n):
     e_single = np.array([mb.find_min_eigenpair(op, k, b, want_vector=False).value
                          for op, b in zip(ops, starts)])
     t_single = time.perf_counter() - t0
-    dims = np.array([op.dimension() for op in ops])
     return {"workload": f"{n} x 1D Hubbard ring ground states, L 8-12, 150 FP64 Lanczos steps",
             "solves_per_s": round(n / t_batch, 1), "batch_s": round(t_batch, 4),
             "p50_ms": round(t_batch * 1e3, 2), "p99_ms": round(t_batch * 1e3, 2),
