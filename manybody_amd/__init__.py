@@ -157,19 +157,13 @@ _SIGS = {
 
 
 def lib():
-    """Load libmanybody_hip.so (raises if it has not been built)."""
+    """Load libmanybody_hip.so (raises if it has not been built).  Loading it
+    touches no device; the functions that create device objects go through
+    _dev(), which brings torch's HIP runtime up first."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
-        # torch carries its own HIP runtime; it has to bring the device up before
-        # this library's runtime does, or torch later finds no device (the
-        # library's tensors are torch's), so initialise it first
-        try:
-            import torch
-            torch.cuda.is_available()
-        except ImportError:
-            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)
@@ -177,6 +171,28 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+_torch_ready = False
+
+
+def _dev():
+    """lib(), once torch's HIP runtime is up.  torch carries its own HIP
+    runtime; it has to bring the device up before this library's runtime does,
+    or torch later finds no device (the library's vectors are torch tensors), so
+    every call that creates a device object (operators, batches, servers,
+    current operators, device-built matrices) comes through here.  Host-only
+    calls (bases, kernels of the KPM and rescaling helpers, the tridiagonal
+    solver) use lib() and never import torch."""
+    global _torch_ready
+    if not _torch_ready:
+        try:
+            import torch
+            torch.cuda.is_available()
+        except ImportError:
+            pass
+        _torch_ready = True
+    return lib()
 
 
 def _check(status: int):
@@ -308,7 +324,7 @@ class _HubbardBase:
     def operator(self, basis: FermionBasis) -> "HubbardOperator":
         sizes = (C.c_int * len(self.sizes))(*self.sizes)
         h = C.c_void_p()
-        _check(lib().mb_hubbard_create(basis._h, len(self.sizes), sizes, self.t, self.u, C.byref(h)))
+        _check(_dev().mb_hubbard_create(basis._h, len(self.sizes), sizes, self.t, self.u, C.byref(h)))
         return HubbardOperator(h, basis, self)
 
 
@@ -346,7 +362,7 @@ def HubbardMomentumFactorizedFixedK(basis: FermionBasis, lattice_size, t: float,
     sizes = tuple(int(v) for v in lattice_size)
     K = tuple(int(v) for v in target_momentum)
     h = C.c_void_p()
-    _check(lib().mb_momentum_hubbard_create(basis._h, len(sizes), (C.c_int * len(sizes))(*sizes),
+    _check(_dev().mb_momentum_hubbard_create(basis._h, len(sizes), (C.c_int * len(sizes))(*sizes),
                                             float(t), float(u), (C.c_int * len(K))(*K), C.byref(h)))
     return HubbardOperator(h, basis, None)
 
@@ -358,7 +374,7 @@ def FermionExpressionOperator(basis: FermionBasis, expression) -> "HubbardOperat
     fixed-momentum basis."""
     coeffs, offsets, ops = expression.arrays()
     h = C.c_void_p()
-    _check(lib().mb_expression_create(basis._h, len(offsets) - 1,
+    _check(_dev().mb_expression_create(basis._h, len(offsets) - 1,
                                       coeffs.ctypes.data_as(C.c_void_p),
                                       offsets.ctypes.data_as(C.c_void_p),
                                       ops.ctypes.data_as(C.c_void_p), C.byref(h)))
@@ -373,7 +389,7 @@ def compute_matrix_elements(basis_or_op, expression=None):
     from .expression import SparseMatrix
     op = basis_or_op if expression is None else FermionExpressionOperator(basis_or_op, expression)
     h = C.c_void_p()
-    _check(lib().mb_compute_matrix_elements(op._h, C.byref(h), _stream()))
+    _check(_dev().mb_compute_matrix_elements(op._h, C.byref(h), _stream()))
     return SparseMatrix(h)
 
 
@@ -387,46 +403,46 @@ def _composite(fn, operands, *args):
 
 def Negated(op: "HubbardOperator") -> "HubbardOperator":
     """Negated (linear_operator.h:93-105): -A."""
-    return _composite(lib().mb_operator_negated, (op,))
+    return _composite(_dev().mb_operator_negated, (op,))
 
 
 def Scaled(op: "HubbardOperator", scale) -> "HubbardOperator":
     """Scaled (linear_operator.h:107-120): c A, c complex."""
     c = complex(scale)
-    return _composite(lib().mb_operator_scaled, (op,), c.real, c.imag)
+    return _composite(_dev().mb_operator_scaled, (op,), c.real, c.imag)
 
 
 def Shifted(op: "HubbardOperator", shift) -> "HubbardOperator":
     """Shifted (linear_operator.h:122-135): A + c I."""
     c = complex(shift)
-    return _composite(lib().mb_operator_shifted, (op,), c.real, c.imag)
+    return _composite(_dev().mb_operator_shifted, (op,), c.real, c.imag)
 
 
 def NegShift(op: "HubbardOperator", shift) -> "HubbardOperator":
     """NegShift (linear_operator.h:137-150): -A + c I."""
     c = complex(shift)
-    return _composite(lib().mb_operator_neg_shifted, (op,), c.real, c.imag)
+    return _composite(_dev().mb_operator_neg_shifted, (op,), c.real, c.imag)
 
 
 def Sum(a: "HubbardOperator", b: "HubbardOperator") -> "HubbardOperator":
     """Sum (linear_operator.h:152-172): A + B on bases of the same kind."""
-    return _composite(lib().mb_operator_sum, (a, b))
+    return _composite(_dev().mb_operator_sum, (a, b))
 
 
 def Difference(a: "HubbardOperator", b: "HubbardOperator") -> "HubbardOperator":
     """Difference (linear_operator.h:174-194): A - B."""
-    return _composite(lib().mb_operator_difference, (a, b))
+    return _composite(_dev().mb_operator_difference, (a, b))
 
 
 def Composed(a: "HubbardOperator", b: "HubbardOperator") -> "HubbardOperator":
     """Composed (linear_operator.h:196-216): A B, B applied first."""
-    return _composite(lib().mb_operator_composed, (a, b))
+    return _composite(_dev().mb_operator_composed, (a, b))
 
 
 def Identity(basis: FermionBasis) -> "HubbardOperator":
     """Identity (linear_operator.h:218-228) on the basis."""
     h = C.c_void_p()
-    _check(lib().mb_operator_identity(basis._h, C.byref(h)))
+    _check(_dev().mb_operator_identity(basis._h, C.byref(h)))
     return HubbardOperator(h, basis, None)
 
 
@@ -439,7 +455,7 @@ def compute_rectangular_matrix_elements(row_basis: FermionBasis, col_basis: Ferm
     from .expression import SparseMatrix
     coeffs, offsets, ops = expression.arrays()
     h = C.c_void_p()
-    _check(lib().mb_compute_rectangular_matrix_elements(
+    _check(_dev().mb_compute_rectangular_matrix_elements(
         row_basis._h, col_basis._h, len(offsets) - 1, coeffs.ctypes.data_as(C.c_void_p),
         offsets.ctypes.data_as(C.c_void_p), ops.ctypes.data_as(C.c_void_p), C.byref(h),
         _stream()))
@@ -452,7 +468,7 @@ def compute_vector_elements(basis: FermionBasis, expression):
     import numpy as np
     coeffs, offsets, ops = expression.arrays()
     out = np.zeros(2 * max(basis.size(), 1), np.float64)
-    _check(lib().mb_compute_vector_elements(
+    _check(_dev().mb_compute_vector_elements(
         basis._h, len(offsets) - 1, coeffs.ctypes.data_as(C.c_void_p),
         offsets.ctypes.data_as(C.c_void_p), ops.ctypes.data_as(C.c_void_p),
         out.ctypes.data_as(C.c_void_p), _stream()))
@@ -466,6 +482,10 @@ class HubbardOperator:
         self._h = handle
         self.basis = basis  # keeps the basis alive (the model references it)
         self.model = model
+        # a real-space Hubbard model on a fixed (N_up, N_dn) sector: the kind
+        # with the FP32 H.v, FP32 Krylov and multi-vector KPM paths
+        self.real_space = (model is not None and basis.n_up is not None
+                           and not hasattr(basis, "momentum"))
         du, dd = C.c_int64(), C.c_int64()
         _check(lib().mb_hubbard_shape(self._h, C.byref(du), C.byref(dd)))
         self.d_up, self.d_dn = int(du.value), int(dd.value)
@@ -580,9 +600,13 @@ class HubbardOperator:
         _check(lib().mb_fill_gaussian_c128(self._h, seed, _ptr(x), _stream()))
         return x
 
-    def random_phase(self, seed: int, nvec: int, fp32: bool = True):
-        """nvec random-phase vectors e^{i phi} in device order (2 nvec, D)."""
+    def random_phase(self, seed: int, nvec: int, fp32: Optional[bool] = None):
+        """nvec random-phase vectors e^{i phi} in device order (2 nvec, D);
+        fp32=None: FP32 for real-space Hubbard operators (the precision their
+        KPM path runs in), FP64 otherwise."""
         import torch
+        if fp32 is None:
+            fp32 = self.real_space
         x = self.empty(torch.float32 if fp32 else torch.float64, nvec)
         if fp32:
             _check(lib().mb_fill_phase_c64_multi(self._h, seed, nvec, _ptr(x), _stream()))
@@ -742,7 +766,7 @@ class CurrentOperatorFixedK:
                  t: float, Q: int, direction: int):
         sizes = tuple(int(v) for v in lattice_size)
         h = C.c_void_p()
-        _check(lib().mb_current_create(source_basis._h, target_basis._h, len(sizes),
+        _check(_dev().mb_current_create(source_basis._h, target_basis._h, len(sizes),
                                        (C.c_int * len(sizes))(*sizes), float(t), int(Q),
                                        int(direction), C.byref(h)))
         self._h = h
@@ -891,12 +915,16 @@ def estimate_bounds(op: HubbardOperator, seed, power_iterations: int,
 
 
 def kpm_dos_moments(op: HubbardOperator, a: float, b: float, M: int, nvec: int, seed: int,
-                    fp32: bool = True):
+                    fp32: Optional[bool] = None):
     """KPM density-of-states moments mu_0 .. mu_{M-1} of (H - b) / a with nvec
     random-phase vectors (phases keyed by reference index).  Real-space Hubbard
     operators run the multi-vector FP32 / FP64 path; every other operator
-    (momentum sectors, fixed-N bases, expressions, composites) runs FP64, one
-    vector at a time, with the same phases."""
+    (momentum sectors, fixed-N bases, expressions, composites) runs FP64 only,
+    one vector at a time, with the same phases.  fp32=None picks FP32 H.v for
+    real-space Hubbard operators and FP64 otherwise; fp32=True on another
+    operator raises ValueError."""
+    if fp32 is None:
+        fp32 = op.real_space
     mu = (C.c_double * M)()
     _check(lib().mb_kpm_dos_moments(op._h, a, b, M, nvec, seed, int(fp32), mu, _stream()))
     return list(mu)
@@ -1037,7 +1065,7 @@ def estimate_rescaling(op: "HubbardOperator", padding: float = 0.01, k: int = 30
 def rescale_hamiltonian(op: "HubbardOperator", r: Rescaling) -> "HubbardOperator":
     """rescaling::rescale_hamiltonian (rescaling.h:40-48): (H - b I) / a as an
     operator (composite; it references op)."""
-    return _composite(lib().mb_operator_rescaled, (op,), r.a, r.b)
+    return _composite(_dev().mb_operator_rescaled, (op,), r.a, r.b)
 
 
 # ------------------------------------------------------------------ batched
@@ -1050,7 +1078,7 @@ class Batch:
         self.ops = list(ops)
         arr = (C.c_void_p * len(self.ops))(*[op._h for op in self.ops])
         h = C.c_void_p()
-        _check(lib().mb_batch_create(len(self.ops), arr, C.byref(h)))
+        _check(_dev().mb_batch_create(len(self.ops), arr, C.byref(h)))
         self._h = h
 
     def __del__(self):
@@ -1169,7 +1197,7 @@ class Server:
 
     def __init__(self, slots: int = 8, arena_bytes: int = 1 << 34, lane_bytes: int = 0):
         h = C.c_void_p()
-        _check(lib().mb_server_create(slots, int(arena_bytes), int(lane_bytes), C.byref(h)))
+        _check(_dev().mb_server_create(slots, int(arena_bytes), int(lane_bytes), C.byref(h)))
         self._h = h
 
     def close(self):

@@ -355,3 +355,44 @@ def test_reference_rescaling_host_cases():
     assert abs(r.a - 3.0) < 1e-14 and abs(r.b - 1.0) < 1e-14
     r = mb.Rescaling.from_bounds(-2.0, 4.0, 0.1)
     assert abs(r.a - 3.3) < 1e-14 and abs(r.b - 1.0) < 1e-14
+
+
+def test_host_only_use_does_not_import_torch():
+    """Loading the library and using its host-side entry points (bases, index
+    lookups, KPM kernels, rescaling, the tridiagonal solver) imports no torch and
+    starts no HIP runtime of torch's; only calls that create device objects
+    bring torch up first.  Child process: nothing imported beforehand."""
+    import subprocess
+    import sys
+    code = r'''
+import sys
+sys.path.insert(0, %r)
+import manybody_amd as mb
+mb.lib()
+b = mb.FermionBasis.sector(6, 3, 3)
+assert b.index_of(*b.state(7)) == 7
+mb.jackson_kernel(8); mb.rescaling_from_bounds(-2.0, 3.0); mb.projector_moments(8, 0.1)
+mb.tridiag_eig([1.0, 2.0], [0.5])
+print("torch imported:", "torch" in sys.modules)
+''' % (str(__import__("os").path.dirname(mb.__path__[0])),)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "torch imported: False" in r.stdout
+
+
+def test_host_sanitizer_driver_host_part():
+    """tests/cpp/test_host_sanitize.cpp against the host-sanitized library
+    (AddressSanitizer with leak detection, UndefinedBehaviorSanitizer): bases,
+    lookups, error paths, host kernels, the sparse container and its cereal
+    file, request footprints.  Any report fails the run (nonzero status)."""
+    import os
+    import subprocess
+    binary = os.path.join(os.path.dirname(__file__), "cpp", "_build", "test_host_sanitize")
+    if not os.path.exists(binary):
+        pytest.skip("host-sanitizer driver not built (__graft_entry__.build())")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1", UBSAN_OPTIONS="halt_on_error=1",
+               HIP_VISIBLE_DEVICES="")
+    r = subprocess.run([binary], capture_output=True, text=True, timeout=300, env=env)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "HOST PASSED" in r.stdout
+    assert "Sanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-3000:]
