@@ -157,13 +157,25 @@ _SIGS = {
 
 
 def lib():
-    """Load libmanybody_hip.so (raises if it has not been built).  Loading it
-    touches no device; the functions that create device objects go through
-    _dev(), which brings torch's HIP runtime up first."""
+    """Load libmanybody_hip.so (raises if it has not been built).
+
+    torch carries its own HIP runtime, and it has to bring the device up before
+    this library's does (loading the library registers its device code with the
+    HIP runtime), or torch later finds no device and the library's allocations
+    fail: so on a machine with a GPU (/dev/kfd present) torch is initialised
+    first.  Without a GPU the library loads alone: host-only users (bases,
+    index lookups, the KPM and rescaling helpers, the tridiagonal solver) never
+    import torch."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        if os.path.exists("/dev/kfd"):
+            try:
+                import torch
+                torch.cuda.is_available()
+            except ImportError:
+                pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)
@@ -171,28 +183,6 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
-
-
-_torch_ready = False
-
-
-def _dev():
-    """lib(), once torch's HIP runtime is up.  torch carries its own HIP
-    runtime; it has to bring the device up before this library's runtime does,
-    or torch later finds no device (the library's vectors are torch tensors), so
-    every call that creates a device object (operators, batches, servers,
-    current operators, device-built matrices) comes through here.  Host-only
-    calls (bases, kernels of the KPM and rescaling helpers, the tridiagonal
-    solver) use lib() and never import torch."""
-    global _torch_ready
-    if not _torch_ready:
-        try:
-            import torch
-            torch.cuda.is_available()
-        except ImportError:
-            pass
-        _torch_ready = True
-    return lib()
 
 
 def _check(status: int):
@@ -324,7 +314,7 @@ class _HubbardBase:
     def operator(self, basis: FermionBasis) -> "HubbardOperator":
         sizes = (C.c_int * len(self.sizes))(*self.sizes)
         h = C.c_void_p()
-        _check(_dev().mb_hubbard_create(basis._h, len(self.sizes), sizes, self.t, self.u, C.byref(h)))
+        _check(lib().mb_hubbard_create(basis._h, len(self.sizes), sizes, self.t, self.u, C.byref(h)))
         return HubbardOperator(h, basis, self)
 
 
@@ -362,7 +352,7 @@ def HubbardMomentumFactorizedFixedK(basis: FermionBasis, lattice_size, t: float,
     sizes = tuple(int(v) for v in lattice_size)
     K = tuple(int(v) for v in target_momentum)
     h = C.c_void_p()
-    _check(_dev().mb_momentum_hubbard_create(basis._h, len(sizes), (C.c_int * len(sizes))(*sizes),
+    _check(lib().mb_momentum_hubbard_create(basis._h, len(sizes), (C.c_int * len(sizes))(*sizes),
                                             float(t), float(u), (C.c_int * len(K))(*K), C.byref(h)))
     return HubbardOperator(h, basis, None)
 
@@ -374,7 +364,7 @@ def FermionExpressionOperator(basis: FermionBasis, expression) -> "HubbardOperat
     fixed-momentum basis."""
     coeffs, offsets, ops = expression.arrays()
     h = C.c_void_p()
-    _check(_dev().mb_expression_create(basis._h, len(offsets) - 1,
+    _check(lib().mb_expression_create(basis._h, len(offsets) - 1,
                                       coeffs.ctypes.data_as(C.c_void_p),
                                       offsets.ctypes.data_as(C.c_void_p),
                                       ops.ctypes.data_as(C.c_void_p), C.byref(h)))
@@ -389,7 +379,7 @@ def compute_matrix_elements(basis_or_op, expression=None):
     from .expression import SparseMatrix
     op = basis_or_op if expression is None else FermionExpressionOperator(basis_or_op, expression)
     h = C.c_void_p()
-    _check(_dev().mb_compute_matrix_elements(op._h, C.byref(h), _stream()))
+    _check(lib().mb_compute_matrix_elements(op._h, C.byref(h), _stream()))
     return SparseMatrix(h)
 
 
@@ -403,46 +393,46 @@ def _composite(fn, operands, *args):
 
 def Negated(op: "HubbardOperator") -> "HubbardOperator":
     """Negated (linear_operator.h:93-105): -A."""
-    return _composite(_dev().mb_operator_negated, (op,))
+    return _composite(lib().mb_operator_negated, (op,))
 
 
 def Scaled(op: "HubbardOperator", scale) -> "HubbardOperator":
     """Scaled (linear_operator.h:107-120): c A, c complex."""
     c = complex(scale)
-    return _composite(_dev().mb_operator_scaled, (op,), c.real, c.imag)
+    return _composite(lib().mb_operator_scaled, (op,), c.real, c.imag)
 
 
 def Shifted(op: "HubbardOperator", shift) -> "HubbardOperator":
     """Shifted (linear_operator.h:122-135): A + c I."""
     c = complex(shift)
-    return _composite(_dev().mb_operator_shifted, (op,), c.real, c.imag)
+    return _composite(lib().mb_operator_shifted, (op,), c.real, c.imag)
 
 
 def NegShift(op: "HubbardOperator", shift) -> "HubbardOperator":
     """NegShift (linear_operator.h:137-150): -A + c I."""
     c = complex(shift)
-    return _composite(_dev().mb_operator_neg_shifted, (op,), c.real, c.imag)
+    return _composite(lib().mb_operator_neg_shifted, (op,), c.real, c.imag)
 
 
 def Sum(a: "HubbardOperator", b: "HubbardOperator") -> "HubbardOperator":
     """Sum (linear_operator.h:152-172): A + B on bases of the same kind."""
-    return _composite(_dev().mb_operator_sum, (a, b))
+    return _composite(lib().mb_operator_sum, (a, b))
 
 
 def Difference(a: "HubbardOperator", b: "HubbardOperator") -> "HubbardOperator":
     """Difference (linear_operator.h:174-194): A - B."""
-    return _composite(_dev().mb_operator_difference, (a, b))
+    return _composite(lib().mb_operator_difference, (a, b))
 
 
 def Composed(a: "HubbardOperator", b: "HubbardOperator") -> "HubbardOperator":
     """Composed (linear_operator.h:196-216): A B, B applied first."""
-    return _composite(_dev().mb_operator_composed, (a, b))
+    return _composite(lib().mb_operator_composed, (a, b))
 
 
 def Identity(basis: FermionBasis) -> "HubbardOperator":
     """Identity (linear_operator.h:218-228) on the basis."""
     h = C.c_void_p()
-    _check(_dev().mb_operator_identity(basis._h, C.byref(h)))
+    _check(lib().mb_operator_identity(basis._h, C.byref(h)))
     return HubbardOperator(h, basis, None)
 
 
@@ -455,7 +445,7 @@ def compute_rectangular_matrix_elements(row_basis: FermionBasis, col_basis: Ferm
     from .expression import SparseMatrix
     coeffs, offsets, ops = expression.arrays()
     h = C.c_void_p()
-    _check(_dev().mb_compute_rectangular_matrix_elements(
+    _check(lib().mb_compute_rectangular_matrix_elements(
         row_basis._h, col_basis._h, len(offsets) - 1, coeffs.ctypes.data_as(C.c_void_p),
         offsets.ctypes.data_as(C.c_void_p), ops.ctypes.data_as(C.c_void_p), C.byref(h),
         _stream()))
@@ -468,7 +458,7 @@ def compute_vector_elements(basis: FermionBasis, expression):
     import numpy as np
     coeffs, offsets, ops = expression.arrays()
     out = np.zeros(2 * max(basis.size(), 1), np.float64)
-    _check(_dev().mb_compute_vector_elements(
+    _check(lib().mb_compute_vector_elements(
         basis._h, len(offsets) - 1, coeffs.ctypes.data_as(C.c_void_p),
         offsets.ctypes.data_as(C.c_void_p), ops.ctypes.data_as(C.c_void_p),
         out.ctypes.data_as(C.c_void_p), _stream()))
@@ -766,7 +756,7 @@ class CurrentOperatorFixedK:
                  t: float, Q: int, direction: int):
         sizes = tuple(int(v) for v in lattice_size)
         h = C.c_void_p()
-        _check(_dev().mb_current_create(source_basis._h, target_basis._h, len(sizes),
+        _check(lib().mb_current_create(source_basis._h, target_basis._h, len(sizes),
                                        (C.c_int * len(sizes))(*sizes), float(t), int(Q),
                                        int(direction), C.byref(h)))
         self._h = h
@@ -1065,7 +1055,7 @@ def estimate_rescaling(op: "HubbardOperator", padding: float = 0.01, k: int = 30
 def rescale_hamiltonian(op: "HubbardOperator", r: Rescaling) -> "HubbardOperator":
     """rescaling::rescale_hamiltonian (rescaling.h:40-48): (H - b I) / a as an
     operator (composite; it references op)."""
-    return _composite(_dev().mb_operator_rescaled, (op,), r.a, r.b)
+    return _composite(lib().mb_operator_rescaled, (op,), r.a, r.b)
 
 
 # ------------------------------------------------------------------ batched
@@ -1078,7 +1068,7 @@ class Batch:
         self.ops = list(ops)
         arr = (C.c_void_p * len(self.ops))(*[op._h for op in self.ops])
         h = C.c_void_p()
-        _check(_dev().mb_batch_create(len(self.ops), arr, C.byref(h)))
+        _check(lib().mb_batch_create(len(self.ops), arr, C.byref(h)))
         self._h = h
 
     def __del__(self):
@@ -1197,7 +1187,7 @@ class Server:
 
     def __init__(self, slots: int = 8, arena_bytes: int = 1 << 34, lane_bytes: int = 0):
         h = C.c_void_p()
-        _check(_dev().mb_server_create(slots, int(arena_bytes), int(lane_bytes), C.byref(h)))
+        _check(lib().mb_server_create(slots, int(arena_bytes), int(lane_bytes), C.byref(h)))
         self._h = h
 
     def close(self):

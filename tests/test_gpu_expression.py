@@ -240,8 +240,10 @@ def test_expression_operator_rejects_invalid_use(cuda):
     x = op.gaussian(seed=1)
     with pytest.raises(ValueError):
         op.hv(x.float())
-    with pytest.raises(ValueError):
-        mb.kpm_dos_moments(op, 10.0, 0.0, 8, 1, seed=1)
+    with pytest.raises(ValueError):  # FP32 KPM needs a real-space Hubbard operator
+        mb.kpm_dos_moments(op, 10.0, 0.0, 8, 1, seed=1, fp32=True)
+    mu = mb.kpm_dos_moments(op, 10.0, 0.0, 8, 1, seed=1)  # the default runs FP64 here
+    assert abs(mu[0] - 1.0) < 1e-14
     with pytest.raises(ValueError):
         op.opposite_spin_correlation(x)
     empty = mb.FermionExpressionOperator(basis, FermionExpression())

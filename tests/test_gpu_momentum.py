@@ -108,8 +108,8 @@ def test_momentum_operator_rejects_real_space_only_calls(cuda):
     x = op.gaussian(seed=3)
     with pytest.raises(ValueError):
         op.hv(x.float())
-    with pytest.raises(ValueError):
-        mb.kpm_dos_moments(op, 30.0, 0.0, 8, 1, seed=1)
+    with pytest.raises(ValueError):  # FP32 KPM needs a real-space Hubbard operator
+        mb.kpm_dos_moments(op, 30.0, 0.0, 8, 1, seed=1, fp32=True)
     with pytest.raises(ValueError):
         op.opposite_spin_correlation(x)
     basis = mb.FermionBasis.with_fixed_particle_number_spin_momentum(6, 5, 1, [6], [0])
