@@ -474,6 +474,10 @@ int mb_batch_fill_gaussian(mb_batch *b, const uint64_t *seeds, mb_stream stream)
 int mb_batch_set_start(mb_batch *b, int i, const void *b_dev, mb_stream stream);
 /* Work mapping: elements per tile (one workgroup) and workgroup size. */
 int mb_batch_set_mapping(mb_batch *b, int tile, int block);
+/* Instances of dimension up to max_dim (default 2048) run all their Lanczos
+ * steps inside one persistent launch, one workgroup per instance, beside the
+ * per-phase kernels of the larger ones (0: every instance tiled). */
+int mb_batch_set_persistent(mb_batch *b, int64_t max_dim);
 /* Times k_probe steps for each candidate mapping and keeps the fastest. */
 int mb_batch_autotune(mb_batch *b, int k_probe, int *tile, int *block, mb_stream stream);
 /* k steps of lanczos_pass_one for every instance (optionally replayed as one

@@ -89,6 +89,7 @@ _SIGS = {
     "mb_batch_set_start": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "mb_batch_set_mapping": (C.c_int, [_vp, C.c_int, C.c_int]),
     "mb_batch_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
+    "mb_batch_set_persistent": (C.c_int, [_vp, C.c_int64]),
     "mb_batch_lanczos": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "mb_batch_ground_energies": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp]),
     "mb_lanczos_pass_one_ex": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(C.c_int),
@@ -1092,6 +1093,11 @@ class Batch:
 
     def set_mapping(self, tile: int, block: int):
         _check(lib().mb_batch_set_mapping(self._h, tile, block))
+
+    def set_persistent(self, max_dim: int):
+        """Instances up to max_dim run every step in the persistent per-instance
+        kernel (0: all tiled over the per-phase kernels)."""
+        _check(lib().mb_batch_set_persistent(self._h, int(max_dim)))
 
     def autotune(self, k_probe: int = 3):
         t, b = C.c_int(), C.c_int()

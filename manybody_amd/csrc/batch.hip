@@ -144,6 +144,7 @@ __global__ __launch_bounds__(kWave) void bl_reduce_kernel(const BInst *__restric
     }
     return;
   }
+  if (inst[i].small) return;  // the persistent kernel keeps its own scalars
   if (done[i] && what != 2) return;
   const BInst I = inst[i];
   double s = 0.0;
@@ -162,6 +163,95 @@ __global__ __launch_bounds__(kWave) void bl_reduce_kernel(const BInst *__restric
       bnorm[i] = b;
       if (b <= kTol) done[i] = 2;
     }
+  }
+}
+
+// Fixed-order sum over the workgroup, the result in every thread.
+template <int BS>
+__device__ __forceinline__ double block_allsum(double v, double *lds) {
+  const int tid = threadIdx.x;
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+  if ((tid & (kWave - 1)) == 0) lds[tid / kWave] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int w = 0; w < BS / kWave; ++w) r += lds[w];
+  __syncthreads();
+  return r;
+}
+
+// One workgroup per small instance runs all k steps of lanczos_pass_one
+// (lanczos.h:130-175, recurrence :15-40) without leaving the kernel: |b|, v =
+// b / |b|, then per step w = H v - beta v_prev, alpha = Re <v|w>, w -= alpha v,
+// beta = |w|, stop at beta <= 1000 eps (the step is counted, its beta kept but
+// not used), else v_next = w / beta.  The three vectors rotate through V, VP
+// and W; every reduction is a fixed-order workgroup sum, so a replay gives the
+// same bits.  Writes alpha / beta / steps / bnorm / done as the per-phase
+// kernels do.
+template <int BS>
+__global__ __launch_bounds__(BS) void bl_persistent_kernel(const BInst *__restrict__ inst,
+                                                           const int *__restrict__ small_ids,
+                                                           const double *__restrict__ B,
+                                                           double *V, double *VP, double *Wv,
+                                                           double *alpha, double *beta,
+                                                           double *bnorm, int *done, int *steps,
+                                                           int k, int kcap) {
+  __shared__ double lds[BS / kWave];
+  const int i = small_ids[blockIdx.x];
+  const BInst I = inst[i];
+  const int64_t D = I.D, n2 = 2 * D;
+  const int tid = threadIdx.x;
+  const double *b = B + I.off;
+  double *v = V + I.off, *vp = VP + I.off, *w = Wv + I.off;
+  double s = 0.0;
+  for (int64_t e = tid; e < n2; e += BS) s += b[e] * b[e];
+  const double bn = sqrt(block_allsum<BS>(s, lds));
+  if (tid == 0) {
+    bnorm[i] = bn;
+    steps[i] = 0;
+    done[i] = bn <= kTol ? 2 : 0;
+  }
+  if (bn <= kTol) return;
+  for (int64_t e = tid; e < n2; e += BS) {
+    v[e] = b[e] / bn;
+    vp[e] = 0.0;
+  }
+  __syncthreads();
+  double bprev = 0.0;
+  for (int j = 0; j < k; ++j) {
+    s = 0.0;
+    for (int64_t e = tid; e < D; e += BS) {
+      double re, im;
+      hv_pair(I.t, v, D, e, re, im);
+      if (bprev != 0.0) {
+        re -= bprev * vp[e];
+        im -= bprev * vp[D + e];
+      }
+      w[e] = re;
+      w[D + e] = im;
+      s += v[e] * re + v[D + e] * im;
+    }
+    const double a = block_allsum<BS>(s, lds);
+    s = 0.0;
+    for (int64_t e = tid; e < n2; e += BS) {
+      const double q = w[e] - a * v[e];
+      w[e] = q;
+      s += q * q;
+    }
+    const double bt = sqrt(block_allsum<BS>(s, lds));
+    if (tid == 0) {
+      alpha[(int64_t)i * kcap + j] = a;
+      beta[(int64_t)i * kcap + j] = bt;
+      steps[i] = j + 1;
+      if (bt <= kTol) done[i] = 1;
+    }
+    if (bt <= kTol) return;
+    for (int64_t e = tid; e < n2; e += BS) w[e] /= bt;
+    double *t = vp;
+    vp = v;
+    v = w;
+    w = t;
+    bprev = bt;
+    __syncthreads();
   }
 }
 
@@ -207,6 +297,12 @@ void Batch::set_mapping(int tile, int block) {
   for (size_t i = 0; i < host_inst_.size(); ++i) {
     BInst &I = host_inst_[i];
     I.tile0 = (int)tiles_.size();
+    I.small = I.D <= persist_max_dim_ ? 1 : 0;
+    if (I.small) {
+      I.tile1 = I.tile0;
+      I.rows = 0;
+      continue;
+    }
     const int64_t Ddn = I.t.d_dn;
     // row tiles when the compact down table exists and a row is short
     const int64_t rows_n = std::max<int64_t>(1, tile / Ddn) * Ddn;
@@ -223,6 +319,18 @@ void Batch::set_mapping(int tile, int block) {
     }
     I.tile1 = (int)tiles_.size();
   }
+  if (tiles_.empty()) {  // every instance is small: one idle tile keeps the launches valid
+    BTile tl;
+    tl.inst = 0;
+    tl.e0 = 0;
+    tl.n = 0;
+    tiles_.push_back(tl);
+  }
+  std::vector<int> small;
+  for (size_t i = 0; i < host_inst_.size(); ++i)
+    if (host_inst_[i].small) small.push_back((int)i);
+  n_small_ = (int)small.size();
+  d_small_ = upload(small.empty() ? std::vector<int>{0} : small);
   d_tiles_ = upload(tiles_);
   MB_HIP(hipMemcpy(d_inst_.p, host_inst_.data(), sizeof(BInst) * host_inst_.size(),
                    hipMemcpyHostToDevice));
@@ -236,6 +344,12 @@ void Batch::set_mapping(int tile, int block) {
 
 Batch::~Batch() {
   if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (side_) {
+    (void)hipStreamSynchronize(side_);
+    (void)hipEventDestroy(fork_);
+    (void)hipEventDestroy(join_);
+    (void)hipStreamDestroy(side_);
+  }
 }
 
 template <int BS>
@@ -302,11 +416,40 @@ void Batch::ensure_k(int k) {
   }
 }
 
+void Batch::persistent_run(int k, hipStream_t s) const {
+  if (n_small_ == 0) return;
+  constexpr int BS = 256;
+  hipLaunchKernelGGL((bl_persistent_kernel<BS>), dim3((unsigned)n_small_), dim3(BS), 0, s,
+                     d_inst_.as<BInst>(), d_small_.as<int>(), B_.as<double>(), V_.as<double>(),
+                     VP_.as<double>(), W_.as<double>(), alpha_.as<double>(), beta_.as<double>(),
+                     bnorm_.as<double>(), done_.as<int>(), steps_.as<int>(), k, kcap_);
+  MB_HIP(hipGetLastError());
+}
+
+void Batch::set_persist_max_dim(int64_t max_dim) {
+  persist_max_dim_ = max_dim;
+  set_mapping(tile_, block_);
+}
+
 // Start vectors (in B_) -> normalised V, zero VP, reset flags; then k steps.
+// The small instances run in the persistent kernel on a second stream forked
+// after the reset (captured as a parallel branch of the graph), beside the
+// per-phase kernels of the tiled instances.
 void Batch::record_run(int k, hipStream_t s) {
   const int n = (int)host_inst_.size();
   (void)n;
   reduce(kcap_, 0, 3, s);  // reset done / steps
+  if (n_small_ > 0) {
+    if (!side_) {
+      MB_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+      MB_HIP(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+      MB_HIP(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
+    }
+    MB_HIP(hipEventRecord(fork_, s));
+    MB_HIP(hipStreamWaitEvent(side_, fork_, 0));
+    persistent_run(k, side_);
+    MB_HIP(hipEventRecord(join_, side_));
+  }
   vec_step(nullptr, nullptr, B_.as<double>(), kcap_, 0, 2, s);  // |b|^2 partials
   reduce(kcap_, 0, 2, s);                                      // bnorm, zero-norm flag
   // x: the unnormalised input of the step (b, then the previous w); the step
@@ -326,6 +469,7 @@ void Batch::record_run(int k, hipStream_t s) {
     v = old_vp;
     w = free_buf;
   }
+  if (n_small_ > 0) MB_HIP(hipStreamWaitEvent(s, join_, 0));
   MB_HIP(hipGetLastError());
 }
 

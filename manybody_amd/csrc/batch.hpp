@@ -17,7 +17,16 @@ struct BInst {
   int64_t D;
   int tile0, tile1;
   int rows;     // 1: tiles are whole rows, down hops read the rows from LDS
+  int small;    // 1: all k steps in the persistent per-instance kernel (no tiles)
 };
+
+// Instances up to this dimension run every Lanczos step inside one persistent
+// launch, one workgroup per instance; larger ones are tiled over the per-phase
+// kernels.  Measured on 512 1D rings (L 8..12, 150 steps): 0.1421 s per batch
+// with every instance tiled, 0.1427 s with the 200 instances up to 2048 in the
+// persistent launch, 0.1442 / 0.1452 / 0.1621 s at 4096 / 8192 / 16384 (one
+// workgroup per instance is latency-bound beyond a few thousand elements).
+constexpr int64_t kPersistMaxDim = 2048;
 
 // Longest row (D_dn) handled by row tiles.
 constexpr int64_t kMaxRowsTileDn = 2048;
@@ -64,11 +73,20 @@ class Batch {
                hipStream_t s) const;
   void vec_step(double *V, double *VP, double *W, int k, int step, int mode, hipStream_t s) const;
   void reduce(int k, int step, int what, hipStream_t s) const;
+  // every step of every small instance, one workgroup each (after the reset)
+  void persistent_run(int k, hipStream_t s) const;
+  // instances up to max_dim go to the persistent kernel (0: none); re-tiles
+  void set_persist_max_dim(int64_t max_dim);
 
   std::vector<const Hubbard *> models_;
   std::vector<BInst> host_inst_;
   std::vector<BTile> tiles_;
   DevBuf d_inst_, d_tiles_, V_, VP_, W_, B_, Q_, partial_, alpha_, beta_, done_, steps_, bnorm_;
+  DevBuf d_small_;      // indices of the persistent (small) instances
+  int n_small_ = 0;
+  hipStream_t side_ = nullptr;  // the persistent kernel's branch of a run
+  hipEvent_t fork_ = nullptr, join_ = nullptr;
+  int64_t persist_max_dim_ = kPersistMaxDim;
   int64_t total_ = 0;
   size_t lds_bytes_ = 0;
   int tile_ = 1024, block_ = 256;

@@ -714,6 +714,14 @@ int mb_batch_set_mapping(mb_batch *b, int tile, int block) {
   });
 }
 
+int mb_batch_set_persistent(mb_batch *b, int64_t max_dim) {
+  return guarded([&] {
+    need(b != nullptr, "null argument");
+    need(max_dim >= 0, "max_dim must be non-negative");
+    b->b->set_persist_max_dim(max_dim);
+  });
+}
+
 int mb_batch_autotune(mb_batch *b, int k_probe, int *tile, int *block, mb_stream stream) {
   return guarded([&] {
     need(b && k_probe > 0, "invalid argument");

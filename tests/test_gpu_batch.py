@@ -112,3 +112,29 @@ def test_batch_c4_sample_energies(cuda):
         assert abs(e0[i] - ref) < 1e-8, (inst[i], e0[i], ref)
         checked += 1
     assert checked >= 20
+
+
+@pytest.mark.parametrize("max_dim", [0, 4096, 1 << 40])
+def test_persistent_and_tiled_instances_agree(cuda, max_dim):
+    """Every instance tiled (0), the small ones persistent (4096) or every one
+    persistent: the same alpha/beta over the first 25 steps (1e-11) and the
+    same lowest Ritz values (1e-10) as the one-at-a-time path; graph replays
+    bit-identical."""
+    ops = [ring(*c) for c in MIX]
+    B = mb.Batch(ops)
+    B.set_persistent(max_dim)
+    seeds = np.arange(300, 300 + len(ops), dtype=np.uint64)
+    B.fill_gaussian(seeds)
+    k = 60
+    decs = B.lanczos(k, use_graph=True)
+    again = B.lanczos(k, use_graph=True)
+    for i, op in enumerate(ops):
+        assert decs[i].alphas == again[i].alphas and decs[i].betas == again[i].betas
+        single = mb.lanczos_pass_one(op, op.gaussian(int(seeds[i])), k)
+        assert decs[i].steps_taken == single.steps_taken, MIX[i]
+        a, b = np.array(decs[i].alphas), np.array(single.alphas)
+        assert np.abs(a[:25] - b[:25]).max() <= 1e-11 * max(1.0, np.abs(b).max()), MIX[i]
+        e_b = mb.tridiag_eig(decs[i].alphas, decs[i].betas, vectors=False)[0][0]
+        e_s = mb.tridiag_eig(single.alphas, single.betas, vectors=False)[0][0]
+        assert abs(e_b - e_s) <= 1e-10 * max(1.0, abs(e_s)), MIX[i]
+    assert decs[4].steps_taken == 1 and decs[4].betas == []
