@@ -207,8 +207,9 @@ MomentumHubbard::MomentumHubbard(const MomentumSector &m, double t, double U) : 
       }
     }
   };
-  const int mu = std::max(1, std::min(m.n_up, N - m.n_up));
-  const int md = std::max(1, std::min(m.n_dn, N - m.n_dn));
+  auto stride = [](int width) { return width <= 8 ? 8 : width; };
+  const int mu = stride(std::max(1, std::min(m.n_up, N - m.n_up)));
+  const int md = stride(std::max(1, std::min(m.n_dn, N - m.n_dn)));
   std::vector<uint16_t> up_src, dn_src;
   std::vector<uint8_t> up_cnt, dn_cnt;
   sources(m.up_str, m.up_off, mu, up_src, up_cnt, true);
@@ -233,6 +234,7 @@ MomentumHubbard::MomentumHubbard(const MomentumSector &m, double t, double U) : 
   d_dn_cnt_ = upload(dn_cnt);
   d_up_grp_ = upload(grp);
   d_up_loc_ = upload(loc);
+  d_up_off_ = upload(m.up_off);
   d_dn_off_ = upload(m.dn_off);
   d_ncols_ = upload(m.ncols);
   d_msub_ = upload(msub);
@@ -254,11 +256,14 @@ MomentumHubbard::MomentumHubbard(const MomentumSector &m, double t, double U) : 
   tb_.dn_cnt = d_dn_cnt_.as<uint8_t>();
   tb_.up_grp = d_up_grp_.as<int32_t>();
   tb_.up_loc = d_up_loc_.as<int32_t>();
+  tb_.up_off = d_up_off_.as<int32_t>();
   tb_.dn_off = d_dn_off_.as<int32_t>();
   tb_.ncols = d_ncols_.as<int32_t>();
   tb_.msub = d_msub_.as<int32_t>();
   tb_.blk_off = d_blk_off_.as<int64_t>();
   tb_.max_cols = *std::max_element(m.ncols.begin(), m.ncols.end());
+  tb_.max_rows = 0;
+  for (int p = 0; p < N; ++p) tb_.max_rows = std::max(tb_.max_rows, m.up_off[p + 1] - m.up_off[p]);
 }
 
 int64_t MomentumHubbard::hv(const double *x, double *y, Epi e, const EpiArgs &a,

@@ -47,25 +47,33 @@ void build_momentum_sector(MomentumSector &m, int ndim, const int *sizes, int n_
 // (hubbard_momentum_factorized_fixed_k.h:92-117): per output up string and
 // q != 0 its source strings under rho_{q,up}, per output down string and q its
 // source strings under rho_{-q,dn}, each as a 15-bit index inside the source
-// momentum group with the fermionic sign in bit 15; q = 0 is N_up N_dn.
+// momentum group with the fermionic sign in bit 15; q = 0 is N_up N_dn.  Lists
+// of at most 8 entries are padded to 8 (one 16-byte load per string and q).
 struct MomTables {
-  int N, K, mu, md;  // sites, total momentum, widths of the source lists
+  int N, K, mu, md;  // sites, total momentum, strides of the source lists
   int64_t D, nrows;  // dimension, up strings (output rows)
   double t, U, q0;                          // q0 = (U/N) N_up N_dn, the q = 0 term
   const double *eps_up, *eps_dn;            // sum of eps(k) over each string
   const uint16_t *up_src, *dn_src;          // [string][q][mu or md]
   const uint8_t *up_cnt, *dn_cnt;           // [string][q]
   const int32_t *up_grp, *up_loc;           // momentum group and index inside it
+  const int32_t *up_off;                    // N + 1 up group offsets (block rows)
   const int32_t *dn_off, *ncols, *msub;     // group offsets, block columns, p - q table
   const int64_t *blk_off;
-  int max_cols;
+  int max_cols, max_rows;                   // largest down / up momentum group
 };
+
+constexpr int kMomLockMaxW = 64;  // workgroups per XCD of the persistent momentum kernel
 
 class MomentumHubbard final : public HvOp {
  public:
   MomentumHubbard(const MomentumSector &m, double t, double U);
   int64_t dim() const override { return m_.D; }
-  int64_t partials() const override { return 2 * m_.up_str.size() + 16; }
+  // per plane one slot per output row (per-row kernel) or per (block, workgroup)
+  // of the persistent kernel, at most kMomLockMaxW workgroups per XCD
+  int64_t partials() const override {
+    return 2 * ((int64_t)m_.up_str.size() + (int64_t)m_.L * kMomLockMaxW) + 16;
+  }
   int64_t hv(const double *x, double *y, Epi e, const EpiArgs &a, hipStream_t s) const override;
   int64_t hv(const float *x, float *y, Epi e, const EpiArgs &a, hipStream_t s) const override;
   const MomentumSector &sector() const { return m_; }
@@ -75,7 +83,7 @@ class MomentumHubbard final : public HvOp {
   const MomentumSector &m_;
   MomTables tb_{};
   DevBuf d_eps_up_, d_eps_dn_, d_up_src_, d_dn_src_, d_up_cnt_, d_dn_cnt_, d_up_grp_, d_up_loc_,
-      d_dn_off_, d_ncols_, d_msub_, d_blk_off_;
+      d_up_off_, d_dn_off_, d_ncols_, d_msub_, d_blk_off_;
 };
 
 // kernels (kernels.hip)

@@ -2,6 +2,7 @@
 operator (hubbard_momentum_factorized_fixed_k.h) against the oracle, and
 against the real-space model through the spectrum.
 
+The H.v tests run each kernel (per-row and persistent, see KERNELS).
 Tolerances: H.v 1e-12 relative to the largest output entry (different summation
 order than the oracle's dense matrix); ground energies 1e-10 relative (Lanczos,
 fully converged on these sizes) and 1e-8 at 4 x 4, 7 + 7 (300 steps on both
@@ -39,8 +40,22 @@ CASES = [([6], 3, 2, 1.0, 4.0), ([5], 2, 2, 0.7, 3.0), ([3, 2], 2, 2, 1.0, 8.0),
          ([2, 2, 2], 2, 2, 1.0, 4.0), ([4, 2], 3, 2, 1.0, 6.0), ([4, 4], 2, 1, 1.0, 8.0)]
 
 
+# H.v kernels: the per-row kernel, and the persistent one (taken by default from
+# 2^20 states on) forced on small sectors with 1 to 4 parts of the source columns
+KERNELS = [("rows", None), ("lock", 1), ("lock", 2), ("lock", 3), ("lock", 4)]
+
+
+@pytest.fixture(params=KERNELS, ids=lambda k: k[0] + ("" if k[1] is None else str(k[1])))
+def mom_kernel(request, monkeypatch):
+    kind, parts = request.param
+    monkeypatch.setenv("MANYBODY_AMD_MOM_KERNEL", kind)
+    if parts is not None:
+        monkeypatch.setenv("MANYBODY_AMD_MOM_PARTS", str(parts))
+    return request.param
+
+
 @pytest.mark.parametrize("sizes,nu,nd,t,U", CASES)
-def test_momentum_hv_matches_oracle(cuda, sizes, nu, nd, t, U):
+def test_momentum_hv_matches_oracle(cuda, mom_kernel, sizes, nu, nd, t, U):
     N = int(np.prod(sizes))
     rng = np.random.default_rng(N * 10 + nu)
     for K in range(N):
@@ -65,7 +80,7 @@ def test_momentum_hv_matches_oracle(cuda, sizes, nu, nd, t, U):
 
 @pytest.mark.parametrize("sizes,nu,nd,U", [([8], 4, 4, 4.0), ([3, 3], 3, 2, 6.0),
                                            ([2, 2, 2], 3, 2, 4.0)])
-def test_momentum_ground_state_is_the_real_space_one(cuda, sizes, nu, nd, U):
+def test_momentum_ground_state_is_the_real_space_one(cuda, mom_kernel, sizes, nu, nd, U):
     """min over K of the sector ground energies = the real-space ground energy."""
     N = int(np.prod(sizes))
     basis = mb.FermionBasis.sector(N, nu, nd)
@@ -137,3 +152,30 @@ def test_momentum_4x4_7p7_ground_state(cuda):
         d = mb.lanczos_pass_one(op, op.gaussian(seed=kx + 4 * ky), 300)
         e_k.append(mb.tridiag_eig(d.alphas, d.betas, vectors=False)[0][0])
     assert abs(min(e_k) - e_real) <= 1e-8 * abs(e_real)
+
+
+def test_persistent_kernel_equals_per_row_kernel_at_4x4_7p7(cuda, monkeypatch):
+    """4 x 4, 7 + 7 at U = 8, sectors K = (0,0) and (2,1) (D_K about 8.2 M): the
+    persistent kernel, the default at this size, against the per-row kernel.
+    H.v to 1e-14 of the largest entry (the two sum the same terms in different
+    orders); 20 Lanczos coefficients to 1e-12 relative."""
+    for K in (0, 2 + 4 * 1):
+        op = make_mop([4, 4], 7, 7, K, 1.0, 8.0)
+        x = op.gaussian(seed=77 + K)
+        out = {}
+        for kind, parts in (("rows", None), ("lock", 1), ("lock", 2), ("lock", 3)):
+            monkeypatch.setenv("MANYBODY_AMD_MOM_KERNEL", kind)
+            if parts is not None:
+                monkeypatch.setenv("MANYBODY_AMD_MOM_PARTS", str(parts))
+            y = op.hv(x)
+            dec = mb.lanczos_pass_one(op, x, 20)
+            out[(kind, parts)] = (y, np.array(dec.alphas), np.array(dec.betas))
+            print(f"K={K} kernel {kind}{parts or ''}: |Hx| = {y.abs().max().item():.6f}", flush=True)
+        y0, a0, b0 = out[("rows", None)]
+        scale = y0.abs().max().item()
+        for key, (y, a, b) in out.items():
+            assert (y - y0).abs().max().item() <= 1e-14 * scale, (K, key)
+            assert np.abs(a - a0).max() <= 1e-12 * np.abs(a0).max(), (K, key)
+            assert np.abs(b - b0).max() <= 1e-12 * np.abs(b0).max(), (K, key)
+        del op, x, out
+        torch.cuda.empty_cache()
