@@ -1309,31 +1309,38 @@ __global__ __launch_bounds__(kBlock) void mom_hv_kernel(MomTables t, const doubl
 // planes do not mix), in a persistent launch of one workgroup per CU.  Work
 // items are (block p, plane); XCD x takes items x, x + 8, ... (workgroups go
 // round-robin over the 8 XCDs), and the W workgroups of an XCD split the rows of
-// the block, rows w, w + W, ...  Each workgroup keeps its rows' sums for all
-// columns in registers (thread c owns column c) while the item runs through
-// q = 1 .. N-1 and, inside q, NPART parts of the source columns.  All workgroups
-// of an XCD are in the same (q, part) phase at about the same time, so the
-// phase reads one part of one source block plane (about 4 MB / NPART at 4x4
-// 7+7) while each of its rows is combined into about four output rows.  A
-// phase first combines, for each of the workgroup's rows, the signed up-source
-// rows restricted to the part into LDS (one wave per row, CB 64-column chunks
-// with all their loads issued before the sums), then each column adds its down
-// sources that fall in the part, for every row at once.  The rows' up-source
-// lists for every q are staged in LDS when the item starts.
-constexpr int kLockBlock = 768;  // threads per workgroup: at most this many columns
+// the block, rows w, w + W, ... (at most kLockRows each).  All workgroups of an
+// XCD run through q = 1 .. N-1 at about the same pace, so at any time the XCD
+// reads about one source block plane (4 MB at 4x4 7+7), each of whose rows is
+// combined into about four output rows.
+//
+// The workgroup is two groups of 6 waves; group h owns rows kLockHalf h ..
+// kLockHalf h + kLockHalf - 1 and, per thread, kLockCols columns of them, whose
+// sums stay in registers.  For each q a group first stages its rows: each row's
+// signed up-source rows combined into an LDS row, comb[j][d] = sum_u s_u x[u, d]
+// (one wave per row, loads through a buffer resource, a batch of 64-column
+// chunks in flight); then it gathers: each owned column adds its down sources,
+// sum_d s_d comb[j][d], for all of the group's rows at once.  The groups run one
+// step apart, so one group's loads overlap the other's LDS gathers, with one
+// barrier per step.  The rows' up-source lists for every q are staged in LDS
+// when the item starts.
+constexpr int kLockBlock = 768;  // threads per workgroup
+constexpr int kLockGroup = 384;  // threads per group (6 waves)
+constexpr int kLockCols = 2;     // columns per thread: at most 768 columns
 constexpr int kLockRows = 24;    // rows per workgroup and item
+constexpr int kLockHalf = 12;    // rows per group
+constexpr int kLockS = 768;      // LDS doubles per comb row
 constexpr int kLockSrc = 8;      // source list entries per (string, q)
 constexpr int kLockQ = 16;       // momenta per item whose up-source lists are staged
 
-template <int NPART>
-constexpr int lock_stride() {  // LDS doubles per row of one part, a multiple of 64
-  return ((kLockBlock + NPART - 1) / NPART + 63) / 64 * 64;
+constexpr size_t lock_lds_bytes() {  // the groups' comb rows, then the staged lists and counts
+  return (size_t)kLockRows * kLockS * sizeof(double) +
+         (size_t)kLockRows * kLockQ * (kLockSrc * sizeof(uint16_t) + sizeof(uint32_t));
 }
 
 // Word 3 of a raw buffer resource (32-bit data format, no swizzle): loads
 // through it take a per-lane byte offset and a uniform one, no 64-bit address.
 constexpr int kBufWord3 = 0x00020000;
-typedef unsigned int lock_u32x2 __attribute__((ext_vector_type(2)));
 
 // Buffer resource over `bytes` bytes from p, built from wave-uniform words so
 // that it lives in scalar registers.
@@ -1346,53 +1353,148 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t lock_rsrc(const double *p, int
                                            kBufWord3);
 }
 
-// One row of comb for the columns d < nh of a part: the NS signed source rows at
-// byte offsets so[i] of the buffer, summed in list order (exact +-1 products),
-// 32 / NS chunks of 64 columns per batch with all of a batch's loads in flight.
-// Lanes past nh load the last column (in bounds) and store nothing.
+// Loads of chunks c0 .. c0 + CB - 1 (64 columns each) of the NS source rows at
+// byte offsets so[i]; chunks past the row and lanes past nc load the last column
+// (in bounds, unused), so a batch is always CB * NS loads.
+template <int NS, int CB>
+__device__ __forceinline__ void lock_load(__amdgpu_buffer_rsrc_t rs, const int (&so)[kLockSrc], int nc,
+                                          int c0, int lane, double (&v)[CB][NS]) {
+#pragma unroll
+  for (int b = 0; b < CB; ++b) {
+    const int off = min((c0 + b) * kWave + lane, nc - 1) * 8;
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+      v[b][i] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, off, so[i], 0));
+  }
+}
+
+// Signed sums of loaded chunks in list order (exact +-1 products) into crow.
+template <int NS, int CB>
+__device__ __forceinline__ void lock_sum(const double (&sg)[kLockSrc], int nc, int c0, int lane,
+                                         const double (&v)[CB][NS], double *__restrict__ crow) {
+#pragma unroll
+  for (int b = 0; b < CB; ++b) {
+    const int d = (c0 + b) * kWave + lane;
+    if (d < nc) {
+      double sum = sg[0] * v[b][0];
+#pragma unroll
+      for (int i = 1; i < NS; ++i) sum = fma(sg[i], v[b][i], sum);
+      crow[d] = sum;
+    }
+  }
+}
+
+// Chunks per batch for NS sources: at most 16 loads, and 12 chunks (a 715 to
+// 768 column row) in an even number of batches where possible.
+template <int NS>
+constexpr int lock_cb() {
+  return NS <= 2 ? 6 : (NS <= 5 ? 3 : 2);
+}
+
+// One row of comb: the NS signed source rows summed over the nc columns, in
+// batches of CB 64-column chunks, the next batch's loads issued before the
+// current batch is summed (two batches in flight).
 template <int NS>
 __device__ __forceinline__ void lock_stage_row(__amdgpu_buffer_rsrc_t rs, const int (&so)[kLockSrc],
-                                               const double (&sg)[kLockSrc], int nh, int lane,
+                                               const double (&sg)[kLockSrc], int nc, int lane,
                                                double *__restrict__ crow) {
-  constexpr int CB = 32 / NS < 12 ? 32 / NS : 12;
-  const int nch = (nh + kWave - 1) / kWave;
-  for (int c0 = 0; c0 < nch; c0 += CB) {
-    double v[CB][NS];
+  constexpr int CB = lock_cb<NS>();
+  const int ncol = (nc + kWave - 1) / kWave * kWave;  // columns of the whole chunks
+  double va[CB][NS], vb[CB][NS];
+  lock_load<NS, CB>(rs, so, nc, 0, lane, va);
+  int c0 = 0;
+  for (; (c0 + CB) * kWave < ncol; c0 += 2 * CB) {
+    lock_load<NS, CB>(rs, so, nc, c0 + CB, lane, vb);
+    lock_sum<NS, CB>(sg, nc, c0, lane, va, crow);
+    lock_load<NS, CB>(rs, so, nc, c0 + 2 * CB, lane, va);
+    lock_sum<NS, CB>(sg, nc, c0 + CB, lane, vb, crow);
+  }
+  if (c0 * kWave < ncol) lock_sum<NS, CB>(sg, nc, c0, lane, va, crow);
+}
+
+// Staging of one group: rows k = k0 + j (j < kLockHalf, k < nr) of the
+// workgroup, momentum q, into comb rows j; gw is the wave within the group.
+__device__ __forceinline__ void lock_stage_half(const double *__restrict__ xs, int ncs, int nrs, int q,
+                                                int k0, int nr, int gw, int lane, const uint4 *ent,
+                                                const uint32_t *ecnt, double *__restrict__ comb) {
+  if (ncs <= 0) return;
+  const __amdgpu_buffer_rsrc_t rs = lock_rsrc(xs, (int)((int64_t)nrs * ncs * 8));
+  const int k1 = min(k0 + kLockHalf, nr);
+  for (int k = k0 + gw; k < k1; k += kLockGroup / kWave) {
+    const uint4 ev = ent[k * kLockQ + q];
+    const int cu = __builtin_amdgcn_readfirstlane(ecnt[k * kLockQ + q]);
+    const uint32_t ew[4] = {(uint32_t)__builtin_amdgcn_readfirstlane(ev.x),
+                            (uint32_t)__builtin_amdgcn_readfirstlane(ev.y),
+                            (uint32_t)__builtin_amdgcn_readfirstlane(ev.z),
+                            (uint32_t)__builtin_amdgcn_readfirstlane(ev.w)};
+    int so[kLockSrc];
+    double sg[kLockSrc];
 #pragma unroll
-    for (int b = 0; b < CB; ++b) {
-      if (c0 + b < nch) {
-        const int d = min((c0 + b) * kWave + lane, nh - 1);
-#pragma unroll
-        for (int i = 0; i < NS; ++i)
-          v[b][i] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, d * 8, so[i], 0));
-      }
+    for (int i = 0; i < kLockSrc; ++i) {
+      const uint32_t e = (ew[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu;
+      so[i] = (int)(e & 0x7FFFu) * ncs * 8;
+      sg[i] = (e & 0x8000u) ? -1.0 : 1.0;
     }
+    double *__restrict__ crow = comb + (k - k0) * kLockS;
+    switch (cu) {
+      case 1: lock_stage_row<1>(rs, so, sg, ncs, lane, crow); break;
+      case 2: lock_stage_row<2>(rs, so, sg, ncs, lane, crow); break;
+      case 3: lock_stage_row<3>(rs, so, sg, ncs, lane, crow); break;
+      case 4: lock_stage_row<4>(rs, so, sg, ncs, lane, crow); break;
+      case 5: lock_stage_row<5>(rs, so, sg, ncs, lane, crow); break;
+      case 6: lock_stage_row<6>(rs, so, sg, ncs, lane, crow); break;
+      case 7: lock_stage_row<7>(rs, so, sg, ncs, lane, crow); break;
+      case 8: lock_stage_row<8>(rs, so, sg, ncs, lane, crow); break;
+      default:  // no sources under this q
+        for (int d = lane; d < ncs; d += kWave) crow[d] = 0.0;
+    }
+  }
+}
+
+// Gathering of one group: for each owned column c < nc, its down sources under
+// q (8 packed entries, cd of them valid) read from the group's comb rows.
+__device__ __forceinline__ void lock_gather_half(const MomTables &t, int q, int dg0, int nc, int gt,
+                                                 const double *__restrict__ comb,
+                                                 double (&acc)[kLockCols][kLockHalf]) {
+  const int N = t.N;
+  uint32_t de[kLockCols][4];
+  int cd[kLockCols];
 #pragma unroll
-    for (int b = 0; b < CB; ++b) {
-      const int d = (c0 + b) * kWave + lane;
-      if (c0 + b < nch && d < nh) {
-        double sum = sg[0] * v[b][0];
+  for (int j = 0; j < kLockCols; ++j) {
+    const int c = gt + j * kLockGroup;
+    de[j][0] = de[j][1] = de[j][2] = de[j][3] = 0u;
+    cd[j] = 0;
+    if (c < nc) {
+      const int64_t hq = (int64_t)(dg0 + c) * N + q;
+      cd[j] = t.dn_cnt[hq];
+      const uint4 v = *reinterpret_cast<const uint4 *>(t.dn_src + hq * kLockSrc);
+      de[j][0] = v.x;
+      de[j][1] = v.y;
+      de[j][2] = v.z;
+      de[j][3] = v.w;
+    }
+  }
 #pragma unroll
-        for (int i = 1; i < NS; ++i) sum = fma(sg[i], v[b][i], sum);
-        crow[d] = sum;
+  for (int j = 0; j < kLockCols; ++j) {
+#pragma unroll
+    for (int l = 0; l < kLockSrc; ++l) {
+      const uint32_t e = (de[j][l >> 1] >> ((l & 1) * 16)) & 0xFFFFu;
+      if (l < cd[j]) {
+        const double sg = (e & 0x8000u) ? -1.0 : 1.0;
+        const double *__restrict__ cb = comb + (e & 0x7FFFu);
+#pragma unroll
+        for (int r = 0; r < kLockHalf; ++r) acc[j][r] = fma(sg, cb[r * kLockS], acc[j][r]);
       }
     }
   }
 }
 
-template <int NPART>
-constexpr size_t lock_lds_bytes() {  // comb, then the staged up-source lists and counts
-  return (size_t)kLockRows * lock_stride<NPART>() * sizeof(double) +
-         (size_t)kLockRows * kLockQ * (kLockSrc * sizeof(uint16_t) + sizeof(uint32_t));
-}
-
-template <Epi E, int NPART>
+template <Epi E>
 __global__ __launch_bounds__(kLockBlock) void mom_hv_lock_kernel(MomTables t, const double *__restrict__ x,
                                                                  double *__restrict__ y, EpiArgs a) {
-  constexpr int S = lock_stride<NPART>();
   constexpr int kWaves = kLockBlock / kWave;
-  extern __shared__ double comb[];  // [kLockRows][S]: combined source rows of one part
-  uint4 *ent = reinterpret_cast<uint4 *>(comb + kLockRows * S);  // [row][q]: 8 packed entries
+  extern __shared__ double lds[];  // comb: [kLockRows][kLockS], group h's rows from h * kLockHalf
+  uint4 *ent = reinterpret_cast<uint4 *>(lds + kLockRows * kLockS);        // [row][q]
   uint32_t *ecnt = reinterpret_cast<uint32_t *>(ent + kLockRows * kLockQ);  // [row][q]
   __shared__ double red[kWaves];
   if constexpr (E == Epi::Lanczos) {
@@ -1400,6 +1502,11 @@ __global__ __launch_bounds__(kLockBlock) void mom_hv_lock_kernel(MomTables t, co
   }
   const int N = t.N, tid = threadIdx.x, lane = tid & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+  const int grp = wave / (kLockGroup / kWave);        // 0 or 1, wave-uniform
+  const int gw = wave - grp * (kLockGroup / kWave);  // wave within the group
+  const int gt = tid - grp * kLockGroup;             // thread within the group
+  const int k0 = grp * kLockHalf;                    // the group's first row
+  double *__restrict__ comb = lds + k0 * kLockS;
   const int xcd = blockIdx.x & 7, w = blockIdx.x >> 3, W = gridDim.x >> 3;
   const int64_t D = t.D;
   const double uo = t.U / N;
@@ -1411,6 +1518,7 @@ __global__ __launch_bounds__(kLockBlock) void mom_hv_lock_kernel(MomTables t, co
     const int nr = nblk > w ? (nblk - w + W - 1) / W : 0;  // rows w, w + W, ... of block p
     const int nc = t.ncols[p];
     const int dg0 = t.dn_off[t.msub[t.K * N + p]];  // first down string of the columns
+    const double *__restrict__ xp = x + (int64_t)pl * D;
     __syncthreads();  // the previous item no longer reads the staged lists
     for (int i = tid; i < nr * N; i += kLockBlock) {
       const int k = i / N, q = i - k * N;
@@ -1418,98 +1526,56 @@ __global__ __launch_bounds__(kLockBlock) void mom_hv_lock_kernel(MomTables t, co
       ent[k * kLockQ + q] = *reinterpret_cast<const uint4 *>(t.up_src + gq * kLockSrc);
       ecnt[k * kLockQ + q] = t.up_cnt[gq];
     }
-    double acc[kLockRows];
+    __syncthreads();
+    double acc[kLockCols][kLockHalf];
 #pragma unroll
-    for (int k = 0; k < kLockRows; ++k) acc[k] = 0.0;
-    for (int q = 1; q < N && nr > 0; ++q) {
-      const int ps = t.msub[p * N + q];
-      const int ncs = t.ncols[ps];
-      const int nrs = t.up_off[ps + 1] - t.up_off[ps];
-      const double *__restrict__ xs = x + (int64_t)pl * D + t.blk_off[ps];
-      // this column's down sources under rho_{-q,dn}: 8 packed 16-bit entries
-      uint32_t de[4] = {0u, 0u, 0u, 0u};
-      int cd = 0;
-      if (tid < nc) {
-        const int64_t hq = (int64_t)(dg0 + tid) * N + q;
-        cd = t.dn_cnt[hq];
-        const uint4 v = *reinterpret_cast<const uint4 *>(t.dn_src + hq * kLockSrc);
-        de[0] = v.x;
-        de[1] = v.y;
-        de[2] = v.z;
-        de[3] = v.w;
-      }
+    for (int j = 0; j < kLockCols; ++j)
 #pragma unroll
-      for (int h = 0; h < NPART; ++h) {
-        const int d0 = (int)((int64_t)ncs * h / NPART);
-        const int nh = (int)((int64_t)ncs * (h + 1) / NPART) - d0;
-        __syncthreads();  // the previous phase no longer reads comb; the lists are staged
-        if (nh > 0) {
-          // the part of the source block plane, as a buffer (byte offsets in 32 bits)
-          const __amdgpu_buffer_rsrc_t rs = lock_rsrc(xs + d0, (int)(((int64_t)nrs * ncs - d0) * 8));
-          for (int k = wave; k < nr; k += kWaves) {
-            const uint4 ev = ent[k * kLockQ + q];
-            const int cu = __builtin_amdgcn_readfirstlane(ecnt[k * kLockQ + q]);
-            const uint32_t ew[4] = {(uint32_t)__builtin_amdgcn_readfirstlane(ev.x),
-                                    (uint32_t)__builtin_amdgcn_readfirstlane(ev.y),
-                                    (uint32_t)__builtin_amdgcn_readfirstlane(ev.z),
-                                    (uint32_t)__builtin_amdgcn_readfirstlane(ev.w)};
-            int so[kLockSrc];
-            double sg[kLockSrc];
-#pragma unroll
-            for (int i = 0; i < kLockSrc; ++i) {
-              const uint32_t e = (ew[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu;
-              so[i] = (int)(e & 0x7FFFu) * ncs * 8;
-              sg[i] = (e & 0x8000u) ? -1.0 : 1.0;
+      for (int k = 0; k < kLockHalf; ++k) acc[j][k] = 0.0;
+    // Each group alternates staging and gathering, q = 1 .. N-1: its own step
+    // u = 2 (q - 1) stages q, u + 1 gathers it.  Group 1 runs one step behind
+    // group 0, so in every workgroup step one group stages while the other gathers.
+    if (nr > 0) {  // uniform
+      for (int s = 0; s <= 2 * N - 2; ++s) {
+        const int u = s - grp;
+        if (u >= 0 && k0 < nr) {  // uniform over the group
+          const int q = u / 2 + 1;
+          if ((u & 1) == 0) {
+            if (q < N) {
+              const int ps = t.msub[p * N + q];
+              lock_stage_half(xp + t.blk_off[ps], t.ncols[ps], t.up_off[ps + 1] - t.up_off[ps], q, k0,
+                              nr, gw, lane, ent, ecnt, comb);
             }
-            double *__restrict__ crow = comb + k * S;
-            switch (cu) {
-              case 1: lock_stage_row<1>(rs, so, sg, nh, lane, crow); break;
-              case 2: lock_stage_row<2>(rs, so, sg, nh, lane, crow); break;
-              case 3: lock_stage_row<3>(rs, so, sg, nh, lane, crow); break;
-              case 4: lock_stage_row<4>(rs, so, sg, nh, lane, crow); break;
-              case 5: lock_stage_row<5>(rs, so, sg, nh, lane, crow); break;
-              case 6: lock_stage_row<6>(rs, so, sg, nh, lane, crow); break;
-              case 7: lock_stage_row<7>(rs, so, sg, nh, lane, crow); break;
-              case 8: lock_stage_row<8>(rs, so, sg, nh, lane, crow); break;
-              default:  // no sources under this q
-                for (int d = lane; d < nh; d += kWave) crow[d] = 0.0;
-            }
+          } else {
+            lock_gather_half(t, q, dg0, nc, gt, comb, acc);
           }
         }
         __syncthreads();
-        if (tid < nc) {
-#pragma unroll
-          for (int l = 0; l < kLockSrc; ++l) {
-            const uint32_t e = (de[l >> 1] >> ((l & 1) * 16)) & 0xFFFFu;
-            const int ld = (int)(e & 0x7FFFu) - d0;
-            if (l < cd && ld >= 0 && ld < nh) {
-              const double sg = (e & 0x8000u) ? -1.0 : 1.0;
-              const double *__restrict__ cb = comb + ld;
-#pragma unroll
-              for (int k = 0; k < kLockRows; ++k) acc[k] = fma(sg, cb[k * S], acc[k]);
-            }
-          }
-        }
       }
     }
     Partial2 ap;
-    if (tid < nc) {
-      const double ed = t.eps_dn[dg0 + tid];
-      const int64_t base = (int64_t)pl * D + t.blk_off[p] + tid;
+    const int64_t base = (int64_t)pl * D + t.blk_off[p];
 #pragma unroll
-      for (int k = 0; k < kLockRows; ++k) {
-        if (k < nr) {
-          const int r = w + k * W;
-          const int64_t i = base + (int64_t)r * nc;
-          const double xv = x[i];
-          const double dg = t.t * (t.eps_up[r0 + r] + ed) + t.q0;
-          epilogue<double, E>(dg * xv + uo * acc[k], xv, i, y, a, bp, ap);
+    for (int j = 0; j < kLockCols; ++j) {
+      const int c = gt + j * kLockGroup;
+      if (c < nc) {
+        const double ed = t.eps_dn[dg0 + c];
+#pragma unroll
+        for (int r = 0; r < kLockHalf; ++r) {
+          const int k = k0 + r;
+          if (k < nr) {
+            const int row = w + k * W;
+            const int64_t i = base + (int64_t)row * nc + c;
+            const double xv = x[i];
+            const double dg = t.t * (t.eps_up[r0 + row] + ed) + t.q0;
+            epilogue<double, E>(dg * xv + uo * acc[j][r], xv, i, y, a, bp, ap);
+          }
         }
       }
     }
     if constexpr (E == Epi::Lanczos) {
-      const double s = wg_sum(ap.s0, red);
-      if (tid == 0) a.partial[item * W + w] = s;
+      const double sum = wg_sum(ap.s0, red);
+      if (tid == 0) a.partial[item * W + w] = sum;
     }
   }
 }
@@ -2069,9 +2135,10 @@ void launch_mom_cpt(const MomTables &t, const double *x, double *y, const EpiArg
 }
 
 // Workgroups per XCD of the persistent momentum kernel (one per CU), or 0 when
-// the sector does not fit it: columns beyond one per thread, rows beyond
-// kLockRows per workgroup, source lists wider than kLockSrc, or more than
-// kMomLockMaxW workgroups per XCD (the partial slots the operator provides).
+// the sector does not fit it: more than kLockCols columns per thread of a group,
+// rows beyond kLockRows per workgroup, more than kLockQ momenta, source lists
+// wider than kLockSrc, or more than kMomLockMaxW workgroups per XCD (the
+// partial slots the operator provides).
 // Sectors below 2^20 states keep the per-row kernel unless
 // MANYBODY_AMD_MOM_KERNEL=lock; MANYBODY_AMD_MOM_KERNEL=rows always keeps it.
 int mom_lock_wgs(const MomTables &t) {
@@ -2080,39 +2147,29 @@ int mom_lock_wgs(const MomTables &t) {
   const bool force = e && std::strcmp(e, "lock") == 0;
   const int W = device_cus() / 8;
   if (W < 1 || W > kMomLockMaxW || t.N > kLockQ || (t.D < (1 << 20) && !force)) return 0;
-  if (t.max_cols > kLockBlock || t.mu != kLockSrc || t.md != kLockSrc) return 0;
+  if (t.max_cols > kLockCols * kLockGroup || t.max_cols > kLockS) return 0;
+  if (t.mu != kLockSrc || t.md != kLockSrc) return 0;
   if ((t.max_rows + W - 1) / W > kLockRows) return 0;
   return W;
 }
 
-int mom_lock_parts() {
-  const char *e = std::getenv("MANYBODY_AMD_MOM_PARTS");
-  const int n = e ? std::atoi(e) : 2;
-  return (n == 1 || n == 2 || n == 3 || n == 4) ? n : 2;
-}
-
-template <Epi E, int NPART>
+template <Epi E>
 void launch_mom_lock(const MomTables &t, int W, const double *x, double *y, const EpiArgs &a,
                      hipStream_t s) {
-  const size_t lds = lock_lds_bytes<NPART>();
+  const size_t lds = lock_lds_bytes();
   static std::atomic<bool> attr_set{false};  // once per template instance (idempotent)
-  const void *k = reinterpret_cast<const void *>(&mom_hv_lock_kernel<E, NPART>);
+  const void *k = reinterpret_cast<const void *>(&mom_hv_lock_kernel<E>);
   if (!attr_set) {
     MB_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_set = true;
   }
-  hipLaunchKernelGGL((mom_hv_lock_kernel<E, NPART>), dim3(8 * W), dim3(kLockBlock), lds, s, t, x, y, a);
+  hipLaunchKernelGGL((mom_hv_lock_kernel<E>), dim3(8 * W), dim3(kLockBlock), lds, s, t, x, y, a);
 }
 
 template <Epi E>
 void launch_mom_e(const MomTables &t, const double *x, double *y, const EpiArgs &a, hipStream_t s) {
   if (const int W = mom_lock_wgs(t)) {
-    switch (mom_lock_parts()) {
-      case 1: launch_mom_lock<E, 1>(t, W, x, y, a, s); break;
-      case 3: launch_mom_lock<E, 3>(t, W, x, y, a, s); break;
-      case 4: launch_mom_lock<E, 4>(t, W, x, y, a, s); break;
-      default: launch_mom_lock<E, 2>(t, W, x, y, a, s); break;
-    }
+    launch_mom_lock<E>(t, W, x, y, a, s);
     return;
   }
   const int cpt = (t.max_cols + kBlock - 1) / kBlock;

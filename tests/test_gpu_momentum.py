@@ -41,16 +41,13 @@ CASES = [([6], 3, 2, 1.0, 4.0), ([5], 2, 2, 0.7, 3.0), ([3, 2], 2, 2, 1.0, 8.0),
 
 
 # H.v kernels: the per-row kernel, and the persistent one (taken by default from
-# 2^20 states on) forced on small sectors with 1 to 4 parts of the source columns
-KERNELS = [("rows", None), ("lock", 1), ("lock", 2), ("lock", 3), ("lock", 4)]
+# 2^20 states on) forced on small sectors
+KERNELS = ["rows", "lock"]
 
 
-@pytest.fixture(params=KERNELS, ids=lambda k: k[0] + ("" if k[1] is None else str(k[1])))
+@pytest.fixture(params=KERNELS)
 def mom_kernel(request, monkeypatch):
-    kind, parts = request.param
-    monkeypatch.setenv("MANYBODY_AMD_MOM_KERNEL", kind)
-    if parts is not None:
-        monkeypatch.setenv("MANYBODY_AMD_MOM_PARTS", str(parts))
+    monkeypatch.setenv("MANYBODY_AMD_MOM_KERNEL", request.param)
     return request.param
 
 
@@ -163,15 +160,13 @@ def test_persistent_kernel_equals_per_row_kernel_at_4x4_7p7(cuda, monkeypatch):
         op = make_mop([4, 4], 7, 7, K, 1.0, 8.0)
         x = op.gaussian(seed=77 + K)
         out = {}
-        for kind, parts in (("rows", None), ("lock", 1), ("lock", 2), ("lock", 3)):
+        for kind in ("rows", "lock"):
             monkeypatch.setenv("MANYBODY_AMD_MOM_KERNEL", kind)
-            if parts is not None:
-                monkeypatch.setenv("MANYBODY_AMD_MOM_PARTS", str(parts))
             y = op.hv(x)
             dec = mb.lanczos_pass_one(op, x, 20)
-            out[(kind, parts)] = (y, np.array(dec.alphas), np.array(dec.betas))
-            print(f"K={K} kernel {kind}{parts or ''}: |Hx| = {y.abs().max().item():.6f}", flush=True)
-        y0, a0, b0 = out[("rows", None)]
+            out[kind] = (y, np.array(dec.alphas), np.array(dec.betas))
+            print(f"K={K} kernel {kind}: |Hx| = {y.abs().max().item():.6f}", flush=True)
+        y0, a0, b0 = out["rows"]
         scale = y0.abs().max().item()
         for key, (y, a, b) in out.items():
             assert (y - y0).abs().max().item() <= 1e-14 * scale, (K, key)
