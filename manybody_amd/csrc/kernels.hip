@@ -500,6 +500,150 @@ __global__ __launch_bounds__(1024) void row_kernel_mp(HvTables t, const T *__res
   }
 }
 
+// H.v in one pass for sectors whose plane group (NP planes of D_up x D_dn) is
+// small enough to stay in the Infinity Cache: row_kernel_mp with the up hops
+// formed in the same kernel instead of by a separate strip pass.  The row's up
+// hop list is staged in LDS; each column then sums its up-source values
+// x[u', d] (read from L2 or the Infinity Cache, coalesced over the columns) in
+// hop order, exactly as up_strip_kernel does, and adds the down hops and the
+// diagonal as row_kernel_mp adds them to the strip result, so the two paths
+// give identical results.  HBM traffic per H.v: x once, y once (and prev for
+// the Lanczos, reconstruction and Chebyshev epilogues), against x twice and the
+// strip result written and read back for the two passes.
+constexpr int kFusedMaxHops = 64;  // up hops per row the fused kernel stages
+
+template <typename T, Epi E, int NP>
+__global__ __launch_bounds__(1024) void row_kernel_fused(HvTables t, const T *__restrict__ x,
+                                                         T *__restrict__ y, EpiArgs a) {
+  constexpr int K = kCompactK, G = 8, NS = (E == Epi::Cheb) ? 2 : 1;
+  extern __shared__ uint4 smem16[];  // 16-byte aligned for the pack reads
+  __shared__ double red[2 * NP][1024 / kWave];
+  __shared__ uint32_t s_off[kFusedMaxHops];  // byte offset of the source row in the plane
+  __shared__ T s_amp[kFusedMaxHops];
+  T *row = reinterpret_cast<T *>(smem16);  // (D_dn + 1) * NP, string-major; string D_dn is zero
+  if constexpr (E == Epi::Lanczos) {
+    if (a.flags[0]) return;
+  }
+  const int64_t Ddn = t.d_dn, Dup = t.d_up, D = Dup * Ddn;
+  const int64_t wg = blockIdx.x;
+  const int64_t u = wg % Dup, pg = wg / Dup;
+  const int64_t base0 = pg * NP * D + u * Ddn;  // plane pg * NP, row u
+  const int hb = t.up_ptr[u], nh = t.up_ptr[u + 1] - hb;
+  const T *__restrict__ amp = Tabs<T>::up_amp(t);
+  const T mag = (T)t.up_mag;
+  for (int k = threadIdx.x; k < nh; k += blockDim.x) {
+    uint32_t tg;
+    T am;
+    if (t.up_c16_ok) {
+      const uint32_t c = t.up_c16[hb + k];
+      tg = c & 0x7FFFu;
+      am = (c & 0x8000u) ? -mag : mag;
+    } else {
+      tg = (uint32_t)t.up_tgt[hb + k];
+      am = amp[hb + k];
+    }
+    s_off[k] = (uint32_t)((uint64_t)tg * (uint64_t)Ddn * sizeof(T));
+    s_amp[k] = am;
+  }
+  for (int64_t d = threadIdx.x; d < Ddn; d += blockDim.x) {
+    T v[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) v[p] = x[base0 + p * D + d];
+    store_pack<T, NP>(row + d * NP, v);
+  }
+  if (threadIdx.x < NP) row[Ddn * NP + threadIdx.x] = T(0);
+  // plane group base as bytes: uniform 64-bit base + 32-bit row offset + column
+  const char *__restrict__ xg = reinterpret_cast<const char *>(x + pg * NP * D);
+  __syncthreads();
+  const uint32_t su = t.s_up[u];
+  const T *__restrict__ diag = Tabs<T>::diag(t);
+  const T h = Tabs<T>::mags(t)[0];
+  const EpiK<T> bp = epi_consts<T, E>(a);
+  const T on_u = t.has_onsite ? Tabs<T>::up_on(t)[u] : T(0);
+  const int lane = threadIdx.x & (kWave - 1);
+  Partial2 acc2[NP];
+  for (int64_t d0 = (int64_t)threadIdx.x - lane; d0 < Ddn; d0 += blockDim.x) {
+    const int64_t d = d0 + lane;  // tables are padded to a multiple of 64 strings
+    const bool ok = d < Ddn;
+    const int64_t dc = ok ? d : Ddn - 1;  // in bounds for the up-source loads
+    // up hops, in hop order as the strip kernel sums them
+    T up[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) up[p] = T(0);
+#pragma unroll 4
+    for (int k = 0; k < nh; ++k) {
+      const T am = s_amp[k];
+      const uint32_t off = s_off[k] + (uint32_t)(dc * (int64_t)sizeof(T));
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        up[p] += am * *reinterpret_cast<const T *>(xg + (int64_t)p * D * (int64_t)sizeof(T) + off);
+    }
+    const uint4 *tp = reinterpret_cast<const uint4 *>(t.dn_comp + d * K);
+    uint4 q[K / 8];
+#pragma unroll
+    for (int j = 0; j < K / 8; ++j) q[j] = tp[j];
+    const uint32_t sm = t.dn_csign[d];
+    const int wm = __builtin_amdgcn_readfirstlane((int)t.dn_cgmax[d0 / kWave]);
+    T dg = diag[popc(su & t.s_dn[ok ? d : 0])];
+    if (t.has_onsite) dg = dg + on_u + Tabs<T>::dn_on(t)[ok ? d : 0];
+    T h0[NP], h1[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) h0[p] = h1[p] = T(0);
+#pragma unroll
+    for (int g = 0; g < K / G; ++g) {
+      if (g * G < wm) {
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const int k = g * G + j;
+          const uint32_t w = reinterpret_cast<const uint32_t *>(&q[k / 8])[(k % 8) / 2];
+          const uint32_t tg = (k & 1) ? (w >> 16) : (w & 0xFFFFu);
+          T v[NP];
+          load_pack<T, NP>(row + (int64_t)tg * NP, v);
+          const uint32_t sb = (sm << (31 - k)) & 0x80000000u;
+#pragma unroll
+          for (int p = 0; p < NP; ++p) {
+            const T fv = flip_sign(v[p], sb);
+            if (j & 1) h1[p] += fv; else h0[p] += fv;
+          }
+        }
+      }
+    }
+    if (ok) {
+      T xv[NP];
+      load_pack<T, NP>(row + d * NP, xv);
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const int64_t i = base0 + p * D + d;
+        const T r = up[p] + (dg * xv[p] + h * (h0[p] + h1[p]));
+        epilogue<T, E>(r, xv[p], i, y, a, bp, acc2[p]);
+      }
+    }
+  }
+  if constexpr (E == Epi::Lanczos || E == Epi::Cheb) {
+    const int wv = threadIdx.x / kWave;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+#pragma unroll
+      for (int c = 0; c < NS; ++c) {
+        double v = c == 0 ? acc2[p].s0 : acc2[p].s1;
+        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+        if (lane == 0) red[NS * p + c][wv] = v;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < NS * NP) {
+      double r = 0.0;
+      for (int w = 0; w < (int)(blockDim.x / kWave); ++w) r += red[threadIdx.x][w];
+      const int p = threadIdx.x / NS, c = threadIdx.x % NS;
+      const int64_t prow = (pg * NP + p) * Dup + u;  // the plane row's slot
+      if constexpr (E == Epi::Lanczos)
+        a.partial[prow] = r;
+      else
+        a.partial[2 * prow + c] = r;
+    }
+  }
+}
+
 // Epilogue with the values it reads from memory loaded beforehand: pv = prev[i]
 // (ignored where the epilogue does not read it).
 template <typename T, Epi E>
@@ -2096,9 +2240,79 @@ int64_t hv_partials(const HvTables &t, int P) {
   return two > gen ? two : gen;
 }
 
+// Planes per workgroup of the one-pass kernel, or 0 for the two passes.  The
+// one pass needs the compact down table, at most kFusedMaxHops up hops per row,
+// a pack of 8 to 32 bytes per string, and a plane group (the planes one
+// workgroup covers) of at most kFusedGroupBytes, so that the up-source rows of
+// the group being worked on stay in the 256 MB Infinity Cache.  Two planes per
+// workgroup (re and im of a vector) where P allows: measured on the L = 14,
+// 7 + 7 ring, one Chebyshev step over 16 vectors takes 2.62 ms in FP32 with
+// two planes (2.72 with four) and 4.50 ms in FP64 with two (4.83 with one),
+// against 3.24 and 6.11 ms for the two passes.  MANYBODY_AMD_ROW_PLANES sets the
+// planes as for the two-pass row kernel; MANYBODY_AMD_FUSED=0 keeps the two
+// passes, =1 takes the one pass wherever the tables allow it, whatever the group
+// size.
+constexpr int64_t kFusedGroupBytes = int64_t(200) << 20;
+
+template <typename T>
+int fused_planes(const HvTables &t, int P) {
+  const char *e = std::getenv("MANYBODY_AMD_FUSED");
+  if (e && std::strcmp(e, "0") == 0) return 0;
+  const bool force = e && std::strcmp(e, "1") == 0;
+  if (!t.comp_ok || t.max_up_row > kFusedMaxHops) return 0;
+  if (t.d_up * t.d_dn * (int64_t)sizeof(T) > (int64_t)UINT32_MAX) return 0;  // 32-bit row offsets
+  int np = row_planes<T>(t, P);
+  if (!std::getenv("MANYBODY_AMD_ROW_PLANES") && P % 2 == 0 &&
+      (size_t)(t.d_dn + 1) * 2 * sizeof(T) <= (size_t)kMpRowLds)
+    np = 2;
+  if (np * sizeof(T) < 8) return 0;  // packs of 8, 16 or 32 bytes
+  if ((size_t)(t.d_dn + 1) * np * sizeof(T) > (size_t)kMpRowLds) return 0;
+  if (!force && (int64_t)np * t.d_up * t.d_dn * (int64_t)sizeof(T) > kFusedGroupBytes) return 0;
+  return np;
+}
+
+template <typename T, Epi E, int NP>
+void launch_fused_np(const HvTables &t, int P, const T *x, T *y, const EpiArgs &a, hipStream_t s) {
+  const dim3 g((unsigned)((int64_t)P / NP * t.d_up)), b(row_block(t.d_dn));
+  const size_t lds = (size_t)(t.d_dn + 1) * NP * sizeof(T);
+  hipLaunchKernelGGL((row_kernel_fused<T, E, NP>), g, b, lds, s, t, x, y, a);
+  MB_HIP(hipGetLastError());
+}
+
+template <typename T, Epi E>
+void launch_fused(const HvTables &t, int P, int np, const T *x, T *y, const EpiArgs &a, hipStream_t s) {
+  switch (np) {
+    case 1:
+      if constexpr (sizeof(T) == 8) {
+        launch_fused_np<T, E, 1>(t, P, x, y, a, s);
+        break;
+      }
+      throw std::logic_error("one-pass H.v: pack narrower than 8 bytes");
+    case 2: launch_fused_np<T, E, 2>(t, P, x, y, a, s); break;
+    case 4: launch_fused_np<T, E, 4>(t, P, x, y, a, s); break;
+    default:
+      if constexpr (sizeof(T) == 4) {
+        launch_fused_np<T, E, 8>(t, P, x, y, a, s);
+        break;
+      }
+      throw std::logic_error("one-pass H.v: pack wider than 32 bytes");
+  }
+}
+
 template <typename T>
 int64_t launch_hv(const HvTables &t, int P, const T *x, T *y, Epi epi, const EpiArgs &a,
                   hipStream_t s) {
+  if (use_two_pass<T>(t) && (int64_t)P * t.d_up < (1ll << 31)) {
+    if (const int np = fused_planes<T>(t, P)) {
+      switch (epi) {
+        case Epi::Plain: launch_fused<T, Epi::Plain>(t, P, np, x, y, a, s); break;
+        case Epi::Lanczos: launch_fused<T, Epi::Lanczos>(t, P, np, x, y, a, s); break;
+        case Epi::Recon: launch_fused<T, Epi::Recon>(t, P, np, x, y, a, s); break;
+        case Epi::Cheb: launch_fused<T, Epi::Cheb>(t, P, np, x, y, a, s); break;
+      }
+      return t.d_up;  // partial slots per plane
+    }
+  }
   if (use_two_pass<T>(t)) {
     if ((int64_t)P * t.d_up >= (1ll << 31))
       throw std::invalid_argument("sector too large for the two-pass H.v");

@@ -51,12 +51,21 @@ def make_op(sizes, nu, nd, t, U):
     return model_of(sizes, t, U).operator(basis)
 
 
+# H.v paths: the two passes (strip, then row) and the one-pass row kernel that
+# forms the up hops itself (the default for sectors whose plane group fits the
+# Infinity Cache); the bit-exact tests run both
+@pytest.fixture(params=["0", "1"], ids=["two_pass", "one_pass"])
+def hv_path(request, monkeypatch):
+    monkeypatch.setenv("MANYBODY_AMD_FUSED", request.param)
+    return request.param
+
+
 def int_vec(rng, n):
     return (rng.integers(-4, 5, n) + 1j * rng.integers(-4, 5, n)).astype(np.complex128)
 
 
 @pytest.mark.parametrize("name,sizes,nu,nd,t,U", CASES, ids=[c[0] for c in CASES])
-def test_hv_c128_bitexact_vs_product_oracle(cuda, name, sizes, nu, nd, t, U):
+def test_hv_c128_bitexact_vs_product_oracle(cuda, hv_path, name, sizes, nu, nd, t, U):
     import torch
     op = make_op(sizes, nu, nd, t, U)
     P = po.ProductOperator(sizes, nu, nd, t, U)
@@ -67,7 +76,7 @@ def test_hv_c128_bitexact_vs_product_oracle(cuda, name, sizes, nu, nd, t, U):
 
 
 @pytest.mark.parametrize("name,sizes,nu,nd,t,U", CASES[:6], ids=[c[0] for c in CASES[:6]])
-def test_apply_reference_order_bitexact_vs_csc_oracle(cuda, name, sizes, nu, nd, t, U):
+def test_apply_reference_order_bitexact_vs_csc_oracle(cuda, hv_path, name, sizes, nu, nd, t, U):
     """The whole boundary: reference order in, gather, H.v, scatter, reference order out."""
     import torch
     op = make_op(sizes, nu, nd, t, U)
@@ -91,7 +100,7 @@ EDGE_CASES = [
 
 
 @pytest.mark.parametrize("name,sizes,nu,nd,t,U", EDGE_CASES, ids=[c[0] for c in EDGE_CASES])
-def test_hv_edge_sectors_bitexact(cuda, name, sizes, nu, nd, t, U):
+def test_hv_edge_sectors_bitexact(cuda, hv_path, name, sizes, nu, nd, t, U):
     """Integer inputs, bit for bit against the product oracle, and through the
     reference-order boundary against the CSC oracle where it applies (L <= 31)."""
     import torch
@@ -126,7 +135,7 @@ def test_permutation_matches_oracle_and_roundtrips(cuda):
 
 
 @pytest.mark.parametrize("name,sizes,nu,nd,t,U", CASES[:4], ids=[c[0] for c in CASES[:4]])
-def test_hv_c64_bitexact_vs_product_oracle(cuda, name, sizes, nu, nd, t, U):
+def test_hv_c64_bitexact_vs_product_oracle(cuda, hv_path, name, sizes, nu, nd, t, U):
     import torch
     op = make_op(sizes, nu, nd, t, U)
     P = po.ProductOperator(sizes, nu, nd, t, U)

@@ -1,4 +1,4 @@
-"""The two row kernels of H.v on sectors with D_dn = 11,440 (4 x 4, 7 down
+"""The row kernels of the two-pass H.v on sectors with D_dn = 11,440 (4 x 4, 7 down
 electrons, 1 or 2 up): the persistent hop-count-sorted kernel (the default at
 this row length) against the per-row compact kernel (selected with
 MANYBODY_AMD_ROW_KERNEL=compact) and the oracle, through every epilogue: plain
@@ -11,6 +11,8 @@ for bit with the oracle; Gaussian inputs 1e-14 relative between the kernels
 relative over 30 steps; eigenvalue 1e-12 relative, eigenvector 1e-9 relative;
 FP64 KPM moments 1e-12 absolute (mu_0 = 1 to 1e-12), FP32 KPM moments 1e-5
 absolute between the kernels and 1e-4 against FP64 (mu_0 = 1 to 1e-6).
+The last test compares the one-pass H.v (the row kernel forms its row's up
+hops itself) with the two passes on 1D, 2D and 3D sectors.
 """
 import contextlib
 import os
@@ -25,6 +27,14 @@ from oracle import pyoracle as po
 pytestmark = pytest.mark.gpu
 
 T_HOP, U = 1.0, 8.0
+
+
+@pytest.fixture(autouse=True)
+def two_passes(monkeypatch):
+    """These tests compare the row kernels of the two-pass H.v, so the one-pass
+    kernel (the default on sectors that fit the Infinity Cache) is switched off;
+    test_one_pass_hv_equals_two_passes switches it on for its own comparison."""
+    monkeypatch.setenv("MANYBODY_AMD_FUSED", "0")
 
 
 @contextlib.contextmanager
@@ -183,3 +193,67 @@ def test_multi_plane_row_kernel_equals_one_plane_kernel(cuda, L, nu, nd):
             yy = op.hv(zz).cpu().numpy()
         for r in range(4):
             assert np.array_equal(yy[2 * r] + 1j * yy[2 * r + 1], ref), (cap, r)
+
+
+@contextlib.contextmanager
+def one_pass(mode):
+    """MANYBODY_AMD_FUSED: "0" the two passes (strip, then row), "1" the one-pass
+    kernel wherever the tables allow it."""
+    old = os.environ.get("MANYBODY_AMD_FUSED")
+    os.environ["MANYBODY_AMD_FUSED"] = mode
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ["MANYBODY_AMD_FUSED"]
+        else:
+            os.environ["MANYBODY_AMD_FUSED"] = old
+
+
+@pytest.mark.parametrize("sizes,nu,nd", [([14], 7, 7), ([12], 6, 6), ([10], 5, 4), ([8], 4, 4),
+                                         ([4, 3], 6, 6), ([3, 3], 4, 4), ([2, 2, 3], 5, 5)])
+def test_one_pass_hv_equals_two_passes(cuda, sizes, nu, nd):
+    """The one-pass H.v (each row kernel forms its row's up hops from the source
+    rows, MANYBODY_AMD_FUSED=1) against the two passes (=0).  Both sum the up
+    hops in the same order and add the down hops and the diagonal the same way;
+    the compiler may contract multiply-adds differently (and in FP64 the two
+    passes use the one-plane row kernel), so: FP64 H.v to 1e-15 relative, FP32
+    H.v over four and eight planes to 1e-6, 12 Lanczos coefficients to 1e-13,
+    FP64 KPM moments to 1e-13 and FP32 moments to 1e-6.  Integer inputs: bit
+    for bit against the oracle."""
+    N = int(np.prod(sizes))
+    model = (mb.HubbardModel(T_HOP, 4.0, sizes[0]) if len(sizes) == 1 else
+             mb.HubbardModel2D(T_HOP, 4.0, *sizes) if len(sizes) == 2 else
+             mb.HubbardModel3D(T_HOP, 4.0, *sizes))
+    op = model.operator(mb.FermionBasis.sector(N, nu, nd))
+    x = op.gaussian(23)
+    xf = torch.cat([x, op.gaussian(29)]).float()
+    x8 = torch.cat([xf, xf.flip(0)])
+    w, _ = mb.tridiag_eig(*[getattr(mb.lanczos_pass_one(op, x, 40), f) for f in ("alphas", "betas")],
+                          vectors=False)
+    a, c = (w[-1] - w[0]) / 2 * 1.05, (w[-1] + w[0]) / 2
+    out = {}
+    for mode in ("0", "1"):
+        with one_pass(mode):
+            dec = mb.lanczos_pass_one(op, x, 12)
+            out[mode] = (op.hv(x), op.hv(xf), op.hv(x8), np.array(dec.alphas), np.array(dec.betas),
+                         np.array(mb.kpm_dos_moments(op, a, c, 32, 8, 5, fp32=True)),
+                         np.array(mb.kpm_dos_moments(op, a, c, 32, 4, 5, fp32=False)))
+        print(f"{sizes} {nu}+{nd} D={op.dimension()} mode {mode}: |Hx| = "
+              f"{out[mode][0].abs().max().item():.6f}", flush=True)
+    tol = (1e-15, 1e-6, 1e-6, 1e-13, 1e-13)
+    for r0, r1, tl in zip(out["0"][:5], out["1"][:5], tol):
+        if isinstance(r0, torch.Tensor):
+            assert (r1 - r0).abs().max().item() <= tl * r0.abs().max().item()
+        else:
+            assert np.abs(r1 - r0).max() <= tl * np.abs(r0).max()
+    assert np.abs(out["1"][5] - out["0"][5]).max() <= 1e-6
+    assert np.abs(out["1"][6] - out["0"][6]).max() <= 1e-13
+    # integer inputs against the oracle, one pass
+    rng = np.random.default_rng(31)
+    D = op.dimension()
+    z = (rng.integers(-4, 5, D) + 1j * rng.integers(-4, 5, D)).astype(np.complex128)
+    P = po.ProductOperator(sizes, nu, nd, T_HOP, 4.0)
+    with one_pass("1"):
+        yz = op.hv(torch.from_numpy(np.stack([z.real, z.imag])).to(cuda)).cpu().numpy()
+    assert np.array_equal(yz[0] + 1j * yz[1], P.matvec(z))
