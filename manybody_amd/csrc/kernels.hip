@@ -2273,7 +2273,9 @@ int fused_planes(const HvTables &t, int P) {
 
 template <typename T, Epi E, int NP>
 void launch_fused_np(const HvTables &t, int P, const T *x, T *y, const EpiArgs &a, hipStream_t s) {
-  const dim3 g((unsigned)((int64_t)P / NP * t.d_up)), b(row_block(t.d_dn));
+  // 1024 threads from 2048 strings on (L = 14 ring, FP64: 4.17 ms per Chebyshev
+  // step over 16 vectors against 4.56 with 512; FP32 the same either way)
+  const dim3 g((unsigned)((int64_t)P / NP * t.d_up)), b(t.d_dn >= 2048 ? 1024 : row_block(t.d_dn));
   const size_t lds = (size_t)(t.d_dn + 1) * NP * sizeof(T);
   hipLaunchKernelGGL((row_kernel_fused<T, E, NP>), g, b, lds, s, t, x, y, a);
   MB_HIP(hipGetLastError());
