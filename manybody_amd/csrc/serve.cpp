@@ -8,9 +8,12 @@
 // Krylov vector live on the device for the whole request.
 //
 // Continuous admission: `slots` worker threads, each with its own non-blocking
-// HIP stream, take the oldest arrived request whose device footprint fits in
-// the free part of one pre-allocated arena (a request that has waited longer
-// than kBypassSeconds is not overtaken).  The admitted range becomes the
+// HIP stream, take, among the arrived requests whose device footprint fits in
+// the free part of one pre-allocated arena, the one with the longest expected
+// service time (a request that has waited longer than kBypassSeconds goes
+// first and is not overtaken).  Longest first keeps a long chain (a Krylov
+// evolution is thousands of dependent small steps) from starting last and
+// setting the end of a burst.  The admitted range becomes the
 // request's private stack allocator, so the loop never calls hipMalloc or
 // hipFree (which would synchronise the whole device) and a request can never
 // take memory it was not admitted with.  Requests that arrive while others run
@@ -22,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "../../include/manybody_hip.h"
 #include "krylov.hpp"
@@ -177,6 +181,22 @@ Footprint footprint_parts(const ServeRequest &r) {
   return {D, tables, work};
 }
 
+// Expected service time of a request running alone (seconds): Lanczos-type
+// steps times a per-step cost of a launch-bound floor plus the HBM passes over
+// the vector.  Only the order matters (admission picks the largest).
+double expected_seconds(const ServeRequest &r, int64_t D) {
+  const double floor_s = 20e-6, per_elem = 6.0 * 16.0 / 5e12;  // ~6 complex passes at 5 TB/s
+  const double step = floor_s + per_elem * (double)D;
+  if (r.kind == kReqGround) return r.steps * step;
+  if (r.kind == kReqKpm)
+    return r.steps * step + 0.5 * r.moments * (floor_s + 0.5 * per_elem * (double)D * r.nvec);
+  if (r.kind == kReqEvolve) {
+    const double n_t = r.dt > 0.0 ? std::ceil(std::fabs(r.t_final) / r.dt) : 1.0;
+    return n_t * std::max(r.steps, 1) * step;
+  }
+  return 0.0;
+}
+
 int64_t request_footprint(const ServeRequest &r) {
   const Footprint f = footprint_parts(r);
   return f.tables + f.work + 64 * 1024;
@@ -311,6 +331,7 @@ int64_t Server::submit(const ServeRequest &r) {
   j->req = r;
   j->res.arrival = r.arrival >= 0.0 ? r.arrival : now();
   j->fp = footprint_parts(r);  // throws on an invalid request
+  j->cost = expected_seconds(r, j->fp.dim);
   j->lane = lane_ && r.kind == kReqGround && j->fp.dim <= lane_max_dim_ && r.steps <= kLaneKmax;
   // lane members hold their four vectors in the lane's buffers
   j->res.footprint = j->lane ? j->fp.tables + 64 * 1024 + 4 * 16 * j->fp.dim : request_footprint(r);
@@ -346,11 +367,15 @@ void Server::finish_locked(Job *j) {
   --pending_;
 }
 
-// Oldest arrived request of the given kind (lane or slot) that fits; sets *off
-// to its arena offset.  For the lane, the arena holds only the tables (the
-// vectors go to the lane's buffers, whose range and slot the caller takes).
+// The next arrived request of the given kind (lane or slot) that fits; sets
+// *off to its arena offset.  Lane requests go in arrival order.  Slot requests:
+// the oldest if it has waited longer than kBypassSeconds (and nothing overtakes
+// it while it does not fit), else the one with the longest expected service
+// time that fits.  For the lane, the arena holds only the tables (the vectors
+// go to the lane's buffers, whose range and slot the caller takes).
 Server::Job *Server::pick_locked(double t, double *next_arrival, int64_t *off, bool lane) {
   *next_arrival = -1.0;
+  std::vector<std::deque<Job *>::iterator> cands;  // arrived slot requests, by arrival
   for (auto it = waiting_.begin(); it != waiting_.end(); ++it) {
     Job *c = *it;
     if (c->res.arrival > t) {
@@ -381,13 +406,27 @@ Server::Job *Server::pick_locked(double t, double *next_arrival, int64_t *off, b
       cv_done_.notify_all();
       return pick_locked(t, next_arrival, off, lane);
     }
-    const int64_t o = arena_.take(c->res.footprint);
+    cands.push_back(it);
+  }
+  if (cands.empty()) return nullptr;
+  if (t - (*cands.front())->res.arrival > kBypassSeconds) {  // a long-waiting request goes first
+    const int64_t o = arena_.take((*cands.front())->res.footprint);
+    if (o < 0) return nullptr;  // and is not overtaken
+    Job *c = *cands.front();
+    *off = o;
+    waiting_.erase(cands.front());
+    return c;
+  }
+  std::stable_sort(cands.begin(), cands.end(),
+                   [](const auto &a, const auto &b) { return (*a)->cost > (*b)->cost; });
+  for (auto &it : cands) {
+    const int64_t o = arena_.take((*it)->res.footprint);
     if (o >= 0) {
+      Job *c = *it;
       *off = o;
       waiting_.erase(it);
       return c;
     }
-    if (t - c->res.arrival > kBypassSeconds) break;  // do not overtake a long-waiting request
   }
   return nullptr;
 }

@@ -124,6 +124,7 @@ class Server {
     ServeRequest req;
     ServeResult res;
     Footprint fp{};
+    double cost = 0.0;  // expected service time alone (seconds), for the admission order
     bool lane = false;  // runs in the Lanczos lane
     bool done = false;
   };
