@@ -255,7 +255,74 @@ __global__ __launch_bounds__(BS) void bl_persistent_kernel(const BInst *__restri
   }
 }
 
+// The Lanczos run of one Krylov time step for one small sector, all k steps in
+// one workgroup (the recurrence of bl_persistent_kernel), with the Krylov
+// vectors kept: v_j is written to basis + 2 D j (normalised), the scalars to
+// scal = alpha[k], beta[k], |b|, steps.  The caller forms exp(-i tau T) e1 and
+// combines the basis.  A zero b leaves steps = 0 and |b| in scal.
+template <int BS>
+__global__ __launch_bounds__(BS) void krylov_small_kernel(HvTables t, int64_t D,
+                                                          const double *__restrict__ b,
+                                                          double *__restrict__ basis,
+                                                          double *__restrict__ scal, int k) {
+  __shared__ double lds[BS / kWave];
+  const int64_t n2 = 2 * D;
+  const int tid = threadIdx.x;
+  double s = 0.0;
+  for (int64_t e = tid; e < n2; e += BS) s += b[e] * b[e];
+  const double bn = sqrt(block_allsum<BS>(s, lds));
+  if (tid == 0) {
+    scal[2 * k] = bn;
+    scal[2 * k + 1] = 0.0;
+  }
+  if (bn <= kTol) return;
+  for (int64_t e = tid; e < n2; e += BS) basis[e] = b[e] / bn;
+  __syncthreads();
+  double bprev = 0.0;
+  for (int j = 0; j < k; ++j) {
+    const double *v = basis + (int64_t)j * n2;
+    const double *vp = j > 0 ? v - n2 : v;  // read only for j > 0
+    double *w = basis + (int64_t)(j + 1) * n2;
+    s = 0.0;
+    for (int64_t e = tid; e < D; e += BS) {
+      double re, im;
+      hv_pair(t, v, D, e, re, im);
+      if (j > 0) {
+        re -= bprev * vp[e];
+        im -= bprev * vp[D + e];
+      }
+      w[e] = re;
+      w[D + e] = im;
+      s += v[e] * re + v[D + e] * im;
+    }
+    const double a = block_allsum<BS>(s, lds);
+    s = 0.0;
+    for (int64_t e = tid; e < n2; e += BS) {
+      const double q = w[e] - a * v[e];
+      w[e] = q;
+      s += q * q;
+    }
+    const double bt = sqrt(block_allsum<BS>(s, lds));
+    if (tid == 0) {
+      scal[j] = a;
+      scal[k + j] = bt;
+      scal[2 * k + 1] = (double)(j + 1);
+    }
+    if (bt <= kTol) return;
+    for (int64_t e = tid; e < n2; e += BS) w[e] /= bt;
+    bprev = bt;
+    __syncthreads();
+  }
+}
+
 }  // namespace
+
+void launch_krylov_small(const HvTables &t, int64_t D, const double *b, double *basis, double *scal,
+                         int k, hipStream_t s) {
+  if (D > kKrylovSmallMaxDim) throw std::invalid_argument("sector too large for the one-workgroup Krylov run");
+  hipLaunchKernelGGL((krylov_small_kernel<256>), dim3(1), dim3(256), 0, s, t, D, b, basis, scal, k);
+  MB_HIP(hipGetLastError());
+}
 
 // ------------------------------------------------------------ host engine
 

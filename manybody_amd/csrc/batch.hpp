@@ -28,6 +28,13 @@ struct BInst {
 // workgroup per instance is latency-bound beyond a few thousand elements).
 constexpr int64_t kPersistMaxDim = 2048;
 
+// Krylov time step of one small real-space sector in one workgroup (batch.hip):
+// k Lanczos steps from b with the vectors kept in basis ((k + 1) * 2 D doubles)
+// and scal = alpha[k], beta[k], |b|, steps.  D <= kKrylovSmallMaxDim.
+constexpr int64_t kKrylovSmallMaxDim = 2048;
+void launch_krylov_small(const HvTables &t, int64_t D, const double *b, double *basis, double *scal,
+                         int k, hipStream_t s);
+
 // Longest row (D_dn) handled by row tiles.
 constexpr int64_t kMaxRowsTileDn = 2048;
 // LDS of a row tile of n elements (whole rows of D_dn): both planes at row

@@ -9,8 +9,12 @@
 // solved on the host by implicit QL.
 #include "krylov.hpp"
 
+#include "batch.hpp"
+
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <functional>
 #include <limits>
 #include <complex>
 
@@ -512,6 +516,179 @@ void time_evolve(const HvOp &op, const void *psi, double t, int krylov_steps, vo
     time_evolve_t<double>(op, psi, t, krylov_steps, out, s);
 }
 
+// The device part of one Krylov step with a stored basis, all on the stream:
+// norm of b, v_0 = b / |b|, then k Lanczos steps writing basis[1 .. k], with
+// alpha, beta, |b| and the step count left in W (the kernels and their order
+// are those of pass_one_t).  No host synchronisation, so it can be captured.
+template <typename T>
+void enqueue_lanczos_basis(const HvOp &op, const void *b, int k, hipStream_t s, LanczosWork &W,
+                           T *basis) {
+  const int64_t D = op.dim();
+  launch_norm((const double *)b, 2 * D, W.st.partial, W.st.bnorm, s);
+  launch_scale_div<T>((const double *)b, W.st.bnorm, basis, 2 * D, s);
+  launch_zero(W.st.flags, 2 * sizeof(int), s);
+  for (int i = 0; i < k; ++i) {
+    T *v = basis + (int64_t)i * 2 * D, *vp = i > 0 ? v - 2 * D : v, *w = v + 2 * D;
+    EpiArgs a;
+    a.prev = vp;
+    a.beta_dev = W.st.beta;
+    a.flags = W.st.flags;
+    a.step = i;
+    a.partial = W.st.partial;
+    a.lazy = 0;
+    const int64_t slots = op.hv(v, w, Epi::Lanczos, a, s);
+    launch_lanczos_finish_alpha(W.st, i, 2 * slots, s);
+    launch_lanczos_axpy_norm<T>(v, w, 2 * D, W.st, i, s, false);
+    launch_lanczos_finish_beta(W.st, i, s);
+    launch_lanczos_normalize<T>(w, 2 * D, W.st, i, s);
+  }
+}
+
+// time_evolve_steps with the device part of every step captured once in a
+// hipGraph and replayed: a step of a small sector is a few hundred launches
+// (k Lanczos steps of five kernels), which the graph issues without the host.
+// The state `out`, the stored basis and the workspace keep their addresses
+// across the steps, so one graph serves all of them; the host reads alpha,
+// beta, |b| and the step count after each replay, forms exp(-i tau T) e1 and
+// enqueues the combine.  Same kernels in the same order as the eager steps,
+// so the same results.  Returns false (nothing done) when the stored basis does
+// not fit or the capture fails; MANYBODY_AMD_KRYLOV_GRAPH=0 turns it off.
+template <typename T>
+bool time_evolve_steps_graph(const HvOp &op, double t0, double t1, double dt, int krylov_steps,
+                             void *out, hipStream_t s, const std::function<void(double)> &report) {
+  const char *e = std::getenv("MANYBODY_AMD_KRYLOV_GRAPH");
+  if (e && std::strcmp(e, "0") == 0) return false;
+  const int64_t D = op.dim();
+  int k = krylov_steps;
+  if (k == 0 || k > D) k = (int)D;
+  if (k < 1) return false;
+  const size_t bytes = (size_t)(k + 1) * 2 * D * sizeof(T);
+  if (!keep_krylov_basis(bytes)) return false;
+  DevBuf basis;
+  try {
+    basis.alloc(bytes);
+  } catch (const NoMemError &) {
+    return false;
+  }
+  LanczosWork W(D, k, op.partials(), sizeof(T), false);
+  DevBuf coef((size_t)2 * k * sizeof(double));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  bool ok = true;
+  try {
+    enqueue_lanczos_basis<T>(op, out, k, s, W, basis.as<T>());
+  } catch (const std::exception &) {
+    ok = false;
+  }
+  const hipError_t ec = hipStreamEndCapture(s, &graph);
+  if (!ok || ec != hipSuccess || !graph ||
+      hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    if (graph) (void)hipGraphDestroy(graph);
+    return false;
+  }
+  struct Guard {
+    hipGraph_t g;
+    hipGraphExec_t x;
+    ~Guard() {
+      (void)hipGraphExecDestroy(x);
+      (void)hipGraphDestroy(g);
+    }
+  } guard{graph, exec};
+  const double direction = (t1 >= t0) ? 1.0 : -1.0;
+  dt = std::fabs(dt) * direction;
+  double t = t0;
+  report(t);
+  std::vector<double> scal(2 * (size_t)k + 2), yc(2 * (size_t)k);
+  while ((direction > 0 && t < t1) || (direction < 0 && t > t1)) {
+    const double step = (direction > 0) ? std::min(dt, t1 - t) : std::max(dt, t1 - t);
+    MB_HIP(hipGraphLaunch(exec, s));
+    // alpha[0..k), beta[0..k), then |b| and the flags in the next two words
+    MB_HIP(hipMemcpyAsync(scal.data(), W.st.alpha, scal.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    MB_HIP(hipStreamSynchronize(s));
+    const double bnorm = scal[2 * k];
+    int flags[2];
+    std::memcpy(flags, &scal[2 * k + 1], sizeof(flags));
+    if (bnorm > kTol && flags[1] > 0) {  // a zero state stays as it is (evolve_state.h:131-134)
+      LanczosResult dec;
+      dec.steps = flags[1];
+      dec.b_norm = bnorm;
+      dec.alphas.assign(scal.begin(), scal.begin() + dec.steps);
+      dec.betas.assign(scal.begin() + k, scal.begin() + k + std::max(dec.steps - 1, 0));
+      auto y = expm_first_column(dec, step);
+      for (size_t j = 0; j < y.size(); ++j) {
+        yc[2 * j] = y[j].real() * bnorm;
+        yc[2 * j + 1] = y[j].imag() * bnorm;
+      }
+      MB_HIP(hipMemcpyAsync(coef.p, yc.data(), y.size() * 2 * sizeof(double), hipMemcpyHostToDevice, s));
+      launch_basis_combine<T>(basis.as<T>(), (int)y.size(), D, coef.as<double>(), (double *)out, s);
+    } else if (bnorm > kTol) {
+      MB_HIP(hipMemsetAsync(out, 0, (size_t)D * 16, s));
+    }
+    t += step;
+    report(t);
+  }
+  MB_HIP(hipStreamSynchronize(s));
+  return true;
+}
+
+// FP64 time_evolve_steps for a real-space sector of at most kKrylovSmallMaxDim
+// states: each step's k Lanczos steps run in one workgroup (launch_krylov_small,
+// the vectors written into the stored basis), then exp(-i tau T) e1 on the host
+// and the combine: three device operations per step where the general path
+// has five launches per Lanczos step.  Its sums run in another order than the
+// general path's, so the states agree to rounding.  Returns false (nothing
+// done) for other operators and sizes; MANYBODY_AMD_KRYLOV_SMALL=0 turns it off.
+bool time_evolve_steps_small(const HvOp &op, double t0, double t1, double dt, int krylov_steps,
+                             void *out, hipStream_t s, const std::function<void(double)> &report) {
+  const char *e = std::getenv("MANYBODY_AMD_KRYLOV_SMALL");
+  if (e && std::strcmp(e, "0") == 0) return false;
+  const HvTables *tb = op.hubbard_tables();
+  const int64_t D = op.dim();
+  if (!tb || D < 1 || D > kKrylovSmallMaxDim) return false;
+  int k = krylov_steps;
+  if (k == 0 || k > D) k = (int)D;
+  DevBuf basis((size_t)(k + 1) * 2 * D * sizeof(double));
+  DevBuf scal((size_t)(2 * k + 2) * sizeof(double));
+  DevBuf coef((size_t)2 * k * sizeof(double));
+  const double direction = (t1 >= t0) ? 1.0 : -1.0;
+  dt = std::fabs(dt) * direction;
+  double t = t0;
+  report(t);
+  std::vector<double> hs(2 * (size_t)k + 2), yc(2 * (size_t)k);
+  while ((direction > 0 && t < t1) || (direction < 0 && t > t1)) {
+    const double step = (direction > 0) ? std::min(dt, t1 - t) : std::max(dt, t1 - t);
+    launch_krylov_small(*tb, D, (const double *)out, basis.as<double>(), scal.as<double>(), k, s);
+    MB_HIP(hipMemcpyAsync(hs.data(), scal.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    MB_HIP(hipStreamSynchronize(s));
+    const double bnorm = hs[2 * k];
+    const int m = (int)hs[2 * k + 1];
+    if (bnorm > kTol && m > 0) {  // a zero state stays as it is (evolve_state.h:131-134)
+      LanczosResult dec;
+      dec.steps = m;
+      dec.b_norm = bnorm;
+      dec.alphas.assign(hs.begin(), hs.begin() + m);
+      dec.betas.assign(hs.begin() + k, hs.begin() + k + (m - 1));
+      auto y = expm_first_column(dec, step);
+      for (size_t j = 0; j < y.size(); ++j) {
+        yc[2 * j] = y[j].real() * bnorm;
+        yc[2 * j + 1] = y[j].imag() * bnorm;
+      }
+      MB_HIP(hipMemcpyAsync(coef.p, yc.data(), y.size() * 2 * sizeof(double), hipMemcpyHostToDevice, s));
+      launch_basis_combine<double>(basis.as<double>(), m, D, coef.as<double>(), (double *)out, s);
+    }
+    t += step;
+    report(t);
+  }
+  MB_HIP(hipStreamSynchronize(s));
+  return true;
+}
+
 void time_evolve_steps(const HvOp &op, const void *psi, double t0, double t1, double dt,
                        int krylov_steps, void *out, hipStream_t s, bool fp32,
                        const StepCallback &callback) {
@@ -527,6 +704,11 @@ void time_evolve_steps(const HvOp &op, const void *psi, double t0, double t1, do
     report(t0);
     return;
   }
+  const std::function<void(double)> rep = report;
+  if (!fp32 && time_evolve_steps_small(op, t0, t1, dt, krylov_steps, out, s, rep)) return;
+  if (fp32 ? time_evolve_steps_graph<float>(op, t0, t1, dt, krylov_steps, out, s, rep)
+           : time_evolve_steps_graph<double>(op, t0, t1, dt, krylov_steps, out, s, rep))
+    return;
   const double direction = (t1 >= t0) ? 1.0 : -1.0;
   dt = std::fabs(dt) * direction;
   double t = t0;

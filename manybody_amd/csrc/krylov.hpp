@@ -20,6 +20,9 @@ struct HvOp {
   virtual int64_t partials() const = 0;  // partial slots one H.v may write (both planes)
   virtual int64_t hv(const double *x, double *y, Epi e, const EpiArgs &a, hipStream_t s) const = 0;
   virtual int64_t hv(const float *x, float *y, Epi e, const EpiArgs &a, hipStream_t s) const = 0;
+  // the real-space Hubbard tables, for the one-workgroup kernels of small
+  // sectors; nullptr for other operators
+  virtual const HvTables *hubbard_tables() const { return nullptr; }
 };
 
 // The real-space Hubbard operator (two-pass H.v)
@@ -33,6 +36,7 @@ struct RealHvOp final : HvOp {
   int64_t hv(const float *x, float *y, Epi e, const EpiArgs &a, hipStream_t s) const override {
     return launch_hv<float>(t_, 2, x, y, e, a, s);
   }
+  const HvTables *hubbard_tables() const override { return &t_; }
   int64_t D_;
   HvTables t_;
 };

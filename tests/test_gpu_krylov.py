@@ -534,3 +534,68 @@ def test_kpm_moments_on_any_operator(cuda):
                  sp.data.astype(np.complex128))
     ref = po.kpm_moments_csc(csc, a, b, 40, vref)
     assert np.abs(mu - ref).max() < 1e-11 and abs(mu[0] - 1.0) < 1e-14
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_time_evolve_steps_graph_replays_equal_eager_steps(cuda, monkeypatch, fp32):
+    """time_evolve_state_steps captures the device part of a Krylov step once
+    (norm, normalisation, k Lanczos steps into the stored basis) and replays it
+    at every step, with the host's exp(-i tau T) e1 and the combine in between;
+    MANYBODY_AMD_KRYLOV_GRAPH=0 runs every step eagerly.  The same kernels run in
+    the same order, so every intermediate state (seen through the callback) and
+    the final one are bit-identical, over 12 steps (the last one shorter) on a
+    real-space sector, a 2D sector, a momentum sector and a zero state."""
+    import torch
+    cases = [make_op([10], 5, 4, 1.0, 4.0), make_op([3, 2], 2, 2, 1.0, 6.0)]
+    if not fp32:  # momentum sectors run in FP64 only
+        kb = mb.FermionBasis.with_fixed_particle_number_spin_momentum(8, 6, 0, [8], [2])
+        cases.append(mb.HubbardMomentumFactorizedFixedK(kb, [8], 1.0, 4.0, [2]))
+    for op in cases:
+        psi = op.gaussian(seed=41)
+        for start in (psi, torch.zeros_like(psi)):
+            runs = {}
+            for mode in ("1", "0"):
+                monkeypatch.setenv("MANYBODY_AMD_KRYLOV_GRAPH", mode)
+                seen = []
+                out = mb.time_evolve_state_steps(op, start, 0.0, 0.53, 0.05, mb.EvolutionOptions(20),
+                                                 fp32=fp32, callback=lambda t, s: seen.append((t, s.clone())))
+                runs[mode] = (out, seen)
+            (o1, s1), (o0, s0) = runs["1"], runs["0"]
+            assert len(s1) == len(s0) == 12
+            for (ta, a), (tb, b) in zip(s1, s0):
+                assert ta == tb and torch.equal(a, b)
+            assert torch.equal(o1, o0)
+            if start is psi:
+                n0 = torch.linalg.vector_norm(psi).item()
+                assert abs(torch.linalg.vector_norm(o1).item() - n0) / n0 < (1e-5 if fp32 else 1e-10)
+            else:
+                assert torch.count_nonzero(o1).item() == 0
+
+
+def test_time_evolve_steps_small_sector_kernel_matches_general_path_and_oracle(cuda, monkeypatch):
+    """FP64 stepping on real-space sectors of at most 2048 states runs each
+    step's Lanczos in one workgroup (MANYBODY_AMD_KRYLOV_SMALL=0: the general
+    path).  Its sums run in another order, so: the states agree to 1e-12
+    relative after 20 steps, and match the dense exp(-iHt) of the oracle's
+    matrix to 1e-9, on 1D and 2D sectors, one where k exceeds the dimension
+    (the run breaks down at the invariant subspace), and a zero state."""
+    import scipy.linalg as sla
+    import torch
+    for sizes, nu, nd in [([8], 2, 3), ([3, 3], 1, 2), ([4], 1, 1)]:
+        op = make_op(sizes, nu, nd, 1.0, 4.0)
+        assert op.dimension() <= 2048
+        psi = op.gaussian(seed=5)
+        outs = {}
+        for mode in ("1", "0"):
+            monkeypatch.setenv("MANYBODY_AMD_KRYLOV_SMALL", mode)
+            outs[mode] = mb.time_evolve_state_steps(op, psi, 0.0, 1.0, 0.05, mb.EvolutionOptions(30))
+        o1, o0 = outs["1"], outs["0"]
+        assert torch.linalg.vector_norm(o1 - o0).item() <= 1e-12 * torch.linalg.vector_norm(o0).item()
+        A = po.build_csc(sizes, nu, nd, 1.0, 4.0).to_scipy().toarray()
+        x = op.to_reference_order(psi).cpu().numpy()
+        want = sla.expm(-1j * 1.0 * A) @ x
+        got = op.to_reference_order(o1).cpu().numpy()
+        assert np.linalg.norm(got - want) <= 1e-9 * np.linalg.norm(want), sizes
+        monkeypatch.setenv("MANYBODY_AMD_KRYLOV_SMALL", "1")
+        z = mb.time_evolve_state_steps(op, torch.zeros_like(psi), 0.0, 0.2, 0.05, mb.EvolutionOptions(30))
+        assert torch.count_nonzero(z).item() == 0
