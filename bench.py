@@ -129,20 +129,24 @@ def main():
                "host": host}
         del xh
 
-    lanczos = None
-    if not args.no_lanczos:
-        lanczos = bench_lanczos_c2(op, x, basis, (Lx, Ly, nu, nd, t, U),
-                                   not args.no_c2_cpu_energy, threads)
-    observables = None if args.no_lanczos else bench_observables(op, x)
+    def leg(name, skip, fn, *fargs):
+        if skip:
+            return None
+        progress(f"leg {name}")
+        return fn(*fargs)
+
+    lanczos = leg("lanczos_4x4", args.no_lanczos, bench_lanczos_c2, op, x, basis,
+                  (Lx, Ly, nu, nd, t, U), not args.no_c2_cpu_energy, threads)
+    observables = leg("observables_4x4", args.no_lanczos, bench_observables, op, x)
     del y
-    momentum = None if args.no_momentum else bench_momentum()
-    conductivity = None if args.no_conductivity else bench_conductivity()
-    expression = None if args.no_expression else bench_expression()
-    batched = None if args.no_batched else bench_batched(args.batched_n, not args.no_single)
-    kpm = None if args.no_kpm else bench_kpm_c3(M=args.kpm_moments)
-    krylov = None if args.no_kpm else bench_krylov_fp32()
-    serving = None if args.no_serving else bench_serving(args.serving_n, args.serving_rate)
-    csc = None if args.no_csc_baseline else bench_cpu_csc_c1(threads)
+    momentum = leg("momentum_4x4_K00", args.no_momentum, bench_momentum)
+    conductivity = leg("conductivity_4x4", args.no_conductivity, bench_conductivity)
+    expression = leg("expression_4x4", args.no_expression, bench_expression)
+    batched = leg("batched", args.no_batched, bench_batched, args.batched_n, not args.no_single)
+    kpm = leg("kpm_l14", args.no_kpm, bench_kpm_c3, args.kpm_moments)
+    krylov = leg("krylov_l14", args.no_kpm, bench_krylov_fp32)
+    serving = leg("serving", args.no_serving, bench_serving, args.serving_n, args.serving_rate)
+    csc = leg("cpu_csc_1d_l8", args.no_csc_baseline, bench_cpu_csc_c1, threads)
 
     line = {
         "metric": "Hubbard H·v GB/s (% HBM peak); batched solves/s + p99 latency at |ΔE|<1e-8",
@@ -179,6 +183,11 @@ def main():
 
 
 D_4X4_7P7 = 11440 * 11440
+
+
+def progress(msg):
+    """One progress line on stderr (stdout carries only the result line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def host_info():
