@@ -194,6 +194,83 @@ void pack_ell(SpeciesHops &h) {
   build_sorted(h);
 }
 
+// Blocks of up strings for the block-staged up kernel.  Strings are in colex
+// (numeric) order, so the strings that agree from bit k upward are a
+// contiguous run of rows, and a hop between two sites below k stays inside
+// the run.  For each k the run sizes bound the LDS a workgroup needs (256
+// bytes of staged strip and 4 bytes of metadata per row, 2 bytes per hop
+// entry); the k with the most in-block hops within kUpBlockLds wins.  Needs
+// fewer than 2^15 strings, at most 64 hops per string, and a fifth of the hops
+// inside blocks (otherwise the per-row strip kernel stays).
+void build_up_blocks(const std::vector<uint32_t> &strings, int L, SpeciesHops &h) {
+  h.blk_ok = false;
+  h.blk.clear();
+  h.nin.clear();
+  const size_t n = strings.size();
+  if (n == 0 || n >= 32768 || h.max_row > 64 || h.tgt.empty()) return;
+  auto runs = [&](int k, std::vector<std::pair<int, int>> *out, int &maxr, int &maxe, int64_t &in) {
+    maxr = maxe = 0;
+    in = 0;
+    size_t r = 0;
+    while (r < n) {
+      size_t r1 = r + 1;
+      while (r1 < n && ((uint64_t)strings[r1] >> k) == ((uint64_t)strings[r] >> k)) ++r1;
+      maxr = std::max(maxr, (int)(r1 - r));
+      maxe = std::max(maxe, h.row_ptr[r1] - h.row_ptr[r]);
+      for (int e = h.row_ptr[r]; e < h.row_ptr[r1]; ++e)
+        in += (h.tgt[e] >= (int32_t)r && h.tgt[e] < (int32_t)r1);
+      if (out) out->push_back({(int)r, (int)(r1 - r)});
+      r = r1;
+    }
+  };
+  int best_k = -1;
+  int64_t best_in = 0;
+  for (int k = 1; k <= L; ++k) {
+    int maxr, maxe;
+    int64_t in;
+    runs(k, nullptr, maxr, maxe, in);
+    const int64_t bytes = (int64_t)maxr * (kUpBlockRowBytes + 4) + (int64_t)maxe * 2;
+    if (bytes > kUpBlockLds || maxe > 65535) continue;
+    if (in > best_in) {
+      best_in = in;
+      best_k = k;
+    }
+  }
+  if (best_k < 0 || best_in * 5 < (int64_t)h.tgt.size()) return;
+  std::vector<std::pair<int, int>> blocks;
+  int64_t in;
+  runs(best_k, &blocks, h.blk_rows, h.blk_ent, in);
+  // out-of-block entries first, in-block after, each part in its old order
+  h.nin.assign(n, 0);
+  std::vector<int32_t> tgt(h.tgt.size());
+  std::vector<double> amp(h.amp.size());
+  for (const auto &b : blocks)
+    for (int r = b.first; r < b.first + b.second; ++r) {
+      int w = h.row_ptr[r];
+      for (int pass = 0; pass < 2; ++pass)
+        for (int e = h.row_ptr[r]; e < h.row_ptr[r + 1]; ++e) {
+          const bool inside = h.tgt[e] >= b.first && h.tgt[e] < b.first + b.second;
+          if (inside != (pass == 1)) continue;
+          tgt[w] = h.tgt[e];
+          amp[w] = h.amp[e];
+          ++w;
+          h.nin[r] += (uint8_t)inside;
+        }
+    }
+  h.tgt.swap(tgt);
+  h.amp.swap(amp);
+  std::stable_sort(blocks.begin(), blocks.end(),
+                   [](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+                     return a.second > b.second;
+                   });
+  for (const auto &b : blocks) {
+    h.blk.push_back(b.first);
+    h.blk.push_back(b.second);
+  }
+  h.blk_bits = best_k;
+  h.blk_ok = true;
+}
+
 void pack_c16(SpeciesHops &h) {
   h.c16_ok = false;
   h.c16.clear();
@@ -228,6 +305,10 @@ void upload_species(SpeciesHops &h) {
     h.d_cgmax = upload(h.cgmax);
   }
   if (h.c16_ok) h.d_c16 = upload(h.c16);
+  if (h.blk_ok) {
+    h.d_blk = upload(h.blk);
+    h.d_nin = upload(h.nin);
+  }
   if (h.srt_ok) {
     h.d_sperm = upload(h.sperm);
     h.d_scomp = upload(h.scomp);
@@ -272,6 +353,7 @@ void build_hubbard(Hubbard &h) {
   species_hops(s.s_up, mono, h.up);
   species_hops(s.s_dn, mono, h.dn);
   pack_ell(h.dn);
+  build_up_blocks(s.s_up, s.L, h.up);  // reorders the up entries: before pack_c16
   pack_c16(h.up);
   h.has_onsite = false;
   for (double v : h.up.onsite) h.has_onsite |= (v != 0.0);
@@ -336,6 +418,12 @@ HvTables tables_of(const Hubbard &h) {
   t.up_c16_ok = h.up.c16_ok ? 1 : 0;
   t.up_c16 = h.up.d_c16.as<uint16_t>();
   t.up_mag = h.up.mag16;
+  t.ub_ok = (h.up.blk_ok && h.up.c16_ok) ? 1 : 0;
+  t.ub_nb = (int)(h.up.blk.size() / 2);
+  t.ub_rows = h.up.blk_rows;
+  t.ub_ent = h.up.blk_ent;
+  t.ub_blk = h.up.d_blk.as<int32_t>();
+  t.ub_nin = h.up.d_nin.as<uint8_t>();
   return t;
 }
 

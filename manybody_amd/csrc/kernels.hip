@@ -256,6 +256,109 @@ __global__ __launch_bounds__(kBlock) void up_strip_kernel(HvTables t, int P, con
   }
 }
 
+// Block-staged up hops.  Up strings that agree from bit k upward are a
+// contiguous run of rows (a block; the strings are in colex order), and a hop
+// between two sites below k stays inside its block.  One workgroup takes one
+// block and one column strip (LPR lanes x 16 bytes per row): it stages the
+// strip of the block's rows and the block's hop entries in LDS, then each row
+// sums the hops that leave the block from L2, where the strip of every row
+// stays as for up_strip_kernel (the blocks of a strip share an XCD), and the
+// in-block hops from LDS, in the order of the row's entries (out-of-block ones
+// first).  Small blocks and many workgroups per CU measured best: on the 4 x 4,
+// 7 + 7 sector k = 8 (blocks of <= 70 rows, 12 of 32 bonds inside, ~21 KB of
+// LDS, 7 workgroups per CU) gives 1.65 ms against 2.17 ms for up_strip_kernel;
+// larger blocks keep more hops inside but leave fewer workgroups per CU
+// (k = 9: 2.0 ms, k = 10: 2.1 ms), and 128-byte strips or 512 threads were
+// slower at every k.
+constexpr int kUbStage = 8;     // staging loads per thread in flight
+constexpr int kUbLanes = 16;    // lanes per row, 16 bytes each: 256-byte strips
+constexpr int kUbThreads = 256;
+static_assert(kUbLanes * 16 == kUpBlockRowBytes, "the host sizes blocks for this strip width");
+
+template <typename T, int LPR = kUbLanes, int NT = kUbThreads>
+__global__ __launch_bounds__(NT) void up_block_kernel(HvTables t, int P, const T *__restrict__ x,
+                                                      T *__restrict__ y, int nJ, const int *flags) {
+  constexpr int VEC = 16 / sizeof(T), W = LPR * VEC;
+  extern __shared__ uint4 ub_smem[];
+  if (flags && flags[0]) return;
+  const int nb = t.ub_nb;
+  const int i = blockIdx.x;
+  const int xcd = i & 7, q = i >> 3;
+  const int s = xcd + 8 * (q / nb);
+  const int b = q % nb;
+  if (s >= P * nJ) return;
+  const int p = s / nJ, J = s - p * nJ;
+  const int r0 = t.ub_blk[2 * b], nr = t.ub_blk[2 * b + 1];
+  uint4 *sx = ub_smem;  // row r of the block at sx[r * LPR + lane]
+  uint32_t *smeta = reinterpret_cast<uint32_t *>(ub_smem + t.ub_rows * LPR);
+  uint16_t *sent = reinterpret_cast<uint16_t *>(smeta + t.ub_rows);
+  const int e0 = t.up_ptr[r0], ne = t.up_ptr[r0 + nr] - e0;
+  const int64_t Ddn = t.d_dn;
+  const uint32_t rowb = (uint32_t)(Ddn * (int64_t)sizeof(T));  // a plane is < 4 GiB
+  const int ln = threadIdx.x & (LPR - 1);
+  const int64_t col = (int64_t)J * W + VEC * ln;
+  const bool col_ok = col < Ddn;  // D_dn is a multiple of VEC
+  const int64_t plane = (int64_t)p * t.d_up * Ddn;
+  const char *__restrict__ xb = reinterpret_cast<const char *>(x + plane);
+  const uint32_t colb = (uint32_t)(col * (int64_t)sizeof(T));
+  // staging loads issued kUbStage at a time before any is stored
+  for (int eb = threadIdx.x; eb < nr * LPR; eb += kUbStage * NT) {
+    uint4 v[kUbStage];
+#pragma unroll
+    for (int j = 0; j < kUbStage; ++j) {
+      const int e = eb + j * NT;
+      const uint32_t r = (uint32_t)(e / LPR);  // e % LPR == ln
+      v[j] = (col_ok && e < nr * LPR)
+                 ? *reinterpret_cast<const uint4 *>(xb + ((uint32_t)r0 + r) * rowb + colb)
+                 : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < kUbStage; ++j)
+      if (eb + j * NT < nr * LPR) sx[eb + j * NT] = v[j];
+  }
+  for (int r = threadIdx.x; r < nr; r += NT) {
+    const int u = r0 + r;
+    smeta[r] = (uint32_t)(t.up_ptr[u] - e0) | (uint32_t)t.ub_nin[u] << 16 |
+               (uint32_t)(t.up_ptr[u + 1] - t.up_ptr[u]) << 24;
+  }
+  for (int e = threadIdx.x; e < ne; e += NT) sent[e] = t.up_c16[e0 + e];
+  __syncthreads();
+  const T mag = (T)t.up_mag;
+  T *__restrict__ yp = y + plane;
+  for (int r = threadIdx.x / LPR; r < nr; r += NT / LPR) {
+    const uint32_t m = smeta[r];
+    const int beg = (int)(m & 0xFFFFu), nin = (int)((m >> 16) & 0xFFu), ntot = (int)(m >> 24);
+    const int nout = ntot - nin;
+    T acc[VEC];
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) acc[j] = T(0);
+#pragma unroll 4
+    for (int k = 0; k < nout; ++k) {  // out-of-block hops from L2
+      const uint32_t c = sent[beg + k];
+      const T a = (c & 0x8000u) ? -mag : mag;
+      Vec<T, VEC> v;
+      v.load(reinterpret_cast<const T *>(xb + ((c & 0x7FFFu) * rowb + colb)));
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) acc[j] += a * v.v[j];
+    }
+#pragma unroll 4
+    for (int k = nout; k < ntot; ++k) {  // in-block hops from the staged rows
+      const uint32_t c = sent[beg + k];
+      const T a = (c & 0x8000u) ? -mag : mag;
+      const uint4 w = sx[((int)(c & 0x7FFFu) - r0) * LPR + ln];
+      T v[VEC];
+      __builtin_memcpy(v, &w, 16);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) acc[j] += a * v[j];
+    }
+    if (col_ok) {
+      T *dst = yp + (int64_t)(r0 + r) * Ddn + col;
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) __builtin_nontemporal_store(acc[j], dst + j);
+    }
+  }
+}
+
 // ------------------------------------------------------------ kernel 2: row
 
 // Shared prologue of the row kernels: stage x[plane row] in LDS.
@@ -2160,6 +2263,30 @@ void launch_strip(const HvTables &t, int P, const T *x, T *y, const int *flags, 
   MB_HIP(hipGetLastError());
 }
 
+// The block-staged up kernel runs where its tables exist and the strip rows
+// are whole 16-byte packs; MANYBODY_AMD_UP_KERNEL=strip keeps the per-row
+// strip kernel (the tests compare the two).
+template <typename T>
+bool use_up_block(const HvTables &t) {
+  if (!t.ub_ok || !t.up_c16_ok || t.d_dn % (int64_t)(16 / sizeof(T)) != 0) return false;
+  const char *e = std::getenv("MANYBODY_AMD_UP_KERNEL");
+  return !(e && std::strcmp(e, "strip") == 0);
+}
+
+template <typename T>
+void launch_up_block(const HvTables &t, int P, const T *x, T *y, const int *flags, hipStream_t s) {
+  constexpr int W = kUbLanes * (16 / (int)sizeof(T));
+  const int nJ = (int)((t.d_dn + W - 1) / W);
+  const int64_t S8 = ((int64_t)P * nJ + 7) / 8;
+  const int64_t grid = 8 * (int64_t)t.ub_nb * S8;
+  if (grid >= (1ll << 31)) throw std::invalid_argument("sector too large for the up kernel");
+  const size_t lds = (size_t)t.ub_rows * (kUbLanes * 16 + 4) + (size_t)t.ub_ent * 2;
+  if (lds > (size_t)kUpBlockLds) throw std::logic_error("up kernel: block tables exceed the LDS budget");
+  hipLaunchKernelGGL((up_block_kernel<T>), dim3((unsigned)grid), dim3(kUbThreads), lds, s, t, P, x, y,
+                     nJ, flags);
+  MB_HIP(hipGetLastError());
+}
+
 // Operator algebra: h = alpha z1 + beta z2 + gamma x per complex element
 // (terms with a null operand left out), then the epilogue on both planes; one
 // partial sum per workgroup (fixed-order reductions).
@@ -2320,7 +2447,9 @@ int64_t launch_hv(const HvTables &t, int P, const T *x, T *y, Epi epi, const Epi
       throw std::invalid_argument("sector too large for the two-pass H.v");
     const int *flags = epi == Epi::Lanczos ? a.flags : nullptr;
     constexpr int V = 16 / sizeof(T);
-    if (t.d_dn % V == 0)
+    if (use_up_block<T>(t))
+      launch_up_block<T>(t, P, x, y, flags, s);
+    else if (t.d_dn % V == 0)
       launch_strip<T, V>(t, P, x, y, flags, s);
     else
       launch_strip<T, 1>(t, P, x, y, flags, s);

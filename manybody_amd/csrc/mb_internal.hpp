@@ -172,6 +172,16 @@ struct SpeciesHops {
   int64_t srt_np = 0;
   std::vector<uint16_t> sperm, scomp, spn;
   DevBuf d_sperm, d_scomp, d_spn;
+  // blocks of strings (up species): strings that agree from bit blk_bits
+  // upward are a contiguous run of rows; every row's entries list the hops
+  // that leave its block first and the ones inside it after (each part in
+  // the original order); nin[r] counts the in-block ones.  blk holds
+  // (first row, rows) per block, largest first.
+  bool blk_ok = false;
+  int blk_bits = 0, blk_rows = 0, blk_ent = 0;
+  std::vector<int32_t> blk;
+  std::vector<uint8_t> nin;
+  DevBuf d_blk, d_nin;
 };
 constexpr int kSortedRound = 1024;  // slots per round (one per thread of the row kernel)
 constexpr int kSortedRounds = 12;   // rounds per row the sorted row kernel unrolls
@@ -227,7 +237,16 @@ struct HvTables {
   int up_c16_ok;
   const uint16_t *up_c16;
   double up_mag;
+  // blocks of up strings (see SpeciesHops::blk): count, largest block's rows
+  // and hop entries, (first row, rows) pairs, in-block hops per row
+  int ub_ok, ub_nb, ub_rows, ub_ent;
+  const int32_t *ub_blk;
+  const uint8_t *ub_nin;
 };
+// LDS a workgroup of the block-staged up kernel may use (six or more per CU)
+// and the bytes of a row's staged strip.
+constexpr int kUpBlockLds = 24 * 1024;
+constexpr int kUpBlockRowBytes = 256;
 HvTables tables_of(const Hubbard &h);
 
 struct PermTables {

@@ -51,12 +51,20 @@ def make_op(sizes, nu, nd, t, U):
     return model_of(sizes, t, U).operator(basis)
 
 
-# H.v paths: the two passes (strip, then row) and the one-pass row kernel that
-# forms the up hops itself (the default for sectors whose plane group fits the
-# Infinity Cache); the bit-exact tests run both
-@pytest.fixture(params=["0", "1"], ids=["two_pass", "one_pass"])
+# H.v paths: the two passes (up hops, then the row pass) with the up hops from
+# the block-staged kernel (the default where its tables exist) or from the
+# per-row strip kernel, and the one-pass row kernel that forms the up hops
+# itself (the default for sectors whose plane group fits the Infinity Cache);
+# the bit-exact tests run all three
+@pytest.fixture(params=[("0", None), ("0", "strip"), ("1", None)],
+                ids=["two_pass", "two_pass_strip", "one_pass"])
 def hv_path(request, monkeypatch):
-    monkeypatch.setenv("MANYBODY_AMD_FUSED", request.param)
+    fused, up = request.param
+    monkeypatch.setenv("MANYBODY_AMD_FUSED", fused)
+    if up is None:
+        monkeypatch.delenv("MANYBODY_AMD_UP_KERNEL", raising=False)
+    else:
+        monkeypatch.setenv("MANYBODY_AMD_UP_KERNEL", up)
     return request.param
 
 
