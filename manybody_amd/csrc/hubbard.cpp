@@ -83,6 +83,13 @@ void species_hops(const std::vector<uint32_t> &strings, const std::vector<Mono> 
   }
 }
 
+// Position of entry e (< 32) of string or slot d in a compact table: groups
+// of 64 strings, each [chunk of 8 entries][string][8], so that a wave reads
+// each chunk of its 64 strings as one contiguous KB.
+size_t comp_pos(size_t d, int e) {
+  return (d / 64) * 2048 + (size_t)(e / 8) * 512 + (d % 64) * 8 + (size_t)(e % 8);
+}
+
 // The compact table in sorted slot order with one sign pattern per wave.
 // Strings are sorted (stable) by hop count, then by positive hop count, cut
 // into groups of 64 (one wave), and the groups of each round of kSortedRound
@@ -131,7 +138,7 @@ void build_sorted(SpeciesHops &h) {
         if (d >= n) continue;
         int kp = 0, kn = P;
         for (int e = h.row_ptr[d]; e < h.row_ptr[d + 1]; ++e)
-          h.scomp[slot * 32 + (h.amp[e] < 0 ? kn++ : kp++)] = (uint16_t)h.tgt[e];
+          h.scomp[comp_pos(slot, h.amp[e] < 0 ? kn++ : kp++)] = (uint16_t)h.tgt[e];
       }
       h.spn[(j * kSortedRound + (size_t)w * 64) / 64] = (uint16_t)(P | N << 8);
     }
@@ -185,7 +192,7 @@ void pack_ell(SpeciesHops &h) {
       const int c = h.row_ptr[d + 1] - h.row_ptr[d];
       for (int k = 0; k < c; ++k) {
         const int e = h.row_ptr[d] + k;
-        h.comp[d * 32 + k] = (uint16_t)h.tgt[e];
+        h.comp[comp_pos(d, k)] = (uint16_t)h.tgt[e];
         if (h.amp[e] < 0) h.csign[d] |= 1u << k;
       }
       h.cgmax[d / 64] = std::max<uint8_t>(h.cgmax[d / 64], (uint8_t)c);

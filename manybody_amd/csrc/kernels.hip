@@ -394,8 +394,15 @@ __device__ __forceinline__ float flip_sign(float v, uint32_t sbit) {
   return __int_as_float(__float_as_int(v) ^ (int)sbit);
 }
 
+// Chunk j (entries 8 j .. 8 j + 7) of string or slot d of a compact down
+// table.  The tables are laid out per group of 64 strings as [chunk][string]
+// (16 bytes each), so the 64 lanes of a wave read one contiguous KB per chunk.
+__device__ __forceinline__ uint4 comp_chunk(const uint16_t *tab, int64_t d, int j) {
+  return reinterpret_cast<const uint4 *>(tab)[(d & ~(int64_t)63) * 4 + j * 64 + (d & 63)];
+}
+
 // Compact down table, one amplitude magnitude h: 32 uint16 target indices per
-// string (fetched as four 16-byte loads), unused slots pointing at a zero slot
+// string (fetched as up to four 16-byte loads, as many as the group needs), unused slots pointing at a zero slot
 // row[D_dn]; bit k of a per-string mask is the sign of slot k; per group of 64
 // strings (one wave) the largest hop count, so every lane runs the same number
 // of slots (rounded up to 8) with no per-lane predicate.  The 8 LDS reads of a
@@ -426,12 +433,11 @@ __global__ __launch_bounds__(1024) void row_kernel_compact(HvTables t, const T *
   for (int64_t d0 = (int64_t)threadIdx.x - lane; d0 < Ddn; d0 += blockDim.x) {
     const int64_t d = d0 + lane;  // tables are padded to a multiple of 64 strings
     const bool ok = d < Ddn;
-    const uint4 *tp = reinterpret_cast<const uint4 *>(t.dn_comp + d * K);
+    const int wm = __builtin_amdgcn_readfirstlane((int)t.dn_cgmax[d0 / kWave]);
     uint4 q[K / 8];
 #pragma unroll
-    for (int j = 0; j < K / 8; ++j) q[j] = tp[j];
+    for (int j = 0; j < K / 8; ++j) q[j] = j * 8 < wm ? comp_chunk(t.dn_comp, d, j) : make_uint4(0, 0, 0, 0);
     const uint32_t sm = t.dn_csign[d];
-    const int wm = __builtin_amdgcn_readfirstlane((int)t.dn_cgmax[d0 / kWave]);
     const T yin = ok ? y[base + d] : T(0);
     const T xv = row[ok ? d : Ddn];
     T dg = diag[popc(su & t.s_dn[ok ? d : 0])];
@@ -536,12 +542,11 @@ __global__ __launch_bounds__(1024) void row_kernel_mp(HvTables t, const T *__res
   for (int64_t d0 = (int64_t)threadIdx.x - lane; d0 < Ddn; d0 += blockDim.x) {
     const int64_t d = d0 + lane;  // tables are padded to a multiple of 64 strings
     const bool ok = d < Ddn;
-    const uint4 *tp = reinterpret_cast<const uint4 *>(t.dn_comp + d * K);
+    const int wm = __builtin_amdgcn_readfirstlane((int)t.dn_cgmax[d0 / kWave]);
     uint4 q[K / 8];
 #pragma unroll
-    for (int j = 0; j < K / 8; ++j) q[j] = tp[j];
+    for (int j = 0; j < K / 8; ++j) q[j] = j * 8 < wm ? comp_chunk(t.dn_comp, d, j) : make_uint4(0, 0, 0, 0);
     const uint32_t sm = t.dn_csign[d];
-    const int wm = __builtin_amdgcn_readfirstlane((int)t.dn_cgmax[d0 / kWave]);
     T dg = diag[popc(su & t.s_dn[ok ? d : 0])];
     if (t.has_onsite) dg = dg + on_u + Tabs<T>::dn_on(t)[ok ? d : 0];
     T h0[NP], h1[NP];
@@ -681,12 +686,11 @@ __global__ __launch_bounds__(1024) void row_kernel_fused(HvTables t, const T *__
       for (int p = 0; p < NP; ++p)
         up[p] += am * *reinterpret_cast<const T *>(xg + (int64_t)p * D * (int64_t)sizeof(T) + off);
     }
-    const uint4 *tp = reinterpret_cast<const uint4 *>(t.dn_comp + d * K);
+    const int wm = __builtin_amdgcn_readfirstlane((int)t.dn_cgmax[d0 / kWave]);
     uint4 q[K / 8];
 #pragma unroll
-    for (int j = 0; j < K / 8; ++j) q[j] = tp[j];
+    for (int j = 0; j < K / 8; ++j) q[j] = j * 8 < wm ? comp_chunk(t.dn_comp, d, j) : make_uint4(0, 0, 0, 0);
     const uint32_t sm = t.dn_csign[d];
-    const int wm = __builtin_amdgcn_readfirstlane((int)t.dn_cgmax[d0 / kWave]);
     T dg = diag[popc(su & t.s_dn[ok ? d : 0])];
     if (t.has_onsite) dg = dg + on_u + Tabs<T>::dn_on(t)[ok ? d : 0];
     T h0[NP], h1[NP];
@@ -850,7 +854,7 @@ __global__ __launch_bounds__(1024) void row_kernel_sorted(HvTables t, const T *_
       if (j < nr) {
         // uniform bases of this round's tables; lanes index them with tid
         const uint16_t *__restrict__ perm = perm_b + j * NT;
-        const uint4 *__restrict__ comp = reinterpret_cast<const uint4 *>(comp_b + (int64_t)j * NT * K);
+        const uint16_t *__restrict__ comp = comp_b + (int64_t)j * NT * K;
         const int d = perm[tid];
         dd[j / 2] = (j & 1) ? ((dd[j / 2] & 0xFFFFu) | ((uint32_t)d << 16))
                             : ((dd[j / 2] & 0xFFFF0000u) | (uint32_t)d);
@@ -860,7 +864,7 @@ __global__ __launch_bounds__(1024) void row_kernel_sorted(HvTables t, const T *_
         uint4 q[K / 8];
 #pragma unroll
         for (int k = 0; k < K / 8; ++k)
-          q[k] = k * 8 < nt ? comp[tid * (K / 8) + k] : make_uint4(0, 0, 0, 0);
+          q[k] = k * 8 < nt ? comp_chunk(comp, tid, k) : make_uint4(0, 0, 0, 0);
         const int dc = d < Ddn ? d : 0;
         const uint32_t sd = t.s_dn[dc];
         T h0 = T(0), h1 = T(0);

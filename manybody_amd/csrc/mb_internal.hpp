@@ -155,8 +155,9 @@ struct SpeciesHops {
   std::vector<uint8_t> cnt;
   std::vector<double> mags;
   DevBuf d_ell, d_cnt, d_mags, d_magsf;
-  // compact copy (single magnitude): padded target indices, sign masks, and the
-  // largest hop count per group of 64 strings
+  // compact copy (single magnitude): padded target indices (per group of 64
+  // strings [chunk of 8 entries][string][8]), sign masks, and the largest hop
+  // count per group of 64 strings
   bool comp_ok = false;
   std::vector<uint16_t> comp;
   std::vector<uint32_t> csign;
@@ -166,8 +167,9 @@ struct SpeciesHops {
   // wave: strings sorted by (hop count, positive hop count) share a wave of 64
   // slots; per group P = most positive and N = most negative hops, and every
   // slot lists its positive targets in [0, P) and negative ones in [P, P + N),
-  // padded with the zero slot n.  slot -> string, targets, per-group P | N << 8;
-  // padded to rounds of kSortedRound slots, pad slots -> string n
+  // padded with the zero slot n.  slot -> string, targets (laid out as the
+  // compact copy, slots in place of strings), per-group P | N << 8; padded to
+  // rounds of kSortedRound slots, pad slots -> string n
   bool srt_ok = false;
   int64_t srt_np = 0;
   std::vector<uint16_t> sperm, scomp, spn;
@@ -221,7 +223,8 @@ struct HvTables {
   const double *mags;
   const float *magsf;
   // compact down table, one magnitude: 32 uint16 target indices per string
-  // (unused slots = d_dn, a zero slot), per-string sign masks, per-64-string
+  // (unused slots = d_dn, a zero slot) stored per group of 64 strings as
+  // [chunk of 8 entries][string], per-string sign masks, per-64-string
   // largest hop count; padded to a multiple of 64 strings
   int comp_ok;
   const uint16_t *dn_comp;
