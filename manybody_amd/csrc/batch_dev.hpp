@@ -115,14 +115,13 @@ __device__ __forceinline__ double hv_rows_tile(const BInst &I, const BTile &tl,
     const int jj = ok ? j : j0;
     const int r = jj / Ddn, d = jj - r * Ddn;
     const int64_t u = u0 + r, e = tl.e0 + jj;
-    const uint4 *tp = reinterpret_cast<const uint4 *>(t.dn_comp + (int64_t)d * K);
-    uint4 q[K / 8];
-#pragma unroll
-    for (int c = 0; c < K / 8; ++c) q[c] = tp[c];
     const uint32_t sm = t.dn_csign[d];
     int wm = t.dn_cnt[d];
     for (int off = kWave / 2; off > 0; off >>= 1) wm = max(wm, __shfl_xor(wm, off, kWave));
     wm = __builtin_amdgcn_readfirstlane(wm);
+    uint4 q[K / 8];
+#pragma unroll
+    for (int c = 0; c < K / 8; ++c) q[c] = c * 8 < wm ? comp_chunk(t.dn_comp, d, c) : make_uint4(0, 0, 0, 0);
     double dg = t.diagU[popc(t.s_up[u] & t.s_dn[d])];
     if (t.has_onsite) dg = dg + t.up_on[u] + t.dn_on[d];
     // up hops: rows of x from global memory

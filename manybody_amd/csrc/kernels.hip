@@ -394,13 +394,6 @@ __device__ __forceinline__ float flip_sign(float v, uint32_t sbit) {
   return __int_as_float(__float_as_int(v) ^ (int)sbit);
 }
 
-// Chunk j (entries 8 j .. 8 j + 7) of string or slot d of a compact down
-// table.  The tables are laid out per group of 64 strings as [chunk][string]
-// (16 bytes each), so the 64 lanes of a wave read one contiguous KB per chunk.
-__device__ __forceinline__ uint4 comp_chunk(const uint16_t *tab, int64_t d, int j) {
-  return reinterpret_cast<const uint4 *>(tab)[(d & ~(int64_t)63) * 4 + j * 64 + (d & 63)];
-}
-
 // Compact down table, one amplitude magnitude h: 32 uint16 target indices per
 // string (fetched as up to four 16-byte loads, as many as the group needs), unused slots pointing at a zero slot
 // row[D_dn]; bit k of a per-string mask is the sign of slot k; per group of 64

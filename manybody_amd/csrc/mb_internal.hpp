@@ -246,6 +246,12 @@ struct HvTables {
   const int32_t *ub_blk;
   const uint8_t *ub_nin;
 };
+// Chunk j (entries 8 j .. 8 j + 7) of string or slot d of a compact down
+// table.  The tables are laid out per group of 64 strings as [chunk][string]
+// (16 bytes each), so the 64 lanes of a wave read one contiguous KB per chunk.
+__device__ __forceinline__ uint4 comp_chunk(const uint16_t *tab, int64_t d, int j) {
+  return reinterpret_cast<const uint4 *>(tab)[(d & ~(int64_t)63) * 4 + j * 64 + (d & 63)];
+}
 // LDS a workgroup of the block-staged up kernel may use (six or more per CU)
 // and the bytes of a row's staged strip.
 constexpr int kUpBlockLds = 24 * 1024;
