@@ -310,7 +310,7 @@ def _pmc_per_kernel(folder, counter):
 
 
 def write_traffic(hv_fetch, hv_write, copy_fetch, copy_write):
-    """HBM bytes per H.v launch pair (strip kernel + row kernel) from FETCH_SIZE
+    """HBM bytes per H.v launch pair (up kernel + row kernel) from FETCH_SIZE
     and WRITE_SIZE, each scaled by the factor that makes the same counter read
     the known byte count of a plain copy of one vector."""
     known = 16.0 * D_4X4_7P7  # bytes read (and written) by the calibration copy
@@ -326,7 +326,8 @@ def write_traffic(hv_fetch, hv_write, copy_fetch, copy_write):
     write = _pmc_per_kernel(hv_write, "WRITE_SIZE")
     kernels = {}
     for name in fetch:
-        for short in ("up_strip_kernel", "row_kernel_sorted", "row_kernel_compact", "row_kernel_ell"):
+        for short in ("up_block_kernel", "up_strip_kernel", "row_kernel_sorted", "row_kernel_compact",
+                      "row_kernel_ell"):
             if short + "<" in name:
                 kernels[short] = {"fetch_bytes": round(fetch[name] * 1024.0 * ff),
                                   "write_bytes": round(write.get(name, 0.0) * 1024.0 * wf)}
