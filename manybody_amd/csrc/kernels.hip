@@ -256,109 +256,6 @@ __global__ __launch_bounds__(kBlock) void up_strip_kernel(HvTables t, int P, con
   }
 }
 
-// Block-staged up hops.  Up strings that agree from bit k upward are a
-// contiguous run of rows (a block; the strings are in colex order), and a hop
-// between two sites below k stays inside its block.  One workgroup takes one
-// block and one column strip (LPR lanes x 16 bytes per row): it stages the
-// strip of the block's rows and the block's hop entries in LDS, then each row
-// sums the hops that leave the block from L2, where the strip of every row
-// stays as for up_strip_kernel (the blocks of a strip share an XCD), and the
-// in-block hops from LDS, in the order of the row's entries (out-of-block ones
-// first).  Small blocks and many workgroups per CU measured best: on the 4 x 4,
-// 7 + 7 sector k = 8 (blocks of <= 70 rows, 12 of 32 bonds inside, ~21 KB of
-// LDS, 7 workgroups per CU) gives 1.65 ms against 2.17 ms for up_strip_kernel;
-// larger blocks keep more hops inside but leave fewer workgroups per CU
-// (k = 9: 2.0 ms, k = 10: 2.1 ms), and 128-byte strips or 512 threads were
-// slower at every k.
-constexpr int kUbStage = 8;     // staging loads per thread in flight
-constexpr int kUbLanes = 16;    // lanes per row, 16 bytes each: 256-byte strips
-constexpr int kUbThreads = 256;
-static_assert(kUbLanes * 16 == kUpBlockRowBytes, "the host sizes blocks for this strip width");
-
-template <typename T, int LPR = kUbLanes, int NT = kUbThreads>
-__global__ __launch_bounds__(NT) void up_block_kernel(HvTables t, int P, const T *__restrict__ x,
-                                                      T *__restrict__ y, int nJ, const int *flags) {
-  constexpr int VEC = 16 / sizeof(T), W = LPR * VEC;
-  extern __shared__ uint4 ub_smem[];
-  if (flags && flags[0]) return;
-  const int nb = t.ub_nb;
-  const int i = blockIdx.x;
-  const int xcd = i & 7, q = i >> 3;
-  const int s = xcd + 8 * (q / nb);
-  const int b = q % nb;
-  if (s >= P * nJ) return;
-  const int p = s / nJ, J = s - p * nJ;
-  const int r0 = t.ub_blk[2 * b], nr = t.ub_blk[2 * b + 1];
-  uint4 *sx = ub_smem;  // row r of the block at sx[r * LPR + lane]
-  uint32_t *smeta = reinterpret_cast<uint32_t *>(ub_smem + t.ub_rows * LPR);
-  uint16_t *sent = reinterpret_cast<uint16_t *>(smeta + t.ub_rows);
-  const int e0 = t.up_ptr[r0], ne = t.up_ptr[r0 + nr] - e0;
-  const int64_t Ddn = t.d_dn;
-  const uint32_t rowb = (uint32_t)(Ddn * (int64_t)sizeof(T));  // a plane is < 4 GiB
-  const int ln = threadIdx.x & (LPR - 1);
-  const int64_t col = (int64_t)J * W + VEC * ln;
-  const bool col_ok = col < Ddn;  // D_dn is a multiple of VEC
-  const int64_t plane = (int64_t)p * t.d_up * Ddn;
-  const char *__restrict__ xb = reinterpret_cast<const char *>(x + plane);
-  const uint32_t colb = (uint32_t)(col * (int64_t)sizeof(T));
-  // staging loads issued kUbStage at a time before any is stored
-  for (int eb = threadIdx.x; eb < nr * LPR; eb += kUbStage * NT) {
-    uint4 v[kUbStage];
-#pragma unroll
-    for (int j = 0; j < kUbStage; ++j) {
-      const int e = eb + j * NT;
-      const uint32_t r = (uint32_t)(e / LPR);  // e % LPR == ln
-      v[j] = (col_ok && e < nr * LPR)
-                 ? *reinterpret_cast<const uint4 *>(xb + ((uint32_t)r0 + r) * rowb + colb)
-                 : make_uint4(0, 0, 0, 0);
-    }
-#pragma unroll
-    for (int j = 0; j < kUbStage; ++j)
-      if (eb + j * NT < nr * LPR) sx[eb + j * NT] = v[j];
-  }
-  for (int r = threadIdx.x; r < nr; r += NT) {
-    const int u = r0 + r;
-    smeta[r] = (uint32_t)(t.up_ptr[u] - e0) | (uint32_t)t.ub_nin[u] << 16 |
-               (uint32_t)(t.up_ptr[u + 1] - t.up_ptr[u]) << 24;
-  }
-  for (int e = threadIdx.x; e < ne; e += NT) sent[e] = t.up_c16[e0 + e];
-  __syncthreads();
-  const T mag = (T)t.up_mag;
-  T *__restrict__ yp = y + plane;
-  for (int r = threadIdx.x / LPR; r < nr; r += NT / LPR) {
-    const uint32_t m = smeta[r];
-    const int beg = (int)(m & 0xFFFFu), nin = (int)((m >> 16) & 0xFFu), ntot = (int)(m >> 24);
-    const int nout = ntot - nin;
-    T acc[VEC];
-#pragma unroll
-    for (int j = 0; j < VEC; ++j) acc[j] = T(0);
-#pragma unroll 4
-    for (int k = 0; k < nout; ++k) {  // out-of-block hops from L2
-      const uint32_t c = sent[beg + k];
-      const T a = (c & 0x8000u) ? -mag : mag;
-      Vec<T, VEC> v;
-      v.load(reinterpret_cast<const T *>(xb + ((c & 0x7FFFu) * rowb + colb)));
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) acc[j] += a * v.v[j];
-    }
-#pragma unroll 4
-    for (int k = nout; k < ntot; ++k) {  // in-block hops from the staged rows
-      const uint32_t c = sent[beg + k];
-      const T a = (c & 0x8000u) ? -mag : mag;
-      const uint4 w = sx[((int)(c & 0x7FFFu) - r0) * LPR + ln];
-      T v[VEC];
-      __builtin_memcpy(v, &w, 16);
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) acc[j] += a * v[j];
-    }
-    if (col_ok) {
-      T *dst = yp + (int64_t)(r0 + r) * Ddn + col;
-#pragma unroll
-      for (int j = 0; j < VEC; ++j) __builtin_nontemporal_store(acc[j], dst + j);
-    }
-  }
-}
-
 // ------------------------------------------------------------ kernel 2: row
 
 // Shared prologue of the row kernels: stage x[plane row] in LDS.
@@ -772,11 +669,183 @@ __device__ __forceinline__ void epilogue_pv(T r, T xv, T pv, int64_t i, T *__res
   }
 }
 
+// Coefficient of the previous Lanczos step's axpy (x = w - c v), as
+// lanczos_axpy_norm_kernel forms it for unnormalised vectors.
+template <typename T>
+__device__ __forceinline__ T fold_coef(const EpiArgs &a) {
+  const int st = a.fold_step;
+  return st > 0 ? (T)(a.alpha_dev[st] / a.beta_dev[st - 1]) : (T)a.alpha_dev[st];
+}
+
+// x[i] = w[i] - c v[i] (the axpy's arithmetic), both read once: non-temporal
+// loads (2.62 against 2.68 ms for the 4 x 4, 7 + 7 row pass).
+template <typename T>
+__device__ __forceinline__ T fold_val(const T *w, const T *v, T c, int64_t i) {
+  return __builtin_nontemporal_load(w + i) - c * __builtin_nontemporal_load(v + i);
+}
+
 template <typename T, Epi E>
 __device__ __forceinline__ bool reads_prev(const EpiArgs &a, const EpiK<T> &bp) {
   if constexpr (E == Epi::Lanczos || E == Epi::Recon) return bp.bp != T(0);
   if constexpr (E == Epi::Cheb) return !a.first;
   return false;
+}
+
+// Block-staged up hops.  Up strings that agree from bit k upward are a
+// contiguous run of rows (a block; the strings are in colex order), and a hop
+// between two sites below k stays inside its block.  One workgroup takes one
+// block and one column strip (LPR lanes x 16 bytes per row): it stages the
+// strip of the block's rows and the block's hop entries in LDS, then each row
+// sums the hops that leave the block from L2, where the strip of every row
+// stays as for up_strip_kernel (the blocks of a strip share an XCD), and the
+// in-block hops from LDS, in the order of the row's entries (out-of-block ones
+// first).  Small blocks and many workgroups per CU measured best: on the 4 x 4,
+// 7 + 7 sector k = 8 (blocks of <= 70 rows, 12 of 32 bonds inside, ~21 KB of
+// LDS, 7 workgroups per CU) gives 1.65 ms against 2.17 ms for up_strip_kernel;
+// larger blocks keep more hops inside but leave fewer workgroups per CU
+// (k = 9: 2.0 ms, k = 10: 2.1 ms), and 128-byte strips or 512 threads were
+// slower at every k.
+constexpr int kUbStage = 8;     // staging loads per thread in flight
+constexpr int kUbLanes = 16;    // lanes per row, 16 bytes each: 256-byte strips
+constexpr int kUbThreads = 256;
+static_assert(kUbLanes * 16 == kUpBlockRowBytes, "the host sizes blocks for this strip width");
+
+// LZ: the up pass of a row-first Lanczos step: y = Lanczos epilogue of
+// (up hops) x + z, z the row pass's (down hops + diagonal) x, with one partial
+// sum per workgroup in a.partial (the string's own value comes from the
+// staged rows, since a block's output rows are its staged rows).
+template <typename T, int LPR = kUbLanes, int NT = kUbThreads, bool LZ = false>
+__global__ __launch_bounds__(NT) void up_block_kernel(HvTables t, int P, const T *__restrict__ x,
+                                                      T *__restrict__ y, int nJ, const int *flags,
+                                                      const T *__restrict__ z, EpiArgs a) {
+  constexpr int VEC = 16 / sizeof(T), W = LPR * VEC;
+  extern __shared__ uint4 ub_smem[];
+  __shared__ double red[LZ ? NT / kWave : 1];
+  if (flags && flags[0]) return;
+  const int nb = t.ub_nb;
+  const int i = blockIdx.x;
+  const int xcd = i & 7, q = i >> 3;
+  const int s = xcd + 8 * (q / nb);
+  const int b = q % nb;
+  if (s >= P * nJ) {
+    if constexpr (LZ) {
+      if (threadIdx.x == 0) a.partial[i] = 0.0;  // a pad workgroup of the XCD map
+    }
+    return;
+  }
+  const int p = s / nJ, J = s - p * nJ;
+  const int r0 = t.ub_blk[2 * b], nr = t.ub_blk[2 * b + 1];
+  uint4 *sx = ub_smem;  // row r of the block at sx[r * LPR + lane]
+  uint32_t *smeta = reinterpret_cast<uint32_t *>(ub_smem + t.ub_rows * LPR);
+  uint16_t *sent = reinterpret_cast<uint16_t *>(smeta + t.ub_rows);
+  const int e0 = t.up_ptr[r0], ne = t.up_ptr[r0 + nr] - e0;
+  const int64_t Ddn = t.d_dn;
+  const uint32_t rowb = (uint32_t)(Ddn * (int64_t)sizeof(T));  // a plane is < 4 GiB
+  const int ln = threadIdx.x & (LPR - 1);
+  const int64_t col = (int64_t)J * W + VEC * ln;
+  const bool col_ok = col < Ddn;  // D_dn is a multiple of VEC
+  const int64_t plane = (int64_t)p * t.d_up * Ddn;
+  const char *__restrict__ xb = reinterpret_cast<const char *>(x + plane);
+  const uint32_t colb = (uint32_t)(col * (int64_t)sizeof(T));
+  // staging loads issued kUbStage at a time before any is stored
+  for (int eb = threadIdx.x; eb < nr * LPR; eb += kUbStage * NT) {
+    uint4 v[kUbStage];
+#pragma unroll
+    for (int j = 0; j < kUbStage; ++j) {
+      const int e = eb + j * NT;
+      const uint32_t r = (uint32_t)(e / LPR);  // e % LPR == ln
+      v[j] = (col_ok && e < nr * LPR)
+                 ? *reinterpret_cast<const uint4 *>(xb + ((uint32_t)r0 + r) * rowb + colb)
+                 : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < kUbStage; ++j)
+      if (eb + j * NT < nr * LPR) sx[eb + j * NT] = v[j];
+  }
+  for (int r = threadIdx.x; r < nr; r += NT) {
+    const int u = r0 + r;
+    smeta[r] = (uint32_t)(t.up_ptr[u] - e0) | (uint32_t)t.ub_nin[u] << 16 |
+               (uint32_t)(t.up_ptr[u + 1] - t.up_ptr[u]) << 24;
+  }
+  for (int e = threadIdx.x; e < ne; e += NT) sent[e] = t.up_c16[e0 + e];
+  __syncthreads();
+  const T mag = (T)t.up_mag;
+  T *__restrict__ yp = y + plane;
+  EpiK<T> bk;
+  bool rp = false;
+  if constexpr (LZ) {
+    bk = epi_consts<T, Epi::Lanczos>(a);
+    rp = reads_prev<T, Epi::Lanczos>(a, bk);
+  }
+  Partial2 part;
+  for (int r = threadIdx.x / LPR; r < nr; r += NT / LPR) {
+    const uint32_t m = smeta[r];
+    const int beg = (int)(m & 0xFFFFu), nin = (int)((m >> 16) & 0xFFu), ntot = (int)(m >> 24);
+    const int nout = ntot - nin;
+    T acc[VEC];
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) acc[j] = T(0);
+#pragma unroll 4
+    for (int k = 0; k < nout; ++k) {  // out-of-block hops from L2
+      const uint32_t c = sent[beg + k];
+      const T a = (c & 0x8000u) ? -mag : mag;
+      Vec<T, VEC> v;
+      v.load(reinterpret_cast<const T *>(xb + ((c & 0x7FFFu) * rowb + colb)));
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) acc[j] += a * v.v[j];
+    }
+#pragma unroll 4
+    for (int k = nout; k < ntot; ++k) {  // in-block hops from the staged rows
+      const uint32_t c = sent[beg + k];
+      const T a = (c & 0x8000u) ? -mag : mag;
+      const uint4 w = sx[((int)(c & 0x7FFFu) - r0) * LPR + ln];
+      T v[VEC];
+      __builtin_memcpy(v, &w, 16);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) acc[j] += a * v[j];
+    }
+    if constexpr (LZ) {
+      if (col_ok) {
+        // w is written once: a non-temporal store, so the stream does not
+        // evict the strip of x from L2 (non-temporal loads of z and prev were
+        // slower: 2.70 against 2.57 ms)
+        const int64_t e = plane + (int64_t)(r0 + r) * Ddn + col;
+        T zv[VEC], pv[VEC];
+        {
+          Vec<T, VEC> zq, pq;
+          zq.load(z + e);
+          if (rp) pq.load(static_cast<const T *>(a.prev) + e);
+#pragma unroll
+          for (int j = 0; j < VEC; ++j) {
+            zv[j] = zq.v[j];
+            pv[j] = rp ? pq.v[j] : T(0);
+          }
+        }
+        const uint4 xw = sx[r * LPR + ln];
+        T xv[VEC];
+        __builtin_memcpy(xv, &xw, 16);
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) {  // the Lanczos epilogue (epilogue_pv) with a streaming store
+          T rr = acc[j] + zv[j], xx = xv[j];
+          if (a.lazy) {
+            rr *= bk.sx;
+            xx *= bk.sx;
+          }
+          if (bk.bp != T(0)) rr -= bk.bp * pv[j];
+          __builtin_nontemporal_store(rr, y + e + j);
+          part.s0 += (double)xx * (double)rr;
+        }
+      }
+    } else if (col_ok) {
+      T *dst = yp + (int64_t)(r0 + r) * Ddn + col;
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) __builtin_nontemporal_store(acc[j], dst + j);
+    }
+  }
+  if constexpr (LZ) {
+    const double tot = wg_sum(part.s0, red);
+    if (threadIdx.x == 0) a.partial[i] = tot;
+  }
 }
 
 // Persistent row kernel over the sorted compact table.  One workgroup of 1024
@@ -808,16 +877,38 @@ __global__ __launch_bounds__(1024) void row_kernel_sorted(HvTables t, const T *_
   if constexpr (E == Epi::Lanczos) {
     if (a.flags[0]) return;
   }
+  if constexpr (E == Epi::Down) {
+    if (a.flags && a.flags[0]) return;
+  }
   int64_t w = blockIdx.x;
   if (w >= nrows) return;
   const int tid = threadIdx.x;
   const int nr = (int)(t.srt_np / NT);  // slot rounds (<= R)
   if (tid < t.n_diag) s_diag[tid] = Tabs<T>::diag(t)[tid];
+  // Down with a fold: rows are staged as x = fold_w - c fold_v (the previous
+  // Lanczos step's axpy, same arithmetic) and written out; ss = |x row|^2
+  const bool fold = E == Epi::Down && a.fold;
+  const T fc = fold ? fold_coef<T>(a) : T(0);
+  const T *__restrict__ fw = static_cast<const T *>(a.fold_w);
+  const T *__restrict__ fv = static_cast<const T *>(a.fold_v);
+  T *__restrict__ fx = static_cast<T *>(a.fold_x);
+  double ss = 0.0;
   {
     const T *__restrict__ xr = x + w * Ddn;  // uniform row base; lanes index it with tid
     T v[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = j * NT + tid < Ddn ? xr[j * NT + tid] : T(0);
+    for (int j = 0; j < R; ++j) {
+      const int64_t i = w * Ddn + j * NT + tid;
+      v[j] = j * NT + tid < Ddn ? (fold ? fold_val(fw, fv, fc, i) : xr[j * NT + tid]) : T(0);
+    }
+    if (fold) {
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (j * NT + tid < Ddn) {
+          __builtin_nontemporal_store(v[j], fx + w * Ddn + j * NT + tid);
+          ss += (double)v[j] * (double)v[j];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < R; ++j)
       if (j * NT + tid < Ddn) row[j * NT + tid] = v[j];
@@ -890,9 +981,13 @@ __global__ __launch_bounds__(1024) void row_kernel_sorted(HvTables t, const T *_
     // order, so loads in flight during A would hold up its table loads.
     T pre[R];
     {
-      const T *__restrict__ xn = x + (nx < nrows ? nx : w) * Ddn;
+      const int64_t rn = nx < nrows ? nx : w;
+      const T *__restrict__ xn = x + rn * Ddn;
 #pragma unroll
-      for (int j = 0; j < R; ++j) pre[j] = j * NT + tid < Ddn ? xn[j * NT + tid] : T(0);
+      for (int j = 0; j < R; ++j) {
+        const int64_t i = rn * Ddn + j * NT + tid;
+        pre[j] = j * NT + tid < Ddn ? (fold ? fold_val(fw, fv, fc, i) : xn[j * NT + tid]) : T(0);
+      }
     }
     T *__restrict__ yr = y + w * Ddn;
     const T *__restrict__ pr = rp ? static_cast<const T *>(a.prev) + w * Ddn : nullptr;
@@ -904,7 +999,7 @@ __global__ __launch_bounds__(1024) void row_kernel_sorted(HvTables t, const T *_
 #pragma unroll
       for (int j = 0; j < H; ++j) {
         const int d = (hf * H + j) * NT + tid;
-        yin[j] = d < Ddn ? yr[d] : T(0);
+        yin[j] = (E != Epi::Down && d < Ddn) ? yr[d] : T(0);
         pv[j] = (rp && d < Ddn) ? pr[d] : T(0);
       }
       // B: results of strings [lo, lo + kSortedHalf) into the LDS buffer
@@ -918,12 +1013,29 @@ __global__ __launch_bounds__(1024) void row_kernel_sorted(HvTables t, const T *_
 #pragma unroll
       for (int j = 0; j < H; ++j) {
         const int d = (hf * H + j) * NT + tid;
-        if (d < Ddn)
-          epilogue_pv<T, E>(yin[j] + scr[d - lo], row[d], pv[j], w * Ddn + d, y, a, bp, acc2);
+        if (d < Ddn) {
+          if constexpr (E == Epi::Down)
+            __builtin_nontemporal_store(scr[d - lo], yr + d);
+          else
+            epilogue_pv<T, E>(yin[j] + scr[d - lo], row[d], pv[j], w * Ddn + d, y, a, bp, acc2);
+        }
       }
       __syncthreads();
     }
     row_reduce<T, E>(acc2, a, w, red);
+    if (fold) {  // |x row|^2 of this row; the next row's staged values are in pre
+      const double r = wg_sum(ss, red[0]);
+      if (tid == 0) a.fold_partial[w] = r;
+      ss = 0.0;
+      if (nx < nrows) {
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+          if (j * NT + tid < Ddn) {
+            __builtin_nontemporal_store(pre[j], fx + nx * Ddn + j * NT + tid);
+            ss += (double)pre[j] * (double)pre[j];
+          }
+      }
+    }
     // D: next row
     if (nx < nrows) {
 #pragma unroll
@@ -1154,6 +1266,18 @@ __global__ __launch_bounds__(kBlock) void fill_phase_ref_kernel(const int64_t *_
 }
 
 // out0 = sum of partial[2 w], out1 = sum of partial[2 w + 1], w < n (fixed order)
+// out[b] = the sum of in[b c, (b + 1) c), c = ceil(n / gridDim.x), in a fixed order.
+__global__ __launch_bounds__(kBlock) void chunk_sum_kernel(const double *__restrict__ in, int64_t n,
+                                                           double *__restrict__ out) {
+  __shared__ double lds[kBlock / kWave];
+  const int64_t c = (n + gridDim.x - 1) / gridDim.x, lo = (int64_t)blockIdx.x * c;
+  const int64_t hi = lo + c < n ? lo + c : n;
+  double s = 0.0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) s += in[i];
+  s = wg_sum(s, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
 __global__ __launch_bounds__(kBlock) void reduce_pairs_kernel(const double *__restrict__ partial,
                                                               int64_t n, double *out0,
                                                               double *out1) {
@@ -1257,10 +1381,11 @@ __global__ __launch_bounds__(kBlock) void lanczos_axpy_norm_kernel(const T *__re
   if (threadIdx.x == 0) st.partial[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kBlock) void lanczos_finish_beta_kernel(LanczosState st, int step) {
+__global__ __launch_bounds__(kBlock) void lanczos_finish_beta_kernel(LanczosState st, int step,
+                                                                     int64_t nparts) {
   __shared__ double lds[kBlock / kWave];
   if (st.flags[0]) return;
-  const double s = final_sum(st.partial, kRedBlocks, lds);
+  const double s = final_sum(st.partial, nparts, lds);
   if (threadIdx.x == 0) {
     const double b = sqrt(s);
     st.beta[step] = b;
@@ -2198,6 +2323,16 @@ void launch_mp(dim3 g, dim3 b, size_t lds, hipStream_t s, const HvTables &t, con
   }
 }
 
+// The sorted row kernel: slots of at most kSortedRounds rounds, 2 halves of
+// kSortedHalf strings; for rows below 8192 strings the per-row kernel, several
+// workgroups per CU, is faster.
+template <typename T>
+bool use_sorted_row(const HvTables &t) {
+  return t.srt_ok && t.d_dn >= 8192 && t.srt_np <= (int64_t)kSortedRound * kSortedRounds &&
+         t.d_dn <= 2 * (int64_t)kSortedHalf && sorted_row_lds<T>(t) <= (size_t)kMaxRowLds &&
+         !force_compact_row();
+}
+
 template <typename T, Epi E>
 void launch_row(const HvTables &t, int P, const T *x, T *y, const EpiArgs &a, hipStream_t s) {
   const size_t lds = (size_t)(t.d_dn + 1) * sizeof(T);  // + the zero slot of the compact table
@@ -2212,11 +2347,7 @@ void launch_row(const HvTables &t, int P, const T *x, T *y, const EpiArgs &a, hi
     attr_set = true;
   }
   const int64_t nrows = (int64_t)P * t.d_up;
-  // the sorted kernel: slots of at most kSortedRounds rounds, 2 halves of kSortedHalf strings;
-  // for rows below 8192 strings the per-row kernel, several workgroups per CU, is faster
-  if (t.srt_ok && t.d_dn >= 8192 && t.srt_np <= (int64_t)kSortedRound * kSortedRounds &&
-      t.d_dn <= 2 * (int64_t)kSortedHalf && sorted_row_lds<T>(t) <= (size_t)kMaxRowLds &&
-      !force_compact_row()) {
+  if (use_sorted_row<T>(t)) {
     const int64_t cus = device_cus();
     const dim3 g((unsigned)(nrows < cus ? nrows : cus)), b(kSortedRound);
     hipLaunchKernelGGL((row_kernel_sorted<T, E>), g, b, sorted_row_lds<T>(t), s, t, x, y, a, nrows);
@@ -2280,7 +2411,7 @@ void launch_up_block(const HvTables &t, int P, const T *x, T *y, const int *flag
   const size_t lds = (size_t)t.ub_rows * (kUbLanes * 16 + 4) + (size_t)t.ub_ent * 2;
   if (lds > (size_t)kUpBlockLds) throw std::logic_error("up kernel: block tables exceed the LDS budget");
   hipLaunchKernelGGL((up_block_kernel<T>), dim3((unsigned)grid), dim3(kUbThreads), lds, s, t, P, x, y,
-                     nJ, flags);
+                     nJ, flags, nullptr, EpiArgs());
   MB_HIP(hipGetLastError());
 }
 
@@ -2467,6 +2598,61 @@ int64_t launch_hv(const HvTables &t, int P, const T *x, T *y, Epi epi, const Epi
   }
   MB_HIP(hipGetLastError());
   return g.x;
+}
+
+// The row-first Lanczos step (see mb_internal.hpp): FP64 two-pass sectors
+// whose up hops run in the block-staged kernel and whose rows run in the
+// sorted row kernel.  MANYBODY_AMD_LANCZOS_FOLD=0 keeps the up-then-row H.v
+// with a separate axpy pass.
+bool lanczos_fold_ok(const HvTables &t) {
+  const char *e = std::getenv("MANYBODY_AMD_LANCZOS_FOLD");
+  if (e && std::strcmp(e, "0") == 0) return false;
+  return use_two_pass<double>(t) && 2 * t.d_up < (1ll << 31) && fused_planes<double>(t, 2) == 0 &&
+         use_up_block<double>(t) && use_sorted_row<double>(t);
+}
+
+int64_t up_block_grid(const HvTables &t, int P, int *nJ_out) {
+  constexpr int W = kUbLanes * 2;  // FP64: 16 lanes x 2 doubles
+  const int nJ = (int)((t.d_dn + W - 1) / W);
+  if (nJ_out) *nJ_out = nJ;
+  return 8 * (int64_t)t.ub_nb * (((int64_t)P * nJ + 7) / 8);
+}
+
+int64_t lanczos_fold_partials(const HvTables &t) {
+  return std::max<int64_t>(2 * t.d_up, kRedBlocks + up_block_grid(t, 2, nullptr));
+}
+
+void launch_row_down(const HvTables &t, const double *x, double *z, const EpiArgs &a, hipStream_t s) {
+  static std::atomic<bool> attr_set{false};
+  if (!attr_set) {
+    MB_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&row_kernel_sorted<double, Epi::Down>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, kMaxRowLds));
+    attr_set = true;
+  }
+  const int64_t nrows = 2 * t.d_up, cus = device_cus();
+  const dim3 g((unsigned)(nrows < cus ? nrows : cus)), b(kSortedRound);
+  hipLaunchKernelGGL((row_kernel_sorted<double, Epi::Down>), g, b, sorted_row_lds<double>(t), s, t, x, z,
+                     a, nrows);
+  MB_HIP(hipGetLastError());
+}
+
+int64_t launch_up_lanczos(const HvTables &t, const double *x, const double *z, double *w,
+                          const EpiArgs &a, hipStream_t s) {
+  int nJ = 0;
+  const int64_t grid = up_block_grid(t, 2, &nJ);
+  if (grid >= (1ll << 31)) throw std::invalid_argument("sector too large for the up kernel");
+  const size_t lds = (size_t)t.ub_rows * (kUbLanes * 16 + 4) + (size_t)t.ub_ent * 2;
+  // one partial sum per workgroup after kRedBlocks slots, then summed in
+  // chunks into the first kRedBlocks slots (one workgroup would take ~0.3 ms
+  // over the ~180 K partial sums of the 4 x 4, 7 + 7 sector)
+  EpiArgs ua = a;
+  ua.partial = a.partial + kRedBlocks;
+  hipLaunchKernelGGL((up_block_kernel<double, kUbLanes, kUbThreads, true>), dim3((unsigned)grid),
+                     dim3(kUbThreads), lds, s, t, 2, x, w, nJ, a.flags, z, ua);
+  MB_HIP(hipGetLastError());
+  hipLaunchKernelGGL(chunk_sum_kernel, dim3(kRedBlocks), dim3(kBlock), 0, s, ua.partial, grid, a.partial);
+  MB_HIP(hipGetLastError());
+  return kRedBlocks;
 }
 
 template <Epi E, int CPT>
@@ -2756,7 +2942,12 @@ template void launch_lanczos_axpy_norm<float>(const float *, float *, int64_t, c
                                               int, hipStream_t, bool);
 
 void launch_lanczos_finish_beta(const LanczosState &st, int step, hipStream_t s) {
-  hipLaunchKernelGGL(lanczos_finish_beta_kernel, dim3(1), dim3(kBlock), 0, s, st, step);
+  hipLaunchKernelGGL(lanczos_finish_beta_kernel, dim3(1), dim3(kBlock), 0, s, st, step, (int64_t)kRedBlocks);
+  MB_HIP(hipGetLastError());
+}
+
+void launch_lanczos_finish_beta_n(const LanczosState &st, int step, int64_t nparts, hipStream_t s) {
+  hipLaunchKernelGGL(lanczos_finish_beta_kernel, dim3(1), dim3(kBlock), 0, s, st, step, nparts);
   MB_HIP(hipGetLastError());
 }
 

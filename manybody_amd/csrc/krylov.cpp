@@ -17,6 +17,7 @@
 #include <functional>
 #include <limits>
 #include <complex>
+#include <type_traits>
 
 namespace mb {
 
@@ -166,6 +167,54 @@ bool keep_krylov_basis(size_t bytes) {
   return bytes <= free_b / 2;
 }
 
+// Pass one of a real-space operator whose H.v runs in the block-staged up
+// kernel and the sorted row kernel, with the row pass first: step i's row pass
+// stages x_i = w - c x_{i-1} (the previous step's axpy, the same arithmetic,
+// done while the row is staged), writes x_i and z = (down hops + diagonal) x_i
+// and the row sums of |x_i|^2, which give beta_{i-1}; the up pass then forms
+// H x_i = (up hops) x_i + z and applies the Lanczos epilogue.  No separate
+// axpy pass: per step the row pass streams w, x_{i-1}, x_i, z and the up pass
+// x_i, z, x_{i-1}, w (8 vector passes against 9).  The same recurrence and the
+// same per-element arithmetic as the up-then-row step; the alpha and beta sums
+// run in another order.  The vectors stay unnormalised (x_i = beta_{i-1} v_i).
+void pass_one_fold(const HvTables &t, LanczosWork &W, int k, hipStream_t s) {
+  const int64_t D = W.D;
+  if (!W.z.p) W.z.alloc((size_t)D * 2 * sizeof(double));
+  const size_t np = (size_t)lanczos_fold_partials(t);
+  if (W.fold_part.bytes < np * sizeof(double)) W.fold_part.alloc(np * sizeof(double));
+  LanczosState st = W.st;
+  st.partial = W.fold_part.as<double>();
+  double *cur = W.v.as<double>(), *prv = W.vp.as<double>(), *wr = W.w.as<double>(), *z = W.z.as<double>();
+  for (int i = 0; i < k; ++i) {
+    EpiArgs ra;
+    ra.flags = st.flags;
+    if (i > 0) {
+      ra.fold = 1;
+      ra.fold_w = wr;
+      ra.fold_v = cur;
+      ra.fold_x = prv;  // x_{i-2}'s buffer receives x_i
+      ra.fold_partial = st.partial;
+      ra.alpha_dev = st.alpha;
+      ra.beta_dev = st.beta;
+      ra.fold_step = i - 1;
+    }
+    launch_row_down(t, cur, z, ra, s);
+    if (i > 0) {
+      launch_lanczos_finish_beta_n(st, i - 1, 2 * t.d_up, s);
+      std::swap(cur, prv);  // cur = x_i, prv = x_{i-1}
+    }
+    EpiArgs ua;
+    ua.prev = prv;
+    ua.beta_dev = st.beta;
+    ua.flags = st.flags;
+    ua.step = i;
+    ua.partial = st.partial;
+    ua.lazy = 1;
+    const int64_t nparts = launch_up_lanczos(t, cur, z, wr, ua, s);
+    launch_lanczos_finish_alpha(st, i, nparts, s);
+  }
+}
+
 // Pass one with Krylov vectors and H.v in T (double, or float for the FP32
 // Krylov option); b is FP64, the norm and the alpha/beta reductions are FP64.
 // With `basis` (k + 1 vectors of 2 D values of T) the recurrence runs in it:
@@ -197,7 +246,13 @@ LanczosResult pass_one_t(const HvOp &op, const void *b, int k, hipStream_t s, La
   // by beta; the next step's epilogue and axpy apply 1 / beta instead), which
   // saves one pass over w per step.  A stored basis is normalised as it is built.
   const bool lazy = basis == nullptr;
-  for (int i = 0; i < k; ++i) {
+  // the row-first step needs a fourth vector, which request regions and the
+  // no-allocation scratch do not budget for: there the step keeps its axpy pass
+  const HvTables *ht = op.hubbard_tables();
+  const bool fold = std::is_same<T, double>::value && lazy && ht && !thread_dev_alloc() &&
+                    lanczos_fold_ok(*ht);
+  if (fold) pass_one_fold(*ht, W, k, s);
+  for (int i = 0; i < (fold ? 0 : k); ++i) {
     if (basis) {
       v = basis + (int64_t)i * 2 * D;
       vp = i > 0 ? v - 2 * D : v;

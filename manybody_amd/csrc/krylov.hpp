@@ -56,6 +56,9 @@ struct LanczosWork {
   size_t esz;  // bytes per vector element: 8 (FP64) or 4 (FP32 Krylov vectors)
   DevBuf v, vp, w, scal;
   LanczosState st;
+  // the row-first step of real-space operators: the row pass's result and
+  // its own partial-sum slots (allocated on first use)
+  DevBuf z, fold_part;
   // vectors = false: scalars only (the recurrence then runs in a stored basis)
   LanczosWork(int64_t D, int k, int64_t nparts, size_t esz = sizeof(double), bool vectors = true);
 };

@@ -272,6 +272,7 @@ enum class Epi : int {
   Lanczos = 1,  // w = H v - beta[step-1] v_prev ; partial[wg] = sum v . w
   Recon = 2,    // w = H v - beta v_prev - alpha v
   Cheb = 3,     // t' = scale2 (H t - shift t) - t_prev ; partial[2 wg + {0,1}] = <t'|t>, <t'|t'>
+  Down = 4,     // z = (down hops + diagonal) x, the row pass of a row-first H.v (see launch_row_down)
 };
 
 struct EpiArgs {
@@ -287,6 +288,15 @@ struct EpiArgs {
   // beta[step-2] v_prev (1 at step 1): the epilogue forms w = H x / beta[step-1]
   // - (beta[step-1] / beta[step-2]) prev and the partial sum of (x / beta[step-1]) . w
   int lazy = 0;
+  // Down with a fold (the Lanczos step of the row-first H.v): the row pass
+  // stages x = fold_w - c fold_v, c = alpha[fold_step] / beta[fold_step - 1]
+  // (alpha[0] at step 0; the axpy of the previous Lanczos step), writes x to
+  // fold_x and |x row|^2 to fold_partial[row]
+  const void *fold_w = nullptr, *fold_v = nullptr;
+  void *fold_x = nullptr;
+  double *fold_partial = nullptr;
+  const double *alpha_dev = nullptr;
+  int fold = 0, fold_step = 0;
 };
 
 // y = epilogue(H x) over P planes; T = double or float.  Returns the number of
@@ -359,6 +369,21 @@ template <typename T>
 void launch_lanczos_axpy_norm(const T *v, T *w, int64_t n, const LanczosState &st, int step,
                               hipStream_t s, bool lazy = false);
 void launch_lanczos_finish_beta(const LanczosState &st, int step, hipStream_t s);
+// the same over nparts partial sums
+void launch_lanczos_finish_beta_n(const LanczosState &st, int step, int64_t nparts, hipStream_t s);
+// Row-first H.v of the Lanczos step on a real-space operator (two-pass path):
+// launch_row_down writes z = (down hops + diagonal) x row by row (with a.fold:
+// x formed from a.fold_w and a.fold_v and written to a.fold_x); launch_up_lanczos
+// then forms H x = (up hops) x + z and applies the Lanczos epilogue (a.prev,
+// a.beta_dev, a.step, a.lazy, a.flags), one partial sum per workgroup into
+// a.partial; it returns the number of partial sums.  lanczos_fold_ok says
+// whether the operator's tables allow it; lanczos_fold_partials bounds the
+// partial slots either launch writes.
+bool lanczos_fold_ok(const HvTables &t);
+int64_t lanczos_fold_partials(const HvTables &t);
+void launch_row_down(const HvTables &t, const double *x, double *z, const EpiArgs &a, hipStream_t s);
+int64_t launch_up_lanczos(const HvTables &t, const double *x, const double *z, double *w,
+                          const EpiArgs &a, hipStream_t s);
 template <typename T>
 void launch_lanczos_normalize(T *w, int64_t n, const LanczosState &st, int step, hipStream_t s);
 // pass two: w /= beta ; x += y w (complex y, planar vectors of D elements)

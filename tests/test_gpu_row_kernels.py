@@ -68,6 +68,49 @@ def test_hv_integer_inputs_bit_exact_vs_oracle(cuda):
     assert np.array_equal(y, A.matvec(x_ref))
 
 
+@contextlib.contextmanager
+def no_fold():
+    old = os.environ.get("MANYBODY_AMD_LANCZOS_FOLD")
+    os.environ["MANYBODY_AMD_LANCZOS_FOLD"] = "0"
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ["MANYBODY_AMD_LANCZOS_FOLD"]
+        else:
+            os.environ["MANYBODY_AMD_LANCZOS_FOLD"] = old
+
+
+@pytest.mark.parametrize("nu", [1, 2])
+def test_row_first_lanczos_matches_up_then_row(cuda, nu):
+    """Lanczos pass one with the row pass first (the previous step's axpy done
+    while the row is staged, no separate axpy pass; the default on these
+    sectors) against the up-then-row step with its axpy pass
+    (MANYBODY_AMD_LANCZOS_FOLD=0) and against the oracle: alpha/beta to 1e-12
+    relative over 40 steps (the two sum alpha and beta in different orders),
+    the lowest Ritz value of 120 steps to 1e-12, and the eigenvector through
+    pass two."""
+    op = make_op(nu)
+    b = op.gaussian(seed=41 + nu)
+    d_f = mb.lanczos_pass_one(op, b, 40)
+    with no_fold():
+        d_a = mb.lanczos_pass_one(op, b, 40)
+    assert d_f.steps_taken == d_a.steps_taken == 40
+    assert np.abs(np.array(d_f.alphas) - d_a.alphas).max() <= 1e-12 * np.abs(d_a.alphas).max()
+    assert np.abs(np.array(d_f.betas) - d_a.betas).max() <= 1e-12 * np.abs(d_a.betas).max()
+    P = po.ProductOperator([4, 4], nu, 7, T_HOP, U)
+    al, be, _ = P.lanczos(mb.from_planar(b), 6)
+    assert np.abs(np.array(d_f.alphas[:6]) - al).max() <= 1e-12 * np.abs(al).max()
+    assert np.abs(np.array(d_f.betas[:5]) - be).max() <= 1e-12 * np.abs(be).max()
+    ep_f = mb.find_min_eigenpair(op, 120, b)
+    with no_fold():
+        ep_a = mb.find_min_eigenpair(op, 120, b)
+    assert abs(ep_f.value - ep_a.value) <= 1e-12 * abs(ep_a.value)
+    assert (ep_f.vector - ep_a.vector).norm().item() <= 1e-9 * ep_a.vector.norm().item()
+    e, r = op.energy_residual(ep_f.vector)
+    assert abs(e - ep_f.value) <= 1e-10 * abs(ep_f.value)
+
+
 @pytest.mark.parametrize("nu", [1, 2])
 def test_hv_sorted_matches_compact(cuda, nu):
     op = make_op(nu)
