@@ -326,7 +326,7 @@ __global__ __launch_bounds__(1024) void row_kernel_compact(HvTables t, const T *
     const int wm = __builtin_amdgcn_readfirstlane((int)t.dn_cgmax[d0 / kWave]);
     uint4 q[K / 8];
 #pragma unroll
-    for (int j = 0; j < K / 8; ++j) q[j] = j * 8 < wm ? comp_chunk(t.dn_comp, d, j) : make_uint4(0, 0, 0, 0);
+    for (int j = 0; j < K / 8; ++j) q[j] = comp_chunk(t.dn_comp, d, j);  // all four: see comp_chunk
     const uint32_t sm = t.dn_csign[d];
     const T yin = ok ? y[base + d] : T(0);
     const T xv = row[ok ? d : Ddn];
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(1024) void row_kernel_mp(HvTables t, const T *__res
     const int wm = __builtin_amdgcn_readfirstlane((int)t.dn_cgmax[d0 / kWave]);
     uint4 q[K / 8];
 #pragma unroll
-    for (int j = 0; j < K / 8; ++j) q[j] = j * 8 < wm ? comp_chunk(t.dn_comp, d, j) : make_uint4(0, 0, 0, 0);
+    for (int j = 0; j < K / 8; ++j) q[j] = comp_chunk(t.dn_comp, d, j);  // all four: see comp_chunk
     const uint32_t sm = t.dn_csign[d];
     T dg = diag[popc(su & t.s_dn[ok ? d : 0])];
     if (t.has_onsite) dg = dg + on_u + Tabs<T>::dn_on(t)[ok ? d : 0];
@@ -579,7 +579,7 @@ __global__ __launch_bounds__(1024) void row_kernel_fused(HvTables t, const T *__
     const int wm = __builtin_amdgcn_readfirstlane((int)t.dn_cgmax[d0 / kWave]);
     uint4 q[K / 8];
 #pragma unroll
-    for (int j = 0; j < K / 8; ++j) q[j] = j * 8 < wm ? comp_chunk(t.dn_comp, d, j) : make_uint4(0, 0, 0, 0);
+    for (int j = 0; j < K / 8; ++j) q[j] = comp_chunk(t.dn_comp, d, j);  // all four: see comp_chunk
     const uint32_t sm = t.dn_csign[d];
     T dg = diag[popc(su & t.s_dn[ok ? d : 0])];
     if (t.has_onsite) dg = dg + on_u + Tabs<T>::dn_on(t)[ok ? d : 0];
