@@ -114,7 +114,8 @@ int mb_hubbard_shape(const mb_hubbard *h, int64_t *d_up, int64_t *d_dn);
 int mb_hv_c128(const mb_hubbard *h, const void *x_dev, void *y_dev, mb_stream stream);
 
 /* y = H x, complex64, device order (FP32 arithmetic; the reference has no
- * FP32 path). */
+ * FP32 path).  Real-space Hubbard operators and expressions of Hubbard form
+ * (see mb_expression_create); MB_ERR_INVALID for other handles. */
 int mb_hv_c64(const mb_hubbard *h, const void *x_dev, void *y_dev, mb_stream stream);
 
 /* y = H x for nvec complex64 vectors in device order (2 nvec planes). */
@@ -232,10 +233,11 @@ int mb_tridiag_eig(int m, const double *alphas, const double *betas, double *w, 
  * mu_n = (1/(nvec D)) sum_r <r|T_n(Hs)|r>.  The Chebyshev recurrence of
  * SparseKPMProjector::apply (kpm.h:212-229); H.v in FP32 complex (fp32 = 1) or
  * FP64 complex (fp32 = 0); inner products and moments accumulate in FP64.
- * M even, 1 <= nvec <= 64.  Real-space Hubbard operators advance all vectors
- * in one launch; any other handle (momentum sector, fixed-N basis, expression,
- * composite) runs FP64 one vector at a time with the same phases
- * (MB_ERR_INVALID for fp32 = 1 there). */
+ * M even, 1 <= nvec <= 64.  Real-space Hubbard operators, and expressions of
+ * Hubbard form on a real-space sector (see mb_expression_create), advance all
+ * vectors in one launch; any other handle (momentum sector, fixed-N basis,
+ * other expressions, composite) runs FP64 one vector at a time with the same
+ * phases (MB_ERR_INVALID for fp32 = 1 there). */
 int mb_kpm_dos_moments(const mb_hubbard *h, double a, double b, int M, int nvec, uint64_t seed,
                        int fp32, double *mu, mb_stream stream);
 
@@ -377,7 +379,13 @@ int mb_fixed_n_basis_create(int orbitals, int particles, mb_basis **out);
  * and fixed-momentum bases (at most 24 orbitals).  The handle works with the
  * FP64 H.v, order, fill, apply_ref, Lanczos, eigenpair, time-evolution and
  * energy/residual entry points; MB_ERR_INVALID for an operator on an orbital
- * >= the basis orbitals. */
+ * >= the basis orbitals.  On a fixed (N_up, N_dn) real-space sector an
+ * expression of Hubbard form (after merging and dropping: every hop one-body
+ * within a species with a real amplitude, real on-site values, and mixed terms
+ * only U n_{i,up} n_{i,dn} with one U on every site) runs on the Hubbard
+ * operator's tables and kernels built from its own hop lists, which also gives
+ * it FP32 H.v and FP32 KPM; MANYBODY_AMD_EXPR_NATIVE=0 keeps the generic
+ * expression kernels. */
 int mb_expression_create(const mb_basis *basis, int64_t n_terms, const double *coeffs,
                          const int64_t *offsets, const uint16_t *ops, mb_hubbard **out);
 

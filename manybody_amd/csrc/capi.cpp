@@ -412,10 +412,21 @@ int mb_hv_c128(const mb_hubbard *h, const void *x_dev, void *y_dev, mb_stream st
   });
 }
 
+// The tables of an FP32 H.v: a real-space Hubbard model, or a compiled
+// expression of Hubbard form (which runs on the same kernels)
+mb::HvTables fp32_tables(const mb_hubbard *h) {
+  if (h && h->expr) {
+    const mb::HvTables *t = h->expr->hubbard_tables();
+    need(t != nullptr, "FP32 H.v needs a real-space Hubbard operator or an expression of Hubbard form");
+    return *t;
+  }
+  return mb::tables_of(H(h));
+}
+
 int mb_hv_c64(const mb_hubbard *h, const void *x_dev, void *y_dev, mb_stream stream) {
   return guarded([&] {
     need(x_dev && y_dev && x_dev != y_dev, "x and y must be distinct device pointers");
-    mb::launch_hv<float>(mb::tables_of(H(h)), 2, (const float *)x_dev, (float *)y_dev,
+    mb::launch_hv<float>(fp32_tables(h), 2, (const float *)x_dev, (float *)y_dev,
                          mb::Epi::Plain, mb::EpiArgs{}, S(stream));
   });
 }
@@ -425,7 +436,7 @@ int mb_hv_c64_multi(const mb_hubbard *h, int nvec, const void *x_dev, void *y_de
   return guarded([&] {
     need(x_dev && y_dev && x_dev != y_dev, "x and y must be distinct device pointers");
     need(nvec >= 1 && nvec <= 64, "nvec must be in [1, 64]");
-    mb::launch_hv<float>(mb::tables_of(H(h)), 2 * nvec, (const float *)x_dev, (float *)y_dev,
+    mb::launch_hv<float>(fp32_tables(h), 2 * nvec, (const float *)x_dev, (float *)y_dev,
                          mb::Epi::Plain, mb::EpiArgs{}, S(stream));
   });
 }
@@ -652,7 +663,12 @@ int mb_kpm_dos_moments(const mb_hubbard *h, double a, double b, int M, int nvec,
       mb::kpm_dos_moments(H(h), *H(h).sec, a, b, M, nvec, seed, fp32 != 0, mu, S(stream));
       return;
     }
-    need(fp32 == 0, "FP32 KPM moments need a real-space Hubbard operator");
+    if (h->expr && h->expr->hubbard()) {  // an expression of Hubbard form
+      const mb::Hubbard &eh = *h->expr->hubbard();
+      mb::kpm_dos_moments(eh, *eh.sec, a, b, M, nvec, seed, fp32 != 0, mu, S(stream));
+      return;
+    }
+    need(fp32 == 0, "FP32 KPM moments need a real-space Hubbard operator or an expression of Hubbard form");
     mb::DevBuf own;
     const int64_t *ref = ref_positions(h, own, S(stream));
     mb::kpm_dos_moments_op(h->op(), ref, a, b, M, nvec, seed, mu, S(stream));

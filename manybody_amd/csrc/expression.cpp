@@ -28,6 +28,8 @@
 #include <algorithm>
 #include <functional>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <stdexcept>
 #include <tuple>
@@ -39,6 +41,13 @@ namespace {
 constexpr double kTol = 1000.0 * 2.220446049250313e-16;  // tolerances::tolerance<double>()
 
 bool tiny(std::complex<double> c) { return std::norm(c) < kTol * kTol; }
+
+// MANYBODY_AMD_EXPR_NATIVE=0 keeps expressions of Hubbard form on the generic
+// expression kernels
+bool native_enabled() {
+  const char *e = std::getenv("MANYBODY_AMD_EXPR_NATIVE");
+  return !(e && std::strcmp(e, "0") == 0);
+}
 
 int parity64(uint64_t v) { return __builtin_popcountll(v) & 1; }
 
@@ -378,6 +387,54 @@ ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector 
     mx_ptr.push_back((int64_t)mx_base.size());
   }
 
+  // Hubbard form on a real-space sector: every hop, on-site value and mixed
+  // amplitude real, and the mixed terms U n_{i up} n_{i dn} with one U on every
+  // site.  Such an expression runs on the Hubbard tables and kernels (block or
+  // strip up pass, sorted row pass; FP32 too) built from its own hop lists.
+  if (real && native_enabled()) {
+    bool ok = true;
+    for (size_t k = 1; ok && k < uo_amp.size(); k += 2) ok = uo_amp[k] == 0.0;
+    for (size_t k = 1; ok && k < do_amp.size(); k += 2) ok = do_amp[k] == 0.0;
+    for (size_t k = 1; ok && k < diag_up.size(); k += 2) ok = diag_up[k] == 0.0;
+    for (size_t k = 1; ok && k < diag_dn.size(); k += 2) ok = diag_dn[k] == 0.0;
+    std::vector<double> u_site(L, 0.0);
+    for (const auto &a : mixed) {
+      if (!ok) break;
+      const uint64_t up = a.touched & U, dn = a.touched >> L;
+      ok = a.in == a.out && a.in == a.touched && a.parity == 0 && a.c.imag() == 0.0 &&
+           __builtin_popcountll(up) == 1 && up == dn;
+      if (ok) u_site[__builtin_ctzll(up)] += a.c.real();
+    }
+    for (int i = 1; ok && i < L; ++i) ok = u_site[i] == u_site[0];
+    if (ok) {
+      hub_ = std::make_unique<Hubbard>();
+      Hubbard &h = *hub_;
+      h.sec = real;
+      h.U = L > 0 ? u_site[0] : 0.0;
+      auto lists = [](SpeciesHops &sp, int64_t n, const std::vector<int64_t> &ptr,
+                      const std::vector<double> &amp, const std::vector<double> &diag,
+                      auto tgt_of) {
+        sp.row_ptr.assign(1, 0);
+        for (int64_t r = 0; r < n; ++r) {
+          for (int64_t e = ptr[r]; e < ptr[r + 1]; ++e) {
+            sp.tgt.push_back(tgt_of(e));
+            sp.amp.push_back(amp[2 * e]);
+          }
+          sp.row_ptr.push_back((int32_t)sp.tgt.size());
+          sp.max_row = std::max(sp.max_row, (int)(ptr[r + 1] - ptr[r]));
+          sp.onsite.push_back(diag[2 * r]);
+        }
+      };
+      const int64_t ddn = real->d_dn;
+      lists(h.up, nrows, uo_ptr, uo_amp, diag_up,
+            [&](int64_t e) { return (int32_t)(uo_base[e] / ddn); });
+      lists(h.dn, ndn, do_ptr, do_amp, diag_dn, [&](int64_t e) { return do_col[e]; });
+      const_cast<Sector &>(*real).ensure_device();
+      finish_hubbard(h);
+      htb_ = tables_of(h);
+    }
+  }
+
   // work units: (row, chunk of columns)
   std::vector<int32_t> wg_row, wg_c0;
   for (int64_t g = 0; g < nrows; ++g)
@@ -489,11 +546,15 @@ ExpressionOperator::ExpressionOperator(const Sector *real, const MomentumSector 
 
 int64_t ExpressionOperator::hv(const double *x, double *y, Epi e, const EpiArgs &a,
                                hipStream_t s) const {
+  if (hub_) return launch_hv<double>(htb_, 2, x, y, e, a, s);
   return launch_expr_hv(tb_, x, y, e, a, s);
 }
 
-int64_t ExpressionOperator::hv(const float *, float *, Epi, const EpiArgs &, hipStream_t) const {
-  throw std::invalid_argument("FP32 H.v is not available for expression operators");
+int64_t ExpressionOperator::hv(const float *x, float *y, Epi e, const EpiArgs &a,
+                               hipStream_t s) const {
+  if (hub_) return launch_hv<float>(htb_, 2, x, y, e, a, s);
+  throw std::invalid_argument(
+      "FP32 H.v is available for expression operators of Hubbard form on real-space sectors only");
 }
 
 void ExpressionOperator::export_coo(std::vector<uint64_t> &rows, std::vector<uint64_t> &cols,

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <complex>
+#include <memory>
 #include <vector>
 
 #include "fixed_n.hpp"
@@ -86,10 +87,17 @@ class ExpressionOperator final : public HvOp {
                      const double *coeffs, const int64_t *offsets, const uint16_t *ops,
                      const FixedNBasis *fn = nullptr);
   int64_t dim() const override { return tb_.D; }
-  int64_t partials() const override { return 2 * tb_.nwg + 16; }
+  int64_t partials() const override {
+    return hub_ ? std::max<int64_t>(hv_partials(htb_, 2), 2 * tb_.nwg + 16) : 2 * tb_.nwg + 16;
+  }
   int64_t hv(const double *x, double *y, Epi e, const EpiArgs &a, hipStream_t s) const override;
   int64_t hv(const float *x, float *y, Epi e, const EpiArgs &a, hipStream_t s) const override;
   const ExprTables &tables() const { return tb_; }
+  // the Hubbard tables the operator runs on when its terms have Hubbard form
+  // (real one-body hops per species, real on-site values, U n_up n_dn with
+  // one U on every site; real-space sectors); nullptr otherwise
+  const HvTables *hubbard_tables() const override { return hub_ ? &htb_ : nullptr; }
+  const Hubbard *hubbard() const { return hub_.get(); }
   int64_t terms_kept() const { return kept_; }
   // the nonzeros in the reference's basis order, column-major with rows
   // ascending (arma::SpMat iteration order), entries below 1000 eps dropped
@@ -101,6 +109,8 @@ class ExpressionOperator final : public HvOp {
   const MomentumSector *mom_;
   const FixedNBasis *fn_;
   ExprTables tb_{};
+  std::unique_ptr<Hubbard> hub_;
+  HvTables htb_{};
   int64_t kept_ = 0;
   std::vector<DevBuf> bufs_;
   template <typename V>

@@ -359,6 +359,11 @@ void build_hubbard(Hubbard &h) {
   const auto mono = merge_monomials(h.bond_from, h.bond_to, h.t);
   species_hops(s.s_up, mono, h.up);
   species_hops(s.s_dn, mono, h.dn);
+  finish_hubbard(h);
+}
+
+void finish_hubbard(Hubbard &h) {
+  const Sector &s = *h.sec;
   pack_ell(h.dn);
   build_up_blocks(s.s_up, s.L, h.up);  // reorders the up entries: before pack_c16
   pack_c16(h.up);

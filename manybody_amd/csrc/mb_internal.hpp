@@ -203,6 +203,11 @@ struct Hubbard {
 };
 
 void build_hubbard(Hubbard &h);
+// The rest of build_hubbard once h.up and h.dn hold their CSR lists (row_ptr,
+// tgt, amp: entry (tgt, amp) of string r adds amp x[tgt] to string r's
+// output), on-site values and max_row: packed tables, U per doubly occupied
+// site from h.U, upload.  Also used for compiled expressions of that form.
+void finish_hubbard(Hubbard &h);
 // Bond list of the periodic hypercubic lattice in the reference's loop order
 // (HubbardModel / HubbardModel2D / HubbardModel3D).
 void lattice_bonds(int ndim, const int *sizes, std::vector<int> &from, std::vector<int> &to);

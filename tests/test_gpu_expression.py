@@ -236,11 +236,13 @@ def test_expression_operator_rejects_invalid_use(cuda):
     basis = mb.FermionBasis.sector(4, 2, 1)
     with pytest.raises(ValueError):  # orbital beyond the basis
         mb.FermionExpressionOperator(basis, mb.density(0, 5))
-    op = mb.FermionExpressionOperator(basis, mb.hopping(1.0, 0, 1, 0) + mb.density(1, 2))
+    # n_0up n_1dn is not of Hubbard form: the generic expression kernels, FP64 only
+    op = mb.FermionExpressionOperator(basis, mb.hopping(1.0, 0, 1, 0) + mb.density(1, 2) +
+                                      mb.density_density(0, 0, 1, 1))
     x = op.gaussian(seed=1)
     with pytest.raises(ValueError):
         op.hv(x.float())
-    with pytest.raises(ValueError):  # FP32 KPM needs a real-space Hubbard operator
+    with pytest.raises(ValueError):  # FP32 KPM needs the Hubbard tables
         mb.kpm_dos_moments(op, 10.0, 0.0, 8, 1, seed=1, fp32=True)
     mu = mb.kpm_dos_moments(op, 10.0, 0.0, 8, 1, seed=1)  # the default runs FP64 here
     assert abs(mu[0] - 1.0) < 1e-14
@@ -423,3 +425,140 @@ def test_rectangular_current_between_momentum_sectors(cuda, sizes, nu, nd, K, Q,
     want = po.current_operator(sizes, nu, nd, Kf, Qf, d, 1.0)
     assert np.count_nonzero(want) > 0
     assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+
+
+def hubbard_form_expression(rng, L, bonds, U, mags, one_way=0):
+    """Real hops on the bonds with amplitudes from `mags` magnitudes (random
+    signs), spin-dependent on-site energies, one U on every site, and
+    `one_way` one-directional hops (a non-Hermitian term; still of Hubbard
+    form: the kernels gather along each entry's own direction)."""
+    amps = rng.choice(np.asarray(mags, dtype=float), size=len(bonds)) * rng.choice([-1.0, 1.0], len(bonds))
+    terms = []
+    for (i, j), t in zip(bonds, amps):
+        for s in (0, 1):
+            terms.append((complex(t), [(0, s, i), (1, s, j)]))
+            terms.append((complex(t), [(0, s, j), (1, s, i)]))
+    for _ in range(one_way):
+        i, j = (int(v) for v in rng.choice(L, 2, replace=False))
+        s = int(rng.integers(2))
+        terms.append((complex(mags[0]), [(0, s, i), (1, s, j)]))
+    for i in range(L):
+        for s in (0, 1):
+            terms.append((complex(rng.standard_normal()), [(0, s, i), (1, s, i)]))
+        terms.append((complex(U), [(0, 0, i), (1, 0, i), (0, 1, i), (1, 1, i)]))
+    return expr_from_terms(terms)
+
+
+def ring_bonds(L):
+    return [(i, (i + 1) % L) for i in range(L)]
+
+
+def grid_bonds(dims):
+    L = int(np.prod(dims))
+    out = []
+    for s in range(L):
+        c = np.unravel_index(s, dims[::-1])[::-1]
+        for ax in range(len(dims)):
+            n = list(c)
+            n[ax] = (n[ax] + 1) % dims[ax]
+            t = int(np.ravel_multi_index(n[::-1], dims[::-1]))
+            if t != s:
+                out.append((s, t))
+    return out
+
+
+@pytest.mark.parametrize("fused", ["1", "0"])
+@pytest.mark.parametrize("dims,nu,nd,mags,one_way", [([8], 3, 3, [1.0], 0), ([3, 3], 2, 3, [0.7, 1.3], 0),
+                                                     ([2, 2, 2], 3, 2, [1.0], 3), ([9], 1, 4, [0.5, 1.0, 2.0], 2)])
+def test_hubbard_form_expression_runs_on_hubbard_kernels(cuda, monkeypatch, fused, dims, nu, nd, mags,
+                                                         one_way):
+    """An expression of Hubbard form (real hops per species, real on-site energies,
+    one U on every site) runs on the Hubbard tables and kernels built from its own
+    hop lists: H.v equals the oracle's matrix, the generic expression kernels
+    (MANYBODY_AMD_EXPR_NATIVE=0) and, in FP32, the FP64 result to FP32 rounding;
+    Lanczos coefficients equal the generic path's.  Several magnitudes take the
+    packed (ELL) row kernel and the strip up pass; one-directional hops check the
+    gather direction."""
+    monkeypatch.setenv("MANYBODY_AMD_FUSED", fused)
+    rng = np.random.default_rng(sum(dims) * 11 + nu + len(mags))
+    L = int(np.prod(dims))
+    e = hubbard_form_expression(rng, L, grid_bonds(dims) if len(dims) > 1 else ring_bonds(L), 4.0,
+                                mags, one_way)
+    basis = basis_for(dims, nu, nd)
+    op = mb.FermionExpressionOperator(basis, e)
+    up, dn = po.basis_states(L, nu, nd)
+    M = po.expression_matrix(L, list(zip(up, dn)), oracle_terms(e))
+    x = rng.standard_normal(M.shape[0]) + 1j * rng.standard_normal(M.shape[0])
+    want = M @ x
+    assert np.abs(apply_ref(op, x, cuda) - want).max() <= 1e-13 * np.abs(want).max()
+    xd = op.to_device_order(torch.from_numpy(x).to(cuda))
+    y = op.hv(xd)
+    y32 = op.hv(xd.float()).double()  # FP32 exists only on the Hubbard kernels
+    assert torch.abs(y32 - y).max().item() <= 2e-6 * torch.abs(y).max().item()
+    d = mb.lanczos_pass_one(op, xd, 30)
+    monkeypatch.setenv("MANYBODY_AMD_EXPR_NATIVE", "0")
+    gen = mb.FermionExpressionOperator(basis, e)
+    monkeypatch.delenv("MANYBODY_AMD_EXPR_NATIVE")
+    yg = gen.hv(xd)
+    assert torch.abs(y - yg).max().item() <= 1e-13 * torch.abs(yg).max().item()
+    dg = mb.lanczos_pass_one(gen, xd, 30)
+    assert d.steps_taken == dg.steps_taken
+    assert np.allclose(d.alphas, dg.alphas, rtol=1e-11, atol=1e-12)
+    assert np.allclose(d.betas, dg.betas, rtol=1e-11, atol=1e-12)
+    with pytest.raises(ValueError, match="Hubbard form"):
+        gen.hv(xd.float())
+    if one_way == 0:  # Hermitian: KPM moments, FP32 on the Hubbard kernels against FP64
+        a = float(np.abs(M).sum(axis=1).max()) * 1.01
+        mu32 = mb.kpm_dos_moments(op, a, 0.0, 32, 2, seed=3, fp32=True)
+        mu64 = mb.kpm_dos_moments(gen, a, 0.0, 32, 2, seed=3)
+        assert abs(mu64[0] - 1.0) < 1e-13
+        assert np.abs(np.asarray(mu32) - mu64).max() <= 1e-4
+
+
+@pytest.mark.parametrize("extra", ["nonuniform_u", "neighbour_v", "complex_hop", "pair_hop"])
+def test_expressions_outside_hubbard_form_keep_the_expression_kernels(cuda, extra):
+    """Terms that leave Hubbard form (a site with another U, n_up n_dn on two
+    sites, a complex hop, a pair hop) keep the generic expression kernels: FP32
+    H.v is refused, and FP64 H.v still equals the oracle."""
+    rng = np.random.default_rng(3)
+    L, nu, nd = 8, 3, 3
+    e = hubbard_form_expression(rng, L, ring_bonds(L), 4.0, [1.0])
+    extra_terms = {
+        "nonuniform_u": [(0.5 + 0j, [(0, 0, 2), (1, 0, 2), (0, 1, 2), (1, 1, 2)])],
+        "neighbour_v": [(0.5 + 0j, [(0, 0, 2), (1, 0, 2), (0, 1, 3), (1, 1, 3)])],
+        "complex_hop": [(0.3j, [(0, 0, 1), (1, 0, 4)]), (-0.3j, [(0, 0, 4), (1, 0, 1)])],
+        "pair_hop": [(0.4 + 0j, [(0, 0, 1), (0, 1, 1), (1, 1, 2), (1, 0, 2)]),
+                     (0.4 + 0j, [(0, 0, 2), (0, 1, 2), (1, 1, 1), (1, 0, 1)])],
+    }[extra]
+    e += expr_from_terms(extra_terms)
+    basis = basis_for([L], nu, nd)
+    op = mb.FermionExpressionOperator(basis, e)
+    xd = op.gaussian(seed=9)
+    with pytest.raises(ValueError, match="Hubbard form"):
+        op.hv(xd.float())
+    up, dn = po.basis_states(L, nu, nd)
+    M = po.expression_matrix(L, list(zip(up, dn)), oracle_terms(e))
+    x = rng.standard_normal(M.shape[0]) + 1j * rng.standard_normal(M.shape[0])
+    want = M @ x
+    assert np.abs(apply_ref(op, x, cuda) - want).max() <= 1e-13 * np.abs(want).max()
+
+
+def test_hubbard_expression_on_4x4_7p7_lanczos_and_fp32(cuda):
+    """The 4 x 4, 7 + 7 Hubbard expression runs the native kernels (block up pass,
+    sorted row pass, row-first Lanczos step): 40 Lanczos coefficients equal the
+    native operator's to 1e-12, and FP32 H.v equals the native FP32 H.v to FP32
+    rounding (the same tables but for the order of each string's hops)."""
+    basis = mb.FermionBasis.sector(16, 7, 7)
+    model = mb.HubbardModel2D(1.0, 8.0, 4, 4)
+    op = mb.FermionExpressionOperator(basis, model.hamiltonian())
+    x = op.gaussian(seed=77)
+    d1 = mb.lanczos_pass_one(op, x, 40)
+    y1 = op.hv(x.float())
+    del op
+    native = model.operator(basis)
+    d0 = mb.lanczos_pass_one(native, x, 40)
+    y0 = native.hv(x.float())
+    assert d1.steps_taken == d0.steps_taken == 40
+    assert np.abs(np.array(d1.alphas) - d0.alphas).max() <= 1e-12 * np.abs(d0.alphas).max()
+    assert np.abs(np.array(d1.betas) - d0.betas).max() <= 1e-12 * np.abs(d0.betas).max()
+    assert torch.abs(y1 - y0).max().item() <= 1e-5 * torch.abs(y0).max().item()
