@@ -2619,7 +2619,7 @@ int64_t up_block_grid(const HvTables &t, int P, int *nJ_out) {
 }
 
 int64_t lanczos_fold_partials(const HvTables &t) {
-  return std::max<int64_t>(2 * t.d_up, kRedBlocks + up_block_grid(t, 2, nullptr));
+  return kRedBlocks + std::max<int64_t>(2 * t.d_up, up_block_grid(t, 2, nullptr));
 }
 
 void launch_row_down(const HvTables &t, const double *x, double *z, const EpiArgs &a, hipStream_t s) {
@@ -2947,7 +2947,13 @@ void launch_lanczos_finish_beta(const LanczosState &st, int step, hipStream_t s)
 }
 
 void launch_lanczos_finish_beta_n(const LanczosState &st, int step, int64_t nparts, hipStream_t s) {
-  hipLaunchKernelGGL(lanczos_finish_beta_kernel, dim3(1), dim3(kBlock), 0, s, st, step, nparts);
+  // two levels, as for the up pass's alpha partials: one workgroup over the
+  // 22,880 row sums of the 4 x 4, 7 + 7 sector took 24 us
+  hipLaunchKernelGGL(chunk_sum_kernel, dim3(kRedBlocks), dim3(kBlock), 0, s, st.partial + kRedBlocks,
+                     nparts, st.partial);
+  MB_HIP(hipGetLastError());
+  hipLaunchKernelGGL(lanczos_finish_beta_kernel, dim3(1), dim3(kBlock), 0, s, st, step,
+                     (int64_t)kRedBlocks);
   MB_HIP(hipGetLastError());
 }
 

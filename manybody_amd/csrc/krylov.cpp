@@ -193,7 +193,7 @@ void pass_one_fold(const HvTables &t, LanczosWork &W, int k, hipStream_t s) {
       ra.fold_w = wr;
       ra.fold_v = cur;
       ra.fold_x = prv;  // x_{i-2}'s buffer receives x_i
-      ra.fold_partial = st.partial;
+      ra.fold_partial = st.partial + kRedBlocks;  // summed by launch_lanczos_finish_beta_n
       ra.alpha_dev = st.alpha;
       ra.beta_dev = st.beta;
       ra.fold_step = i - 1;

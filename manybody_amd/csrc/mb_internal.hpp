@@ -378,7 +378,8 @@ template <typename T>
 void launch_lanczos_axpy_norm(const T *v, T *w, int64_t n, const LanczosState &st, int step,
                               hipStream_t s, bool lazy = false);
 void launch_lanczos_finish_beta(const LanczosState &st, int step, hipStream_t s);
-// the same over nparts partial sums
+// the same over nparts partial sums at st.partial + kRedBlocks (summed in two
+// levels; st.partial[0, kRedBlocks) is overwritten)
 void launch_lanczos_finish_beta_n(const LanczosState &st, int step, int64_t nparts, hipStream_t s);
 // Row-first H.v of the Lanczos step on a real-space operator (two-pass path):
 // launch_row_down writes z = (down hops + diagonal) x row by row (with a.fold:

@@ -500,8 +500,9 @@ def test_kpm_moments_on_any_operator(cuda):
     """KPM DOS moments through the generic path (any operator, FP64): the
     compiled Hubbard expression gives the native real-space moments (same
     phases, 1e-12); a fixed-momentum sector gives the oracle's moments for the
-    same vectors (1e-11); Shifted(H, 1) with b + 1 gives H's moments; FP32 is
-    refused off the real-space kernels."""
+    same vectors (1e-11); Shifted(H, 1) with b + 1 gives H's moments.  FP32
+    runs on the Hubbard kernels only: the expression (Hubbard form) gives the
+    native FP32 moments to FP32 rounding, the composite is refused."""
     sizes, nu, nd, t, U = [8], 3, 3, 1.0, 4.0
     native = make_op(sizes, nu, nd, t, U)
     A = po.build_csc(sizes, nu, nd, t, U)
@@ -515,8 +516,12 @@ def test_kpm_moments_on_any_operator(cuda):
     mu_s = np.array(mb.kpm_dos_moments(mb.Shifted(native, 1.0), a, b + 1.0, M, nvec, seed,
                                        fp32=False))
     assert np.abs(mu_s - mu_n).max() < 1e-12
+    mu32_e = np.array(mb.kpm_dos_moments(expr, a, b, M, nvec, seed, fp32=True))
+    mu32_n = np.array(mb.kpm_dos_moments(native, a, b, M, nvec, seed, fp32=True))
+    assert np.abs(mu32_e - mu32_n).max() < 1e-5
+    assert np.abs(mu32_n - mu_n).max() < 1e-4
     with pytest.raises(ValueError):
-        mb.kpm_dos_moments(expr, a, b, M, nvec, seed, fp32=True)
+        mb.kpm_dos_moments(mb.Shifted(native, 1.0), a, b + 1.0, M, nvec, seed, fp32=True)
     # a fixed-momentum sector against the oracle with the same phase vectors
     import scipy.sparse as sps
     msz, K = [6], 1
