@@ -918,32 +918,47 @@ __global__ __launch_bounds__(1024) void row_kernel_sorted(HvTables t, const T *_
   const T h = Tabs<T>::mags(t)[0];
   const EpiK<T> bp = epi_consts<T, E>(a);
   const bool rp = reads_prev<T, E>(a, bp);
+  // The slots do not depend on the row: each thread's slot strings and the
+  // waves' sign patterns per round (both packed two per word) are loaded once,
+  // so a round waits on one L2 round trip (its table chunks and the strings'
+  // bits); loaded per round, the sign pattern had to arrive before the chunks
+  // were requested, and the string before its bits.
+  uint32_t dd[R / 2], pns[R / 2];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    if (j % 2 == 0) {
+      dd[j / 2] = (uint32_t)Ddn | ((uint32_t)Ddn << 16);
+      pns[j / 2] = 0;
+    }
+    if (j < nr) {
+      const uint32_t d = t.srt_perm[j * NT + tid];
+      const uint32_t pn = (uint32_t)__builtin_amdgcn_readfirstlane((int)t.srt_pn[(j * NT + tid) >> 6]);
+      dd[j / 2] = (j & 1) ? ((dd[j / 2] & 0xFFFFu) | (d << 16)) : ((dd[j / 2] & 0xFFFF0000u) | d);
+      pns[j / 2] |= pn << (16 * (j & 1));
+    }
+  }
   for (; w < nrows; w += gridDim.x) {
     const int64_t nx = w + gridDim.x;
     const int64_t u = w % Dup;
     const uint32_t su = t.s_up[u];
     const T on_u = t.has_onsite ? Tabs<T>::up_on(t)[u] : T(0);
-    // An opaque zero added to the table bases per row: otherwise the per-round
-    // addresses derived from them are hoisted out of the row loop and spilled.
+    // An opaque zero added per row to the table base and the slot data:
+    // otherwise the per-round addresses, sign selects and branch conditions
+    // derived from them are hoisted out of the row loop and spilled.
     int zero = 0;
     asm volatile("" : "+s"(zero));
-    const uint16_t *perm_b = t.srt_perm + zero, *comp_b = t.srt_comp + zero, *pn_b = t.srt_pn + zero;
-    // A: hop sums of every slot of this thread; slot strings packed two per word
+    const uint16_t *comp_b = t.srt_comp + zero;
+    // A: hop sums of every slot of this thread
     T res[R];
-    uint32_t dd[R / 2];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       res[j] = T(0);
-      if (j % 2 == 0) dd[j / 2] = (uint32_t)Ddn | ((uint32_t)Ddn << 16);
       if (j < nr) {
-        // uniform bases of this round's tables; lanes index them with tid
-        const uint16_t *__restrict__ perm = perm_b + j * NT;
+        // uniform base of this round's table; lanes index it with tid
         const uint16_t *__restrict__ comp = comp_b + (int64_t)j * NT * K;
-        const int d = perm[tid];
-        dd[j / 2] = (j & 1) ? ((dd[j / 2] & 0xFFFFu) | ((uint32_t)d << 16))
-                            : ((dd[j / 2] & 0xFFFF0000u) | (uint32_t)d);
+        const int d = (int)(((dd[j / 2] + (uint32_t)zero) >> (16 * (j & 1))) & 0xFFFFu);
         // wave-uniform sign pattern: slots [0, np_) positive, [np_, nt) negative
-        const int pn = __builtin_amdgcn_readfirstlane((int)pn_b[(j * NT + tid) >> 6]);
+        const int pn = (int)(((pns[j / 2] + (uint32_t)zero) >> (16 * (j & 1))) & 0xFFFFu);
         const int np_ = pn & 0xFF, nt = np_ + (pn >> 8);
         uint4 q[K / 8];
 #pragma unroll
