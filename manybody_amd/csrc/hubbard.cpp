@@ -90,6 +90,80 @@ size_t comp_pos(size_t d, int e) {
   return (d / 64) * 2048 + (size_t)(e / 8) * 512 + (d % 64) * 8 + (size_t)(e % 8);
 }
 
+// Orders the targets of one sign class of a wave's 64 slots over the columns
+// [0, C) so that in a column the lanes of each half-wave read distinct LDS
+// bank pairs where the targets allow it: the LDS serves a wave's 8-byte reads
+// as two halves of 32 lanes over 32 bank pairs (bank pair = element index mod
+// 32), and a column costs its largest number of distinct elements on one pair.
+// Per half-wave this is an edge colouring of the bipartite multigraph lanes x
+// residues with C colours (a residue with more than C targets is split into
+// copies of at most C, which may then share a column); Koenig's alternating
+// path recolouring finds one.  Columns a lane has no target for get the pad.
+// Measured on the 4 x 4, 7 + 7 sector: 2.0 elements per bank pair and column
+// instead of 3.4 in hop order.
+void colour_columns(const std::vector<std::vector<uint16_t>> &lists, int C, uint16_t pad,
+                    std::vector<uint16_t> &out) {
+  out.assign(64 * (size_t)C, pad);
+  if (C == 0) return;
+  struct Edge {
+    int lane, rv;
+    uint16_t t;
+    int col;
+  };
+  constexpr int kCopies = 32;
+  std::vector<int> lane_c(32 * (size_t)C), rv_c(32 * kCopies * (size_t)C);
+  for (int h0 = 0; h0 < 64; h0 += 32) {
+    std::vector<Edge> edges;
+    int cnt[32] = {0};
+    for (int i = 0; i < 32; ++i)
+      for (uint16_t t : lists[h0 + i]) {
+        const int r = t % 32, k = cnt[r]++ / C;
+        edges.push_back({i, r * kCopies + k, t, -1});
+      }
+    std::fill(lane_c.begin(), lane_c.end(), -1);
+    std::fill(rv_c.begin(), rv_c.end(), -1);
+    auto at = [&](bool lane_side, int x, int c) -> int & {
+      return lane_side ? lane_c[(size_t)x * C + c] : rv_c[(size_t)x * C + c];
+    };
+    for (size_t e = 0; e < edges.size(); ++e) {
+      const int u = edges[e].lane, v = edges[e].rv;
+      int a = 0, b = 0;
+      while (at(true, u, a) >= 0) ++a;
+      while (at(false, v, b) >= 0) ++b;
+      int c = a;
+      if (at(false, v, a) >= 0) {
+        if (at(true, u, b) < 0) {
+          c = b;
+        } else {  // swap a and b along the path from v that starts with colour a
+          std::vector<int> path;
+          bool lane_side = false;
+          int x = v, want = a;
+          while (at(lane_side, x, want) >= 0) {
+            const int f = at(lane_side, x, want);
+            path.push_back(f);
+            x = lane_side ? edges[f].rv : edges[f].lane;
+            lane_side = !lane_side;
+            want = want == a ? b : a;
+          }
+          for (int f : path) {
+            at(true, edges[f].lane, edges[f].col) = -1;
+            at(false, edges[f].rv, edges[f].col) = -1;
+          }
+          for (int f : path) {
+            edges[f].col = edges[f].col == a ? b : a;
+            at(true, edges[f].lane, edges[f].col) = f;
+            at(false, edges[f].rv, edges[f].col) = f;
+          }
+        }
+      }
+      edges[e].col = c;
+      at(true, u, c) = (int)e;
+      at(false, v, c) = (int)e;
+    }
+    for (const Edge &e : edges) out[(size_t)(h0 + e.lane) * C + e.col] = e.t;
+  }
+}
+
 // The compact table in sorted slot order with one sign pattern per wave.
 // Strings are sorted (stable) by hop count, then by positive hop count, cut
 // into groups of 64 (one wave), and the groups of each round of kSortedRound
@@ -131,14 +205,22 @@ void build_sorted(SpeciesHops &h) {
         }
       }
       if (P + N > 32) return;  // srt_ok stays false
+      std::vector<std::vector<uint16_t>> lp(64), ln(64);
       for (int l = 0; l < 64; ++l) {
         const size_t slot = j * kSortedRound + (size_t)w * 64 + l;
         const uint32_t d = order[src * 64 + l];
         h.sperm[slot] = (uint16_t)d;
         if (d >= n) continue;
-        int kp = 0, kn = P;
         for (int e = h.row_ptr[d]; e < h.row_ptr[d + 1]; ++e)
-          h.scomp[comp_pos(slot, h.amp[e] < 0 ? kn++ : kp++)] = (uint16_t)h.tgt[e];
+          (h.amp[e] < 0 ? ln : lp)[l].push_back((uint16_t)h.tgt[e]);
+      }
+      std::vector<uint16_t> cp_, cn_;
+      colour_columns(lp, P, (uint16_t)n, cp_);
+      colour_columns(ln, N, (uint16_t)n, cn_);
+      for (int l = 0; l < 64; ++l) {
+        const size_t slot = j * kSortedRound + (size_t)w * 64 + l;
+        for (int k = 0; k < P; ++k) h.scomp[comp_pos(slot, k)] = cp_[(size_t)l * P + k];
+        for (int k = 0; k < N; ++k) h.scomp[comp_pos(slot, P + k)] = cn_[(size_t)l * N + k];
       }
       h.spn[(j * kSortedRound + (size_t)w * 64) / 64] = (uint16_t)(P | N << 8);
     }
