@@ -851,10 +851,13 @@ __global__ __launch_bounds__(NT) void up_block_kernel(HvTables t, int P, const T
 // Persistent row kernel over the sorted compact table.  One workgroup of 1024
 // threads per CU walks plane rows w, w + gridDim.x, ...  Per row:
 //  A. every thread takes one slot per round (kSortedRound slots): the string of
-//     the slot, its hop targets and signs; the trip count is the wave's shared
-//     hop count, so lanes do no padded work beyond rounding to even; the
-//     diagonal comes from a copy of diagU in LDS.  Results stay in registers.
-//     Then the next row is requested into registers; it lands during B/C.
+//     the slot and the wave's sign pattern (both loaded once per kernel: slots
+//     do not depend on the row), its hop targets (requested a round ahead, in
+//     an order that spreads a half-wave's reads over the LDS bank pairs); the
+//     trip count is the wave's shared hop count, so lanes do no padded work
+//     beyond rounding to a multiple of four; the diagonal comes from a copy of
+//     diagU in LDS.  Results stay in registers.  Then the next row is
+//     requested into registers; it lands during B/C.
 //  B/C. in two halves of kSortedHalf strings (the LDS left over holds half a
 //     row): results are written to an LDS buffer at their string, then one
 //     coalesced sweep adds the strip result and applies the epilogue (y, and
