@@ -30,8 +30,10 @@ namespace {
 using namespace bdev;
 
 // v = sigma X (sigma = 1 / |b| at step 0, 1 / beta[step-1] after: the previous
-// step's normalisation, applied here instead of in a pass of its own), written
-// to V; w = H v - beta[step-1] v_prev ; partial[tile] = sum v . w
+// step's normalisation, applied as X is read instead of in a pass of its own);
+// w = H v - beta[step-1] v_prev ; partial[tile] = sum v . w.  The normalised
+// vectors are not stored: v_prev is read as sigma_prev X_prev (VP = the
+// previous step's X), and V, when not null, receives v.
 template <int BS>
 __global__ __launch_bounds__(BS) void bl_hv_kernel(const BInst *__restrict__ inst,
                                                    const BTile *__restrict__ tiles,
@@ -49,10 +51,13 @@ __global__ __launch_bounds__(BS) void bl_hv_kernel(const BInst *__restrict__ ins
     return;
   }
   const BInst I = inst[tl.inst];
-  const double bp = step > 0 ? beta[(int64_t)tl.inst * k + step - 1] : 0.0;
-  const double sigma = 1.0 / (step > 0 ? bp : bnorm[tl.inst]);
+  const double b1 = step > 0 ? beta[(int64_t)tl.inst * k + step - 1] : 0.0;
+  const double sigma = 1.0 / (step > 0 ? b1 : bnorm[tl.inst]);
+  // beta[step-1] v_prev = beta[step-1] sigma_prev X_prev
+  const double bp = step > 0 ? b1 / (step > 1 ? beta[(int64_t)tl.inst * k + step - 2] : bnorm[tl.inst])
+                             : 0.0;
   const double *x = X + I.off, *vp = VP + I.off;
-  double *v = V + I.off, *w = Wv + I.off;
+  double *v = V ? V + I.off : nullptr, *w = Wv + I.off;
   const int64_t D = I.D;
   double s = 0.0;
   if (I.rows) {
@@ -67,8 +72,10 @@ __global__ __launch_bounds__(BS) void bl_hv_kernel(const BInst *__restrict__ ins
     re *= sigma;
     im *= sigma;
     const double vr = sigma * x[e], vi = sigma * x[D + e];
-    v[e] = vr;
-    v[D + e] = vi;
+    if (v) {
+      v[e] = vr;
+      v[D + e] = vi;
+    }
     if (bp != 0.0) {
       re -= bp * vp[e];
       im -= bp * vp[D + e];
@@ -81,8 +88,9 @@ __global__ __launch_bounds__(BS) void bl_hv_kernel(const BInst *__restrict__ ins
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// mode 0: w -= alpha v, partial = |w|^2 ; mode 1: w /= beta ; mode 2: partial = |w|^2 ;
-// mode 3: v = w / bnorm (start vector), vp = 0
+// mode 0: w -= alpha v with v = sigma V (sigma as in bl_hv_kernel: V holds the
+// step's unnormalised input), partial = |w|^2 ; mode 1: w /= beta ;
+// mode 2: partial = |w|^2 ; mode 3: v = w / bnorm (start vector), vp = 0
 template <int BS>
 __global__ __launch_bounds__(BS) void bl_vec_kernel(const BInst *__restrict__ inst,
                                                     const BTile *__restrict__ tiles,
@@ -101,7 +109,10 @@ __global__ __launch_bounds__(BS) void bl_vec_kernel(const BInst *__restrict__ in
   const int64_t D = I.D;
   double *v = V + I.off, *vp = VP + I.off, *w = Wv + I.off;
   double s = 0.0;
-  const double a = mode == 0 ? alpha[(int64_t)tl.inst * k + step] : 0.0;
+  const double a =
+      mode == 0 ? alpha[(int64_t)tl.inst * k + step] /
+                      (step > 0 ? beta[(int64_t)tl.inst * k + step - 1] : bnorm[tl.inst])
+                : 0.0;
   const double b = mode == 1 ? beta[(int64_t)tl.inst * k + step] : (mode == 3 ? bnorm[tl.inst] : 1.0);
   for (int64_t e = tl.e0 + threadIdx.x; e < tl.e0 + tl.n; e += BS) {
     for (int p = 0; p < 2; ++p) {
@@ -519,22 +530,21 @@ void Batch::record_run(int k, hipStream_t s) {
   }
   vec_step(nullptr, nullptr, B_.as<double>(), kcap_, 0, 2, s);  // |b|^2 partials
   reduce(kcap_, 0, 2, s);                                      // bnorm, zero-norm flag
-  // x: the unnormalised input of the step (b, then the previous w); the step
-  // writes v = x / (|b| or beta) and w.  B_ is never overwritten, so a replay
-  // starts from the same vectors.
-  double *x = B_.as<double>(), *v = V_.as<double>(), *vp = VP_.as<double>(), *w = W_.as<double>();
-  double *spare = Q_.as<double>();
+  // x: the unnormalised input of the step (b, then the previous w), read as
+  // v = x / (|b| or beta); the previous step's x stands for v_prev.  The
+  // normalised vectors are never written (two vector passes per step fewer).
+  // B_ is never overwritten, so a replay starts from the same vectors; the
+  // steps' outputs rotate through three buffers.
+  double *rot[3] = {W_.as<double>(), V_.as<double>(), VP_.as<double>()};
+  double *x = B_.as<double>(), *xp = B_.as<double>();  // xp unread at step 0
   for (int i = 0; i < k; ++i) {
-    hv_step(x, v, vp, w, kcap_, i, s);
+    double *w = rot[i % 3];
+    hv_step(x, nullptr, xp, w, kcap_, i, s);
     reduce(kcap_, i, 0, s);                   // alpha
-    vec_step(v, nullptr, w, kcap_, i, 0, s);  // w -= alpha v, |w|^2
+    vec_step(x, nullptr, w, kcap_, i, 0, s);  // w -= alpha v, |w|^2
     reduce(kcap_, i, 1, s);                   // beta, breakdown
-    double *free_buf = i == 0 ? spare : x;
-    double *old_vp = vp;
+    xp = x;
     x = w;
-    vp = v;
-    v = old_vp;
-    w = free_buf;
   }
   if (n_small_ > 0) MB_HIP(hipStreamWaitEvent(s, join_, 0));
   MB_HIP(hipGetLastError());
