@@ -353,7 +353,6 @@ Batch::Batch(std::vector<const Hubbard *> models) : models_(std::move(models)) {
   VP_.alloc(bytes);
   W_.alloc(bytes);
   B_.alloc(bytes);
-  Q_.alloc(bytes);
   d_inst_.alloc(sizeof(BInst) * std::max<size_t>(host_inst_.size(), 1));
   done_.alloc(sizeof(int) * std::max<size_t>(host_inst_.size(), 1));
   steps_.alloc(sizeof(int) * std::max<size_t>(host_inst_.size(), 1));
@@ -362,7 +361,7 @@ Batch::Batch(std::vector<const Hubbard *> models) : models_(std::move(models)) {
 }
 
 int64_t Batch::footprint_bytes() const {
-  return 5 * total_ * (int64_t)sizeof(double);
+  return 4 * total_ * (int64_t)sizeof(double);
 }
 
 void Batch::set_mapping(int tile, int block) {

@@ -88,7 +88,10 @@ class Batch {
   std::vector<const Hubbard *> models_;
   std::vector<BInst> host_inst_;
   std::vector<BTile> tiles_;
-  DevBuf d_inst_, d_tiles_, V_, VP_, W_, B_, Q_, partial_, alpha_, beta_, done_, steps_, bnorm_;
+  // B_: the start vectors (never overwritten); W_, V_, VP_: the steps'
+  // unnormalised outputs rotate through them (the persistent kernel uses the
+  // same buffers in its own instances' ranges)
+  DevBuf d_inst_, d_tiles_, V_, VP_, W_, B_, partial_, alpha_, beta_, done_, steps_, bnorm_;
   DevBuf d_small_;      // indices of the persistent (small) instances
   int n_small_ = 0;
   hipStream_t side_ = nullptr;  // the persistent kernel's branch of a run
