@@ -3,13 +3,14 @@
 // medium Hubbard models share every launch, and models join or leave between
 // steps without stopping the others.
 //
-// Per tick, for all members at once: (1) v = x / (|b| at step 0, beta[s-1]
-// after) written out, w = H v - beta[s-1] v_prev, per-tile partial v.w, (2)
-// alpha[s] per member, (3) w -= alpha v with |w|^2 partials, (4) beta[s] per
-// member, breakdown (1000 eps) or step count reached -> done, next step index.
-// The normalisation w / beta of a step is applied when the next step reads it,
-// so there is no pass of its own.  The four vector buffers (x, v, v_prev, w)
-// rotate roles after each tick for every member together; each member keeps its
+// Per tick, for all members at once: (1) w = H v - beta[s-1] v_prev with
+// v = x / (|b| at step 0, beta[s-1] after) and v_prev = x_prev / (|b| at step 1,
+// beta[s-2] after), per-tile partial v.w, (2) alpha[s] per member, (3) w -=
+// alpha v with |w|^2 partials, (4) beta[s] per member, breakdown (1000 eps) or
+// step count reached -> done, next step index.  Only the unnormalised inputs
+// x are kept: the normalisations are applied as the vectors are read, so there
+// is no pass for them and v is never written.  The vector buffers (x, x_prev,
+// w) rotate roles after each tick for every member together; each member keeps its
 // own step index (double-buffered: a tick reads one copy and writes the other),
 // so a member that joined late simply counts from 0.  Tiles are per member and the
 // per-member sums run over its own tiles in order, so a member's numbers do not
@@ -43,10 +44,12 @@ __global__ __launch_bounds__(kLaneBlock) void ln_hv_kernel(
     return;
   }
   const int s = step[I.slot];
-  const double bp = s > 0 ? beta[(int64_t)I.slot * kmax + s - 1] : 0.0;
-  const double sigma = 1.0 / (s > 0 ? bp : bnorm[I.slot]);
+  const double b1 = s > 0 ? beta[(int64_t)I.slot * kmax + s - 1] : 0.0;
+  const double sigma = 1.0 / (s > 0 ? b1 : bnorm[I.slot]);
+  // beta[s-1] v_prev = beta[s-1] / (beta[s-2] or |b|) x_prev
+  const double bp = s > 0 ? b1 / (s > 1 ? beta[(int64_t)I.slot * kmax + s - 2] : bnorm[I.slot]) : 0.0;
   const double *x = X + I.off, *vp = VP + I.off;
-  double *v = V + I.off, *w = Wv + I.off;
+  double *v = V ? V + I.off : nullptr, *w = Wv + I.off;
   const int64_t D = I.D;
   double acc = 0.0;
   if (I.rows) {
@@ -65,8 +68,10 @@ __global__ __launch_bounds__(kLaneBlock) void ln_hv_kernel(
       re *= sigma;
       im *= sigma;
       const double vr = sigma * x[e], vi = sigma * x[D + e];
-      v[e] = vr;
-      v[D + e] = vi;
+      if (v) {
+        v[e] = vr;
+        v[D + e] = vi;
+      }
       if (bp != 0.0) {
         re -= bp * vp[e];
         im -= bp * vp[D + e];
@@ -80,11 +85,12 @@ __global__ __launch_bounds__(kLaneBlock) void ln_hv_kernel(
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
-// w -= alpha v, partial = |w|^2
+// w -= alpha v with v = x / (|b| or beta[s-1]) read from the step's input X,
+// partial = |w|^2
 __global__ __launch_bounds__(kLaneBlock) void ln_axpy_kernel(
     const LaneInst *__restrict__ inst, const BTile *__restrict__ tiles, const double *__restrict__ V,
-    double *__restrict__ Wv, const double *alpha, const int *step, const int *done, int kmax,
-    double *partial) {
+    double *__restrict__ Wv, const double *alpha, const double *beta, const double *bnorm,
+    const int *step, const int *done, int kmax, double *partial) {
   __shared__ double lds[kLaneBlock / kWave];
   const BTile tl = tiles[blockIdx.x];
   const LaneInst I = inst[tl.inst];
@@ -95,7 +101,9 @@ __global__ __launch_bounds__(kLaneBlock) void ln_axpy_kernel(
   const int64_t D = I.D;
   const double *v = V + I.off;
   double *w = Wv + I.off;
-  const double a = alpha[(int64_t)I.slot * kmax + step[I.slot]];
+  const int s = step[I.slot];
+  const double a = alpha[(int64_t)I.slot * kmax + s] /
+                   (s > 0 ? beta[(int64_t)I.slot * kmax + s - 1] : bnorm[I.slot]);
   double acc = 0.0;
   for (int64_t e = tl.e0 + threadIdx.x; e < tl.e0 + tl.n; e += kLaneBlock) {
     for (int p = 0; p < 2; ++p) {
@@ -298,25 +306,25 @@ void LaneEngine::set_members(const std::vector<LaneInst> &members, hipStream_t s
 
 void LaneEngine::tick(hipStream_t s) {
   if (n_members_ == 0) return;
-  double *x = role_[0], *v = role_[1], *vp = role_[2], *w = role_[3];
+  double *x = role_[0], *xp = role_[2], *w = role_[3];
   const int *sin = step_[cur_];
   int *sout = step_[cur_ ^ 1];
   const dim3 gt((unsigned)n_tiles_), gm((unsigned)n_members_), bl(kLaneBlock), bw(kWave);
-  hipLaunchKernelGGL(ln_hv_kernel, gt, bl, lds_bytes_, s, d_inst_, d_tiles_, (const double *)x, v,
-                     (const double *)vp, w, (const double *)beta_, (const double *)bnorm_, sin,
+  hipLaunchKernelGGL(ln_hv_kernel, gt, bl, lds_bytes_, s, d_inst_, d_tiles_, (const double *)x, nullptr,
+                     (const double *)xp, w, (const double *)beta_, (const double *)bnorm_, sin,
                      (const int *)done_, kmax_, partial_);
   hipLaunchKernelGGL(ln_reduce_kernel, gm, bw, 0, s, d_inst_, (const double *)partial_, alpha_, beta_,
                      done_, sin, sout, kmax_, 0);
-  hipLaunchKernelGGL(ln_axpy_kernel, gt, bl, 0, s, d_inst_, d_tiles_, (const double *)v, w,
-                     (const double *)alpha_, sin, (const int *)done_, kmax_, partial_);
+  hipLaunchKernelGGL(ln_axpy_kernel, gt, bl, 0, s, d_inst_, d_tiles_, (const double *)x, w,
+                     (const double *)alpha_, (const double *)beta_, (const double *)bnorm_, sin,
+                     (const int *)done_, kmax_, partial_);
   hipLaunchKernelGGL(ln_reduce_kernel, gm, bw, 0, s, d_inst_, (const double *)partial_, alpha_, beta_,
                      done_, sin, sout, kmax_, 1);
   MB_HIP(hipGetLastError());
-  // rotate: the next input is w, v becomes v_prev; the old v_prev and x are free
+  // rotate: the next input is w, x becomes x_prev; the old x_prev is free
   role_[0] = w;
-  role_[1] = vp;
-  role_[2] = v;
-  role_[3] = x;
+  role_[2] = x;
+  role_[3] = xp;
   cur_ ^= 1;
 }
 

@@ -56,7 +56,7 @@ class LaneEngine {
   int nslots_, kmax_, tile_;
   int64_t max_tiles_;
   DevBuf mem_;
-  double *role_[4];                    // role_[0] = x (step input), [1] = v, [2] = v_prev, [3] = w
+  double *role_[4];                    // role_[0] = x (step input), [2] = x_prev, [3] = w; [1] unused
   double *alpha_, *beta_, *bnorm_, *partial_;
   int *done_, *step_[2];
   int cur_ = 0;                        // step_[cur_] is read by the next tick
