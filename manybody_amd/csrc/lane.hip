@@ -159,7 +159,7 @@ int64_t align_up(int64_t n, int64_t a) { return (n + a - 1) / a * a; }
 
 int64_t LaneEngine::bytes_needed(int64_t buf_doubles, int nslots, int kmax, int tile) {
   const int64_t max_tiles = buf_doubles / tile + nslots + 1;
-  return 4 * align_up(buf_doubles * 8, 256) + 3 * align_up((int64_t)nslots * kmax * 8, 256) +
+  return 3 * align_up(buf_doubles * 8, 256) + 3 * align_up((int64_t)nslots * kmax * 8, 256) +
          align_up((int64_t)nslots * 8, 256) + align_up((max_tiles + 2 * kRedBlocks) * 8, 256) +
          3 * align_up((int64_t)nslots * 4, 256) + align_up((int64_t)nslots * sizeof(LaneInst), 256) +
          align_up(max_tiles * (int64_t)sizeof(BTile), 256) + 4096;
@@ -306,7 +306,7 @@ void LaneEngine::set_members(const std::vector<LaneInst> &members, hipStream_t s
 
 void LaneEngine::tick(hipStream_t s) {
   if (n_members_ == 0) return;
-  double *x = role_[0], *xp = role_[2], *w = role_[3];
+  double *x = role_[0], *xp = role_[1], *w = role_[2];
   const int *sin = step_[cur_];
   int *sout = step_[cur_ ^ 1];
   const dim3 gt((unsigned)n_tiles_), gm((unsigned)n_members_), bl(kLaneBlock), bw(kWave);
@@ -323,8 +323,8 @@ void LaneEngine::tick(hipStream_t s) {
   MB_HIP(hipGetLastError());
   // rotate: the next input is w, x becomes x_prev; the old x_prev is free
   role_[0] = w;
-  role_[2] = x;
-  role_[3] = xp;
+  role_[1] = x;
+  role_[2] = xp;
   cur_ ^= 1;
 }
 

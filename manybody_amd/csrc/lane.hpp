@@ -18,7 +18,7 @@ struct LaneInst {
   int k;           // Lanczos steps requested
 };
 
-// Device side of the lane: four role buffers (x, v, v_prev, w) whose roles rotate
+// Device side of the lane: three role buffers (x, x_prev, w) whose roles rotate
 // every tick, per-slot Lanczos state (alpha, beta, step, done, start norm),
 // member and tile tables.  Members join and leave between ticks; every member
 // does one Lanczos step per tick, counted by its own step index, so members at
@@ -56,7 +56,7 @@ class LaneEngine {
   int nslots_, kmax_, tile_;
   int64_t max_tiles_;
   DevBuf mem_;
-  double *role_[4];                    // role_[0] = x (step input), [2] = x_prev, [3] = w; [1] unused
+  double *role_[3];                    // role_[0] = x (step input), [1] = x_prev, [2] = w
   double *alpha_, *beta_, *bnorm_, *partial_;
   int *done_, *step_[2];
   int cur_ = 0;                        // step_[cur_] is read by the next tick

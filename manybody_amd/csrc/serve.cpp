@@ -333,8 +333,8 @@ int64_t Server::submit(const ServeRequest &r) {
   j->fp = footprint_parts(r);  // throws on an invalid request
   j->cost = expected_seconds(r, j->fp.dim);
   j->lane = lane_ && r.kind == kReqGround && j->fp.dim <= lane_max_dim_ && r.steps <= kLaneKmax;
-  // lane members hold their four vectors in the lane's buffers
-  j->res.footprint = j->lane ? j->fp.tables + 64 * 1024 + 4 * 16 * j->fp.dim : request_footprint(r);
+  // lane members hold their three vectors in the lane's buffers
+  j->res.footprint = j->lane ? j->fp.tables + 64 * 1024 + 3 * 16 * j->fp.dim : request_footprint(r);
   std::lock_guard<std::mutex> lk(m_);
   j->id = next_id_++;
   Job *p = j.get();
