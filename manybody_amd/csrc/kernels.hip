@@ -2198,6 +2198,7 @@ __global__ __launch_bounds__(kBlock) void expr_hv_kernel(ExprTables t, const dou
     }
     if (t.do_pack) {  // column-major packed lists: one coalesced word an entry
       const int n = t.do_cnt[h];
+#pragma unroll 4  // four entries' loads in flight (9.56 -> 9.47 ms on the 4 x 4 Hubbard expression)
       for (int k = 0; k < n; ++k) {
         const uint32_t w2 = t.do_pack[(int64_t)k * t.n_dstr + h];
         const int64_t o = base + (w2 >> 12);
